@@ -1,0 +1,191 @@
+"""Seeded synthetic inputs for the lakcluster elimination (``.b16`` files + ``.in``).
+
+The ``.b16`` layout follows the reference's writer (binary_read.c:57-81,
+dir_m/tutorial_binary_compress.m): an ``int32`` header ``bitj, nrows, ncols``
+followed by the matrix stored column by column, each column packed most
+significant bit first and padded to a multiple of ``bitj`` bits.  Bit 1 means
++1 and bit 0 means -1.
+
+For a patients-by-SNPs matrix ``A`` the reference reads ``A_n.b16`` (``A``
+itself) as its SNP-major copy and ``A_t.b16`` (``A`` transposed) as its
+patient-major copy (bcc.c:281-301); both are written here.
+
+The planted model: each SNP column ``j`` is Bernoulli(``p_j``) with
+``p_j ~ U[pmin, pmax]``; a bicluster of ``m`` rows by ``m'`` columns is
+overwritten with the rank-one sign pattern ``u_i v_j`` with each entry flipped
+with probability ``flip``.  Covariates (``T`` for cases, ``S`` for controls)
+are an all-ones intercept column plus +-1 columns with P(+1)=0.5.
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+BITJ = 16
+
+
+def write_b16(path: str, mat: np.ndarray) -> None:
+    """Write a boolean (nrows x ncols) matrix in the reference's .b16 format."""
+    mat = np.asarray(mat, dtype=bool)
+    nrows, ncols = mat.shape
+    ext = (BITJ - nrows % BITJ) % BITJ
+    with open(path, "wb") as fp:
+        np.array([BITJ, nrows, ncols], dtype=np.int32).tofile(fp)
+        step = max(1, (1 << 27) // max(1, nrows))
+        for c0 in range(0, ncols, step):
+            cols = mat[:, c0:c0 + step].T
+            if ext:
+                cols = np.concatenate([cols, np.zeros((cols.shape[0], ext), dtype=bool)], axis=1)
+            np.packbits(cols, axis=1, bitorder="big").tofile(fp)
+
+
+def read_b16(path: str) -> np.ndarray:
+    """Read a .b16 file back into a boolean (nrows x ncols) matrix."""
+    with open(path, "rb") as fp:
+        bitj, nrows, ncols = np.fromfile(fp, dtype=np.int32, count=3)
+        wp = (nrows + (bitj - nrows % bitj) % bitj) // 8
+        raw = np.fromfile(fp, dtype=np.uint8, count=int(ncols) * int(wp)).reshape(int(ncols), int(wp))
+    bits = np.unpackbits(raw, axis=1, bitorder="big")[:, :nrows]
+    return bits.T.astype(bool)
+
+
+@dataclass
+class Dataset:
+    """Paths and shapes of one generated problem."""
+    in_path: str
+    out_dir: str
+    nA: int
+    nZ: int
+    ncols: int
+    T: int
+    gamma: float
+    B: int
+    planted_rows: List[np.ndarray] = field(default_factory=list)
+    planted_cols: List[np.ndarray] = field(default_factory=list)
+
+
+def _bernoulli_columns(rng: np.random.Generator, n: int, p: np.ndarray, chunk: int = 2048) -> np.ndarray:
+    out = np.empty((n, p.size), dtype=bool)
+    for r0 in range(0, n, chunk):
+        r1 = min(n, r0 + chunk)
+        out[r0:r1] = rng.random((r1 - r0, p.size), dtype=np.float32) < p[None, :]
+    return out
+
+
+def _plant(rng: np.random.Generator, A: np.ndarray, m: int, mc: int, flip: float) -> Tuple[np.ndarray, np.ndarray]:
+    n, M = A.shape
+    rows = np.sort(rng.permutation(n)[:m])
+    cols = np.sort(rng.permutation(M)[:mc])
+    u = rng.random(m) < 0.5
+    v = rng.random(mc) < 0.5
+    block = ~(u[:, None] ^ v[None, :])          # +1 where u_i v_j = +1
+    block ^= rng.random((m, mc)) < flip
+    A[np.ix_(rows, cols)] = block
+    return rows, cols
+
+
+def _covariates(rng: np.random.Generator, n: int, T: int) -> np.ndarray:
+    cov = rng.random((n, T)) < 0.5
+    cov[:, 0] = True
+    return cov
+
+
+def make_dataset(root: str, name: str, nA: int, ncols: int, nZ: int = 0, T: int = 1,
+                 pmin: float = 0.05, pmax: float = 0.5, planted: Sequence[float] = (0.55,),
+                 flip: float = 0.1, gamma: float = 0.05, B: int = 32, seed: int = 1,
+                 gamma_type: Optional[int] = None, kappa_squared: Optional[float] = None,
+                 row_mask_frac: float = 0.0, col_mask_frac: float = 0.0,
+                 write_T_file: Optional[bool] = None) -> Dataset:
+    """Generate A (and optionally Z, T, S) and the .in file that drives the elimination.
+
+    ``planted`` lists bicluster exponents e: m = round(nA**e) rows by
+    round(ncols**e) columns, planted in A only.  ``row_mask_frac`` /
+    ``col_mask_frac`` > 0 write GLOBAL_A_n_rind_/GLOBAL_A_n_cind masks with that
+    fraction of entries switched off.
+    """
+    rng = np.random.default_rng(seed)
+    d_in = os.path.join(root, f"dir_{name}_in")
+    d_out = os.path.join(root, f"dir_{name}_out")
+    os.makedirs(d_in, exist_ok=True)
+    os.makedirs(d_out, exist_ok=True)
+    p = rng.uniform(pmin, pmax, size=ncols).astype(np.float32)
+    A = _bernoulli_columns(rng, nA, p)
+    prow, pcol = [], []
+    for e in planted:
+        m, mc = int(round(nA ** e)), int(round(ncols ** e))
+        r, c = _plant(rng, A, min(m, nA), min(mc, ncols), flip)
+        prow.append(r)
+        pcol.append(c)
+    write_b16(os.path.join(d_in, "A_n.b16"), A)
+    write_b16(os.path.join(d_in, "A_t.b16"), A.T)
+    del A
+    if nZ > 0:
+        Z = _bernoulli_columns(rng, nZ, p)
+        write_b16(os.path.join(d_in, "Z_n.b16"), Z)
+        write_b16(os.path.join(d_in, "Z_t.b16"), Z.T)
+        del Z
+    if write_T_file is None:
+        write_T_file = T > 1
+    if write_T_file:
+        Tm = _covariates(rng, nA, T)
+        write_b16(os.path.join(d_in, "T_n.b16"), Tm)
+        write_b16(os.path.join(d_in, "T_t.b16"), Tm.T)
+        if nZ > 0:
+            Sm = _covariates(rng, nZ, T)
+            write_b16(os.path.join(d_in, "S_n.b16"), Sm)
+            write_b16(os.path.join(d_in, "S_t.b16"), Sm.T)
+    lines = [
+        "GLOBAL_verbose= 0;",
+        "GLOBAL_thread_count= 8;",
+        "GLOBAL_omp_type= 1;",
+        "GLOBAL_TEST_TYPE= lakcluster_driver;",
+        "GLOBAL_QR_strategy= YnWt;",
+        "GLOBAL_QC_strategy= YnWt;",
+        "GLOBAL_NBINS= 1;",
+        f"GLOBAL_B_MLT= {B};",
+        f"GLOBAL_gamma= {gamma:.4f};",
+    ]
+    if gamma_type is not None:
+        lines.append(f"GLOBAL_gamma_type= {gamma_type};")
+    if kappa_squared is not None:
+        lines.append(f"GLOBAL_kappa_squared= {kappa_squared};")
+    lines += [
+        f"GLOBAL_DIR_XPRE= {d_in};",
+        "GLOBAL_A_n_name_= A_n.b16;",
+        "GLOBAL_A_t_name_= A_t.b16;",
+        f"GLOBAL_A_n_rows_= {nA};",
+        f"GLOBAL_A_n_cols= {ncols};",
+    ]
+    if row_mask_frac > 0:
+        write_b16(os.path.join(d_in, "mr_A.b16"), (rng.random(nA) >= row_mask_frac)[:, None])
+        lines.append("GLOBAL_A_n_rind_= mr_A.b16;")
+    if col_mask_frac > 0:
+        write_b16(os.path.join(d_in, "mc_A.b16"), (rng.random(ncols) >= col_mask_frac)[:, None])
+        lines.append("GLOBAL_A_n_cind= mc_A.b16;")
+    if nZ > 0:
+        lines += ["GLOBAL_Z_n_name_= Z_n.b16;", "GLOBAL_Z_t_name_= Z_t.b16;", f"GLOBAL_Z_n_rows_= {nZ};"]
+    else:
+        lines.append("GLOBAL_Z_n_rows_= 0;")
+    lines.append("GLOBAL_Y_n_cols= 0;")
+    lines.append(f"GLOBAL_T_n_cols= {T};")
+    if write_T_file:
+        lines += ["GLOBAL_T_n_name_= T_n.b16;", "GLOBAL_T_t_name_= T_t.b16;"]
+        if nZ > 0:
+            lines += ["GLOBAL_S_n_name_= S_n.b16;", "GLOBAL_S_t_name_= S_t.b16;"]
+    lines += [f"GLOBAL_DIR_NAME= {d_out};", "END= 0;"]
+    in_path = os.path.join(d_in, f"{name}.in")
+    with open(in_path, "w") as fp:
+        fp.write("\n".join(lines) + "\n")
+    return Dataset(in_path, d_out, nA, nZ, ncols, T, gamma, B, prow, pcol)
+
+
+# the benchmark-sized problems (rows x SNP columns)
+CONFIGS = {
+    "case_only_4096x16384": dict(nA=4096, ncols=16384, nZ=0, T=1, pmin=0.05, pmax=0.5,
+                                 planted=(0.55,), gamma=0.05, B=32),
+    "case_control_16384x65536": dict(nA=16384, ncols=65536, nZ=16384, T=3, pmin=0.05, pmax=0.5,
+                                     planted=(0.55,), gamma=0.02, B=32),
+}

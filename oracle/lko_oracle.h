@@ -1,0 +1,79 @@
+/*
+ * lko_oracle.h -- CPU oracle for the lakcluster `bcc` elimination path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * product path is compared against; nothing in the product links it.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * may load it.
+ *
+ * It restates, in plain C, the arithmetic of the reference's
+ * lakcluster_driver() path (adirangan/lakcluster_c, dir_c/lakcluster_driver.c:14-251)
+ * for one bin, no Y/W genetic controls and the "YnWt" QR/QC strategies.
+ * Every floating-point expression keeps the reference's evaluation order;
+ * every function cites the reference file:line it restates.
+ */
+#ifndef LKO_ORACLE_H
+#define LKO_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LKO_POP 1920          /* bits per weighting block (lakcluster_header.h:268) */
+#define LKO_POPB 240          /* bytes per weighting block */
+#define LKO_AJDK 15           /* number of alpha^j D^k tables (lakcluster_header.h:77) */
+
+/* bytes of one packed bit-row of n entries, padded to 16 then to 1920 bits
+   (bsize, lakcluster_header.h:262) */
+size_t lko_bsize(int n);
+/* number of 1920-bit blocks (psize, lakcluster_header.h:263) */
+int lko_psize(int n);
+
+/* ---- scalar helpers exported for tests ---- */
+/* get_xdrop (xdrop.c:5-64); gamma/gamma_type passed explicitly */
+void lko_get_xdrop(double lrij, double lcij, double gamma, int gamma_type, int *rdrop, int *cdrop);
+/* calc_A_ajdk (An_ajdk_v.c:5-30): out[nc + jk*pcols] */
+void lko_calc_ajdk(const double *p, int pcols, double *out);
+/* dQuickSort_xij (Quicksort.c:83-141) on d with up to six companion int arrays (may be NULL) */
+unsigned int lko_quicksort_xij(double *d, int *i0, int *i1, int *i2, int *i3, int *i4, int *i5, int l, int r);
+
+/* ---- kernels (compact outputs; lists are ascending original indices) ---- */
+/* get_An_ajdk_v (An_ajdk_v.c:272-363): out[r*15+jk] for rows in rows[] */
+void lko_an_ajdk_v(const uint8_t *X, size_t rb, const int *rows, int nrows,
+                   const uint8_t *colmask, int ncols, const double *ajdk, double *out);
+/* get_AnZt_vv__run (AnZt_vv.c:5-183), TYPE_pm.  If ajdk==NULL the plain +-1 product
+   is formed (and xa,xz are ignored).  out[i*nz + j]. */
+void lko_anzt_vv(const uint8_t *A, size_t rbA, const int *arows, int na,
+                 const uint8_t *Z, size_t rbZ, const int *zrows, int nz,
+                 const uint8_t *colmask, int ncols, const double *ajdk,
+                 const double *xa, const double *xz, double *out);
+/* get_xcalc (xcalc.c:41-246): quantises L[i*nw + j] (i over n-rows list, j over w list)
+   into B bit planes.  Returns min/mlt; cz[j*nn + i] holds the integers (for tests). */
+typedef struct {
+  int B; int n_total; size_t rb;   /* planes are bit rows over the n dimension */
+  int nw;                           /* number of active w */
+  uint8_t *row0;                    /* bsize(n_total) bytes: the active n mask */
+  uint8_t *planes;                  /* nw*B rows of rb bytes; plane (j,nb) holds bit nb */
+  double min_d, max_d, mlt_d;
+} lko_planes;
+void lko_xcalc(const double *L, const int *nrows_list, int nn, int n_total, int nw,
+               int B, long long D_MLT, lko_planes *out, unsigned long long *cz);
+void lko_planes_free(lko_planes *p);
+
+/* whole-run driver (lakcluster_driver.c:14-251); returns an opaque result handle */
+void *lko_run_text(const char *in_text, const char *out_dir, int max_iter, int write_files);
+int lko_status(void *h, char *err, int n);
+int lko_n_trace(void *h);
+int lko_n_xdrop(void *h);
+int lko_n_iter(void *h);
+long long lko_ops(void *h);
+void lko_times(void *h, double *t3);
+void lko_get_trace(void *h, double *out);
+void lko_get_drops(void *h, int *a, int *b);
+void lko_free(void *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
