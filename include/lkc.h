@@ -1,0 +1,122 @@
+/*
+ * lkc.h -- C ABI of the MI355X (gfx950) implementation of lakcluster's `bcc`
+ * loop-counting elimination (adirangan/lakcluster_c, dir_c/lakcluster_driver.c).
+ *
+ * The library owns all device memory.  Every entry point returns 0 on success
+ * and a negative status on failure (the message is available from
+ * lkc_last_error / lkc_bcc_error); nothing in the library calls exit().
+ * One handle is one elimination; a handle may be used from one host thread at
+ * a time, different handles are independent.
+ *
+ * Scope of the path (as in the reference driver with these settings): one bin
+ * (GLOBAL_NBINS= 1), no Y/W genetic controls (GLOBAL_Y_n_cols= 0), QR/QC
+ * strategies "YnWt", no scramble, at most 8 covariate columns.
+ */
+#ifndef LKC_H
+#define LKC_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LKC_OK 0
+#define LKC_ERR_CONFIG -1     /* unknown key or unsupported setting in the .in text */
+#define LKC_ERR_IO -2         /* cannot read or write a file */
+#define LKC_ERR_SHAPE -3      /* input files disagree in shape */
+#define LKC_ERR_HIP -4        /* HIP runtime error (no device, out of memory, launch failure) */
+#define LKC_ERR_STATE -5      /* call out of order or bad argument */
+
+typedef struct lkc_bcc lkc_bcc;
+
+/* Last error message of the calling thread (load failures, kernel-level calls). */
+const char *lkc_last_error(void);
+/* Error message recorded in a handle. */
+const char *lkc_bcc_error(const lkc_bcc *D);
+
+/* ---- whole run ------------------------------------------------------------ */
+/* lakcluster_driver (lakcluster_driver.c:14-251) driven by the key/value text of
+   an .in file (updateglobals, updateglobals.c:5-199).  Writes out_trace.txt,
+   out_xdrop_a.txt, out_xdrop_b.txt into out_dir (NULL: GLOBAL_DIR_NAME). */
+int lkc_lakcluster_driver(const char *in_text, const char *out_dir);
+
+/* ---- elimination handle: the bcc stage API -------------------------------- */
+/* bcc_load + bcc_init (bcc.c:1482-1540, 1369-1438): reads the .b16 inputs and
+   masks, computes A_p and the A_ajdk tables, uploads everything to device 0. */
+int lkc_bcc_load(const char *in_text, lkc_bcc **out);
+/* the initial loop-subscores (lakcluster_driver.c:73-106): bcc_An_ajdk,
+   bcc_lf_ZtSn, bcc_lf_YnWt, bcc_M_YnWt + bcc_lf_At_T_YnWt_S_Zn,
+   bcc_lf_AnZt_S_WnYt, bcc_singlestudy_ww, bcc_doublestudy_ww, bcc_flattenloop */
+int lkc_bcc_init_lf(lkc_bcc *D);
+/* one pass of the elimination loop (lakcluster_driver.c:107-246): bcc_sumscores_
+   nrm/ifT/cmb/xij/mxA, bcc_lrup_QR_YnWt_stage_0..3, bcc_lrup_QC_YnWt_stage_a0..c,
+   bcc_lrup_mxdup, then the re-scoring stages.  *done is set to 1 once no row and
+   no column remains active. */
+int lkc_bcc_iterate(lkc_bcc *D, int *done);
+/* init (if not yet done) then iterate until done or max_iter passes (max_iter<0: no limit) */
+int lkc_bcc_run(lkc_bcc *D, int max_iter);
+/* bcc_sumscores_dmp output files (bcc_sumscores.c:127-157 format of out_trace /
+   out_xdrop_a / out_xdrop_b) */
+int lkc_bcc_write_outputs(const lkc_bcc *D, const char *out_dir);
+
+/* The individual stages, in the order lakcluster_driver calls them.  They act on
+   the device-resident state of D (lakcluster_header.h:394-502 names). */
+int lkc_bcc_An_ajdk(lkc_bcc *D);                 /* bcc_lf.c:7-51 */
+int lkc_bcc_lf_ZtSn(lkc_bcc *D);                 /* bcc_lf.c:53-101 */
+int lkc_bcc_lf_YnWt(lkc_bcc *D);                 /* bcc_lf.c:223-302 */
+int lkc_bcc_lf_At_T_YnWt_S_Zn(lkc_bcc *D);       /* bcc_lf.c:318-356 + 636-709 */
+int lkc_bcc_lf_AnZt_S_WnYt(lkc_bcc *D);          /* bcc_lf.c:360-409 */
+int lkc_bcc_singlestudy_ww(lkc_bcc *D);          /* bcc_flattenloop.c:753-831 */
+int lkc_bcc_doublestudy_ww(lkc_bcc *D);          /* bcc_flattenloop.c:977-1029 */
+int lkc_bcc_flattenloop(lkc_bcc *D);             /* bcc_flattenloop.c:1431-1889 */
+int lkc_bcc_sumscores(lkc_bcc *D);               /* nrm, ifT, cmb, xij, mxA: bcc_sumscores.c:37-779 */
+int lkc_bcc_lrup(lkc_bcc *D);                    /* bcc_lrup.c:5-1138 incl. bcc_lrup_mxdup */
+
+/* ---- results ------------------------------------------------------------- */
+int lkc_bcc_n_iter(const lkc_bcc *D);
+int lkc_bcc_n_trace(const lkc_bcc *D);
+/* 6 doubles per line: iteration, rows, cols, mean QR, mean QC, Irem */
+int lkc_bcc_get_trace(const lkc_bcc *D, double *out);
+int lkc_bcc_n_xdrop(const lkc_bcc *D);
+/* two ints per line each: a = (row, col) original indices, b = (row, col) loaded indices */
+int lkc_bcc_get_xdrop(const lkc_bcc *D, int *a, int *b);
+/* the reference's bit-operation accounting, summed over all stage calls */
+long long lkc_bcc_ops(const lkc_bcc *D);
+/* seconds: [load, init, loop] */
+int lkc_bcc_times(const lkc_bcc *D, double *t3);
+/* problem size: [rows of A, rows of Z, columns, covariate columns] */
+int lkc_bcc_shape(const lkc_bcc *D, int *out4);
+void lkc_bcc_free(lkc_bcc *D);
+/* State inspection (no counterpart in the reference, which keeps these arrays in
+   struct bcc_ajdk/bcc_single/bcc_double): copies the named score array, e.g.
+   "lf_AnZt", "QC_TAT", "et_An", in row-major (row, column) order with the
+   covariate last, into out (at most n values); returns its length or -1. */
+long lkc_bcc_debug_get(lkc_bcc *D, const char *name, double *out, long n);
+
+/* ---- single kernels on host buffers (copy in, run on device 0, copy out) --- */
+/* Bit rows: row r of X starts at X + r*rb; bit i of a row is bit (7 - i%8) of byte i/8; 1 <=> +1.
+   rb must be at least bsize(ncols) = ceil(round16(ncols)/1920)*240 bytes. */
+size_t lkc_bsize(int ncols);
+/* get_An_ajdk_v (An_ajdk_v.c:272-363): out[i*15 + jk] for rows[i] */
+int lkc_An_ajdk_v(const uint8_t *X, size_t rb, int nrows_total, const int *rows, int nrows,
+                  const uint8_t *colmask, int ncols, const double *ajdk, double *out);
+/* get_AnZt_vv (AnZt_vv.c:5-223): out[i*nz + j] for (arows[i], zrows[j]); ajdk==NULL gives the
+   plain +-1 product, else the centred D-weighted form with xa[i], xz[j] (the rows' lf_An_ajdk a1d1 values) */
+int lkc_AnZt_vv(const uint8_t *A, size_t rbA, int na_total, const int *arows, int na,
+                const uint8_t *Z, size_t rbZ, int nz_total, const int *zrows, int nz,
+                const uint8_t *colmask, int ncols, const double *ajdk,
+                const double *xa, const double *xz, double *out);
+/* get_xcalc (xcalc.c:41-246): L[i*nw + j] for n-row nrows_list[i] and w j.  Returns the
+   quantised integers cz[j*nn + i] and [min, mlt]. */
+int lkc_xcalc(const double *L, const int *nrows_list, int nn, int n_total, int nw, int B,
+              long long D_MLT, unsigned long long *cz, double *min_mlt);
+/* get_An_ZtSn_ww (An_ZtSn_ww.c:5-155) with its xcalc: L[i*ns + s] holds the (n-row
+   nlist[i], s) values quantised to B-bit planes over n_total bits; A rows (rbA bytes)
+   are contracted with the planes under amask; out[a*ns + s] for a in alist, 0 elsewhere. */
+int lkc_An_ZtSn_ww(const uint8_t *A, size_t rbA, int nr_total, const int *alist, int na, const uint8_t *amask,
+                   const double *L, const int *nlist, int nn, int n_total, int ns, int B, long long D_MLT, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,291 @@
+// lkc_capi.cpp -- the extern "C" boundary declared in include/lkc.h.
+#include "lkc_engine.h"
+#include "lkc_stages.h"
+
+#include <string.h>
+#include <string>
+#include <vector>
+
+struct lkc_bcc { lkc::Run run; };
+
+static thread_local std::string g_last_error;
+
+template <class F>
+static int guard(lkc_bcc* D, F&& f)
+{
+  try { f(); return LKC_OK; }
+  catch (const lkc::Error& e) { g_last_error = e.what(); if (D) D->run.err = e.what(); return e.code; }
+  catch (const std::exception& e) { g_last_error = e.what(); if (D) D->run.err = e.what(); return LKC_ERR_HIP; }
+}
+
+#define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) \
+  throw lkc::Error(LKC_ERR_HIP, std::string("HIP error: ") + hipGetErrorString(e_) + " (" #x ")"); } while (0)
+
+extern "C" {
+
+const char* lkc_last_error(void) { return g_last_error.c_str(); }
+const char* lkc_bcc_error(const lkc_bcc* D) { return D ? D->run.err.c_str() : "null handle"; }
+size_t lkc_bsize(int ncols) { return lkc::bsize(ncols); }
+
+int lkc_bcc_load(const char* in_text, lkc_bcc** out)
+{
+  if (!out) { g_last_error = "null output pointer"; return LKC_ERR_STATE; }
+  *out = nullptr;
+  lkc_bcc* D = new lkc_bcc();
+  int rc = guard(nullptr, [&] { D->run.load(in_text); });
+  if (rc != LKC_OK) { delete D; return rc; }
+  *out = D;
+  return LKC_OK;
+}
+
+void lkc_bcc_free(lkc_bcc* D) { delete D; }
+
+#define STAGE(fn, call) int fn(lkc_bcc* D) { if (!D) return LKC_ERR_STATE; return guard(D, [&] { call; }); }
+STAGE(lkc_bcc_init_lf, D->run.init_lf())
+STAGE(lkc_bcc_An_ajdk, D->run.st_ajdk())
+STAGE(lkc_bcc_lf_ZtSn, D->run.st_ZtSn())
+STAGE(lkc_bcc_lf_YnWt, D->run.st_YnWt())
+STAGE(lkc_bcc_lf_At_T_YnWt_S_Zn, D->run.st_At_T())
+STAGE(lkc_bcc_lf_AnZt_S_WnYt, D->run.st_QR())
+STAGE(lkc_bcc_singlestudy_ww, D->run.st_single())
+STAGE(lkc_bcc_doublestudy_ww, D->run.st_double())
+STAGE(lkc_bcc_flattenloop, D->run.st_flatten())
+STAGE(lkc_bcc_sumscores, D->run.sumscores())
+STAGE(lkc_bcc_lrup, D->run.lrup())
+#undef STAGE
+
+int lkc_bcc_iterate(lkc_bcc* D, int* done)
+{
+  if (!D) return LKC_ERR_STATE;
+  return guard(D, [&] { D->run.iterate(done); });
+}
+
+int lkc_bcc_run(lkc_bcc* D, int max_iter)
+{
+  if (!D) return LKC_ERR_STATE;
+  return guard(D, [&] { D->run.run(max_iter); });
+}
+
+int lkc_bcc_write_outputs(const lkc_bcc* D, const char* out_dir)
+{
+  if (!D) return LKC_ERR_STATE;
+  return guard(const_cast<lkc_bcc*>(D), [&] {
+    D->run.write_outputs(out_dir && out_dir[0] ? std::string(out_dir) : D->run.default_out_dir());
+  });
+}
+
+int lkc_lakcluster_driver(const char* in_text, const char* out_dir)
+{
+  lkc_bcc* D = nullptr;
+  int rc = lkc_bcc_load(in_text, &D);
+  if (rc) return rc;
+  rc = lkc_bcc_run(D, -1);
+  if (!rc) rc = lkc_bcc_write_outputs(D, out_dir);
+  lkc_bcc_free(D);
+  return rc;
+}
+
+int lkc_bcc_n_iter(const lkc_bcc* D) { return D ? D->run.n_iter : -1; }
+int lkc_bcc_n_trace(const lkc_bcc* D) { return D ? (int)(D->run.trace.size() / 6) : -1; }
+int lkc_bcc_get_trace(const lkc_bcc* D, double* out)
+{
+  if (!D || !out) return LKC_ERR_STATE;
+  if (!D->run.trace.empty()) memcpy(out, D->run.trace.data(), sizeof(double) * D->run.trace.size());
+  return LKC_OK;
+}
+int lkc_bcc_n_xdrop(const lkc_bcc* D) { return D ? (int)(D->run.xdrop_a.size() / 2) : -1; }
+int lkc_bcc_get_xdrop(const lkc_bcc* D, int* a, int* b)
+{
+  if (!D || !a || !b) return LKC_ERR_STATE;
+  if (!D->run.xdrop_a.empty()) {
+    memcpy(a, D->run.xdrop_a.data(), sizeof(int) * D->run.xdrop_a.size());
+    memcpy(b, D->run.xdrop_b.data(), sizeof(int) * D->run.xdrop_b.size());
+  }
+  return LKC_OK;
+}
+long long lkc_bcc_ops(const lkc_bcc* D) { return D ? D->run.ops : -1; }
+int lkc_bcc_times(const lkc_bcc* D, double* t3)
+{
+  if (!D || !t3) return LKC_ERR_STATE;
+  t3[0] = D->run.load_s; t3[1] = D->run.init_s; t3[2] = D->run.loop_s;
+  return LKC_OK;
+}
+int lkc_bcc_shape(const lkc_bcc* D, int* o)
+{
+  if (!D || !o) return LKC_ERR_STATE;
+  o[0] = D->run.NA; o[1] = D->run.NZ; o[2] = D->run.M; o[3] = D->run.T;
+  return LKC_OK;
+}
+
+long lkc_bcc_debug_get(lkc_bcc* D, const char* name, double* out, long n)
+{
+  if (!D || !name) return -1;
+  long len = -1;
+  std::vector<double> v;
+  if (guard(D, [&] { len = D->run.debug_get(name, v); }) != LKC_OK) return -1;
+  if (len > 0 && out) memcpy(out, v.data(), sizeof(double) * (size_t)std::min(len, n));
+  return len;
+}
+
+}  // extern "C"
+
+// ---- single kernels on host buffers ---------------------------------------
+namespace {
+struct Scoped {
+  lkc::Dev dev; hipStream_t st = nullptr;
+  Scoped() { HIPCHK(hipSetDevice(0)); HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)); }
+  ~Scoped() { if (st) (void)hipStreamDestroy(st); }
+  template <class T> T* put(const T* h, size_t n)
+  {
+    T* d = dev.alloc<T>(n ? n : 1);
+    if (n) HIPCHK(hipMemcpyAsync(d, h, sizeof(T) * n, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return d;
+  }
+};
+// repack caller rows (stride rb) into rows of bsize(ncols) bytes
+static lkc::DMat put_bits(Scoped& s, const uint8_t* X, size_t rb, int nrows, int ncols, std::vector<uint8_t>& tmp)
+{
+  size_t b = lkc::bsize(ncols);
+  if (rb < b) throw lkc::Error(LKC_ERR_STATE, "row stride smaller than bsize(ncols)");
+  tmp.assign((size_t)std::max(nrows, 1) * b, 0);
+  for (int r = 0; r < nrows; r++) memcpy(tmp.data() + (size_t)r * b, X + (size_t)r * rb, b);
+  lkc::DMat m; m.nr = nrows; m.nc = ncols; m.rb = b;
+  m.p = s.put(tmp.data(), tmp.size());
+  return m;
+}
+static void check_rows(const int* rows, int n, int total)
+{
+  for (int i = 0; i < n; i++) if (rows[i] < 0 || rows[i] >= total) throw lkc::Error(LKC_ERR_STATE, "row index out of range");
+}
+}  // namespace
+
+extern "C" {
+
+int lkc_An_ajdk_v(const uint8_t* X, size_t rb, int nrows_total, const int* rows, int nrows,
+                  const uint8_t* colmask, int ncols, const double* ajdk, double* out)
+{
+  return guard(nullptr, [&] {
+    if (nrows < 0 || ncols <= 0 || !X || !colmask || !ajdk || !out || (nrows && !rows)) throw lkc::Error(LKC_ERR_STATE, "bad argument");
+    check_rows(rows, nrows, nrows_total);
+    Scoped s;
+    int pc = lkc::psize(ncols);
+    std::vector<uint8_t> tmp;
+    lkc::DMat dX = put_bits(s, X, rb, nrows_total, ncols, tmp);
+    uint32_t* dm = (uint32_t*)s.put(colmask, lkc::bsize(ncols));
+    int* dr = s.put(rows, (size_t)nrows);
+    double* da = s.put(ajdk, (size_t)pc * lkc::kAjdk);
+    double* base = s.dev.alloc<double>(16);
+    double* dout = s.dev.alloc<double>((size_t)lkc::kAjdk * std::max(nrows_total, 1));
+    lkc::mask_lf(dm, pc, da, base, s.st);
+    lkc::an_ajdk(dX.b(), dr, nrows, dm, pc, da, base, dout, nrows_total, s.st);
+    std::vector<double> h((size_t)lkc::kAjdk * std::max(nrows_total, 1));
+    HIPCHK(hipMemcpyAsync(h.data(), dout, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s.st));
+    HIPCHK(hipStreamSynchronize(s.st));
+    for (int i = 0; i < nrows; i++)
+      for (int jk = 0; jk < lkc::kAjdk; jk++) out[(size_t)i * lkc::kAjdk + jk] = h[(size_t)jk * nrows_total + rows[i]];
+  });
+}
+
+int lkc_AnZt_vv(const uint8_t* A, size_t rbA, int na_total, const int* arows, int na,
+                const uint8_t* Z, size_t rbZ, int nz_total, const int* zrows, int nz,
+                const uint8_t* colmask, int ncols, const double* ajdk,
+                const double* xa, const double* xz, double* out)
+{
+  return guard(nullptr, [&] {
+    if (na < 0 || nz < 0 || ncols <= 0 || !A || !Z || !colmask || !out) throw lkc::Error(LKC_ERR_STATE, "bad argument");
+    if (ajdk && (!xa || !xz)) throw lkc::Error(LKC_ERR_STATE, "xa/xz required with ajdk");
+    check_rows(arows, na, na_total); check_rows(zrows, nz, nz_total);
+    Scoped s;
+    int pc = lkc::psize(ncols);
+    std::vector<uint8_t> ta, tz;
+    lkc::DMat dA = put_bits(s, A, rbA, na_total, ncols, ta);
+    lkc::DMat dZ = put_bits(s, Z, rbZ, nz_total, ncols, tz);
+    uint32_t* dm = (uint32_t*)s.put(colmask, lkc::bsize(ncols));
+    int* dar = s.put(arows, (size_t)na);
+    int* dzr = s.put(zrows, (size_t)nz);
+    double *da = nullptr, *dxa = nullptr, *dxz = nullptr;
+    if (ajdk) {
+      da = s.put(ajdk, (size_t)pc * lkc::kAjdk);
+      std::vector<double> fa(std::max(na_total, 1), 0.0), fz(std::max(nz_total, 1), 0.0);
+      for (int i = 0; i < na; i++) fa[arows[i]] = xa[i];
+      for (int j = 0; j < nz; j++) fz[zrows[j]] = xz[j];
+      dxa = s.put(fa.data(), fa.size());
+      dxz = s.put(fz.data(), fz.size());
+    }
+    double* base = s.dev.alloc<double>(16);
+    double* dout = s.dev.alloc<double>((size_t)std::max(na, 1) * std::max(nz, 1));
+    lkc::mask_lf(dm, pc, da, base, s.st);
+    lkc::gram(dA.b(), dar, na, dZ.b(), dzr, nz, dm, pc, da, base, dxa, dxz, ajdk ? 1 : 0, dout, na, 0, 0, 0, s.st);
+    std::vector<double> h((size_t)std::max(na, 1) * std::max(nz, 1));
+    HIPCHK(hipMemcpyAsync(h.data(), dout, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s.st));
+    HIPCHK(hipStreamSynchronize(s.st));
+    for (int i = 0; i < na; i++) for (int j = 0; j < nz; j++) out[(size_t)i * nz + j] = h[(size_t)j * na + i];
+  });
+}
+
+int lkc_xcalc(const double* L, const int* nrows_list, int nn, int n_total, int nw, int B,
+              long long D_MLT, unsigned long long* cz, double* min_mlt)
+{
+  return guard(nullptr, [&] {
+    if (nn < 0 || nw < 0 || B < 1 || B > 32 || !min_mlt || (nn * (long long)nw && (!L || !cz))) throw lkc::Error(LKC_ERR_STATE, "bad argument");
+    (void)nrows_list; (void)n_total;
+    Scoped s;
+    double* dL = s.put(L, (size_t)nn * nw);
+    std::vector<int> ident(std::max(nn, 1));
+    for (int i = 0; i < nn; i++) ident[i] = i;
+    int* dn = s.put(ident.data(), ident.size());
+    uint32_t* dcz = s.dev.alloc<uint32_t>((size_t)std::max(nn, 1) * std::max(nw, 1));
+    double* scal = s.dev.alloc<double>(16);
+    double* work = s.dev.alloc<double>(1024);
+    lkc::xcalc(dL, nw, 1, nullptr, nullptr, nn, nw, dn, B, D_MLT, dcz, std::max(nn, 1), scal, work, s.st);
+    std::vector<uint32_t> h((size_t)std::max(nn, 1) * std::max(nw, 1));
+    double hs[16];
+    HIPCHK(hipMemcpyAsync(h.data(), dcz, sizeof(uint32_t) * h.size(), hipMemcpyDeviceToHost, s.st));
+    HIPCHK(hipMemcpyAsync(hs, scal, sizeof(hs), hipMemcpyDeviceToHost, s.st));
+    HIPCHK(hipStreamSynchronize(s.st));
+    for (int j = 0; j < nw; j++) for (int i = 0; i < nn; i++) cz[(size_t)j * nn + i] = h[(size_t)j * std::max(nn, 1) + i];
+    min_mlt[0] = hs[2]; min_mlt[1] = hs[4];
+  });
+}
+
+int lkc_An_ZtSn_ww(const uint8_t* A, size_t rbA, int nr_total, const int* alist, int na, const uint8_t* amask,
+                   const double* L, const int* nlist, int nn, int n_total, int ns, int B, long long D_MLT, double* out)
+{
+  return guard(nullptr, [&] {
+    if (na < 0 || nn < 0 || ns < 0 || n_total <= 0 || B < 1 || B > 32 || !A || !amask || !out) throw lkc::Error(LKC_ERR_STATE, "bad argument");
+    check_rows(alist, na, nr_total); check_rows(nlist, nn, n_total);
+    Scoped s;
+    std::vector<uint8_t> ta;
+    lkc::DMat dA = put_bits(s, A, rbA, nr_total, n_total, ta);
+    size_t prb = dA.rb;
+    std::vector<uint8_t> hm(prb, 0);
+    memcpy(hm.data(), amask, lkc::bsize(n_total));
+    uint32_t* dm = (uint32_t*)s.put(hm.data(), prb);
+    int* dal = s.put(alist, (size_t)na);
+    int* dnl = s.put(nlist, (size_t)nn);
+    double* dL = s.put(L, (size_t)nn * ns);
+    std::vector<int> sl(std::max(ns, 1));
+    for (int i = 0; i < ns; i++) sl[i] = i;
+    int* dsl = s.put(sl.data(), sl.size());
+    // the active n rows: the bits of nlist
+    std::vector<uint8_t> row0(prb, 0);
+    for (int i = 0; i < nn; i++) row0[nlist[i] >> 3] |= (uint8_t)(1u << (7 - (nlist[i] & 7)));
+    uint32_t* drow0 = (uint32_t*)s.put(row0.data(), prb);
+    uint32_t* cz = s.dev.alloc<uint32_t>((size_t)std::max(ns, 1) * n_total);
+    uint8_t* planes = s.dev.alloc<uint8_t>((size_t)std::max(ns, 1) * B * prb);
+    double* scal = s.dev.alloc<double>(16);
+    double* work = s.dev.alloc<double>(1024);
+    double* dout = s.dev.alloc<double>((size_t)std::max(ns, 1) * std::max(nr_total, 1));
+    lkc::fill(dout, (long long)std::max(ns, 1) * std::max(nr_total, 1), 0.0, s.st);
+    lkc::xcalc(dL, ns, 1, nullptr, nullptr, nn, ns, dnl, B, D_MLT, cz, n_total, scal, work, s.st);
+    lkc::cz_planes(cz, n_total, ns, n_total, drow0, B, planes, prb, s.st);
+    lkc::an_ztsn(dA.b(), dal, na, dm, planes, prb, drow0, ns, dsl, B, scal, dout, nr_total, s.st);
+    std::vector<double> h((size_t)std::max(ns, 1) * std::max(nr_total, 1));
+    HIPCHK(hipMemcpyAsync(h.data(), dout, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s.st));
+    HIPCHK(hipStreamSynchronize(s.st));
+    for (int a = 0; a < nr_total; a++) for (int j = 0; j < ns; j++) out[(size_t)a * ns + j] = h[(size_t)j * nr_total + a];
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,30 @@
+// lkc_cli.cpp -- the .in-driven command line: reads the key/value text of an
+// .in file on stdin (as lakcluster_ver18 does, lakcluster_ver18.c:337) and runs
+// the elimination on device 0, writing out_trace.txt / out_xdrop_{a,b}.txt.
+#include "../../include/lkc.h"
+#include <stdio.h>
+#include <string>
+
+int main(int argc, char** argv)
+{
+  std::string text;
+  char buf[65536];
+  size_t n;
+  while ((n = fread(buf, 1, sizeof buf, stdin)) > 0) text.append(buf, n);
+  const char* out_dir = argc > 1 ? argv[1] : nullptr;
+  lkc_bcc* D = nullptr;
+  int rc = lkc_bcc_load(text.c_str(), &D);
+  if (rc) { fprintf(stderr, "lkc: %s\n", lkc_last_error()); return 1; }
+  rc = lkc_bcc_run(D, -1);
+  if (!rc) rc = lkc_bcc_write_outputs(D, out_dir);
+  if (rc) { fprintf(stderr, "lkc: %s\n", lkc_bcc_error(D)); lkc_bcc_free(D); return 1; }
+  double t[3];
+  lkc_bcc_times(D, t);
+  int shp[4];
+  lkc_bcc_shape(D, shp);
+  printf("rows %d controls %d cols %d covariates %d\n", shp[0], shp[1], shp[2], shp[3]);
+  printf("iterations %d\nload %.3f s\ninit %.3f s\nloop %.3f s\nelim-run %.3f s\nbit-ops %lld\n",
+         lkc_bcc_n_iter(D), t[0], t[1], t[2], t[1] + t[2], lkc_bcc_ops(D));
+  lkc_bcc_free(D);
+  return 0;
+}

@@ -1,0 +1,867 @@
+// lkc_engine.cpp -- host side of the bcc elimination: configuration, loading,
+// device-resident state, stage sequencing, sort and drop selection, outputs.
+//
+// Stage order and semantics follow lakcluster_driver (lakcluster_driver.c:14-251)
+// for one bin, no Y/W controls and the YnWt strategies.  All score arrays live
+// on the device; per step only the two score columns that are sorted travel to
+// the host, and the new masks and index lists travel back.
+#include "lkc_engine.h"
+#include "lkc_stages.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <sys/time.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+namespace lkc {
+
+#define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) \
+  throw Error(LKC_ERR_HIP, std::string("HIP error: ") + hipGetErrorString(e_) + " (" #x ")"); } while (0)
+
+static inline int bget(const uint8_t* b, long i) { return (b[i >> 3] >> (7 - (i & 7))) & 1; }
+static inline void bset(uint8_t* b, long i) { b[i >> 3] |= (uint8_t)(1u << (7 - (i & 7))); }
+static inline void bclr(uint8_t* b, long i) { b[i >> 3] &= (uint8_t)~(1u << (7 - (i & 7))); }
+static int popmask(const std::vector<uint8_t>& m, int n) { int s = 0; for (int i = 0; i < n; i++) s += bget(m.data(), i); return s; }
+static double now_s() { struct timeval tv; gettimeofday(&tv, NULL); return tv.tv_sec + 1e-6 * tv.tv_usec; }
+
+// ------------------------------------------------------------------
+// configuration (updateglobals, updateglobals.c:5-199)
+// ------------------------------------------------------------------
+static std::string trim(const std::string& s)
+{
+  size_t a = s.find_first_not_of(" \t\r\n"), b = s.find_last_not_of(" \t\r\n");
+  return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+}
+
+void parse_cfg(const char* text, Cfg& c)
+{
+  c = Cfg();
+  std::string s(text ? text : "");
+  size_t p = 0;
+  while (p < s.size()) {
+    size_t eq = s.find('=', p);
+    if (eq == std::string::npos) break;
+    std::string name = trim(s.substr(p, eq - p));
+    size_t semi = s.find(';', eq + 1);
+    std::string val = trim(s.substr(eq + 1, (semi == std::string::npos ? s.size() : semi) - eq - 1));
+    p = semi == std::string::npos ? s.size() : semi + 1;
+    if (name == "END") break;
+    std::string first = trim(val.substr(0, val.find(',')));   // list-valued keys: the first bin
+    auto path = [&](const std::string& v) { return c.xpre.empty() ? v : c.xpre + "/" + v; };
+    if (name == "GLOBAL_B_MLT") { c.B = atoi(val.c_str()); c.D_MLT = (long long)pow(2.0, (double)c.B); }
+    else if (name == "GLOBAL_D_MLT") { c.D_MLT = atoll(val.c_str()); c.B = (int)llround(log2((double)c.D_MLT)); }
+    else if (name == "GLOBAL_gamma") c.gamma = atof(val.c_str());
+    else if (name == "GLOBAL_gamma_type") c.gamma_type = atoi(val.c_str());
+    else if (name == "GLOBAL_kappa_squared") c.kappa2 = atof(val.c_str());
+    else if (name == "GLOBAL_TEST_sparse") c.sparse = atoi(val.c_str());
+    else if (name == "GLOBAL_NBINS") c.nbins = atoi(val.c_str());
+    else if (name == "GLOBAL_Ireq") c.Ireq = atoi(val.c_str());
+    else if (name == "GLOBAL_QR_strategy") c.QR = val;
+    else if (name == "GLOBAL_QC_strategy") c.QC = val;
+    else if (name == "GLOBAL_A_n_cols") c.A_ncols = atoi(val.c_str());
+    else if (name == "GLOBAL_Y_n_cols") c.Y_ncols = atoi(val.c_str());
+    else if (name == "GLOBAL_T_n_cols") c.T_ncols = atoi(val.c_str());
+    else if (name == "GLOBAL_DIR_XPRE") c.xpre = val;
+    else if (name == "GLOBAL_DIR_NAME") c.dir_name = val;
+    else if (name == "GLOBAL_DIR_BASE") c.dir_base = val;
+    else if (name == "GLOBAL_out_name") c.out_name = val;
+    else if (name == "GLOBAL_A_n_name_") c.A_n = path(first);
+    else if (name == "GLOBAL_A_t_name_") c.A_t = path(first);
+    else if (name == "GLOBAL_A_n_rind_") c.A_rind = path(first);
+    else if (name == "GLOBAL_A_n_cind") c.A_cind = path(val);
+    else if (name == "GLOBAL_A_p_name") c.A_p = path(val);
+    else if (name == "GLOBAL_Z_n_name_") c.Z_n = path(first);
+    else if (name == "GLOBAL_Z_t_name_") c.Z_t = path(first);
+    else if (name == "GLOBAL_Z_n_rind_") c.Z_rind = path(first);
+    else if (name == "GLOBAL_T_n_name_") c.T_n = path(first);
+    else if (name == "GLOBAL_T_t_name_") c.T_t = path(first);
+    else if (name == "GLOBAL_T_n_cind") c.T_cind = path(val);
+    else if (name == "GLOBAL_S_n_name_") c.S_n = path(first);
+    else if (name == "GLOBAL_S_t_name_") c.S_t = path(first);
+    else if (name == "GLOBAL_scramble_num") c.scramble_num = atoi(val.c_str());
+    else if (name.compare(0, 7, "GLOBAL_") == 0) { /* accepted; no effect on this path */ }
+    else throw Error(LKC_ERR_CONFIG, "unknown key " + name);
+  }
+}
+
+// ------------------------------------------------------------------
+// .b16 input (binary_read.c:5-55; M_load, M_init.c:229-242): a file holding an
+// (fnr x fnc) matrix column by column yields fnc rows of fnr bits.
+// ------------------------------------------------------------------
+struct HostBits { std::vector<uint8_t> b; int nr = 0, nc = 0; size_t rb = 0; };
+
+static HostBits read_b16(const std::string& path)
+{
+  FILE* fp = fopen(path.c_str(), "rb");
+  if (!fp) throw Error(LKC_ERR_IO, "cannot open " + path);
+  int hdr[3];
+  if (fread(hdr, sizeof(int), 3, fp) != 3) { fclose(fp); throw Error(LKC_ERR_IO, "short header in " + path); }
+  int bitj = hdr[0] > 0 ? hdr[0] : 16, fnr = hdr[1], fnc = hdr[2];
+  if (fnr < 0 || fnc < 0) { fclose(fp); throw Error(LKC_ERR_IO, "bad header in " + path); }
+  size_t wp = (size_t)(fnr + (bitj - (fnr % bitj)) % bitj) / 8;
+  HostBits h; h.nr = fnc; h.nc = fnr; h.rb = bsize(fnr);
+  h.b.assign((size_t)std::max(fnc, 1) * h.rb, 0);
+  size_t take = std::min(wp, h.rb);
+  std::vector<uint8_t> col(wp);
+  for (int r = 0; r < fnc; r++) {
+    if (wp && fread(col.data(), 1, wp, fp) != wp) { fclose(fp); throw Error(LKC_ERR_IO, "short data in " + path); }
+    memcpy(h.b.data() + (size_t)r * h.rb, col.data(), take);
+  }
+  fclose(fp);
+  return h;
+}
+
+static std::vector<uint8_t> read_mask(const std::string& path, int n)
+{
+  HostBits m = read_b16(path);
+  std::vector<uint8_t> out(bsize(n) + 8, 0);
+  if (m.nr > 0) memcpy(out.data(), m.b.data(), std::min(bsize(n), m.rb));
+  for (long i = n; i < (long)bsize(n) * 8; i++) bclr(out.data(), i);
+  return out;
+}
+
+// ------------------------------------------------------------------
+// get_xdrop (xdrop.c:5-64) and dQuickSort_xij (Quicksort.c:83-141)
+// ------------------------------------------------------------------
+void get_xdrop(double lrij, double lcij, double gamma, int gamma_type, int* rdrop_p, int* cdrop_p)
+{
+  if (lrij > lcij) { get_xdrop(lcij, lrij, gamma, gamma_type, cdrop_p, rdrop_p); return; }
+  const double dbl1 = 1.000000 - 0.000001;
+  double gtr = 0, atr = 0, gtc = 0, atc = 0; int rdrop = 0, cdrop = 0;
+  if (lrij <= 2) { rdrop = (int)lrij; cdrop = (int)lcij; }
+  else {
+    double one_r = std::max(1.0, lrij);
+    if (gamma > 0) { gtr = std::max(gamma, dbl1 / one_r); atr = std::min(1 - gamma, (lrij - 1) / one_r); }
+    else { gtr = dbl1 / one_r; atr = (lrij - 1) / one_r; }
+    if (gamma_type == 0) atc = exp(log(atr) * log(lcij) / std::max(1.0, log(one_r)));
+    else if (gamma_type == 1) atc = sqrt(atr * lrij) * sqrt(lcij) / std::max(1.0, sqrt(one_r)) / ((double)std::max(1.0, lcij));
+    else if (gamma_type == 2) atc = atr;
+    else if (gamma_type == 3) atc = 0;
+    gtc = dbl1 - atc; rdrop = (int)ceil(gtr * lrij); cdrop = (int)ceil(gtc * lcij);
+    if (gamma_type == 3) cdrop = 0;
+  }
+  rdrop = (int)std::min((double)rdrop, lrij); cdrop = (int)std::min((double)cdrop, lcij);
+  if (rdrop_p) *rdrop_p = rdrop;
+  if (cdrop_p) *cdrop_p = cdrop;
+}
+
+// Hoare partition around the first element; equal keys stay left of the pivot.
+static int partition_first(double* d, int* perm, int l, int r)
+{
+  double pivot = d[l];
+  int i = l, j = r + 1;
+  for (;;) {
+    do { i++; } while (i <= r && d[i] <= pivot);
+    do { j--; } while (d[j] > pivot);
+    if (i >= j) break;
+    std::swap(d[i], d[j]); std::swap(perm[i], perm[j]);
+  }
+  std::swap(d[l], d[j]); std::swap(perm[l], perm[j]);
+  return j;
+}
+static void quicksort_first(double* d, int* perm, int l, int r, unsigned depth)
+{
+  if (l >= r) return;
+  int j = partition_first(d, perm, l, r);
+  if (depth < 32768u) { quicksort_first(d, perm, l, j - 1, depth + 1); quicksort_first(d, perm, j + 1, r, depth + 1); }
+}
+void sort_scores(double* d, int* perm, int n) { quicksort_first(d, perm, 0, n - 1, 0); }
+
+// ------------------------------------------------------------------
+// device memory
+// ------------------------------------------------------------------
+Dev::~Dev() { for (void* p : ptrs) (void)hipFree(p); }
+void* Dev::raw(size_t bytes)
+{
+  void* p = nullptr;
+  HIPCHK(hipMalloc(&p, std::max<size_t>(bytes, 256)));
+  ptrs.push_back(p);
+  return p;
+}
+void* Grow::need(Dev& d, size_t bytes)
+{
+  if (bytes <= cap && p) return p;
+  if (p) { HIPCHK(hipDeviceSynchronize()); HIPCHK(hipFree(p)); d.ptrs.erase(std::find(d.ptrs.begin(), d.ptrs.end(), p)); }
+  cap = std::max(bytes, cap + cap / 2);
+  p = d.raw(cap);
+  return p;
+}
+
+static void upload_bits(Dev& dev, DMat& m, const HostBits& h)
+{
+  m.nr = h.nr; m.nc = h.nc; m.rb = h.rb;
+  size_t bytes = (size_t)std::max(h.nr, 1) * h.rb;
+  m.p = dev.alloc<uint8_t>(bytes);
+  HIPCHK(hipMemcpy(m.p, h.b.data(), bytes, hipMemcpyHostToDevice));
+}
+
+// ------------------------------------------------------------------
+// Run: construction (bcc_load, bcc.c:1482-1540)
+// ------------------------------------------------------------------
+Run::~Run()
+{
+  if (hctl) (void)hipHostFree(hctl);
+  if (hsv) (void)hipHostFree(hsv);
+  if (st) (void)hipStreamDestroy(st);
+}
+
+void Run::load(const char* in_text)
+{
+  double t0 = now_s();
+  parse_cfg(in_text, c);
+  if (c.nbins != 1) throw Error(LKC_ERR_CONFIG, "only GLOBAL_NBINS= 1 is supported");
+  if (c.Y_ncols != 0) throw Error(LKC_ERR_CONFIG, "Y/W genetic controls are not supported");
+  if (c.QR.find("YnWt") == std::string::npos || c.QC.find("YnWt") == std::string::npos ||
+      c.QR.find("condense") != std::string::npos || c.QC.find("store one") != std::string::npos ||
+      c.QR.find("ZtSWn") != std::string::npos || c.QC.find("ZtSWn") != std::string::npos)
+    throw Error(LKC_ERR_CONFIG, "only the YnWt strategies are supported");
+  if (c.scramble_num > 0) throw Error(LKC_ERR_CONFIG, "scramble is not supported");
+  if (c.B < 1 || c.B > 32) throw Error(LKC_ERR_CONFIG, "GLOBAL_B_MLT must be in [1,32]");
+  if (c.A_t.empty() || c.A_n.empty()) throw Error(LKC_ERR_CONFIG, "A_n/A_t files required");
+  HIPCHK(hipSetDevice(0));
+  HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+
+  HostBits hAn = read_b16(c.A_t), hAt = read_b16(c.A_n);
+  NA = hAn.nr; M = hAn.nc;
+  if (c.A_ncols && c.A_ncols != M) throw Error(LKC_ERR_SHAPE, "GLOBAL_A_n_cols does not match A");
+  if (hAt.nr != M || hAt.nc != NA) throw Error(LKC_ERR_SHAPE, "A_n/A_t shapes disagree");
+  if (NA <= 0 || M <= 0) throw Error(LKC_ERR_SHAPE, "A is empty");
+  has_Z = !c.Z_t.empty();
+  HostBits hZn, hZt, hTt, hSt;
+  if (has_Z) {
+    if (c.Z_n.empty()) throw Error(LKC_ERR_CONFIG, "Z_n file required with Z_t");
+    hZn = read_b16(c.Z_t); hZt = read_b16(c.Z_n);
+    NZ = hZn.nr;
+    if (hZn.nc != M || hZt.nr != M || hZt.nc != NZ) throw Error(LKC_ERR_SHAPE, "Z shapes disagree");
+  } else {
+    NZ = 0; hZn.nr = 0; hZn.nc = M; hZn.rb = bsize(M); hZn.b.assign(hZn.rb, 0);
+    hZt.nr = M; hZt.nc = 0; hZt.rb = bsize(0); hZt.b.assign((size_t)M * hZt.rb, 0);
+  }
+  if (!c.T_n.empty()) {
+    hTt = read_b16(c.T_n);
+    T = hTt.nr;
+    if (hTt.nc != NA) throw Error(LKC_ERR_SHAPE, "T shape disagrees");
+    if (has_Z) {
+      if (c.S_n.empty()) throw Error(LKC_ERR_CONFIG, "S_n file required with T and Z");
+      hSt = read_b16(c.S_n);
+      if (hSt.nr != T || hSt.nc != NZ) throw Error(LKC_ERR_SHAPE, "S shape disagrees");
+    } else { hSt.nr = T; hSt.nc = 0; hSt.rb = bsize(0); hSt.b.assign((size_t)std::max(T, 1) * hSt.rb, 0); }
+  } else {
+    // no covariate file: one all-ones intercept column (bcc.c:361-373)
+    T = 1;
+    hTt.nr = 1; hTt.nc = NA; hTt.rb = bsize(NA); hTt.b.assign(hTt.rb, 0);
+    for (int a = 0; a < NA; a++) bset(hTt.b.data(), a);
+    hSt.nr = 1; hSt.nc = NZ; hSt.rb = bsize(NZ); hSt.b.assign(hSt.rb, 0);
+    for (int z = 0; z < NZ; z++) bset(hSt.b.data(), z);
+  }
+  if (T < 1 || T > kMaxT) throw Error(LKC_ERR_CONFIG, "between 1 and 8 covariate columns are supported");
+  pcols = psize(M);
+
+  auto mk = [](int n) { return std::vector<uint8_t>(bsize(n) + 8, 0); };
+  hAr_b = mk(NA); hAr_j = mk(NA); hAr_rtn = mk(NA); hAr_rmv = mk(NA);
+  hZr_b = mk(NZ); hZr_j = mk(NZ);
+  hAc_b = mk(M); hAc_j = mk(M); hAc_rtn = mk(M); hAc_rmv = mk(M);
+  hTc_j = mk(T);
+  if (!c.A_rind.empty()) hAr_b = read_mask(c.A_rind, NA); else for (int i = 0; i < NA; i++) bset(hAr_b.data(), i);
+  if (!c.Z_rind.empty() && has_Z) hZr_b = read_mask(c.Z_rind, NZ); else for (int i = 0; i < NZ; i++) bset(hZr_b.data(), i);
+  if (!c.A_cind.empty()) hAc_b = read_mask(c.A_cind, M); else for (int i = 0; i < M; i++) bset(hAc_b.data(), i);
+  if (!c.T_cind.empty()) hTc_j = read_mask(c.T_cind, T);
+  else { int nt = c.T_ncols > 0 ? std::min(c.T_ncols, T) : T; for (int i = 0; i < nt; i++) bset(hTc_j.data(), i); }
+  for (int i = 1; i < T; i++) if (bget(hTc_j.data(), i) && !bget(hTc_j.data(), i - 1)) throw Error(LKC_ERR_CONFIG, "T column mask must be a prefix");
+  hAr_j = hAr_b; hZr_j = hZr_b; hAc_j = hAc_b;
+  Arb = popmask(hAr_b, NA); Zrb = popmask(hZr_b, NZ);
+  if (Arb == 0) throw Error(LKC_ERR_SHAPE, "A has no active rows");
+  Ar_bidx.assign(NA + 1, 0); Ac_bidx.assign(M + 1, 0);
+  for (int i = 0, b = 0; i < NA; i++) { Ar_bidx[i] = b; if (bget(hAr_b.data(), i)) b++; }
+  for (int i = 0, b = 0; i < M; i++) { Ac_bidx[i] = b; if (bget(hAc_b.data(), i)) b++; }
+
+  // device-resident bit matrices
+  upload_bits(dev, An, hAn); upload_bits(dev, At, hAt);
+  upload_bits(dev, Zn, hZn); upload_bits(dev, Zt, hZt);
+  upload_bits(dev, Tt, hTt); upload_bits(dev, St, hSt);
+  { HostBits().b.swap(hAn.b); }
+
+  // control block: masks and index lists, uploaded together once per step
+  size_t off = 0;
+  auto reserve = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
+  size_t o_m[11]; int mn[11] = { NA, NA, NA, NA, NZ, NZ, M, M, M, M, T };
+  for (int i = 0; i < 11; i++) o_m[i] = reserve(bsize(mn[i]) + 8);
+  IList* L[8] = { &ar, &kl, &tl, &zr, &rtn, &rmv, &crt, &crm };
+  int lcap[8] = { NA, M, T, NZ, NA, NA, M, M };
+  size_t o_l[8];
+  for (int i = 0; i < 8; i++) o_l[i] = reserve(sizeof(int) * (size_t)(lcap[i] + 1));
+  size_t o_zero = reserve(sizeof(int) * 4);
+  ctl_bytes = off;
+  HIPCHK(hipHostMalloc((void**)&hctl, ctl_bytes, hipHostMallocDefault));
+  memset(hctl, 0, ctl_bytes);
+  dctl = dev.alloc<uint8_t>(ctl_bytes);
+  uint32_t** dm[11] = { &Ar_b, &Ar_j, &Ar_rtn, &Ar_rmv, &Zr_b, &Zr_j, &Ac_b, &Ac_j, &Ac_rtn, &Ac_rmv, &Tc_j };
+  for (int i = 0; i < 11; i++) { *dm[i] = (uint32_t*)(dctl + o_m[i]); ctl_mask_off[i] = o_m[i]; }
+  for (int i = 0; i < 8; i++) { L[i]->d = (int*)(dctl + o_l[i]); ctl_list_off[i] = o_l[i]; }
+  d_zero = (int*)(dctl + o_zero);
+
+  // A_p per 1920-column block (bcc_load_A_p, bcc.c:1197-1313)
+  hA_p.assign(pcols, 0.5);
+  push_ctl();
+  {
+    int* cnt = dev.alloc<int>((size_t)M * 2);
+    popc_rows(At.b(), nullptr, M, Ar_b, (int)(At.rb / 4), cnt, st);
+    if (has_Z) popc_rows(Zt.b(), nullptr, M, Zr_b, (int)(Zt.rb / 4), cnt + M, st);
+    std::vector<int> hc((size_t)M * 2, 0);
+    HIPCHK(hipMemcpyAsync(hc.data(), cnt, sizeof(int) * (size_t)M * (has_Z ? 2 : 1), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<double> AZ(M, 0.0);
+    for (int k = 0; k < M; k++) {
+      double a = 0, z = 0;
+      if (bget(hAc_b.data(), k)) { a = (double)hc[k]; if (has_Z) z = (double)hc[(size_t)M + k]; }
+      AZ[k] = 0; AZ[k] += a; AZ[k] += z;
+    }
+    if (!c.A_p.empty()) {
+      FILE* fp = fopen(c.A_p.c_str(), "rb"); int hdr[3];
+      bool ok = fp && fread(hdr, sizeof(int), 3, fp) == 3 && hdr[1] == pcols &&
+                fread(hA_p.data(), sizeof(double), pcols, fp) == (size_t)pcols;
+      if (fp) fclose(fp);
+      if (!ok) throw Error(LKC_ERR_IO, "cannot read A_p");
+    } else if (c.sparse) {
+      for (int b = 0; b < pcols; b++) {
+        int s0 = std::min(M - 1, std::max(0, b * kPopBits)), s1 = std::min(M - 1, std::max(0, (b + 1) * kPopBits - 1));
+        double tsum = 0, tnum = 0;
+        for (int k = s0; k <= s1; k++) if (bget(hAc_b.data(), k)) { tsum += AZ[k]; tnum += 1; }
+        if (tnum == 0) hA_p[b] = 0.5;
+        else { double v = (tsum) / (tnum * (Arb + Zrb)); hA_p[b] = v < 0.01 ? 0.01 : (v > 0.99 ? 0.99 : v); }
+      }
+    }
+  }
+  // calc_A_ajdk (An_ajdk_v.c:5-30)
+  hajdk.assign((size_t)pcols * kAjdk, 0.0);
+  for (int j = 0; j < 5; j++) for (int k = 0; k < 3; k++) {
+    int jk = j + 5 * k;
+    for (int nc = 0; nc < pcols; nc++)
+      hajdk[nc + (size_t)jk * pcols] = pow(hA_p[nc] - (1 - hA_p[nc]), j) * pow(1.0 / std::max(0.01, (4.0 * hA_p[nc] * (1 - hA_p[nc]))), k);
+  }
+  ajdk = dev.alloc<double>(hajdk.size());
+  HIPCHK(hipMemcpy(ajdk, hajdk.data(), sizeof(double) * hajdk.size(), hipMemcpyHostToDevice));
+
+  // xcalc orientation flags (bcc_double_init_lf, bcc.c:1005-1150)
+  int rdrop0 = 0; get_xdrop(Arb, popmask(hAc_b, M), c.gamma, c.gamma_type, &rdrop0, NULL);
+  int rdrop2 = std::max(2, rdrop0);
+  trm_AnZt = (Zrb <= Arb) ? 0 : 1;
+  trm_knzt = (Zrb <= std::min(rdrop2, Arb)) ? 0 : 1;
+  trm_hnvt = trm_knzt;
+
+  // persistent device arrays
+  size_t nA = NA, nZ = std::max(NZ, 1), nM = M, nT = T;
+  mlf = dev.alloc<double>(16 * 8);
+  lf_An_ajdk = dev.alloc<double>(nA * kAjdk); lf_Zn_ajdk = dev.alloc<double>(nZ * kAjdk);
+  jn_ajdk = dev.alloc<double>(nA * kAjdk); vn_ajdk = dev.alloc<double>(nZ * kAjdk);
+  kn_ajdk = dev.alloc<double>(nA * kAjdk); hn_ajdk = dev.alloc<double>(nA * kAjdk);
+  an_ajdk_ = dev.alloc<double>(nA * kAjdk); zn_ajdk = dev.alloc<double>(nZ * kAjdk);
+  lf_AtTn = dev.alloc<double>(nM * nT); lf_ZtSn = dev.alloc<double>(nM * nT);
+  lf_AnZt = dev.alloc<double>(nA * nZ); lf_AnAt = dev.alloc<double>(nA * nA);
+  lf_AtTAnZtSZn = dev.alloc<double>(nM * nT); lf_AtTAnAtTAn = dev.alloc<double>(nM * nT);
+  lf_AnZtSZnAt = dev.alloc<double>(nA * nT); lf_AnAtTAnAt = dev.alloc<double>(nA * nT);
+  At_An_a1d1 = dev.alloc<double>(nM); et_An = dev.alloc<double>(nM);
+  An_a2d2_ZtSn = dev.alloc<double>(nA * nT); An_a2d2_AtTn = dev.alloc<double>(nA * nT);
+  sc = dev.alloc<double>(6 * nT + 2);
+  T_AnAt_AnAt = dev.alloc<double>(nT * nA);
+  QR_ZSZ = dev.alloc<double>(nA * nT); QR_TAT = dev.alloc<double>(nA * nT);
+  QC_ZSZ = dev.alloc<double>(nM * nT); QC_TAT = dev.alloc<double>(nM * nT);
+  for (int i = 0; i < 14; i++) mt[i] = dev.alloc<double>(nM * nT);
+  {
+    // zero the persistent arrays once: entries outside the active sets then read as 0
+    double* zs[] = { lf_An_ajdk, lf_Zn_ajdk, lf_AtTn, lf_ZtSn, lf_AnZt, lf_AnAt, lf_AtTAnZtSZn, lf_AtTAnAtTAn,
+                     lf_AnZtSZnAt, lf_AnAtTAnAt, At_An_a1d1, et_An, An_a2d2_ZtSn, An_a2d2_AtTn, T_AnAt_AnAt,
+                     QR_ZSZ, QR_TAT, QC_ZSZ, QC_TAT };
+    size_t zn[] = { nA * kAjdk, nZ * kAjdk, nM * nT, nM * nT, nA * nZ, nA * nA, nM * nT, nM * nT,
+                    nA * nT, nA * nT, nM, nM, nA * nT, nA * nT, nT * nA, nA * nT, nA * nT, nM * nT, nM * nT };
+    for (size_t i = 0; i < sizeof zs / sizeof zs[0]; i++) fill(zs[i], (long long)zn[i], 0.0, st);
+  }
+  cnt_buf = dev.alloc<int>(std::max(nM, nA) + nT + 8);
+  xwork = dev.alloc<double>(1024);
+  scal = dev.alloc<double>(16);
+  dsv = dev.alloc<double>(nM + nA);
+  HIPCHK(hipHostMalloc((void**)&hsv, sizeof(double) * (nM + nA), hipHostMallocDefault));
+  trace.reserve((size_t)(M + NA + 2) * 6);
+  HIPCHK(hipStreamSynchronize(st));
+  load_s = now_s() - t0;
+}
+
+// masks and lists: host -> pinned block -> device (one copy)
+void Run::push_ctl()
+{
+  const std::vector<uint8_t>* hm[11] = { &hAr_b, &hAr_j, &hAr_rtn, &hAr_rmv, &hZr_b, &hZr_j, &hAc_b, &hAc_j, &hAc_rtn, &hAc_rmv, &hTc_j };
+  int mn[11] = { NA, NA, NA, NA, NZ, NZ, M, M, M, M, T };
+  for (int i = 0; i < 11; i++) memcpy(hctl + ctl_mask_off[i], hm[i]->data(), bsize(mn[i]));
+  auto fill_list = [&](IList& l, const std::vector<uint8_t>& m, int n) {
+    l.h.clear();
+    for (int i = 0; i < n; i++) if (bget(m.data(), i)) l.h.push_back(i);
+  };
+  fill_list(ar, hAr_j, NA); fill_list(kl, hAc_j, M); fill_list(tl, hTc_j, T); fill_list(zr, hZr_j, NZ);
+  fill_list(rtn, hAr_rtn, NA); fill_list(rmv, hAr_rmv, NA); fill_list(crt, hAc_rtn, M); fill_list(crm, hAc_rmv, M);
+  IList* L[8] = { &ar, &kl, &tl, &zr, &rtn, &rmv, &crt, &crm };
+  size_t last = 0;
+  for (int i = 0; i < 8; i++) {
+    if (!L[i]->h.empty()) memcpy(hctl + ctl_list_off[i], L[i]->h.data(), sizeof(int) * L[i]->h.size());
+    last = std::max(last, ctl_list_off[i] + sizeof(int) * L[i]->h.size());
+  }
+  HIPCHK(hipMemcpyAsync(dctl, hctl, ctl_bytes, hipMemcpyHostToDevice, st));
+}
+
+// ------------------------------------------------------------------
+// stages
+// ------------------------------------------------------------------
+// bcc_An_ajdk (bcc_lf.c:7-51)
+void Run::st_ajdk()
+{
+  mask_lf(Ac_j, pcols, ajdk, mlf + 0 * 16, st);
+  an_ajdk(An.b(), ar.d, ar.n(), Ac_j, pcols, ajdk, mlf, lf_An_ajdk, NA, st);
+  ops += 15LL * ar.n() * (long long)An.rb * 8;
+  if (has_Z) {
+    an_ajdk(Zn.b(), zr.d, zr.n(), Ac_j, pcols, ajdk, mlf, lf_Zn_ajdk, NZ, st);
+    ops += 15LL * zr.n() * (long long)Zn.rb * 8;
+  }
+}
+
+// bcc_lf_ZtSn (bcc_lf.c:53-101): plain +-1 products At_k . T_t over the active rows
+void Run::st_ZtSn()
+{
+  mask_lf(Ar_j, psize(NA), nullptr, mlf + 3 * 16, st);
+  gram(At.b(), kl.d, kl.n(), Tt.b(), tl.d, tl.n(), Ar_j, psize(NA), nullptr, mlf + 3 * 16,
+       nullptr, nullptr, 0, lf_AtTn, M, 1, 1, 0, st);
+  ops += (long long)kl.n() * tl.n() * (long long)bsize(NA) * 8;
+  if (has_Z) {
+    mask_lf(Zr_j, psize(NZ), nullptr, mlf + 4 * 16, st);
+    gram(Zt.b(), kl.d, kl.n(), St.b(), tl.d, tl.n(), Zr_j, psize(NZ), nullptr, mlf + 4 * 16,
+         nullptr, nullptr, 0, lf_ZtSn, M, 1, 1, 0, st);
+    ops += (long long)kl.n() * tl.n() * (long long)bsize(NZ) * 8;
+  }
+}
+
+// bcc_lf_YnWt (bcc_lf.c:223-302): centred Grams; lf_AnZt[z*NA + a], lf_AnAt[a2*NA + a1]
+void Run::st_YnWt()
+{
+  if (has_Z) {
+    gram(An.b(), ar.d, ar.n(), Zn.b(), zr.d, zr.n(), Ac_j, pcols, ajdk, mlf,
+         lf_An_ajdk + 6 * (size_t)NA, lf_Zn_ajdk + 6 * (size_t)NZ, 1, lf_AnZt, NA, 1, 1, 0, st);
+    ops += (long long)ar.n() * zr.n() * (long long)bsize(M) * 8;
+  }
+  gram(An.b(), ar.d, ar.n(), An.b(), ar.d, ar.n(), Ac_j, pcols, ajdk, mlf,
+       lf_An_ajdk + 6 * (size_t)NA, lf_An_ajdk + 6 * (size_t)NA, 1, lf_AnAt, NA, 1, 1, 0, st);
+  ops += (long long)ar.n() * ar.n() * (long long)bsize(M) * 8;
+}
+
+// bcc_M_YnWt_excerpt + bcc_lf_At_T_YnWt_S_Zn_excerpt (bcc_lf.c:304-316, 615-634).
+// L(r1, r2) = src[idx1(r1) + idx2(r2)*ld2] with idx = the side's original index when full.
+void Run::ww_excerpt(int trm, const Side& s1, const Side& s2, const IList& kls, const double* src, long long ld2,
+                     int full, double* out)
+{
+  const Side& pn = trm ? s2 : s1;
+  const Side& pw = trm ? s1 : s2;
+  int nn = pn.rows->n(), nw = pw.rows->n();
+  long long sn, sw; const int *rl = nullptr, *cl = nullptr;
+  if (!trm) { sn = 1; sw = ld2; if (full) { rl = s1.rows->d; cl = s2.rows->d; } }
+  else { sn = ld2; sw = 1; if (full) { rl = s2.rows->d; cl = s1.rows->d; } }
+  int n_total = pn.X->nc;
+  long long ldc = std::max(n_total, 1);
+  uint32_t* cz = (uint32_t*)g_cz.need(dev, sizeof(uint32_t) * (size_t)std::max(nw, 1) * ldc);
+  size_t prb = pn.X->rb;
+  uint8_t* planes = (uint8_t*)g_planes.need(dev, (size_t)std::max(nw, 1) * c.B * prb);
+  xcalc(src, sn, sw, rl, cl, nn, nw, pn.rows->d, c.B, c.D_MLT, cz, ldc, scal, xwork, st);
+  cz_planes(cz, ldc, nw, n_total, pn.mask, c.B, planes, prb, st);
+  ops += (long long)nn * nw * (c.B + 1) * 8;
+  WwArgs a;
+  a.P = pn.X->b(); a.klist = kls.d; a.qlist = kls.d; a.nk = kls.n(); a.pm = pn.mask;
+  a.Tc = pn.Cov->b(); a.tlist = tl.d; a.slist = tl.d; a.nt = tl.n();
+  a.planes = planes; a.prb = prb; a.row0 = pn.mask; a.nw = nw; a.wlist = pw.rows->d; a.B = c.B;
+  a.cz = cz; a.ldc = ldc;
+  a.Sc = pw.Cov->b(); a.Q = pw.X->b(); a.ajdk = ajdk; a.pcols = pcols; a.scal = scal;
+  a.out = out; a.ldo = M;
+  size_t wb = ww_work_bytes(a.nk, a.nt, nw);
+  double* work = (double*)g_work.need(dev, wb);
+  ww_valu(a, work, wb, st);
+  ops += (long long)tl.n() * kls.n() * nw * (c.B + 1) * (long long)prb * 8;
+}
+
+// bcc_M_YnWt + bcc_lf_At_T_YnWt_S_Zn (bcc_lf.c:318-356, 636-709), initial pass
+void Run::st_At_T()
+{
+  Side sA{ &At, Ar_j, &ar, &Tt }, sZ{ &Zt, Zr_j, &zr, &St };
+  if (has_Z) ww_excerpt(trm_AnZt, sA, sZ, kl, lf_AnZt, NA, 1, lf_AtTAnZtSZn);
+  ww_excerpt(0, sA, sA, kl, lf_AnAt, NA, 1, lf_AtTAnAtTAn);
+}
+
+// bcc_lf_AnZt_S_WnYt (bcc_lf.c:360-409)
+void Run::st_QR()
+{
+  if (has_Z) qr(lf_AnZt, NA, ar.d, ar.n(), zr.d, zr.n(), St.b(), tl.d, tl.n(), lf_AnZtSZnAt, NA, st);
+  qr(lf_AnAt, NA, ar.d, ar.n(), ar.d, ar.n(), Tt.b(), tl.d, tl.n(), lf_AnAtTAnAt, NA, st);
+}
+
+// bcc_singlestudy_ww (bcc_flattenloop.c:753-831) and get_singlestudy_vv (bcc_flattenloop.c:359-731)
+void Run::st_single()
+{
+  const double* x6 = lf_An_ajdk + 6 * (size_t)NA;
+  // At_Yn_a1d1: xcalc of the rows' a1d1 values, then An_ZtSn_ww over the columns
+  size_t prb = At.rb;
+  uint32_t* cz = (uint32_t*)g_cz.need(dev, sizeof(uint32_t) * (size_t)NA);
+  uint8_t* planes = (uint8_t*)g_planes.need(dev, (size_t)c.B * prb);
+  xcalc(x6, 1, 0, ar.d, nullptr, ar.n(), 1, ar.d, c.B, c.D_MLT, cz, NA, scal, xwork, st);
+  cz_planes(cz, NA, 1, NA, Ar_j, c.B, planes, prb, st);
+  ops += (long long)ar.n() * (c.B + 1) * 8;
+  an_ztsn(At.b(), kl.d, kl.n(), Ar_j, planes, prb, Ar_j, 1, d_zero, c.B, scal, At_An_a1d1, 0, st);
+  ops += (long long)kl.n() * (c.B + 1) * (long long)prb * 8;
+  // ordered sums over the active rows
+  fill(sc, 6 * (long long)T + 2, 0.0, st);
+  OrderedJobs J; J.n = 0;
+  auto add = [&](const IList& l, const DMat* sgn, int t, const double* vals, double* out) {
+    OrderedJob& q = J.job[J.n++];
+    q.list = l.d; q.n = l.n(); q.sign = sgn ? sgn->p + (size_t)t * sgn->rb : nullptr; q.vals = vals; q.out = out;
+  };
+  for (int t : tl.h) {
+    add(ar, &Tt, t, lf_An_ajdk + 11 * (size_t)NA, sc + 0 * T + t);
+    add(ar, &Tt, t, lf_An_ajdk + 13 * (size_t)NA, sc + 1 * T + t);
+    if (has_Z) {
+      add(zr, &St, t, lf_Zn_ajdk + 11 * (size_t)NZ, sc + 2 * T + t);
+      add(zr, &St, t, lf_Zn_ajdk + 13 * (size_t)NZ, sc + 3 * T + t);
+    }
+  }
+  add(ar, nullptr, 0, x6, sc + 6 * T);
+  ordered_sums(J, st);
+  // et_An[k] = -|rows| + 2 pop(At_k & rows); et_Tn, et_Sn likewise
+  popc_rows(At.b(), kl.d, kl.n(), Ar_j, (int)(At.rb / 4), cnt_buf, st);
+  et_from_counts(cnt_buf, kl.d, kl.n(), ar.n(), et_An, 0, st);
+  int* ct = cnt_buf + std::max(M, NA);
+  popc_rows(Tt.b(), tl.d, tl.n(), Ar_j, (int)(Tt.rb / 4), ct, st);
+  et_from_counts(ct, tl.d, tl.n(), ar.n(), sc + 4 * T, 1, st);
+  if (has_Z) {
+    popc_rows(St.b(), tl.d, tl.n(), Zr_j, (int)(St.rb / 4), ct, st);
+    et_from_counts(ct, tl.d, tl.n(), zr.n(), sc + 5 * T, 1, st);
+  }
+  t_anat(ar.d, ar.n(), tl.d, tl.n(), Tt.b(), x6, mlf, T_AnAt_AnAt, NA, st);
+}
+
+// bcc_doublestudy_ww (bcc_flattenloop.c:977-1029) with An_a2d2_ZtSn_excerpt (An_ZtSn_ww.c:267-290)
+void Run::st_double()
+{
+  const double* a2d2 = ajdk + 12 * (size_t)pcols;
+  int nk = kl.n(), nt = tl.n();
+  double* L = (double*)g_L.need(dev, sizeof(double) * ((size_t)std::max(nk, 1) * std::max(nt, 1)));
+  size_t prb = An.rb;
+  uint32_t* cz = (uint32_t*)g_cz.need(dev, sizeof(uint32_t) * (size_t)std::max(nt, 1) * M);
+  uint8_t* planes = (uint8_t*)g_planes.need(dev, (size_t)std::max(nt, 1) * c.B * prb);
+  for (int pass = 0; pass < 2; pass++) {
+    if (pass == 0 && !has_Z) continue;
+    const double* src = pass == 0 ? lf_ZtSn : lf_AtTn;
+    double* out = pass == 0 ? An_a2d2_ZtSn : An_a2d2_AtTn;
+    scale_rows(src, M, kl.d, nk, tl.d, nt, a2d2, L, st);
+    xcalc(L, 1, nk, nullptr, nullptr, nk, nt, kl.d, c.B, c.D_MLT, cz, M, scal, xwork, st);
+    cz_planes(cz, M, nt, M, Ac_j, c.B, planes, prb, st);
+    an_ztsn(An.b(), ar.d, ar.n(), Ac_j, planes, prb, Ac_j, nt, tl.d, c.B, scal, out, NA, st);
+    ops += (long long)ar.n() * nt * (c.B + 1) * (long long)prb * 8;
+  }
+}
+
+// get_flattenloop (bcc_flattenloop.c:1431-1889), one bin, no Y
+void Run::st_flatten()
+{
+  FlatQR f;
+  f.ar = ar.d; f.nar = ar.n(); f.tl = tl.d; f.nt = tl.n(); f.T = T; f.NA = NA; f.has_Z = has_Z;
+  f.ajdk_rows = lf_An_ajdk; f.mlf = mlf; f.sc = sc;
+  f.lf_AnZtSZnAt = lf_AnZtSZnAt; f.lf_AnAtTAnAt = lf_AnAtTAnAt;
+  f.An_a2d2_ZtSn = An_a2d2_ZtSn; f.An_a2d2_AtTn = An_a2d2_AtTn; f.T_AnAt_AnAt = T_AnAt_AnAt;
+  f.Tt = Tt.p; f.rbT = Tt.rb; f.QR_ZSZ = QR_ZSZ; f.QR_TAT = QR_TAT;
+  flat_qr(f, st);
+  FlatQC g;
+  g.kl = kl.d; g.nk = kl.n(); g.tl = tl.d; g.nt = tl.n(); g.T = T; g.M = M; g.has_Z = has_Z;
+  g.ajdk = ajdk; g.pcols = pcols; g.mlf = mlf; g.sc = sc;
+  g.lf_AtTAnZtSZn = lf_AtTAnZtSZn; g.lf_AtTAnAtTAn = lf_AtTAnAtTAn; g.lf_AtTn = lf_AtTn; g.lf_ZtSn = lf_ZtSn;
+  g.et_An = et_An; g.At_An_a1d1 = At_An_a1d1; g.rpop = ar.n();
+  g.QC_ZSZ = QC_ZSZ; g.QC_TAT = QC_TAT;
+  flat_qc(g, st);
+}
+
+void Run::init_lf()
+{
+  if (inited) throw Error(LKC_ERR_STATE, "already initialised");
+  double t0 = now_s();
+  st_ajdk(); st_ZtSn(); st_YnWt(); st_At_T(); st_QR();
+  st_single(); st_double(); st_flatten();
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipGetLastError());
+  inited = true;
+  init_s = now_s() - t0;
+}
+
+// bcc_sumscores_nrm/ifT/cmb/xij/mxA (bcc_sumscores.c:37-779), one bin, Ireq<=0
+void Run::sumscores()
+{
+  unsigned int Ar = (unsigned int)ar.n(), Zr = (unsigned int)zr.n();
+  long long cpop = kl.n();
+  auto den = [](long long lld) { double d = (double)lld; if (fabs(d) < 0.000000000001) d = 1; return d; };
+  int n_mds = std::max(0, tl.n() - 1); if (n_mds > 6) n_mds = 6;
+  double dmds = n_mds > 0 ? n_mds * c.kappa2 : 0.0;
+  Scores q;
+  q.list = ar.d; q.n = ar.n(); q.tl = tl.d; q.nt = tl.n(); q.ld = NA; q.has_Z = has_Z;
+  q.ZSZ = QR_ZSZ; q.TAT = QR_TAT;
+  q.dZ = den((long long)(Zr - 0) * (long long)(cpop) * (long long)(cpop - 1));
+  q.dT = den((long long)(Ar - 1) * (long long)(cpop) * (long long)(cpop - 1));
+  q.den = dmds; q.pow_form = 1; q.cmb = has_Z; q.out = dsv + M;
+  scores(q, st);
+  Scores p;
+  p.list = kl.d; p.n = kl.n(); p.tl = tl.d; p.nt = tl.n(); p.ld = M; p.has_Z = has_Z;
+  p.ZSZ = QC_ZSZ; p.TAT = QC_TAT;
+  p.dZ = den((long long)(Ar) * (long long)(cpop - 1) * (long long)(Zr));
+  p.dT = den((long long)(Ar) * (long long)(cpop - 1) * (long long)(Ar - 1));
+  p.den = dmds; p.pow_form = 0; p.cmb = has_Z && Zrb > 0 && Arb > 0; p.out = dsv;
+  scores(p, st);
+  int nk = kl.n(), na = ar.n();
+  if (nk) HIPCHK(hipMemcpyAsync(hsv, dsv, sizeof(double) * nk, hipMemcpyDeviceToHost, st));
+  if (na) HIPCHK(hipMemcpyAsync(hsv + M, dsv + M, sizeof(double) * na, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipGetLastError());
+  // xij: ascending sort of the column then the row scores
+  int Irem = na > 0 ? 1 : 0;
+  QC_sv.assign(hsv, hsv + nk); QC_perm.resize(nk);
+  QR_sv.assign(hsv + M, hsv + M + na); QR_perm.resize(na);
+  for (int j = 0; j < nk; j++) QC_perm[j] = j;
+  for (int j = 0; j < na; j++) QR_perm[j] = j;
+  sort_scores(QC_sv.data(), QC_perm.data(), nk);
+  sort_scores(QR_sv.data(), QR_perm.data(), na);
+  // mxA: drop counts, _rtn/_rmv masks, trace and xdrop lines
+  int rpop = na, cpop_i = nk, rdrop = 0, cdrop = 0;
+  get_xdrop(rpop, cpop_i, c.gamma, c.gamma_type, &rdrop, &cdrop);
+  hAc_rtn = hAc_j; std::fill(hAc_rmv.begin(), hAc_rmv.end(), 0);
+  hAr_rtn = hAr_j; std::fill(hAr_rmv.begin(), hAr_rmv.end(), 0);
+  int nc_drop = std::min(cdrop, cpop_i), nr_drop = std::min(rdrop, rpop);
+  for (int i = 0; i < nc_drop; i++) { int k = kl.h[QC_perm[i]]; bset(hAc_rmv.data(), k); bclr(hAc_rtn.data(), k); }
+  for (int i = 0; i < nr_drop; i++) { int a = ar.h[QR_perm[i]]; bset(hAr_rmv.data(), a); bclr(hAr_rtn.data(), a); }
+  auto mean = [](const std::vector<double>& v, unsigned long long n) {
+    double m = 0; for (unsigned long long i = 0; i < n; i++) m += v[i]; m /= (double)n; return m;
+  };
+  double mr = mean(QR_sv, (unsigned long long)rpop); if (!std::isfinite(mr)) mr = 1.0;
+  double mc = mean(QC_sv, (unsigned long long)cpop_i); if (!std::isfinite(mc)) mc = 1.0;
+  double tr[6] = { (double)n_iter, (double)rpop, (double)cpop_i, mr, mc, (double)Irem };
+  trace.insert(trace.end(), tr, tr + 6);
+  for (int i = 0; i < nr_drop; i++) {
+    int a = ar.h[QR_perm[i]];
+    xdrop_a.push_back(a); xdrop_a.push_back(-1); xdrop_b.push_back(Ar_bidx[a]); xdrop_b.push_back(-1);
+  }
+  for (int i = 0; i < nc_drop; i++) {
+    int k = kl.h[QC_perm[i]];
+    xdrop_a.push_back(-1); xdrop_a.push_back(k); xdrop_b.push_back(-1); xdrop_b.push_back(Ac_bidx[k]);
+  }
+  // bcc_lrup_mxdup takes effect for the re-scoring; the update itself uses _rtn/_rmv
+  hAr_j = hAr_rtn; hAc_j = hAc_rtn;
+  push_ctl();
+}
+
+// low-rank update after a drop (bcc_lrup.c:5-1138)
+void Run::lrup()
+{
+  const int sA = psize(NA), sZ = psize(NZ);
+  double* mlf_rmv = mlf + 1 * 16;     // column masks
+  double* mlf_rtn = mlf + 2 * 16;
+  double* mlf_nrtn = mlf + 5 * 16;    // row masks
+  double* mlf_nrmv = mlf + 6 * 16;
+  double* mlf_nz = mlf + 7 * 16;
+  mask_lf(Ac_rmv, pcols, ajdk, mlf_rmv, st);
+  mask_lf(Ac_rtn, pcols, ajdk, mlf_rtn, st);
+  const long long rbm = (long long)An.rb * 8;
+  // QR stages 0/1/3: lf_AnZt -= jn.vt, lf_AnAt -= jn.jt over the dropped columns
+  an_ajdk(An.b(), rtn.d, rtn.n(), Ac_rmv, pcols, ajdk, mlf_rmv, jn_ajdk, NA, st);
+  ops += 15LL * rtn.n() * rbm;
+  if (has_Z) {
+    an_ajdk(Zn.b(), zr.d, zr.n(), Ac_rmv, pcols, ajdk, mlf_rmv, vn_ajdk, NZ, st);
+    ops += 15LL * zr.n() * (long long)Zn.rb * 8;
+    gram(An.b(), rtn.d, rtn.n(), Zn.b(), zr.d, zr.n(), Ac_rmv, pcols, ajdk, mlf_rmv,
+         jn_ajdk + 6 * (size_t)NA, vn_ajdk + 6 * (size_t)NZ, 1, lf_AnZt, NA, 1, 1, 1, st);
+    ops += (long long)rtn.n() * zr.n() * rbm;
+  }
+  gram(An.b(), rtn.d, rtn.n(), An.b(), rtn.d, rtn.n(), Ac_rmv, pcols, ajdk, mlf_rmv,
+       jn_ajdk + 6 * (size_t)NA, jn_ajdk + 6 * (size_t)NA, 1, lf_AnAt, NA, 1, 1, 1, st);
+  ops += (long long)rtn.n() * rtn.n() * rbm;
+  // QC a0
+  an_ajdk(An.b(), rmv.d, rmv.n(), Ac_rtn, pcols, ajdk, mlf_rtn, kn_ajdk, NA, st);
+  an_ajdk(An.b(), rmv.d, rmv.n(), Ac_rmv, pcols, ajdk, mlf_rmv, hn_ajdk, NA, st);
+  an_ajdk(An.b(), rtn.d, rtn.n(), Ac_rtn, pcols, ajdk, mlf_rtn, an_ajdk_, NA, st);
+  ops += 15LL * (2LL * rmv.n() + rtn.n()) * rbm;
+  if (has_Z) {
+    an_ajdk(Zn.b(), zr.d, zr.n(), Ac_rtn, pcols, ajdk, mlf_rtn, zn_ajdk, NZ, st);
+    an_ajdk(Zn.b(), zr.d, zr.n(), Ac_rmv, pcols, ajdk, mlf_rmv, vn_ajdk, NZ, st);
+    ops += 30LL * zr.n() * (long long)Zn.rb * 8;
+  }
+  // QC a1: compact Grams G[j*n1 + i]
+  int nrm = rmv.n(), nrt = rtn.n(), nz = zr.n();
+  double *knzt = nullptr, *hnvt = nullptr;
+  if (has_Z) {
+    knzt = (double*)g_knzt.need(dev, sizeof(double) * ((size_t)nrm * nz + 1));
+    hnvt = (double*)g_hnvt.need(dev, sizeof(double) * ((size_t)nrm * nz + 1));
+    gram(An.b(), rmv.d, nrm, Zn.b(), zr.d, nz, Ac_rtn, pcols, ajdk, mlf_rtn,
+         kn_ajdk + 6 * (size_t)NA, zn_ajdk + 6 * (size_t)NZ, 1, knzt, nrm, 0, 0, 0, st);
+    gram(An.b(), rmv.d, nrm, Zn.b(), zr.d, nz, Ac_rmv, pcols, ajdk, mlf_rmv,
+         hn_ajdk + 6 * (size_t)NA, vn_ajdk + 6 * (size_t)NZ, 1, hnvt, nrm, 0, 0, 0, st);
+    ops += 2LL * nrm * nz * rbm;
+  }
+  double* ankt = (double*)g_ankt.need(dev, sizeof(double) * ((size_t)nrt * nrm + 1));
+  double* knkt = (double*)g_knkt.need(dev, sizeof(double) * ((size_t)nrm * nrm + 1));
+  gram(An.b(), rtn.d, nrt, An.b(), rmv.d, nrm, Ac_rtn, pcols, ajdk, mlf_rtn,
+       an_ajdk_ + 6 * (size_t)NA, kn_ajdk + 6 * (size_t)NA, 1, ankt, nrt, 0, 0, 0, st);
+  gram(An.b(), rmv.d, nrm, An.b(), rmv.d, nrm, Ac_rtn, pcols, ajdk, mlf_rtn,
+       kn_ajdk + 6 * (size_t)NA, kn_ajdk + 6 * (size_t)NA, 1, knkt, nrm, 0, 0, 0, st);
+  ops += ((long long)nrt * nrm + (long long)nrm * nrm) * rbm;
+  // QC a2 + a3
+  double *t_knzt = mt[0], *t_hnvt = mt[1], *t_ankt = mt[2], *t_knkt = mt[3];
+  Side s_rmv{ &At, Ar_rmv, &rmv, &Tt }, s_rtn{ &At, Ar_rtn, &rtn, &Tt }, s_Z{ &Zt, Zr_j, &zr, &St };
+  if (has_Z) {
+    ww_excerpt(trm_knzt, s_rmv, s_Z, crt, knzt, nrm, 0, t_knzt);
+    ww_excerpt(trm_hnvt, s_rmv, s_Z, crt, hnvt, nrm, 0, t_hnvt);
+  }
+  ww_excerpt(0, s_rtn, s_rmv, crt, ankt, nrt, 0, t_ankt);
+  ww_excerpt(0, s_rmv, s_rmv, crt, knkt, nrm, 0, t_knkt);
+  // QC b0: plain +-1 products, [t*M + k]
+  double *attn = mt[4], *jttn = mt[5], *ktrn = mt[6], *htrn = mt[7], *ztsn = mt[8], *vtsn = mt[9];
+  mask_lf(Ar_rtn, sA, nullptr, mlf_nrtn, st);
+  mask_lf(Ar_rmv, sA, nullptr, mlf_nrmv, st);
+  const long long rbn = (long long)At.rb * 8;
+  gram(At.b(), crt.d, crt.n(), Tt.b(), tl.d, tl.n(), Ar_rtn, sA, nullptr, mlf_nrtn, nullptr, nullptr, 0, attn, M, 1, 1, 0, st);
+  gram(At.b(), crm.d, crm.n(), Tt.b(), tl.d, tl.n(), Ar_rtn, sA, nullptr, mlf_nrtn, nullptr, nullptr, 0, jttn, M, 1, 1, 0, st);
+  gram(At.b(), crt.d, crt.n(), Tt.b(), tl.d, tl.n(), Ar_rmv, sA, nullptr, mlf_nrmv, nullptr, nullptr, 0, ktrn, M, 1, 1, 0, st);
+  gram(At.b(), crm.d, crm.n(), Tt.b(), tl.d, tl.n(), Ar_rmv, sA, nullptr, mlf_nrmv, nullptr, nullptr, 0, htrn, M, 1, 1, 0, st);
+  ops += 2LL * (crt.n() + crm.n()) * tl.n() * rbn;
+  if (has_Z) {
+    mask_lf(Zr_j, sZ, nullptr, mlf_nz, st);
+    gram(Zt.b(), crt.d, crt.n(), St.b(), tl.d, tl.n(), Zr_j, sZ, nullptr, mlf_nz, nullptr, nullptr, 0, ztsn, M, 1, 1, 0, st);
+    gram(Zt.b(), crm.d, crm.n(), St.b(), tl.d, tl.n(), Zr_j, sZ, nullptr, mlf_nz, nullptr, nullptr, 0, vtsn, M, 1, 1, 0, st);
+    ops += (long long)(crt.n() + crm.n()) * tl.n() * (long long)Zt.rb * 8;
+  }
+  // QC b1: compact [(y*nt + t)*nk + k]
+  size_t n3 = (size_t)crt.n() * crm.n() * tl.n() + 1;
+  double* ztsvn = has_Z ? (double*)g_ztsvn.need(dev, sizeof(double) * n3) : nullptr;
+  double* attjn = (double*)g_attjn.need(dev, sizeof(double) * n3);
+  double* ktrhn = (double*)g_ktrhn.need(dev, sizeof(double) * n3);
+  if (has_Z) attyn(Zt.b(), crt.d, crt.n(), Zr_j, nz, St.b(), tl.d, tl.n(), Zr_j, nz, Zt.b(), crm.d, crm.n(),
+                   ztsn, vtsn, M, ajdk, pcols, ztsvn, st);
+  attyn(At.b(), crt.d, crt.n(), Ar_rtn, nrt, Tt.b(), tl.d, tl.n(), Ar_rtn, nrt, At.b(), crm.d, crm.n(),
+        attn, jttn, M, ajdk, pcols, attjn, st);
+  attyn(At.b(), crt.d, crt.n(), Ar_rmv, nrm, Tt.b(), tl.d, tl.n(), Ar_rmv, nrm, At.b(), crm.d, crm.n(),
+        ktrn, htrn, M, ajdk, pcols, ktrhn, st);
+  ops += (long long)tl.n() * crt.n() * crm.n() * (rbn * 2 + (has_Z ? (long long)Zt.rb * 8 : 0));
+  // QC b2
+  double *vtszn = mt[10], *jttan = mt[11], *htrkn = mt[12], *hhtrkn = mt[13];
+  if (has_Z) attyn_wtszn(attjn, ztsvn, crt.d, crt.n(), crm.d, crm.n(), tl.d, tl.n(), ajdk, pcols, vtszn, M, st);
+  attyn_wtszn(attjn, attjn, crt.d, crt.n(), crm.d, crm.n(), tl.d, tl.n(), ajdk, pcols, jttan, M, st);
+  attyn_wtszn(attjn, ktrhn, crt.d, crt.n(), crm.d, crm.n(), tl.d, tl.n(), ajdk, pcols, htrkn, M, st);
+  attyn_wtszn(ktrhn, ktrhn, crt.d, crt.n(), crm.d, crm.n(), tl.d, tl.n(), ajdk, pcols, hhtrkn, M, st);
+  // QC stage c
+  StageC s;
+  s.M = M; s.T = T; s.has_Z = has_Z; s.Ac_rtn = Ac_rtn; s.Ac_b = Ac_b; s.Tc_j = Tc_j;
+  s.vtszn = vtszn; s.t_knzt = t_knzt; s.t_hnvt = t_hnvt; s.jttan = jttan; s.htrkn = htrkn; s.hhtrkn = hhtrkn;
+  s.t_ankt = t_ankt; s.t_knkt = t_knkt; s.ZSZ = lf_AtTAnZtSZn; s.TAT = lf_AtTAnAtTAn;
+  stage_c(s, st);
+}
+
+bool Run::active() const { return !ar.h.empty() || !kl.h.empty(); }
+
+void Run::iterate(int* done)
+{
+  if (!inited) throw Error(LKC_ERR_STATE, "call init first");
+  if (!active()) { if (done) *done = 1; return; }
+  double t0 = now_s();
+  sumscores();
+  lrup();
+  st_QR(); st_ajdk(); st_ZtSn(); st_single(); st_double(); st_flatten();
+  HIPCHK(hipGetLastError());
+  n_iter++;
+  if (done) *done = active() ? 0 : 1;
+  if (!active()) HIPCHK(hipStreamSynchronize(st));
+  loop_s += now_s() - t0;
+}
+
+void Run::run(int max_iter)
+{
+  if (!inited) init_lf();
+  int done = active() ? 0 : 1;
+  double t0 = now_s();
+  loop_s = 0;
+  while (!done && (max_iter < 0 || n_iter < max_iter)) {
+    sumscores();
+    lrup();
+    st_QR(); st_ajdk(); st_ZtSn(); st_single(); st_double(); st_flatten();
+    n_iter++;
+    done = active() ? 0 : 1;
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipGetLastError());
+  loop_s = now_s() - t0;
+}
+
+void Run::write_outputs(const std::string& dir) const
+{
+  mkdir(dir.c_str(), 0755);
+  std::string p = dir + "/out_trace.txt";
+  FILE* fp = fopen(p.c_str(), "w");
+  if (!fp) throw Error(LKC_ERR_IO, "cannot write " + p);
+  for (size_t l = 0; l + 5 < trace.size(); l += 6) {
+    const double* t = &trace[l];
+    fprintf(fp, "%d %d %d %0.16f %0.16f %d\n", (int)t[0], (int)t[1], (int)t[2], t[3], t[4], (int)t[5]);
+  }
+  fclose(fp);
+  std::string pa = dir + "/out_xdrop_a.txt", pb = dir + "/out_xdrop_b.txt";
+  FILE* fa = fopen(pa.c_str(), "w");
+  if (!fa) throw Error(LKC_ERR_IO, "cannot write " + pa);
+  FILE* fb = fopen(pb.c_str(), "w");
+  if (!fb) { fclose(fa); throw Error(LKC_ERR_IO, "cannot write " + pb); }
+  for (size_t l = 0; l + 1 < xdrop_a.size(); l += 2) {
+    fprintf(fa, "%d %d\n", xdrop_a[l], xdrop_a[l + 1]);
+    fprintf(fb, "%d %d\n", xdrop_b[l], xdrop_b[l + 1]);
+  }
+  fclose(fa); fclose(fb);
+}
+
+// copy of a named state array in the layout of the CPU restatement ([row*T + t], [a*NZ + z], [r*15 + jk])
+long Run::debug_get(const std::string& name, std::vector<double>& out)
+{
+  HIPCHK(hipStreamSynchronize(st));
+  auto fetch = [&](const double* p, size_t n) {
+    std::vector<double> h(n);
+    if (n) HIPCHK(hipMemcpy(h.data(), p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return h;
+  };
+  auto transpose = [&](const double* p, size_t rows, size_t cols) {   // device [c*rows + r] -> [r*cols + c]
+    std::vector<double> h = fetch(p, rows * cols);
+    out.assign(rows * cols, 0.0);
+    for (size_t r = 0; r < rows; r++) for (size_t c = 0; c < cols; c++) out[r * cols + c] = h[c * rows + r];
+  };
+  size_t nA = NA, nZ = NZ, nM = M, nT = T;
+  struct { const char* nm; const double* p; size_t rows, cols; } mt_tab[] = {
+    { "lf_An_ajdk", lf_An_ajdk, nA, (size_t)kAjdk }, { "lf_Zn_ajdk", lf_Zn_ajdk, nZ, (size_t)kAjdk },
+    { "lf_AtTn", lf_AtTn, nM, nT }, { "lf_ZtSn", lf_ZtSn, nM, nT },
+    { "lf_AnZt", lf_AnZt, nA, nZ }, { "lf_AnAt", lf_AnAt, nA, nA },
+    { "lf_AtTAnZtSZn", lf_AtTAnZtSZn, nM, nT }, { "lf_AtTAnAtTAn", lf_AtTAnAtTAn, nM, nT },
+    { "lf_AnZtSZnAt", lf_AnZtSZnAt, nA, nT }, { "lf_AnAtTAnAt", lf_AnAtTAnAt, nA, nT },
+    { "At_An_a1d1", At_An_a1d1, nM, 1 }, { "An_a2d2_ZtSn", An_a2d2_ZtSn, nA, nT }, { "An_a2d2_AtTn", An_a2d2_AtTn, nA, nT },
+    { "et_An", et_An, nM, 1 }, { "T_AnAt_AnAt", T_AnAt_AnAt, nT * nA, 1 },
+    { "QR_ZSZ", QR_ZSZ, nA, nT }, { "QR_TAT", QR_TAT, nA, nT }, { "QC_ZSZ", QC_ZSZ, nM, nT }, { "QC_TAT", QC_TAT, nM, nT },
+  };
+  for (auto& e : mt_tab) if (name == e.nm) { transpose(e.p, e.rows, e.cols); return (long)out.size(); }
+  const char* scn[] = { "a1d2_AtTn", "a3d2_AtTn", "a1d2_ZtSn", "a3d2_ZtSn", "et_Tn", "et_Sn" };
+  std::vector<double> h = fetch(sc, 6 * nT + 1);
+  for (int i = 0; i < 6; i++) if (name == scn[i]) { out.assign(h.begin() + i * nT, h.begin() + (i + 1) * nT); return (long)nT; }
+  if (name == "et_An_a1d1") { out.assign(1, h[6 * nT]); return 1; }
+  return -1;
+}
+
+std::string Run::default_out_dir() const
+{
+  if (!c.dir_name.empty()) return c.dir_name;
+  if (!c.out_name.empty()) return (c.dir_base.empty() ? std::string(".") : c.dir_base) + "/dir_" + c.out_name;
+  return ".";
+}
+
+}  // namespace lkc

@@ -1,0 +1,99 @@
+// lkc_engine.h -- host-side state of one bcc elimination on the device.
+#pragma once
+#include "lkc_kernels.h"
+#include "../../include/lkc.h"
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace lkc {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+struct Cfg {
+  int B = 32; long long D_MLT = 4294967296LL; double gamma = 0.125; int gamma_type = 0; double kappa2 = 0;
+  int sparse = 1; int nbins = 0; int Ireq = 0; int A_ncols = 0, Y_ncols = 0, T_ncols = 0; int scramble_num = 0;
+  std::string QR, QC, A_n, A_t, A_rind, A_cind, A_p, Z_n, Z_t, Z_rind, T_n, T_t, T_cind, S_n, S_t;
+  std::string xpre, dir_name, dir_base, out_name;
+};
+void parse_cfg(const char* text, Cfg& c);
+void get_xdrop(double lrij, double lcij, double gamma, int gamma_type, int* rdrop, int* cdrop);
+void sort_scores(double* d, int* perm, int n);
+
+struct Dev {
+  std::vector<void*> ptrs;
+  ~Dev();
+  void* raw(size_t bytes);
+  template <class T> T* alloc(size_t n) { return (T*)raw(n * sizeof(T)); }
+};
+struct Grow { void* p = nullptr; size_t cap = 0; void* need(Dev& d, size_t bytes); };
+
+struct DMat { uint8_t* p = nullptr; int nr = 0, nc = 0; size_t rb = 0; Bits b() const { return Bits{ p, nr, nc, rb }; } };
+struct IList { std::vector<int> h; int* d = nullptr; int n() const { return (int)h.size(); } };
+// one patient group of an At_T_YnWt_S_Zn term: SNP-major bits, active-row mask and list, covariates
+struct Side { const DMat* X; const uint32_t* mask; const IList* rows; const DMat* Cov; };
+
+struct Run {
+  Cfg c;
+  hipStream_t st = nullptr;
+  Dev dev;
+  int NA = 0, NZ = 0, M = 0, T = 0, pcols = 0, has_Z = 0, Arb = 0, Zrb = 0;
+  DMat An, At, Zn, Zt, Tt, St;
+  std::vector<uint8_t> hAr_b, hAr_j, hAr_rtn, hAr_rmv, hZr_b, hZr_j, hAc_b, hAc_j, hAc_rtn, hAc_rmv, hTc_j;
+  uint32_t *Ar_b = nullptr, *Ar_j = nullptr, *Ar_rtn = nullptr, *Ar_rmv = nullptr, *Zr_b = nullptr, *Zr_j = nullptr,
+           *Ac_b = nullptr, *Ac_j = nullptr, *Ac_rtn = nullptr, *Ac_rmv = nullptr, *Tc_j = nullptr;
+  IList ar, kl, tl, zr, rtn, rmv, crt, crm;
+  int* d_zero = nullptr;
+  std::vector<int> Ar_bidx, Ac_bidx;
+  std::vector<double> hA_p, hajdk;
+  double* ajdk = nullptr;
+  int trm_AnZt = 0, trm_knzt = 0, trm_hnvt = 0;
+  // control block
+  uint8_t* hctl = nullptr; uint8_t* dctl = nullptr; size_t ctl_bytes = 0;
+  size_t ctl_mask_off[11] = {}, ctl_list_off[8] = {};
+  // device arrays; (row, covariate) arrays are [t*rows + row]
+  double *mlf = nullptr;
+  double *lf_An_ajdk = nullptr, *lf_Zn_ajdk = nullptr;              // [jk*rows + r]
+  double *jn_ajdk = nullptr, *vn_ajdk = nullptr, *kn_ajdk = nullptr, *hn_ajdk = nullptr, *an_ajdk_ = nullptr, *zn_ajdk = nullptr;
+  double *lf_AtTn = nullptr, *lf_ZtSn = nullptr;
+  double *lf_AnZt = nullptr, *lf_AnAt = nullptr;                      // [z*NA + a]
+  double *lf_AtTAnZtSZn = nullptr, *lf_AtTAnAtTAn = nullptr;
+  double *lf_AnZtSZnAt = nullptr, *lf_AnAtTAnAt = nullptr;
+  double *At_An_a1d1 = nullptr, *et_An = nullptr, *An_a2d2_ZtSn = nullptr, *An_a2d2_AtTn = nullptr;
+  double *sc = nullptr, *T_AnAt_AnAt = nullptr;
+  double *QR_ZSZ = nullptr, *QR_TAT = nullptr, *QC_ZSZ = nullptr, *QC_TAT = nullptr;
+  double *mt[14] = {};
+  int* cnt_buf = nullptr;
+  double *xwork = nullptr, *scal = nullptr, *dsv = nullptr, *hsv = nullptr;
+  Grow g_cz, g_planes, g_work, g_L, g_knzt, g_hnvt, g_ankt, g_knkt, g_ztsvn, g_attjn, g_ktrhn;
+  // host sort state and outputs
+  std::vector<double> QC_sv, QR_sv; std::vector<int> QC_perm, QR_perm;
+  std::vector<double> trace; std::vector<int> xdrop_a, xdrop_b;
+  int n_iter = 0; bool inited = false;
+  long long ops = 0;
+  double load_s = 0, init_s = 0, loop_s = 0;
+  std::string err;
+
+  ~Run();
+  void load(const char* in_text);
+  void push_ctl();
+  void st_ajdk(); void st_ZtSn(); void st_YnWt(); void st_At_T(); void st_QR();
+  void st_single(); void st_double(); void st_flatten();
+  void ww_excerpt(int trm, const Side& s1, const Side& s2, const IList& kls, const double* src, long long ld2,
+                  int full, double* out);
+  void init_lf();
+  void sumscores();
+  void lrup();
+  bool active() const;
+  void iterate(int* done);
+  void run(int max_iter);
+  void write_outputs(const std::string& dir) const;
+  std::string default_out_dir() const;
+  long debug_get(const std::string& name, std::vector<double>& out);
+};
+
+}  // namespace lkc

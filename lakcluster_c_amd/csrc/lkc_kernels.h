@@ -1,0 +1,91 @@
+// lkc_kernels.h -- launchers for the HIP kernels of the bcc elimination path (gfx950).
+//
+// Bit matrices are row-major packed bit rows, most significant bit first in
+// each byte, bit 1 <=> +1; every row is bsize(ncols) bytes (a multiple of
+// the 240-byte / 1920-bit weighting block).  Double arrays indexed by a
+// (row, covariate) pair are stored covariate-major: X[t*ld + row].
+// All launchers are asynchronous on `st`.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+namespace lkc {
+
+constexpr int kPopBits = 1920;   // weighting block (lakcluster_header.h:268)
+constexpr int kPopBytes = 240;
+constexpr int kPopWords = 60;    // 32-bit words per block
+constexpr int kAjdk = 15;
+constexpr int kMaxT = 8;         // covariate columns supported by the fused kernels
+
+size_t bsize(int n);             // bytes of one packed row of n bits
+int psize(int n);                // number of 1920-bit blocks
+
+struct Bits { const uint8_t* p; int nr; int nc; size_t rb; };
+
+// Σ_b w_jk[b]·|mask ∩ block b| for the 15 tables (out[0..14], only when ajdk
+// is non-null) and the plain count Σ_b 1.0·|mask ∩ block b| (out[15]); ordered over b.
+void mask_lf(const uint32_t* mask, int pcols, const double* ajdk, double* out16, hipStream_t st);
+
+// An_ajdk_v: out[jk*ld + rows[i]] = -base[jk] + 2 Σ_b ajdk[jk][b]·pop(X_row ∩ mask ∩ b)
+void an_ajdk(const Bits& X, const int* rows, int nrows, const uint32_t* mask, int pcols,
+             const double* ajdk, const double* base16, double* out, int ld, hipStream_t st);
+
+// AnZt_vv.  Output element for (alist[i], zlist[j]) goes to
+//   out[zidx*ldo + aidx]  with aidx = a_orig ? alist[i] : i, zidx = z_orig ? zlist[j] : j.
+// centred: D-weighted xor counts + ajdk correction with xa[alist[i]], xz[zlist[j]];
+// otherwise the plain +-1 product.  op_sub: out -= value instead of out = value.
+void gram(const Bits& A, const int* alist, int na, const Bits& Z, const int* zlist, int nz,
+          const uint32_t* mask, int pcols, const double* ajdk, const double* base16,
+          const double* xa, const double* xz, int centred,
+          double* out, long long ldo, int a_orig, int z_orig, int op_sub, hipStream_t st);
+
+// xcalc.  Values v(i,j) = src[ridx(i)*sn + cidx(j)*sw] for i<nn, j<nw, with
+// ridx(i) = rlist ? rlist[i] : i (same for cidx).  scal receives
+// [min_data, max_data, min, max, mlt (guarded), mlt (raw)].  cz[j*ldc + nout[i]]
+// receives the B-bit integers.  work: 768 doubles.
+void xcalc(const double* src, long long sn, long long sw, const int* rlist, const int* clist,
+           int nn, int nw, const int* nout, int B, long long D_MLT,
+           uint32_t* cz, long long ldc, double* scal, double* work, hipStream_t st);
+// bit planes of cz restricted to nmask: planes[(j*B + nb)*rb + ...] over n (n_total bits per row)
+void cz_planes(const uint32_t* cz, long long ldc, int nw, int n_total, const uint32_t* nmask,
+               int B, uint8_t* planes, size_t rb, hipStream_t st);
+
+// At_T_YnWt_S_Zn_ww.
+//   P rows klist (n-dimension bits), mask pm (n), covariate rows Tc[tlist] over n,
+//   planes (nw x B) over n with row0 (active n mask), Sc covariates over w, Q rows
+//   qlist over w.  Result: out[tlist[t]*ldo + klist[k]].
+struct WwArgs {
+  Bits P; const int* klist; const int* qlist; int nk; const uint32_t* pm;
+  Bits Tc; const int* tlist; const int* slist; int nt;
+  const uint8_t* planes; size_t prb; const uint32_t* row0; int nw; const int* wlist; int B;
+  const uint32_t* cz; long long ldc;      // cz[j*ldc + n]
+  Bits Sc; Bits Q; const double* ajdk; int pcols; const double* scal;
+  double* out; long long ldo;
+};
+// bit-plane VALU formulation (popcount of plane bit-rows)
+size_t ww_work_bytes(int nk, int nt, int nw);
+void ww_valu(const WwArgs& a, double* work, size_t work_bytes, hipStream_t st);
+
+// An_ZtSn_ww: out[slist[s]*ldo + alist[i]] = (base + Σ 2^nb pm0(A_a, plane(s,nb)))/mlt
+void an_ztsn(const Bits& A, const int* alist, int na, const uint32_t* amask,
+             const uint8_t* planes, size_t prb, const uint32_t* row0, int ns, const int* slist, int B,
+             const double* scal, double* out, long long ldo, hipStream_t st);
+
+// AnZt_S_WnYt_vv with X==Y: out[tlist[t]*ldo + a] = Σ_z (X[z*ldx+a]*vS(z))*X[z*ldx+a]
+void qr(const double* X, long long ldx, const int* alist, int na, const int* zlist, int nz,
+        const Bits& Sc, const int* tlist, int nt, double* out, long long ldo, hipStream_t st);
+
+// AtTYn_vv: out[(y*nt + t)*nk + k] (compact indices); lf_AtTn/lf_YtTn read at [t*ldT + row]
+void attyn(const Bits& At, const int* alist, int nk, const uint32_t* amask, int acpop,
+           const Bits& Tc, const int* tlist, int nt, const uint32_t* tmask, int tcpop,
+           const Bits& Yt, const int* ylist, int ny, const double* lf_AtTn, const double* lf_YtTn, long long ldT,
+           const double* ajdk, int pcols, double* out, hipStream_t st);
+// AtTYn____WtSZn_vv: out[tlist[t]*ldo + klist[k]] = Σ_y (L1*D_y)*L2 (L1/L2 in attyn layout)
+void attyn_wtszn(const double* L1, const double* L2, const int* klist, int nk, const int* ylist, int ny,
+                 const int* tlist, int nt, const double* ajdk, int pcols, double* out, long long ldo, hipStream_t st);
+
+// out[i] = pop(X[rows[i]] & mask) over nwords 32-bit words (rows==NULL: row i)
+void popc_rows(const Bits& X, const int* rows, int nrows, const uint32_t* mask, int nwords, int* out, hipStream_t st);
+
+}  // namespace lkc

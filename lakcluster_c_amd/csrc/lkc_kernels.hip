@@ -1,0 +1,559 @@
+// lkc_kernels.hip -- HIP kernels (gfx950) for the lakcluster bcc elimination path.
+//
+// Integer work (popcounts, +-1 products, quantised-plane sums) may run in any
+// order; every double reduction keeps the order of the reference restatement
+// cited at its kernel, and the file is compiled with -ffp-contract=off so no
+// multiply/add pair is fused.
+#include "lkc_kernels.h"
+#include <stdio.h>
+
+namespace lkc {
+
+size_t bsize(int n)
+{
+  long long e = n + ((16 - (n % 16)) % 16);
+  long long r = ((e + kPopBits - 1) / kPopBits) * kPopBits;
+  return (size_t)(r / 8);
+}
+int psize(int n) { return (int)(bsize(n) / kPopBytes); }
+
+static inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+
+// signed popcount: HIP's __popc returns unsigned int, so a difference of two of them
+// would wrap before widening
+__device__ __forceinline__ int popc(uint32_t x) { return (int)__builtin_popcount(x); }
+__device__ __forceinline__ int bit_at(const uint8_t* row, int i) { return (row[i >> 3] >> (7 - (i & 7))) & 1; }
+__device__ __forceinline__ int pm_at(const uint8_t* row, int i) { return bit_at(row, i) ? 1 : -1; }
+
+__device__ __forceinline__ int wave_sum_i(int v)
+{
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ long long wave_sum_ll(long long v)
+{
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// ------------------------------------------------------------------
+// mask_lf: block counts of a mask and the ordered weighted sums
+// (dtmp_base An_ajdk_v.c:312, dtmp_a0d1/a2d1 AnZt_vv.c:62-63, lf_A_a*d* bcc_flattenloop.c:1520-1534)
+// ------------------------------------------------------------------
+__global__ void k_mask_lf(const uint32_t* mask, int pcols, const double* ajdk, double* out16)
+{
+  extern __shared__ int cnt[];
+  for (int b = threadIdx.x; b < pcols; b += blockDim.x) {
+    int c = 0;
+    for (int w = 0; w < kPopWords; w++) c += popc(mask[(size_t)b * kPopWords + w]);
+    cnt[b] = c;
+  }
+  __syncthreads();
+  int jk = threadIdx.x;
+  if (jk < kAjdk) {
+    if (!ajdk) return;
+    double s = 0;
+    for (int b = 0; b < pcols; b++) s += (double)ajdk[(size_t)jk * pcols + b] * (double)cnt[b];
+    out16[jk] = s;
+  } else if (jk == kAjdk) {
+    double s = 0;
+    for (int b = 0; b < pcols; b++) s += (double)1 * (double)cnt[b];
+    out16[kAjdk] = s;
+  }
+}
+void mask_lf(const uint32_t* mask, int pcols, const double* ajdk, double* out16, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_mask_lf, dim3(1), dim3(64), sizeof(int) * (pcols + 1), st, mask, pcols, ajdk, out16);
+}
+
+// ------------------------------------------------------------------
+// An_ajdk_v (An_ajdk_v.c:272-363): one wave per row
+// ------------------------------------------------------------------
+__global__ void k_an_ajdk(const uint8_t* X, size_t rb, const int* rows, int nrows, const uint32_t* mask, int pcols,
+                          const double* ajdk, const double* base, double* out, int ld)
+{
+  extern __shared__ int cnt[];
+  int r = blockIdx.x;
+  if (r >= nrows) return;
+  int row = rows[r];
+  const uint32_t* xr = (const uint32_t*)(X + (size_t)row * rb);
+  for (int b = 0; b < pcols; b++) {
+    int c = 0;
+    if (threadIdx.x < kPopWords) c = popc(xr[(size_t)b * kPopWords + threadIdx.x] & mask[(size_t)b * kPopWords + threadIdx.x]);
+    c = wave_sum_i(c);
+    if (threadIdx.x == 0) cnt[b] = c;
+  }
+  __syncthreads();
+  int jk = threadIdx.x;
+  if (jk < kAjdk) {
+    const double* w = ajdk + (size_t)jk * pcols;
+    double d = 0;
+    for (int b = 0; b < pcols; b++) d += (double)w[b] * (double)(unsigned)cnt[b];
+    double v = -base[jk];
+    v += (2 * d);
+    out[(size_t)jk * ld + row] = v;
+  }
+}
+void an_ajdk(const Bits& X, const int* rows, int nrows, const uint32_t* mask, int pcols,
+             const double* ajdk, const double* base16, double* out, int ld, hipStream_t st)
+{
+  if (nrows <= 0) return;
+  hipLaunchKernelGGL(k_an_ajdk, dim3(nrows), dim3(64), sizeof(int) * (pcols + 1), st,
+                     X.p, X.rb, rows, nrows, mask, pcols, ajdk, base16, out, ld);
+}
+
+// ------------------------------------------------------------------
+// AnZt_vv (AnZt_vv.c:62-95): 64x64 pair tile per block, 4x4 pairs per thread,
+// one 1920-bit block of both operands staged (masked) in LDS at a time.
+// ------------------------------------------------------------------
+#define GT 64
+#define GLD 61   // padded LDS row stride in words
+__global__ __launch_bounds__(256) void k_gram(const uint8_t* A, size_t rbA, const int* alist, int na,
+                                              const uint8_t* Z, size_t rbZ, const int* zlist, int nz,
+                                              const uint32_t* mask, int pcols, const double* ajdk, const double* base,
+                                              const double* xa, const double* xz, int centred,
+                                              double* out, long long ldo, int a_orig, int z_orig, int op_sub)
+{
+  __shared__ uint32_t As[GT * GLD];
+  __shared__ uint32_t Zs[GT * GLD];
+  int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  int a0 = blockIdx.x * GT, z0 = blockIdx.y * GT;
+  const double* D = centred ? ajdk + (size_t)5 * pcols : nullptr;
+  double dsum[4][4];
+  long long isum[4][4];
+  for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) { dsum[i][j] = 0; isum[i][j] = 0; }
+  for (int b = 0; b < pcols; b++) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < GT * kPopWords; e += 256) {
+      int r = e / kPopWords, w = e % kPopWords;
+      uint32_t m = mask[(size_t)b * kPopWords + w];
+      uint32_t va = 0, vz = 0;
+      if (a0 + r < na) va = ((const uint32_t*)(A + (size_t)alist[a0 + r] * rbA))[(size_t)b * kPopWords + w] & m;
+      if (z0 + r < nz) vz = ((const uint32_t*)(Z + (size_t)zlist[z0 + r] * rbZ))[(size_t)b * kPopWords + w] & m;
+      As[r * GLD + w] = va; Zs[r * GLD + w] = vz;
+    }
+    __syncthreads();
+    int acc[4][4];
+    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) acc[i][j] = 0;
+    for (int w = 0; w < kPopWords; w++) {
+      uint32_t av[4], zv[4];
+      for (int i = 0; i < 4; i++) av[i] = As[(tx * 4 + i) * GLD + w];
+      for (int j = 0; j < 4; j++) zv[j] = Zs[(ty * 4 + j) * GLD + w];
+      for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) acc[i][j] += popc(av[i] ^ zv[j]);
+    }
+    if (centred) {
+      double wb = D[b];
+      for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) dsum[i][j] += (double)wb * (double)(unsigned)acc[i][j];
+    } else {
+      for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) isum[i][j] += acc[i][j];
+    }
+  }
+  double a0d1 = centred ? base[5] : base[kAjdk];
+  double a2d1 = centred ? base[7] : 0.0;
+  for (int j = 0; j < 4; j++) {
+    int zc = z0 + ty * 4 + j; if (zc >= nz) continue;
+    for (int i = 0; i < 4; i++) {
+      int ac = a0 + tx * 4 + i; if (ac >= na) continue;
+      double dtmp = centred ? dsum[i][j] : (double)isum[i][j];
+      double o = a0d1;
+      o -= (2 * dtmp);
+      if (centred) o -= xa[alist[ac]] + xz[zlist[zc]] - a2d1;
+      long long ai = a_orig ? alist[ac] : ac, zi = z_orig ? zlist[zc] : zc;
+      double* dst = out + zi * ldo + ai;
+      if (op_sub) *dst -= o; else *dst = o;
+    }
+  }
+}
+void gram(const Bits& A, const int* alist, int na, const Bits& Z, const int* zlist, int nz,
+          const uint32_t* mask, int pcols, const double* ajdk, const double* base16,
+          const double* xa, const double* xz, int centred,
+          double* out, long long ldo, int a_orig, int z_orig, int op_sub, hipStream_t st)
+{
+  if (na <= 0 || nz <= 0) return;
+  hipLaunchKernelGGL(k_gram, dim3(cdiv(na, GT), cdiv(nz, GT)), dim3(256), 0, st,
+                     A.p, A.rb, alist, na, Z.p, Z.rb, zlist, nz, mask, pcols, ajdk, base16, xa, xz, centred,
+                     out, ldo, a_orig, z_orig, op_sub);
+}
+
+// ------------------------------------------------------------------
+// xcalc (xcalc.c:5-246): min/max, then (u64)(mlt*(x-min)) clamped to [1, D_MLT-1]
+// ------------------------------------------------------------------
+__device__ __forceinline__ double xval(const double* src, long long sn, long long sw, const int* rl, const int* cl, int i, int j)
+{
+  long long r = rl ? rl[i] : i, c = cl ? cl[j] : j;
+  return src[r * sn + c * sw];
+}
+__global__ void k_minmax(const double* src, long long sn, long long sw, const int* rl, const int* cl, int nn, int nw, double* part)
+{
+  __shared__ double smin[256], smax[256]; __shared__ int sset[256];
+  long long tot = (long long)nn * nw;
+  double mn = 0, mx = 0; int set = 0;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
+    int j = (int)(e / nn), i = (int)(e % nn);
+    double v = xval(src, sn, sw, rl, cl, i, j);
+    if (!set) { mn = v; mx = v; set = 1; } else { mn = v < mn ? v : mn; mx = v > mx ? v : mx; }
+  }
+  smin[threadIdx.x] = mn; smax[threadIdx.x] = mx; sset[threadIdx.x] = set;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      int o = threadIdx.x + s;
+      if (sset[o]) {
+        if (!sset[threadIdx.x]) { smin[threadIdx.x] = smin[o]; smax[threadIdx.x] = smax[o]; sset[threadIdx.x] = 1; }
+        else { smin[threadIdx.x] = smin[o] < smin[threadIdx.x] ? smin[o] : smin[threadIdx.x];
+               smax[threadIdx.x] = smax[o] > smax[threadIdx.x] ? smax[o] : smax[threadIdx.x]; }
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { part[3 * blockIdx.x] = smin[0]; part[3 * blockIdx.x + 1] = smax[0]; part[3 * blockIdx.x + 2] = sset[0]; }
+}
+__global__ void k_minmax_final(const double* part, int np, long long D_MLT, double* scal)
+{
+  if (threadIdx.x != 0) return;
+  double mn = 0, mx = 0; int set = 0;
+  for (int p = 0; p < np; p++) {
+    if (part[3 * p + 2] == 0) continue;
+    if (!set) { mn = part[3 * p]; mx = part[3 * p + 1]; set = 1; }
+    else { mn = part[3 * p] < mn ? part[3 * p] : mn; mx = part[3 * p + 1] > mx ? part[3 * p + 1] : mx; }
+  }
+  if (mn >= mx) mx = mn + 1;
+  double lo = mn - 0.0625 * (mx - mn);
+  double hi = mx + 0.0625 * (mx - mn);
+  double mlt = (double)(D_MLT) / (hi - lo);
+  scal[0] = mn; scal[1] = mx; scal[2] = lo; scal[3] = hi;
+  scal[4] = (!isfinite(mlt) || mlt <= 0) ? 1.0 : mlt;
+  scal[5] = mlt;   // the value used for quantisation (before the guard)
+}
+__global__ void k_quantize(const double* src, long long sn, long long sw, const int* rl, const int* cl, int nn, int nw,
+                           const int* nout, long long D_MLT, const double* scal, uint32_t* cz, long long ldc)
+{
+  long long tot = (long long)nn * nw;
+  double lo = scal[2], mlt = scal[5];
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
+    int j = (int)(e / nn), i = (int)(e % nn);
+    double v = xval(src, sn, sw, rl, cl, i, j);
+    unsigned long long c = (unsigned long long)(mlt * (double)(v - lo));
+    if (c < 1ull) c = 1ull;
+    if (c > (unsigned long long)(D_MLT - 1)) c = (unsigned long long)(D_MLT - 1);
+    cz[(long long)j * ldc + nout[i]] = (uint32_t)c;
+  }
+}
+void xcalc(const double* src, long long sn, long long sw, const int* rlist, const int* clist,
+           int nn, int nw, const int* nout, int B, long long D_MLT,
+           uint32_t* cz, long long ldc, double* scal, double* work, hipStream_t st)
+{
+  int np = 256;
+  long long tot = (long long)nn * nw;
+  if (tot < 256LL * 64) np = (int)cdiv(tot > 0 ? tot : 1, 256);
+  hipLaunchKernelGGL(k_minmax, dim3(np), dim3(256), 0, st, src, sn, sw, rlist, clist, nn, nw, work);
+  hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(64), 0, st, work, np, D_MLT, scal);
+  if (tot > 0)
+    hipLaunchKernelGGL(k_quantize, dim3(cdiv(tot, 256) < 65536 ? cdiv(tot, 256) : 65536), dim3(256), 0, st,
+                       src, sn, sw, rlist, clist, nn, nw, nout, D_MLT, scal, cz, ldc);
+}
+
+// planes[(j*B+nb)*rb] bit n = bit nb of cz[j*ldc+n]; one thread per 32 n
+__global__ void k_cz_planes(const uint32_t* cz, long long ldc, int nw, int n_total, const uint32_t* nmask, int B, uint8_t* planes, size_t rb)
+{
+  int nwords = (int)(rb / 4);
+  long long tot = (long long)nw * nwords;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
+    int j = (int)(e / nwords), wd = (int)(e % nwords);
+    uint32_t pw[32];
+    for (int nb = 0; nb < 32; nb++) pw[nb] = 0;
+    for (int q = 0; q < 32; q++) {
+      int n = wd * 32 + q;
+      if (n >= n_total) break;
+      if (!((nmask[wd] >> ((q >> 3) * 8 + (7 - (q & 7)))) & 1u)) continue;
+      uint32_t v = cz[(long long)j * ldc + n];
+      uint32_t bitpos = (uint32_t)((q >> 3) * 8 + (7 - (q & 7)));
+      for (int nb = 0; nb < B; nb++) pw[nb] |= ((v >> nb) & 1u) << bitpos;
+    }
+    uint32_t m = nmask[wd];
+    for (int nb = 0; nb < B; nb++) ((uint32_t*)(planes + ((size_t)j * B + nb) * rb))[wd] = pw[nb] & m;
+  }
+}
+void cz_planes(const uint32_t* cz, long long ldc, int nw, int n_total, const uint32_t* nmask, int B, uint8_t* planes, size_t rb, hipStream_t st)
+{
+  long long tot = (long long)nw * (long long)(rb / 4);
+  if (tot <= 0) return;
+  hipLaunchKernelGGL(k_cz_planes, dim3(cdiv(tot, 256) < 65536 ? cdiv(tot, 256) : 65536), dim3(256), 0, st,
+                     cz, ldc, nw, n_total, nmask, B, planes, rb);
+}
+
+// ------------------------------------------------------------------
+// At_T_YnWt_S_Zn_ww (At_T_YnWt_S_Zn_ww.c:44-110), bit-plane VALU form.
+//   H[t][w] and base'[t] (row0 terms), bk[t][k] (P_k terms), then per (k,w) the
+//   integer G and its epilogue term, then the ordered sum over w per (k,t).
+// ------------------------------------------------------------------
+__device__ __forceinline__ int pmpm0_word(uint32_t x0, uint32_t x1, uint32_t x2, uint32_t m)
+{
+  uint32_t e = x0 ^ x1, b = x2 & m;
+  return popc(~e & b) - popc(e & b);
+}
+// one wave per (t, w) with w == nw meaning the base' term
+__global__ void k_ww_H(WwArgs a, double* H, double* bp)
+{
+  int t = blockIdx.y, j = blockIdx.x;
+  const uint32_t* Tr = (const uint32_t*)(a.Tc.p + (size_t)a.tlist[t] * a.Tc.rb);
+  int nwd = (int)(a.prb / 4);
+  double mn = a.scal[2], mlt = a.scal[4];
+  if (j == a.nw) {
+    long long c = 0;
+    for (int w = threadIdx.x; w < nwd; w += 64) c += pmpm0_word(a.row0[w], Tr[w], a.row0[w], a.pm[w]);
+    c = wave_sum_ll(c);
+    if (threadIdx.x == 0) { double d = c * 1.0; bp[t] = d * mn * mlt; }
+    return;
+  }
+  double h = 0; long long n2 = 1;
+  for (int nb = 0; nb < a.B; nb++) {
+    const uint32_t* pl = (const uint32_t*)(a.planes + ((size_t)j * a.B + nb) * a.prb);
+    long long c = 0;
+    for (int w = threadIdx.x; w < nwd; w += 64) c += pmpm0_word(a.row0[w], Tr[w], pl[w], a.pm[w]);
+    c = wave_sum_ll(c);
+    double d = c * 1.0;
+    h += n2 * d; n2 *= 2;
+  }
+  if (threadIdx.x == 0) H[(size_t)t * a.nw + j] = h;
+}
+// one wave per (t, k)
+__global__ void k_ww_bk(WwArgs a, double* bk)
+{
+  int t = blockIdx.y, k = blockIdx.x;
+  const uint32_t* Tr = (const uint32_t*)(a.Tc.p + (size_t)a.tlist[t] * a.Tc.rb);
+  const uint32_t* Pr = (const uint32_t*)(a.P.p + (size_t)a.klist[k] * a.P.rb);
+  int nwd = (int)(a.prb / 4);
+  long long c = 0;
+  for (int w = threadIdx.x; w < nwd; w += 64) c += pmpm0_word(Pr[w], Tr[w], a.row0[w], a.pm[w]);
+  c = wave_sum_ll(c);
+  if (threadIdx.x == 0) bk[(size_t)t * a.nk + k] = c * a.scal[2] * a.scal[4];
+}
+// one thread per (k, w) for a chunk of w; term[(w-w0)*ntk + t*nk + k]
+__global__ __launch_bounds__(256) void k_ww_term_valu(WwArgs a, int t, int w0, int wn, const double* H, const double* bp,
+                                                      const double* bk, double* term)
+{
+  int k = blockIdx.x * 256 + threadIdx.x;
+  int jw = blockIdx.y;
+  if (k >= a.nk || jw >= wn) return;
+  int j = w0 + jw;
+  const uint32_t* Tr = (const uint32_t*)(a.Tc.p + (size_t)a.tlist[t] * a.Tc.rb);
+  const uint32_t* Pr = (const uint32_t*)(a.P.p + (size_t)a.klist[k] * a.P.rb);
+  int nwd = (int)(a.prb / 4);
+  long long G = 0;
+  for (int w = 0; w < nwd; w++) {
+    uint32_t m = a.pm[w];
+    uint32_t e = Pr[w] ^ Tr[w];
+    uint32_t up = ~e & m, un = e & m;
+    for (int nb = 0; nb < a.B; nb++) {
+      uint32_t pl = ((const uint32_t*)(a.planes + ((size_t)j * a.B + nb) * a.prb))[w];
+      G += (long long)(popc(up & pl) - popc(un & pl)) << nb;
+    }
+  }
+  double mlt = a.scal[4];
+  const double* alpha = a.ajdk + (size_t)1 * a.pcols;
+  int kk = a.klist[k], qq = a.qlist[k], ww = a.wlist[j];
+  double alpha_k = alpha[kk / kPopBits], alpha_q = alpha[qq / kPopBits];
+  int vS = pm_at(a.Sc.p + (size_t)a.slist[t] * a.Sc.rb, ww);
+  int vZ = pm_at(a.Q.p + (size_t)qq * a.Q.rb, ww);
+  double bkk = bk[(size_t)t * a.nk + k];
+  double at_base = bp[t] * alpha_k;
+  double at_tmp = H[(size_t)t * a.nw + j] * alpha_k;
+  double g = (double)G;
+  double tmp = (double)(bkk + g) / mlt - (double)(at_base + at_tmp) / mlt;
+  term[(size_t)jw * a.nk + k] = tmp * vS * (vZ - alpha_q);
+}
+__global__ void k_fill0(double* p, long long n)
+{
+  long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+// ordered sum over the chunk's w for each k: acc[t*nk+k] += term[...]
+__global__ void k_ww_accum(const double* term, int nk, int wn, double* acc)
+{
+  int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= nk) return;
+  double o = acc[k];
+  for (int jw = 0; jw < wn; jw++) o += term[(size_t)jw * nk + k];
+  acc[k] = o;
+}
+__global__ void k_ww_store(const double* acc, const int* klist, const int* tlist, int nk, int nt, double* out, long long ldo)
+{
+  int k = blockIdx.x * 256 + threadIdx.x, t = blockIdx.y;
+  if (k >= nk || t >= nt) return;
+  out[(long long)tlist[t] * ldo + klist[k]] = acc[(size_t)t * nk + k];
+}
+size_t ww_work_bytes(int nk, int nt, int nw)
+{
+  return sizeof(double) * ((size_t)nt * nw + nt + 2 * (size_t)nt * nk + (size_t)64 * (nk > 0 ? nk : 1));
+}
+void ww_valu(const WwArgs& a, double* work, size_t work_bytes, hipStream_t st)
+{
+  if (a.nk <= 0 || a.nt <= 0) return;
+  // work layout: H[nt*nw] bp[nt] bk[nt*nk] acc[nt*nk] term[chunk*nk]
+  double* H = work;
+  double* bp = H + (size_t)a.nt * a.nw;
+  double* bk = bp + a.nt;
+  double* acc = bk + (size_t)a.nt * a.nk;
+  double* term = acc + (size_t)a.nt * a.nk;
+  size_t used = (size_t)(term - work) * sizeof(double);
+  long long chunk = (long long)((work_bytes - used) / sizeof(double)) / (a.nk > 0 ? a.nk : 1);
+  if (chunk > 65535) chunk = 65535;
+  if (chunk < 1) { fprintf(stderr, "lkc: ww work buffer too small\n"); return; }
+  hipLaunchKernelGGL(k_ww_H, dim3(a.nw + 1, a.nt), dim3(64), 0, st, a, H, bp);
+  hipLaunchKernelGGL(k_ww_bk, dim3(a.nk, a.nt), dim3(64), 0, st, a, bk);
+  hipLaunchKernelGGL(k_fill0, dim3(cdiv((long long)a.nt * a.nk, 256)), dim3(256), 0, st, acc, (long long)a.nt * a.nk);
+  for (int t = 0; t < a.nt; t++) {
+    for (int w0 = 0; w0 < a.nw; w0 += (int)chunk) {
+      int wn = (int)((a.nw - w0) < chunk ? (a.nw - w0) : chunk);
+      hipLaunchKernelGGL(k_ww_term_valu, dim3(cdiv(a.nk, 256), wn), dim3(256), 0, st, a, t, w0, wn, H, bp, bk, term);
+      hipLaunchKernelGGL(k_ww_accum, dim3(cdiv(a.nk, 256)), dim3(256), 0, st, term, a.nk, wn, acc + (size_t)t * a.nk);
+    }
+  }
+  hipLaunchKernelGGL(k_ww_store, dim3(cdiv(a.nk, 256), a.nt), dim3(256), 0, st, acc, a.klist, a.tlist, a.nk, a.nt, a.out, a.ldo);
+}
+
+// ------------------------------------------------------------------
+// An_ZtSn_ww (An_ZtSn_ww.c:44-70): one wave per row
+// ------------------------------------------------------------------
+__global__ void k_an_ztsn(const uint8_t* A, size_t rbA, const int* alist, int na, const uint32_t* amask,
+                          const uint8_t* planes, size_t prb, const uint32_t* row0, int ns, const int* slist, int B,
+                          const double* scal, double* out, long long ldo)
+{
+  int i = blockIdx.x; if (i >= na) return;
+  const uint32_t* Ar = (const uint32_t*)(A + (size_t)alist[i] * rbA);
+  int nwd = (int)(prb / 4);
+  long long c = 0;
+  for (int w = threadIdx.x; w < nwd; w += 64) { uint32_t b = row0[w] & amask[w]; c += popc(Ar[w] & b) - popc(~Ar[w] & b); }
+  c = wave_sum_ll(c);
+  double mn = scal[2], mlt = scal[4];
+  double ll = (double)c;
+  double base = ll * mn * mlt;
+  for (int s = 0; s < ns; s++) {
+    double tmp = 0; long long n2 = 1;
+    for (int nb = 0; nb < B; nb++) {
+      const uint32_t* pl = (const uint32_t*)(planes + ((size_t)s * B + nb) * prb);
+      long long d = 0;
+      for (int w = threadIdx.x; w < nwd; w += 64) { uint32_t b = pl[w] & amask[w]; d += popc(Ar[w] & b) - popc(~Ar[w] & b); }
+      d = wave_sum_ll(d);
+      double l2 = (double)d;
+      tmp += n2 * l2; n2 *= 2;
+    }
+    if (threadIdx.x == 0) out[(long long)slist[s] * ldo + alist[i]] = (double)(base + tmp) / mlt;
+  }
+}
+void an_ztsn(const Bits& A, const int* alist, int na, const uint32_t* amask,
+             const uint8_t* planes, size_t prb, const uint32_t* row0, int ns, const int* slist, int B,
+             const double* scal, double* out, long long ldo, hipStream_t st)
+{
+  if (na <= 0) return;
+  hipLaunchKernelGGL(k_an_ztsn, dim3(na), dim3(64), 0, st, A.p, A.rb, alist, na, amask, planes, prb, row0, ns, slist, B, scal, out, ldo);
+}
+
+// ------------------------------------------------------------------
+// AnZt_S_WnYt_vv (AnZt_S_WnYt_vv.c:49-67) with X==Y; X[z*ldx + a] (a contiguous)
+// ------------------------------------------------------------------
+__global__ void k_qr(const double* X, long long ldx, const int* alist, int na, const int* zlist, int nz,
+                     const uint8_t* Sc, size_t rbS, const int* tlist, int nt, double* out, long long ldo)
+{
+  int i = blockIdx.x * 256 + threadIdx.x;
+  bool live = i < na;
+  int a = live ? alist[i] : 0;
+  double acc[kMaxT];
+  for (int t = 0; t < kMaxT; t++) acc[t] = 0;
+  for (int zz = 0; zz < nz; zz++) {
+    int z = zlist[zz];
+    double x = live ? X[(long long)z * ldx + a] : 0.0;
+    for (int t = 0; t < nt; t++) {
+      int vS = pm_at(Sc + (size_t)tlist[t] * rbS, z);
+      acc[t] += x * vS * x;
+    }
+  }
+  if (live) for (int t = 0; t < nt; t++) out[(long long)tlist[t] * ldo + a] = acc[t];
+}
+void qr(const double* X, long long ldx, const int* alist, int na, const int* zlist, int nz,
+        const Bits& Sc, const int* tlist, int nt, double* out, long long ldo, hipStream_t st)
+{
+  if (na <= 0) return;
+  hipLaunchKernelGGL(k_qr, dim3(cdiv(na, 256)), dim3(256), 0, st, X, ldx, alist, na, zlist, nz, Sc.p, Sc.rb, tlist, nt, out, ldo);
+}
+
+// ------------------------------------------------------------------
+// AtTYn_vv (AtTYn_vv.c:79-120)
+// ------------------------------------------------------------------
+__global__ void k_attyn(const uint8_t* At, size_t rbA, const int* alist, int nk, const uint32_t* amask, int acpop,
+                        const uint8_t* Tc, size_t rbT, const int* tlist, int nt, const uint32_t* tmask, int tcpop,
+                        const uint8_t* Yt, size_t rbY, const int* ylist, int ny,
+                        const double* lf_AtTn, const double* lf_YtTn, long long ldT, const double* ajdk, int pcols, double* out)
+{
+  int k = blockIdx.x * 64 + (threadIdx.x & 63);
+  int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  int t = blockIdx.z;
+  if (k >= nk || y >= ny) return;
+  const uint32_t* Ar = (const uint32_t*)(At + (size_t)alist[k] * rbA);
+  const uint32_t* Tr = (const uint32_t*)(Tc + (size_t)tlist[t] * rbT);
+  const uint32_t* Yr = (const uint32_t*)(Yt + (size_t)ylist[y] * rbY);
+  int nwd = (int)(rbA / 4);
+  long long cnt = 0, tc = 0;
+  for (int w = 0; w < nwd; w++) {
+    cnt += popc(~(Ar[w] ^ Tr[w] ^ Yr[w]) & amask[w]);
+    tc += popc(Tr[w] & tmask[w]);
+  }
+  long long hhay = -(long long)tcpop + 2 * tc;
+  long long hhAy = (long long)lf_AtTn[(long long)tlist[t] * ldT + alist[k]];
+  long long hhaY = (long long)lf_YtTn[(long long)tlist[t] * ldT + ylist[y]];
+  long long hhAY = +(long long)acpop;
+  hhAY -= (long long)(2 * cnt);
+  const double* al = ajdk + (size_t)pcols;
+  double aa = al[alist[k] / kPopBits], ay = al[ylist[y] / kPopBits];
+  out[((size_t)y * nt + t) * nk + k] = (hhAY - hhAy * ay - aa * hhaY + aa * hhay * ay);
+}
+void attyn(const Bits& At, const int* alist, int nk, const uint32_t* amask, int acpop,
+           const Bits& Tc, const int* tlist, int nt, const uint32_t* tmask, int tcpop,
+           const Bits& Yt, const int* ylist, int ny, const double* lf_AtTn, const double* lf_YtTn, long long ldT,
+           const double* ajdk, int pcols, double* out, hipStream_t st)
+{
+  if (nk <= 0 || ny <= 0 || nt <= 0) return;
+  hipLaunchKernelGGL(k_attyn, dim3(cdiv(nk, 64), cdiv(ny, 4), nt), dim3(256), 0, st,
+                     At.p, At.rb, alist, nk, amask, acpop, Tc.p, Tc.rb, tlist, nt, tmask, tcpop,
+                     Yt.p, Yt.rb, ylist, ny, lf_AtTn, lf_YtTn, ldT, ajdk, pcols, out);
+}
+__global__ void k_attyn_wtszn(const double* L1, const double* L2, const int* klist, int nk, const int* ylist, int ny,
+                              const int* tlist, int nt, const double* ajdk, int pcols, double* out, long long ldo)
+{
+  int k = blockIdx.x * 256 + threadIdx.x, t = blockIdx.y;
+  if (k >= nk) return;
+  const double* D = ajdk + (size_t)5 * pcols;
+  double o = 0;
+  for (int y = 0; y < ny; y++) {
+    size_t q = ((size_t)y * nt + t) * nk + k;
+    o += L1[q] * D[ylist[y] / kPopBits] * L2[q];
+  }
+  out[(long long)tlist[t] * ldo + klist[k]] = o;
+}
+void attyn_wtszn(const double* L1, const double* L2, const int* klist, int nk, const int* ylist, int ny,
+                 const int* tlist, int nt, const double* ajdk, int pcols, double* out, long long ldo, hipStream_t st)
+{
+  if (nk <= 0 || nt <= 0) return;
+  hipLaunchKernelGGL(k_attyn_wtszn, dim3(cdiv(nk, 256), nt), dim3(256), 0, st, L1, L2, klist, nk, ylist, ny, tlist, nt, ajdk, pcols, out, ldo);
+}
+
+// ------------------------------------------------------------------
+// row popcounts under a mask: one wave per row
+// ------------------------------------------------------------------
+__global__ void k_popc_rows(const uint8_t* X, size_t rb, const int* rows, int nrows, const uint32_t* mask, int nwords, int* out)
+{
+  int i = blockIdx.x; if (i >= nrows) return;
+  const uint32_t* xr = (const uint32_t*)(X + (size_t)(rows ? rows[i] : i) * rb);
+  int c = 0;
+  for (int w = threadIdx.x; w < nwords; w += 64) c += popc(xr[w] & mask[w]);
+  c = wave_sum_i(c);
+  if (threadIdx.x == 0) out[i] = c;
+}
+void popc_rows(const Bits& X, const int* rows, int nrows, const uint32_t* mask, int nwords, int* out, hipStream_t st)
+{
+  if (nrows <= 0) return;
+  hipLaunchKernelGGL(k_popc_rows, dim3(nrows), dim3(64), 0, st, X.p, X.rb, rows, nrows, mask, nwords, out);
+}
+
+}  // namespace lkc

@@ -1,0 +1,281 @@
+// lkc_stages.hip -- element-wise and ordered-sum kernels of the bcc stages that sit
+// between the bit-matrix contractions: single/double-study terms, flattenloop,
+// score normalisation, the QC stage-c update.
+//
+// Arrays indexed by (row, covariate) are covariate-major, X[t*ld + row].
+// Every expression keeps the operand order of the reference line it cites
+// (compiled with -ffp-contract=off), so results are bit-identical to it.
+#include "lkc_stages.h"
+
+namespace lkc {
+
+static inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+__device__ __forceinline__ int bit_of(const uint8_t* row, int i) { return (row[i >> 3] >> (7 - (i & 7))) & 1; }
+__device__ __forceinline__ int pm_of(const uint8_t* row, int i) { return bit_of(row, i) ? 1 : -1; }
+__device__ __forceinline__ int bit_w(const uint32_t* m, int i) { return bit_of((const uint8_t*)m, i); }
+
+// ------------------------------------------------------------------
+// ordered sums Σ_i s_i·v[list[i]] (get_singlestudy_vv, bcc_flattenloop.c:359-731):
+// one block per job; 256 values are staged in LDS and lane 0 adds them in order.
+// ------------------------------------------------------------------
+__global__ void k_ordered_sums(OrderedJobs J)
+{
+  __shared__ double buf[256];
+  int q = blockIdx.x;
+  if (q >= J.n) return;
+  const OrderedJob& jb = J.job[q];
+  double o = 0.0;
+  for (int i0 = 0; i0 < jb.n; i0 += 256) {
+    int i = i0 + threadIdx.x;
+    if (i < jb.n) {
+      int r = jb.list[i];
+      double v = jb.vals[r];
+      if (jb.sign) v = pm_of(jb.sign, r) * v;
+      buf[threadIdx.x] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int m = jb.n - i0 < 256 ? jb.n - i0 : 256;
+      for (int e = 0; e < m; e++) o += buf[e];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *jb.out = o;
+}
+void ordered_sums(const OrderedJobs& J, hipStream_t st)
+{
+  if (J.n <= 0) return;
+  hipLaunchKernelGGL(k_ordered_sums, dim3(J.n), dim3(256), 0, st, J);
+}
+
+// out[list[i]] = 0 + (double)(-(long long)base + 2*cnt[i])  (bcc_flattenloop.c:690-701)
+__global__ void k_et(const int* cnt, const int* list, int n, long long base, double* out, int assign_only)
+{
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  long long v = -base + 2 * (long long)cnt[i];
+  int r = list ? list[i] : i;
+  out[r] = assign_only ? (double)v : 0.0 + (double)v;
+}
+void et_from_counts(const int* cnt, const int* list, int n, long long base, double* out, int assign_only, hipStream_t st)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_et, dim3(cdiv(n, 256)), dim3(256), 0, st, cnt, list, n, base, out, assign_only);
+}
+
+// T_AnAt_AnAt (bcc_flattenloop.c:702-731): the A row's centred Gram with itself, squared, signed by T
+__global__ void k_t_anat(const int* ar, int nar, const int* tl, int nt, const uint8_t* Tt, size_t rbT,
+                         const double* x6, const double* mlf, double* out, long long ld)
+{
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nar) return;
+  int a = ar[i];
+  double x11 = x6[a];
+  double Ad = mlf[5];
+  Ad -= 2 * 0.0;
+  Ad -= x11 + x11 - mlf[7];
+  for (int t = 0; t < nt; t++) {
+    int tt = tl[t];
+    int vT = pm_of(Tt + (size_t)tt * rbT, a);
+    out[(long long)tt * ld + a] = vT * Ad * Ad;
+  }
+}
+void t_anat(const int* ar, int nar, const int* tl, int nt, const Bits& Tt, const double* x6, const double* mlf,
+            double* out, long long ld, hipStream_t st)
+{
+  if (nar <= 0) return;
+  hipLaunchKernelGGL(k_t_anat, dim3(cdiv(nar, 256)), dim3(256), 0, st, ar, nar, tl, nt, Tt.p, Tt.rb, x6, mlf, out, ld);
+}
+
+// L[t*nk + k] = w[kl[k]/1920] * src[tl[t]*ld + kl[k]]  (An_a2d2_ZtSn_excerpt, An_ZtSn_ww.c:267-290)
+__global__ void k_scale_rows(const double* src, long long ld, const int* kl, int nk, const int* tl, int nt,
+                             const double* w, double* L)
+{
+  long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (long long)nk * nt) return;
+  int t = (int)(e / nk), k = (int)(e % nk);
+  int kk = kl[k];
+  L[e] = w[kk / kPopBits] * src[(long long)tl[t] * ld + kk];
+}
+void scale_rows(const double* src, long long ld, const int* kl, int nk, const int* tl, int nt, const double* w,
+                double* L, hipStream_t st)
+{
+  long long n = (long long)nk * nt;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_scale_rows, dim3(cdiv(n, 256)), dim3(256), 0, st, src, ld, kl, nk, tl, nt, w, L);
+}
+
+// ------------------------------------------------------------------
+// get_flattenloop, QR half (bcc_flattenloop.c:1599-1716)
+// ------------------------------------------------------------------
+__global__ void k_flat_qr(FlatQR f)
+{
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= f.nar) return;
+  int a = f.ar[i];
+  const double* x11p = f.ajdk_rows + (size_t)11 * f.NA;
+  const double* x13p = f.ajdk_rows + (size_t)13 * f.NA;
+  double x11 = x11p[a], x13 = x13p[a];
+  double a0d2 = f.mlf[10], a2d2 = f.mlf[12], a4d2 = f.mlf[14];
+  for (int t = 0; t < f.nt; t++) {
+    int tt = f.tl[t];
+    long long o = (long long)tt * f.NA + a;
+    double eT = f.sc[4 * f.T + tt];
+    double dtmp;
+    if (f.has_Z) {
+      double eS = f.sc[5 * f.T + tt];
+      double q = f.lf_AnZtSZnAt[o];
+      dtmp = + 1*a0d2*eS - 2*f.sc[2 * f.T + tt] + 1*a2d2*eS - 2*x11*eS + 4*f.An_a2d2_ZtSn[o]
+             - 2*x13*eS + 1*a2d2*eS - 2*f.sc[3 * f.T + tt] + 1*a4d2*eS;
+      q += -dtmp;
+      f.QR_ZSZ[o] = q;
+    }
+    double r = f.lf_AnAtTAnAt[o];
+    dtmp = + 1*a0d2*eT - 2*f.sc[0 * f.T + tt] + 1*a2d2*eT - 2*x11*eT + 4*f.An_a2d2_AtTn[o]
+           - 2*x13*eT + 1*a2d2*eT - 2*f.sc[1 * f.T + tt] + 1*a4d2*eT;
+    r += -dtmp;
+    dtmp = f.T_AnAt_AnAt[o];
+    r += -dtmp;
+    int vT = pm_of(f.Tt + (size_t)tt * f.rbT, a);
+    dtmp = vT*( + 1*a0d2 - 4*(x11) + 6*a2d2 - 4*(x13) + 1*a4d2 );
+    r += +dtmp;
+    f.QR_TAT[o] = r;
+  }
+}
+void flat_qr(const FlatQR& f, hipStream_t st)
+{
+  if (f.nar <= 0) return;
+  hipLaunchKernelGGL(k_flat_qr, dim3(cdiv(f.nar, 256)), dim3(256), 0, st, f);
+}
+
+// get_flattenloop, QC half (bcc_flattenloop.c:1720-1889)
+__global__ void k_flat_qc(FlatQC f)
+{
+  int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= f.nk) return;
+  int kk = f.kl[j];
+  int b = kk / kPopBits;
+  const double* J = f.ajdk; int P = f.pcols;
+  double D01 = J[b + 5 * P], D11 = J[b + 6 * P], D21 = J[b + 7 * P], a10 = J[b + 1 * P], a20 = J[b + 2 * P];
+  double eA = f.et_An[kk], eAa = f.sc[6 * f.T], AtA = f.At_An_a1d1[kk];
+  double a0d1 = f.mlf[5], a2d1 = f.mlf[7];
+  long long rpop = f.rpop;
+  for (int t = 0; t < f.nt; t++) {
+    int tt = f.tl[t];
+    long long o = (long long)tt * f.M + kk;
+    double eT = f.sc[4 * f.T + tt];
+    double dtmp;
+    double D = J[kk / kPopBits + 5 * P];
+    if (f.has_Z) {
+      double q = D * f.lf_AtTAnZtSZn[o];
+      double eS = f.sc[5 * f.T + tt];
+      dtmp = D01 * ( +(1*eT) - (2*a10 * f.lf_AtTn[o]) + (a20 * eT) ) * D01 * ( +(1*eS) - (2*a10 * f.lf_ZtSn[o]) + (a20 * eS) );
+      q += -dtmp;
+      f.QC_ZSZ[o] = q;
+    }
+    double r = D * f.lf_AtTAnAtTAn[o];
+    dtmp = D01 * ( +(1*eT) - (2*a10 * f.lf_AtTn[o]) + (a20 * eT) ) * D01 * ( +(1*eT) - (2*a10 * f.lf_AtTn[o]) + (a20 * eT) );
+    r += -dtmp;
+    dtmp = ( +(1*D01*(rpop)*a0d1) - (2*D11*eA*a0d1) + (1*D21*(rpop)*a0d1)
+             - (2*D01*eAa) + (4*D11*AtA) - (2*D21*eAa)
+             + (1*D01*(rpop)*a2d1) - (2*D11*eA*a2d1) + (1*D21*(rpop)*a2d1) );
+    r += -dtmp;
+    dtmp = ( +(1*J[b + 10 * P]*(rpop)) - (4*J[b + 11 * P]*eA) + (6*J[b + 12 * P]*(rpop))
+             - (4*J[b + 13 * P]*eA) + (1*J[b + 14 * P]*(rpop)) );
+    r += +dtmp;
+    f.QC_TAT[o] = r;
+  }
+}
+void flat_qc(const FlatQC& f, hipStream_t st)
+{
+  if (f.nk <= 0) return;
+  hipLaunchKernelGGL(k_flat_qc, dim3(cdiv(f.nk, 256)), dim3(256), 0, st, f);
+}
+
+// ------------------------------------------------------------------
+// bcc_sumscores_nrm / _ifT / _cmb (bcc_sumscores.c:324-356, 483-548, 550-779) for one
+// bin, followed by the xij gather of column 0: out[i] = score of list[i].
+// pow_form: the QR rows' root-of-squares form; else the QC rows' subtractive form.
+// ------------------------------------------------------------------
+__global__ void k_scores(Scores s)
+{
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= s.n) return;
+  int r = s.list[i];
+  double z[kMaxT], x[kMaxT];
+  for (int t = 0; t < kMaxT; t++) { z[t] = 0.0; x[t] = 0.0; }
+  for (int t = 0; t < s.nt; t++) {
+    int tt = s.tl[t];
+    if (s.has_Z) z[tt] = 0.0 + s.ZSZ[(long long)tt * s.ld + r] / s.dZ;
+    x[tt] = 0.0 + s.TAT[(long long)tt * s.ld + r] / s.dT;
+  }
+  if (s.den > 0) {
+    for (int pass = 0; pass < 2; pass++) {
+      double* v = pass == 0 ? z : x;
+      if (pass == 0 && !s.has_Z) continue;
+      if (s.pow_form) {
+        if (v[0] >= 0) {
+          v[0] *= v[0];
+          for (int c = 0; c < s.nt; c++) if (s.tl[c] >= 1) v[0] -= v[s.tl[c]] * v[s.tl[c]] / s.den;
+          v[0] = v[0] >= 0 ? sqrt(v[0]) : -sqrt(-v[0]);
+        }
+      } else {
+        for (int c = 0; c < s.nt; c++) if (s.tl[c] >= 1) v[0] -= v[s.tl[c]] / s.den;
+      }
+    }
+  }
+  if (s.cmb) x[0] += -1.0 * z[0];
+  s.out[i] = x[0];
+}
+void scores(const Scores& s, hipStream_t st)
+{
+  if (s.n <= 0) return;
+  hipLaunchKernelGGL(k_scores, dim3(cdiv(s.n, 256)), dim3(256), 0, st, s);
+}
+
+// ------------------------------------------------------------------
+// bcc_lrup_QC_YnWt_stage_c (bcc_lrup.c:592-655): L2_clean, then subtract the low-rank terms
+// ------------------------------------------------------------------
+__global__ void k_stage_c(StageC c)
+{
+  long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (long long)c.M * c.T) return;
+  int t = (int)(e / c.M), k = (int)(e % c.M);
+  int keep = bit_w(c.Ac_rtn, k) && bit_w(c.Ac_b, k) && bit_w(c.Tc_j, t);
+  if (!keep) { if (c.has_Z) c.ZSZ[e] = 0; c.TAT[e] = 0; return; }
+  if (c.has_Z) {
+    double q = c.ZSZ[e];
+    q += -1.0 * c.vtszn[e];
+    q += -1.0 * c.t_knzt[e];
+    q += -1.0 * c.t_hnvt[e];
+    c.ZSZ[e] = q;
+  }
+  double r = c.TAT[e];
+  r += -1.0 * c.jttan[e];
+  r += -1.0 * c.htrkn[e];
+  r += -1.0 * c.htrkn[e];
+  r += -1.0 * c.hhtrkn[e];
+  r += -1.0 * c.t_ankt[e];
+  r += -1.0 * c.t_ankt[e];
+  r += -1.0 * c.t_knkt[e];
+  c.TAT[e] = r;
+}
+void stage_c(const StageC& c, hipStream_t st)
+{
+  long long n = (long long)c.M * c.T;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_stage_c, dim3(cdiv(n, 256)), dim3(256), 0, st, c);
+}
+
+__global__ void k_fill(double* p, long long n, double v)
+{
+  long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+void fill(double* p, long long n, double v, hipStream_t st)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill, dim3(cdiv(n, 256)), dim3(256), 0, st, p, n, v);
+}
+
+}  // namespace lkc

@@ -1,0 +1,243 @@
+"""ctypes binding of the gfx950 library ``liblkc.so`` (C ABI in ``include/lkc.h``).
+
+The binding mirrors the reference's driver-level interface: an elimination is
+configured by the key/value text of a lakcluster ``.in`` file
+(updateglobals.c:5-199) and produces the trace and xdrop lists that
+``lakcluster_driver`` writes (lakcluster_driver.c:14-251,
+bcc_sumscores.c:127-157).  There is no CPU fallback: loading fails with an
+error when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblkc.so")
+DRIVER_PATH = os.path.join(_HERE, "lkc_driver")
+
+_lib = None
+
+
+class LkcError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load liblkc.so (built by ``make -C lakcluster_c_amd/csrc``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LkcError(f"{LIB_PATH} is missing: build it with `make -C lakcluster_c_amd/csrc`")
+    L = C.CDLL(LIB_PATH)
+    vp, ip, dp, cp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_char_p
+    u8p = C.POINTER(C.c_uint8)
+    sig = {
+        "lkc_last_error": (cp, []),
+        "lkc_bcc_error": (cp, [vp]),
+        "lkc_bsize": (C.c_size_t, [C.c_int]),
+        "lkc_lakcluster_driver": (C.c_int, [cp, cp]),
+        "lkc_bcc_load": (C.c_int, [cp, C.POINTER(vp)]),
+        "lkc_bcc_init_lf": (C.c_int, [vp]),
+        "lkc_bcc_iterate": (C.c_int, [vp, ip]),
+        "lkc_bcc_run": (C.c_int, [vp, C.c_int]),
+        "lkc_bcc_write_outputs": (C.c_int, [vp, cp]),
+        "lkc_bcc_An_ajdk": (C.c_int, [vp]),
+        "lkc_bcc_lf_ZtSn": (C.c_int, [vp]),
+        "lkc_bcc_lf_YnWt": (C.c_int, [vp]),
+        "lkc_bcc_lf_At_T_YnWt_S_Zn": (C.c_int, [vp]),
+        "lkc_bcc_lf_AnZt_S_WnYt": (C.c_int, [vp]),
+        "lkc_bcc_singlestudy_ww": (C.c_int, [vp]),
+        "lkc_bcc_doublestudy_ww": (C.c_int, [vp]),
+        "lkc_bcc_flattenloop": (C.c_int, [vp]),
+        "lkc_bcc_sumscores": (C.c_int, [vp]),
+        "lkc_bcc_lrup": (C.c_int, [vp]),
+        "lkc_bcc_n_iter": (C.c_int, [vp]),
+        "lkc_bcc_n_trace": (C.c_int, [vp]),
+        "lkc_bcc_get_trace": (C.c_int, [vp, dp]),
+        "lkc_bcc_n_xdrop": (C.c_int, [vp]),
+        "lkc_bcc_get_xdrop": (C.c_int, [vp, ip, ip]),
+        "lkc_bcc_ops": (C.c_longlong, [vp]),
+        "lkc_bcc_times": (C.c_int, [vp, dp]),
+        "lkc_bcc_shape": (C.c_int, [vp, ip]),
+        "lkc_bcc_free": (None, [vp]),
+        "lkc_bcc_debug_get": (C.c_long, [vp, cp, dp, C.c_long]),
+        "lkc_An_ajdk_v": (C.c_int, [u8p, C.c_size_t, C.c_int, ip, C.c_int, u8p, C.c_int, dp, dp]),
+        "lkc_AnZt_vv": (C.c_int, [u8p, C.c_size_t, C.c_int, ip, C.c_int, u8p, C.c_size_t, C.c_int, ip, C.c_int,
+                                  u8p, C.c_int, dp, dp, dp, dp]),
+        "lkc_An_ZtSn_ww": (C.c_int, [u8p, C.c_size_t, C.c_int, ip, C.c_int, u8p, dp, ip, C.c_int, C.c_int, C.c_int,
+                                     C.c_int, C.c_longlong, dp]),
+        "lkc_xcalc": (C.c_int, [dp, ip, C.c_int, C.c_int, C.c_int, C.c_int, C.c_longlong,
+                                C.POINTER(C.c_ulonglong), dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int, handle=None) -> None:
+    if rc != 0:
+        msg = lib().lkc_bcc_error(handle) if handle else lib().lkc_last_error()
+        raise LkcError(f"status {rc}: {msg.decode() if msg else ''}")
+
+
+def _ptr(a: np.ndarray, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+class Bcc:
+    """One elimination resident on the device (lkc_bcc handle)."""
+
+    def __init__(self, in_text: str):
+        self._h = C.c_void_p()
+        _check(lib().lkc_bcc_load(in_text.encode(), C.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h:
+            lib().lkc_bcc_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def init_lf(self) -> None:
+        _check(lib().lkc_bcc_init_lf(self._h), self._h)
+
+    def iterate(self) -> bool:
+        done = C.c_int(0)
+        _check(lib().lkc_bcc_iterate(self._h, C.byref(done)), self._h)
+        return bool(done.value)
+
+    def run(self, max_iter: int = -1) -> None:
+        _check(lib().lkc_bcc_run(self._h, max_iter), self._h)
+
+    def write_outputs(self, out_dir: Optional[str] = None) -> None:
+        _check(lib().lkc_bcc_write_outputs(self._h, out_dir.encode() if out_dir else None), self._h)
+
+    @property
+    def n_iter(self) -> int:
+        return lib().lkc_bcc_n_iter(self._h)
+
+    def trace(self) -> np.ndarray:
+        n = lib().lkc_bcc_n_trace(self._h)
+        out = np.zeros((max(n, 1), 6), dtype=np.float64)
+        _check(lib().lkc_bcc_get_trace(self._h, _ptr(out, C.c_double)), self._h)
+        return out[:n]
+
+    def xdrop(self) -> Tuple[np.ndarray, np.ndarray]:
+        n = lib().lkc_bcc_n_xdrop(self._h)
+        a = np.zeros((max(n, 1), 2), dtype=np.int32)
+        b = np.zeros((max(n, 1), 2), dtype=np.int32)
+        _check(lib().lkc_bcc_get_xdrop(self._h, _ptr(a, C.c_int), _ptr(b, C.c_int)), self._h)
+        return a[:n], b[:n]
+
+    @property
+    def ops(self) -> int:
+        return int(lib().lkc_bcc_ops(self._h))
+
+    def times(self) -> Tuple[float, float, float]:
+        t = np.zeros(3, dtype=np.float64)
+        _check(lib().lkc_bcc_times(self._h, _ptr(t, C.c_double)), self._h)
+        return float(t[0]), float(t[1]), float(t[2])
+
+    def state(self, name: str) -> np.ndarray:
+        """A named score array of the device state (row-major, covariate last)."""
+        n = lib().lkc_bcc_debug_get(self._h, name.encode(), None, 0)
+        if n < 0:
+            raise LkcError(f"unknown state array {name}")
+        out = np.zeros(max(n, 1), dtype=np.float64)
+        lib().lkc_bcc_debug_get(self._h, name.encode(), _ptr(out, C.c_double), n)
+        return out[:n]
+
+    def shape(self) -> Tuple[int, int, int, int]:
+        s = np.zeros(4, dtype=np.int32)
+        _check(lib().lkc_bcc_shape(self._h, _ptr(s, C.c_int)), self._h)
+        return tuple(int(x) for x in s)
+
+
+def lakcluster_driver(in_text: str, out_dir: Optional[str] = None) -> None:
+    """Whole run with the three output files written (lakcluster_driver)."""
+    _check(lib().lkc_lakcluster_driver(in_text.encode(), out_dir.encode() if out_dir else None))
+
+
+def bsize(ncols: int) -> int:
+    return int(lib().lkc_bsize(ncols))
+
+
+def an_ajdk_v(X: np.ndarray, rows: Sequence[int], colmask: np.ndarray, ncols: int, ajdk: np.ndarray) -> np.ndarray:
+    """get_An_ajdk_v on packed rows X (nrows_total x rb bytes); returns (len(rows), 15)."""
+    X = np.ascontiguousarray(X, dtype=np.uint8)
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    colmask = np.ascontiguousarray(colmask, dtype=np.uint8)
+    ajdk = np.ascontiguousarray(ajdk, dtype=np.float64)
+    out = np.zeros((max(rows.size, 1), 15), dtype=np.float64)
+    _check(lib().lkc_An_ajdk_v(_ptr(X, C.c_uint8), X.shape[1], X.shape[0], _ptr(rows, C.c_int), rows.size,
+                               _ptr(colmask, C.c_uint8), ncols, _ptr(ajdk, C.c_double), _ptr(out, C.c_double)))
+    return out[:rows.size]
+
+
+def anzt_vv(A: np.ndarray, arows, Z: np.ndarray, zrows, colmask: np.ndarray, ncols: int,
+            ajdk: Optional[np.ndarray] = None, xa=None, xz=None) -> np.ndarray:
+    """get_AnZt_vv on packed rows; returns (len(arows), len(zrows))."""
+    A = np.ascontiguousarray(A, dtype=np.uint8)
+    Z = np.ascontiguousarray(Z, dtype=np.uint8)
+    arows = np.ascontiguousarray(arows, dtype=np.int32)
+    zrows = np.ascontiguousarray(zrows, dtype=np.int32)
+    colmask = np.ascontiguousarray(colmask, dtype=np.uint8)
+    out = np.zeros((max(arows.size, 1), max(zrows.size, 1)), dtype=np.float64)
+    dnull = C.POINTER(C.c_double)()
+    aj = _ptr(np.ascontiguousarray(ajdk, dtype=np.float64), C.c_double) if ajdk is not None else dnull
+    xa_ = np.ascontiguousarray(xa, dtype=np.float64) if xa is not None else None
+    xz_ = np.ascontiguousarray(xz, dtype=np.float64) if xz is not None else None
+    _check(lib().lkc_AnZt_vv(_ptr(A, C.c_uint8), A.shape[1], A.shape[0], _ptr(arows, C.c_int), arows.size,
+                             _ptr(Z, C.c_uint8), Z.shape[1], Z.shape[0], _ptr(zrows, C.c_int), zrows.size,
+                             _ptr(colmask, C.c_uint8), ncols, aj,
+                             _ptr(xa_, C.c_double) if xa_ is not None else dnull,
+                             _ptr(xz_, C.c_double) if xz_ is not None else dnull, _ptr(out, C.c_double)))
+    return out[:arows.size, :zrows.size]
+
+
+def xcalc(L: np.ndarray, B: int, D_MLT: Optional[int] = None) -> Tuple[np.ndarray, float, float]:
+    """get_xcalc quantiser on L (nn x nw); returns cz (nw x nn), min, mlt."""
+    L = np.ascontiguousarray(L, dtype=np.float64)
+    nn, nw = L.shape
+    D = (1 << B) if D_MLT is None else D_MLT
+    cz = np.zeros((max(nw, 1), max(nn, 1)), dtype=np.uint64)
+    mm = np.zeros(2, dtype=np.float64)
+    rows = np.arange(nn, dtype=np.int32)
+    _check(lib().lkc_xcalc(_ptr(L, C.c_double), _ptr(rows, C.c_int), nn, nn, nw, B, D,
+                           _ptr(cz, C.c_ulonglong), _ptr(mm, C.c_double)))
+    return cz[:nw, :nn], float(mm[0]), float(mm[1])
+
+
+def an_ztsn_ww(A: np.ndarray, alist, amask: np.ndarray, L: np.ndarray, nlist, n_total: int, B: int,
+               D_MLT: Optional[int] = None) -> np.ndarray:
+    """xcalc of L (len(nlist) x ns) then An_ZtSn_ww for rows alist of A; returns (A rows, ns)."""
+    A = np.ascontiguousarray(A, dtype=np.uint8)
+    alist = np.ascontiguousarray(alist, dtype=np.int32)
+    amask = np.ascontiguousarray(amask, dtype=np.uint8)
+    L = np.ascontiguousarray(L, dtype=np.float64)
+    nlist = np.ascontiguousarray(nlist, dtype=np.int32)
+    ns = L.shape[1]
+    out = np.zeros((A.shape[0], max(ns, 1)), dtype=np.float64)
+    D = (1 << B) if D_MLT is None else D_MLT
+    _check(lib().lkc_An_ZtSn_ww(_ptr(A, C.c_uint8), A.shape[1], A.shape[0], _ptr(alist, C.c_int), alist.size,
+                                _ptr(amask, C.c_uint8), _ptr(L, C.c_double), _ptr(nlist, C.c_int), nlist.size,
+                                n_total, ns, B, D, _ptr(out, C.c_double)))
+    return out[:, :ns]

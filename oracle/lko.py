@@ -1,0 +1,182 @@
+"""ctypes binding of the CPU oracle (oracle/bin/liblko_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker the HIP path is compared against.
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from typing import Optional, Tuple
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "bin", "liblko_oracle.so")
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = C.CDLL(LIB_PATH)
+    vp, ip, dp, cp, u8p = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_char_p, C.POINTER(C.c_uint8)
+    sig = {
+        "lko_bsize": (C.c_size_t, [C.c_int]),
+        "lko_psize": (C.c_int, [C.c_int]),
+        "lko_get_xdrop": (None, [C.c_double, C.c_double, C.c_double, C.c_int, ip, ip]),
+        "lko_calc_ajdk": (None, [dp, C.c_int, dp]),
+        "lko_quicksort_xij": (C.c_uint, [dp, ip, ip, ip, ip, ip, ip, C.c_int, C.c_int]),
+        "lko_an_ajdk_v": (None, [u8p, C.c_size_t, ip, C.c_int, u8p, C.c_int, dp, dp]),
+        "lko_anzt_vv": (None, [u8p, C.c_size_t, ip, C.c_int, u8p, C.c_size_t, ip, C.c_int, u8p, C.c_int, dp, dp, dp, dp]),
+        "lko_run_text": (vp, [cp, cp, C.c_int, C.c_int]),
+        "lko_status": (C.c_int, [vp, cp, C.c_int]),
+        "lko_n_trace": (C.c_int, [vp]),
+        "lko_n_xdrop": (C.c_int, [vp]),
+        "lko_n_iter": (C.c_int, [vp]),
+        "lko_ops": (C.c_longlong, [vp]),
+        "lko_times": (None, [vp, dp]),
+        "lko_get_trace": (None, [vp, dp]),
+        "lko_get_drops": (None, [vp, ip, ip]),
+        "lko_free": (None, [vp]),
+        "lko_state_new": (vp, [cp]),
+        "lko_an_ztsn_ww_L": (None, [u8p, C.c_size_t, C.c_int, ip, C.c_int, u8p, dp, ip, C.c_int, C.c_int, C.c_int,
+                                    C.c_int, C.c_longlong, dp]),
+        "lko_state_iterate": (C.c_int, [vp]),
+        "lko_state_get": (C.c_long, [vp, cp, dp, C.c_long]),
+        "lko_state_free": (None, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+class OracleRun:
+    def __init__(self, trace, xa, xb, ops, times, n_iter):
+        self.trace, self.xdrop_a, self.xdrop_b, self.ops, self.times, self.n_iter = trace, xa, xb, ops, times, n_iter
+
+
+def run_text(in_text: str, max_iter: int = -1, out_dir: Optional[str] = None, write_files: bool = False) -> OracleRun:
+    L = lib()
+    h = L.lko_run_text(in_text.encode(), out_dir.encode() if out_dir else None, max_iter, 1 if write_files else 0)
+    err = C.create_string_buffer(256)
+    st = L.lko_status(h, err, 256)
+    if st != 0:
+        L.lko_free(h)
+        raise RuntimeError(f"oracle: {err.value.decode()}")
+    nt, nx = L.lko_n_trace(h), L.lko_n_xdrop(h)
+    tr = np.zeros((max(nt, 1), 6)); xa = np.zeros((max(nx, 1), 2), np.int32); xb = np.zeros((max(nx, 1), 2), np.int32)
+    L.lko_get_trace(h, _p(tr, C.c_double))
+    L.lko_get_drops(h, _p(xa, C.c_int), _p(xb, C.c_int))
+    t = np.zeros(3)
+    L.lko_times(h, _p(t, C.c_double))
+    r = OracleRun(tr[:nt], xa[:nx], xb[:nx], int(L.lko_ops(h)), tuple(t), L.lko_n_iter(h))
+    L.lko_free(h)
+    return r
+
+
+def get_xdrop(r: float, c: float, gamma: float, gamma_type: int = 0) -> Tuple[int, int]:
+    a, b = C.c_int(0), C.c_int(0)
+    lib().lko_get_xdrop(r, c, gamma, gamma_type, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def calc_ajdk(p: np.ndarray) -> np.ndarray:
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    out = np.zeros(15 * p.size)
+    lib().lko_calc_ajdk(_p(p, C.c_double), p.size, _p(out, C.c_double))
+    return out
+
+
+def quicksort(d: np.ndarray):
+    d = np.ascontiguousarray(d, dtype=np.float64).copy()
+    ix = np.arange(d.size, dtype=np.int32)
+    null = C.POINTER(C.c_int)()
+    lib().lko_quicksort_xij(_p(d, C.c_double), _p(ix, C.c_int), null, null, null, null, null, 0, d.size - 1)
+    return d, ix
+
+
+def an_ajdk_v(X, rows, colmask, ncols, ajdk):
+    X = np.ascontiguousarray(X, np.uint8); rows = np.ascontiguousarray(rows, np.int32)
+    colmask = np.ascontiguousarray(colmask, np.uint8); ajdk = np.ascontiguousarray(ajdk, np.float64)
+    out = np.zeros((max(rows.size, 1), 15))
+    lib().lko_an_ajdk_v(_p(X, C.c_uint8), X.shape[1], _p(rows, C.c_int), rows.size, _p(colmask, C.c_uint8), ncols,
+                        _p(ajdk, C.c_double), _p(out, C.c_double))
+    return out[:rows.size]
+
+
+def anzt_vv(A, arows, Z, zrows, colmask, ncols, ajdk=None, xa=None, xz=None):
+    A = np.ascontiguousarray(A, np.uint8); Z = np.ascontiguousarray(Z, np.uint8)
+    arows = np.ascontiguousarray(arows, np.int32); zrows = np.ascontiguousarray(zrows, np.int32)
+    colmask = np.ascontiguousarray(colmask, np.uint8)
+    out = np.zeros((max(arows.size, 1), max(zrows.size, 1)))
+    dnull = C.POINTER(C.c_double)()
+    f = lambda v: _p(np.ascontiguousarray(v, np.float64), C.c_double) if v is not None else dnull
+    keep = [np.ascontiguousarray(v, np.float64) for v in (ajdk, xa, xz) if v is not None]
+    lib().lko_anzt_vv(_p(A, C.c_uint8), A.shape[1], _p(arows, C.c_int), arows.size, _p(Z, C.c_uint8), Z.shape[1],
+                      _p(zrows, C.c_int), zrows.size, _p(colmask, C.c_uint8), ncols, f(ajdk), f(xa), f(xz),
+                      _p(out, C.c_double))
+    del keep
+    return out[:arows.size, :zrows.size]
+
+
+def an_ztsn_ww(A, alist, amask, L, nlist, n_total, B, D_MLT=None):
+    """xcalc of L (len(nlist) x ns) then An_ZtSn_ww for rows alist of A; returns (A rows, ns)."""
+    A = np.ascontiguousarray(A, np.uint8); alist = np.ascontiguousarray(alist, np.int32)
+    amask = np.ascontiguousarray(amask, np.uint8); L = np.ascontiguousarray(L, np.float64)
+    nlist = np.ascontiguousarray(nlist, np.int32)
+    ns = L.shape[1]
+    out = np.zeros((A.shape[0], max(ns, 1)))
+    D = (1 << B) if D_MLT is None else D_MLT
+    lib().lko_an_ztsn_ww_L(_p(A, C.c_uint8), A.shape[1], A.shape[0], _p(alist, C.c_int), alist.size, _p(amask, C.c_uint8),
+                           _p(L, C.c_double), _p(nlist, C.c_int), nlist.size, n_total, ns, B, D, _p(out, C.c_double))
+    return out[:, :ns]
+
+
+class State:
+    """Stepwise oracle state: load + initial pass, then one loop pass per iterate()."""
+
+    def __init__(self, in_text: str):
+        self._h = lib().lko_state_new(in_text.encode())
+        if not self._h:
+            raise RuntimeError("oracle: load failed")
+
+    def iterate(self) -> bool:
+        return bool(lib().lko_state_iterate(self._h))
+
+    def get(self, name: str) -> np.ndarray:
+        n = lib().lko_state_get(self._h, name.encode(), None, 0)
+        if n < 0:
+            raise KeyError(name)
+        out = np.zeros(max(n, 1))
+        lib().lko_state_get(self._h, name.encode(), _p(out, C.c_double), n)
+        return out[:n]
+
+    def close(self):
+        if self._h:
+            lib().lko_state_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+STATE_NAMES = ["lf_An_ajdk", "lf_Zn_ajdk", "lf_AtTn", "lf_ZtSn", "lf_AnZt", "lf_AnAt", "lf_AtTAnZtSZn",
+               "lf_AtTAnAtTAn", "lf_AnZtSZnAt", "lf_AnAtTAnAt", "At_An_a1d1", "An_a2d2_ZtSn", "An_a2d2_AtTn",
+               "a1d2_AtTn", "a3d2_AtTn", "a1d2_ZtSn", "a3d2_ZtSn", "et_Tn", "et_Sn", "et_An", "et_An_a1d1",
+               "T_AnAt_AnAt", "QR_ZSZ", "QR_TAT", "QC_ZSZ", "QC_TAT"]

@@ -1390,3 +1390,76 @@ void lko_times(void *h, double *t3) { lko_result *R = (lko_result *)h; t3[0] = R
 void lko_get_trace(void *h, double *out) { lko_result *R = (lko_result *)h; memcpy(out, R->trace, sizeof(double) * 6 * R->n_trace); }
 void lko_get_drops(void *h, int *a, int *b) { lko_result *R = (lko_result *)h; memcpy(a, R->xdrop_a, sizeof(int) * 2 * R->n_xdrop); memcpy(b, R->xdrop_b, sizeof(int) * 2 * R->n_xdrop); }
 void lko_free(void *h) { lko_result *R = (lko_result *)h; if (R) { free(R->trace); free(R->xdrop_a); free(R->xdrop_b); free(R); } }
+
+/* ------------------------------------------------------------------ */
+/* stepwise access to the state, for stage-level comparisons in tests  */
+/* ------------------------------------------------------------------ */
+typedef struct { lko_state S; lko_result R; int iter; } lko_stepper;
+
+void *lko_state_new(const char *in_text)
+{
+  lko_stepper *P = (lko_stepper *)xcalloc(1, sizeof(lko_stepper));
+  if (cfg_parse(in_text, &P->S.c) || st_load(&P->S, &P->R)) { st_free(&P->S); free(P); return NULL; }
+  st_ajdk(&P->S); st_ZtSn(&P->S); st_YnWt(&P->S); st_At_T(&P->S); st_QR(&P->S);
+  st_single(&P->S); st_double(&P->S); st_flatten(&P->S);
+  return P;
+}
+
+int lko_state_iterate(void *h)
+{
+  lko_stepper *P = (lko_stepper *)h; lko_state *S = &P->S;
+  if (!(popmask(S->Ar_j, S->NA) > 0 || popmask(S->Ac_j, S->M) > 0)) return 1;
+  int Irem = 0, rdrop = 0, cdrop = 0;
+  st_nrm_ifT_cmb(S); st_xij(S, &Irem); st_mxA(S, P->iter, Irem, &rdrop, &cdrop);
+  st_lrup(S); st_rescore_tail(S);
+  P->iter++;
+  return (popmask(S->Ar_j, S->NA) > 0 || popmask(S->Ac_j, S->M) > 0) ? 0 : 1;
+}
+
+/* copies the named array (oracle layout) into out; returns its length, or -1 */
+long lko_state_get(void *h, const char *name, double *out, long n)
+{
+  lko_state *S = &((lko_stepper *)h)->S;
+  size_t NA = S->NA, NZ = S->NZ, M = S->M, T = S->T;
+  struct { const char *nm; const double *p; size_t len; } tab[] = {
+    { "lf_An_ajdk", S->lf_An_ajdk, NA * LKO_AJDK }, { "lf_Zn_ajdk", S->lf_Zn_ajdk, NZ * LKO_AJDK },
+    { "lf_AtTn", S->lf_AtTn, M * T }, { "lf_ZtSn", S->lf_ZtSn, M * T },
+    { "lf_AnZt", S->lf_AnZt, NA * NZ }, { "lf_AnAt", S->lf_AnAt, NA * NA },
+    { "lf_AtTAnZtSZn", S->lf_AtTAnZtSZn, M * T }, { "lf_AtTAnAtTAn", S->lf_AtTAnAtTAn, M * T },
+    { "lf_AnZtSZnAt", S->lf_AnZtSZnAt, NA * T }, { "lf_AnAtTAnAt", S->lf_AnAtTAnAt, NA * T },
+    { "At_An_a1d1", S->At_An_a1d1, M }, { "An_a2d2_ZtSn", S->An_a2d2_ZtSn, NA * T }, { "An_a2d2_AtTn", S->An_a2d2_AtTn, NA * T },
+    { "a1d2_AtTn", S->a1d2_AtTn, T }, { "a3d2_AtTn", S->a3d2_AtTn, T }, { "a1d2_ZtSn", S->a1d2_ZtSn, T }, { "a3d2_ZtSn", S->a3d2_ZtSn, T },
+    { "et_Tn", S->et_Tn, T }, { "et_Sn", S->et_Sn, T }, { "et_An", S->et_An, M }, { "et_An_a1d1", &S->et_An_a1d1, 1 },
+    { "T_AnAt_AnAt", S->T_AnAt_AnAt, T * NA },
+    { "QR_ZSZ", S->QR_ZSZ, NA * T }, { "QR_TAT", S->QR_TAT, NA * T }, { "QC_ZSZ", S->QC_ZSZ, M * T }, { "QC_TAT", S->QC_TAT, M * T },
+  };
+  for (size_t i = 0; i < sizeof tab / sizeof tab[0]; i++) {
+    if (strcmp(tab[i].nm, name)) continue;
+    if (out) memcpy(out, tab[i].p, sizeof(double) * (size_t)MINI((long)tab[i].len, n));
+    return (long)tab[i].len;
+  }
+  return -1;
+}
+
+void lko_state_free(void *h)
+{
+  lko_stepper *P = (lko_stepper *)h;
+  if (!P) return;
+  free(P->S.trace); free(P->S.xdrop_a); free(P->S.xdrop_b);
+  st_free(&P->S); free(P);
+}
+
+/* get_An_ZtSn_ww with its xcalc (An_ZtSn_ww.c:44-70, xcalc.c:41-246) on caller buffers:
+   L[i*ns + s] for n-row nlist[i]; out[alist[a]*ns + s] */
+void lko_an_ztsn_ww_L(const uint8_t *A, size_t rbA, int nr_total, const int *alist, int na, const uint8_t *amask,
+                      const double *L, const int *nlist, int nn, int n_total, int ns, int B, long long D_MLT,
+                      double *out)
+{
+  lko_planes P; memset(&P, 0, sizeof P);
+  lko_xcalc(L, nlist, nn, n_total, ns, B, D_MLT, &P, NULL);
+  bm_t m; m.b = (uint8_t *)A; m.nr = nr_total; m.nc = n_total; m.rb = rbA;
+  int *sl = (int *)xcalloc(ns + 1, sizeof(int));
+  for (int s = 0; s < ns; s++) sl[s] = s;
+  an_ztsn_ww(&m, alist, na, amask, &P, sl, ns, out, ns);
+  free(sl); lko_planes_free(&P);
+}

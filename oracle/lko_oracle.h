@@ -73,6 +73,20 @@ void lko_get_trace(void *h, double *out);
 void lko_get_drops(void *h, int *a, int *b);
 void lko_free(void *h);
 
+/* get_An_ZtSn_ww with its xcalc on caller buffers: L[i*ns + s] for n-row nlist[i];
+   rows of A have rbA = lko_bsize(n_total) bytes; out[alist[a]*ns + s] */
+void lko_an_ztsn_ww_L(const uint8_t *A, size_t rbA, int nr_total, const int *alist, int na, const uint8_t *amask,
+                      const double *L, const int *nlist, int nn, int n_total, int ns, int B, long long D_MLT,
+                      double *out);
+
+/* stepwise access for stage-level comparisons: load + initial pass, one loop
+   pass (returns 1 when no row or column remains), named arrays in the layouts
+   of the restatement ([row*T + t], [a*NZ + z], [r*15 + jk]) */
+void *lko_state_new(const char *in_text);
+int lko_state_iterate(void *S);
+long lko_state_get(void *S, const char *name, double *out, long n);
+void lko_state_free(void *S);
+
 #ifdef __cplusplus
 }
 #endif
