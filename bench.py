@@ -1,0 +1,149 @@
+#!/usr/bin/env python3
+"""Benchmark: one whole bcc elimination on one MI355X.
+
+Workload: the 16384 x 65536 case-control problem (A and Z 16384 x 65536, planted
+208-row bicluster in A, T/S with 3 covariate columns, gamma 0.02, B_MLT 32),
+generated with a fixed seed.  A step is one elimination run to the end: the
+initial loop-subscores plus every elimination pass, with the inputs already
+resident in device memory (loading the .b16 files is not timed; between steps
+the handle is reset to its loaded state).  The metric is the reference's
+bit-operation accounting (the blop formulas of AnZt_vv.c:99, At_T_YnWt_S_Zn_ww.c:110,
+An_ZtSn_ww.c:66, An_ajdk_v.c:328, xcalc.c:243) summed over the run, divided by the
+elimination wall time ("elim-run s").
+
+Output: one JSON line (see --help).  --gpus above 1 is refused: the path runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_INT8_TOPS = 5000.0      # MI355X dense int8 MFMA, TOP/s
+PEAK_HBM_GBS = 8000.0
+
+
+def make_data(root, name, kw):
+    from lakcluster_c_amd import gen
+    marker = os.path.join(root, f"{name}.ready")
+    in_path = os.path.join(root, f"dir_{name}_in", f"{name}.in")
+    if os.path.exists(marker) and os.path.exists(in_path):
+        return in_path
+    t0 = time.time()
+    ds = gen.make_dataset(root, name, **kw)
+    with open(marker, "w") as f:
+        f.write(f"{time.time() - t0:.1f}\n")
+    return ds.in_path
+
+
+def cpu_baseline(root, seconds_cap=30.0):
+    """The CPU restatement (OpenMP) on a bounded sample of the same problem family."""
+    from oracle import lko
+    kw = dict(nA=1024, ncols=8192, nZ=1024, T=3, pmin=0.05, pmax=0.5, planted=(0.55,), gamma=0.02, B=32, seed=2)
+    in_path = make_data(root, "cpu_sample_1024x8192", kw)
+    text = open(in_path).read()
+    max_iter = 40
+    r = lko.run_text(text, max_iter=max_iter)
+    el = r.times[1] + r.times[2]
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": r.ops / el / 1e12 if el > 0 else None,
+        "unit": "Tbit-op/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"case-control A,Z 1024x8192, T=3, gamma 0.02, B 32: initial pass + {r.n_iter} elimination "
+                  f"steps, {r.ops:.3e} bit-ops in {el:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="case_control_16384x65536")
+    ap.add_argument("--max-iter", type=int, default=-1, help="elimination passes per run (-1: to the end)")
+    ap.add_argument("--data-root", default=os.path.join(tempfile.gettempdir(), "lkc_bench_data"))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this path runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from lakcluster_c_amd import gen, lkc
+    os.makedirs(args.data_root, exist_ok=True)
+    kw = dict(gen.CONFIGS[args.config])
+    kw.setdefault("seed", 1)
+    print(f"[bench] generating {args.config} ...", file=sys.stderr, flush=True)
+    in_path = make_data(args.data_root, args.config, kw)
+    text = open(in_path).read()
+    print("[bench] loading ...", file=sys.stderr, flush=True)
+    run = lkc.Bcc(text)
+    run.set_profiling(True)
+    for w in range(args.warmup):
+        t0 = time.perf_counter()
+        run.run(args.max_iter)
+        print(f"[bench] warmup {w}: {time.perf_counter() - t0:.2f} s, {run.n_iter} passes", file=sys.stderr, flush=True)
+        run.reset()
+    elim, ops, kms, kwork, kl = [], [], 0.0, 0.0, 0
+    n_iter = 0
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        run.run(args.max_iter)
+        t1 = time.perf_counter()
+        elim.append(t1 - t0)
+        ops.append(run.ops)
+        n_iter = run.n_iter
+        launches, ms, work = run.kernel_stats("ww")
+        kl += launches; kms += ms; kwork += work
+        print(f"[bench] step {k}: {t1 - t0:.3f} s, {run.n_iter} passes, {run.ops:.3e} bit-ops, "
+              f"ww {launches} launches {ms:.1f} ms", file=sys.stderr, flush=True)
+        if k + 1 < args.steps:
+            run.reset()
+    shape = run.shape()
+    run.close()
+    tot_s = sum(elim)
+    value = sum(ops) / tot_s / 1e12
+    achieved = kwork / (kms / 1e3) / 1e12 if kms > 0 else None
+    line = {
+        "metric": "elim-run s & Tbitop/s at 16k×64k; batched null runs/hr vs plain, 1 GPU",
+        "value": value,
+        "unit": "Tbit-op/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * tot_s / args.steps,
+        "elim_run_s": tot_s / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8/int32 popcount, f64",
+        "data": "synthetic (seeded generator: Bernoulli columns, planted rank-1 bicluster)",
+        "config": {"workload": args.config, "rows_A": shape[0], "rows_Z": shape[1], "cols": shape[2],
+                   "covariates": shape[3], "gamma": kw["gamma"], "B_MLT": kw["B"],
+                   "passes": n_iter, "bit_ops_per_run": ops[-1] if ops else None},
+        "roofline": {
+            "kernel": "At_T_YnWt_S_Zn_ww contraction",
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": PEAK_INT8_TOPS,
+            "unit": "TOP/s",
+            "frac": achieved / PEAK_INT8_TOPS if achieved else None,
+            "traffic": None,
+            "launches": kl,
+            "avg_launch_ms": kms / kl if kl else None,
+        },
+    }
+    if not args.no_cpu:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(args.data_root)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -87,6 +87,19 @@ int lkc_bcc_times(const lkc_bcc *D, double *t3);
 /* problem size: [rows of A, rows of Z, columns, covariate columns] */
 int lkc_bcc_shape(const lkc_bcc *D, int *out4);
 void lkc_bcc_free(lkc_bcc *D);
+/* Back to the state right after lkc_bcc_load (all rows and columns active, no
+   trace), keeping the device-resident inputs; the next run starts with init. */
+int lkc_bcc_reset(lkc_bcc *D);
+/* HIP-event timing of kernel groups on the handle's stream (off by default).
+   lkc_bcc_kernel_stats reports, for a group name ("ww": the At_T_YnWt_S_Zn_ww
+   contraction), the launches, the summed device milliseconds and the summed
+   algorithmic work (int8 multiply-adds x2 of the exact integer contraction)
+   since the last reset. */
+int lkc_bcc_set_profiling(lkc_bcc *D, int on);
+/* Implementation of the At_T_YnWt_S_Zn_ww contraction: 0 = int8 matrix cores
+   (default), 1 = VALU popcount over the B_MLT bit planes.  Both are bit-exact. */
+int lkc_bcc_set_ww_kernel(lkc_bcc *D, int which);
+int lkc_bcc_kernel_stats(const lkc_bcc *D, const char *name, long long *launches, double *ms, double *work);
 /* State inspection (no counterpart in the reference, which keeps these arrays in
    struct bcc_ajdk/bcc_single/bcc_double): copies the named score array, e.g.
    "lf_AnZt", "QC_TAT", "et_An", in row-major (row, column) order with the

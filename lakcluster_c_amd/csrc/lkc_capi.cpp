@@ -117,6 +117,41 @@ int lkc_bcc_shape(const lkc_bcc* D, int* o)
   return LKC_OK;
 }
 
+int lkc_bcc_reset(lkc_bcc* D)
+{
+  if (!D) return LKC_ERR_STATE;
+  return guard(D, [&] { D->run.reset(); });
+}
+
+int lkc_bcc_set_profiling(lkc_bcc* D, int on)
+{
+  if (!D) return LKC_ERR_STATE;
+  D->run.prof.on = on != 0;
+  return LKC_OK;
+}
+
+int lkc_bcc_set_ww_kernel(lkc_bcc* D, int which)
+{
+  if (!D || which < 0 || which > 1) return LKC_ERR_STATE;
+  D->run.ww_kernel = which;
+  return LKC_OK;
+}
+
+int lkc_bcc_kernel_stats(const lkc_bcc* D, const char* name, long long* launches, double* ms, double* work)
+{
+  if (!D || !name) return LKC_ERR_STATE;
+  for (const auto& k : D->run.prof.stats) if (k.name == name) {
+    if (launches) *launches = k.launches;
+    if (ms) *ms = k.ms;
+    if (work) *work = k.work;
+    return LKC_OK;
+  }
+  if (launches) *launches = 0;
+  if (ms) *ms = 0;
+  if (work) *work = 0;
+  return LKC_OK;
+}
+
 long lkc_bcc_debug_get(lkc_bcc* D, const char* name, double* out, long n)
 {
   if (!D || !name) return -1;

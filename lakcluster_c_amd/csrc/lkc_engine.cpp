@@ -202,6 +202,49 @@ static void upload_bits(Dev& dev, DMat& m, const HostBits& h)
 }
 
 // ------------------------------------------------------------------
+// kernel-group timing
+// ------------------------------------------------------------------
+Prof::~Prof()
+{
+  for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+  for (Pending& p : pend) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+}
+int Prof::id(const char* name)
+{
+  for (size_t i = 0; i < stats.size(); i++) if (stats[i].name == name) return (int)i;
+  stats.push_back(Stat()); stats.back().name = name;
+  return (int)stats.size() - 1;
+}
+hipEvent_t Prof::ev()
+{
+  if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+  hipEvent_t e; HIPCHK(hipEventCreate(&e)); return e;
+}
+void Prof::begin(const char* name, double work, hipStream_t st, Pending& p)
+{
+  p.id = -1;
+  if (!on) return;
+  p.id = id(name); p.a = ev(); p.b = ev();
+  stats[p.id].launches++; stats[p.id].work += work;
+  HIPCHK(hipEventRecord(p.a, st));
+}
+void Prof::end(Pending& p, hipStream_t st)
+{
+  if (p.id < 0) return;
+  HIPCHK(hipEventRecord(p.b, st));
+  pend.push_back(p);
+}
+void Prof::collect()
+{
+  for (Pending& p : pend) {
+    float ms = 0; HIPCHK(hipEventElapsedTime(&ms, p.a, p.b));
+    stats[p.id].ms += ms;
+    pool.push_back(p.a); pool.push_back(p.b);
+  }
+  pend.clear();
+}
+
+// ------------------------------------------------------------------
 // Run: construction (bcc_load, bcc.c:1482-1540)
 // ------------------------------------------------------------------
 Run::~Run()
@@ -297,6 +340,13 @@ void Run::load(const char* in_text)
   size_t o_l[8];
   for (int i = 0; i < 8; i++) o_l[i] = reserve(sizeof(int) * (size_t)(lcap[i] + 1));
   size_t o_zero = reserve(sizeof(int) * 4);
+  cmp_Wmax = pcols + (M + 31) / 32;
+  ctl_cmp_off[0] = reserve(sizeof(int) * 3 * (size_t)(pcols + 1));
+  ctl_cmp_off[1] = reserve(sizeof(int) * 32 * (size_t)cmp_Wmax);
+  ctl_cmp_off[2] = reserve(sizeof(uint32_t) * (size_t)cmp_Wmax);
+  rmv_Wmax = (((NA + 31) / 32) + 3) / 4 * 4;
+  ctl_cmp_off[3] = reserve(sizeof(int) * 32 * (size_t)rmv_Wmax);
+  ctl_cmp_off[4] = reserve(sizeof(uint32_t) * (size_t)rmv_Wmax);
   ctl_bytes = off;
   HIPCHK(hipHostMalloc((void**)&hctl, ctl_bytes, hipHostMallocDefault));
   memset(hctl, 0, ctl_bytes);
@@ -305,6 +355,12 @@ void Run::load(const char* in_text)
   for (int i = 0; i < 11; i++) { *dm[i] = (uint32_t*)(dctl + o_m[i]); ctl_mask_off[i] = o_m[i]; }
   for (int i = 0; i < 8; i++) { L[i]->d = (int*)(dctl + o_l[i]); ctl_list_off[i] = o_l[i]; }
   d_zero = (int*)(dctl + o_zero);
+  d_seg = (int*)(dctl + ctl_cmp_off[0]); d_src = (int*)(dctl + ctl_cmp_off[1]); d_cmask = (uint32_t*)(dctl + ctl_cmp_off[2]);
+  d_src_rmv = (int*)(dctl + ctl_cmp_off[3]); d_cmask_rmv = (uint32_t*)(dctl + ctl_cmp_off[4]);
+  At_rmvc = dev.alloc<uint32_t>((size_t)M * rmv_Wmax);
+  Tt_rmvc = dev.alloc<uint32_t>((size_t)T * rmv_Wmax);
+  An_c = dev.alloc<uint32_t>((size_t)NA * cmp_Wmax);
+  Zn_c = dev.alloc<uint32_t>((size_t)std::max(NZ, 1) * cmp_Wmax);
 
   // A_p per 1920-column block (bcc_load_A_p, bcc.c:1197-1313)
   hA_p.assign(pcols, 0.5);
@@ -404,6 +460,43 @@ void Run::push_ctl()
   };
   fill_list(ar, hAr_j, NA); fill_list(kl, hAc_j, M); fill_list(tl, hTc_j, T); fill_list(zr, hZr_j, NZ);
   fill_list(rtn, hAr_rtn, NA); fill_list(rmv, hAr_rmv, NA); fill_list(crt, hAc_rtn, M); fill_list(crm, hAc_rmv, M);
+  // compacted layout of the dropped columns: per non-empty 1920-block a segment of whole words
+  {
+    int* seg = (int*)(hctl + ctl_cmp_off[0]);
+    int* src = (int*)(hctl + ctl_cmp_off[1]);
+    uint32_t* cm = (uint32_t*)(hctl + ctl_cmp_off[2]);
+    int W = 0, ns = 0;
+    size_t i = 0;
+    while (i < crm.h.size()) {
+      int b = crm.h[i] / kPopBits;
+      size_t j = i;
+      while (j < crm.h.size() && crm.h[j] / kPopBits == b) j++;
+      int cnt = (int)(j - i), nw = (cnt + 31) / 32;
+      seg[3 * ns] = W; seg[3 * ns + 1] = nw; seg[3 * ns + 2] = b; ns++;
+      for (int q = 0; q < 32 * nw; q++) {
+        int c = q < cnt ? crm.h[i + q] : -1;
+        src[32 * W + q] = c;
+      }
+      for (int w = 0; w < nw; w++) {
+        uint32_t v = 0;
+        for (int q = 0; q < 32; q++) if (32 * w + q < cnt) v |= 1u << ((q >> 3) * 8 + (7 - (q & 7)));
+        cm[W + w] = v;
+      }
+      W += nw; i = j;
+    }
+    cmp_W = W; cmp_nseg = ns;
+    // removed rows, packed in whole 128-bit groups
+    int* srr = (int*)(hctl + ctl_cmp_off[3]);
+    uint32_t* cmr = (uint32_t*)(hctl + ctl_cmp_off[4]);
+    int nr = rmv.n(), Wr = (((nr + 31) / 32) + 3) / 4 * 4;
+    for (int q = 0; q < 32 * Wr; q++) srr[q] = q < nr ? rmv.h[q] : -1;
+    for (int w = 0; w < Wr; w++) {
+      uint32_t v = 0;
+      for (int q = 0; q < 32; q++) if (32 * w + q < nr) v |= 1u << ((q >> 3) * 8 + (7 - (q & 7)));
+      cmr[w] = v;
+    }
+    rmv_W = Wr;
+  }
   IList* L[8] = { &ar, &kl, &tl, &zr, &rtn, &rmv, &crt, &crm };
   size_t last = 0;
   for (int i = 0; i < 8; i++) {
@@ -471,9 +564,12 @@ void Run::ww_excerpt(int trm, const Side& s1, const Side& s2, const IList& kls, 
   long long ldc = std::max(n_total, 1);
   uint32_t* cz = (uint32_t*)g_cz.need(dev, sizeof(uint32_t) * (size_t)std::max(nw, 1) * ldc);
   size_t prb = pn.X->rb;
-  uint8_t* planes = (uint8_t*)g_planes.need(dev, (size_t)std::max(nw, 1) * c.B * prb);
+  uint8_t* planes = nullptr;
   xcalc(src, sn, sw, rl, cl, nn, nw, pn.rows->d, c.B, c.D_MLT, cz, ldc, scal, xwork, st);
-  cz_planes(cz, ldc, nw, n_total, pn.mask, c.B, planes, prb, st);
+  if (ww_kernel == 1) {
+    planes = (uint8_t*)g_planes.need(dev, (size_t)std::max(nw, 1) * c.B * prb);
+    cz_planes(cz, ldc, nw, n_total, pn.mask, c.B, planes, prb, st);
+  }
   ops += (long long)nn * nw * (c.B + 1) * 8;
   WwArgs a;
   a.P = pn.X->b(); a.klist = kls.d; a.qlist = kls.d; a.nk = kls.n(); a.pm = pn.mask;
@@ -482,9 +578,16 @@ void Run::ww_excerpt(int trm, const Side& s1, const Side& s2, const IList& kls, 
   a.cz = cz; a.ldc = ldc;
   a.Sc = pw.Cov->b(); a.Q = pw.X->b(); a.ajdk = ajdk; a.pcols = pcols; a.scal = scal;
   a.out = out; a.ldo = M;
-  size_t wb = ww_work_bytes(a.nk, a.nt, nw);
-  double* work = (double*)g_work.need(dev, wb);
-  ww_valu(a, work, wb, st);
+  Prof::Pending pp;
+  prof.begin("ww", 2.0 * tl.n() * kls.n() * nw * (double)nn * ((c.B + 7) / 8), st, pp);
+  if (ww_kernel == 1) {
+    size_t wb = ww_work_bytes(a.nk, a.nt, nw);
+    ww_valu(a, (double*)g_work.need(dev, wb), wb, st);
+  } else {
+    size_t wb = ww_mfma_work_bytes(a.nk, a.nt, nw, prb, c.B);
+    ww_mfma(a, g_work.need(dev, wb), wb, st);
+  }
+  prof.end(pp, st);
   ops += (long long)tl.n() * kls.n() * nw * (c.B + 1) * (long long)prb * 8;
 }
 
@@ -594,6 +697,7 @@ void Run::init_lf()
   st_single(); st_double(); st_flatten();
   HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipGetLastError());
+  prof.collect();
   inited = true;
   init_s = now_s() - t0;
 }
@@ -625,6 +729,7 @@ void Run::sumscores()
   if (na) HIPCHK(hipMemcpyAsync(hsv + M, dsv + M, sizeof(double) * na, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipGetLastError());
+  prof.collect();
   // xij: ascending sort of the column then the row scores
   int Irem = na > 0 ? 1 : 0;
   QC_sv.assign(hsv, hsv + nk); QC_perm.resize(nk);
@@ -676,15 +781,20 @@ void Run::lrup()
   // QR stages 0/1/3: lf_AnZt -= jn.vt, lf_AnAt -= jn.jt over the dropped columns
   an_ajdk(An.b(), rtn.d, rtn.n(), Ac_rmv, pcols, ajdk, mlf_rmv, jn_ajdk, NA, st);
   ops += 15LL * rtn.n() * rbm;
+  // the Grams over the dropped columns run on the column-compacted rows
+  const int W = cmp_W;
+  Bits Anc{ (const uint8_t*)An_c, NA, W * 32, (size_t)W * 4 }, Znc{ (const uint8_t*)Zn_c, NZ, W * 32, (size_t)W * 4 };
+  compact_cols(An.b(), nullptr, NA, d_src, W, An_c, st);
   if (has_Z) {
+    compact_cols(Zn.b(), nullptr, NZ, d_src, W, Zn_c, st);
     an_ajdk(Zn.b(), zr.d, zr.n(), Ac_rmv, pcols, ajdk, mlf_rmv, vn_ajdk, NZ, st);
     ops += 15LL * zr.n() * (long long)Zn.rb * 8;
-    gram(An.b(), rtn.d, rtn.n(), Zn.b(), zr.d, zr.n(), Ac_rmv, pcols, ajdk, mlf_rmv,
-         jn_ajdk + 6 * (size_t)NA, vn_ajdk + 6 * (size_t)NZ, 1, lf_AnZt, NA, 1, 1, 1, st);
+    gram(Anc, rtn.d, rtn.n(), Znc, zr.d, zr.n(), d_cmask, pcols, ajdk, mlf_rmv,
+         jn_ajdk + 6 * (size_t)NA, vn_ajdk + 6 * (size_t)NZ, 1, lf_AnZt, NA, 1, 1, 1, st, d_seg, cmp_nseg);
     ops += (long long)rtn.n() * zr.n() * rbm;
   }
-  gram(An.b(), rtn.d, rtn.n(), An.b(), rtn.d, rtn.n(), Ac_rmv, pcols, ajdk, mlf_rmv,
-       jn_ajdk + 6 * (size_t)NA, jn_ajdk + 6 * (size_t)NA, 1, lf_AnAt, NA, 1, 1, 1, st);
+  gram(Anc, rtn.d, rtn.n(), Anc, rtn.d, rtn.n(), d_cmask, pcols, ajdk, mlf_rmv,
+       jn_ajdk + 6 * (size_t)NA, jn_ajdk + 6 * (size_t)NA, 1, lf_AnAt, NA, 1, 1, 1, st, d_seg, cmp_nseg);
   ops += (long long)rtn.n() * rtn.n() * rbm;
   // QC a0
   an_ajdk(An.b(), rmv.d, rmv.n(), Ac_rtn, pcols, ajdk, mlf_rtn, kn_ajdk, NA, st);
@@ -704,8 +814,8 @@ void Run::lrup()
     hnvt = (double*)g_hnvt.need(dev, sizeof(double) * ((size_t)nrm * nz + 1));
     gram(An.b(), rmv.d, nrm, Zn.b(), zr.d, nz, Ac_rtn, pcols, ajdk, mlf_rtn,
          kn_ajdk + 6 * (size_t)NA, zn_ajdk + 6 * (size_t)NZ, 1, knzt, nrm, 0, 0, 0, st);
-    gram(An.b(), rmv.d, nrm, Zn.b(), zr.d, nz, Ac_rmv, pcols, ajdk, mlf_rmv,
-         hn_ajdk + 6 * (size_t)NA, vn_ajdk + 6 * (size_t)NZ, 1, hnvt, nrm, 0, 0, 0, st);
+    gram(Anc, rmv.d, nrm, Znc, zr.d, nz, d_cmask, pcols, ajdk, mlf_rmv,
+         hn_ajdk + 6 * (size_t)NA, vn_ajdk + 6 * (size_t)NZ, 1, hnvt, nrm, 0, 0, 0, st, d_seg, cmp_nseg);
     ops += 2LL * nrm * nz * rbm;
   }
   double* ankt = (double*)g_ankt.need(dev, sizeof(double) * ((size_t)nrt * nrm + 1));
@@ -740,6 +850,32 @@ void Run::lrup()
     gram(Zt.b(), crm.d, crm.n(), St.b(), tl.d, tl.n(), Zr_j, sZ, nullptr, mlf_nz, nullptr, nullptr, 0, vtsn, M, 1, 1, 0, st);
     ops += (long long)(crt.n() + crm.n()) * tl.n() * (long long)Zt.rb * 8;
   }
+  double *vtszn = mt[10], *jttan = mt[11], *htrkn = mt[12], *hhtrkn = mt[13];
+  if (ww_kernel == 0) {
+    // QC b1 + b2 fused on the matrix cores; the removed rows are packed first
+    const int Wr = rmv_W;
+    compact_cols(At.b(), nullptr, M, d_src_rmv, Wr, At_rmvc, st);
+    compact_cols(Tt.b(), nullptr, T, d_src_rmv, Wr, Tt_rmvc, st);
+    AttynArgs x;
+    const uint8_t* Arc = (const uint8_t*)At_rmvc;
+    const uint8_t* Trc = (const uint8_t*)Tt_rmvc;
+    x.term[0] = AttynTerm{ At.p, At.rb, Tt.p, Tt.rb, At.p, At.rb, (const uint8_t*)Ar_rtn, (long long)At.rb * 8, nrt,
+                           attn, jttn, M };
+    x.term[1] = AttynTerm{ Arc, (size_t)Wr * 4, Trc, (size_t)Wr * 4, Arc, (size_t)Wr * 4, (const uint8_t*)d_cmask_rmv,
+                           (long long)Wr * 32, nrm, ktrn, htrn, M };
+    x.term[2] = AttynTerm{ Zt.p, Zt.rb, St.p, St.rb, Zt.p, Zt.rb, (const uint8_t*)Zr_j, (long long)Zt.rb * 8, nz,
+                           ztsn, vtsn, M };
+    x.nterm = has_Z ? 3 : 2;
+    x.klist = crt.d; x.nk = crt.n(); x.ylist = crm.d; x.ny = crm.n(); x.tlist = tl.d; x.nt = tl.n();
+    x.ajdk = ajdk; x.pcols = pcols;
+    x.vtszn = vtszn; x.jttan = jttan; x.htrkn = htrkn; x.hhtrkn = hhtrkn; x.ldo = M;
+    size_t wb = attyn_mfma_work_bytes(x.ny, x.nt);
+    Prof::Pending pa;
+    prof.begin("attyn", 2.0 * crt.n() * crm.n() * tl.n() * ((double)nrt + nrm + (has_Z ? nz : 0)), st, pa);
+    attyn_mfma(x, g_attyn.need(dev, wb), wb, st);
+    prof.end(pa, st);
+    ops += (long long)tl.n() * crt.n() * crm.n() * (rbn * 2 + (has_Z ? (long long)Zt.rb * 8 : 0));
+  } else {
   // QC b1: compact [(y*nt + t)*nk + k]
   size_t n3 = (size_t)crt.n() * crm.n() * tl.n() + 1;
   double* ztsvn = has_Z ? (double*)g_ztsvn.need(dev, sizeof(double) * n3) : nullptr;
@@ -753,11 +889,11 @@ void Run::lrup()
         ktrn, htrn, M, ajdk, pcols, ktrhn, st);
   ops += (long long)tl.n() * crt.n() * crm.n() * (rbn * 2 + (has_Z ? (long long)Zt.rb * 8 : 0));
   // QC b2
-  double *vtszn = mt[10], *jttan = mt[11], *htrkn = mt[12], *hhtrkn = mt[13];
   if (has_Z) attyn_wtszn(attjn, ztsvn, crt.d, crt.n(), crm.d, crm.n(), tl.d, tl.n(), ajdk, pcols, vtszn, M, st);
   attyn_wtszn(attjn, attjn, crt.d, crt.n(), crm.d, crm.n(), tl.d, tl.n(), ajdk, pcols, jttan, M, st);
   attyn_wtszn(attjn, ktrhn, crt.d, crt.n(), crm.d, crm.n(), tl.d, tl.n(), ajdk, pcols, htrkn, M, st);
   attyn_wtszn(ktrhn, ktrhn, crt.d, crt.n(), crm.d, crm.n(), tl.d, tl.n(), ajdk, pcols, hhtrkn, M, st);
+  }
   // QC stage c
   StageC s;
   s.M = M; s.T = T; s.has_Z = has_Z; s.Ac_rtn = Ac_rtn; s.Ac_b = Ac_b; s.Tc_j = Tc_j;
@@ -798,7 +934,23 @@ void Run::run(int max_iter)
   }
   HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipGetLastError());
+  prof.collect();
   loop_s = now_s() - t0;
+}
+
+// back to the state right after loading: all rows and columns active, no outputs
+void Run::reset()
+{
+  HIPCHK(hipStreamSynchronize(st));
+  prof.collect();
+  hAr_j = hAr_b; hZr_j = hZr_b; hAc_j = hAc_b;
+  std::fill(hAr_rtn.begin(), hAr_rtn.end(), 0); std::fill(hAr_rmv.begin(), hAr_rmv.end(), 0);
+  std::fill(hAc_rtn.begin(), hAc_rtn.end(), 0); std::fill(hAc_rmv.begin(), hAc_rmv.end(), 0);
+  push_ctl();
+  HIPCHK(hipStreamSynchronize(st));
+  trace.clear(); xdrop_a.clear(); xdrop_b.clear();
+  n_iter = 0; inited = false; ops = 0; init_s = 0; loop_s = 0;
+  for (auto& k : prof.stats) { k.launches = 0; k.ms = 0; k.work = 0; }
 }
 
 void Run::write_outputs(const std::string& dir) const

@@ -37,8 +37,25 @@ struct IList { std::vector<int> h; int* d = nullptr; int n() const { return (int
 // one patient group of an At_T_YnWt_S_Zn term: SNP-major bits, active-row mask and list, covariates
 struct Side { const DMat* X; const uint32_t* mask; const IList* rows; const DMat* Cov; };
 
+// HIP-event timing of named kernel groups on the run's stream
+struct Prof {
+  struct Pending { int id; hipEvent_t a, b; };
+  struct Stat { std::string name; long long launches = 0; double ms = 0, work = 0; };
+  bool on = false;
+  std::vector<Stat> stats;
+  std::vector<Pending> pend;
+  std::vector<hipEvent_t> pool;
+  ~Prof();
+  int id(const char* name);
+  hipEvent_t ev();
+  void begin(const char* name, double work, hipStream_t st, Pending& p);
+  void end(Pending& p, hipStream_t st);
+  void collect();                     // after a stream sync
+};
+
 struct Run {
   Cfg c;
+  Prof prof;
   hipStream_t st = nullptr;
   Dev dev;
   int NA = 0, NZ = 0, M = 0, T = 0, pcols = 0, has_Z = 0, Arb = 0, Zrb = 0;
@@ -54,7 +71,16 @@ struct Run {
   int trm_AnZt = 0, trm_knzt = 0, trm_hnvt = 0;
   // control block
   uint8_t* hctl = nullptr; uint8_t* dctl = nullptr; size_t ctl_bytes = 0;
-  size_t ctl_mask_off[11] = {}, ctl_list_off[8] = {};
+  size_t ctl_mask_off[11] = {}, ctl_list_off[8] = {}, ctl_cmp_off[5] = {};
+  // column compaction of the dropped columns (crm), block by block
+  int cmp_W = 0, cmp_nseg = 0, cmp_Wmax = 0;
+  int *d_seg = nullptr, *d_src = nullptr; uint32_t* d_cmask = nullptr;
+  uint32_t *An_c = nullptr, *Zn_c = nullptr;
+  // compaction of the removed rows (rmv) for the patient dimension of At / Tt
+  int rmv_W = 0, rmv_Wmax = 0;
+  int* d_src_rmv = nullptr; uint32_t* d_cmask_rmv = nullptr;
+  uint32_t *At_rmvc = nullptr, *Tt_rmvc = nullptr;
+  Grow g_attyn;
   // device arrays; (row, covariate) arrays are [t*rows + row]
   double *mlf = nullptr;
   double *lf_An_ajdk = nullptr, *lf_Zn_ajdk = nullptr;              // [jk*rows + r]
@@ -74,6 +100,7 @@ struct Run {
   std::vector<double> QC_sv, QR_sv; std::vector<int> QC_perm, QR_perm;
   std::vector<double> trace; std::vector<int> xdrop_a, xdrop_b;
   int n_iter = 0; bool inited = false;
+  int ww_kernel = 0;                        // 0: int8 MFMA, 1: VALU bit-plane popcount
   long long ops = 0;
   double load_s = 0, init_s = 0, loop_s = 0;
   std::string err;
@@ -91,6 +118,7 @@ struct Run {
   bool active() const;
   void iterate(int* done);
   void run(int max_iter);
+  void reset();
   void write_outputs(const std::string& dir) const;
   std::string default_out_dir() const;
   long debug_get(const std::string& name, std::vector<double>& out);

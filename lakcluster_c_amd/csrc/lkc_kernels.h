@@ -38,7 +38,12 @@ void an_ajdk(const Bits& X, const int* rows, int nrows, const uint32_t* mask, in
 void gram(const Bits& A, const int* alist, int na, const Bits& Z, const int* zlist, int nz,
           const uint32_t* mask, int pcols, const double* ajdk, const double* base16,
           const double* xa, const double* xz, int centred,
-          double* out, long long ldo, int a_orig, int z_orig, int op_sub, hipStream_t st);
+          double* out, long long ldo, int a_orig, int z_orig, int op_sub, hipStream_t st,
+          const int* seg = nullptr, int nseg = 0);
+// Column compaction for grams over a sparse column set: seg (3 ints per non-empty
+// 1920-block: word offset, word count, block index) describes the compacted rows.
+// out row i, word w = bits of X row (rows ? rows[i] : i) at columns src[32w .. 32w+31] (-1: none).
+void compact_cols(const Bits& X, const int* rows, int nrows, const int* src, int W, uint32_t* out, hipStream_t st);
 
 // xcalc.  Values v(i,j) = src[ridx(i)*sn + cidx(j)*sw] for i<nn, j<nw, with
 // ridx(i) = rlist ? rlist[i] : i (same for cidx).  scal receives
@@ -67,6 +72,10 @@ struct WwArgs {
 size_t ww_work_bytes(int nk, int nt, int nw);
 void ww_valu(const WwArgs& a, double* work, size_t work_bytes, hipStream_t st);
 
+// int8 matrix-core formulation (cz split into signed bytes); planes are not used
+size_t ww_mfma_work_bytes(int nk, int nt, int nw, size_t prb, int B);
+void ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t st);
+
 // An_ZtSn_ww: out[slist[s]*ldo + alist[i]] = (base + Σ 2^nb pm0(A_a, plane(s,nb)))/mlt
 void an_ztsn(const Bits& A, const int* alist, int na, const uint32_t* amask,
              const uint8_t* planes, size_t prb, const uint32_t* row0, int ns, const int* slist, int B,
@@ -84,6 +93,28 @@ void attyn(const Bits& At, const int* alist, int nk, const uint32_t* amask, int 
 // AtTYn____WtSZn_vv: out[tlist[t]*ldo + klist[k]] = Σ_y (L1*D_y)*L2 (L1/L2 in attyn layout)
 void attyn_wtszn(const double* L1, const double* L2, const int* klist, int nk, const int* ylist, int ny,
                  const int* tlist, int nt, const double* ajdk, int pcols, double* out, long long ldo, hipStream_t st);
+
+// AtTYn_vv for the three (A, T, Y, mask) triples of the QC stage b1 fused with the
+// four AtTYn____WtSZn_vv sums of stage b2, on the int8 matrix cores.
+//   term 0 (attjn): A = At, T = Tt, Y = At, mask = retained rows
+//   term 1 (ktrhn): the same over the removed rows
+//   term 2 (ztsvn): A = Zt, T = St, Y = Zt, mask = active Z rows (absent without Z)
+// A and Y rows are indexed by the original column index; mask has ldn bits (a
+// multiple of 128) and mcount set bits; hAy/haY are lf_AtTn-like [t*ldh + row].
+// Outputs [t*ldo + k]: vtszn (term0 x term2), jttan (0 x 0), htrkn (0 x 1), hhtrkn (1 x 1).
+struct AttynTerm {
+  const uint8_t* A; size_t rbA; const uint8_t* T; size_t rbT; const uint8_t* Y; size_t rbY;
+  const uint8_t* mask; long long ldn; int mcount;
+  const double* hAy; const double* haY; long long ldh;
+};
+struct AttynArgs {
+  AttynTerm term[3]; int nterm;
+  const int* klist; int nk; const int* ylist; int ny; const int* tlist; int nt;
+  const double* ajdk; int pcols;
+  double* vtszn; double* jttan; double* htrkn; double* hhtrkn; long long ldo;
+};
+size_t attyn_mfma_work_bytes(int ny, int nt);
+void attyn_mfma(const AttynArgs& a, void* work, size_t work_bytes, hipStream_t st);
 
 // out[i] = pop(X[rows[i]] & mask) over nwords 32-bit words (rows==NULL: row i)
 void popc_rows(const Bits& X, const int* rows, int nrows, const uint32_t* mask, int nwords, int* out, hipStream_t st);

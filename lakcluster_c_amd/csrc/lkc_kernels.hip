@@ -108,9 +108,13 @@ void an_ajdk(const Bits& X, const int* rows, int nrows, const uint32_t* mask, in
 // ------------------------------------------------------------------
 #define GT 64
 #define GLD 61   // padded LDS row stride in words
+// Segments: the full layout walks the pcols 1920-bit blocks (60 words each); a
+// column-compacted layout (only some columns kept, block by block) passes for each
+// non-empty block its word offset, word count (<= 60) and block index.
 __global__ __launch_bounds__(256) void k_gram(const uint8_t* A, size_t rbA, const int* alist, int na,
                                               const uint8_t* Z, size_t rbZ, const int* zlist, int nz,
-                                              const uint32_t* mask, int pcols, const double* ajdk, const double* base,
+                                              const uint32_t* mask, int pcols, const int* seg, int nseg,
+                                              const double* ajdk, const double* base,
                                               const double* xa, const double* xz, int centred,
                                               double* out, long long ldo, int a_orig, int z_orig, int op_sub)
 {
@@ -122,20 +126,24 @@ __global__ __launch_bounds__(256) void k_gram(const uint8_t* A, size_t rbA, cons
   double dsum[4][4];
   long long isum[4][4];
   for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) { dsum[i][j] = 0; isum[i][j] = 0; }
-  for (int b = 0; b < pcols; b++) {
+  const int ns = seg ? nseg : pcols;
+  for (int q = 0; q < ns; q++) {
+    const int woff = seg ? seg[3 * q] : q * kPopWords;
+    const int wlen = seg ? seg[3 * q + 1] : kPopWords;
+    const int b = seg ? seg[3 * q + 2] : q;
     __syncthreads();
-    for (int e = threadIdx.x; e < GT * kPopWords; e += 256) {
-      int r = e / kPopWords, w = e % kPopWords;
-      uint32_t m = mask[(size_t)b * kPopWords + w];
+    for (int e = threadIdx.x; e < GT * wlen; e += 256) {
+      int r = e / wlen, w = e % wlen;
+      uint32_t m = mask[woff + w];
       uint32_t va = 0, vz = 0;
-      if (a0 + r < na) va = ((const uint32_t*)(A + (size_t)alist[a0 + r] * rbA))[(size_t)b * kPopWords + w] & m;
-      if (z0 + r < nz) vz = ((const uint32_t*)(Z + (size_t)zlist[z0 + r] * rbZ))[(size_t)b * kPopWords + w] & m;
+      if (a0 + r < na) va = ((const uint32_t*)(A + (size_t)alist[a0 + r] * rbA))[woff + w] & m;
+      if (z0 + r < nz) vz = ((const uint32_t*)(Z + (size_t)zlist[z0 + r] * rbZ))[woff + w] & m;
       As[r * GLD + w] = va; Zs[r * GLD + w] = vz;
     }
     __syncthreads();
     int acc[4][4];
     for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) acc[i][j] = 0;
-    for (int w = 0; w < kPopWords; w++) {
+    for (int w = 0; w < wlen; w++) {
       uint32_t av[4], zv[4];
       for (int i = 0; i < 4; i++) av[i] = As[(tx * 4 + i) * GLD + w];
       for (int j = 0; j < 4; j++) zv[j] = Zs[(ty * 4 + j) * GLD + w];
@@ -167,12 +175,36 @@ __global__ __launch_bounds__(256) void k_gram(const uint8_t* A, size_t rbA, cons
 void gram(const Bits& A, const int* alist, int na, const Bits& Z, const int* zlist, int nz,
           const uint32_t* mask, int pcols, const double* ajdk, const double* base16,
           const double* xa, const double* xz, int centred,
-          double* out, long long ldo, int a_orig, int z_orig, int op_sub, hipStream_t st)
+          double* out, long long ldo, int a_orig, int z_orig, int op_sub, hipStream_t st,
+          const int* seg, int nseg)
 {
   if (na <= 0 || nz <= 0) return;
   hipLaunchKernelGGL(k_gram, dim3(cdiv(na, GT), cdiv(nz, GT)), dim3(256), 0, st,
-                     A.p, A.rb, alist, na, Z.p, Z.rb, zlist, nz, mask, pcols, ajdk, base16, xa, xz, centred,
+                     A.p, A.rb, alist, na, Z.p, Z.rb, zlist, nz, mask, pcols, seg, nseg, ajdk, base16, xa, xz, centred,
                      out, ldo, a_orig, z_orig, op_sub);
+}
+
+// column compaction: out word w of row r gathers bits src[32w + q] (q = 0..31, MSB-first
+// within each byte; -1 = padding, 0); rows rows[i] (or i) of X -> row i of out.
+__global__ void k_compact(const uint8_t* X, size_t rb, const int* rows, int nrows, const int* src, int W,
+                          uint32_t* out)
+{
+  long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)nrows * W) return;
+  int i = (int)(e / W), w = (int)(e % W);
+  const uint8_t* xr = X + (size_t)(rows ? rows[i] : i) * rb;
+  uint32_t v = 0;
+  for (int q = 0; q < 32; q++) {
+    int c = src[32 * w + q];
+    if (c >= 0 && bit_at(xr, c)) v |= 1u << ((q >> 3) * 8 + (7 - (q & 7)));
+  }
+  out[(size_t)i * W + w] = v;
+}
+void compact_cols(const Bits& X, const int* rows, int nrows, const int* src, int W, uint32_t* out, hipStream_t st)
+{
+  long long n = (long long)nrows * W;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_compact, dim3(cdiv(n, 256)), dim3(256), 0, st, X.p, X.rb, rows, nrows, src, W, out);
 }
 
 // ------------------------------------------------------------------
@@ -453,29 +485,66 @@ void an_ztsn(const Bits& A, const int* alist, int na, const uint32_t* amask,
 // ------------------------------------------------------------------
 // AnZt_S_WnYt_vv (AnZt_S_WnYt_vv.c:49-67) with X==Y; X[z*ldx + a] (a contiguous)
 // ------------------------------------------------------------------
-__global__ void k_qr(const double* X, long long ldx, const int* alist, int na, const int* zlist, int nz,
-                     const uint8_t* Sc, size_t rbS, const int* tlist, int nt, double* out, long long ldo)
+// One block per 64 rows a: all 256 threads stream a 128(z) x 64(a) tile of X into
+// LDS while wave 0 adds the previous tile into its 64 ordered per-row sums.
+#define QR_ZC 128
+__global__ __launch_bounds__(256) void k_qr(const double* X, long long ldx, const int* alist, int na, const int* zlist,
+                                            int nz, const uint8_t* Sc, size_t rbS, const int* tlist, int nt,
+                                            double* out, long long ldo)
 {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  bool live = i < na;
-  int a = live ? alist[i] : 0;
+  __shared__ double tile[2][QR_ZC][64];
+  __shared__ signed char sgn[2][kMaxT][QR_ZC];
+  const int tid = threadIdx.x, i0 = blockIdx.x * 64;
+  const int ai = tid & 63;
+  const bool alive = i0 + ai < na;
+  const long long acol = alive ? alist[i0 + ai] : 0;
+  const int nch = (nz + QR_ZC - 1) / QR_ZC;
+  double reg[QR_ZC / 4];
+  auto load = [&](int c) {
+    const int z0 = c * QR_ZC;
+#pragma unroll
+    for (int j = 0; j < QR_ZC / 4; j++) {
+      const int zi = (tid >> 6) + 4 * j;
+      reg[j] = (alive && z0 + zi < nz) ? X[(long long)zlist[z0 + zi] * ldx + acol] : 0.0;
+    }
+  };
+  auto store = [&](int c, int b) {
+    const int z0 = c * QR_ZC;
+#pragma unroll
+    for (int j = 0; j < QR_ZC / 4; j++) tile[b][(tid >> 6) + 4 * j][ai] = reg[j];
+    for (int e = tid; e < nt * QR_ZC; e += 256) {
+      const int t = e / QR_ZC, zi = e % QR_ZC;
+      sgn[b][t][zi] = (z0 + zi < nz) ? (signed char)pm_at(Sc + (size_t)tlist[t] * rbS, zlist[z0 + zi]) : 0;
+    }
+  };
   double acc[kMaxT];
   for (int t = 0; t < kMaxT; t++) acc[t] = 0;
-  for (int zz = 0; zz < nz; zz++) {
-    int z = zlist[zz];
-    double x = live ? X[(long long)z * ldx + a] : 0.0;
-    for (int t = 0; t < nt; t++) {
-      int vS = pm_at(Sc + (size_t)tlist[t] * rbS, z);
-      acc[t] += x * vS * x;
+  if (nch > 0) { load(0); store(0, 0); }
+  __syncthreads();
+  for (int c = 0; c < nch; c++) {
+    const int b = c & 1;
+    if (c + 1 < nch) load(c + 1);
+    if (tid < 64) {
+      const int zn = min(QR_ZC, nz - c * QR_ZC);
+#pragma unroll 8
+      for (int zi = 0; zi < zn; zi++) {
+        const double x = tile[b][zi][ai];
+        for (int t = 0; t < nt; t++) {
+          const int vS = sgn[b][t][zi];
+          acc[t] += x * vS * x;
+        }
+      }
     }
+    if (c + 1 < nch) store(c + 1, b ^ 1);
+    __syncthreads();
   }
-  if (live) for (int t = 0; t < nt; t++) out[(long long)tlist[t] * ldo + a] = acc[t];
+  if (tid < 64 && alive) for (int t = 0; t < nt; t++) out[(long long)tlist[t] * ldo + acol] = acc[t];
 }
 void qr(const double* X, long long ldx, const int* alist, int na, const int* zlist, int nz,
         const Bits& Sc, const int* tlist, int nt, double* out, long long ldo, hipStream_t st)
 {
   if (na <= 0) return;
-  hipLaunchKernelGGL(k_qr, dim3(cdiv(na, 256)), dim3(256), 0, st, X, ldx, alist, na, zlist, nz, Sc.p, Sc.rb, tlist, nt, out, ldo);
+  hipLaunchKernelGGL(k_qr, dim3(cdiv(na, 64)), dim3(256), 0, st, X, ldx, alist, na, zlist, nz, Sc.p, Sc.rb, tlist, nt, out, ldo);
 }
 
 // ------------------------------------------------------------------

@@ -186,6 +186,8 @@ def make_dataset(root: str, name: str, nA: int, ncols: int, nZ: int = 0, T: int 
 CONFIGS = {
     "case_only_4096x16384": dict(nA=4096, ncols=16384, nZ=0, T=1, pmin=0.05, pmax=0.5,
                                  planted=(0.55,), gamma=0.05, B=32),
+    "case_control_4096x16384": dict(nA=4096, ncols=16384, nZ=4096, T=3, pmin=0.05, pmax=0.5,
+                                    planted=(0.55,), gamma=0.02, B=32),
     "case_control_16384x65536": dict(nA=16384, ncols=65536, nZ=16384, T=3, pmin=0.05, pmax=0.5,
                                      planted=(0.55,), gamma=0.02, B=32),
 }

@@ -66,6 +66,10 @@ def lib():
         "lkc_bcc_shape": (C.c_int, [vp, ip]),
         "lkc_bcc_free": (None, [vp]),
         "lkc_bcc_debug_get": (C.c_long, [vp, cp, dp, C.c_long]),
+        "lkc_bcc_reset": (C.c_int, [vp]),
+        "lkc_bcc_set_profiling": (C.c_int, [vp, C.c_int]),
+        "lkc_bcc_set_ww_kernel": (C.c_int, [vp, C.c_int]),
+        "lkc_bcc_kernel_stats": (C.c_int, [vp, cp, C.POINTER(C.c_longlong), dp, dp]),
         "lkc_An_ajdk_v": (C.c_int, [u8p, C.c_size_t, C.c_int, ip, C.c_int, u8p, C.c_int, dp, dp]),
         "lkc_AnZt_vv": (C.c_int, [u8p, C.c_size_t, C.c_int, ip, C.c_int, u8p, C.c_size_t, C.c_int, ip, C.c_int,
                                   u8p, C.c_int, dp, dp, dp, dp]),
@@ -155,6 +159,22 @@ class Bcc:
         t = np.zeros(3, dtype=np.float64)
         _check(lib().lkc_bcc_times(self._h, _ptr(t, C.c_double)), self._h)
         return float(t[0]), float(t[1]), float(t[2])
+
+    def reset(self) -> None:
+        _check(lib().lkc_bcc_reset(self._h), self._h)
+
+    def set_profiling(self, on: bool = True) -> None:
+        _check(lib().lkc_bcc_set_profiling(self._h, 1 if on else 0), self._h)
+
+    def set_ww_kernel(self, which: str) -> None:
+        """'mfma' (int8 matrix cores, default) or 'valu' (bit-plane popcount)."""
+        _check(lib().lkc_bcc_set_ww_kernel(self._h, {"mfma": 0, "valu": 1}[which]), self._h)
+
+    def kernel_stats(self, name: str) -> Tuple[int, float, float]:
+        """(launches, device ms, algorithmic work) of a timed kernel group."""
+        n, ms, w = C.c_longlong(0), C.c_double(0), C.c_double(0)
+        _check(lib().lkc_bcc_kernel_stats(self._h, name.encode(), C.byref(n), C.byref(ms), C.byref(w)), self._h)
+        return int(n.value), float(ms.value), float(w.value)
 
     def state(self, name: str) -> np.ndarray:
         """A named score array of the device state (row-major, covariate last)."""
