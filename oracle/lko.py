@@ -38,6 +38,9 @@ def lib():
         "lko_an_ajdk_v": (None, [u8p, C.c_size_t, ip, C.c_int, u8p, C.c_int, dp, dp]),
         "lko_anzt_vv": (None, [u8p, C.c_size_t, ip, C.c_int, u8p, C.c_size_t, ip, C.c_int, u8p, C.c_int, dp, dp, dp, dp]),
         "lko_run_text": (vp, [cp, cp, C.c_int, C.c_int]),
+        "lko_run_text_recompute": (vp, [cp, C.c_int]),
+        "lko_xcalc": (None, [dp, ip, C.c_int, C.c_int, C.c_int, C.c_int, C.c_longlong, vp, C.POINTER(C.c_ulonglong)]),
+        "lko_planes_free": (None, [vp]),
         "lko_status": (C.c_int, [vp, cp, C.c_int]),
         "lko_n_trace": (C.c_int, [vp]),
         "lko_n_xdrop": (C.c_int, [vp]),
@@ -71,9 +74,14 @@ class OracleRun:
         self.trace, self.xdrop_a, self.xdrop_b, self.ops, self.times, self.n_iter = trace, xa, xb, ops, times, n_iter
 
 
-def run_text(in_text: str, max_iter: int = -1, out_dir: Optional[str] = None, write_files: bool = False) -> OracleRun:
+def run_text(in_text: str, max_iter: int = -1, out_dir: Optional[str] = None, write_files: bool = False,
+             recompute: bool = False) -> OracleRun:
+    """Whole elimination; recompute=True replaces the low-rank update by full recomputation."""
     L = lib()
-    h = L.lko_run_text(in_text.encode(), out_dir.encode() if out_dir else None, max_iter, 1 if write_files else 0)
+    if recompute:
+        h = L.lko_run_text_recompute(in_text.encode(), max_iter)
+    else:
+        h = L.lko_run_text(in_text.encode(), out_dir.encode() if out_dir else None, max_iter, 1 if write_files else 0)
     err = C.create_string_buffer(256)
     st = L.lko_status(h, err, 256)
     if st != 0:
@@ -100,6 +108,28 @@ def calc_ajdk(p: np.ndarray) -> np.ndarray:
     p = np.ascontiguousarray(p, dtype=np.float64)
     out = np.zeros(15 * p.size)
     lib().lko_calc_ajdk(_p(p, C.c_double), p.size, _p(out, C.c_double))
+    return out
+
+
+class _Planes(C.Structure):
+    _fields_ = [("B", C.c_int), ("n_total", C.c_int), ("rb", C.c_size_t), ("nw", C.c_int),
+                ("row0", C.POINTER(C.c_uint8)), ("planes", C.POINTER(C.c_uint8)),
+                ("min_d", C.c_double), ("max_d", C.c_double), ("mlt_d", C.c_double)]
+
+
+def xcalc(L: np.ndarray, nlist, n_total: int, B: int, D_MLT: Optional[int] = None):
+    """get_xcalc of L (len(nlist) x nw): returns cz (nw x nn), min, mlt, planes (nw x B x rb bytes)."""
+    L = np.ascontiguousarray(L, np.float64)
+    nlist = np.ascontiguousarray(nlist, np.int32)
+    nn, nw = L.shape
+    D = (1 << B) if D_MLT is None else D_MLT
+    P = _Planes()
+    cz = np.zeros(max(nn * nw, 1), np.uint64)
+    lib().lko_xcalc(_p(L, C.c_double), _p(nlist, C.c_int), nn, n_total, nw, B, D, C.byref(P), _p(cz, C.c_ulonglong))
+    rb = P.rb
+    planes = np.ctypeslib.as_array(P.planes, shape=(max(nw * B * rb, 1),)).copy()[: nw * B * rb].reshape(nw, B, rb)
+    out = (cz[: nn * nw].reshape(nw, nn), P.min_d, P.mlt_d, planes)
+    lib().lko_planes_free(C.byref(P))
     return out
 
 

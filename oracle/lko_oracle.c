@@ -1343,7 +1343,7 @@ static int write_outputs(const lko_state *S, const char *dir)
 /* Runs the whole elimination described by an .in text.  max_iter<0 runs to the end.
    out_dir (optional) overrides GLOBAL_DIR_NAME; outputs are written only when
    write_files is nonzero. */
-void *lko_run_text(const char *in_text, const char *out_dir, int max_iter, int write_files)
+static void *run_impl(const char *in_text, const char *out_dir, int max_iter, int write_files, int recompute)
 {
   lko_result *R = (lko_result *)xcalloc(1, sizeof(lko_result));
   lko_state *S = (lko_state *)xcalloc(1, sizeof(lko_state));
@@ -1367,7 +1367,12 @@ void *lko_run_text(const char *in_text, const char *out_dir, int max_iter, int w
     st_nrm_ifT_cmb(S);
     st_xij(S, &Irem);
     st_mxA(S, iter, Irem, &rdrop, &cdrop);
-    st_lrup(S);
+    if (!recompute) st_lrup(S);
+    else {
+      /* no low-rank update: drop, then every loop-subscore from scratch on the new masks */
+      memcpy(S->Ar_j, S->Ar_rtn, lko_bsize(S->NA)); memcpy(S->Ac_j, S->Ac_rtn, lko_bsize(S->M));
+      st_ajdk(S); st_ZtSn(S); st_YnWt(S); st_At_T(S);
+    }
     st_rescore_tail(S);
     iter++;
   }
@@ -1379,6 +1384,15 @@ void *lko_run_text(const char *in_text, const char *out_dir, int max_iter, int w
   R->load_s = t1 - t0; R->init_s = t2 - t1; R->loop_s = t3 - t2;
   st_free(S); free(S);
   return R;
+}
+
+void *lko_run_text(const char *in_text, const char *out_dir, int max_iter, int write_files)
+{
+  return run_impl(in_text, out_dir, max_iter, write_files, 0);
+}
+void *lko_run_text_recompute(const char *in_text, int max_iter)
+{
+  return run_impl(in_text, NULL, max_iter, 0, 1);
 }
 
 int lko_status(void *h, char *err, int n) { lko_result *R = (lko_result *)h; if (err && n > 0) snprintf(err, n, "%s", R->err); return R->status; }

@@ -63,6 +63,9 @@ void lko_planes_free(lko_planes *p);
 
 /* whole-run driver (lakcluster_driver.c:14-251); returns an opaque result handle */
 void *lko_run_text(const char *in_text, const char *out_dir, int max_iter, int write_files);
+/* the same elimination with the low-rank update replaced by recomputing every
+   loop-subscore from scratch after each drop (a check of the update) */
+void *lko_run_text_recompute(const char *in_text, int max_iter);
 int lko_status(void *h, char *err, int n);
 int lko_n_trace(void *h);
 int lko_n_xdrop(void *h);
