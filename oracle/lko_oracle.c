@@ -1446,7 +1446,16 @@ long lko_state_get(void *h, const char *name, double *out, long n)
     { "et_Tn", S->et_Tn, T }, { "et_Sn", S->et_Sn, T }, { "et_An", S->et_An, M }, { "et_An_a1d1", &S->et_An_a1d1, 1 },
     { "T_AnAt_AnAt", S->T_AnAt_AnAt, T * NA },
     { "QR_ZSZ", S->QR_ZSZ, NA * T }, { "QR_TAT", S->QR_TAT, NA * T }, { "QC_ZSZ", S->QC_ZSZ, M * T }, { "QC_TAT", S->QC_TAT, M * T },
+    { "ajdk", S->ajdk, (size_t)S->pcols * LKO_AJDK }, { "A_p", S->A_p, (size_t)S->pcols },
   };
+  struct { const char *nm; const uint8_t *m; int n; } mt[] = {
+    { "mask_Ar_j", S->Ar_j, S->NA }, { "mask_Zr_j", S->Zr_j, S->NZ }, { "mask_Ac_j", S->Ac_j, S->M },
+  };
+  for (size_t i = 0; i < sizeof mt / sizeof mt[0]; i++) {
+    if (strcmp(mt[i].nm, name)) continue;
+    if (out) for (long j = 0; j < MINI((long)mt[i].n, n); j++) out[j] = bget(mt[i].m, j);
+    return mt[i].n;
+  }
   for (size_t i = 0; i < sizeof tab / sizeof tab[0]; i++) {
     if (strcmp(tab[i].nm, name)) continue;
     if (out) memcpy(out, tab[i].p, sizeof(double) * (size_t)MINI((long)tab[i].len, n));

@@ -218,3 +218,73 @@ def test_run_invariants(oracle, datasets):
             assert r.trace[s + 1, 1] == r.trace[s, 1] - min(rd, r.trace[s, 1])
             assert r.trace[s + 1, 2] == r.trace[s, 2] - min(cd, r.trace[s, 2])
         assert np.all(np.isfinite(r.trace[:, 3:5]))
+
+
+def _loop_count_scores(A, Z, Tm, Sm, tab, ar, zr, kl, nt):
+    """The reference's brute-force loop counts (get_QR_AnZtSWnYt_uu, bcc_flattenloop.c:1033-1131;
+    get_QC_AtTYnWtSZn_uu, bcc_flattenloop.c:1226-1328) over the active rows ar, zr and columns kl:
+      QR_TAT[a,t] = sum_{a'!=a} sum_{j!=j'} Ah(a,j)D_j Ah(a',j) T(a',t) Ah(a',j')D_j' Ah(a,j')
+      QR_ZSZ[a,t] = sum_z      sum_{j!=j'} Ah(a,j)D_j Zh(z,j)  S(z,t)  Zh(z,j')D_j' Ah(a,j')
+      QC_TAT[k,t] = D_k sum_{a!=a'} Ah(a,k)T(a,t) Ah(a',k)T(a',t) sum_{y!=k} Ah(a,y)D_y Ah(a',y)
+      QC_ZSZ[k,t] = D_k sum_{a,z}   Ah(a,k)T(a,t) Zh(z,k)S(z,t)   sum_{y!=k} Ah(a,y)D_y Zh(z,y)
+    with Ah = A(+-1) - alpha, Zh = Z(+-1) - alpha (alpha, D per 1920-column block)."""
+    b = np.arange(A.shape[1]) // 1920
+    al, D = tab[1][b], tab[5][b]
+    Ah = (np.where(A, 1.0, -1.0) - al)[np.ix_(ar, kl)]
+    Zh = (np.where(Z, 1.0, -1.0) - al)[np.ix_(zr, kl)] if Z is not None else None
+    D = D[kl]
+    T = np.where(Tm, 1.0, -1.0)[ar][:, :nt]
+    S = np.where(Sm, 1.0, -1.0)[zr][:, :nt] if Z is not None else None
+    G = (Ah * D) @ Ah.T
+    G2 = (Ah * D) ** 2 @ (Ah ** 2).T
+    W = G ** 2 - G2
+    np.fill_diagonal(W, 0.0)
+    out = {"QR_TAT": W @ T}
+    if Z is not None:
+        out["QR_ZSZ"] = (((Ah * D) @ Zh.T) ** 2 - (Ah * D) ** 2 @ (Zh ** 2).T) @ S
+    qct = np.zeros((len(kl), nt)); qcz = np.zeros((len(kl), nt))
+    for i in range(len(kl)):
+        m = np.ones(len(kl), bool); m[i] = False
+        Gk = (Ah[:, m] * D[m]) @ Ah[:, m].T
+        np.fill_diagonal(Gk, 0.0)
+        u = Ah[:, i:i + 1] * T
+        qct[i] = D[i] * np.einsum("at,ab,bt->t", u, Gk, u)
+        if Z is not None:
+            v = Zh[:, i:i + 1] * S
+            qcz[i] = D[i] * np.einsum("at,az,zt->t", u, (Ah[:, m] * D[m]) @ Zh[:, m].T, v)
+    out["QC_TAT"] = qct
+    if Z is not None:
+        out["QC_ZSZ"] = qcz
+    return out
+
+
+@pytest.mark.parametrize("kw,steps", [
+    (dict(nA=12, ncols=20, nZ=10, T=3, seed=5, gamma=0.1, planted=()), 0),
+    (dict(nA=14, ncols=30, nZ=0, T=1, seed=6, gamma=0.1), 0),
+    (dict(nA=16, ncols=25, nZ=11, T=2, seed=7, gamma=0.1, row_mask_frac=0.2, col_mask_frac=0.2), 0),
+    (dict(nA=16, ncols=40, nZ=12, T=3, seed=8, gamma=0.1), 3),
+])
+def test_scores_match_loop_count_definition(kw, steps, oracle, tmp_path):
+    """The flattened-loop scores equal the reference's brute-force loop counts, at the
+    initial pass and after elimination steps (so through the low-rank update)."""
+    from lakcluster_c_amd import gen
+    ds = gen.make_dataset(str(tmp_path), "bf", **kw)
+    d = str(tmp_path / "dir_bf_in")
+    S = oracle.State(open(ds.in_path).read())
+    for _ in range(steps):
+        S.iterate()
+    A = gen.read_b16(d + "/A_n.b16")
+    Z = gen.read_b16(d + "/Z_n.b16") if kw["nZ"] else None
+    Tm = gen.read_b16(d + "/T_n.b16") if kw["T"] > 1 else np.ones((A.shape[0], 1), bool)
+    Sm = gen.read_b16(d + "/S_n.b16") if (kw["T"] > 1 and Z is not None) else (np.ones((Z.shape[0], 1), bool) if Z is not None else None)
+    ar = np.flatnonzero(S.get("mask_Ar_j")); kl = np.flatnonzero(S.get("mask_Ac_j"))
+    zr = np.flatnonzero(S.get("mask_Zr_j")) if Z is not None else None
+    tab = S.get("ajdk").reshape(15, -1)
+    nt = kw["T"]
+    bf = _loop_count_scores(A, Z, Tm, Sm, tab, ar, zr, kl, nt)
+    for name, want in bf.items():
+        rows = kl if name.startswith("QC") else ar
+        got = S.get(name).reshape(-1, nt)[rows]
+        scale = np.abs(want).max()
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-8 * scale, err_msg=name)
+    S.close()
