@@ -602,8 +602,12 @@ void Run::st_At_T()
 // bcc_lf_AnZt_S_WnYt (bcc_lf.c:360-409)
 void Run::st_QR()
 {
-  if (has_Z) qr(lf_AnZt, NA, ar.d, ar.n(), zr.d, zr.n(), St.b(), tl.d, tl.n(), lf_AnZtSZnAt, NA, st);
-  qr(lf_AnAt, NA, ar.d, ar.n(), ar.d, ar.n(), Tt.b(), tl.d, tl.n(), lf_AnAtTAnAt, NA, st);
+  if (has_Z) {
+    signed char* sg = (signed char*)g_qrs.need(dev, qr_sign_bytes(zr.n(), tl.n()));
+    qr(lf_AnZt, NA, ar.d, ar.n(), zr.d, zr.n(), St.b(), tl.d, tl.n(), lf_AnZtSZnAt, NA, sg, st);
+  }
+  signed char* sg2 = (signed char*)g_qrs2.need(dev, qr_sign_bytes(ar.n(), tl.n()));
+  qr(lf_AnAt, NA, ar.d, ar.n(), ar.d, ar.n(), Tt.b(), tl.d, tl.n(), lf_AnAtTAnAt, NA, sg2, st);
 }
 
 // bcc_singlestudy_ww (bcc_flattenloop.c:753-831) and get_singlestudy_vv (bcc_flattenloop.c:359-731)

@@ -80,7 +80,7 @@ struct Run {
   int rmv_W = 0, rmv_Wmax = 0;
   int* d_src_rmv = nullptr; uint32_t* d_cmask_rmv = nullptr;
   uint32_t *At_rmvc = nullptr, *Tt_rmvc = nullptr;
-  Grow g_attyn;
+  Grow g_attyn, g_qrs, g_qrs2;
   // device arrays; (row, covariate) arrays are [t*rows + row]
   double *mlf = nullptr;
   double *lf_An_ajdk = nullptr, *lf_Zn_ajdk = nullptr;              // [jk*rows + r]

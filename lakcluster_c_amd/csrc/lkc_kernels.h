@@ -82,8 +82,10 @@ void an_ztsn(const Bits& A, const int* alist, int na, const uint32_t* amask,
              const double* scal, double* out, long long ldo, hipStream_t st);
 
 // AnZt_S_WnYt_vv with X==Y: out[tlist[t]*ldo + a] = Σ_z (X[z*ldx+a]*vS(z))*X[z*ldx+a]
+// sgbuf: qr_sign_bytes(nz, nt) bytes of device scratch
 void qr(const double* X, long long ldx, const int* alist, int na, const int* zlist, int nz,
-        const Bits& Sc, const int* tlist, int nt, double* out, long long ldo, hipStream_t st);
+        const Bits& Sc, const int* tlist, int nt, double* out, long long ldo, signed char* sgbuf, hipStream_t st);
+size_t qr_sign_bytes(int nz, int nt);
 
 // AtTYn_vv: out[(y*nt + t)*nk + k] (compact indices); lf_AtTn/lf_YtTn read at [t*ldT + row]
 void attyn(const Bits& At, const int* alist, int nk, const uint32_t* amask, int acpop,

@@ -107,6 +107,7 @@ void an_ajdk(const Bits& X, const int* rows, int nrows, const uint32_t* mask, in
 // one 1920-bit block of both operands staged (masked) in LDS at a time.
 // ------------------------------------------------------------------
 #define GT 64
+#define GCW 60   // words staged per round
 #define GLD 61   // padded LDS row stride in words
 // Segments: the full layout walks the pcols 1920-bit blocks (60 words each); a
 // column-compacted layout (only some columns kept, block by block) passes for each
@@ -127,41 +128,53 @@ __global__ __launch_bounds__(256) void k_gram(const uint8_t* A, size_t rbA, cons
   long long isum[4][4];
   for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) { dsum[i][j] = 0; isum[i][j] = 0; }
   const int ns = seg ? nseg : pcols;
-  for (int q = 0; q < ns; q++) {
-    const int woff = seg ? seg[3 * q] : q * kPopWords;
-    const int wlen = seg ? seg[3 * q + 1] : kPopWords;
-    const int b = seg ? seg[3 * q + 2] : q;
+  // stage runs of consecutive segments (at most GCW words) at a time
+  int q = 0;
+  while (q < ns) {
+    const int woff0 = seg ? seg[3 * q] : q * kPopWords;
+    int q1 = q, wtot = 0;
+    while (q1 < ns) {
+      const int l = seg ? seg[3 * q1 + 1] : kPopWords;
+      if (q1 > q && wtot + l > GCW) break;
+      wtot += l; q1++;
+    }
     __syncthreads();
-    for (int e = threadIdx.x; e < GT * wlen; e += 256) {
-      int r = e / wlen, w = e % wlen;
-      uint32_t m = mask[woff + w];
+    for (int e = threadIdx.x; e < GT * wtot; e += 256) {
+      int r = e / wtot, w = e % wtot;
+      uint32_t m = mask[woff0 + w];
       uint32_t va = 0, vz = 0;
-      if (a0 + r < na) va = ((const uint32_t*)(A + (size_t)alist[a0 + r] * rbA))[woff + w] & m;
-      if (z0 + r < nz) vz = ((const uint32_t*)(Z + (size_t)zlist[z0 + r] * rbZ))[woff + w] & m;
+      if (a0 + r < na) va = ((const uint32_t*)(A + (size_t)alist[a0 + r] * rbA))[woff0 + w] & m;
+      if (z0 + r < nz) vz = ((const uint32_t*)(Z + (size_t)zlist[z0 + r] * rbZ))[woff0 + w] & m;
       As[r * GLD + w] = va; Zs[r * GLD + w] = vz;
     }
     __syncthreads();
-    int acc[4][4];
-    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) acc[i][j] = 0;
-    for (int w = 0; w < wlen; w++) {
-      uint32_t av[4], zv[4];
-      for (int i = 0; i < 4; i++) av[i] = As[(tx * 4 + i) * GLD + w];
-      for (int j = 0; j < 4; j++) zv[j] = Zs[(ty * 4 + j) * GLD + w];
-      for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) acc[i][j] += popc(av[i] ^ zv[j]);
+    for (int qq = q; qq < q1; qq++) {
+      const int wrel = (seg ? seg[3 * qq] : qq * kPopWords) - woff0;
+      const int wlen = seg ? seg[3 * qq + 1] : kPopWords;
+      const int b = seg ? seg[3 * qq + 2] : qq;
+      int acc[4][4];
+      for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) acc[i][j] = 0;
+      for (int w = wrel; w < wrel + wlen; w++) {
+        uint32_t av[4], zv[4];
+        for (int i = 0; i < 4; i++) av[i] = As[(tx + 16 * i) * GLD + w];
+        for (int j = 0; j < 4; j++) zv[j] = Zs[(ty * 4 + j) * GLD + w];
+        for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) acc[i][j] += popc(av[i] ^ zv[j]);
+      }
+      if (centred) {
+        double wb = D[b];
+        for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) dsum[i][j] += (double)wb * (double)(unsigned)acc[i][j];
+      } else {
+        for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) isum[i][j] += acc[i][j];
+      }
     }
-    if (centred) {
-      double wb = D[b];
-      for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) dsum[i][j] += (double)wb * (double)(unsigned)acc[i][j];
-    } else {
-      for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) isum[i][j] += acc[i][j];
-    }
+    q = q1;
   }
   double a0d1 = centred ? base[5] : base[kAjdk];
   double a2d1 = centred ? base[7] : 0.0;
   for (int j = 0; j < 4; j++) {
     int zc = z0 + ty * 4 + j; if (zc >= nz) continue;
     for (int i = 0; i < 4; i++) {
-      int ac = a0 + tx * 4 + i; if (ac >= na) continue;
+      int ac = a0 + tx + 16 * i; if (ac >= na) continue;
       double dtmp = centred ? dsum[i][j] : (double)isum[i][j];
       double o = a0d1;
       o -= (2 * dtmp);
@@ -485,21 +498,33 @@ void an_ztsn(const Bits& A, const int* alist, int na, const uint32_t* amask,
 // ------------------------------------------------------------------
 // AnZt_S_WnYt_vv (AnZt_S_WnYt_vv.c:49-67) with X==Y; X[z*ldx + a] (a contiguous)
 // ------------------------------------------------------------------
+// Signs of the covariate columns at the listed z: sg[t*nzp + i] = S_t(zlist[i]) (+-1).
+__global__ void k_qr_signs(const uint8_t* Sc, size_t rbS, const int* tlist, int nt, const int* zlist, int nz,
+                           int nzp, signed char* sg)
+{
+  long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (long long)nt * nzp) return;
+  int t = (int)(e / nzp), i = (int)(e % nzp);
+  sg[e] = i < nz ? (signed char)pm_at(Sc + (size_t)tlist[t] * rbS, zlist[i]) : 0;
+}
+
 // One block per 64 rows a: all 256 threads stream a 128(z) x 64(a) tile of X into
-// LDS while wave 0 adds the previous tile into its 64 ordered per-row sums.
+// LDS while wave w adds the previous tile into the ordered sums of covariate w (and
+// w+4): one sequential chain per (a, t), in ascending z.
 #define QR_ZC 128
 __global__ __launch_bounds__(256) void k_qr(const double* X, long long ldx, const int* alist, int na, const int* zlist,
-                                            int nz, const uint8_t* Sc, size_t rbS, const int* tlist, int nt,
+                                            int nz, const signed char* sg, int nzp, const int* tlist, int nt,
                                             double* out, long long ldo)
 {
   __shared__ double tile[2][QR_ZC][64];
   __shared__ signed char sgn[2][kMaxT][QR_ZC];
   const int tid = threadIdx.x, i0 = blockIdx.x * 64;
-  const int ai = tid & 63;
+  const int ai = tid & 63, wave = tid >> 6;
   const bool alive = i0 + ai < na;
   const long long acol = alive ? alist[i0 + ai] : 0;
   const int nch = (nz + QR_ZC - 1) / QR_ZC;
   double reg[QR_ZC / 4];
+  signed char sreg[kMaxT * QR_ZC / 256];
   auto load = [&](int c) {
     const int z0 = c * QR_ZC;
 #pragma unroll
@@ -507,45 +532,63 @@ __global__ __launch_bounds__(256) void k_qr(const double* X, long long ldx, cons
       const int zi = (tid >> 6) + 4 * j;
       reg[j] = (alive && z0 + zi < nz) ? X[(long long)zlist[z0 + zi] * ldx + acol] : 0.0;
     }
-  };
-  auto store = [&](int c, int b) {
-    const int z0 = c * QR_ZC;
 #pragma unroll
-    for (int j = 0; j < QR_ZC / 4; j++) tile[b][(tid >> 6) + 4 * j][ai] = reg[j];
-    for (int e = tid; e < nt * QR_ZC; e += 256) {
-      const int t = e / QR_ZC, zi = e % QR_ZC;
-      sgn[b][t][zi] = (z0 + zi < nz) ? (signed char)pm_at(Sc + (size_t)tlist[t] * rbS, zlist[z0 + zi]) : 0;
+    for (int j = 0; j < kMaxT * QR_ZC / 256; j++) {
+      const int e = tid + 256 * j, t = e / QR_ZC, zi = e % QR_ZC;
+      sreg[j] = (t < nt) ? sg[(long long)t * nzp + z0 + zi] : 0;
     }
   };
-  double acc[kMaxT];
-  for (int t = 0; t < kMaxT; t++) acc[t] = 0;
-  if (nch > 0) { load(0); store(0, 0); }
+  auto store = [&](int b) {
+#pragma unroll
+    for (int j = 0; j < QR_ZC / 4; j++) tile[b][(tid >> 6) + 4 * j][ai] = reg[j];
+#pragma unroll
+    for (int j = 0; j < kMaxT * QR_ZC / 256; j++) {
+      const int e = tid + 256 * j, t = e / QR_ZC, zi = e % QR_ZC;
+      sgn[b][t][zi] = sreg[j];
+    }
+  };
+  double acc0 = 0.0, acc1 = 0.0;
+  const int ta = wave, tb = wave + 4;
+  if (nch > 0) { load(0); store(0); }
   __syncthreads();
   for (int c = 0; c < nch; c++) {
     const int b = c & 1;
     if (c + 1 < nch) load(c + 1);
-    if (tid < 64) {
-      const int zn = min(QR_ZC, nz - c * QR_ZC);
+    const int zn = min(QR_ZC, nz - c * QR_ZC);
+    if (ta < nt) {
 #pragma unroll 8
       for (int zi = 0; zi < zn; zi++) {
         const double x = tile[b][zi][ai];
-        for (int t = 0; t < nt; t++) {
-          const int vS = sgn[b][t][zi];
-          acc[t] += x * vS * x;
-        }
+        const int vS = sgn[b][ta][zi];
+        acc0 += x * vS * x;
       }
     }
-    if (c + 1 < nch) store(c + 1, b ^ 1);
+    if (tb < nt) {
+#pragma unroll 8
+      for (int zi = 0; zi < zn; zi++) {
+        const double x = tile[b][zi][ai];
+        const int vS = sgn[b][tb][zi];
+        acc1 += x * vS * x;
+      }
+    }
+    if (c + 1 < nch) store(b ^ 1);
     __syncthreads();
   }
-  if (tid < 64 && alive) for (int t = 0; t < nt; t++) out[(long long)tlist[t] * ldo + acol] = acc[t];
+  if (alive) {
+    if (ta < nt) out[(long long)tlist[ta] * ldo + acol] = acc0;
+    if (tb < nt) out[(long long)tlist[tb] * ldo + acol] = acc1;
+  }
 }
 void qr(const double* X, long long ldx, const int* alist, int na, const int* zlist, int nz,
-        const Bits& Sc, const int* tlist, int nt, double* out, long long ldo, hipStream_t st)
+        const Bits& Sc, const int* tlist, int nt, double* out, long long ldo, signed char* sgbuf, hipStream_t st)
 {
   if (na <= 0) return;
-  hipLaunchKernelGGL(k_qr, dim3(cdiv(na, 64)), dim3(256), 0, st, X, ldx, alist, na, zlist, nz, Sc.p, Sc.rb, tlist, nt, out, ldo);
+  const int nzp = ((nz + QR_ZC - 1) / QR_ZC) * QR_ZC + QR_ZC;
+  if (nt > 0 && nzp > 0)
+    hipLaunchKernelGGL(k_qr_signs, dim3(cdiv((long long)nt * nzp, 256)), dim3(256), 0, st, Sc.p, Sc.rb, tlist, nt, zlist, nz, nzp, sgbuf);
+  hipLaunchKernelGGL(k_qr, dim3(cdiv(na, 64)), dim3(256), 0, st, X, ldx, alist, na, zlist, nz, sgbuf, nzp, tlist, nt, out, ldo);
 }
+size_t qr_sign_bytes(int nz, int nt) { return (size_t)nt * (((nz + QR_ZC - 1) / QR_ZC) * QR_ZC + QR_ZC) + 256; }
 
 // ------------------------------------------------------------------
 // AtTYn_vv (AtTYn_vv.c:79-120)
