@@ -578,16 +578,20 @@ void Run::ww_excerpt(int trm, const Side& s1, const Side& s2, const IList& kls, 
   a.cz = cz; a.ldc = ldc;
   a.Sc = pw.Cov->b(); a.Q = pw.X->b(); a.ajdk = ajdk; a.pcols = pcols; a.scal = scal;
   a.out = out; a.ldo = M;
-  Prof::Pending pp;
-  prof.begin("ww", 2.0 * tl.n() * kls.n() * nw * (double)nn * ((c.B + 7) / 8), st, pp);
+  // timed: the contraction kernel itself (the quantised-value preparation is not included)
+  struct Ctx { Prof* prof; Prof::Pending pp; double work; };
+  Ctx cx{ &prof, Prof::Pending(), 2.0 * tl.n() * kls.n() * nw * (double)nn * ((c.B + 7) / 8) };
+  WwHook hook{ [](void* p, hipStream_t s) { Ctx* x = (Ctx*)p; x->prof->begin("ww", x->work, s, x->pp); },
+               [](void* p, hipStream_t s) { Ctx* x = (Ctx*)p; x->prof->end(x->pp, s); }, &cx };
   if (ww_kernel == 1) {
     size_t wb = ww_work_bytes(a.nk, a.nt, nw);
+    hook.before(&cx, st);
     ww_valu(a, (double*)g_work.need(dev, wb), wb, st);
+    hook.after(&cx, st);
   } else {
     size_t wb = ww_mfma_work_bytes(a.nk, a.nt, nw, prb, c.B);
-    ww_mfma(a, g_work.need(dev, wb), wb, st);
+    ww_mfma(a, g_work.need(dev, wb), wb, st, &hook);
   }
-  prof.end(pp, st);
   ops += (long long)tl.n() * kls.n() * nw * (c.B + 1) * (long long)prb * 8;
 }
 
@@ -602,6 +606,7 @@ void Run::st_At_T()
 // bcc_lf_AnZt_S_WnYt (bcc_lf.c:360-409)
 void Run::st_QR()
 {
+  if (qr_fresh) { qr_fresh = false; return; }
   if (has_Z) {
     signed char* sg = (signed char*)g_qrs.need(dev, qr_sign_bytes(zr.n(), tl.n()));
     qr(lf_AnZt, NA, ar.d, ar.n(), zr.d, zr.n(), St.b(), tl.d, tl.n(), lf_AnZtSZnAt, NA, sg, st);
@@ -793,12 +798,31 @@ void Run::lrup()
     compact_cols(Zn.b(), nullptr, NZ, d_src, W, Zn_c, st);
     an_ajdk(Zn.b(), zr.d, zr.n(), Ac_rmv, pcols, ajdk, mlf_rmv, vn_ajdk, NZ, st);
     ops += 15LL * zr.n() * (long long)Zn.rb * 8;
-    gram(Anc, rtn.d, rtn.n(), Znc, zr.d, zr.n(), d_cmask, pcols, ajdk, mlf_rmv,
-         jn_ajdk + 6 * (size_t)NA, vn_ajdk + 6 * (size_t)NZ, 1, lf_AnZt, NA, 1, 1, 1, st, d_seg, cmp_nseg);
-    ops += (long long)rtn.n() * zr.n() * rbm;
   }
-  gram(Anc, rtn.d, rtn.n(), Anc, rtn.d, rtn.n(), d_cmask, pcols, ajdk, mlf_rmv,
-       jn_ajdk + 6 * (size_t)NA, jn_ajdk + 6 * (size_t)NA, 1, lf_AnAt, NA, 1, 1, 1, st, d_seg, cmp_nseg);
+  if (W <= kGramQrMaxW) {
+    // the updated Grams' QR sums (the next bcc_lf_AnZt_S_WnYt) come out of the same pass
+    auto fused = [&](const Bits& Zc, const IList& zl, const double* xz, double* X, const DMat& Cov, double* qout, Grow& gs) {
+      signed char* sg = (signed char*)gs.need(dev, qr_sign_bytes(zl.n(), tl.n()));
+      GramQrArgs g;
+      g.A = Anc; g.alist = rtn.d; g.na = rtn.n(); g.Z = Zc; g.zlist = zl.d; g.nz = zl.n();
+      g.mask = d_cmask; g.W = W; g.seg = d_seg; g.nseg = cmp_nseg;
+      g.ajdk = ajdk; g.pcols = pcols; g.base16 = mlf_rmv; g.xa = jn_ajdk + 6 * (size_t)NA; g.xz = xz;
+      g.X = X; g.ldx = NA;
+      g.nzp = qr_signs(Cov.b(), tl.d, tl.n(), zl.d, zl.n(), sg, st); g.sg = sg; g.tlist = tl.d; g.nt = tl.n();
+      g.qr = qout; g.ldq = NA;
+      gram_qr(g, st);
+    };
+    if (has_Z) fused(Znc, zr, vn_ajdk + 6 * (size_t)NZ, lf_AnZt, St, lf_AnZtSZnAt, g_qrs);
+    fused(Anc, rtn, jn_ajdk + 6 * (size_t)NA, lf_AnAt, Tt, lf_AnAtTAnAt, g_qrs2);
+    qr_fresh = true;
+  } else {
+    if (has_Z)
+      gram(Anc, rtn.d, rtn.n(), Znc, zr.d, zr.n(), d_cmask, pcols, ajdk, mlf_rmv,
+           jn_ajdk + 6 * (size_t)NA, vn_ajdk + 6 * (size_t)NZ, 1, lf_AnZt, NA, 1, 1, 1, st, d_seg, cmp_nseg);
+    gram(Anc, rtn.d, rtn.n(), Anc, rtn.d, rtn.n(), d_cmask, pcols, ajdk, mlf_rmv,
+         jn_ajdk + 6 * (size_t)NA, jn_ajdk + 6 * (size_t)NA, 1, lf_AnAt, NA, 1, 1, 1, st, d_seg, cmp_nseg);
+  }
+  if (has_Z) ops += (long long)rtn.n() * zr.n() * rbm;
   ops += (long long)rtn.n() * rtn.n() * rbm;
   // QC a0
   an_ajdk(An.b(), rmv.d, rmv.n(), Ac_rtn, pcols, ajdk, mlf_rtn, kn_ajdk, NA, st);
@@ -953,7 +977,7 @@ void Run::reset()
   push_ctl();
   HIPCHK(hipStreamSynchronize(st));
   trace.clear(); xdrop_a.clear(); xdrop_b.clear();
-  n_iter = 0; inited = false; ops = 0; init_s = 0; loop_s = 0;
+  n_iter = 0; inited = false; ops = 0; init_s = 0; loop_s = 0; qr_fresh = false;
   for (auto& k : prof.stats) { k.launches = 0; k.ms = 0; k.work = 0; }
 }
 

@@ -100,7 +100,8 @@ struct Run {
   std::vector<double> QC_sv, QR_sv; std::vector<int> QC_perm, QR_perm;
   std::vector<double> trace; std::vector<int> xdrop_a, xdrop_b;
   int n_iter = 0; bool inited = false;
-  int ww_kernel = 0;                        // 0: int8 MFMA, 1: VALU bit-plane popcount
+  int ww_kernel = 0;
+  bool qr_fresh = false;                    // the lrup already left the QR sums of the updated Grams                        // 0: int8 MFMA, 1: VALU bit-plane popcount
   long long ops = 0;
   double load_s = 0, init_s = 0, loop_s = 0;
   std::string err;

@@ -74,7 +74,9 @@ void ww_valu(const WwArgs& a, double* work, size_t work_bytes, hipStream_t st);
 
 // int8 matrix-core formulation (cz split into signed bytes); planes are not used
 size_t ww_mfma_work_bytes(int nk, int nt, int nw, size_t prb, int B);
-void ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t st);
+// hook: called on the stream right before and after the matrix-core kernel (timing)
+struct WwHook { void (*before)(void*, hipStream_t); void (*after)(void*, hipStream_t); void* ctx; };
+void ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t st, const WwHook* hook = nullptr);
 
 // An_ZtSn_ww: out[slist[s]*ldo + alist[i]] = (base + Σ 2^nb pm0(A_a, plane(s,nb)))/mlt
 void an_ztsn(const Bits& A, const int* alist, int na, const uint32_t* amask,
@@ -117,6 +119,22 @@ struct AttynArgs {
 };
 size_t attyn_mfma_work_bytes(int ny, int nt);
 void attyn_mfma(const AttynArgs& a, void* work, size_t work_bytes, hipStream_t st);
+
+// QR stage of the low-rank update fused with the QR row sums (column-compacted rows,
+// at most kGramQrMaxW words): X[z*ldx + a] -= centred Gram(a, z) over the compacted
+// columns, then qr[tlist[t]*ldq + a] = sum_z (x*S_t(z))*x over the updated values.
+constexpr int kGramQrMaxW = 120;
+struct GramQrArgs {
+  Bits A; const int* alist; int na; Bits Z; const int* zlist; int nz;
+  const uint32_t* mask; int W; const int* seg; int nseg;
+  const double* ajdk; int pcols; const double* base16; const double* xa; const double* xz;
+  double* X; long long ldx;
+  const signed char* sg; int nzp; const int* tlist; int nt;
+  double* qr; long long ldq;
+};
+void gram_qr(const GramQrArgs& g, hipStream_t st);
+// sg[t*nzp + i] = S_t(zlist[i]) (+-1) for the QR sums; returns nzp.  sgbuf: qr_sign_bytes(nz, nt)
+int qr_signs(const Bits& Sc, const int* tlist, int nt, const int* zlist, int nz, signed char* sgbuf, hipStream_t st);
 
 // out[i] = pop(X[rows[i]] & mask) over nwords 32-bit words (rows==NULL: row i)
 void popc_rows(const Bits& X, const int* rows, int nrows, const uint32_t* mask, int nwords, int* out, hipStream_t st);

@@ -588,6 +588,13 @@ void qr(const double* X, long long ldx, const int* alist, int na, const int* zli
     hipLaunchKernelGGL(k_qr_signs, dim3(cdiv((long long)nt * nzp, 256)), dim3(256), 0, st, Sc.p, Sc.rb, tlist, nt, zlist, nz, nzp, sgbuf);
   hipLaunchKernelGGL(k_qr, dim3(cdiv(na, 64)), dim3(256), 0, st, X, ldx, alist, na, zlist, nz, sgbuf, nzp, tlist, nt, out, ldo);
 }
+int qr_signs(const Bits& Sc, const int* tlist, int nt, const int* zlist, int nz, signed char* sgbuf, hipStream_t st)
+{
+  const int nzp = ((nz + QR_ZC - 1) / QR_ZC) * QR_ZC + QR_ZC;
+  if (nt > 0)
+    hipLaunchKernelGGL(k_qr_signs, dim3(cdiv((long long)nt * nzp, 256)), dim3(256), 0, st, Sc.p, Sc.rb, tlist, nt, zlist, nz, nzp, sgbuf);
+  return nzp;
+}
 size_t qr_sign_bytes(int nz, int nt) { return (size_t)nt * (((nz + QR_ZC - 1) / QR_ZC) * QR_ZC + QR_ZC) + 256; }
 
 // ------------------------------------------------------------------

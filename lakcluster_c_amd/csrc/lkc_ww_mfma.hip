@@ -224,7 +224,7 @@ size_t ww_mfma_work_bytes(int nk, int nt, int nw, size_t prb, int B)
   return b + 1024;
 }
 
-void ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t st)
+void ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t st, const WwHook* hook)
 {
   if (a.nk <= 0 || a.nt <= 0) return;
   int ndig = (a.B + 7) / 8;
@@ -248,12 +248,14 @@ void ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t st)
   hipLaunchKernelGGL(k_ww_hg, dim3(a.nw + 1), dim3(64), 0, st, a, H, bp, Gm);
   hipLaunchKernelGGL(k_ww_cntp, dim3(a.nk), dim3(64), 0, st, a, cntp, llk);
   dim3 grid(cdiv(a.nk, 128), cdiv(a.nt, a.nt >= 3 ? 3 : a.nt));
+  if (hook && hook->before) hook->before(hook->ctx, st);
 #define WW_LAUNCH(NTG, ND) hipLaunchKernelGGL((k_ww_mfma<NTG, ND>), grid, dim3(256), 0, st, a, Dg, ldn, nwp, H, bp, Gm, cntp, llk)
 #define WW_ND(NTG) switch (ndig) { case 1: WW_LAUNCH(NTG, 1); break; case 2: WW_LAUNCH(NTG, 2); break; \
                                    case 3: WW_LAUNCH(NTG, 3); break; default: WW_LAUNCH(NTG, 4); break; }
   if (a.nt >= 3) { WW_ND(3) } else if (a.nt == 2) { WW_ND(2) } else { WW_ND(1) }
 #undef WW_ND
 #undef WW_LAUNCH
+  if (hook && hook->after) hook->after(hook->ctx, st);
 }
 
 }  // namespace lkc
