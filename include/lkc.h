@@ -96,9 +96,10 @@ int lkc_bcc_reset(lkc_bcc *D);
    algorithmic work (int8 multiply-adds x2 of the exact integer contraction)
    since the last reset. */
 int lkc_bcc_set_profiling(lkc_bcc *D, int on);
-/* Implementation of the At_T_YnWt_S_Zn_ww contraction: 0 = int8 matrix cores
-   (default), 1 = VALU popcount over the B_MLT bit planes.  Both are bit-exact. */
-int lkc_bcc_set_ww_kernel(lkc_bcc *D, int which);
+/* Implementation of the contractions (the centred AnZt_vv Grams, At_T_YnWt_S_Zn_ww
+   and the AtTYn_vv terms of the update): 0 = int8 matrix cores (default), 1 = VALU
+   popcount over packed bits (and the B_MLT bit planes).  Both are bit-exact. */
+int lkc_bcc_set_kernels(lkc_bcc *D, int which);
 int lkc_bcc_kernel_stats(const lkc_bcc *D, const char *name, long long *launches, double *ms, double *work);
 /* State inspection (no counterpart in the reference, which keeps these arrays in
    struct bcc_ajdk/bcc_single/bcc_double): copies the named score array, e.g.

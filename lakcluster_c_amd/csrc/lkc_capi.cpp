@@ -130,7 +130,7 @@ int lkc_bcc_set_profiling(lkc_bcc* D, int on)
   return LKC_OK;
 }
 
-int lkc_bcc_set_ww_kernel(lkc_bcc* D, int which)
+int lkc_bcc_set_kernels(lkc_bcc* D, int which)
 {
   if (!D || which < 0 || which > 1) return LKC_ERR_STATE;
   D->run.ww_kernel = which;

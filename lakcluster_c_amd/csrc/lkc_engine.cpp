@@ -340,7 +340,7 @@ void Run::load(const char* in_text)
   size_t o_l[8];
   for (int i = 0; i < 8; i++) o_l[i] = reserve(sizeof(int) * (size_t)(lcap[i] + 1));
   size_t o_zero = reserve(sizeof(int) * 4);
-  cmp_Wmax = pcols + (M + 31) / 32;
+  cmp_Wmax = 4 * pcols + (M + 31) / 32 + 4;   // segments padded to whole 4-word groups
   ctl_cmp_off[0] = reserve(sizeof(int) * 3 * (size_t)(pcols + 1));
   ctl_cmp_off[1] = reserve(sizeof(int) * 32 * (size_t)cmp_Wmax);
   ctl_cmp_off[2] = reserve(sizeof(uint32_t) * (size_t)cmp_Wmax);
@@ -357,6 +357,7 @@ void Run::load(const char* in_text)
   d_zero = (int*)(dctl + o_zero);
   d_seg = (int*)(dctl + ctl_cmp_off[0]); d_src = (int*)(dctl + ctl_cmp_off[1]); d_cmask = (uint32_t*)(dctl + ctl_cmp_off[2]);
   d_src_rmv = (int*)(dctl + ctl_cmp_off[3]); d_cmask_rmv = (uint32_t*)(dctl + ctl_cmp_off[4]);
+  for (int i = 0; i < 6; i++) gcnt[i] = dev.alloc<int>((size_t)std::max(NA, NZ) * (pcols + 1));
   At_rmvc = dev.alloc<uint32_t>((size_t)M * rmv_Wmax);
   Tt_rmvc = dev.alloc<uint32_t>((size_t)T * rmv_Wmax);
   An_c = dev.alloc<uint32_t>((size_t)NA * cmp_Wmax);
@@ -460,31 +461,32 @@ void Run::push_ctl()
   };
   fill_list(ar, hAr_j, NA); fill_list(kl, hAc_j, M); fill_list(tl, hTc_j, T); fill_list(zr, hZr_j, NZ);
   fill_list(rtn, hAr_rtn, NA); fill_list(rmv, hAr_rmv, NA); fill_list(crt, hAc_rtn, M); fill_list(crm, hAc_rmv, M);
-  // compacted layout of the dropped columns: per non-empty 1920-block a segment of whole words
-  {
-    int* seg = (int*)(hctl + ctl_cmp_off[0]);
-    int* src = (int*)(hctl + ctl_cmp_off[1]);
-    uint32_t* cm = (uint32_t*)(hctl + ctl_cmp_off[2]);
+  // compacted column layout: per non-empty 1920-block a segment of whole 4-word groups
+  // (offset and length in 32-column words, block), the source column of each position
+  // and the mask of the valid positions
+  auto layout = [&](const std::vector<int>& cols, int* seg, int* src, uint32_t* cm, int& Wout, int& nsout) {
     int W = 0, ns = 0;
     size_t i = 0;
-    while (i < crm.h.size()) {
-      int b = crm.h[i] / kPopBits;
+    while (i < cols.size()) {
+      int b = cols[i] / kPopBits;
       size_t j = i;
-      while (j < crm.h.size() && crm.h[j] / kPopBits == b) j++;
-      int cnt = (int)(j - i), nw = (cnt + 31) / 32;
+      while (j < cols.size() && cols[j] / kPopBits == b) j++;
+      int cnt = (int)(j - i), nw = ((cnt + 31) / 32 + 3) / 4 * 4;
       seg[3 * ns] = W; seg[3 * ns + 1] = nw; seg[3 * ns + 2] = b; ns++;
-      for (int q = 0; q < 32 * nw; q++) {
-        int c = q < cnt ? crm.h[i + q] : -1;
-        src[32 * W + q] = c;
-      }
-      for (int w = 0; w < nw; w++) {
-        uint32_t v = 0;
-        for (int q = 0; q < 32; q++) if (32 * w + q < cnt) v |= 1u << ((q >> 3) * 8 + (7 - (q & 7)));
-        cm[W + w] = v;
-      }
+      for (int q = 0; q < 32 * nw; q++) src[32 * W + q] = q < cnt ? cols[i + q] : -1;
+      if (cm)
+        for (int w = 0; w < nw; w++) {
+          uint32_t v = 0;
+          for (int q = 0; q < 32; q++) if (32 * w + q < cnt) v |= 1u << ((q >> 3) * 8 + (7 - (q & 7)));
+          cm[W + w] = v;
+        }
       W += nw; i = j;
     }
-    cmp_W = W; cmp_nseg = ns;
+    Wout = W; nsout = ns;
+  };
+  {
+    layout(crm.h, (int*)(hctl + ctl_cmp_off[0]), (int*)(hctl + ctl_cmp_off[1]), (uint32_t*)(hctl + ctl_cmp_off[2]),
+           cmp_W, cmp_nseg);
     // removed rows, packed in whole 128-bit groups
     int* srr = (int*)(hctl + ctl_cmp_off[3]);
     uint32_t* cmr = (uint32_t*)(hctl + ctl_cmp_off[4]);
@@ -539,6 +541,22 @@ void Run::st_ZtSn()
 // bcc_lf_YnWt (bcc_lf.c:223-302): centred Grams; lf_AnZt[z*NA + a], lf_AnAt[a2*NA + a1]
 void Run::st_YnWt()
 {
+  if (ww_kernel == 0) {
+    int *cA = gcnt[0], *cZ = gcnt[1];
+    seg_counts(An.b(), ar.d, ar.n(), Ac_j, nullptr, pcols, cA, NA, st);
+    if (has_Z) {
+      seg_counts(Zn.b(), zr.d, zr.n(), Ac_j, nullptr, pcols, cZ, NZ, st);
+      GramMArgs g{ An.p, An.rb, ar.d, ar.n(), Zn.p, Zn.rb, zr.d, zr.n(), Ac_j, nullptr, pcols, cA, NA, cZ, NZ,
+                   ajdk, pcols, mlf, lf_An_ajdk + 6 * (size_t)NA, lf_Zn_ajdk + 6 * (size_t)NZ, lf_AnZt, NA, 1, 1, 0 };
+      gram_mfma(g, st);
+      ops += (long long)ar.n() * zr.n() * (long long)bsize(M) * 8;
+    }
+    GramMArgs g{ An.p, An.rb, ar.d, ar.n(), An.p, An.rb, ar.d, ar.n(), Ac_j, nullptr, pcols, cA, NA, cA, NA,
+                 ajdk, pcols, mlf, lf_An_ajdk + 6 * (size_t)NA, lf_An_ajdk + 6 * (size_t)NA, lf_AnAt, NA, 1, 1, 0 };
+    gram_mfma(g, st);
+    ops += (long long)ar.n() * ar.n() * (long long)bsize(M) * 8;
+    return;
+  }
   if (has_Z) {
     gram(An.b(), ar.d, ar.n(), Zn.b(), zr.d, zr.n(), Ac_j, pcols, ajdk, mlf,
          lf_An_ajdk + 6 * (size_t)NA, lf_Zn_ajdk + 6 * (size_t)NZ, 1, lf_AnZt, NA, 1, 1, 0, st);
@@ -606,7 +624,6 @@ void Run::st_At_T()
 // bcc_lf_AnZt_S_WnYt (bcc_lf.c:360-409)
 void Run::st_QR()
 {
-  if (qr_fresh) { qr_fresh = false; return; }
   if (has_Z) {
     signed char* sg = (signed char*)g_qrs.need(dev, qr_sign_bytes(zr.n(), tl.n()));
     qr(lf_AnZt, NA, ar.d, ar.n(), zr.d, zr.n(), St.b(), tl.d, tl.n(), lf_AnZtSZnAt, NA, sg, st);
@@ -793,28 +810,25 @@ void Run::lrup()
   // the Grams over the dropped columns run on the column-compacted rows
   const int W = cmp_W;
   Bits Anc{ (const uint8_t*)An_c, NA, W * 32, (size_t)W * 4 }, Znc{ (const uint8_t*)Zn_c, NZ, W * 32, (size_t)W * 4 };
-  compact_cols(An.b(), nullptr, NA, d_src, W, An_c, st);
   if (has_Z) {
-    compact_cols(Zn.b(), nullptr, NZ, d_src, W, Zn_c, st);
     an_ajdk(Zn.b(), zr.d, zr.n(), Ac_rmv, pcols, ajdk, mlf_rmv, vn_ajdk, NZ, st);
     ops += 15LL * zr.n() * (long long)Zn.rb * 8;
   }
-  if (W <= kGramQrMaxW) {
-    // the updated Grams' QR sums (the next bcc_lf_AnZt_S_WnYt) come out of the same pass
-    auto fused = [&](const Bits& Zc, const IList& zl, const double* xz, double* X, const DMat& Cov, double* qout, Grow& gs) {
-      signed char* sg = (signed char*)gs.need(dev, qr_sign_bytes(zl.n(), tl.n()));
-      GramQrArgs g;
-      g.A = Anc; g.alist = rtn.d; g.na = rtn.n(); g.Z = Zc; g.zlist = zl.d; g.nz = zl.n();
-      g.mask = d_cmask; g.W = W; g.seg = d_seg; g.nseg = cmp_nseg;
-      g.ajdk = ajdk; g.pcols = pcols; g.base16 = mlf_rmv; g.xa = jn_ajdk + 6 * (size_t)NA; g.xz = xz;
-      g.X = X; g.ldx = NA;
-      g.nzp = qr_signs(Cov.b(), tl.d, tl.n(), zl.d, zl.n(), sg, st); g.sg = sg; g.tlist = tl.d; g.nt = tl.n();
-      g.qr = qout; g.ldq = NA;
-      gram_qr(g, st);
-    };
-    if (has_Z) fused(Znc, zr, vn_ajdk + 6 * (size_t)NZ, lf_AnZt, St, lf_AnZtSZnAt, g_qrs);
-    fused(Anc, rtn, jn_ajdk + 6 * (size_t)NA, lf_AnAt, Tt, lf_AnAtTAnAt, g_qrs2);
-    qr_fresh = true;
+  compact_cols(An.b(), nullptr, NA, d_src, W, An_c, W, st);
+  if (has_Z) compact_cols(Zn.b(), nullptr, NZ, d_src, W, Zn_c, W, st);
+  int *c_mr = gcnt[0], *c_mm = gcnt[1], *c_mz = gcnt[2];
+  if (ww_kernel == 0) {
+    seg_counts(Anc, rtn.d, rtn.n(), nullptr, d_seg, cmp_nseg, c_mr, NA, st);
+    seg_counts(Anc, rmv.d, rmv.n(), nullptr, d_seg, cmp_nseg, c_mm, NA, st);
+    if (has_Z) {
+      seg_counts(Znc, zr.d, zr.n(), nullptr, d_seg, cmp_nseg, c_mz, NZ, st);
+      GramMArgs g{ Anc.p, Anc.rb, rtn.d, rtn.n(), Znc.p, Znc.rb, zr.d, zr.n(), nullptr, d_seg, cmp_nseg, c_mr, NA, c_mz, NZ,
+                   ajdk, pcols, mlf_rmv, jn_ajdk + 6 * (size_t)NA, vn_ajdk + 6 * (size_t)NZ, lf_AnZt, NA, 1, 1, 1 };
+      gram_mfma(g, st);
+    }
+    GramMArgs g{ Anc.p, Anc.rb, rtn.d, rtn.n(), Anc.p, Anc.rb, rtn.d, rtn.n(), nullptr, d_seg, cmp_nseg, c_mr, NA, c_mr, NA,
+                 ajdk, pcols, mlf_rmv, jn_ajdk + 6 * (size_t)NA, jn_ajdk + 6 * (size_t)NA, lf_AnAt, NA, 1, 1, 1 };
+    gram_mfma(g, st);
   } else {
     if (has_Z)
       gram(Anc, rtn.d, rtn.n(), Znc, zr.d, zr.n(), d_cmask, pcols, ajdk, mlf_rmv,
@@ -837,21 +851,45 @@ void Run::lrup()
   // QC a1: compact Grams G[j*n1 + i]
   int nrm = rmv.n(), nrt = rtn.n(), nz = zr.n();
   double *knzt = nullptr, *hnvt = nullptr;
+  double* ankt = (double*)g_ankt.need(dev, sizeof(double) * ((size_t)nrt * nrm + 1));
+  double* knkt = (double*)g_knkt.need(dev, sizeof(double) * ((size_t)nrm * nrm + 1));
   if (has_Z) {
     knzt = (double*)g_knzt.need(dev, sizeof(double) * ((size_t)nrm * nz + 1));
     hnvt = (double*)g_hnvt.need(dev, sizeof(double) * ((size_t)nrm * nz + 1));
-    gram(An.b(), rmv.d, nrm, Zn.b(), zr.d, nz, Ac_rtn, pcols, ajdk, mlf_rtn,
-         kn_ajdk + 6 * (size_t)NA, zn_ajdk + 6 * (size_t)NZ, 1, knzt, nrm, 0, 0, 0, st);
-    gram(Anc, rmv.d, nrm, Znc, zr.d, nz, d_cmask, pcols, ajdk, mlf_rmv,
-         hn_ajdk + 6 * (size_t)NA, vn_ajdk + 6 * (size_t)NZ, 1, hnvt, nrm, 0, 0, 0, st, d_seg, cmp_nseg);
-    ops += 2LL * nrm * nz * rbm;
   }
-  double* ankt = (double*)g_ankt.need(dev, sizeof(double) * ((size_t)nrt * nrm + 1));
-  double* knkt = (double*)g_knkt.need(dev, sizeof(double) * ((size_t)nrm * nrm + 1));
-  gram(An.b(), rtn.d, nrt, An.b(), rmv.d, nrm, Ac_rtn, pcols, ajdk, mlf_rtn,
-       an_ajdk_ + 6 * (size_t)NA, kn_ajdk + 6 * (size_t)NA, 1, ankt, nrt, 0, 0, 0, st);
-  gram(An.b(), rmv.d, nrm, An.b(), rmv.d, nrm, Ac_rtn, pcols, ajdk, mlf_rtn,
-       kn_ajdk + 6 * (size_t)NA, kn_ajdk + 6 * (size_t)NA, 1, knkt, nrm, 0, 0, 0, st);
+  if (ww_kernel == 0) {
+    int *c_kr = gcnt[3], *c_km = gcnt[4], *c_kz = gcnt[5];
+    seg_counts(An.b(), rtn.d, nrt, Ac_rtn, nullptr, pcols, c_kr, NA, st);
+    seg_counts(An.b(), rmv.d, nrm, Ac_rtn, nullptr, pcols, c_km, NA, st);
+    if (has_Z) {
+      seg_counts(Zn.b(), zr.d, nz, Ac_rtn, nullptr, pcols, c_kz, NZ, st);
+      GramMArgs g1{ An.p, An.rb, rmv.d, nrm, Zn.p, Zn.rb, zr.d, nz, Ac_rtn, nullptr, pcols, c_km, NA, c_kz, NZ,
+                    ajdk, pcols, mlf_rtn, kn_ajdk + 6 * (size_t)NA, zn_ajdk + 6 * (size_t)NZ, knzt, nrm, 0, 0, 0 };
+      gram_mfma(g1, st);
+      GramMArgs g2{ Anc.p, Anc.rb, rmv.d, nrm, Znc.p, Znc.rb, zr.d, nz, nullptr, d_seg, cmp_nseg, c_mm, NA, c_mz, NZ,
+                    ajdk, pcols, mlf_rmv, hn_ajdk + 6 * (size_t)NA, vn_ajdk + 6 * (size_t)NZ, hnvt, nrm, 0, 0, 0 };
+      gram_mfma(g2, st);
+      ops += 2LL * nrm * nz * rbm;
+    }
+    GramMArgs g3{ An.p, An.rb, rtn.d, nrt, An.p, An.rb, rmv.d, nrm, Ac_rtn, nullptr, pcols, c_kr, NA, c_km, NA,
+                  ajdk, pcols, mlf_rtn, an_ajdk_ + 6 * (size_t)NA, kn_ajdk + 6 * (size_t)NA, ankt, nrt, 0, 0, 0 };
+    gram_mfma(g3, st);
+    GramMArgs g4{ An.p, An.rb, rmv.d, nrm, An.p, An.rb, rmv.d, nrm, Ac_rtn, nullptr, pcols, c_km, NA, c_km, NA,
+                  ajdk, pcols, mlf_rtn, kn_ajdk + 6 * (size_t)NA, kn_ajdk + 6 * (size_t)NA, knkt, nrm, 0, 0, 0 };
+    gram_mfma(g4, st);
+  } else {
+    if (has_Z) {
+      gram(An.b(), rmv.d, nrm, Zn.b(), zr.d, nz, Ac_rtn, pcols, ajdk, mlf_rtn,
+           kn_ajdk + 6 * (size_t)NA, zn_ajdk + 6 * (size_t)NZ, 1, knzt, nrm, 0, 0, 0, st);
+      gram(Anc, rmv.d, nrm, Znc, zr.d, nz, d_cmask, pcols, ajdk, mlf_rmv,
+           hn_ajdk + 6 * (size_t)NA, vn_ajdk + 6 * (size_t)NZ, 1, hnvt, nrm, 0, 0, 0, st, d_seg, cmp_nseg);
+      ops += 2LL * nrm * nz * rbm;
+    }
+    gram(An.b(), rtn.d, nrt, An.b(), rmv.d, nrm, Ac_rtn, pcols, ajdk, mlf_rtn,
+         an_ajdk_ + 6 * (size_t)NA, kn_ajdk + 6 * (size_t)NA, 1, ankt, nrt, 0, 0, 0, st);
+    gram(An.b(), rmv.d, nrm, An.b(), rmv.d, nrm, Ac_rtn, pcols, ajdk, mlf_rtn,
+         kn_ajdk + 6 * (size_t)NA, kn_ajdk + 6 * (size_t)NA, 1, knkt, nrm, 0, 0, 0, st);
+  }
   ops += ((long long)nrt * nrm + (long long)nrm * nrm) * rbm;
   // QC a2 + a3
   double *t_knzt = mt[0], *t_hnvt = mt[1], *t_ankt = mt[2], *t_knkt = mt[3];
@@ -882,8 +920,8 @@ void Run::lrup()
   if (ww_kernel == 0) {
     // QC b1 + b2 fused on the matrix cores; the removed rows are packed first
     const int Wr = rmv_W;
-    compact_cols(At.b(), nullptr, M, d_src_rmv, Wr, At_rmvc, st);
-    compact_cols(Tt.b(), nullptr, T, d_src_rmv, Wr, Tt_rmvc, st);
+    compact_cols(At.b(), nullptr, M, d_src_rmv, Wr, At_rmvc, Wr, st);
+    compact_cols(Tt.b(), nullptr, T, d_src_rmv, Wr, Tt_rmvc, Wr, st);
     AttynArgs x;
     const uint8_t* Arc = (const uint8_t*)At_rmvc;
     const uint8_t* Trc = (const uint8_t*)Tt_rmvc;
@@ -977,7 +1015,7 @@ void Run::reset()
   push_ctl();
   HIPCHK(hipStreamSynchronize(st));
   trace.clear(); xdrop_a.clear(); xdrop_b.clear();
-  n_iter = 0; inited = false; ops = 0; init_s = 0; loop_s = 0; qr_fresh = false;
+  n_iter = 0; inited = false; ops = 0; init_s = 0; loop_s = 0;
   for (auto& k : prof.stats) { k.launches = 0; k.ms = 0; k.work = 0; }
 }
 

@@ -80,6 +80,8 @@ struct Run {
   int rmv_W = 0, rmv_Wmax = 0;
   int* d_src_rmv = nullptr; uint32_t* d_cmask_rmv = nullptr;
   uint32_t *At_rmvc = nullptr, *Tt_rmvc = nullptr;
+  // ones per (1920-block segment, list position) of the matrix-core Grams' rows
+  int* gcnt[6] = {};
   Grow g_attyn, g_qrs, g_qrs2;
   // device arrays; (row, covariate) arrays are [t*rows + row]
   double *mlf = nullptr;
@@ -100,8 +102,7 @@ struct Run {
   std::vector<double> QC_sv, QR_sv; std::vector<int> QC_perm, QR_perm;
   std::vector<double> trace; std::vector<int> xdrop_a, xdrop_b;
   int n_iter = 0; bool inited = false;
-  int ww_kernel = 0;
-  bool qr_fresh = false;                    // the lrup already left the QR sums of the updated Grams                        // 0: int8 MFMA, 1: VALU bit-plane popcount
+  int ww_kernel = 0;                        // contractions: 0 int8 matrix cores, 1 VALU popcount
   long long ops = 0;
   double load_s = 0, init_s = 0, loop_s = 0;
   std::string err;

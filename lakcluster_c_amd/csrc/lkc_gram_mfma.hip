@@ -1,0 +1,223 @@
+// lkc_gram_mfma.hip -- centred AnZt_vv Grams (AnZt_vv.c:62-95) on the int8 matrix cores.
+//
+// The reference forms, per pair (a, z), dtmp = sum_b D_b pop((a ^ z) & m)_b over the
+// 1920-bit blocks b in order.  Per block
+//     pop((a ^ z) & m)_b = |a & m|_b + |z & m|_b - 2 sum_{n in b} (a & m)(n) z(n)
+// and the last sum is an int8 MFMA contraction of 0/1 values.  The kernel reads the
+// packed bit rows (1 bit per column, 8x less traffic than int8 rows) and spreads
+// every 16-bit half word into 16 bytes in registers; A and B operands use the same
+// bit-to-slot map, so the contraction is exact.  The double sum over the blocks keeps
+// the reference's order; the epilogue is the reference's expression.
+#include "lkc_kernels.h"
+
+namespace lkc {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+static inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+
+// cnt[q*ld + i] = pop(X row rows[i] & mask) over the words of segment q
+// (seg == NULL: segment q = block q, words 60q .. 60q+59; mask == NULL: no mask)
+__global__ __launch_bounds__(64) void k_seg_pop(const uint8_t* X, size_t rb, const int* rows, int nrows,
+                                                const uint32_t* mask, const int* seg, int nseg, int* cnt, long long ld)
+{
+  const int i = blockIdx.x;
+  if (i >= nrows) return;
+  const uint32_t* xr = (const uint32_t*)(X + (size_t)(rows ? rows[i] : i) * rb);
+  for (int q = 0; q < nseg; q++) {
+    const int w0 = seg ? seg[3 * q] : kPopWords * q, nw = seg ? seg[3 * q + 1] : kPopWords;
+    int c = 0;
+    for (int w = threadIdx.x; w < nw; w += 64) {
+      uint32_t v = xr[w0 + w];
+      if (mask) v &= mask[w0 + w];
+      c += __builtin_popcount(v);
+    }
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if (threadIdx.x == 0) cnt[(long long)q * ld + i] = c;
+  }
+}
+void seg_counts(const Bits& X, const int* rows, int nrows, const uint32_t* mask, const int* seg, int nseg,
+                int* cnt, long long ld, hipStream_t st)
+{
+  if (nrows <= 0 || nseg <= 0) return;
+  hipLaunchKernelGGL(k_seg_pop, dim3(nrows), dim3(64), 0, st, X.p, X.rb, rows, nrows, mask, seg, nseg, cnt, ld);
+}
+
+// 16 bits -> 16 bytes 0/1: byte 4d + j of the fragment = bit 4d + j of h
+__device__ __forceinline__ v4i spread16(uint32_t h)
+{
+  v4i f;
+#pragma unroll
+  for (int d = 0; d < 4; d++) f[d] = (int)((((h >> (4 * d)) & 15u) * 0x204081u) & 0x01010101u);
+  return f;
+}
+
+#define GM_QG 48   // segments whose counts are staged in LDS at a time
+
+// Block tile: 128 z (M, the MFMA rows) x 128 a (N, the MFMA columns); 4 waves of 64x64.
+// Columns a run along the lanes, so the update of out[z*ldo + a] is coalesced.
+// Segment lengths are multiples of 4 words (128 columns, four MFMA steps): lane (r, h)
+// loads words 2h and 2h+1 of its rows and feeds one half word to each step, the lanes
+// h = 0 and 1 covering the four words.  Two chunks are in flight while one is used.
+template <int OPSUB>
+__global__ __launch_bounds__(256) void k_gram_mfma(GramMArgs g)
+{
+  __shared__ int czs[GM_QG][128], cas[GM_QG][128];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, hg = lane >> 5, r = lane & 31;
+  const int wm = wave & 1, wn = wave >> 1;
+  // XCD-aware tile order: consecutive block ids go to different XCDs; each XCD takes a band
+  const int ntx = (g.na + 127) / 128, nty = (g.nz + 127) / 128;
+  const int nb = ntx * nty, bid = blockIdx.x;
+  const int per_xcd = (nb + 7) / 8;
+  const int lin = (bid % 8) * per_xcd + bid / 8;
+  if (lin >= nb) return;
+  const int tyi = lin / ntx, txi = lin % ntx;
+  const int zt0 = tyi * 128, at0 = txi * 128;
+  const int zb = zt0 + 64 * wm, ab = at0 + 64 * wn;
+  // lane (r, h) reads words 2h, 2h+1 of every 4-word chunk of its rows
+  const uint2* Zr[2];
+  const uint2* Ar[2];
+#pragma unroll
+  for (int i = 0; i < 2; i++) Zr[i] = (const uint2*)(g.Z + (size_t)g.zlist[min(zb + 32 * i + r, g.nz - 1)] * g.rbZ) + hg;
+#pragma unroll
+  for (int j = 0; j < 2; j++) Ar[j] = (const uint2*)(g.A + (size_t)g.alist[min(ab + 32 * j + r, g.na - 1)] * g.rbA) + hg;
+  const uint2* mk = g.mask ? (const uint2*)g.mask + hg : nullptr;
+  const double* D = g.ajdk + (size_t)5 * g.pcols;
+  double dsum[2][2][16];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int v = 0; v < 16; v++) dsum[i][j][v] = 0.0;
+  // total chunks (4 words each); two chunks are in flight ahead of the one in use
+  const int wtot = g.seg ? (g.nseg > 0 ? g.seg[3 * (g.nseg - 1)] + g.seg[3 * (g.nseg - 1) + 1] : 0) : kPopWords * g.nseg;
+  const int nch = wtot / 4;
+  const uint2 ones = make_uint2(~0u, ~0u);
+  uint2 z1[2], a1[2], m1, z2[2], a2[2], m2;
+  auto fetch = [&](int c, uint2* zz, uint2* aa, uint2& mm) {
+    const int cc = 2 * min(c, nch - 1);
+#pragma unroll
+    for (int i = 0; i < 2; i++) zz[i] = Zr[i][cc];
+#pragma unroll
+    for (int j = 0; j < 2; j++) aa[j] = Ar[j][cc];
+    mm = mk ? mk[cc] : ones;
+  };
+  if (nch > 0) { fetch(0, z1, a1, m1); fetch(1, z2, a2, m2); }
+  int ch = 0;
+  for (int q = 0; q < g.nseg; q++) {
+    const int qq = q % GM_QG;
+    if (qq == 0) {
+      __syncthreads();
+      for (int e = tid; e < GM_QG * 128; e += 256) {
+        const int s = e / 128, p = e % 128, qs = q + s;
+        int vz = 0, va = 0;
+        if (qs < g.nseg) {
+          if (zt0 + p < g.nz) vz = g.cz[(long long)qs * g.ldcz + zt0 + p];
+          if (at0 + p < g.na) va = g.ca[(long long)qs * g.ldca + at0 + p];
+        }
+        czs[s][p] = vz; cas[s][p] = va;
+      }
+      __syncthreads();
+    }
+    const int qch = g.seg ? g.seg[3 * q + 1] / 4 : kPopWords / 4;
+    v16i acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int v = 0; v < 16; v++) acc[i][j][v] = 0;
+    for (int c = 0; c < qch; c++, ch++) {
+      const uint2 zc0 = z1[0], zc1 = z1[1], ac0 = a1[0], ac1 = a1[1], mc = m1;
+      z1[0] = z2[0]; z1[1] = z2[1]; a1[0] = a2[0]; a1[1] = a2[1]; m1 = m2;
+      if (ch + 2 < nch) fetch(ch + 2, z2, a2, m2);
+      const uint32_t zw[2][2] = { { zc0.x, zc0.y }, { zc1.x, zc1.y } };
+      const uint32_t aw[2][2] = { { ac0.x & mc.x, ac0.y & mc.y }, { ac1.x & mc.x, ac1.y & mc.y } };
+      // step p takes half (p & 1) of word (p >> 1) of the lane's pair
+#pragma unroll
+      for (int p = 0; p < 4; p++) {
+        v4i af[2], bf[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++) af[i] = spread16((zw[i][p >> 1] >> (16 * (p & 1))) & 0xFFFFu);
+#pragma unroll
+        for (int j = 0; j < 2; j++) bf[j] = spread16((aw[j][p >> 1] >> (16 * (p & 1))) & 0xFFFFu);
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+          for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+      }
+    }
+    // segment epilogue: dtmp += D_b * pop((a ^ z) & m)_b
+    const double wb = D[g.seg ? g.seg[3 * q + 2] : q];
+    int caj[2];
+#pragma unroll
+    for (int j = 0; j < 2; j++) caj[j] = cas[qq][64 * wn + 32 * j + r];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int v4 = 0; v4 < 4; v4++) {
+        const int4 cz4 = *(const int4*)&czs[qq][64 * wm + 32 * i + 8 * v4 + 4 * hg];
+        const int czv[4] = { cz4.x, cz4.y, cz4.z, cz4.w };
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+          for (int j = 0; j < 2; j++) {
+            const int x = czv[u] + caj[j] - 2 * acc[i][j][4 * v4 + u];
+            dsum[i][j][4 * v4 + u] += (double)wb * (double)(unsigned)x;
+          }
+      }
+  }
+  // Epilogue.  Every load below reads a valid (clamped) address without a branch, so
+  // they all go out together; only the stores are predicated.
+  const double a0d1 = g.base16[5], a2d1 = g.base16[7];
+  int zrow[2][16];
+  double xzv[2][16];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+      zrow[i][v] = g.zlist[min(zb + 32 * i + 8 * (v / 4) + 4 * hg + (v % 4), g.nz - 1)];
+      xzv[i][v] = g.xz[zrow[i][v]];
+    }
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const int ac = ab + 32 * j + r, acl = min(ac, g.na - 1);
+    const int aor = g.alist[acl];
+    const double xav = g.xa[aor];
+    const long long aidx = g.a_orig ? aor : acl;
+    double val[2][16];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int v = 0; v < 16; v++) {
+        const int zc = min(zb + 32 * i + 8 * (v / 4) + 4 * hg + (v % 4), g.nz - 1);
+        double o = a0d1;
+        o -= (2 * dsum[i][j][v]);
+        o -= xav + xzv[i][v] - a2d1;
+        const long long zidx = g.z_orig ? zrow[i][v] : zc;
+        val[i][v] = o;
+        if (OPSUB) val[i][v] = g.out[zidx * g.ldo + aidx] - o;
+      }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int v = 0; v < 16; v++) {
+        const int zc = zb + 32 * i + 8 * (v / 4) + 4 * hg + (v % 4);
+        const long long zidx = g.z_orig ? zrow[i][v] : min(zc, g.nz - 1);
+        if (ac < g.na && zc < g.nz) g.out[zidx * g.ldo + aidx] = val[i][v];
+      }
+  }
+}
+
+void gram_mfma(const GramMArgs& g, hipStream_t st)
+{
+  if (g.na <= 0 || g.nz <= 0) return;
+  const long long nb = (long long)cdiv(g.na, 128) * cdiv(g.nz, 128);
+  const long long grid = ((nb + 7) / 8) * 8;
+  if (g.op_sub) hipLaunchKernelGGL(k_gram_mfma<1>, dim3((unsigned)grid), dim3(256), 0, st, g);
+  else hipLaunchKernelGGL(k_gram_mfma<0>, dim3((unsigned)grid), dim3(256), 0, st, g);
+}
+
+}  // namespace lkc

@@ -42,8 +42,8 @@ void gram(const Bits& A, const int* alist, int na, const Bits& Z, const int* zli
           const int* seg = nullptr, int nseg = 0);
 // Column compaction for grams over a sparse column set: seg (3 ints per non-empty
 // 1920-block: word offset, word count, block index) describes the compacted rows.
-// out row i, word w = bits of X row (rows ? rows[i] : i) at columns src[32w .. 32w+31] (-1: none).
-void compact_cols(const Bits& X, const int* rows, int nrows, const int* src, int W, uint32_t* out, hipStream_t st);
+// out[i*ldw + w] = bits of X row (rows ? rows[i] : i) at columns src[32w .. 32w+31] (-1: none), w < W.
+void compact_cols(const Bits& X, const int* rows, int nrows, const int* src, int W, uint32_t* out, int ldw, hipStream_t st);
 
 // xcalc.  Values v(i,j) = src[ridx(i)*sn + cidx(j)*sw] for i<nn, j<nw, with
 // ridx(i) = rlist ? rlist[i] : i (same for cidx).  scal receives
@@ -120,19 +120,24 @@ struct AttynArgs {
 size_t attyn_mfma_work_bytes(int ny, int nt);
 void attyn_mfma(const AttynArgs& a, void* work, size_t work_bytes, hipStream_t st);
 
-// QR stage of the low-rank update fused with the QR row sums (column-compacted rows,
-// at most kGramQrMaxW words): X[z*ldx + a] -= centred Gram(a, z) over the compacted
-// columns, then qr[tlist[t]*ldq + a] = sum_z (x*S_t(z))*x over the updated values.
-constexpr int kGramQrMaxW = 120;
-struct GramQrArgs {
-  Bits A; const int* alist; int na; Bits Z; const int* zlist; int nz;
-  const uint32_t* mask; int W; const int* seg; int nseg;
+// Centred Grams on the int8 matrix cores, over packed bit rows.  A column layout is
+// either the full one (seg == NULL: segment q = 1920-block q, words 60q .. 60q+59,
+// with the column mask applied) or a compacted one (segments of 3 ints: word offset,
+// word count, block; word counts multiples of 4; rows hold only the kept columns, the
+// rest of every word zero; mask NULL).  seg_counts: cnt[q*ld + i] = ones of
+// (row rows[i] & mask) in segment q, indexed by list position.  gram_mfma: output
+// and semantics as gram() with centred = 1; ca/cz are seg_counts of alist/zlist.
+struct GramMArgs {
+  const uint8_t* A; size_t rbA; const int* alist; int na;
+  const uint8_t* Z; size_t rbZ; const int* zlist; int nz;
+  const uint32_t* mask; const int* seg; int nseg;
+  const int* ca; long long ldca; const int* cz; long long ldcz;
   const double* ajdk; int pcols; const double* base16; const double* xa; const double* xz;
-  double* X; long long ldx;
-  const signed char* sg; int nzp; const int* tlist; int nt;
-  double* qr; long long ldq;
+  double* out; long long ldo; int a_orig; int z_orig; int op_sub;
 };
-void gram_qr(const GramQrArgs& g, hipStream_t st);
+void seg_counts(const Bits& X, const int* rows, int nrows, const uint32_t* mask, const int* seg, int nseg,
+                int* cnt, long long ld, hipStream_t st);
+void gram_mfma(const GramMArgs& g, hipStream_t st);
 // sg[t*nzp + i] = S_t(zlist[i]) (+-1) for the QR sums; returns nzp.  sgbuf: qr_sign_bytes(nz, nt)
 int qr_signs(const Bits& Sc, const int* tlist, int nt, const int* zlist, int nz, signed char* sgbuf, hipStream_t st);
 

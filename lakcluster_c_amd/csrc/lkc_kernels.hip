@@ -200,7 +200,7 @@ void gram(const Bits& A, const int* alist, int na, const Bits& Z, const int* zli
 // column compaction: out word w of row r gathers bits src[32w + q] (q = 0..31, MSB-first
 // within each byte; -1 = padding, 0); rows rows[i] (or i) of X -> row i of out.
 __global__ void k_compact(const uint8_t* X, size_t rb, const int* rows, int nrows, const int* src, int W,
-                          uint32_t* out)
+                          uint32_t* out, int ldw)
 {
   long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long long)nrows * W) return;
@@ -211,13 +211,13 @@ __global__ void k_compact(const uint8_t* X, size_t rb, const int* rows, int nrow
     int c = src[32 * w + q];
     if (c >= 0 && bit_at(xr, c)) v |= 1u << ((q >> 3) * 8 + (7 - (q & 7)));
   }
-  out[(size_t)i * W + w] = v;
+  out[(size_t)i * ldw + w] = v;
 }
-void compact_cols(const Bits& X, const int* rows, int nrows, const int* src, int W, uint32_t* out, hipStream_t st)
+void compact_cols(const Bits& X, const int* rows, int nrows, const int* src, int W, uint32_t* out, int ldw, hipStream_t st)
 {
   long long n = (long long)nrows * W;
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_compact, dim3(cdiv(n, 256)), dim3(256), 0, st, X.p, X.rb, rows, nrows, src, W, out);
+  hipLaunchKernelGGL(k_compact, dim3(cdiv(n, 256)), dim3(256), 0, st, X.p, X.rb, rows, nrows, src, W, out, ldw);
 }
 
 // ------------------------------------------------------------------
@@ -508,76 +508,73 @@ __global__ void k_qr_signs(const uint8_t* Sc, size_t rbS, const int* tlist, int 
   sg[e] = i < nz ? (signed char)pm_at(Sc + (size_t)tlist[t] * rbS, zlist[i]) : 0;
 }
 
-// One block per 64 rows a: all 256 threads stream a 128(z) x 64(a) tile of X into
-// LDS while wave w adds the previous tile into the ordered sums of covariate w (and
-// w+4): one sequential chain per (a, t), in ascending z.
-#define QR_ZC 128
+// One block per 16 rows a (QR_AC): all 256 threads stream a 256(z) x 16(a) tile of X
+// through registers into LDS while thread (t, a) of the first waves adds the previous
+// tile into its ordered sum: one sequential chain per (a, t), in ascending z.  Small
+// blocks keep four of them on a compute unit, so enough loads are in flight.
+#define QR_ZC 256
+#define QR_AC 16
 __global__ __launch_bounds__(256) void k_qr(const double* X, long long ldx, const int* alist, int na, const int* zlist,
                                             int nz, const signed char* sg, int nzp, const int* tlist, int nt,
                                             double* out, long long ldo)
 {
-  __shared__ double tile[2][QR_ZC][64];
-  __shared__ signed char sgn[2][kMaxT][QR_ZC];
-  const int tid = threadIdx.x, i0 = blockIdx.x * 64;
-  const int ai = tid & 63, wave = tid >> 6;
-  const bool alive = i0 + ai < na;
-  const long long acol = alive ? alist[i0 + ai] : 0;
+  __shared__ double tile[QR_ZC][QR_AC];
+  __shared__ signed char sgn[kMaxT][QR_ZC];
+  __shared__ int zrow[2][QR_ZC];          // zlist of the next two chunks
+  const int tid = threadIdx.x, i0 = blockIdx.x * QR_AC;
+  // loader role: column lc, rows lr + 16 j
+  const int lc = tid & (QR_AC - 1), lr = tid / QR_AC;
+  const long long lcol = alist[min(i0 + lc, na - 1)];
+  // chain role: covariate ct, column cc
+  const int ct = tid / QR_AC, cc = tid & (QR_AC - 1);
+  const bool chain = ct < nt && i0 + cc < na;
   const int nch = (nz + QR_ZC - 1) / QR_ZC;
-  double reg[QR_ZC / 4];
-  signed char sreg[kMaxT * QR_ZC / 256];
+  double reg[QR_ZC / 16];
+  signed char sreg[kMaxT];
+  // The row indices come from LDS and every address is valid (clamped), so the 16
+  // loads of a thread go out back to back.  Entries past nz or na are never read.
   auto load = [&](int c) {
     const int z0 = c * QR_ZC;
+    const int* zr = zrow[c & 1];
 #pragma unroll
-    for (int j = 0; j < QR_ZC / 4; j++) {
-      const int zi = (tid >> 6) + 4 * j;
-      reg[j] = (alive && z0 + zi < nz) ? X[(long long)zlist[z0 + zi] * ldx + acol] : 0.0;
-    }
+    for (int j = 0; j < QR_ZC / 16; j++) reg[j] = X[(long long)zr[lr + 16 * j] * ldx + lcol];
 #pragma unroll
-    for (int j = 0; j < kMaxT * QR_ZC / 256; j++) {
-      const int e = tid + 256 * j, t = e / QR_ZC, zi = e % QR_ZC;
-      sreg[j] = (t < nt) ? sg[(long long)t * nzp + z0 + zi] : 0;
-    }
+    for (int t = 0; t < kMaxT; t++) sreg[t] = sg[(long long)min(t, nt - 1) * nzp + z0 + tid];
   };
-  auto store = [&](int b) {
+  auto zload = [&](int c) { return zlist[min(c * QR_ZC + tid, nz - 1)]; };
+  auto store = [&]() {
 #pragma unroll
-    for (int j = 0; j < QR_ZC / 4; j++) tile[b][(tid >> 6) + 4 * j][ai] = reg[j];
+    for (int j = 0; j < QR_ZC / 16; j++) tile[lr + 16 * j][lc] = reg[j];
 #pragma unroll
-    for (int j = 0; j < kMaxT * QR_ZC / 256; j++) {
-      const int e = tid + 256 * j, t = e / QR_ZC, zi = e % QR_ZC;
-      sgn[b][t][zi] = sreg[j];
-    }
+    for (int t = 0; t < kMaxT; t++) sgn[t][tid] = sreg[t];
   };
-  double acc0 = 0.0, acc1 = 0.0;
-  const int ta = wave, tb = wave + 4;
-  if (nch > 0) { load(0); store(0); }
+  double acc = 0.0;
+  if (nch > 0) {
+    zrow[0][tid] = zload(0);
+    __syncthreads();
+    load(0);
+    int zn1 = zload(1);
+    store();
+    zrow[1][tid] = zn1;
+  }
   __syncthreads();
   for (int c = 0; c < nch; c++) {
-    const int b = c & 1;
-    if (c + 1 < nch) load(c + 1);
+    int zn2 = 0;
+    if (c + 1 < nch) { load(c + 1); zn2 = zload(c + 2); }
     const int zn = min(QR_ZC, nz - c * QR_ZC);
-    if (ta < nt) {
+    if (chain) {
 #pragma unroll 8
       for (int zi = 0; zi < zn; zi++) {
-        const double x = tile[b][zi][ai];
-        const int vS = sgn[b][ta][zi];
-        acc0 += x * vS * x;
+        const double x = tile[zi][cc];
+        const int vS = sgn[ct][zi];
+        acc += x * vS * x;
       }
     }
-    if (tb < nt) {
-#pragma unroll 8
-      for (int zi = 0; zi < zn; zi++) {
-        const double x = tile[b][zi][ai];
-        const int vS = sgn[b][tb][zi];
-        acc1 += x * vS * x;
-      }
-    }
-    if (c + 1 < nch) store(b ^ 1);
+    __syncthreads();
+    if (c + 1 < nch) { store(); zrow[c & 1][tid] = zn2; }
     __syncthreads();
   }
-  if (alive) {
-    if (ta < nt) out[(long long)tlist[ta] * ldo + acol] = acc0;
-    if (tb < nt) out[(long long)tlist[tb] * ldo + acol] = acc1;
-  }
+  if (chain) out[(long long)tlist[ct] * ldo + alist[i0 + cc]] = acc;
 }
 void qr(const double* X, long long ldx, const int* alist, int na, const int* zlist, int nz,
         const Bits& Sc, const int* tlist, int nt, double* out, long long ldo, signed char* sgbuf, hipStream_t st)
@@ -586,7 +583,7 @@ void qr(const double* X, long long ldx, const int* alist, int na, const int* zli
   const int nzp = ((nz + QR_ZC - 1) / QR_ZC) * QR_ZC + QR_ZC;
   if (nt > 0 && nzp > 0)
     hipLaunchKernelGGL(k_qr_signs, dim3(cdiv((long long)nt * nzp, 256)), dim3(256), 0, st, Sc.p, Sc.rb, tlist, nt, zlist, nz, nzp, sgbuf);
-  hipLaunchKernelGGL(k_qr, dim3(cdiv(na, 64)), dim3(256), 0, st, X, ldx, alist, na, zlist, nz, sgbuf, nzp, tlist, nt, out, ldo);
+  hipLaunchKernelGGL(k_qr, dim3(cdiv(na, QR_AC)), dim3(256), 0, st, X, ldx, alist, na, zlist, nz, sgbuf, nzp, tlist, nt, out, ldo);
 }
 int qr_signs(const Bits& Sc, const int* tlist, int nt, const int* zlist, int nz, signed char* sgbuf, hipStream_t st)
 {

@@ -68,7 +68,7 @@ def lib():
         "lkc_bcc_debug_get": (C.c_long, [vp, cp, dp, C.c_long]),
         "lkc_bcc_reset": (C.c_int, [vp]),
         "lkc_bcc_set_profiling": (C.c_int, [vp, C.c_int]),
-        "lkc_bcc_set_ww_kernel": (C.c_int, [vp, C.c_int]),
+        "lkc_bcc_set_kernels": (C.c_int, [vp, C.c_int]),
         "lkc_bcc_kernel_stats": (C.c_int, [vp, cp, C.POINTER(C.c_longlong), dp, dp]),
         "lkc_An_ajdk_v": (C.c_int, [u8p, C.c_size_t, C.c_int, ip, C.c_int, u8p, C.c_int, dp, dp]),
         "lkc_AnZt_vv": (C.c_int, [u8p, C.c_size_t, C.c_int, ip, C.c_int, u8p, C.c_size_t, C.c_int, ip, C.c_int,
@@ -166,9 +166,9 @@ class Bcc:
     def set_profiling(self, on: bool = True) -> None:
         _check(lib().lkc_bcc_set_profiling(self._h, 1 if on else 0), self._h)
 
-    def set_ww_kernel(self, which: str) -> None:
-        """'mfma' (int8 matrix cores, default) or 'valu' (bit-plane popcount)."""
-        _check(lib().lkc_bcc_set_ww_kernel(self._h, {"mfma": 0, "valu": 1}[which]), self._h)
+    def set_kernels(self, which: str) -> None:
+        """Contractions on 'mfma' (int8 matrix cores, default) or 'valu' (popcount)."""
+        _check(lib().lkc_bcc_set_kernels(self._h, {"mfma": 0, "valu": 1}[which]), self._h)
 
     def kernel_stats(self, name: str) -> Tuple[int, float, float]:
         """(launches, device ms, algorithmic work) of a timed kernel group."""

@@ -18,16 +18,16 @@ def _read(path):
         return f.read()
 
 
-@pytest.mark.parametrize("ww", ["mfma", "valu"])
+@pytest.mark.parametrize("kern", ["mfma", "valu"])
 @pytest.mark.parametrize("case", CASES)
-def test_elimination_matches_oracle(case, ww, datasets, oracle):
+def test_elimination_matches_oracle(case, kern, datasets, oracle):
     from lakcluster_c_amd import lkc
     ds = datasets[case]
     text = _read(ds.in_path)
-    print(f"case {case} ww {ww}: A {ds.nA}x{ds.ncols} Z {ds.nZ} T {ds.T} B {ds.B}", flush=True)
+    print(f"case {case} kernels {kern}: A {ds.nA}x{ds.ncols} Z {ds.nZ} T {ds.T} B {ds.B}", flush=True)
     ref = oracle.run_text(text)
     with lkc.Bcc(text) as run:
-        run.set_ww_kernel(ww)
+        run.set_kernels(kern)
         run.run()
         tr = run.trace()
         xa, xb = run.xdrop()
