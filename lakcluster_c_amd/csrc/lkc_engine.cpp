@@ -527,13 +527,11 @@ void Run::st_ajdk()
 void Run::st_ZtSn()
 {
   mask_lf(Ar_j, psize(NA), nullptr, mlf + 3 * 16, st);
-  gram(At.b(), kl.d, kl.n(), Tt.b(), tl.d, tl.n(), Ar_j, psize(NA), nullptr, mlf + 3 * 16,
-       nullptr, nullptr, 0, lf_AtTn, M, 1, 1, 0, st);
+  pm_few(At.b(), kl.d, kl.n(), Tt.b(), tl.d, tl.n(), Ar_j, psize(NA), mlf + 3 * 16, lf_AtTn, M, st);
   ops += (long long)kl.n() * tl.n() * (long long)bsize(NA) * 8;
   if (has_Z) {
     mask_lf(Zr_j, psize(NZ), nullptr, mlf + 4 * 16, st);
-    gram(Zt.b(), kl.d, kl.n(), St.b(), tl.d, tl.n(), Zr_j, psize(NZ), nullptr, mlf + 4 * 16,
-         nullptr, nullptr, 0, lf_ZtSn, M, 1, 1, 0, st);
+    pm_few(Zt.b(), kl.d, kl.n(), St.b(), tl.d, tl.n(), Zr_j, psize(NZ), mlf + 4 * 16, lf_ZtSn, M, st);
     ops += (long long)kl.n() * tl.n() * (long long)bsize(NZ) * 8;
   }
 }
@@ -641,9 +639,14 @@ void Run::st_single()
   uint32_t* cz = (uint32_t*)g_cz.need(dev, sizeof(uint32_t) * (size_t)NA);
   uint8_t* planes = (uint8_t*)g_planes.need(dev, (size_t)c.B * prb);
   xcalc(x6, 1, 0, ar.d, nullptr, ar.n(), 1, ar.d, c.B, c.D_MLT, cz, NA, scal, xwork, st);
-  cz_planes(cz, NA, 1, NA, Ar_j, c.B, planes, prb, st);
   ops += (long long)ar.n() * (c.B + 1) * 8;
-  an_ztsn(At.b(), kl.d, kl.n(), Ar_j, planes, prb, Ar_j, 1, d_zero, c.B, scal, At_An_a1d1, 0, st);
+  if (ww_kernel == 0) {
+    size_t wb = ztsn_mfma_work_bytes(1, prb, c.B);
+    ztsn_mfma(At.b(), kl.d, kl.n(), Ar_j, cz, NA, NA, 1, d_zero, c.B, scal, At_An_a1d1, 0, g_ztsn.need(dev, wb), wb, st);
+  } else {
+    cz_planes(cz, NA, 1, NA, Ar_j, c.B, planes, prb, st);
+    an_ztsn(At.b(), kl.d, kl.n(), Ar_j, planes, prb, Ar_j, 1, d_zero, c.B, scal, At_An_a1d1, 0, st);
+  }
   ops += (long long)kl.n() * (c.B + 1) * (long long)prb * 8;
   // ordered sums over the active rows
   fill(sc, 6 * (long long)T + 2, 0.0, st);
@@ -690,8 +693,13 @@ void Run::st_double()
     double* out = pass == 0 ? An_a2d2_ZtSn : An_a2d2_AtTn;
     scale_rows(src, M, kl.d, nk, tl.d, nt, a2d2, L, st);
     xcalc(L, 1, nk, nullptr, nullptr, nk, nt, kl.d, c.B, c.D_MLT, cz, M, scal, xwork, st);
-    cz_planes(cz, M, nt, M, Ac_j, c.B, planes, prb, st);
-    an_ztsn(An.b(), ar.d, ar.n(), Ac_j, planes, prb, Ac_j, nt, tl.d, c.B, scal, out, NA, st);
+    if (ww_kernel == 0) {
+      size_t wb = ztsn_mfma_work_bytes(nt, prb, c.B);
+      ztsn_mfma(An.b(), ar.d, ar.n(), Ac_j, cz, M, M, nt, tl.d, c.B, scal, out, NA, g_ztsn.need(dev, wb), wb, st);
+    } else {
+      cz_planes(cz, M, nt, M, Ac_j, c.B, planes, prb, st);
+      an_ztsn(An.b(), ar.d, ar.n(), Ac_j, planes, prb, Ac_j, nt, tl.d, c.B, scal, out, NA, st);
+    }
     ops += (long long)ar.n() * nt * (c.B + 1) * (long long)prb * 8;
   }
 }
@@ -905,15 +913,15 @@ void Run::lrup()
   mask_lf(Ar_rtn, sA, nullptr, mlf_nrtn, st);
   mask_lf(Ar_rmv, sA, nullptr, mlf_nrmv, st);
   const long long rbn = (long long)At.rb * 8;
-  gram(At.b(), crt.d, crt.n(), Tt.b(), tl.d, tl.n(), Ar_rtn, sA, nullptr, mlf_nrtn, nullptr, nullptr, 0, attn, M, 1, 1, 0, st);
-  gram(At.b(), crm.d, crm.n(), Tt.b(), tl.d, tl.n(), Ar_rtn, sA, nullptr, mlf_nrtn, nullptr, nullptr, 0, jttn, M, 1, 1, 0, st);
-  gram(At.b(), crt.d, crt.n(), Tt.b(), tl.d, tl.n(), Ar_rmv, sA, nullptr, mlf_nrmv, nullptr, nullptr, 0, ktrn, M, 1, 1, 0, st);
-  gram(At.b(), crm.d, crm.n(), Tt.b(), tl.d, tl.n(), Ar_rmv, sA, nullptr, mlf_nrmv, nullptr, nullptr, 0, htrn, M, 1, 1, 0, st);
+  pm_few(At.b(), crt.d, crt.n(), Tt.b(), tl.d, tl.n(), Ar_rtn, sA, mlf_nrtn, attn, M, st);
+  pm_few(At.b(), crm.d, crm.n(), Tt.b(), tl.d, tl.n(), Ar_rtn, sA, mlf_nrtn, jttn, M, st);
+  pm_few(At.b(), crt.d, crt.n(), Tt.b(), tl.d, tl.n(), Ar_rmv, sA, mlf_nrmv, ktrn, M, st);
+  pm_few(At.b(), crm.d, crm.n(), Tt.b(), tl.d, tl.n(), Ar_rmv, sA, mlf_nrmv, htrn, M, st);
   ops += 2LL * (crt.n() + crm.n()) * tl.n() * rbn;
   if (has_Z) {
     mask_lf(Zr_j, sZ, nullptr, mlf_nz, st);
-    gram(Zt.b(), crt.d, crt.n(), St.b(), tl.d, tl.n(), Zr_j, sZ, nullptr, mlf_nz, nullptr, nullptr, 0, ztsn, M, 1, 1, 0, st);
-    gram(Zt.b(), crm.d, crm.n(), St.b(), tl.d, tl.n(), Zr_j, sZ, nullptr, mlf_nz, nullptr, nullptr, 0, vtsn, M, 1, 1, 0, st);
+    pm_few(Zt.b(), crt.d, crt.n(), St.b(), tl.d, tl.n(), Zr_j, sZ, mlf_nz, ztsn, M, st);
+    pm_few(Zt.b(), crm.d, crm.n(), St.b(), tl.d, tl.n(), Zr_j, sZ, mlf_nz, vtsn, M, st);
     ops += (long long)(crt.n() + crm.n()) * tl.n() * (long long)Zt.rb * 8;
   }
   double *vtszn = mt[10], *jttan = mt[11], *htrkn = mt[12], *hhtrkn = mt[13];

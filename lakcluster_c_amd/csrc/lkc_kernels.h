@@ -83,6 +83,14 @@ void an_ztsn(const Bits& A, const int* alist, int na, const uint32_t* amask,
              const uint8_t* planes, size_t prb, const uint32_t* row0, int ns, const int* slist, int B,
              const double* scal, double* out, long long ldo, hipStream_t st);
 
+// An_ZtSn_ww on the int8 matrix cores from the quantised values themselves (no planes):
+// cz[s*ldc + n] for n < n_total, mask = the active n (the planes' row0 and amask, equal
+// at every call site).  Same output as an_ztsn.
+size_t ztsn_mfma_work_bytes(int ns, size_t prb, int B);
+void ztsn_mfma(const Bits& A, const int* alist, int na, const uint32_t* mask, const uint32_t* cz, long long ldc,
+               int n_total, int ns, const int* slist, int B, const double* scal, double* out, long long ldo,
+               void* work, size_t work_bytes, hipStream_t st);
+
 // AnZt_S_WnYt_vv with X==Y: out[tlist[t]*ldo + a] = Σ_z (X[z*ldx+a]*vS(z))*X[z*ldx+a]
 // sgbuf: qr_sign_bytes(nz, nt) bytes of device scratch
 void qr(const double* X, long long ldx, const int* alist, int na, const int* zlist, int nz,
@@ -140,6 +148,11 @@ void seg_counts(const Bits& X, const int* rows, int nrows, const uint32_t* mask,
 void gram_mfma(const GramMArgs& g, hipStream_t st);
 // sg[t*nzp + i] = S_t(zlist[i]) (+-1) for the QR sums; returns nzp.  sgbuf: qr_sign_bytes(nz, nt)
 int qr_signs(const Bits& Sc, const int* tlist, int nt, const int* zlist, int nz, signed char* sgbuf, hipStream_t st);
+
+// plain +-1 products with a few (<= kMaxT) covariate rows, as gram() with centred = 0,
+// a_orig = z_orig = 1: out[tlist[t]*ldo + alist[i]] = base16[15] - 2 pop((A_i ^ T_t) & mask)
+void pm_few(const Bits& A, const int* alist, int na, const Bits& T, const int* tlist, int nt,
+            const uint32_t* mask, int pcols, const double* base16, double* out, long long ldo, hipStream_t st);
 
 // out[i] = pop(X[rows[i]] & mask) over nwords 32-bit words (rows==NULL: row i)
 void popc_rows(const Bits& X, const int* rows, int nrows, const uint32_t* mask, int nwords, int* out, hipStream_t st);

@@ -672,4 +672,47 @@ void popc_rows(const Bits& X, const int* rows, int nrows, const uint32_t* mask, 
   hipLaunchKernelGGL(k_popc_rows, dim3(nrows), dim3(64), 0, st, X.p, X.rb, rows, nrows, mask, nwords, out);
 }
 
+// Plain +-1 products of many rows with a few covariate rows (AnZt_vv with Z = T,
+// AnZt_vv.c:62-95): out[tlist[t]*ldo + alist[i]] = base[15] - 2 pop((A_i ^ T_t) & mask).
+// One wave per row, 4 rows per block; the covariate rows stay in L1/L2.
+__global__ __launch_bounds__(256) void k_pm_few(const uint8_t* A, size_t rbA, const int* alist, int na,
+                                                const uint8_t* T, size_t rbT, const int* tlist, int nt,
+                                                const uint32_t* mask, int nwords, const double* base,
+                                                double* out, long long ldo)
+{
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (i >= na) return;
+  const uint32_t* ar = (const uint32_t*)(A + (size_t)alist[i] * rbA);
+  const uint32_t* tr[kMaxT];
+#pragma unroll
+  for (int t = 0; t < kMaxT; t++) tr[t] = (const uint32_t*)(T + (size_t)tlist[min(t, nt - 1)] * rbT);
+  int c[kMaxT];
+#pragma unroll
+  for (int t = 0; t < kMaxT; t++) c[t] = 0;
+  for (int w = lane; w < nwords; w += 64) {
+    const uint32_t a = ar[w], m = mask[w];
+#pragma unroll
+    for (int t = 0; t < kMaxT; t++) if (t < nt) c[t] += popc((a ^ tr[t][w]) & m);
+  }
+#pragma unroll
+  for (int t = 0; t < kMaxT; t++) if (t < nt) c[t] = wave_sum_i(c[t]);
+  if (lane == 0) {
+    const double a0 = base[kAjdk];
+#pragma unroll
+    for (int t = 0; t < kMaxT; t++)
+      if (t < nt) {
+        double o = a0;
+        o -= (2 * (double)c[t]);
+        out[(long long)tlist[t] * ldo + alist[i]] = o;
+      }
+  }
+}
+void pm_few(const Bits& A, const int* alist, int na, const Bits& T, const int* tlist, int nt,
+            const uint32_t* mask, int pcols, const double* base16, double* out, long long ldo, hipStream_t st)
+{
+  if (na <= 0 || nt <= 0) return;
+  hipLaunchKernelGGL(k_pm_few, dim3(cdiv(na, 4)), dim3(256), 0, st, A.p, A.rb, alist, na, T.p, T.rb, tlist, nt,
+                     mask, pcols * kPopWords, base16, out, ldo);
+}
+
 }  // namespace lkc

@@ -94,52 +94,51 @@ __global__ void k_ww_cntp(WwArgs a, int* cntp, int* llk)
   }
 }
 
-// byte x -> 8 bytes, byte j = bit (7-j) of x (the bit of element 8m+j of a packed row)
-__device__ __forceinline__ uint64_t expand_byte(uint32_t x)
+// 16 packed columns -> 16 bytes 0/1 in column order.  wr is the bit-reversed 32-bit
+// word of the packed row (bfrev: column 8b + j of the word, MSB-first in byte b, lands
+// on bit 8(3-b) + j); half h selects columns 16h .. 16h+15.  Each nibble becomes four
+// bytes with one multiply: (n * 0x204081) & 0x01010101 puts bit j of n into byte j.
+__device__ __forceinline__ v4i spread16_cols(uint32_t wr, int h)
 {
-  uint64_t r = 0;
-  for (int j = 0; j < 8; j++) r |= (uint64_t)((x >> (7 - j)) & 1u) << (8 * j);
-  return r;
+  v4i f;
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    const int sh = 24 - 16 * h - 8 * (d >> 1) + 4 * (d & 1);
+    f[d] = (int)((((wr >> sh) & 15u) * 0x204081u) & 0x01010101u);
+  }
+  return f;
 }
 
+#define WW_BROW 144   // LDS row of one w: 128 digit bytes + 16 bytes of padding (no bank conflicts)
+
+// Block: 4 waves x 32 rows k; the 32 w of a step are shared, their digits staged in LDS
+// per 128-column chunk of n (double buffered, the next chunk loaded while the current
+// one is used).  Lane (r, g) of a step s feeds columns n0 + 64g + 16s .. +15: the
+// packed bits of P, T and the mask come as one 8-byte word per lane and chunk.
 template <int NTG, int ND>
 __global__ __launch_bounds__(256) void k_ww_mfma(WwArgs a, const int8_t* Dg, long long ldn, int nwp,
                                                  const double* H, const double* bp, const long long* Gm,
                                                  const int* cntp, const int* llk)
 {
-  __shared__ uint64_t lut[256];
+  __shared__ __attribute__((aligned(16))) int8_t Bs[2][ND][32][WW_BROW];
   __shared__ long long Gs[4][32][33];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, g = lane >> 5, r = lane & 31;
-  for (int i = tid; i < 256; i += 256) lut[i] = expand_byte(i);
   const int t0 = blockIdx.y * NTG;
   const int nts = min(NTG, a.nt - t0);
   const int kbase = blockIdx.x * 128 + wave * 32;
   const int krow = min(kbase + r, a.nk - 1);
-  const uint8_t* Prow = a.P.p + (size_t)a.klist[krow] * a.P.rb;
-  const uint8_t* pm = (const uint8_t*)a.pm;
+  const uint8_t* Prow = a.P.p + (size_t)a.klist[krow] * a.P.rb + 8 * g;
+  const uint8_t* pm = (const uint8_t*)a.pm + 8 * g;
   const uint8_t* Trow[NTG];
 #pragma unroll
-  for (int t = 0; t < NTG; t++) Trow[t] = a.Tc.p + (size_t)a.tlist[t0 + min(t, nts - 1)] * a.Tc.rb;
-  // per-row constants of the epilogue (lane < 32 owns row kbase + lane)
-  const double mn = a.scal[2], mlt = a.scal[4];
-  const double* alpha = a.ajdk + (size_t)1 * a.pcols;
-  const int myk = min(kbase + lane, a.nk - 1);
-  const int kk = a.klist[myk], qq = a.qlist[myk];
-  const double alpha_k = alpha[kk / kPopBits], alpha_q = alpha[qq / kPopBits];
-  const uint8_t* Qrow = a.Q.p + (size_t)qq * a.Q.rb;
-  long long S = 0;
+  for (int t = 0; t < NTG; t++) Trow[t] = a.Tc.p + (size_t)a.tlist[t0 + min(t, nts - 1)] * a.Tc.rb + 8 * g;
+  // ordered sums over w of row kbase + lane (lane < 32), per covariate
+  double o[NTG];
 #pragma unroll
-  for (int i = 0; i < ND; i++) S += 1LL << (8 * i);
-  double o[NTG], bkk[NTG];
-  long long off[NTG];
-#pragma unroll
-  for (int t = 0; t < NTG; t++) {
-    o[t] = 0.0;
-    const int tt = t0 + min(t, nts - 1);
-    bkk[t] = (double)llk[(size_t)tt * a.nk + myk] * mn * mlt;
-    off[t] = 128 * S * (long long)cntp[(size_t)tt * a.nk + myk];
-  }
-  __syncthreads();
+  for (int t = 0; t < NTG; t++) o[t] = 0.0;
+  const int nchunk = (int)(ldn / 128);
+  // digit loader: piece e = tid + 256 j -> digit plane j, w row tid / 8, 16 bytes (tid % 8)
+  const int lw = tid >> 3, lp = tid & 7;
 
   for (int w0 = 0; w0 < a.nw; w0 += 32) {
     v16i acc[NTG][ND];
@@ -149,31 +148,52 @@ __global__ __launch_bounds__(256) void k_ww_mfma(WwArgs a, const int8_t* Dg, lon
       for (int i = 0; i < ND; i++)
 #pragma unroll
         for (int v = 0; v < 16; v++) acc[t][i][v] = 0;
-    const int8_t* Bcol = Dg + (long long)(w0 + r) * ldn + 64 * g;
-    for (long long n0 = 0; n0 < ldn; n0 += 128) {
-      const long long bo = n0 / 8 + 8 * g;
-      const uint64_t m8 = *(const uint64_t*)(pm + bo);
-      if (!__any(m8 != 0)) continue;
-      const uint64_t p8 = *(const uint64_t*)(Prow + bo);
+    const int8_t* Bsrc = Dg + (long long)(w0 + lw) * ldn + 16 * lp;
+    const long long bplane = (long long)nwp * ldn;
+    uint4 b0 = make_uint4(0, 0, 0, 0), b1 = b0, b2 = b0, b3 = b0;
+    uint64_t pn = 0, tn0 = 0, tn1 = 0, tn2 = 0, mk = 0;
+#define WW_BLOAD(c) do { const int8_t* q_ = Bsrc + 128LL * (c); b0 = *(const uint4*)q_; \
+      if (ND > 1) b1 = *(const uint4*)(q_ + bplane); if (ND > 2) b2 = *(const uint4*)(q_ + 2 * bplane); \
+      if (ND > 3) b3 = *(const uint4*)(q_ + 3 * bplane); } while (0)
+#define WW_BSTORE(buf) do { *(uint4*)&Bs[buf][0][lw][16 * lp] = b0; if (ND > 1) *(uint4*)&Bs[buf][ND > 1 ? 1 : 0][lw][16 * lp] = b1; \
+      if (ND > 2) *(uint4*)&Bs[buf][ND > 2 ? 2 : 0][lw][16 * lp] = b2; if (ND > 3) *(uint4*)&Bs[buf][ND > 3 ? 3 : 0][lw][16 * lp] = b3; } while (0)
+#define WW_ALOAD(c) do { const long long bo_ = 16LL * (c); pn = *(const uint64_t*)(Prow + bo_); \
+      tn0 = *(const uint64_t*)(Trow[0] + bo_); if (NTG > 1) tn1 = *(const uint64_t*)(Trow[NTG > 1 ? 1 : 0] + bo_); \
+      if (NTG > 2) tn2 = *(const uint64_t*)(Trow[NTG > 2 ? 2 : 0] + bo_); mk = *(const uint64_t*)(pm + bo_); } while (0)
+    __syncthreads();   // the previous w step's readers are done with Bs
+    if (nchunk > 0) { WW_BLOAD(0); WW_ALOAD(0); WW_BSTORE(0); }
+    __syncthreads();
+    for (int c = 0; c < nchunk; c++) {
+      const int buf = c & 1;
       uint64_t pos8[NTG];
+      pos8[0] = ~(pn ^ tn0) & mk;
+      if (NTG > 1) pos8[NTG > 1 ? 1 : 0] = ~(pn ^ tn1) & mk;
+      if (NTG > 2) pos8[NTG > 2 ? 2 : 0] = ~(pn ^ tn2) & mk;
+      uint32_t prv[NTG][2];
 #pragma unroll
-      for (int t = 0; t < NTG; t++) pos8[t] = ~(p8 ^ *(const uint64_t*)(Trow[t] + bo)) & m8;
+      for (int t = 0; t < NTG; t++) {
+        prv[t][0] = __builtin_bitreverse32((uint32_t)pos8[t]);
+        prv[t][1] = __builtin_bitreverse32((uint32_t)(pos8[t] >> 32));
+      }
+      if (c + 1 < nchunk) { WW_BLOAD(c + 1); WW_ALOAD(c + 1); }
 #pragma unroll 2
       for (int s = 0; s < 4; s++) {
         v4i bf[ND];
 #pragma unroll
-        for (int i = 0; i < ND; i++) bf[i] = *(const v4i*)(Bcol + (long long)i * nwp * ldn + n0 + 16 * s);
+        for (int i = 0; i < ND; i++) bf[i] = *(const v4i*)&Bs[buf][i][r][64 * g + 16 * s];
 #pragma unroll
         for (int t = 0; t < NTG; t++) {
-          v4i af;
-          const uint64_t e0 = lut[(pos8[t] >> (16 * s)) & 0xFF], e1 = lut[(pos8[t] >> (16 * s + 8)) & 0xFF];
-          af[0] = (int)(uint32_t)e0; af[1] = (int)(uint32_t)(e0 >> 32);
-          af[2] = (int)(uint32_t)e1; af[3] = (int)(uint32_t)(e1 >> 32);
+          const v4i af = spread16_cols(prv[t][s >> 1], s & 1);
 #pragma unroll
           for (int i = 0; i < ND; i++) acc[t][i] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af, bf[i], acc[t][i], 0, 0, 0);
         }
       }
+      if (c + 1 < nchunk) WW_BSTORE(buf ^ 1);
+      __syncthreads();
     }
+#undef WW_BLOAD
+#undef WW_BSTORE
+#undef WW_ALOAD
     // epilogue: each lane < 32 walks its row's 32 w in ascending order
     const int wn = min(32, a.nw - w0);
 #pragma unroll
@@ -188,17 +208,29 @@ __global__ __launch_bounds__(256) void k_ww_mfma(WwArgs a, const int8_t* Dg, lon
       }
       __syncthreads();
       if (lane < 32) {
+        // per-row constants (recomputed here to keep them out of the main loop's registers)
+        const double mn = a.scal[2], mlt = a.scal[4];
+        const double* alpha = a.ajdk + (size_t)1 * a.pcols;
+        const int myk = min(kbase + lane, a.nk - 1);
+        const int kk = a.klist[myk], qq = a.qlist[myk];
+        const double alpha_k = alpha[kk / kPopBits], alpha_q = alpha[qq / kPopBits];
+        const uint8_t* Qrow = a.Q.p + (size_t)qq * a.Q.rb;
+        long long S = 0;
+#pragma unroll
+        for (int i = 0; i < ND; i++) S += 1LL << (8 * i);
         const int tt = t0 + t;
+        const double bkk = (double)llk[(size_t)tt * a.nk + myk] * mn * mlt;
+        const long long off = 128 * S * (long long)cntp[(size_t)tt * a.nk + myk];
         const double at_base = bp[tt] * alpha_k;
         const uint8_t* Srow = a.Sc.p + (size_t)a.slist[tt] * a.Sc.rb;
         const double* Ht = H + (size_t)tt * a.nw;
         double acc_o = o[t];
         for (int j = 0; j < wn; j++) {
           const int w = w0 + j, ww = a.wlist[w];
-          const long long G = 2 * (Gs[wave][lane][j] + off[t]) - Gm[w];
+          const long long G = 2 * (Gs[wave][lane][j] + off) - Gm[w];
           const double at_tmp = Ht[w] * alpha_k;
           const double gd = (double)G;
-          const double tmp = (double)(bkk[t] + gd) / mlt - (double)(at_base + at_tmp) / mlt;
+          const double tmp = (double)(bkk + gd) / mlt - (double)(at_base + at_tmp) / mlt;
           const int vS = bitv(Srow, ww) ? 1 : -1, vZ = bitv(Qrow, ww) ? 1 : -1;
           acc_o += tmp * vS * (vZ - alpha_q);
         }
@@ -256,6 +288,113 @@ void ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t st, con
 #undef WW_ND
 #undef WW_LAUNCH
   if (hook && hook->after) hook->after(hook->ctx, st);
+}
+
+// ------------------------------------------------------------------
+// An_ZtSn_ww (An_ZtSn_ww.c:44-70) on the matrix cores.  The reference's plane sums
+// give the exact integer x_s = sum_{n in m} a(n) cz_s(n) (+-1 a; every partial sum
+// stays below 2^53), so any exact evaluation reproduces it:
+//     x_s = 2 (sum_i 256^i P_i + 128 S |a & m|) - sum_{n in m} cz_s(n)
+// with P_i = sum_n (a & m)(n) d_i(n) over the signed digits d_i = byte_i(cz) - 128.
+// One block per 32 rows; its 4 waves split n and add their integer sums in LDS.
+// ------------------------------------------------------------------
+// Gs[s] = sum_{n in mask} cz[s*ldc + n], n < n_total
+__global__ void k_cz_sums(const uint32_t* cz, long long ldc, int n_total, const uint32_t* mask, long long* Gs)
+{
+  const int s = blockIdx.x;
+  long long v = 0;
+  for (int n = threadIdx.x; n < n_total; n += 256)
+    if ((mask[n >> 5] >> (((n >> 3) & 3) * 8 + 7 - (n & 7))) & 1u) v += cz[(long long)s * ldc + n];
+  v = wsum64(v);
+  __shared__ long long part[4];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) Gs[s] = part[0] + part[1] + part[2] + part[3];
+}
+
+template <int ND>
+__global__ __launch_bounds__(256) void k_ztsn_mfma(const uint8_t* A, size_t rbA, const int* alist, int na,
+                                                   const uint8_t* mask, const int8_t* Dg, long long ldn, int ns,
+                                                   const long long* Gsum, const int* slist, const double* scal,
+                                                   double* out, long long ldo)
+{
+  __shared__ int Ps[32][33];
+  __shared__ int Cs[32], Ms[32];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, g = lane >> 5, r = lane & 31;
+  const int i0 = blockIdx.x * 32;
+  const uint8_t* Arow = A + (size_t)alist[min(i0 + r, na - 1)] * rbA + 8 * g;
+  const uint8_t* mrow = mask + 8 * g;
+  const int ncol = ND * ns;
+  const int8_t* Bcol = Dg + (long long)min(r, ncol - 1) * ldn + 64 * g;
+  for (int e = tid; e < 32 * 33; e += 256) (&Ps[0][0])[e] = 0;
+  if (tid < 32) { Cs[tid] = 0; Ms[tid] = 0; }
+  v16i acc;
+#pragma unroll
+  for (int v = 0; v < 16; v++) acc[v] = 0;
+  int cnt = 0, mcnt = 0;
+  const int nchunk = (int)(ldn / 128);
+  for (int c = wave; c < nchunk; c += 4) {
+    const uint64_t m8 = *(const uint64_t*)(mrow + 16LL * c);
+    const uint64_t a8 = *(const uint64_t*)(Arow + 16LL * c) & m8;
+    v4i bf[4];
+#pragma unroll
+    for (int s = 0; s < 4; s++) bf[s] = *(const v4i*)(Bcol + 128LL * c + 16 * s);
+    cnt += __popcll(a8); mcnt += __popcll(m8);
+    const uint32_t wr0 = __builtin_bitreverse32((uint32_t)a8), wr1 = __builtin_bitreverse32((uint32_t)(a8 >> 32));
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+      acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(spread16_cols(s < 2 ? wr0 : wr1, s & 1), bf[s], acc, 0, 0, 0);
+  }
+  __syncthreads();
+  // integer sums over the waves (order free): D row 8(v/4) + 4g + v%4, column r
+#pragma unroll
+  for (int v = 0; v < 16; v++) atomicAdd(&Ps[8 * (v / 4) + 4 * g + (v % 4)][r], acc[v]);
+  atomicAdd(&Cs[r], cnt);
+  atomicAdd(&Ms[r], mcnt);
+  __syncthreads();
+  const double mn = scal[2], mlt = scal[4];
+  long long S = 0;
+#pragma unroll
+  for (int i = 0; i < ND; i++) S += 1LL << (8 * i);
+  for (int e = tid; e < 32 * ns; e += 256) {
+    const int rr = e & 31, s = e >> 5;
+    if (i0 + rr >= na) continue;
+    long long P = 0;
+#pragma unroll
+    for (int i = ND - 1; i >= 0; i--) P = P * 256 + (long long)Ps[rr][i * ns + s];
+    const long long x = 2 * (P + 128 * S * (long long)Cs[rr]) - Gsum[s];
+    const double ll = (double)(2LL * Cs[rr] - Ms[rr]);
+    const double base = ll * mn * mlt;
+    const double tmp = (double)x;
+    out[(long long)slist[s] * ldo + alist[i0 + rr]] = (double)(base + tmp) / mlt;
+  }
+}
+
+size_t ztsn_mfma_work_bytes(int ns, size_t prb, int B)
+{
+  const int ndig = (B + 7) / 8;
+  return (((size_t)ndig * ns * prb * 8 + 255) & ~(size_t)255) + sizeof(long long) * (ns + 8) + 256;
+}
+
+void ztsn_mfma(const Bits& A, const int* alist, int na, const uint32_t* mask, const uint32_t* cz, long long ldc,
+               int n_total, int ns, const int* slist, int B, const double* scal, double* out, long long ldo,
+               void* work, size_t work_bytes, hipStream_t st)
+{
+  if (na <= 0 || ns <= 0) return;
+  const int ndig = (B + 7) / 8;
+  const long long ldn = (long long)A.rb * 8;
+  if (ndig * ns > 32 || work_bytes < ztsn_mfma_work_bytes(ns, A.rb, B)) { fprintf(stderr, "lkc: ztsn work buffer too small\n"); return; }
+  int8_t* Dg = (int8_t*)work;
+  long long* Gsum = (long long*)((uint8_t*)work + (((size_t)ndig * ns * ldn + 255) & ~(size_t)255));
+  const int nt_ = n_total < ldn ? n_total : (int)ldn;
+  const long long tot = (long long)ns * (ldn / 4);
+  hipLaunchKernelGGL(k_ww_digits, dim3(cdiv(tot, 256) < 65536 ? cdiv(tot, 256) : 65536), dim3(256), 0, st,
+                     cz, ldc, ns, ns, nt_, (const uint8_t*)mask, ndig, ldn, Dg);
+  hipLaunchKernelGGL(k_cz_sums, dim3(ns), dim3(256), 0, st, cz, ldc, nt_, mask, Gsum);
+#define ZT_LAUNCH(N) hipLaunchKernelGGL((k_ztsn_mfma<N>), dim3(cdiv(na, 32)), dim3(256), 0, st, A.p, A.rb, alist, na, \
+                                        (const uint8_t*)mask, Dg, ldn, ns, Gsum, slist, scal, out, ldo)
+  switch (ndig) { case 1: ZT_LAUNCH(1); break; case 2: ZT_LAUNCH(2); break; case 3: ZT_LAUNCH(3); break; default: ZT_LAUNCH(4); break; }
+#undef ZT_LAUNCH
 }
 
 }  // namespace lkc
