@@ -49,62 +49,51 @@ __global__ void k_attyn_pre(AttynArgs a, long long* cy, long long* hhay)
   }
 }
 
-// byte x -> 8 bytes for the 8 elements it packs (element j = bit 7-j): 0/1, or -1/+1
-__device__ __forceinline__ uint64_t expand01(uint32_t x)
+// 16 packed columns -> 16 bytes 0/1 in column order.  wr is the bit-reversed 32-bit
+// word of the packed row (bfrev: column 8b + j, MSB-first in byte b, lands on bit
+// 8(3-b) + j); half h selects columns 16h .. 16h+15.  Each nibble becomes four bytes
+// with one multiply: (n * 0x204081) & 0x01010101 puts bit j of n into byte j.
+__device__ __forceinline__ v4i spread16_cols(uint32_t wr, int h)
 {
-  uint64_t r = 0;
-  for (int j = 0; j < 8; j++) r |= (uint64_t)((x >> (7 - j)) & 1u) << (8 * j);
-  return r;
-}
-__device__ __forceinline__ uint64_t expandpm(uint32_t x)
-{
-  uint64_t r = 0;
-  for (int j = 0; j < 8; j++) r |= (uint64_t)(((x >> (7 - j)) & 1u) ? 0x01u : 0xFFu) << (8 * j);
-  return r;
-}
-__device__ __forceinline__ v4i frag(const uint64_t* lut, uint64_t w, int s)
-{
-  const uint64_t e0 = lut[(w >> (16 * s)) & 0xFF], e1 = lut[(w >> (16 * s + 8)) & 0xFF];
   v4i f;
-  f[0] = (int)(uint32_t)e0; f[1] = (int)(uint32_t)(e0 >> 32);
-  f[2] = (int)(uint32_t)e1; f[3] = (int)(uint32_t)(e1 >> 32);
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    const int sh = 24 - 16 * h - 8 * (d >> 1) + 4 * (d & 1);
+    f[d] = (int)((((wr >> sh) & 15u) * 0x204081u) & 0x01010101u);
+  }
   return f;
 }
+// bytes 0/1 -> -1/+1 (0xFF / 0x01): ~(b * 0xFE) per byte, no carries between bytes
+__device__ __forceinline__ uint32_t pm_bytes(uint32_t b) { return ~((b << 8) - (b << 1)); }
 
+#define AT_YROW 144   // LDS row of one y: 128 bytes of +-1 values + 16 bytes of padding
+
+// Block: 4 waves x 32 rows k, looping over the y in tiles of 32.  Per term and
+// 128-column chunk of n, the tile's y rows are spread once into +-1 bytes in LDS
+// (double buffered; the next chunk's bits are loaded while the current one is used);
+// each wave spreads its pos = (A xnor T) & mask bits in registers.  Lane (r, g) of
+// step s feeds columns n0 + 64g + 16s .. +15.
 template <int NTG, int NQ>
-__global__ __launch_bounds__(256) void k_attyn_mfma(AttynArgs a, const long long* cy, const long long* hhay)
+__global__ __launch_bounds__(256, 2) void k_attyn_mfma(AttynArgs a, const long long* cy, const long long* hhay)
 {
-  __shared__ uint64_t lut01[256], lutpm[256];
+  __shared__ __attribute__((aligned(16))) int8_t Ys[2][32][AT_YROW];
   __shared__ int Gs[4][NQ][32][33];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, g = lane >> 5, r = lane & 31;
-  lut01[tid] = expand01(tid);
-  lutpm[tid] = expandpm(tid);
   const int t0 = blockIdx.y * NTG;
   const int nts = min(NTG, a.nt - t0);
   const int kbase = blockIdx.x * 128 + wave * 32;
   const int krow = min(kbase + r, a.nk - 1);
   const int kr_orig = a.klist[krow];
-  // per-row constants of the epilogue (lane < 32 owns row kbase + lane)
-  const int myk = min(kbase + lane, a.nk - 1), kk = a.klist[myk];
-  const double* alpha = a.ajdk + (size_t)1 * a.pcols;
-  const double* Dt = a.ajdk + (size_t)5 * a.pcols;
-  const double aa = alpha[kk / kPopBits];
-  long long hAy[NQ][NTG];
-#pragma unroll
-  for (int q = 0; q < NQ; q++)
-#pragma unroll
-    for (int t = 0; t < NTG; t++)
-      hAy[q][t] = (long long)a.term[q].hAy[(long long)a.tlist[t0 + min(t, nts - 1)] * a.term[q].ldh + kk];
+  // y loader role: row ly of the tile, columns 16 lp .. 16 lp + 15 of the chunk
+  const int ly = tid >> 3, lp = tid & 7;
   double o[4][NTG];
 #pragma unroll
   for (int p = 0; p < 4; p++)
 #pragma unroll
     for (int t = 0; t < NTG; t++) o[p][t] = 0.0;
-  __syncthreads();
 
   for (int y0 = 0; y0 < a.ny; y0 += 32) {
-    const int yrow = min(y0 + r, a.ny - 1);
-    const int yr_orig = a.ylist[yrow];
+    const int yl_orig = a.ylist[min(y0 + ly, a.ny - 1)];
     v16i acc[NQ][NTG];
 #pragma unroll
     for (int q = 0; q < NQ; q++)
@@ -115,28 +104,48 @@ __global__ __launch_bounds__(256) void k_attyn_mfma(AttynArgs a, const long long
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
       const AttynTerm& tm = a.term[q];
-      const uint8_t* Arow = tm.A + (size_t)kr_orig * tm.rbA;
-      const uint8_t* Yrow = tm.Y + (size_t)yr_orig * tm.rbY;
+      const uint8_t* Arow = tm.A + (size_t)kr_orig * tm.rbA + 8 * g;
       const uint8_t* Trow[NTG];
 #pragma unroll
-      for (int t = 0; t < NTG; t++) Trow[t] = tm.T + (size_t)a.tlist[t0 + min(t, nts - 1)] * tm.rbT;
-      for (long long n0 = 0; n0 < tm.ldn; n0 += 128) {
-        const long long bo = n0 / 8 + 8 * g;
-        const uint64_t m8 = *(const uint64_t*)(tm.mask + bo);
-        if (!__any(m8 != 0)) continue;
-        const uint64_t a8 = *(const uint64_t*)(Arow + bo);
-        const uint64_t y8 = *(const uint64_t*)(Yrow + bo);
-        uint64_t pos8[NTG];
+      for (int t = 0; t < NTG; t++) Trow[t] = tm.T + (size_t)a.tlist[t0 + min(t, nts - 1)] * tm.rbT + 8 * g;
+      const uint8_t* mrow = tm.mask + 8 * g;
+      const uint8_t* Ysrc = tm.Y + (size_t)yl_orig * tm.rbY + 2 * lp;
+      const int nchunk = (int)(tm.ldn / 128);
+      uint64_t an = 0, tn[NTG], mk = 0;
+      uint32_t yb = 0;
 #pragma unroll
-        for (int t = 0; t < NTG; t++) pos8[t] = ~(a8 ^ *(const uint64_t*)(Trow[t] + bo)) & m8;
+      for (int t = 0; t < NTG; t++) tn[t] = 0;
+#define AT_LOAD(c) do { const long long bo_ = 16LL * (c); an = *(const uint64_t*)(Arow + bo_); \
+        for (int t_ = 0; t_ < NTG; t_++) tn[t_] = *(const uint64_t*)(Trow[t_] + bo_); \
+        mk = *(const uint64_t*)(mrow + bo_); yb = *(const uint16_t*)(Ysrc + bo_); } while (0)
+#define AT_YSTORE(buf) do { const uint32_t wr_ = __builtin_bitreverse32(yb); \
+        const v4i f_ = spread16_cols(wr_, 0); uint4 u_; u_.x = pm_bytes((uint32_t)f_[0]); u_.y = pm_bytes((uint32_t)f_[1]); \
+        u_.z = pm_bytes((uint32_t)f_[2]); u_.w = pm_bytes((uint32_t)f_[3]); *(uint4*)&Ys[buf][ly][16 * lp] = u_; } while (0)
+      __syncthreads();   // the previous readers of Ys are done
+      if (nchunk > 0) { AT_LOAD(0); AT_YSTORE(0); }
+      __syncthreads();
+      for (int c = 0; c < nchunk; c++) {
+        const int buf = c & 1;
+        uint32_t prv[NTG][2];
+#pragma unroll
+        for (int t = 0; t < NTG; t++) {
+          const uint64_t pos8 = ~(an ^ tn[t]) & mk;
+          prv[t][0] = __builtin_bitreverse32((uint32_t)pos8);
+          prv[t][1] = __builtin_bitreverse32((uint32_t)(pos8 >> 32));
+        }
+        if (c + 1 < nchunk) AT_LOAD(c + 1);
 #pragma unroll 2
         for (int s = 0; s < 4; s++) {
-          const v4i bf = frag(lutpm, y8, s);
+          const v4i bf = *(const v4i*)&Ys[buf][r][64 * g + 16 * s];
 #pragma unroll
           for (int t = 0; t < NTG; t++)
-            acc[q][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(frag(lut01, pos8[t], s), bf, acc[q][t], 0, 0, 0);
+            acc[q][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(spread16_cols(prv[t][s >> 1], s & 1), bf, acc[q][t], 0, 0, 0);
         }
+        if (c + 1 < nchunk) AT_YSTORE(buf ^ 1);
+        __syncthreads();
       }
+#undef AT_LOAD
+#undef AT_YSTORE
     }
     // epilogue: lane < 32 walks its row's y in ascending order
     const int yn = min(32, a.ny - y0);
@@ -149,7 +158,15 @@ __global__ __launch_bounds__(256) void k_attyn_mfma(AttynArgs a, const long long
         for (int v = 0; v < 16; v++) Gs[wave][q][8 * (v / 4) + 4 * g + (v % 4)][r] = acc[q][t][v];
       __syncthreads();
       if (lane < 32) {
+        // per-row constants (recomputed here to keep them out of the main loop's registers)
+        const double* alpha = a.ajdk + (size_t)1 * a.pcols;
+        const double* Dt = a.ajdk + (size_t)5 * a.pcols;
+        const int kk = a.klist[min(kbase + lane, a.nk - 1)];
+        const double aa = alpha[kk / kPopBits];
         const int tt = a.tlist[t0 + t];
+        long long hAy[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; q++) hAy[q] = (long long)a.term[q].hAy[(long long)tt * a.term[q].ldh + kk];
         double L[NQ];
         double p0 = o[0][t], p1 = o[1][t], p2 = o[2][t], p3 = o[3][t];
         for (int j = 0; j < yn; j++) {
@@ -160,7 +177,7 @@ __global__ __launch_bounds__(256) void k_attyn_mfma(AttynArgs a, const long long
             const long long hhAY = 2LL * Gs[wave][q][lane][j] - cy[(size_t)q * a.ny + y0 + j];
             const long long hhaY = (long long)a.term[q].haY[(long long)tt * a.term[q].ldh + yy];
             const long long hhay_ = hhay[q * kMaxT + t0 + t];
-            L[q] = (hhAY - hAy[q][t] * ay - aa * hhaY + aa * hhay_ * ay);
+            L[q] = (hhAY - hAy[q] * ay - aa * hhaY + aa * hhay_ * ay);
           }
           // attjn = L[0], ktrhn = L[1], ztsvn = L[2]
           if (NQ > 2) p0 += L[0] * Dy * L[NQ - 1];

@@ -52,22 +52,36 @@ __global__ void k_ww_digits(const uint32_t* cz, long long ldc, int nw, int nwp, 
 
 // per w: Gm[w] = sum_{n in mask} cz(w,n);  H[t][w] = (double) sum_{n in mask} T_t(n)+-1 cz(w,n)
 // block nw: bp[t] = (double)(sum_{n in mask} T_t(n)+-1) * min * mlt
-__global__ void k_ww_hg(WwArgs a, double* H, double* bp, long long* Gm)
+__global__ __launch_bounds__(256) void k_ww_hg(WwArgs a, double* H, double* bp, long long* Gm)
 {
-  int j = blockIdx.x;
+  __shared__ long long red[4][kMaxT + 1];
+  const int j = blockIdx.x, wave = threadIdx.x >> 6;
   const uint8_t* pm = (const uint8_t*)a.pm;
-  int n_total = (int)(a.prb * 8);
+  const int n_total = (int)(a.prb * 8);
   long long g = 0, h[kMaxT];
+#pragma unroll
   for (int t = 0; t < kMaxT; t++) h[t] = 0;
-  for (int n = threadIdx.x; n < n_total; n += 64) {
+  for (int n = threadIdx.x; n < n_total; n += 256) {
     if (!bitv(pm, n)) continue;
-    long long v = (j < a.nw) ? (long long)a.cz[(long long)j * a.ldc + n] : 1;
+    const long long v = (j < a.nw) ? (long long)a.cz[(long long)j * a.ldc + n] : 1;
     g += v;
-    for (int t = 0; t < a.nt; t++) h[t] += bitv(a.Tc.p + (size_t)a.tlist[t] * a.Tc.rb, n) ? v : -v;
+#pragma unroll
+    for (int t = 0; t < kMaxT; t++)
+      if (t < a.nt) h[t] += bitv(a.Tc.p + (size_t)a.tlist[t] * a.Tc.rb, n) ? v : -v;
   }
+  // integer sums: the order of the partial sums does not matter
   g = wsum64(g);
-  for (int t = 0; t < a.nt; t++) h[t] = wsum64(h[t]);
+#pragma unroll
+  for (int t = 0; t < kMaxT; t++) if (t < a.nt) h[t] = wsum64(h[t]);
+  if ((threadIdx.x & 63) == 0) {
+    red[wave][kMaxT] = g;
+#pragma unroll
+    for (int t = 0; t < kMaxT; t++) red[wave][t] = h[t];
+  }
+  __syncthreads();
   if (threadIdx.x != 0) return;
+  g = red[0][kMaxT] + red[1][kMaxT] + red[2][kMaxT] + red[3][kMaxT];
+  for (int t = 0; t < a.nt; t++) h[t] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
   if (j < a.nw) {
     Gm[j] = g;
     for (int t = 0; t < a.nt; t++) H[(size_t)t * a.nw + j] = (double)h[t];
@@ -277,7 +291,7 @@ void ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t st, con
                        a.cz, a.ldc, a.nw, nwp, (int)(a.prb * 8) < a.ldc ? (int)(a.prb * 8) : (int)a.ldc,
                        (const uint8_t*)a.pm, ndig, ldn, Dg);
   }
-  hipLaunchKernelGGL(k_ww_hg, dim3(a.nw + 1), dim3(64), 0, st, a, H, bp, Gm);
+  hipLaunchKernelGGL(k_ww_hg, dim3(a.nw + 1), dim3(256), 0, st, a, H, bp, Gm);
   hipLaunchKernelGGL(k_ww_cntp, dim3(a.nk), dim3(64), 0, st, a, cntp, llk);
   dim3 grid(cdiv(a.nk, 128), cdiv(a.nt, a.nt >= 3 ? 3 : a.nt));
   if (hook && hook->before) hook->before(hook->ctx, st);
@@ -298,18 +312,22 @@ void ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t st, con
 // with P_i = sum_n (a & m)(n) d_i(n) over the signed digits d_i = byte_i(cz) - 128.
 // One block per 32 rows; its 4 waves split n and add their integer sums in LDS.
 // ------------------------------------------------------------------
-// Gs[s] = sum_{n in mask} cz[s*ldc + n], n < n_total
-__global__ void k_cz_sums(const uint32_t* cz, long long ldc, int n_total, const uint32_t* mask, long long* Gs)
+// Gpart[s*ZT_PARTS + b] = sum over the n of part b (n in mask, n < n_total) of cz[s*ldc + n]
+#define ZT_PARTS 32
+__global__ __launch_bounds__(256) void k_cz_sums(const uint32_t* cz, long long ldc, int n_total, const uint32_t* mask,
+                                                 long long* Gpart)
 {
-  const int s = blockIdx.x;
+  const int s = blockIdx.y, b = blockIdx.x;
+  const int per = (n_total + ZT_PARTS - 1) / ZT_PARTS;
+  const int n0 = b * per, n1 = min(n_total, n0 + per);
   long long v = 0;
-  for (int n = threadIdx.x; n < n_total; n += 256)
+  for (int n = n0 + (int)threadIdx.x; n < n1; n += 256)
     if ((mask[n >> 5] >> (((n >> 3) & 3) * 8 + 7 - (n & 7))) & 1u) v += cz[(long long)s * ldc + n];
   v = wsum64(v);
   __shared__ long long part[4];
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
   __syncthreads();
-  if (threadIdx.x == 0) Gs[s] = part[0] + part[1] + part[2] + part[3];
+  if (threadIdx.x == 0) Gpart[s * ZT_PARTS + b] = part[0] + part[1] + part[2] + part[3];
 }
 
 template <int ND>
@@ -362,7 +380,9 @@ __global__ __launch_bounds__(256) void k_ztsn_mfma(const uint8_t* A, size_t rbA,
     long long P = 0;
 #pragma unroll
     for (int i = ND - 1; i >= 0; i--) P = P * 256 + (long long)Ps[rr][i * ns + s];
-    const long long x = 2 * (P + 128 * S * (long long)Cs[rr]) - Gsum[s];
+    long long G = 0;
+    for (int b = 0; b < ZT_PARTS; b++) G += Gsum[s * ZT_PARTS + b];
+    const long long x = 2 * (P + 128 * S * (long long)Cs[rr]) - G;
     const double ll = (double)(2LL * Cs[rr] - Ms[rr]);
     const double base = ll * mn * mlt;
     const double tmp = (double)x;
@@ -373,7 +393,7 @@ __global__ __launch_bounds__(256) void k_ztsn_mfma(const uint8_t* A, size_t rbA,
 size_t ztsn_mfma_work_bytes(int ns, size_t prb, int B)
 {
   const int ndig = (B + 7) / 8;
-  return (((size_t)ndig * ns * prb * 8 + 255) & ~(size_t)255) + sizeof(long long) * (ns + 8) + 256;
+  return (((size_t)ndig * ns * prb * 8 + 255) & ~(size_t)255) + sizeof(long long) * ((size_t)ns * ZT_PARTS + 8) + 256;
 }
 
 void ztsn_mfma(const Bits& A, const int* alist, int na, const uint32_t* mask, const uint32_t* cz, long long ldc,
@@ -390,7 +410,7 @@ void ztsn_mfma(const Bits& A, const int* alist, int na, const uint32_t* mask, co
   const long long tot = (long long)ns * (ldn / 4);
   hipLaunchKernelGGL(k_ww_digits, dim3(cdiv(tot, 256) < 65536 ? cdiv(tot, 256) : 65536), dim3(256), 0, st,
                      cz, ldc, ns, ns, nt_, (const uint8_t*)mask, ndig, ldn, Dg);
-  hipLaunchKernelGGL(k_cz_sums, dim3(ns), dim3(256), 0, st, cz, ldc, nt_, mask, Gsum);
+  hipLaunchKernelGGL(k_cz_sums, dim3(ZT_PARTS, ns), dim3(256), 0, st, cz, ldc, nt_, mask, Gsum);
 #define ZT_LAUNCH(N) hipLaunchKernelGGL((k_ztsn_mfma<N>), dim3(cdiv(na, 32)), dim3(256), 0, st, A.p, A.rb, alist, na, \
                                         (const uint8_t*)mask, Dg, ldn, ns, Gsum, slist, scal, out, ldo)
   switch (ndig) { case 1: ZT_LAUNCH(1); break; case 2: ZT_LAUNCH(2); break; case 3: ZT_LAUNCH(3); break; default: ZT_LAUNCH(4); break; }
