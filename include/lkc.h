@@ -129,6 +129,10 @@ int lkc_xcalc(const double *L, const int *nrows_list, int nn, int n_total, int n
    are contracted with the planes under amask; out[a*ns + s] for a in alist, 0 elsewhere. */
 int lkc_An_ZtSn_ww(const uint8_t *A, size_t rbA, int nr_total, const int *alist, int na, const uint8_t *amask,
                    const double *L, const int *nlist, int nn, int n_total, int ns, int B, long long D_MLT, double *out);
+/* wrap_transpose (binary_read.c:174-269), on the device: Y (ncols rows of rbY >=
+   bsize(nrows) bytes) receives the transpose of X (nrows rows of ncols bits), padding
+   bits zero.  lkc_bcc_load does the same when a .in names only one of *_n / *_t. */
+int lkc_transpose_bits(const uint8_t *X, size_t rbX, int nrows, int ncols, uint8_t *Y, size_t rbY);
 
 #ifdef __cplusplus
 }

@@ -284,6 +284,29 @@ int lkc_xcalc(const double* L, const int* nrows_list, int nn, int n_total, int n
   });
 }
 
+int lkc_transpose_bits(const uint8_t* X, size_t rbX, int nrows, int ncols, uint8_t* Y, size_t rbY)
+{
+  return guard(nullptr, [&] {
+    if (nrows < 0 || ncols < 0 || (nrows && ncols && (!X || !Y))) throw lkc::Error(LKC_ERR_STATE, "bad argument");
+    if (nrows == 0 || ncols == 0) return;
+    if (rbY < lkc::bsize(nrows)) throw lkc::Error(LKC_ERR_STATE, "row stride smaller than bsize(nrows)");
+    Scoped s;
+    std::vector<uint8_t> tmp;
+    lkc::DMat dX = put_bits(s, X, rbX, nrows, ncols, tmp);
+    const size_t rbd = lkc::bsize(nrows);
+    uint8_t* dY = s.dev.alloc<uint8_t>((size_t)ncols * rbd);
+    HIPCHK(hipMemsetAsync(dY, 0, (size_t)ncols * rbd, s.st));
+    lkc::transpose_bits(dX.b(), dY, rbd, s.st);
+    std::vector<uint8_t> h((size_t)ncols * rbd);
+    HIPCHK(hipMemcpyAsync(h.data(), dY, h.size(), hipMemcpyDeviceToHost, s.st));
+    HIPCHK(hipStreamSynchronize(s.st));
+    for (int j = 0; j < ncols; j++) {
+      memcpy(Y + (size_t)j * rbY, h.data() + (size_t)j * rbd, rbd);
+      if (rbY > rbd) memset(Y + (size_t)j * rbY + rbd, 0, rbY - rbd);
+    }
+  });
+}
+
 int lkc_An_ZtSn_ww(const uint8_t* A, size_t rbA, int nr_total, const int* alist, int na, const uint8_t* amask,
                    const double* L, const int* nlist, int nn, int n_total, int ns, int B, long long D_MLT, double* out)
 {

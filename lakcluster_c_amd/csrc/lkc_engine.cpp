@@ -266,20 +266,28 @@ void Run::load(const char* in_text)
     throw Error(LKC_ERR_CONFIG, "only the YnWt strategies are supported");
   if (c.scramble_num > 0) throw Error(LKC_ERR_CONFIG, "scramble is not supported");
   if (c.B < 1 || c.B > 32) throw Error(LKC_ERR_CONFIG, "GLOBAL_B_MLT must be in [1,32]");
-  if (c.A_t.empty() || c.A_n.empty()) throw Error(LKC_ERR_CONFIG, "A_n/A_t files required");
+  if (c.A_t.empty() && c.A_n.empty()) throw Error(LKC_ERR_CONFIG, "an A_n or A_t file is required");
   HIPCHK(hipSetDevice(0));
   HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
 
-  HostBits hAn = read_b16(c.A_t), hAt = read_b16(c.A_n);
+  // Either copy of a matrix may be omitted: the other one is transposed on the device
+  // (the file pair of the reference, wrap_transpose in binary_read.c:174-269).
+  HostBits hAn, hAt;
+  if (!c.A_t.empty()) hAn = read_b16(c.A_t);
+  if (!c.A_n.empty()) hAt = read_b16(c.A_n);
+  if (c.A_t.empty()) { hAn.nr = hAt.nc; hAn.nc = hAt.nr; hAn.rb = bsize(hAn.nc); }
+  if (c.A_n.empty()) { hAt.nr = hAn.nc; hAt.nc = hAn.nr; hAt.rb = bsize(hAt.nc); }
   NA = hAn.nr; M = hAn.nc;
   if (c.A_ncols && c.A_ncols != M) throw Error(LKC_ERR_SHAPE, "GLOBAL_A_n_cols does not match A");
   if (hAt.nr != M || hAt.nc != NA) throw Error(LKC_ERR_SHAPE, "A_n/A_t shapes disagree");
   if (NA <= 0 || M <= 0) throw Error(LKC_ERR_SHAPE, "A is empty");
-  has_Z = !c.Z_t.empty();
+  has_Z = !c.Z_t.empty() || !c.Z_n.empty();
   HostBits hZn, hZt, hTt, hSt;
   if (has_Z) {
-    if (c.Z_n.empty()) throw Error(LKC_ERR_CONFIG, "Z_n file required with Z_t");
-    hZn = read_b16(c.Z_t); hZt = read_b16(c.Z_n);
+    if (!c.Z_t.empty()) hZn = read_b16(c.Z_t);
+    if (!c.Z_n.empty()) hZt = read_b16(c.Z_n);
+    if (c.Z_t.empty()) { hZn.nr = hZt.nc; hZn.nc = hZt.nr; hZn.rb = bsize(hZn.nc); }
+    if (c.Z_n.empty()) { hZt.nr = hZn.nc; hZt.nc = hZn.nr; hZt.rb = bsize(hZt.nc); }
     NZ = hZn.nr;
     if (hZn.nc != M || hZt.nr != M || hZt.nc != NZ) throw Error(LKC_ERR_SHAPE, "Z shapes disagree");
   } else {
@@ -325,8 +333,20 @@ void Run::load(const char* in_text)
   for (int i = 0, b = 0; i < M; i++) { Ac_bidx[i] = b; if (bget(hAc_b.data(), i)) b++; }
 
   // device-resident bit matrices
-  upload_bits(dev, An, hAn); upload_bits(dev, At, hAt);
-  upload_bits(dev, Zn, hZn); upload_bits(dev, Zt, hZt);
+  auto place = [&](DMat& m, const HostBits& h, bool from_file) {
+    if (from_file) { upload_bits(dev, m, h); return; }
+    m.nr = h.nr; m.nc = h.nc; m.rb = h.rb;
+    size_t bytes = (size_t)std::max(h.nr, 1) * h.rb;
+    m.p = dev.alloc<uint8_t>(bytes);
+    HIPCHK(hipMemsetAsync(m.p, 0, bytes, st));
+  };
+  place(An, hAn, !c.A_t.empty()); place(At, hAt, !c.A_n.empty());
+  if (c.A_t.empty()) transpose_bits(At.b(), An.p, An.rb, st);
+  if (c.A_n.empty()) transpose_bits(An.b(), At.p, At.rb, st);
+  const bool zn_file = !has_Z || !c.Z_t.empty(), zt_file = !has_Z || !c.Z_n.empty();
+  place(Zn, hZn, zn_file); place(Zt, hZt, zt_file);
+  if (!zn_file) transpose_bits(Zt.b(), Zn.p, Zn.rb, st);
+  if (!zt_file) transpose_bits(Zn.b(), Zt.p, Zt.rb, st);
   upload_bits(dev, Tt, hTt); upload_bits(dev, St, hSt);
   { HostBits().b.swap(hAn.b); }
 

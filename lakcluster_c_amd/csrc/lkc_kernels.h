@@ -154,6 +154,9 @@ int qr_signs(const Bits& Sc, const int* tlist, int nt, const int* zlist, int nz,
 void pm_few(const Bits& A, const int* alist, int na, const Bits& T, const int* tlist, int nt,
             const uint32_t* mask, int pcols, const double* base16, double* out, long long ldo, hipStream_t st);
 
+// Y (X.nc rows of rbY = bsize(X.nr) bytes, zero padded by the caller) = transpose of X
+void transpose_bits(const Bits& X, uint8_t* Y, size_t rbY, hipStream_t st);
+
 // out[i] = pop(X[rows[i]] & mask) over nwords 32-bit words (rows==NULL: row i)
 void popc_rows(const Bits& X, const int* rows, int nrows, const uint32_t* mask, int nwords, int* out, hipStream_t st);
 

@@ -672,6 +672,35 @@ void popc_rows(const Bits& X, const int* rows, int nrows, const uint32_t* mask, 
   hipLaunchKernelGGL(k_popc_rows, dim3(nrows), dim3(64), 0, st, X.p, X.rb, rows, nrows, mask, nwords, out);
 }
 
+// Bit-matrix transpose (the on-device counterpart of wrap_transpose, binary_read.c:174-269):
+// Y row j, bit i = X row i, bit j.  One wave per 64x64 tile: lane i holds 64 bits of
+// row r0 + i; a ballot per column gives that column's 64 row bits (lane i -> bit i),
+// which bit-reverse + byte-swap turn into the packed MSB-first order.
+__global__ __launch_bounds__(256) void k_transpose_bits(const uint8_t* X, size_t rbX, int nr, int nc,
+                                                        uint8_t* Y, size_t rbY)
+{
+  const int lane = threadIdx.x & 63;
+  const long long r0 = 64LL * blockIdx.y, c0 = 64LL * (blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (c0 >= (long long)rbX * 8) return;
+  const long long r = r0 + lane;
+  const uint64_t x = r < nr ? *(const uint64_t*)(X + (size_t)r * rbX + c0 / 8) : 0ull;
+  uint64_t mine = 0;
+#pragma unroll 8
+  for (int j = 0; j < 64; j++) {
+    const uint64_t m = __ballot((x >> (8 * (j >> 3) + 7 - (j & 7))) & 1ull);
+    if (lane == j) mine = m;
+  }
+  const long long cj = c0 + lane;
+  if (cj < nc) *(uint64_t*)(Y + (size_t)cj * rbY + r0 / 8) = __builtin_bswap64(__builtin_bitreverse64(mine));
+}
+void transpose_bits(const Bits& X, uint8_t* Y, size_t rbY, hipStream_t st)
+{
+  if (X.nr <= 0 || X.nc <= 0) return;
+  const long long tiles_c = ((long long)X.rb * 8 + 63) / 64;
+  dim3 grid((unsigned)((tiles_c + 3) / 4), (unsigned)((X.nr + 63) / 64));
+  hipLaunchKernelGGL(k_transpose_bits, grid, dim3(256), 0, st, X.p, X.rb, X.nr, X.nc, Y, rbY);
+}
+
 // Plain +-1 products of many rows with a few covariate rows (AnZt_vv with Z = T,
 // AnZt_vv.c:62-95): out[tlist[t]*ldo + alist[i]] = base[15] - 2 pop((A_i ^ T_t) & mask).
 // One wave per row, 4 rows per block; the covariate rows stay in L1/L2.

@@ -190,7 +190,7 @@ __global__ __launch_bounds__(256) void k_ww_mfma(WwArgs a, const int8_t* Dg, lon
         prv[t][1] = __builtin_bitreverse32((uint32_t)(pos8[t] >> 32));
       }
       if (c + 1 < nchunk) { WW_BLOAD(c + 1); WW_ALOAD(c + 1); }
-#pragma unroll 2
+#pragma unroll
       for (int s = 0; s < 4; s++) {
         v4i bf[ND];
 #pragma unroll

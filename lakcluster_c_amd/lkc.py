@@ -77,6 +77,7 @@ def lib():
                                      C.c_int, C.c_longlong, dp]),
         "lkc_xcalc": (C.c_int, [dp, ip, C.c_int, C.c_int, C.c_int, C.c_int, C.c_longlong,
                                 C.POINTER(C.c_ulonglong), dp]),
+        "lkc_transpose_bits": (C.c_int, [u8p, C.c_size_t, C.c_int, C.c_int, u8p, C.c_size_t]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -198,6 +199,14 @@ def lakcluster_driver(in_text: str, out_dir: Optional[str] = None) -> None:
 
 def bsize(ncols: int) -> int:
     return int(lib().lkc_bsize(ncols))
+
+
+def transpose_bits(X: np.ndarray, nrows: int, ncols: int) -> np.ndarray:
+    """wrap_transpose on the device: packed rows X (nrows x rb) -> (ncols x bsize(nrows))."""
+    X = np.ascontiguousarray(X, dtype=np.uint8)
+    Y = np.zeros((max(ncols, 1), bsize(nrows)), dtype=np.uint8)
+    _check(lib().lkc_transpose_bits(_ptr(X, C.c_uint8), X.shape[1], nrows, ncols, _ptr(Y, C.c_uint8), Y.shape[1]))
+    return Y[:ncols]
 
 
 def an_ajdk_v(X: np.ndarray, rows: Sequence[int], colmask: np.ndarray, ncols: int, ajdk: np.ndarray) -> np.ndarray:

@@ -196,3 +196,38 @@ def test_kernel_an_ztsn_ww(shape, B, oracle):
     if got.tobytes() != ref.tobytes():
         i = np.argwhere(got != ref)[:3]
         raise AssertionError(f"{len(np.argwhere(got != ref))} differ; e.g. {[(tuple(x), got[tuple(x)], ref[tuple(x)]) for x in i]}")
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (64, 64), (65, 1921), (1000, 3001), (3, 5000), (2000, 65)])
+def test_kernel_transpose_bits(shape):
+    """Device transpose of packed bit rows equals numpy's transpose, padding zero."""
+    from lakcluster_c_amd import lkc
+    nrows, ncols = shape
+    rng = np.random.default_rng(nrows * 7 + ncols)
+    X = _packed(rng, nrows, ncols)
+    Y = lkc.transpose_bits(X, nrows, ncols)
+    bits = np.unpackbits(X, axis=1, bitorder="big")[:, :ncols]
+    ref = np.zeros((ncols, lkc.bsize(nrows)), dtype=np.uint8)
+    ref[:, : (nrows + 7) // 8] = np.packbits(bits.T, axis=1, bitorder="big")
+    assert Y.shape == ref.shape
+    assert np.array_equal(Y, ref)
+
+
+@pytest.mark.parametrize("drop", ["n", "t"])
+@pytest.mark.parametrize("case", ["casectrl", "masked", "blocks"])
+def test_elimination_from_one_copy(case, drop, datasets, oracle):
+    """A .in naming only one of A_n/A_t (and Z_n/Z_t) builds the other on the device;
+    the trace and drop order are unchanged."""
+    from lakcluster_c_amd import lkc
+    text = _read(datasets[case].in_path)
+    ref = oracle.run_text(text)
+    keys = (f"GLOBAL_A_{drop}_name_", f"GLOBAL_Z_{drop}_name_")
+    one = "\n".join(l for l in text.splitlines() if not l.strip().startswith(keys)) + "\n"
+    assert one != text
+    with lkc.Bcc(one) as run:
+        run.run()
+        tr = run.trace()
+        xa, xb = run.xdrop()
+    assert tr.tobytes() == ref.trace.tobytes()
+    np.testing.assert_array_equal(xa, ref.xdrop_a)
+    np.testing.assert_array_equal(xb, ref.xdrop_b)
