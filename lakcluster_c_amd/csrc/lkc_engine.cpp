@@ -251,6 +251,7 @@ Run::~Run()
 {
   if (hctl) (void)hipHostFree(hctl);
   if (hsv) (void)hipHostFree(hsv);
+  if (hidx) (void)hipHostFree(hidx);
   if (st) (void)hipStreamDestroy(st);
 }
 
@@ -464,6 +465,12 @@ void Run::load(const char* in_text)
   scal = dev.alloc<double>(16);
   dsv = dev.alloc<double>(nM + nA);
   HIPCHK(hipHostMalloc((void**)&hsv, sizeof(double) * (nM + nA), hipHostMallocDefault));
+  skey = dev.alloc<double>(nM + nA);
+  sidx_in = dev.alloc<int>(nM + nA); sidx = dev.alloc<int>(nM + nA);
+  HIPCHK(hipHostMalloc((void**)&hidx, sizeof(int) * (nM + nA), hipHostMallocDefault));
+  sort_tmp_bytes = std::max(sort_pairs_temp_bytes(M), sort_pairs_temp_bytes(NA));
+  if (!sort_tmp_bytes) throw Error(LKC_ERR_HIP, "radix sort set-up failed");
+  sort_tmp = dev.raw(sort_tmp_bytes);
   trace.reserve((size_t)(M + NA + 2) * 6);
   HIPCHK(hipStreamSynchronize(st));
   load_s = now_s() - t0;
@@ -779,25 +786,45 @@ void Run::sumscores()
   p.den = dmds; p.pow_form = 0; p.cmb = has_Z && Zrb > 0 && Arb > 0; p.out = dsv;
   scores(p, st);
   int nk = kl.n(), na = ar.n();
-  if (nk) HIPCHK(hipMemcpyAsync(hsv, dsv, sizeof(double) * nk, hipMemcpyDeviceToHost, st));
-  if (na) HIPCHK(hipMemcpyAsync(hsv + M, dsv + M, sizeof(double) * na, hipMemcpyDeviceToHost, st));
+  int rpop = na, cpop_i = nk, rdrop = 0, cdrop = 0;
+  get_xdrop(rpop, cpop_i, c.gamma, c.gamma_type, &rdrop, &cdrop);
+  int nc_drop = std::min(cdrop, cpop_i), nr_drop = std::min(rdrop, rpop);
+  // xij: ascending sort of the column then the row scores.  The device sorts both
+  // columns (stable radix sort of (score, position)); the sorted values are unique, and
+  // where the dropped prefix is strictly increasing its positions are the reference
+  // quicksort's as well.  Otherwise (ties or NaN in the prefix) the host runs the
+  // reference's quicksort on the raw scores.
+  HIPCHK(sort_pairs(dsv, skey, sidx_in, sidx, nk, sort_tmp, sort_tmp_bytes, st));
+  HIPCHK(sort_pairs(dsv + M, skey + M, sidx_in + M, sidx + M, na, sort_tmp, sort_tmp_bytes, st));
+  if (nk) HIPCHK(hipMemcpyAsync(hsv, skey, sizeof(double) * nk, hipMemcpyDeviceToHost, st));
+  if (na) HIPCHK(hipMemcpyAsync(hsv + M, skey + M, sizeof(double) * na, hipMemcpyDeviceToHost, st));
+  if (nc_drop) HIPCHK(hipMemcpyAsync(hidx, sidx, sizeof(int) * nc_drop, hipMemcpyDeviceToHost, st));
+  if (nr_drop) HIPCHK(hipMemcpyAsync(hidx + M, sidx + M, sizeof(int) * nr_drop, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipGetLastError());
   prof.collect();
-  // xij: ascending sort of the column then the row scores
   int Irem = na > 0 ? 1 : 0;
   QC_sv.assign(hsv, hsv + nk); QC_perm.resize(nk);
   QR_sv.assign(hsv + M, hsv + M + na); QR_perm.resize(na);
-  for (int j = 0; j < nk; j++) QC_perm[j] = j;
-  for (int j = 0; j < na; j++) QR_perm[j] = j;
-  sort_scores(QC_sv.data(), QC_perm.data(), nk);
-  sort_scores(QR_sv.data(), QR_perm.data(), na);
+  auto strict = [](const double* v, int n, int k) {
+    for (int i = 0; i < n; i++) if (v[i] != v[i]) return false;   // a NaN anywhere moves quicksort's pivots
+    for (int i = 0; i < k && i + 1 < n; i++) if (!(v[i] < v[i + 1])) return false;
+    return true;
+  };
+  auto exact_order = [&](int off, int n, std::vector<double>& sv, std::vector<int>& perm) {
+    std::vector<double> raw(n);
+    if (n) HIPCHK(hipMemcpy(raw.data(), dsv + off, sizeof(double) * n, hipMemcpyDeviceToHost));
+    for (int j = 0; j < n; j++) perm[j] = j;
+    sort_scores(raw.data(), perm.data(), n);
+    sv.assign(raw.begin(), raw.end());
+  };
+  if (strict(QC_sv.data(), nk, nc_drop)) { for (int i = 0; i < nc_drop; i++) QC_perm[i] = hidx[i]; }
+  else { exact_order(0, nk, QC_sv, QC_perm); n_sort_fallback++; }
+  if (strict(QR_sv.data(), na, nr_drop)) { for (int i = 0; i < nr_drop; i++) QR_perm[i] = hidx[M + i]; }
+  else { exact_order(M, na, QR_sv, QR_perm); n_sort_fallback++; }
   // mxA: drop counts, _rtn/_rmv masks, trace and xdrop lines
-  int rpop = na, cpop_i = nk, rdrop = 0, cdrop = 0;
-  get_xdrop(rpop, cpop_i, c.gamma, c.gamma_type, &rdrop, &cdrop);
   hAc_rtn = hAc_j; std::fill(hAc_rmv.begin(), hAc_rmv.end(), 0);
   hAr_rtn = hAr_j; std::fill(hAr_rmv.begin(), hAr_rmv.end(), 0);
-  int nc_drop = std::min(cdrop, cpop_i), nr_drop = std::min(rdrop, rpop);
   for (int i = 0; i < nc_drop; i++) { int k = kl.h[QC_perm[i]]; bset(hAc_rmv.data(), k); bclr(hAc_rtn.data(), k); }
   for (int i = 0; i < nr_drop; i++) { int a = ar.h[QR_perm[i]]; bset(hAr_rmv.data(), a); bclr(hAr_rtn.data(), a); }
   auto mean = [](const std::vector<double>& v, unsigned long long n) {

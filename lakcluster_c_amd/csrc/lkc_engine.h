@@ -97,6 +97,9 @@ struct Run {
   double *mt[14] = {};
   int* cnt_buf = nullptr;
   double *xwork = nullptr, *scal = nullptr, *dsv = nullptr, *hsv = nullptr;
+  double* skey = nullptr; int *sidx_in = nullptr, *sidx = nullptr, *hidx = nullptr;   // device score sort
+  void* sort_tmp = nullptr; size_t sort_tmp_bytes = 0;
+  long n_sort_fallback = 0;                 // passes that needed the host quicksort
   Grow g_cz, g_planes, g_work, g_L, g_knzt, g_hnvt, g_ankt, g_knkt, g_ztsvn, g_attjn, g_ktrhn;
   // host sort state and outputs
   std::vector<double> QC_sv, QR_sv; std::vector<int> QC_perm, QR_perm;

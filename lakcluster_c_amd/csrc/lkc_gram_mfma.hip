@@ -55,38 +55,40 @@ __device__ __forceinline__ v4i spread16(uint32_t h)
 
 #define GM_QG 48   // segments whose counts are staged in LDS at a time
 
-// Block tile: 128 z (M, the MFMA rows) x 128 a (N, the MFMA columns); 4 waves of 64x64.
+// Block tile: 64 WI z (M, the MFMA rows) x 128 a (N, the MFMA columns); 4 waves of
+// (32 WI) x 64.
 // Columns a run along the lanes, so the update of out[z*ldo + a] is coalesced.
 // Segment lengths are multiples of 4 words (128 columns, four MFMA steps): lane (r, h)
 // loads words 2h and 2h+1 of its rows and feeds one half word to each step, the lanes
 // h = 0 and 1 covering the four words.  Two chunks are in flight while one is used.
-template <int OPSUB>
-__global__ __launch_bounds__(256) void k_gram_mfma(GramMArgs g)
+template <int OPSUB, int WI>
+__global__ __launch_bounds__(256, 3 - WI) void k_gram_mfma(GramMArgs g)
 {
-  __shared__ int czs[GM_QG][128], cas[GM_QG][128];
+  constexpr int BZ = 64 * WI;   // block rows z (the waves wm = 0, 1 take 32 WI each)
+  __shared__ int czs[GM_QG][BZ], cas[GM_QG][128];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, hg = lane >> 5, r = lane & 31;
   const int wm = wave & 1, wn = wave >> 1;
   // XCD-aware tile order: consecutive block ids go to different XCDs; each XCD takes a band
-  const int ntx = (g.na + 127) / 128, nty = (g.nz + 127) / 128;
+  const int ntx = (g.na + 127) / 128, nty = (g.nz + BZ - 1) / BZ;
   const int nb = ntx * nty, bid = blockIdx.x;
   const int per_xcd = (nb + 7) / 8;
   const int lin = (bid % 8) * per_xcd + bid / 8;
   if (lin >= nb) return;
   const int tyi = lin / ntx, txi = lin % ntx;
-  const int zt0 = tyi * 128, at0 = txi * 128;
-  const int zb = zt0 + 64 * wm, ab = at0 + 64 * wn;
+  const int zt0 = tyi * BZ, at0 = txi * 128;
+  const int zb = zt0 + 32 * WI * wm, ab = at0 + 64 * wn;
   // lane (r, h) reads words 2h, 2h+1 of every 4-word chunk of its rows
-  const uint2* Zr[2];
+  const uint2* Zr[WI];
   const uint2* Ar[2];
 #pragma unroll
-  for (int i = 0; i < 2; i++) Zr[i] = (const uint2*)(g.Z + (size_t)g.zlist[min(zb + 32 * i + r, g.nz - 1)] * g.rbZ) + hg;
+  for (int i = 0; i < WI; i++) Zr[i] = (const uint2*)(g.Z + (size_t)g.zlist[min(zb + 32 * i + r, g.nz - 1)] * g.rbZ) + hg;
 #pragma unroll
   for (int j = 0; j < 2; j++) Ar[j] = (const uint2*)(g.A + (size_t)g.alist[min(ab + 32 * j + r, g.na - 1)] * g.rbA) + hg;
   const uint2* mk = g.mask ? (const uint2*)g.mask + hg : nullptr;
   const double* D = g.ajdk + (size_t)5 * g.pcols;
-  double dsum[2][2][16];
+  double dsum[WI][2][16];
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < WI; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++)
 #pragma unroll
@@ -95,11 +97,11 @@ __global__ __launch_bounds__(256) void k_gram_mfma(GramMArgs g)
   const int wtot = g.seg ? (g.nseg > 0 ? g.seg[3 * (g.nseg - 1)] + g.seg[3 * (g.nseg - 1) + 1] : 0) : kPopWords * g.nseg;
   const int nch = wtot / 4;
   const uint2 ones = make_uint2(~0u, ~0u);
-  uint2 z1[2], a1[2], m1, z2[2], a2[2], m2;
+  uint2 z1[WI], a1[2], m1, z2[WI], a2[2], m2;
   auto fetch = [&](int c, uint2* zz, uint2* aa, uint2& mm) {
     const int cc = 2 * min(c, nch - 1);
 #pragma unroll
-    for (int i = 0; i < 2; i++) zz[i] = Zr[i][cc];
+    for (int i = 0; i < WI; i++) zz[i] = Zr[i][cc];
 #pragma unroll
     for (int j = 0; j < 2; j++) aa[j] = Ar[j][cc];
     mm = mk ? mk[cc] : ones;
@@ -112,39 +114,44 @@ __global__ __launch_bounds__(256) void k_gram_mfma(GramMArgs g)
       __syncthreads();
       for (int e = tid; e < GM_QG * 128; e += 256) {
         const int s = e / 128, p = e % 128, qs = q + s;
-        int vz = 0, va = 0;
-        if (qs < g.nseg) {
-          if (zt0 + p < g.nz) vz = g.cz[(long long)qs * g.ldcz + zt0 + p];
-          if (at0 + p < g.na) va = g.ca[(long long)qs * g.ldca + at0 + p];
-        }
-        czs[s][p] = vz; cas[s][p] = va;
+        int va = 0;
+        if (qs < g.nseg && at0 + p < g.na) va = g.ca[(long long)qs * g.ldca + at0 + p];
+        cas[s][p] = va;
+      }
+      for (int e = tid; e < GM_QG * BZ; e += 256) {
+        const int s = e / BZ, p = e % BZ, qs = q + s;
+        int vz = 0;
+        if (qs < g.nseg && zt0 + p < g.nz) vz = g.cz[(long long)qs * g.ldcz + zt0 + p];
+        czs[s][p] = vz;
       }
       __syncthreads();
     }
     const int qch = g.seg ? g.seg[3 * q + 1] / 4 : kPopWords / 4;
-    v16i acc[2][2];
+    v16i acc[WI][2];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < WI; i++)
 #pragma unroll
       for (int j = 0; j < 2; j++)
 #pragma unroll
         for (int v = 0; v < 16; v++) acc[i][j][v] = 0;
     for (int c = 0; c < qch; c++, ch++) {
-      const uint2 zc0 = z1[0], zc1 = z1[1], ac0 = a1[0], ac1 = a1[1], mc = m1;
-      z1[0] = z2[0]; z1[1] = z2[1]; a1[0] = a2[0]; a1[1] = a2[1]; m1 = m2;
+      uint32_t zw[WI][2];
+#pragma unroll
+      for (int i = 0; i < WI; i++) { zw[i][0] = z1[i].x; zw[i][1] = z1[i].y; z1[i] = z2[i]; }
+      const uint2 ac0 = a1[0], ac1 = a1[1], mc = m1;
+      a1[0] = a2[0]; a1[1] = a2[1]; m1 = m2;
       if (ch + 2 < nch) fetch(ch + 2, z2, a2, m2);
-      const uint32_t zw[2][2] = { { zc0.x, zc0.y }, { zc1.x, zc1.y } };
       const uint32_t aw[2][2] = { { ac0.x & mc.x, ac0.y & mc.y }, { ac1.x & mc.x, ac1.y & mc.y } };
       // step p takes half (p & 1) of word (p >> 1) of the lane's pair
 #pragma unroll
       for (int p = 0; p < 4; p++) {
-        v4i af[2], bf[2];
+        v4i af[WI], bf[2];
 #pragma unroll
-        for (int i = 0; i < 2; i++) af[i] = spread16((zw[i][p >> 1] >> (16 * (p & 1))) & 0xFFFFu);
+        for (int i = 0; i < WI; i++) af[i] = spread16((zw[i][p >> 1] >> (16 * (p & 1))) & 0xFFFFu);
 #pragma unroll
         for (int j = 0; j < 2; j++) bf[j] = spread16((aw[j][p >> 1] >> (16 * (p & 1))) & 0xFFFFu);
 #pragma unroll
-        for (int i = 0; i < 2; i++)
+        for (int i = 0; i < WI; i++)
 #pragma unroll
           for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
       }
@@ -155,10 +162,10 @@ __global__ __launch_bounds__(256) void k_gram_mfma(GramMArgs g)
 #pragma unroll
     for (int j = 0; j < 2; j++) caj[j] = cas[qq][64 * wn + 32 * j + r];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < WI; i++)
 #pragma unroll
       for (int v4 = 0; v4 < 4; v4++) {
-        const int4 cz4 = *(const int4*)&czs[qq][64 * wm + 32 * i + 8 * v4 + 4 * hg];
+        const int4 cz4 = *(const int4*)&czs[qq][32 * WI * wm + 32 * i + 8 * v4 + 4 * hg];
         const int czv[4] = { cz4.x, cz4.y, cz4.z, cz4.w };
 #pragma unroll
         for (int u = 0; u < 4; u++)
@@ -172,10 +179,10 @@ __global__ __launch_bounds__(256) void k_gram_mfma(GramMArgs g)
   // Epilogue.  Every load below reads a valid (clamped) address without a branch, so
   // they all go out together; only the stores are predicated.
   const double a0d1 = g.base16[5], a2d1 = g.base16[7];
-  int zrow[2][16];
-  double xzv[2][16];
+  int zrow[WI][16];
+  double xzv[WI][16];
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < WI; i++)
 #pragma unroll
     for (int v = 0; v < 16; v++) {
       zrow[i][v] = g.zlist[min(zb + 32 * i + 8 * (v / 4) + 4 * hg + (v % 4), g.nz - 1)];
@@ -187,9 +194,9 @@ __global__ __launch_bounds__(256) void k_gram_mfma(GramMArgs g)
     const int aor = g.alist[acl];
     const double xav = g.xa[aor];
     const long long aidx = g.a_orig ? aor : acl;
-    double val[2][16];
+    double val[WI][16];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < WI; i++)
 #pragma unroll
       for (int v = 0; v < 16; v++) {
         const int zc = min(zb + 32 * i + 8 * (v / 4) + 4 * hg + (v % 4), g.nz - 1);
@@ -201,7 +208,7 @@ __global__ __launch_bounds__(256) void k_gram_mfma(GramMArgs g)
         if (OPSUB) val[i][v] = g.out[zidx * g.ldo + aidx] - o;
       }
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < WI; i++)
 #pragma unroll
       for (int v = 0; v < 16; v++) {
         const int zc = zb + 32 * i + 8 * (v / 4) + 4 * hg + (v % 4);
@@ -214,10 +221,15 @@ __global__ __launch_bounds__(256) void k_gram_mfma(GramMArgs g)
 void gram_mfma(const GramMArgs& g, hipStream_t st)
 {
   if (g.na <= 0 || g.nz <= 0) return;
-  const long long nb = (long long)cdiv(g.na, 128) * cdiv(g.nz, 128);
+  // Updates over a few compacted columns are bound by their per-block epilogues and the
+  // output traffic: smaller z tiles keep more waves resident for those.
+  const int wi = g.seg ? 1 : 2;
+  const long long nb = (long long)cdiv(g.na, 128) * cdiv(g.nz, 64 * wi);
   const long long grid = ((nb + 7) / 8) * 8;
-  if (g.op_sub) hipLaunchKernelGGL(k_gram_mfma<1>, dim3((unsigned)grid), dim3(256), 0, st, g);
-  else hipLaunchKernelGGL(k_gram_mfma<0>, dim3((unsigned)grid), dim3(256), 0, st, g);
+#define GM_LAUNCH(S, W) hipLaunchKernelGGL((k_gram_mfma<S, W>), dim3((unsigned)grid), dim3(256), 0, st, g)
+  if (g.op_sub) { if (wi == 1) GM_LAUNCH(1, 1); else GM_LAUNCH(1, 2); }
+  else { if (wi == 1) GM_LAUNCH(0, 1); else GM_LAUNCH(0, 2); }
+#undef GM_LAUNCH
 }
 
 }  // namespace lkc

@@ -157,6 +157,11 @@ void pm_few(const Bits& A, const int* alist, int na, const Bits& T, const int* t
 // Y (X.nc rows of rbY = bsize(X.nr) bytes, zero padded by the caller) = transpose of X
 void transpose_bits(const Bits& X, uint8_t* Y, size_t rbY, hipStream_t st);
 
+// stable ascending radix sort of (keys[i], i): keys_out, idx_out; temp of sort_pairs_temp_bytes(n)
+size_t sort_pairs_temp_bytes(int n);
+hipError_t sort_pairs(const double* keys, double* keys_out, int* idx_in, int* idx_out, int n,
+                      void* temp, size_t temp_bytes, hipStream_t st);
+
 // out[i] = pop(X[rows[i]] & mask) over nwords 32-bit words (rows==NULL: row i)
 void popc_rows(const Bits& X, const int* rows, int nrows, const uint32_t* mask, int nwords, int* out, hipStream_t st);
 
