@@ -60,6 +60,20 @@ def cpu_baseline(root, seconds_cap=30.0):
     }
 
 
+def _traffic(workload):
+    """HBM bytes per launch of the ww kernel from the committed FETCH_SIZE/WRITE_SIZE
+    counter runs (copy-calibrated), or None for a workload without such a profile."""
+    p = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "c2_ww_traffic.json")
+    try:
+        with open(p) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload:
+        return None
+    return t["fetch_kib_per_launch_raw"] * 1024 * t["fetch_scale"] + t["write_kib_per_launch_raw"] * 1024 * t["write_scale"]
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,7 +148,7 @@ def main():
             "peak": PEAK_INT8_TOPS,
             "unit": "TOP/s",
             "frac": achieved / PEAK_INT8_TOPS if achieved else None,
-            "traffic": None,
+            "traffic": _traffic(args.config),
             "launches": kl,
             "avg_launch_ms": kms / kl if kl else None,
         },
