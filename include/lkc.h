@@ -134,6 +134,43 @@ int lkc_An_ZtSn_ww(const uint8_t *A, size_t rbA, int nr_total, const int *alist,
    bits zero.  lkc_bcc_load does the same when a .in names only one of *_n / *_t. */
 int lkc_transpose_bits(const uint8_t *X, size_t rbX, int nrows, int ncols, uint8_t *Y, size_t rbY);
 
+/* ---- many eliminations on one GPU: permutation-null batches and serving ------
+   (no counterpart in the reference, which runs one problem per process; the nulls of
+   its MATLAB drivers, e.g. dir_m/lakcluster_uADZSZDA_shuffle_1.m:1-2, re-run the binary
+   once per shuffle.)  A pool keeps the patients' bits on the device; a job takes rows of
+   it as A (cases) and Z (controls), with an optional column mask, and runs the same
+   elimination as lkc_bcc_run on a .in naming those rows and columns as masks. */
+typedef struct lkc_pool lkc_pool;
+/* rows: nrows patients of ncols bits (row stride rb >= bsize(ncols)); cov: ncov covariate
+   columns as rows of nrows bits (stride rbc >= bsize(nrows)) */
+int lkc_pool_create(const uint8_t *rows, size_t rb, int nrows, int ncols,
+                    const uint8_t *cov, size_t rbc, int ncov, lkc_pool **out);
+void lkc_pool_free(lkc_pool *P);
+typedef struct lkc_job_params {
+  double gamma; int gamma_type; int B_MLT; double kappa_squared; int sparse;
+} lkc_job_params;
+/* one job as a handle (then lkc_bcc_run / lkc_bcc_get_trace as usual) */
+int lkc_bcc_load_job(const lkc_pool *P, const int *rowsA, int nA, const int *rowsZ, int nZ,
+                     const uint8_t *colmask, const lkc_job_params *prm, lkc_bcc **out);
+typedef struct lkc_job {
+  const int *rowsA; int nA; const int *rowsZ; int nZ; const uint8_t *colmask;
+  lkc_job_params params; double arrival_s;   /* seconds after the call starts */
+} lkc_job;
+typedef struct lkc_job_result {
+  int status; int n_iter; long long ops; double arrival_s, start_s, end_s;
+  long n_trace; double *trace;               /* 6 doubles per line, as lkc_bcc_get_trace */
+  long n_xdrop; int *xdrop_a; int *xdrop_b;  /* 2 ints per line, as lkc_bcc_get_xdrop */
+} lkc_job_result;
+/* Runs the jobs, at most max_concurrent at a time, each admitted in order once it has
+   arrived and its device-memory estimate fits hbm_budget_bytes (<= 0: 80% of the free
+   device memory).  Results are allocated by the library; free them with
+   lkc_batch_results_free.  Returns LKC_OK if the batch ran (per-job status inside). */
+int lkc_batch_run(const lkc_pool *P, const lkc_job *jobs, int njobs, int max_concurrent,
+                  double hbm_budget_bytes, lkc_job_result *results);
+void lkc_batch_results_free(lkc_job_result *results, int njobs);
+/* the device-memory estimate the admission uses for one job */
+double lkc_job_bytes(int nA, int nZ, int ncols, int B_MLT);
+
 #ifdef __cplusplus
 }
 #endif

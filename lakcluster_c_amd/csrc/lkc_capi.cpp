@@ -1,5 +1,6 @@
 // lkc_capi.cpp -- the extern "C" boundary declared in include/lkc.h.
 #include "lkc_engine.h"
+#include <cmath>
 #include "lkc_stages.h"
 
 #include <string.h>
@@ -7,6 +8,7 @@
 #include <vector>
 
 struct lkc_bcc { lkc::Run run; };
+struct lkc_pool { lkc::Pool pool; };
 
 static thread_local std::string g_last_error;
 
@@ -282,6 +284,104 @@ int lkc_xcalc(const double* L, const int* nrows_list, int nn, int n_total, int n
     for (int j = 0; j < nw; j++) for (int i = 0; i < nn; i++) cz[(size_t)j * nn + i] = h[(size_t)j * std::max(nn, 1) + i];
     min_mlt[0] = hs[2]; min_mlt[1] = hs[4];
   });
+}
+
+static lkc::Cfg job_cfg(const lkc_job_params* p)
+{
+  lkc::Cfg c;
+  if (p) {
+    c.gamma = p->gamma; c.gamma_type = p->gamma_type; c.B = p->B_MLT; c.kappa2 = p->kappa_squared; c.sparse = p->sparse;
+    c.D_MLT = (long long)pow(2.0, (double)c.B);
+  }
+  return c;
+}
+
+int lkc_pool_create(const uint8_t* rows, size_t rb, int nrows, int ncols, const uint8_t* cov, size_t rbc, int ncov,
+                    lkc_pool** out)
+{
+  if (!out) { g_last_error = "null output pointer"; return LKC_ERR_STATE; }
+  *out = nullptr;
+  lkc_pool* P = new lkc_pool();
+  int rc = guard(nullptr, [&] {
+    if (nrows <= 0 || ncols <= 0 || ncov < 1 || ncov > lkc::kMaxT || !rows || !cov) throw lkc::Error(LKC_ERR_STATE, "bad argument");
+    const size_t b = lkc::bsize(ncols), bc = lkc::bsize(nrows);
+    if (rb < b || rbc < bc) throw lkc::Error(LKC_ERR_STATE, "row stride smaller than bsize");
+    HIPCHK(hipSetDevice(0));
+    lkc::Pool& p = P->pool;
+    p.P = nrows; p.M = ncols; p.T = ncov;
+    std::vector<uint8_t> h((size_t)nrows * b);
+    for (int r = 0; r < nrows; r++) memcpy(h.data() + (size_t)r * b, rows + (size_t)r * rb, b);
+    p.Pn.nr = nrows; p.Pn.nc = ncols; p.Pn.rb = b; p.Pn.p = p.dev.alloc<uint8_t>(h.size());
+    HIPCHK(hipMemcpy(p.Pn.p, h.data(), h.size(), hipMemcpyHostToDevice));
+    std::vector<uint8_t> hc((size_t)ncov * bc);
+    for (int t = 0; t < ncov; t++) memcpy(hc.data() + (size_t)t * bc, cov + (size_t)t * rbc, bc);
+    p.Tp.nr = ncov; p.Tp.nc = nrows; p.Tp.rb = bc; p.Tp.p = p.dev.alloc<uint8_t>(hc.size());
+    HIPCHK(hipMemcpy(p.Tp.p, hc.data(), hc.size(), hipMemcpyHostToDevice));
+  });
+  if (rc != LKC_OK) { delete P; return rc; }
+  *out = P;
+  return LKC_OK;
+}
+
+void lkc_pool_free(lkc_pool* P) { delete P; }
+
+int lkc_bcc_load_job(const lkc_pool* P, const int* rowsA, int nA, const int* rowsZ, int nZ, const uint8_t* colmask,
+                     const lkc_job_params* prm, lkc_bcc** out)
+{
+  if (!out || !P) { g_last_error = "null pointer"; return LKC_ERR_STATE; }
+  *out = nullptr;
+  lkc_bcc* D = new lkc_bcc();
+  int rc = guard(nullptr, [&] { D->run.load_job(P->pool, rowsA, nA, rowsZ, nZ, colmask, job_cfg(prm)); });
+  if (rc != LKC_OK) { delete D; return rc; }
+  *out = D;
+  return LKC_OK;
+}
+
+double lkc_job_bytes(int nA, int nZ, int ncols, int B_MLT) { return lkc::job_bytes(nA, nZ, ncols, B_MLT); }
+
+int lkc_batch_run(const lkc_pool* P, const lkc_job* jobs, int njobs, int max_concurrent, double hbm_budget_bytes,
+                  lkc_job_result* results)
+{
+  return guard(nullptr, [&] {
+    if (!P || njobs < 0 || (njobs && (!jobs || !results))) throw lkc::Error(LKC_ERR_STATE, "bad argument");
+    if (hbm_budget_bytes <= 0) {
+      size_t fr = 0, tot = 0;
+      HIPCHK(hipSetDevice(0));
+      HIPCHK(hipMemGetInfo(&fr, &tot));
+      hbm_budget_bytes = 0.8 * (double)fr;
+    }
+    std::vector<lkc::JobIn> in(njobs);
+    for (int j = 0; j < njobs; j++) {
+      in[j].rowsA = jobs[j].rowsA; in[j].nA = jobs[j].nA; in[j].rowsZ = jobs[j].rowsZ; in[j].nZ = jobs[j].nZ;
+      in[j].colmask = jobs[j].colmask; in[j].cfg = job_cfg(&jobs[j].params); in[j].arrival_s = jobs[j].arrival_s;
+    }
+    std::vector<lkc::JobOut> o(njobs);
+    lkc::run_batch(P->pool, in.data(), njobs, max_concurrent, hbm_budget_bytes, o.data());
+    for (int j = 0; j < njobs; j++) {
+      lkc_job_result& r = results[j];
+      memset(&r, 0, sizeof r);
+      r.status = o[j].status; r.n_iter = o[j].n_iter; r.ops = o[j].ops;
+      r.arrival_s = o[j].arrival_s; r.start_s = o[j].start_s; r.end_s = o[j].end_s;
+      r.n_trace = (long)(o[j].trace.size() / 6);
+      r.trace = (double*)malloc(sizeof(double) * std::max<size_t>(o[j].trace.size(), 1));
+      if (!o[j].trace.empty()) memcpy(r.trace, o[j].trace.data(), sizeof(double) * o[j].trace.size());
+      r.n_xdrop = (long)(o[j].xdrop_a.size() / 2);
+      r.xdrop_a = (int*)malloc(sizeof(int) * std::max<size_t>(o[j].xdrop_a.size(), 1));
+      r.xdrop_b = (int*)malloc(sizeof(int) * std::max<size_t>(o[j].xdrop_b.size(), 1));
+      if (!o[j].xdrop_a.empty()) memcpy(r.xdrop_a, o[j].xdrop_a.data(), sizeof(int) * o[j].xdrop_a.size());
+      if (!o[j].xdrop_b.empty()) memcpy(r.xdrop_b, o[j].xdrop_b.data(), sizeof(int) * o[j].xdrop_b.size());
+      if (o[j].status != LKC_OK) g_last_error = o[j].err;
+    }
+  });
+}
+
+void lkc_batch_results_free(lkc_job_result* results, int njobs)
+{
+  if (!results) return;
+  for (int j = 0; j < njobs; j++) {
+    free(results[j].trace); free(results[j].xdrop_a); free(results[j].xdrop_b);
+    results[j].trace = nullptr; results[j].xdrop_a = results[j].xdrop_b = nullptr;
+  }
 }
 
 int lkc_transpose_bits(const uint8_t* X, size_t rbX, int nrows, int ncols, uint8_t* Y, size_t rbY)

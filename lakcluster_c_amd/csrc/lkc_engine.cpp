@@ -315,23 +315,8 @@ void Run::load(const char* in_text)
   if (T < 1 || T > kMaxT) throw Error(LKC_ERR_CONFIG, "between 1 and 8 covariate columns are supported");
   pcols = psize(M);
 
-  auto mk = [](int n) { return std::vector<uint8_t>(bsize(n) + 8, 0); };
-  hAr_b = mk(NA); hAr_j = mk(NA); hAr_rtn = mk(NA); hAr_rmv = mk(NA);
-  hZr_b = mk(NZ); hZr_j = mk(NZ);
-  hAc_b = mk(M); hAc_j = mk(M); hAc_rtn = mk(M); hAc_rmv = mk(M);
-  hTc_j = mk(T);
-  if (!c.A_rind.empty()) hAr_b = read_mask(c.A_rind, NA); else for (int i = 0; i < NA; i++) bset(hAr_b.data(), i);
-  if (!c.Z_rind.empty() && has_Z) hZr_b = read_mask(c.Z_rind, NZ); else for (int i = 0; i < NZ; i++) bset(hZr_b.data(), i);
-  if (!c.A_cind.empty()) hAc_b = read_mask(c.A_cind, M); else for (int i = 0; i < M; i++) bset(hAc_b.data(), i);
-  if (!c.T_cind.empty()) hTc_j = read_mask(c.T_cind, T);
-  else { int nt = c.T_ncols > 0 ? std::min(c.T_ncols, T) : T; for (int i = 0; i < nt; i++) bset(hTc_j.data(), i); }
-  for (int i = 1; i < T; i++) if (bget(hTc_j.data(), i) && !bget(hTc_j.data(), i - 1)) throw Error(LKC_ERR_CONFIG, "T column mask must be a prefix");
-  hAr_j = hAr_b; hZr_j = hZr_b; hAc_j = hAc_b;
-  Arb = popmask(hAr_b, NA); Zrb = popmask(hZr_b, NZ);
-  if (Arb == 0) throw Error(LKC_ERR_SHAPE, "A has no active rows");
-  Ar_bidx.assign(NA + 1, 0); Ac_bidx.assign(M + 1, 0);
-  for (int i = 0, b = 0; i < NA; i++) { Ar_bidx[i] = b; if (bget(hAr_b.data(), i)) b++; }
-  for (int i = 0, b = 0; i < M; i++) { Ac_bidx[i] = b; if (bget(hAc_b.data(), i)) b++; }
+  masks_from_files();
+  finish_masks();
 
   // device-resident bit matrices
   auto place = [&](DMat& m, const HostBits& h, bool from_file) {
@@ -351,6 +336,88 @@ void Run::load(const char* in_text)
   upload_bits(dev, Tt, hTt); upload_bits(dev, St, hSt);
   { HostBits().b.swap(hAn.b); }
 
+  setup_device(t0);
+}
+
+// A job drawn from a device-resident pool (permutation nulls, serving): the listed
+// rows become A and Z, compacted in the order given; all of them are active; colmask
+// (bsize(M) bytes, NULL = all) selects the columns, as the mr_/mc_ masks of a .in do.
+// The matrices never leave the device: rows are gathered, the transposed copies built
+// with the bit transpose, the covariate columns gathered bit by bit.
+void Run::load_job(const Pool& P, const int* rowsA, int nA, const int* rowsZ, int nZ, const uint8_t* colmask,
+                   const Cfg& params)
+{
+  double t0 = now_s();
+  c = params; c.nbins = 1; c.QR = c.QC = "YnWt";
+  if (c.B < 1 || c.B > 32) throw Error(LKC_ERR_CONFIG, "B_MLT must be in [1,32]");
+  if (nA <= 0 || nZ < 0 || P.M <= 0) throw Error(LKC_ERR_SHAPE, "a job needs at least one A row");
+  for (int i = 0; i < nA; i++) if (rowsA[i] < 0 || rowsA[i] >= P.P) throw Error(LKC_ERR_SHAPE, "A row outside the pool");
+  for (int i = 0; i < nZ; i++) if (rowsZ[i] < 0 || rowsZ[i] >= P.P) throw Error(LKC_ERR_SHAPE, "Z row outside the pool");
+  HIPCHK(hipSetDevice(0));
+  HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  NA = nA; NZ = nZ; M = P.M; T = P.T; has_Z = nZ > 0; pcols = psize(M);
+  if (T < 1 || T > kMaxT) throw Error(LKC_ERR_CONFIG, "between 1 and 8 covariate columns are supported");
+  auto mk = [](int n) { return std::vector<uint8_t>(bsize(n) + 8, 0); };
+  hAr_b = mk(NA); hAr_j = mk(NA); hAr_rtn = mk(NA); hAr_rmv = mk(NA);
+  hZr_b = mk(NZ); hZr_j = mk(NZ);
+  hAc_b = mk(M); hAc_j = mk(M); hAc_rtn = mk(M); hAc_rmv = mk(M);
+  hTc_j = mk(T);
+  for (int i = 0; i < NA; i++) bset(hAr_b.data(), i);
+  for (int i = 0; i < NZ; i++) bset(hZr_b.data(), i);
+  if (colmask) { memcpy(hAc_b.data(), colmask, bsize(M)); for (long i = M; i < (long)bsize(M) * 8; i++) bclr(hAc_b.data(), i); }
+  else for (int i = 0; i < M; i++) bset(hAc_b.data(), i);
+  for (int i = 0; i < T; i++) bset(hTc_j.data(), i);
+  finish_masks();
+
+  int* drows = dev.alloc<int>((size_t)nA + nZ + 1);
+  HIPCHK(hipMemcpyAsync(drows, rowsA, sizeof(int) * nA, hipMemcpyHostToDevice, st));
+  if (nZ) HIPCHK(hipMemcpyAsync(drows + nA, rowsZ, sizeof(int) * nZ, hipMemcpyHostToDevice, st));
+  auto alloc0 = [&](DMat& m, int nr, int nc) {
+    m.nr = nr; m.nc = nc; m.rb = bsize(nc);
+    size_t bytes = (size_t)std::max(nr, 1) * m.rb;
+    m.p = dev.alloc<uint8_t>(bytes);
+    HIPCHK(hipMemsetAsync(m.p, 0, bytes, st));
+  };
+  alloc0(An, NA, M); gather_rows(P.Pn.b(), drows, NA, An.p, st);
+  alloc0(At, M, NA); transpose_bits(An.b(), At.p, At.rb, st);
+  alloc0(Zn, NZ, M); alloc0(Zt, M, NZ);
+  if (has_Z) { gather_rows(P.Pn.b(), drows + nA, NZ, Zn.p, st); transpose_bits(Zn.b(), Zt.p, Zt.rb, st); }
+  alloc0(Tt, T, NA); gather_bits(P.Tp.b(), drows, NA, Tt.p, Tt.rb, st);
+  alloc0(St, T, NZ); if (has_Z) gather_bits(P.Tp.b(), drows + nA, NZ, St.p, St.rb, st);
+  setup_device(t0);
+}
+
+// row / column / covariate masks from the .in files (mr_*, mc_*, T column mask)
+void Run::masks_from_files()
+{
+  auto mk = [](int n) { return std::vector<uint8_t>(bsize(n) + 8, 0); };
+  hAr_b = mk(NA); hAr_j = mk(NA); hAr_rtn = mk(NA); hAr_rmv = mk(NA);
+  hZr_b = mk(NZ); hZr_j = mk(NZ);
+  hAc_b = mk(M); hAc_j = mk(M); hAc_rtn = mk(M); hAc_rmv = mk(M);
+  hTc_j = mk(T);
+  if (!c.A_rind.empty()) hAr_b = read_mask(c.A_rind, NA); else for (int i = 0; i < NA; i++) bset(hAr_b.data(), i);
+  if (!c.Z_rind.empty() && has_Z) hZr_b = read_mask(c.Z_rind, NZ); else for (int i = 0; i < NZ; i++) bset(hZr_b.data(), i);
+  if (!c.A_cind.empty()) hAc_b = read_mask(c.A_cind, M); else for (int i = 0; i < M; i++) bset(hAc_b.data(), i);
+  if (!c.T_cind.empty()) hTc_j = read_mask(c.T_cind, T);
+  else { int nt = c.T_ncols > 0 ? std::min(c.T_ncols, T) : T; for (int i = 0; i < nt; i++) bset(hTc_j.data(), i); }
+}
+
+// working masks, active counts and storage index maps (bcc_M_mxset, bcc.c:1414-1438)
+void Run::finish_masks()
+{
+  for (int i = 1; i < T; i++) if (bget(hTc_j.data(), i) && !bget(hTc_j.data(), i - 1)) throw Error(LKC_ERR_CONFIG, "T column mask must be a prefix");
+  hAr_j = hAr_b; hZr_j = hZr_b; hAc_j = hAc_b;
+  Arb = popmask(hAr_b, NA); Zrb = popmask(hZr_b, NZ);
+  if (Arb == 0) throw Error(LKC_ERR_SHAPE, "A has no active rows");
+  Ar_bidx.assign(NA + 1, 0); Ac_bidx.assign(M + 1, 0);
+  for (int i = 0, b = 0; i < NA; i++) { Ar_bidx[i] = b; if (bget(hAr_b.data(), i)) b++; }
+  for (int i = 0, b = 0; i < M; i++) { Ac_bidx[i] = b; if (bget(hAc_b.data(), i)) b++; }
+}
+
+// everything after the bit matrices are resident: control block, A_p / ajdk tables,
+// persistent score arrays (bcc_init, bcc.c:1005-1150, 1197-1313)
+void Run::setup_device(double t0)
+{
   // control block: masks and index lists, uploaded together once per step
   size_t off = 0;
   auto reserve = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };

@@ -33,6 +33,17 @@ struct Dev {
 struct Grow { void* p = nullptr; size_t cap = 0; void* need(Dev& d, size_t bytes); };
 
 struct DMat { uint8_t* p = nullptr; int nr = 0, nc = 0; size_t rb = 0; Bits b() const { return Bits{ p, nr, nc, rb }; } };
+// Device-resident pool of patients that jobs draw their rows from (batched nulls,
+// serving): Pn holds P rows of M bits, Tp the T covariate columns as rows of P bits.
+struct Pool { Dev dev; int P = 0, M = 0, T = 0; DMat Pn, Tp; };
+struct JobIn { const int* rowsA = nullptr; int nA = 0; const int* rowsZ = nullptr; int nZ = 0;
+               const uint8_t* colmask = nullptr; Cfg cfg; double arrival_s = 0; };
+struct JobOut { int status = 0; std::string err; int n_iter = 0; long long ops = 0;
+                double arrival_s = 0, start_s = 0, end_s = 0;
+                std::vector<double> trace; std::vector<int> xdrop_a, xdrop_b; };
+double job_bytes(int nA, int nZ, int M, int B);
+// Runs the jobs with at most max_conc at a time, admitted in order within budget bytes.
+void run_batch(const Pool& P, const JobIn* jobs, int n, int max_conc, double budget, JobOut* out);
 struct IList { std::vector<int> h; int* d = nullptr; int n() const { return (int)h.size(); } };
 // one patient group of an At_T_YnWt_S_Zn term: SNP-major bits, active-row mask and list, covariates
 struct Side { const DMat* X; const uint32_t* mask; const IList* rows; const DMat* Cov; };
@@ -112,6 +123,11 @@ struct Run {
 
   ~Run();
   void load(const char* in_text);
+  void load_job(const struct Pool& P, const int* rowsA, int nA, const int* rowsZ, int nZ, const uint8_t* colmask,
+                const Cfg& params);
+  void masks_from_files();
+  void finish_masks();
+  void setup_device(double t0);
   void push_ctl();
   void st_ajdk(); void st_ZtSn(); void st_YnWt(); void st_At_T(); void st_QR();
   void st_single(); void st_double(); void st_flatten();

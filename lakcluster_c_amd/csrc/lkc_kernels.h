@@ -154,6 +154,10 @@ int qr_signs(const Bits& Sc, const int* tlist, int nt, const int* zlist, int nz,
 void pm_few(const Bits& A, const int* alist, int na, const Bits& T, const int* tlist, int nt,
             const uint32_t* mask, int pcols, const double* base16, double* out, long long ldo, hipStream_t st);
 
+// dst row i = src row rows[i]; dst row t, bit i = src row t, bit idx[i] (rows of rbd bytes)
+void gather_rows(const Bits& src, const int* rows, int n, uint8_t* dst, hipStream_t st);
+void gather_bits(const Bits& src, const int* idx, int n, uint8_t* dst, size_t rbd, hipStream_t st);
+
 // Y (X.nc rows of rbY = bsize(X.nr) bytes, zero padded by the caller) = transpose of X
 void transpose_bits(const Bits& X, uint8_t* Y, size_t rbY, hipStream_t st);
 

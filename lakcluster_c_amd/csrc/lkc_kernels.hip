@@ -701,6 +701,46 @@ void transpose_bits(const Bits& X, uint8_t* Y, size_t rbY, hipStream_t st)
   hipLaunchKernelGGL(k_transpose_bits, grid, dim3(256), 0, st, X.p, X.rb, X.nr, X.nc, Y, rbY);
 }
 
+// Job inputs from a device-resident pool: dst row i = src row rows[i] (rb bytes, a
+// multiple of 16), and dst row t bit i = src row t bit idx[i] (the covariate columns of
+// the chosen patients; bits past n stay zero).
+__global__ void k_gather_rows(const uint8_t* src, size_t rb, const int* rows, int n, uint8_t* dst)
+{
+  const long long per = (long long)(rb / 16), tot = (long long)n * per;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
+    const long long i = e / per, q = e % per;
+    ((uint4*)(dst + (size_t)i * rb))[q] = ((const uint4*)(src + (size_t)rows[i] * rb))[q];
+  }
+}
+void gather_rows(const Bits& src, const int* rows, int n, uint8_t* dst, hipStream_t st)
+{
+  const long long tot = (long long)n * (long long)(src.rb / 16);
+  if (tot <= 0) return;
+  hipLaunchKernelGGL(k_gather_rows, dim3(cdiv(tot, 256) < 65536 ? cdiv(tot, 256) : 65536), dim3(256), 0, st,
+                     src.p, src.rb, rows, n, dst);
+}
+__global__ void k_gather_bits(const uint8_t* src, size_t rbs, int nr, const int* idx, int n, uint8_t* dst, size_t rbd)
+{
+  const long long per = (long long)(rbd / 4), tot = (long long)nr * per;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
+    const long long t = e / per, w = e % per;
+    const uint8_t* sr = src + (size_t)t * rbs;
+    uint32_t v = 0;
+    for (int q = 0; q < 32; q++) {
+      const long long i = 32 * w + q;
+      if (i < n && bit_at(sr, idx[i])) v |= 1u << ((q >> 3) * 8 + (7 - (q & 7)));
+    }
+    ((uint32_t*)(dst + (size_t)t * rbd))[w] = v;
+  }
+}
+void gather_bits(const Bits& src, const int* idx, int n, uint8_t* dst, size_t rbd, hipStream_t st)
+{
+  const long long tot = (long long)src.nr * (long long)(rbd / 4);
+  if (tot <= 0) return;
+  hipLaunchKernelGGL(k_gather_bits, dim3(cdiv(tot, 256) < 65536 ? cdiv(tot, 256) : 65536), dim3(256), 0, st,
+                     src.p, src.rb, src.nr, idx, n, dst, rbd);
+}
+
 // Plain +-1 products of many rows with a few covariate rows (AnZt_vv with Z = T,
 // AnZt_vv.c:62-95): out[tlist[t]*ldo + alist[i]] = base[15] - 2 pop((A_i ^ T_t) & mask).
 // One wave per row, 4 rows per block; the covariate rows stay in L1/L2.

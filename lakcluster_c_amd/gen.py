@@ -191,3 +191,126 @@ CONFIGS = {
     "case_control_16384x65536": dict(nA=16384, ncols=65536, nZ=16384, T=3, pmin=0.05, pmax=0.5,
                                      planted=(0.55,), gamma=0.02, B=32),
 }
+
+
+# ---------------------------------------------------------------------------
+# pooled patients and permutation-null jobs (many eliminations on one GPU)
+# ---------------------------------------------------------------------------
+def pack_rows(bits: np.ndarray, rb: int) -> np.ndarray:
+    """Boolean (n x m) -> packed rows of rb bytes, MSB first (the device row layout)."""
+    n, m = bits.shape
+    out = np.zeros((n, rb), dtype=np.uint8)
+    out[:, : (m + 7) // 8] = np.packbits(bits, axis=1, bitorder="big")
+    return out
+
+
+def bsize(n: int) -> int:
+    """Bytes of one packed row of n bits: whole 1920-bit blocks (lakcluster_header.h:268)."""
+    e = n + (16 - n % 16) % 16
+    return ((e + 1919) // 1920) * 240
+
+
+@dataclass
+class Pool:
+    """Patients of a case-control problem pooled: rows (packed, nrows x bsize(ncols)),
+    covariate columns as rows (ncov x bsize(nrows)); the first nA rows are the cases."""
+    rows: np.ndarray
+    cov: np.ndarray
+    nrows: int
+    ncols: int
+    ncov: int
+    nA: int
+
+
+def make_pool(nA: int, nZ: int, ncols: int, T: int = 3, pmin: float = 0.05, pmax: float = 0.5,
+              planted: Sequence[float] = (0.55,), flip: float = 0.1, seed: int = 1,
+              chunk: int = 2048) -> Pool:
+    """The case-control generator of make_dataset, kept packed in memory: A (with the
+    planted bicluster) then Z, the same column frequencies, covariates T then S."""
+    rng = np.random.default_rng(seed)
+    p = rng.uniform(pmin, pmax, size=ncols).astype(np.float32)
+    rb = bsize(ncols)
+    n = nA + nZ
+    rows = np.zeros((n, rb), dtype=np.uint8)
+    plants = []
+    for e in planted:
+        m, mc = min(int(round(nA ** e)), nA), min(int(round(ncols ** e)), ncols)
+        r = np.sort(rng.permutation(nA)[:m]); cidx = np.sort(rng.permutation(ncols)[:mc])
+        u = rng.random(m) < 0.5; v = rng.random(mc) < 0.5
+        blk = ~(u[:, None] ^ v[None, :]); blk ^= rng.random((m, mc)) < flip
+        plants.append((r, cidx, blk))
+    for r0 in range(0, n, chunk):
+        r1 = min(n, r0 + chunk)
+        bits = rng.random((r1 - r0, ncols), dtype=np.float32) < p[None, :]
+        for (r, cidx, blk) in plants:
+            sel = (r >= r0) & (r < r1)
+            if sel.any():
+                bits[np.ix_(r[sel] - r0, cidx)] = blk[sel]
+        rows[r0:r1] = pack_rows(bits, rb)
+    cov_rows = np.concatenate([_covariates(rng, nA, T), _covariates(rng, nZ, T)], axis=0) if nZ else _covariates(rng, nA, T)
+    cov = pack_rows(cov_rows.T, bsize(n))
+    return Pool(rows, cov, n, ncols, T, nA)
+
+
+@dataclass
+class NullJob:
+    rowsA: np.ndarray
+    rowsZ: np.ndarray
+    colmask: np.ndarray      # packed, bsize(ncols) bytes
+    ncols_active: int
+
+
+def null_jobs(pool: Pool, n_jobs: int, seed: int, rows: Tuple[int, int] = (2048, 16384),
+              cols: Tuple[int, int] = (16384, 65536)) -> List[NullJob]:
+    """Permutation nulls: each job takes a uniform row subsample of the pooled patients,
+    log-uniform in size, whose case/control labels are a fresh shuffle of the pooled
+    labels (SURVEY C4 / dir_m/lakcluster_uADZSZDA_shuffle_1.m), and a uniform column
+    subset, log-uniform in size, as a column mask."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_jobs):
+        nr = int(round(np.exp(rng.uniform(np.log(rows[0]), np.log(min(rows[1], pool.nrows))))))
+        nc = int(round(np.exp(rng.uniform(np.log(cols[0]), np.log(min(cols[1], pool.ncols))))))
+        labels = rng.permutation(np.r_[np.ones(pool.nA, bool), np.zeros(pool.nrows - pool.nA, bool)])
+        pick = np.sort(rng.permutation(pool.nrows)[:nr])
+        a = pick[labels[pick]].astype(np.int32)
+        z = pick[~labels[pick]].astype(np.int32)
+        if a.size == 0:
+            a, z = z[:1], z[1:]
+        cm = np.zeros(pool.ncols, bool)
+        cm[rng.permutation(pool.ncols)[:nc]] = True
+        out.append(NullJob(a, z, pack_rows(cm[None, :], bsize(pool.ncols))[0], nc))
+    return out
+
+
+def write_job(root: str, name: str, pool: Pool, job: NullJob, gamma: float, B: int) -> Dataset:
+    """The same job as files and a .in (rows as A/Z, covariates, the column mask as
+    mc_A.b16), for the oracle and for the file-driven path."""
+    d_in = os.path.join(root, f"dir_{name}_in")
+    d_out = os.path.join(root, f"dir_{name}_out")
+    os.makedirs(d_in, exist_ok=True)
+    os.makedirs(d_out, exist_ok=True)
+    unpack = lambda R: np.unpackbits(R, axis=1, bitorder="big")[:, :pool.ncols].astype(bool)
+    A = unpack(pool.rows[job.rowsA]); Z = unpack(pool.rows[job.rowsZ])
+    covb = np.unpackbits(pool.cov, axis=1, bitorder="big")[:, :pool.nrows].astype(bool).T   # nrows x T
+    write_b16(os.path.join(d_in, "A_n.b16"), A); write_b16(os.path.join(d_in, "A_t.b16"), A.T)
+    write_b16(os.path.join(d_in, "Z_n.b16"), Z); write_b16(os.path.join(d_in, "Z_t.b16"), Z.T)
+    Tm, Sm = covb[job.rowsA], covb[job.rowsZ]
+    write_b16(os.path.join(d_in, "T_n.b16"), Tm); write_b16(os.path.join(d_in, "T_t.b16"), Tm.T)
+    write_b16(os.path.join(d_in, "S_n.b16"), Sm); write_b16(os.path.join(d_in, "S_t.b16"), Sm.T)
+    cm = np.unpackbits(job.colmask, bitorder="big")[:pool.ncols].astype(bool)
+    write_b16(os.path.join(d_in, "mc_A.b16"), cm[:, None])
+    lines = ["GLOBAL_verbose= 0;", "GLOBAL_thread_count= 8;", "GLOBAL_omp_type= 1;",
+             "GLOBAL_TEST_TYPE= lakcluster_driver;", "GLOBAL_QR_strategy= YnWt;", "GLOBAL_QC_strategy= YnWt;",
+             "GLOBAL_NBINS= 1;", f"GLOBAL_B_MLT= {B};", f"GLOBAL_gamma= {gamma:.4f};",
+             f"GLOBAL_DIR_XPRE= {d_in};", "GLOBAL_A_n_name_= A_n.b16;", "GLOBAL_A_t_name_= A_t.b16;",
+             f"GLOBAL_A_n_rows_= {A.shape[0]};", f"GLOBAL_A_n_cols= {pool.ncols};", "GLOBAL_A_n_cind= mc_A.b16;",
+             "GLOBAL_Z_n_name_= Z_n.b16;", "GLOBAL_Z_t_name_= Z_t.b16;", f"GLOBAL_Z_n_rows_= {Z.shape[0]};",
+             "GLOBAL_Y_n_cols= 0;", f"GLOBAL_T_n_cols= {pool.ncov};",
+             "GLOBAL_T_n_name_= T_n.b16;", "GLOBAL_T_t_name_= T_t.b16;",
+             "GLOBAL_S_n_name_= S_n.b16;", "GLOBAL_S_t_name_= S_t.b16;",
+             f"GLOBAL_DIR_NAME= {d_out};", "END= 0;"]
+    in_path = os.path.join(d_in, f"{name}.in")
+    with open(in_path, "w") as fp:
+        fp.write("\n".join(lines) + "\n")
+    return Dataset(in_path, d_out, A.shape[0], Z.shape[0], pool.ncols, pool.ncov, gamma, B)

@@ -78,6 +78,12 @@ def lib():
         "lkc_xcalc": (C.c_int, [dp, ip, C.c_int, C.c_int, C.c_int, C.c_int, C.c_longlong,
                                 C.POINTER(C.c_ulonglong), dp]),
         "lkc_transpose_bits": (C.c_int, [u8p, C.c_size_t, C.c_int, C.c_int, u8p, C.c_size_t]),
+        "lkc_pool_create": (C.c_int, [u8p, C.c_size_t, C.c_int, C.c_int, u8p, C.c_size_t, C.c_int, C.POINTER(vp)]),
+        "lkc_pool_free": (None, [vp]),
+        "lkc_bcc_load_job": (C.c_int, [vp, ip, C.c_int, ip, C.c_int, u8p, C.POINTER(JobParams), C.POINTER(vp)]),
+        "lkc_batch_run": (C.c_int, [vp, C.POINTER(Job), C.c_int, C.c_int, C.c_double, C.POINTER(JobResult)]),
+        "lkc_batch_results_free": (None, [C.POINTER(JobResult), C.c_int]),
+        "lkc_job_bytes": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -95,6 +101,23 @@ def _check(rc: int, handle=None) -> None:
 
 def _ptr(a: np.ndarray, ct):
     return a.ctypes.data_as(C.POINTER(ct))
+
+
+class JobParams(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("gamma_type", C.c_int), ("B_MLT", C.c_int),
+                ("kappa_squared", C.c_double), ("sparse", C.c_int)]
+
+
+class Job(C.Structure):
+    _fields_ = [("rowsA", C.POINTER(C.c_int)), ("nA", C.c_int), ("rowsZ", C.POINTER(C.c_int)), ("nZ", C.c_int),
+                ("colmask", C.POINTER(C.c_uint8)), ("params", JobParams), ("arrival_s", C.c_double)]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("n_iter", C.c_int), ("ops", C.c_longlong), ("arrival_s", C.c_double),
+                ("start_s", C.c_double), ("end_s", C.c_double), ("n_trace", C.c_long),
+                ("trace", C.POINTER(C.c_double)), ("n_xdrop", C.c_long), ("xdrop_a", C.POINTER(C.c_int)),
+                ("xdrop_b", C.POINTER(C.c_int))]
 
 
 class Bcc:
@@ -270,3 +293,78 @@ def an_ztsn_ww(A: np.ndarray, alist, amask: np.ndarray, L: np.ndarray, nlist, n_
                                 _ptr(amask, C.c_uint8), _ptr(L, C.c_double), _ptr(nlist, C.c_int), nlist.size,
                                 n_total, ns, B, D, _ptr(out, C.c_double)))
     return out[:, :ns]
+
+
+def _params(gamma: float, B: int, gamma_type: int = 0, kappa_squared: float = 0.0, sparse: int = 1) -> JobParams:
+    return JobParams(gamma, gamma_type, B, kappa_squared, sparse)
+
+
+class Pool:
+    """Patients kept on the device (lkc_pool): packed rows (nrows x rb) and the
+    covariate columns as rows (ncov x rbc)."""
+
+    def __init__(self, rows: np.ndarray, nrows: int, ncols: int, cov: np.ndarray, ncov: int):
+        self._keep = (np.ascontiguousarray(rows, dtype=np.uint8), np.ascontiguousarray(cov, dtype=np.uint8))
+        r, c = self._keep
+        self._h = C.c_void_p()
+        _check(lib().lkc_pool_create(_ptr(r, C.c_uint8), r.shape[1], nrows, ncols, _ptr(c, C.c_uint8), c.shape[1],
+                                     ncov, C.byref(self._h)))
+        self.nrows, self.ncols, self.ncov = nrows, ncols, ncov
+
+    def close(self) -> None:
+        if self._h:
+            lib().lkc_pool_free(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def job(self, rowsA, rowsZ, colmask: Optional[np.ndarray], gamma: float, B: int, **kw) -> "Bcc":
+        """One job as a Bcc handle (lkc_bcc_load_job)."""
+        a = np.ascontiguousarray(rowsA, dtype=np.int32)
+        z = np.ascontiguousarray(rowsZ, dtype=np.int32)
+        cm = _ptr(np.ascontiguousarray(colmask, dtype=np.uint8), C.c_uint8) if colmask is not None else C.POINTER(C.c_uint8)()
+        h = C.c_void_p()
+        prm = _params(gamma, B, **kw)
+        _check(lib().lkc_bcc_load_job(self._h, _ptr(a, C.c_int), a.size, _ptr(z, C.c_int), z.size, cm,
+                                      C.byref(prm), C.byref(h)))
+        b = Bcc.__new__(Bcc)
+        b._h = h
+        return b
+
+    def batch(self, jobs, max_concurrent: int, budget_bytes: float = 0.0, arrivals=None):
+        """Run jobs [(rowsA, rowsZ, colmask, gamma, B)] with at most max_concurrent at a time
+        (lkc_batch_run).  Returns a list of dicts: status, n_iter, ops, arrival/start/end
+        seconds, trace (n x 6), xdrop_a, xdrop_b (n x 2)."""
+        n = len(jobs)
+        keep = []
+        arr = (Job * max(n, 1))()
+        for j, (ra, rz, cm, gamma, B) in enumerate(jobs):
+            a = np.ascontiguousarray(ra, dtype=np.int32); z = np.ascontiguousarray(rz, dtype=np.int32)
+            keep += [a, z]
+            arr[j].rowsA = _ptr(a, C.c_int); arr[j].nA = a.size
+            arr[j].rowsZ = _ptr(z, C.c_int); arr[j].nZ = z.size
+            if cm is not None:
+                m = np.ascontiguousarray(cm, dtype=np.uint8); keep.append(m)
+                arr[j].colmask = _ptr(m, C.c_uint8)
+            arr[j].params = _params(gamma, B)
+            arr[j].arrival_s = float(arrivals[j]) if arrivals is not None else 0.0
+        res = (JobResult * max(n, 1))()
+        _check(lib().lkc_batch_run(self._h, arr, n, max_concurrent, budget_bytes, res))
+        out = []
+        for j in range(n):
+            r = res[j]
+            tr = np.ctypeslib.as_array(r.trace, shape=(max(r.n_trace, 1) * 6,))[: r.n_trace * 6].reshape(-1, 6).copy()
+            xa = np.ctypeslib.as_array(r.xdrop_a, shape=(max(r.n_xdrop, 1) * 2,))[: r.n_xdrop * 2].reshape(-1, 2).copy()
+            xb = np.ctypeslib.as_array(r.xdrop_b, shape=(max(r.n_xdrop, 1) * 2,))[: r.n_xdrop * 2].reshape(-1, 2).copy()
+            out.append(dict(status=r.status, n_iter=r.n_iter, ops=r.ops, arrival_s=r.arrival_s, start_s=r.start_s,
+                            end_s=r.end_s, trace=tr, xdrop_a=xa, xdrop_b=xb))
+        lib().lkc_batch_results_free(res, n)
+        return out
+
+
+def job_bytes(nA: int, nZ: int, ncols: int, B: int) -> float:
+    return float(lib().lkc_job_bytes(nA, nZ, ncols, B))
