@@ -21,6 +21,8 @@ import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+# concurrent eliminations each drive their own stream: give them hardware queues
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 sys.path.insert(0, ROOT)
 
 PEAK_INT8_TOPS = 5000.0      # MI355X dense int8 MFMA, TOP/s
@@ -74,6 +76,66 @@ def _traffic(workload):
     return t["fetch_kib_per_launch_raw"] * 1024 * t["fetch_scale"] + t["write_kib_per_launch_raw"] * 1024 * t["write_scale"]
 
 
+def batched_legs(n_null=16, k_null=8, n_serve=24, k_serve=8, seed=7):
+    """Many eliminations on one GPU, from a device-resident pool of the C2 patients
+    (16384 cases + 16384 controls x 65536 SNPs, T/S 3 columns):
+    * permutation nulls: row subsamples log-uniform in [2048, 16384] with re-shuffled
+      case/control labels, column subsets log-uniform in [16384, 65536], gamma 0.05,
+      B 32; the same jobs run one after another (the plain GPU path) and k at a time;
+    * serving: a Poisson stream of jobs (rows log-uniform in [1024, 32768], columns in
+      [8192, 65536], half case-only, gamma ~ U[0.01, 0.1]) admitted by an HBM budget.
+    Traces of the batched runs are compared with the sequential ones (bit for bit)."""
+    import numpy as np
+    from lakcluster_c_amd import gen, lkc
+    t0 = time.time()
+    pool = gen.make_pool(16384, 16384, 65536, T=3, seed=1)
+    print(f"[bench] pool generated in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    jobs = gen.null_jobs(pool, n_null, seed=seed, rows=(2048, 16384), cols=(16384, 65536))
+    spec = [(j.rowsA, j.rowsZ, j.colmask, 0.05, 32) for j in jobs]
+    rng = np.random.default_rng(seed + 1)
+    sjobs = gen.null_jobs(pool, n_serve, seed=seed + 2, rows=(1024, 32768), cols=(8192, 65536))
+    sspec = []
+    for i, j in enumerate(sjobs):
+        g = float(rng.uniform(0.01, 0.1))
+        if i % 2:
+            sspec.append((np.sort(np.r_[j.rowsA, j.rowsZ]).astype(np.int32), np.zeros(0, np.int32), j.colmask, g, 32))
+        else:
+            sspec.append((j.rowsA, j.rowsZ, j.colmask, g, 32))
+    with lkc.Pool(pool.rows, pool.nrows, pool.ncols, pool.cov, pool.ncov) as P:
+        P.batch(spec[:1], max_concurrent=1)                      # warm-up
+        t = time.time(); plain = P.batch(spec, max_concurrent=1); tp = time.time() - t
+        print(f"[bench] nulls: {n_null} runs one at a time in {tp:.2f} s", file=sys.stderr, flush=True)
+        t = time.time(); bat = P.batch(spec, max_concurrent=k_null); tb = time.time() - t
+        print(f"[bench] nulls: {n_null} runs {k_null} at a time in {tb:.2f} s", file=sys.stderr, flush=True)
+        same = all(a["status"] == 0 and a["trace"].tobytes() == b["trace"].tobytes()
+                   and np.array_equal(a["xdrop_a"], b["xdrop_a"]) for a, b in zip(plain, bat))
+        # serving: mean service time from a short sequential pass sets the arrival rate (~0.8 load)
+        t = time.time(); probe = P.batch(sspec[:4], max_concurrent=1); ts = (time.time() - t) / 4
+        rate = 0.8 * k_serve / max(ts, 1e-3) / 2.0
+        arr = np.cumsum(rng.exponential(1.0 / rate, size=len(sspec)))
+        arr -= arr[0]
+        t = time.time(); srv = P.batch(sspec, max_concurrent=k_serve, arrivals=arr); tsrv = time.time() - t
+    lat = np.array([r["end_s"] - r["arrival_s"] for r in srv])
+    ok = all(r["status"] == 0 for r in srv)
+    print(f"[bench] serving: {len(sspec)} jobs in {tsrv:.2f} s", file=sys.stderr, flush=True)
+    return {
+        "batched_nulls": {
+            "runs": n_null, "plain_runs_per_hour": n_null / tp * 3600, "batched_runs_per_hour": n_null / tb * 3600,
+            "ratio": tp / tb, "concurrent": k_null, "identical_to_plain": bool(same),
+            "passes_mean": float(np.mean([r["n_iter"] for r in plain])),
+            "sample": f"{n_null} permutation-null jobs of the C4 family (rows log-U[2048,16384] of the pooled "
+                      "16384+16384 C2 patients, labels re-shuffled, columns log-U[16384,65536], gamma 0.05, B 32)",
+        },
+        "serving": {
+            "jobs": len(sspec), "concurrent": k_serve, "arrival_rate_per_s": rate, "jobs_per_hour": len(sspec) / tsrv * 3600,
+            "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+            "all_ok": bool(ok),
+            "sample": "Poisson arrivals; rows log-U[1024,32768], columns log-U[8192,65536] from the C2 pool, "
+                      "half case-only, gamma ~ U[0.01,0.1], B 32 (C5 sizes capped at the pool)",
+        },
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +145,7 @@ def main():
     ap.add_argument("--max-iter", type=int, default=-1, help="elimination passes per run (-1: to the end)")
     ap.add_argument("--data-root", default=os.path.join(tempfile.gettempdir(), "lkc_bench_data"))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-null and serving legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this path runs on one GPU; --gpus must be 1",
@@ -153,6 +216,9 @@ def main():
             "avg_launch_ms": kms / kl if kl else None,
         },
     }
+    if not args.no_batch:
+        print("[bench] batched nulls and serving ...", file=sys.stderr, flush=True)
+        line.update(batched_legs())
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(args.data_root)

@@ -176,21 +176,51 @@ void sort_scores(double* d, int* perm, int n) { quicksort_first(d, perm, 0, n - 
 // ------------------------------------------------------------------
 // device memory
 // ------------------------------------------------------------------
-Dev::~Dev() { for (void* p : ptrs) (void)hipFree(p); }
+Dev::~Dev() { release_all(); }
+void Dev::release_all()
+{
+  for (void* p : ptrs) (void)(st ? hipFreeAsync(p, st) : hipFree(p));
+  ptrs.clear();
+}
 void* Dev::raw(size_t bytes)
 {
   void* p = nullptr;
-  HIPCHK(hipMalloc(&p, std::max<size_t>(bytes, 256)));
+  bytes = std::max<size_t>(bytes, 256);
+  if (st) HIPCHK(hipMallocAsync(&p, bytes, st));
+  else HIPCHK(hipMalloc(&p, bytes));
   ptrs.push_back(p);
   return p;
+}
+void Dev::release(void* p)
+{
+  if (st) HIPCHK(hipFreeAsync(p, st));
+  else { HIPCHK(hipDeviceSynchronize()); HIPCHK(hipFree(p)); }
+  ptrs.erase(std::find(ptrs.begin(), ptrs.end(), p));
 }
 void* Grow::need(Dev& d, size_t bytes)
 {
   if (bytes <= cap && p) return p;
-  if (p) { HIPCHK(hipDeviceSynchronize()); HIPCHK(hipFree(p)); d.ptrs.erase(std::find(d.ptrs.begin(), d.ptrs.end(), p)); }
+  if (p) d.release(p);
   cap = std::max(bytes, cap + cap / 2);
   p = d.raw(cap);
   return p;
+}
+
+// the run's stream; device memory of the run comes from the pool, kept for reuse
+static void make_stream(hipStream_t* st, Dev& dev)
+{
+  HIPCHK(hipSetDevice(0));
+  HIPCHK(hipStreamCreateWithFlags(st, hipStreamNonBlocking));
+  static const bool pool_set = [] {   // once per process, thread-safe
+    hipMemPool_t mp;
+    if (hipDeviceGetDefaultMemPool(&mp, 0) == hipSuccess) {
+      uint64_t keep = UINT64_MAX;
+      (void)hipMemPoolSetAttribute(mp, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+    return true;
+  }();
+  (void)pool_set;
+  dev.st = *st;
 }
 
 static void upload_bits(Dev& dev, DMat& m, const HostBits& h)
@@ -198,7 +228,8 @@ static void upload_bits(Dev& dev, DMat& m, const HostBits& h)
   m.nr = h.nr; m.nc = h.nc; m.rb = h.rb;
   size_t bytes = (size_t)std::max(h.nr, 1) * h.rb;
   m.p = dev.alloc<uint8_t>(bytes);
-  HIPCHK(hipMemcpy(m.p, h.b.data(), bytes, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpyAsync(m.p, h.b.data(), bytes, hipMemcpyHostToDevice, dev.st));
+  HIPCHK(hipStreamSynchronize(dev.st));
 }
 
 // ------------------------------------------------------------------
@@ -249,6 +280,8 @@ void Prof::collect()
 // ------------------------------------------------------------------
 Run::~Run()
 {
+  dev.release_all();
+  if (st) (void)hipStreamSynchronize(st);
   if (hctl) (void)hipHostFree(hctl);
   if (hsv) (void)hipHostFree(hsv);
   if (hidx) (void)hipHostFree(hidx);
@@ -268,8 +301,7 @@ void Run::load(const char* in_text)
   if (c.scramble_num > 0) throw Error(LKC_ERR_CONFIG, "scramble is not supported");
   if (c.B < 1 || c.B > 32) throw Error(LKC_ERR_CONFIG, "GLOBAL_B_MLT must be in [1,32]");
   if (c.A_t.empty() && c.A_n.empty()) throw Error(LKC_ERR_CONFIG, "an A_n or A_t file is required");
-  HIPCHK(hipSetDevice(0));
-  HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  make_stream(&st, dev);
 
   // Either copy of a matrix may be omitted: the other one is transposed on the device
   // (the file pair of the reference, wrap_transpose in binary_read.c:174-269).
@@ -353,8 +385,7 @@ void Run::load_job(const Pool& P, const int* rowsA, int nA, const int* rowsZ, in
   if (nA <= 0 || nZ < 0 || P.M <= 0) throw Error(LKC_ERR_SHAPE, "a job needs at least one A row");
   for (int i = 0; i < nA; i++) if (rowsA[i] < 0 || rowsA[i] >= P.P) throw Error(LKC_ERR_SHAPE, "A row outside the pool");
   for (int i = 0; i < nZ; i++) if (rowsZ[i] < 0 || rowsZ[i] >= P.P) throw Error(LKC_ERR_SHAPE, "Z row outside the pool");
-  HIPCHK(hipSetDevice(0));
-  HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  make_stream(&st, dev);
   NA = nA; NZ = nZ; M = P.M; T = P.T; has_Z = nZ > 0; pcols = psize(M);
   if (T < 1 || T > kMaxT) throw Error(LKC_ERR_CONFIG, "between 1 and 8 covariate columns are supported");
   auto mk = [](int n) { return std::vector<uint8_t>(bsize(n) + 8, 0); };
@@ -491,7 +522,8 @@ void Run::setup_device(double t0)
       hajdk[nc + (size_t)jk * pcols] = pow(hA_p[nc] - (1 - hA_p[nc]), j) * pow(1.0 / std::max(0.01, (4.0 * hA_p[nc] * (1 - hA_p[nc]))), k);
   }
   ajdk = dev.alloc<double>(hajdk.size());
-  HIPCHK(hipMemcpy(ajdk, hajdk.data(), sizeof(double) * hajdk.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpyAsync(ajdk, hajdk.data(), sizeof(double) * hajdk.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
 
   // xcalc orientation flags (bcc_double_init_lf, bcc.c:1005-1150)
   int rdrop0 = 0; get_xdrop(Arb, popmask(hAc_b, M), c.gamma, c.gamma_type, &rdrop0, NULL);
@@ -880,7 +912,8 @@ void Run::sumscores()
   };
   auto exact_order = [&](int off, int n, std::vector<double>& sv, std::vector<int>& perm) {
     std::vector<double> raw(n);
-    if (n) HIPCHK(hipMemcpy(raw.data(), dsv + off, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (n) HIPCHK(hipMemcpyAsync(raw.data(), dsv + off, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     for (int j = 0; j < n; j++) perm[j] = j;
     sort_scores(raw.data(), perm.data(), n);
     sv.assign(raw.begin(), raw.end());
@@ -1170,7 +1203,8 @@ long Run::debug_get(const std::string& name, std::vector<double>& out)
   HIPCHK(hipStreamSynchronize(st));
   auto fetch = [&](const double* p, size_t n) {
     std::vector<double> h(n);
-    if (n) HIPCHK(hipMemcpy(h.data(), p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (n) HIPCHK(hipMemcpyAsync(h.data(), p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     return h;
   };
   auto transpose = [&](const double* p, size_t rows, size_t cols) {   // device [c*rows + r] -> [r*cols + c]

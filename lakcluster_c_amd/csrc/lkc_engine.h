@@ -24,10 +24,16 @@ void parse_cfg(const char* text, Cfg& c);
 void get_xdrop(double lrij, double lcij, double gamma, int gamma_type, int* rdrop, int* cdrop);
 void sort_scores(double* d, int* perm, int n);
 
+// Device allocations of one owner.  With a stream set, they are stream-ordered
+// (hipMallocAsync / hipFreeAsync from the device's memory pool): growing or freeing a
+// buffer then never synchronises the device, which matters when jobs run side by side.
 struct Dev {
   std::vector<void*> ptrs;
+  hipStream_t st = nullptr;
   ~Dev();
   void* raw(size_t bytes);
+  void release(void* p);
+  void release_all();
   template <class T> T* alloc(size_t n) { return (T*)raw(n * sizeof(T)); }
 };
 struct Grow { void* p = nullptr; size_t cap = 0; void* need(Dev& d, size_t bytes); };
