@@ -109,9 +109,10 @@ def batched_legs(n_null=16, k_null=8, n_serve=24, k_serve=8, seed=7):
         print(f"[bench] nulls: {n_null} runs {k_null} at a time in {tb:.2f} s", file=sys.stderr, flush=True)
         same = all(a["status"] == 0 and a["trace"].tobytes() == b["trace"].tobytes()
                    and np.array_equal(a["xdrop_a"], b["xdrop_a"]) for a, b in zip(plain, bat))
-        # serving: mean service time from a short sequential pass sets the arrival rate (~0.8 load)
-        t = time.time(); probe = P.batch(sspec[:4], max_concurrent=1); ts = (time.time() - t) / 4
-        rate = 0.8 * k_serve / max(ts, 1e-3) / 2.0
+        # serving: arrivals at about 70% of the batched throughput measured above (scaled
+        # by the serving jobs' mean sequential service time relative to the null jobs')
+        t = time.time(); P.batch(sspec[:4], max_concurrent=1); ts = (time.time() - t) / 4
+        rate = 0.7 * (n_null / tb) * (tp / n_null) / max(ts, 1e-3)
         arr = np.cumsum(rng.exponential(1.0 / rate, size=len(sspec)))
         arr -= arr[0]
         t = time.time(); srv = P.batch(sspec, max_concurrent=k_serve, arrivals=arr); tsrv = time.time() - t
