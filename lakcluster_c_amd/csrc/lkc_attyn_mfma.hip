@@ -66,34 +66,37 @@ __device__ __forceinline__ v4i spread16_cols(uint32_t wr, int h)
 // bytes 0/1 -> -1/+1 (0xFF / 0x01): ~(b * 0xFE) per byte, no carries between bytes
 __device__ __forceinline__ uint32_t pm_bytes(uint32_t b) { return ~((b << 8) - (b << 1)); }
 
-#define AT_YROW 144   // LDS row of one y: 128 bytes of +-1 values + 16 bytes of padding
+#define AT_PROW 144   // LDS row of one k: 128 position bytes + 16 bytes of padding (no bank conflicts)
 
-// Block: 4 waves x 32 rows k, looping over the y in tiles of 32.  Per term and
-// 128-column chunk of n, the tile's y rows are spread once into +-1 bytes in LDS
-// (double buffered; the next chunk's bits are loaded while the current one is used);
-// each wave spreads its pos = (A xnor T) & mask bits in registers.  Lane (r, g) of
-// step s feeds columns n0 + 64g + 16s .. +15.
+// Block: 32 rows k shared by its 4 waves; wave w owns the y sub-tile y0 + 32 w of a
+// 128-y tile.  Per term and 128-column chunk of n, the block spreads the position
+// bits pos_t = (A_k xnor T_t) & mask of its 32 k once into 0/1 bytes in LDS (double
+// buffered, the next chunk's bits loaded meanwhile), and each wave spreads its own
+// y bits (0/1) in registers: one y fragment feeds the NTG MFMAs of a step.
+//     C = sum_n pos y01,   hhAY = sum_{n in mask} a t y (+-1) = 4 C - 2 |pos| - sum_{n in mask} y
+// Lane (r, g) of step s feeds columns n0 + 64g + 16s .. +15.  The ordered y sums run
+// sub-tile by sub-tile in y order, one thread per (k, t).
 template <int NTG, int NQ>
 __global__ __launch_bounds__(256, 2) void k_attyn_mfma(AttynArgs a, const long long* cy, const long long* hhay)
 {
-  __shared__ __attribute__((aligned(16))) int8_t Ys[2][32][AT_YROW];
-  __shared__ int Gs[4][NQ][32][33];
+  __shared__ __attribute__((aligned(16))) int8_t Ps[2][NTG][32][AT_PROW];
+  __shared__ int Dm[NQ][NTG][32][33];
+  __shared__ int cpos[NQ][NTG][32];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, g = lane >> 5, r = lane & 31;
   const int t0 = blockIdx.y * NTG;
   const int nts = min(NTG, a.nt - t0);
-  const int kbase = blockIdx.x * 128 + wave * 32;
-  const int krow = min(kbase + r, a.nk - 1);
-  const int kr_orig = a.klist[krow];
-  // y loader role: row ly of the tile, columns 16 lp .. 16 lp + 15 of the chunk
-  const int ly = tid >> 3, lp = tid & 7;
-  double o[4][NTG];
-#pragma unroll
-  for (int p = 0; p < 4; p++)
-#pragma unroll
-    for (int t = 0; t < NTG; t++) o[p][t] = 0.0;
+  const int kb = blockIdx.x * 32;
+  // staging role: k row sk of the tile, 16-column piece sp of the chunk
+  const int sk = tid >> 3, sp = tid & 7;
+  const int sk_orig = a.klist[min(kb + sk, a.nk - 1)];
+  // chain role (epilogue): thread ce < 32 * nts walks (k = ce % 32, t = ce / 32)
+  const int ck = tid & 31, ct = tid >> 5;
+  const bool chain = ct < nts;
+  double o0 = 0.0, o1 = 0.0, o2 = 0.0, o3 = 0.0;
+  for (int e = tid; e < NQ * NTG * 32; e += 256) (&cpos[0][0][0])[e] = 0;
 
-  for (int y0 = 0; y0 < a.ny; y0 += 32) {
-    const int yl_orig = a.ylist[min(y0 + ly, a.ny - 1)];
+  for (int y0 = 0; y0 < a.ny; y0 += 128) {
+    const int yrow = a.ylist[min(y0 + 32 * wave + r, a.ny - 1)];
     v16i acc[NQ][NTG];
 #pragma unroll
     for (int q = 0; q < NQ; q++)
@@ -104,102 +107,106 @@ __global__ __launch_bounds__(256, 2) void k_attyn_mfma(AttynArgs a, const long l
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
       const AttynTerm& tm = a.term[q];
-      const uint8_t* Arow = tm.A + (size_t)kr_orig * tm.rbA + 8 * g;
+      const uint8_t* Arow = tm.A + (size_t)sk_orig * tm.rbA + 2 * sp;
       const uint8_t* Trow[NTG];
 #pragma unroll
-      for (int t = 0; t < NTG; t++) Trow[t] = tm.T + (size_t)a.tlist[t0 + min(t, nts - 1)] * tm.rbT + 8 * g;
-      const uint8_t* mrow = tm.mask + 8 * g;
-      const uint8_t* Ysrc = tm.Y + (size_t)yl_orig * tm.rbY + 2 * lp;
+      for (int t = 0; t < NTG; t++) Trow[t] = tm.T + (size_t)a.tlist[t0 + min(t, nts - 1)] * tm.rbT + 2 * sp;
+      const uint8_t* mrow = tm.mask + 2 * sp;
+      const uint8_t* Yrow = tm.Y + (size_t)yrow * tm.rbY + 8 * g;
       const int nchunk = (int)(tm.ldn / 128);
-      uint64_t an = 0, tn[NTG], mk = 0;
-      uint32_t yb = 0;
+      uint32_t sa = 0, sm = 0, stv[NTG];
+      uint64_t yb = 0;
+      int cnt[NTG];
 #pragma unroll
-      for (int t = 0; t < NTG; t++) tn[t] = 0;
-#define AT_LOAD(c) do { const long long bo_ = 16LL * (c); an = *(const uint64_t*)(Arow + bo_); \
-        for (int t_ = 0; t_ < NTG; t_++) tn[t_] = *(const uint64_t*)(Trow[t_] + bo_); \
-        mk = *(const uint64_t*)(mrow + bo_); yb = *(const uint16_t*)(Ysrc + bo_); } while (0)
-#define AT_YSTORE(buf) do { const uint32_t wr_ = __builtin_bitreverse32(yb); \
-        const v4i f_ = spread16_cols(wr_, 0); uint4 u_; u_.x = pm_bytes((uint32_t)f_[0]); u_.y = pm_bytes((uint32_t)f_[1]); \
-        u_.z = pm_bytes((uint32_t)f_[2]); u_.w = pm_bytes((uint32_t)f_[3]); *(uint4*)&Ys[buf][ly][16 * lp] = u_; } while (0)
-      __syncthreads();   // the previous readers of Ys are done
-      if (nchunk > 0) { AT_LOAD(0); AT_YSTORE(0); }
+      for (int t = 0; t < NTG; t++) { stv[t] = 0; cnt[t] = 0; }
+#define AT_LOAD(c) do { const long long bo_ = 16LL * (c); sa = *(const uint16_t*)(Arow + bo_); \
+        sm = *(const uint16_t*)(mrow + bo_); \
+        _Pragma("unroll") for (int t_ = 0; t_ < NTG; t_++) stv[t_] = *(const uint16_t*)(Trow[t_] + bo_); \
+        yb = *(const uint64_t*)(Yrow + bo_); } while (0)
+#define AT_STAGE(buf) do { _Pragma("unroll") for (int t_ = 0; t_ < NTG; t_++) { \
+          const uint32_t p_ = ~(sa ^ stv[t_]) & sm & 0xFFFFu; cnt[t_] += __builtin_popcount(p_); \
+          const v4i f_ = spread16_cols(__builtin_bitreverse32(p_), 0); \
+          *(v4i*)&Ps[buf][t_][sk][16 * sp] = f_; } } while (0)
+      __syncthreads();   // the previous readers of Ps are done
+      uint64_t ycur = 0;
+      if (nchunk > 0) { AT_LOAD(0); AT_STAGE(0); ycur = yb; }
       __syncthreads();
       for (int c = 0; c < nchunk; c++) {
         const int buf = c & 1;
-        uint32_t prv[NTG][2];
-#pragma unroll
-        for (int t = 0; t < NTG; t++) {
-          const uint64_t pos8 = ~(an ^ tn[t]) & mk;
-          prv[t][0] = __builtin_bitreverse32((uint32_t)pos8);
-          prv[t][1] = __builtin_bitreverse32((uint32_t)(pos8 >> 32));
-        }
+        const uint32_t yr0 = __builtin_bitreverse32((uint32_t)ycur), yr1 = __builtin_bitreverse32((uint32_t)(ycur >> 32));
         if (c + 1 < nchunk) AT_LOAD(c + 1);
-#pragma unroll 2
+#pragma unroll
         for (int s = 0; s < 4; s++) {
-          const v4i bf = *(const v4i*)&Ys[buf][r][64 * g + 16 * s];
+          const v4i bf = spread16_cols(s < 2 ? yr0 : yr1, s & 1);
 #pragma unroll
           for (int t = 0; t < NTG; t++)
-            acc[q][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(spread16_cols(prv[t][s >> 1], s & 1), bf, acc[q][t], 0, 0, 0);
+            acc[q][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(*(const v4i*)&Ps[buf][t][r][64 * g + 16 * s], bf,
+                                                               acc[q][t], 0, 0, 0);
         }
-        if (c + 1 < nchunk) AT_YSTORE(buf ^ 1);
+        if (c + 1 < nchunk) { AT_STAGE(buf ^ 1); ycur = yb; }
         __syncthreads();
       }
 #undef AT_LOAD
-#undef AT_YSTORE
+#undef AT_STAGE
+      // |pos| per (k, t) of this term: the 8 pieces of a row add up (integers)
+      if (y0 == 0) {
+#pragma unroll
+        for (int t = 0; t < NTG; t++) atomicAdd(&cpos[q][t][sk], cnt[t]);
+      }
     }
-    // epilogue: lane < 32 walks its row's y in ascending order
-    const int yn = min(32, a.ny - y0);
-#pragma unroll
-    for (int t = 0; t < NTG; t++) {
-      if (t >= nts) break;
-#pragma unroll
-      for (int q = 0; q < NQ; q++)
-#pragma unroll
-        for (int v = 0; v < 16; v++) Gs[wave][q][8 * (v / 4) + 4 * g + (v % 4)][r] = acc[q][t][v];
+    // ordered sums over the tile's y: sub-tile by sub-tile, in y order
+    for (int w = 0; w < 4; w++) {
+      const int ys = y0 + 32 * w;
+      if (ys >= a.ny) break;
       __syncthreads();
-      if (lane < 32) {
-        // per-row constants (recomputed here to keep them out of the main loop's registers)
-        const double* alpha = a.ajdk + (size_t)1 * a.pcols;
-        const double* Dt = a.ajdk + (size_t)5 * a.pcols;
-        const int kk = a.klist[min(kbase + lane, a.nk - 1)];
-        const double aa = alpha[kk / kPopBits];
-        const int tt = a.tlist[t0 + t];
-        long long hAy[NQ];
+      if (wave == w) {
 #pragma unroll
-        for (int q = 0; q < NQ; q++) hAy[q] = (long long)a.term[q].hAy[(long long)tt * a.term[q].ldh + kk];
-        double L[NQ];
-        double p0 = o[0][t], p1 = o[1][t], p2 = o[2][t], p3 = o[3][t];
-        for (int j = 0; j < yn; j++) {
-          const int yy = a.ylist[y0 + j];
-          const double ay = alpha[yy / kPopBits], Dy = Dt[yy / kPopBits];
+        for (int q = 0; q < NQ; q++)
 #pragma unroll
-          for (int q = 0; q < NQ; q++) {
-            const long long hhAY = 2LL * Gs[wave][q][lane][j] - cy[(size_t)q * a.ny + y0 + j];
-            const long long hhaY = (long long)a.term[q].haY[(long long)tt * a.term[q].ldh + yy];
-            const long long hhay_ = hhay[q * kMaxT + t0 + t];
-            L[q] = (hhAY - hAy[q] * ay - aa * hhaY + aa * hhay_ * ay);
-          }
-          // attjn = L[0], ktrhn = L[1], ztsvn = L[2]
-          if (NQ > 2) p0 += L[0] * Dy * L[NQ - 1];
-          p1 += L[0] * Dy * L[0];
-          p2 += L[0] * Dy * L[1];
-          p3 += L[1] * Dy * L[1];
-        }
-        o[0][t] = p0; o[1][t] = p1; o[2][t] = p2; o[3][t] = p3;
+          for (int t = 0; t < NTG; t++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) Dm[q][t][8 * (v / 4) + 4 * g + (v % 4)][r] = acc[q][t][v];
       }
       __syncthreads();
+      if (chain) {
+        const double* alpha = a.ajdk + (size_t)1 * a.pcols;
+        const double* Dt = a.ajdk + (size_t)5 * a.pcols;
+        const int kk = a.klist[min(kb + ck, a.nk - 1)];
+        const double aa = alpha[kk / kPopBits];
+        const int tt = a.tlist[t0 + ct];
+        long long hAy[NQ], base[NQ], hhay_[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+          hAy[q] = (long long)a.term[q].hAy[(long long)tt * a.term[q].ldh + kk];
+          base[q] = 2LL * cpos[q][ct][ck];
+          hhay_[q] = hhay[q * kMaxT + t0 + ct];
+        }
+        const int yn = min(32, a.ny - ys);
+        for (int j = 0; j < yn; j++) {
+          const int yy = a.ylist[ys + j];
+          const double ay = alpha[yy / kPopBits], Dy = Dt[yy / kPopBits];
+          double L[NQ];
+#pragma unroll
+          for (int q = 0; q < NQ; q++) {
+            const long long hhAY = 4LL * Dm[q][ct][ck][j] - base[q] - cy[(size_t)q * a.ny + ys + j];
+            const long long hhaY = (long long)a.term[q].haY[(long long)tt * a.term[q].ldh + yy];
+            L[q] = (hhAY - hAy[q] * ay - aa * hhaY + aa * hhay_[q] * ay);
+          }
+          // attjn = L[0], ktrhn = L[1], ztsvn = L[2]
+          if (NQ > 2) o0 += L[0] * Dy * L[NQ - 1];
+          o1 += L[0] * Dy * L[0];
+          o2 += L[0] * Dy * L[1];
+          o3 += L[1] * Dy * L[1];
+        }
+      }
     }
   }
-  if (lane < 32 && kbase + lane < a.nk) {
-#pragma unroll
-    for (int t = 0; t < NTG; t++) {
-      if (t >= nts) break;
-      const long long oo = (long long)a.tlist[t0 + t] * a.ldo + a.klist[kbase + lane];
-      if (NQ > 2) a.vtszn[oo] = o[0][t];
-      a.jttan[oo] = o[1][t];
-      a.htrkn[oo] = o[2][t];
-      a.hhtrkn[oo] = o[3][t];
-    }
+  if (chain && kb + ck < a.nk) {
+    const long long oo = (long long)a.tlist[t0 + ct] * a.ldo + a.klist[kb + ck];
+    if (NQ > 2) a.vtszn[oo] = o0;
+    a.jttan[oo] = o1;
+    a.htrkn[oo] = o2;
+    a.hhtrkn[oo] = o3;
   }
 }
 
@@ -217,7 +224,7 @@ void attyn_mfma(const AttynArgs& a, void* work, size_t work_bytes, hipStream_t s
   long long* hhay = cy + 3 * (size_t)(a.ny + 1);
   hipLaunchKernelGGL(k_attyn_pre, dim3(a.ny + 1, a.nterm), dim3(64), 0, st, a, cy, hhay);
   const int ntg = a.nt >= 3 ? 3 : a.nt;
-  dim3 grid(cdiv(a.nk, 128), cdiv(a.nt, ntg));
+  dim3 grid(cdiv(a.nk, 32), cdiv(a.nt, ntg));
 #define AT_LAUNCH(NTG, NQ) hipLaunchKernelGGL((k_attyn_mfma<NTG, NQ>), grid, dim3(256), 0, st, a, cy, hhay)
   if (a.nterm == 3) {
     if (ntg == 3) AT_LAUNCH(3, 3); else if (ntg == 2) AT_LAUNCH(2, 3); else AT_LAUNCH(1, 3);
