@@ -947,6 +947,14 @@ void Run::sumscores()
   push_ctl();
 }
 
+// a Gram over the full column layout: split over the 1920-blocks when it has few tiles
+void Run::gram_any(const GramMArgs& g)
+{
+  const size_t b = gram_split_bytes(g);
+  if (b) gram_split(g, (uint32_t*)g_split.need(dev, b), st);
+  else gram_mfma(g, st);
+}
+
 // low-rank update after a drop (bcc_lrup.c:5-1138)
 void Run::lrup()
 {
@@ -969,8 +977,18 @@ void Run::lrup()
     an_ajdk(Zn.b(), zr.d, zr.n(), Ac_rmv, pcols, ajdk, mlf_rmv, vn_ajdk, NZ, st);
     ops += 15LL * zr.n() * (long long)Zn.rb * 8;
   }
-  compact_cols(An.b(), nullptr, NA, d_src, W, An_c, W, st);
-  if (has_Z) compact_cols(Zn.b(), nullptr, NZ, d_src, W, Zn_c, W, st);
+  // compacted rows over the dropped columns: gather those columns as rows of the
+  // transposed copies (padding positions give zero rows), transpose back
+  {
+    const size_t need = (size_t)32 * std::max(W, 1) * std::max(At.rb, Zt.rb);
+    uint8_t* tmp = (uint8_t*)g_ctmp.need(dev, need);
+    gather_rows(At.b(), d_src, 32 * W, tmp, st);
+    transpose_bits(Bits{ tmp, 32 * W, NA, At.rb }, (uint8_t*)An_c, (size_t)W * 4, st);
+    if (has_Z) {
+      gather_rows(Zt.b(), d_src, 32 * W, tmp, st);
+      transpose_bits(Bits{ tmp, 32 * W, NZ, Zt.rb }, (uint8_t*)Zn_c, (size_t)W * 4, st);
+    }
+  }
   int *c_mr = gcnt[0], *c_mm = gcnt[1], *c_mz = gcnt[2];
   if (ww_kernel == 0) {
     seg_counts(Anc, rtn.d, rtn.n(), nullptr, d_seg, cmp_nseg, c_mr, NA, st);
@@ -1020,7 +1038,7 @@ void Run::lrup()
       seg_counts(Zn.b(), zr.d, nz, Ac_rtn, nullptr, pcols, c_kz, NZ, st);
       GramMArgs g1{ An.p, An.rb, rmv.d, nrm, Zn.p, Zn.rb, zr.d, nz, Ac_rtn, nullptr, pcols, c_km, NA, c_kz, NZ,
                     ajdk, pcols, mlf_rtn, kn_ajdk + 6 * (size_t)NA, zn_ajdk + 6 * (size_t)NZ, knzt, nrm, 0, 0, 0 };
-      gram_mfma(g1, st);
+      gram_any(g1);
       GramMArgs g2{ Anc.p, Anc.rb, rmv.d, nrm, Znc.p, Znc.rb, zr.d, nz, nullptr, d_seg, cmp_nseg, c_mm, NA, c_mz, NZ,
                     ajdk, pcols, mlf_rmv, hn_ajdk + 6 * (size_t)NA, vn_ajdk + 6 * (size_t)NZ, hnvt, nrm, 0, 0, 0 };
       gram_mfma(g2, st);
@@ -1028,10 +1046,10 @@ void Run::lrup()
     }
     GramMArgs g3{ An.p, An.rb, rtn.d, nrt, An.p, An.rb, rmv.d, nrm, Ac_rtn, nullptr, pcols, c_kr, NA, c_km, NA,
                   ajdk, pcols, mlf_rtn, an_ajdk_ + 6 * (size_t)NA, kn_ajdk + 6 * (size_t)NA, ankt, nrt, 0, 0, 0 };
-    gram_mfma(g3, st);
+    gram_any(g3);
     GramMArgs g4{ An.p, An.rb, rmv.d, nrm, An.p, An.rb, rmv.d, nrm, Ac_rtn, nullptr, pcols, c_km, NA, c_km, NA,
                   ajdk, pcols, mlf_rtn, kn_ajdk + 6 * (size_t)NA, kn_ajdk + 6 * (size_t)NA, knkt, nrm, 0, 0, 0 };
-    gram_mfma(g4, st);
+    gram_any(g4);
   } else {
     if (has_Z) {
       gram(An.b(), rmv.d, nrm, Zn.b(), zr.d, nz, Ac_rtn, pcols, ajdk, mlf_rtn,
@@ -1075,7 +1093,11 @@ void Run::lrup()
   if (ww_kernel == 0) {
     // QC b1 + b2 fused on the matrix cores; the removed rows are packed first
     const int Wr = rmv_W;
-    compact_cols(At.b(), nullptr, M, d_src_rmv, Wr, At_rmvc, Wr, st);
+    {
+      uint8_t* tmp = (uint8_t*)g_ctmp.need(dev, (size_t)32 * std::max(Wr, 1) * An.rb);
+      gather_rows(An.b(), d_src_rmv, 32 * Wr, tmp, st);
+      transpose_bits(Bits{ tmp, 32 * Wr, M, An.rb }, (uint8_t*)At_rmvc, (size_t)Wr * 4, st);
+    }
     compact_cols(Tt.b(), nullptr, T, d_src_rmv, Wr, Tt_rmvc, Wr, st);
     AttynArgs x;
     const uint8_t* Arc = (const uint8_t*)At_rmvc;

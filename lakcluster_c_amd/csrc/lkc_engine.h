@@ -99,7 +99,7 @@ struct Run {
   uint32_t *At_rmvc = nullptr, *Tt_rmvc = nullptr;
   // ones per (1920-block segment, list position) of the matrix-core Grams' rows
   int* gcnt[6] = {};
-  Grow g_attyn, g_qrs, g_qrs2, g_ztsn;
+  Grow g_attyn, g_qrs, g_qrs2, g_ztsn, g_split, g_ctmp;
   // device arrays; (row, covariate) arrays are [t*rows + row]
   double *mlf = nullptr;
   double *lf_An_ajdk = nullptr, *lf_Zn_ajdk = nullptr;              // [jk*rows + r]
@@ -142,6 +142,7 @@ struct Run {
   void init_lf();
   void sumscores();
   void lrup();
+  void gram_any(const GramMArgs& g);
   bool active() const;
   void iterate(int* done);
   void run(int max_iter);

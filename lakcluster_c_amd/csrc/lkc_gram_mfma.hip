@@ -218,6 +218,118 @@ __global__ __launch_bounds__(256, 3 - WI) void k_gram_mfma(GramMArgs g)
   }
 }
 
+// Split over the blocks, for Grams with few output tiles (the kept-column Grams of the
+// removed rows): block (tile, q) contracts block q's 60 words alone and stores the
+// integers x_q = |a&m|_q + |z&m|_q - 2 sum_q a z; a second kernel adds D_q x_q over q in
+// order per pair and applies the epilogue, so the double arithmetic is unchanged.
+// X[(q*nz + zpos)*na + apos].
+__global__ __launch_bounds__(256) void k_gram_split(GramMArgs g, uint32_t* X)
+{
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, hg = lane >> 5, r = lane & 31;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int ntx = (g.na + 127) / 128;
+  const int tyi = blockIdx.x / ntx, txi = blockIdx.x % ntx, q = blockIdx.y;
+  const int zb = tyi * 128 + 64 * wm, ab = txi * 128 + 64 * wn;
+  const uint2* Zr[2];
+  const uint2* Ar[2];
+#pragma unroll
+  for (int i = 0; i < 2; i++) Zr[i] = (const uint2*)(g.Z + (size_t)g.zlist[min(zb + 32 * i + r, g.nz - 1)] * g.rbZ) + hg;
+#pragma unroll
+  for (int j = 0; j < 2; j++) Ar[j] = (const uint2*)(g.A + (size_t)g.alist[min(ab + 32 * j + r, g.na - 1)] * g.rbA) + hg;
+  const uint2* mk = (const uint2*)g.mask + hg;
+  v16i acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int v = 0; v < 16; v++) acc[i][j][v] = 0;
+  const int c0 = q * (kPopWords / 4);
+  uint2 zn[2], an[2], mn;
+#pragma unroll
+  for (int i = 0; i < 2; i++) zn[i] = Zr[i][2 * c0];
+#pragma unroll
+  for (int j = 0; j < 2; j++) an[j] = Ar[j][2 * c0];
+  mn = mk[2 * c0];
+  for (int c = 0; c < kPopWords / 4; c++) {
+    uint32_t zw[2][2], aw[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) { zw[i][0] = zn[i].x; zw[i][1] = zn[i].y; }
+#pragma unroll
+    for (int j = 0; j < 2; j++) { aw[j][0] = an[j].x & mn.x; aw[j][1] = an[j].y & mn.y; }
+    if (c + 1 < kPopWords / 4) {
+      const int cc = 2 * (c0 + c + 1);
+#pragma unroll
+      for (int i = 0; i < 2; i++) zn[i] = Zr[i][cc];
+#pragma unroll
+      for (int j = 0; j < 2; j++) an[j] = Ar[j][cc];
+      mn = mk[cc];
+    }
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+      v4i af[2], bf[2];
+#pragma unroll
+      for (int i = 0; i < 2; i++) af[i] = spread16((zw[i][p >> 1] >> (16 * (p & 1))) & 0xFFFFu);
+#pragma unroll
+      for (int j = 0; j < 2; j++) bf[j] = spread16((aw[j][p >> 1] >> (16 * (p & 1))) & 0xFFFFu);
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  const int* cq_a = g.ca + (long long)q * g.ldca;
+  const int* cq_z = g.cz + (long long)q * g.ldcz;
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const int ac = ab + 32 * j + r;
+    const int caj = cq_a[min(ac, g.na - 1)];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int v = 0; v < 16; v++) {
+        const int zc = zb + 32 * i + 8 * (v / 4) + 4 * hg + (v % 4);
+        const int x = cq_z[min(zc, g.nz - 1)] + caj - 2 * acc[i][j][v];
+        if (ac < g.na && zc < g.nz) X[((long long)q * g.nz + zc) * g.na + ac] = (uint32_t)x;
+      }
+  }
+}
+
+__global__ void k_gram_combine(GramMArgs g, const uint32_t* X)
+{
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)g.na * g.nz) return;
+  const int zc = (int)(e / g.na), ac = (int)(e % g.na);
+  const double* D = g.ajdk + (size_t)5 * g.pcols;
+  const long long plane = (long long)g.na * g.nz;
+  double dsum = 0.0;
+  for (int q = 0; q < g.nseg; q++) dsum += (double)D[q] * (double)X[q * plane + e];
+  const int aor = g.alist[ac], zor = g.zlist[zc];
+  double o = g.base16[5];
+  o -= (2 * dsum);
+  o -= g.xa[aor] + g.xz[zor] - g.base16[7];
+  const long long aidx = g.a_orig ? aor : ac, zidx = g.z_orig ? zor : zc;
+  double* dst = g.out + zidx * g.ldo + aidx;
+  if (g.op_sub) *dst -= o; else *dst = o;
+}
+
+size_t gram_split_bytes(const GramMArgs& g)
+{
+  if (g.seg || g.na <= 0 || g.nz <= 0 || g.nseg < 2) return 0;
+  const long long tiles = (long long)cdiv(g.na, 128) * cdiv(g.nz, 128);
+  if (tiles >= 1024) return 0;
+  const double b = 4.0 * g.nseg * (double)g.na * g.nz;
+  return b <= 4.0e9 ? (size_t)b + 256 : 0;
+}
+
+void gram_split(const GramMArgs& g, uint32_t* X, hipStream_t st)
+{
+  const unsigned tiles = cdiv(g.na, 128) * cdiv(g.nz, 128);
+  hipLaunchKernelGGL(k_gram_split, dim3(tiles, g.nseg), dim3(256), 0, st, g, X);
+  const long long n = (long long)g.na * g.nz;
+  hipLaunchKernelGGL(k_gram_combine, dim3(cdiv(n, 256)), dim3(256), 0, st, g, (const uint32_t*)X);
+}
+
 void gram_mfma(const GramMArgs& g, hipStream_t st)
 {
   if (g.na <= 0 || g.nz <= 0) return;

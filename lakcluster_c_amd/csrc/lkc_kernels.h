@@ -146,6 +146,10 @@ struct GramMArgs {
 void seg_counts(const Bits& X, const int* rows, int nrows, const uint32_t* mask, const int* seg, int nseg,
                 int* cnt, long long ld, hipStream_t st);
 void gram_mfma(const GramMArgs& g, hipStream_t st);
+// The same Gram split over the 1920-blocks for few output tiles (full layout only):
+// gram_split_bytes(g) = scratch bytes it needs, 0 when the split does not apply.
+size_t gram_split_bytes(const GramMArgs& g);
+void gram_split(const GramMArgs& g, uint32_t* X, hipStream_t st);
 // sg[t*nzp + i] = S_t(zlist[i]) (+-1) for the QR sums; returns nzp.  sgbuf: qr_sign_bytes(nz, nt)
 int qr_signs(const Bits& Sc, const int* tlist, int nt, const int* zlist, int nz, signed char* sgbuf, hipStream_t st);
 
@@ -154,7 +158,7 @@ int qr_signs(const Bits& Sc, const int* tlist, int nt, const int* zlist, int nz,
 void pm_few(const Bits& A, const int* alist, int na, const Bits& T, const int* tlist, int nt,
             const uint32_t* mask, int pcols, const double* base16, double* out, long long ldo, hipStream_t st);
 
-// dst row i = src row rows[i]; dst row t, bit i = src row t, bit idx[i] (rows of rbd bytes)
+// dst row i = src row rows[i] (rows[i] < 0: zeros); dst row t, bit i = src row t, bit idx[i] (rows of rbd bytes)
 void gather_rows(const Bits& src, const int* rows, int n, uint8_t* dst, hipStream_t st);
 void gather_bits(const Bits& src, const int* idx, int n, uint8_t* dst, size_t rbd, hipStream_t st);
 

@@ -701,15 +701,17 @@ void transpose_bits(const Bits& X, uint8_t* Y, size_t rbY, hipStream_t st)
   hipLaunchKernelGGL(k_transpose_bits, grid, dim3(256), 0, st, X.p, X.rb, X.nr, X.nc, Y, rbY);
 }
 
-// Job inputs from a device-resident pool: dst row i = src row rows[i] (rb bytes, a
-// multiple of 16), and dst row t bit i = src row t bit idx[i] (the covariate columns of
-// the chosen patients; bits past n stay zero).
+// Row gathers: dst row i = src row rows[i] (rb bytes, a multiple of 16; rows[i] < 0
+// gives a zero row), and dst row t bit i = src row t bit idx[i] (bits past n stay zero).
+// With the bit transpose the first also compacts columns: the columns of X are the rows
+// of its transposed copy.
 __global__ void k_gather_rows(const uint8_t* src, size_t rb, const int* rows, int n, uint8_t* dst)
 {
   const long long per = (long long)(rb / 16), tot = (long long)n * per;
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
     const long long i = e / per, q = e % per;
-    ((uint4*)(dst + (size_t)i * rb))[q] = ((const uint4*)(src + (size_t)rows[i] * rb))[q];
+    const int ri = rows[i];
+    ((uint4*)(dst + (size_t)i * rb))[q] = ri >= 0 ? ((const uint4*)(src + (size_t)ri * rb))[q] : make_uint4(0, 0, 0, 0);
   }
 }
 void gather_rows(const Bits& src, const int* rows, int n, uint8_t* dst, hipStream_t st)
