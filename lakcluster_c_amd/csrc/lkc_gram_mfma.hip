@@ -25,16 +25,18 @@ __global__ __launch_bounds__(64) void k_seg_pop(const uint8_t* X, size_t rb, con
   const int i = blockIdx.x;
   if (i >= nrows) return;
   const uint32_t* xr = (const uint32_t*)(X + (size_t)(rows ? rows[i] : i) * rb);
-  for (int q = 0; q < nseg; q++) {
+  // thread q counts segment q (words contiguous, independent loads)
+  for (int q = threadIdx.x; q < nseg; q += 64) {
     const int w0 = seg ? seg[3 * q] : kPopWords * q, nw = seg ? seg[3 * q + 1] : kPopWords;
     int c = 0;
-    for (int w = threadIdx.x; w < nw; w += 64) {
-      uint32_t v = xr[w0 + w];
-      if (mask) v &= mask[w0 + w];
-      c += __builtin_popcount(v);
+    for (int w = 0; w < nw; w += 4) {
+      const uint4 x = *(const uint4*)(xr + w0 + w);
+      uint4 m = make_uint4(~0u, ~0u, ~0u, ~0u);
+      if (mask) m = *(const uint4*)(mask + w0 + w);
+      c += __builtin_popcount(x.x & m.x) + __builtin_popcount(x.y & m.y) + __builtin_popcount(x.z & m.z) +
+           __builtin_popcount(x.w & m.w);
     }
-    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-    if (threadIdx.x == 0) cnt[(long long)q * ld + i] = c;
+    cnt[(long long)q * ld + i] = c;
   }
 }
 void seg_counts(const Bits& X, const int* rows, int nrows, const uint32_t* mask, const int* seg, int nseg,

@@ -69,19 +69,26 @@ void mask_lf(const uint32_t* mask, int pcols, const double* ajdk, double* out16,
 // ------------------------------------------------------------------
 // An_ajdk_v (An_ajdk_v.c:272-363): one wave per row
 // ------------------------------------------------------------------
-__global__ void k_an_ajdk(const uint8_t* X, size_t rb, const int* rows, int nrows, const uint32_t* mask, int pcols,
-                          const double* ajdk, const double* base, double* out, int ld)
+// One wave per row; thread b counts block b (15 x 16-byte words, independent loads),
+// then one thread per table adds the weighted counts over the blocks in order.
+__global__ __launch_bounds__(64) void k_an_ajdk(const uint8_t* X, size_t rb, const int* rows, int nrows,
+                                                const uint32_t* mask, int pcols, const double* ajdk,
+                                                const double* base, double* out, int ld)
 {
   extern __shared__ int cnt[];
   int r = blockIdx.x;
   if (r >= nrows) return;
   int row = rows[r];
-  const uint32_t* xr = (const uint32_t*)(X + (size_t)row * rb);
-  for (int b = 0; b < pcols; b++) {
+  const uint4* xr = (const uint4*)(X + (size_t)row * rb);
+  const uint4* mr = (const uint4*)mask;
+  for (int b = threadIdx.x; b < pcols; b += 64) {
     int c = 0;
-    if (threadIdx.x < kPopWords) c = popc(xr[(size_t)b * kPopWords + threadIdx.x] & mask[(size_t)b * kPopWords + threadIdx.x]);
-    c = wave_sum_i(c);
-    if (threadIdx.x == 0) cnt[b] = c;
+#pragma unroll
+    for (int q = 0; q < kPopWords / 4; q++) {
+      const uint4 x = xr[(size_t)b * (kPopWords / 4) + q], m = mr[(size_t)b * (kPopWords / 4) + q];
+      c += popc(x.x & m.x) + popc(x.y & m.y) + popc(x.z & m.z) + popc(x.w & m.w);
+    }
+    cnt[b] = c;
   }
   __syncthreads();
   int jk = threadIdx.x;

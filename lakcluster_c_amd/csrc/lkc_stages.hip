@@ -16,29 +16,42 @@ __device__ __forceinline__ int bit_w(const uint32_t* m, int i) { return bit_of((
 
 // ------------------------------------------------------------------
 // ordered sums Σ_i s_i·v[list[i]] (get_singlestudy_vv, bcc_flattenloop.c:359-731):
-// one block per job; 256 values are staged in LDS and lane 0 adds them in order.
+// one block per job; chunks of 2048 values are staged in LDS (the next chunk's
+// gathers in flight meanwhile) and lane 0 adds them in order.
 // ------------------------------------------------------------------
-__global__ void k_ordered_sums(OrderedJobs J)
+#define OS_CH 2048
+__global__ __launch_bounds__(256) void k_ordered_sums(OrderedJobs J)
 {
-  __shared__ double buf[256];
+  __shared__ double buf[OS_CH];
   int q = blockIdx.x;
   if (q >= J.n) return;
   const OrderedJob& jb = J.job[q];
-  double o = 0.0;
-  for (int i0 = 0; i0 < jb.n; i0 += 256) {
-    int i = i0 + threadIdx.x;
-    if (i < jb.n) {
-      int r = jb.list[i];
+  constexpr int PER = OS_CH / 256;
+  double reg[PER];
+  auto load = [&](int i0) {
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      // clamped index, no branch: all the gathers of a thread are in flight together
+      const int i = i0 + threadIdx.x + 256 * u, ic = min(i, jb.n - 1);
+      const int r = jb.list[ic];
       double v = jb.vals[r];
       if (jb.sign) v = pm_of(jb.sign, r) * v;
-      buf[threadIdx.x] = v;
+      reg[u] = i < jb.n ? v : 0.0;
     }
+  };
+  double o = 0.0;
+  if (jb.n > 0) load(0);
+  for (int i0 = 0; i0 < jb.n; i0 += OS_CH) {
     __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PER; u++) buf[threadIdx.x + 256 * u] = reg[u];
+    __syncthreads();
+    if (i0 + OS_CH < jb.n) load(i0 + OS_CH);
     if (threadIdx.x == 0) {
-      int m = jb.n - i0 < 256 ? jb.n - i0 : 256;
+      const int m = jb.n - i0 < OS_CH ? jb.n - i0 : OS_CH;
+#pragma unroll 16
       for (int e = 0; e < m; e++) o += buf[e];
     }
-    __syncthreads();
   }
   if (threadIdx.x == 0) *jb.out = o;
 }
