@@ -466,6 +466,8 @@ void Run::setup_device(double t0)
   rmv_Wmax = (((NA + 31) / 32) + 3) / 4 * 4;
   ctl_cmp_off[3] = reserve(sizeof(int) * 32 * (size_t)rmv_Wmax);
   ctl_cmp_off[4] = reserve(sizeof(uint32_t) * (size_t)rmv_Wmax);
+  ctl_cmp_off[5] = reserve(sizeof(int) * 32 * (size_t)rmv_Wmax);
+  ctl_cmp_off[6] = reserve(sizeof(uint32_t) * (size_t)rmv_Wmax);
   ctl_bytes = off;
   HIPCHK(hipHostMalloc((void**)&hctl, ctl_bytes, hipHostMallocDefault));
   memset(hctl, 0, ctl_bytes);
@@ -476,9 +478,19 @@ void Run::setup_device(double t0)
   d_zero = (int*)(dctl + o_zero);
   d_seg = (int*)(dctl + ctl_cmp_off[0]); d_src = (int*)(dctl + ctl_cmp_off[1]); d_cmask = (uint32_t*)(dctl + ctl_cmp_off[2]);
   d_src_rmv = (int*)(dctl + ctl_cmp_off[3]); d_cmask_rmv = (uint32_t*)(dctl + ctl_cmp_off[4]);
+  d_src_rtn = (int*)(dctl + ctl_cmp_off[5]); d_cmask_rtn = (uint32_t*)(dctl + ctl_cmp_off[6]);
   for (int i = 0; i < 6; i++) gcnt[i] = dev.alloc<int>((size_t)std::max(NA, NZ) * (pcols + 1));
   At_rmvc = dev.alloc<uint32_t>((size_t)M * rmv_Wmax);
   Tt_rmvc = dev.alloc<uint32_t>((size_t)T * rmv_Wmax);
+  At_rtnc = dev.alloc<uint32_t>((size_t)M * rmv_Wmax);
+  Tt_rtnc = dev.alloc<uint32_t>((size_t)T * rmv_Wmax);
+  {
+    std::vector<int> io((size_t)std::max(NA, NZ) + 1);
+    for (size_t i = 0; i < io.size(); i++) io[i] = (int)i;
+    d_iota = dev.alloc<int>(io.size());
+    HIPCHK(hipMemcpyAsync(d_iota, io.data(), sizeof(int) * io.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
   An_c = dev.alloc<uint32_t>((size_t)NA * cmp_Wmax);
   Zn_c = dev.alloc<uint32_t>((size_t)std::max(NZ, 1) * cmp_Wmax);
 
@@ -613,17 +625,19 @@ void Run::push_ctl()
   {
     layout(crm.h, (int*)(hctl + ctl_cmp_off[0]), (int*)(hctl + ctl_cmp_off[1]), (uint32_t*)(hctl + ctl_cmp_off[2]),
            cmp_W, cmp_nseg);
-    // removed rows, packed in whole 128-bit groups
-    int* srr = (int*)(hctl + ctl_cmp_off[3]);
-    uint32_t* cmr = (uint32_t*)(hctl + ctl_cmp_off[4]);
-    int nr = rmv.n(), Wr = (((nr + 31) / 32) + 3) / 4 * 4;
-    for (int q = 0; q < 32 * Wr; q++) srr[q] = q < nr ? rmv.h[q] : -1;
-    for (int w = 0; w < Wr; w++) {
-      uint32_t v = 0;
-      for (int q = 0; q < 32; q++) if (32 * w + q < nr) v |= 1u << ((q >> 3) * 8 + (7 - (q & 7)));
-      cmr[w] = v;
-    }
-    rmv_W = Wr;
+    // removed and retained rows, each packed in whole 128-bit groups
+    auto pack = [&](const IList& l, int* srr, uint32_t* cmr) {
+      int nr = l.n(), W = (((nr + 31) / 32) + 3) / 4 * 4;
+      for (int q = 0; q < 32 * W; q++) srr[q] = q < nr ? l.h[q] : -1;
+      for (int w = 0; w < W; w++) {
+        uint32_t v = 0;
+        for (int q = 0; q < 32; q++) if (32 * w + q < nr) v |= 1u << ((q >> 3) * 8 + (7 - (q & 7)));
+        cmr[w] = v;
+      }
+      return W;
+    };
+    rmv_W = pack(rmv, (int*)(hctl + ctl_cmp_off[3]), (uint32_t*)(hctl + ctl_cmp_off[4]));
+    rtn_W = pack(rtn, (int*)(hctl + ctl_cmp_off[5]), (uint32_t*)(hctl + ctl_cmp_off[6]));
   }
   IList* L[8] = { &ar, &kl, &tl, &zr, &rtn, &rmv, &crt, &crm };
   size_t last = 0;
@@ -694,7 +708,7 @@ void Run::st_YnWt()
 // bcc_M_YnWt_excerpt + bcc_lf_At_T_YnWt_S_Zn_excerpt (bcc_lf.c:304-316, 615-634).
 // L(r1, r2) = src[idx1(r1) + idx2(r2)*ld2] with idx = the side's original index when full.
 void Run::ww_excerpt(int trm, const Side& s1, const Side& s2, const IList& kls, const double* src, long long ld2,
-                     int full, double* out)
+                     int full, double* out, const Packed* pk)
 {
   const Side& pn = trm ? s2 : s1;
   const Side& pw = trm ? s1 : s2;
@@ -706,8 +720,14 @@ void Run::ww_excerpt(int trm, const Side& s1, const Side& s2, const IList& kls, 
   long long ldc = std::max(n_total, 1);
   uint32_t* cz = (uint32_t*)g_cz.need(dev, sizeof(uint32_t) * (size_t)std::max(nw, 1) * ldc);
   size_t prb = pn.X->rb;
+  // The matrix-core contraction runs over the patients of the n side only: packed to the
+  // front when a packed copy is given (the quantised values then go to list positions),
+  // else the rows' whole 128-bit groups.  The integer sums over n do not depend on where
+  // each patient sits, so both forms give the same result.
+  const bool packed = ww_kernel == 0 && pk && pk->nbits > 0;
+  const int* nout = packed ? d_iota : pn.rows->d;
   uint8_t* planes = nullptr;
-  xcalc(src, sn, sw, rl, cl, nn, nw, pn.rows->d, c.B, c.D_MLT, cz, ldc, scal, xwork, st);
+  xcalc(src, sn, sw, rl, cl, nn, nw, nout, c.B, c.D_MLT, cz, ldc, scal, xwork, st);
   if (ww_kernel == 1) {
     planes = (uint8_t*)g_planes.need(dev, (size_t)std::max(nw, 1) * c.B * prb);
     cz_planes(cz, ldc, nw, n_total, pn.mask, c.B, planes, prb, st);
@@ -720,6 +740,12 @@ void Run::ww_excerpt(int trm, const Side& s1, const Side& s2, const IList& kls, 
   a.cz = cz; a.ldc = ldc;
   a.Sc = pw.Cov->b(); a.Q = pw.X->b(); a.ajdk = ajdk; a.pcols = pcols; a.scal = scal;
   a.out = out; a.ldo = M;
+  if (packed) {
+    a.P = Bits{ pk->X, pn.X->nr, pk->nbits, pk->rb }; a.Tc = Bits{ pk->Cov, pn.Cov->nr, pk->nbits, pk->rb };
+    a.pm = pk->mask; a.row0 = pk->mask; a.prb = (size_t)pk->nbits / 8;
+  } else if (ww_kernel == 0) {
+    a.prb = std::min(prb, (size_t)((n_total + 127) / 128) * 16);
+  }
   // timed: the contraction kernel itself (the quantised-value preparation is not included)
   struct Ctx { Prof* prof; Prof::Pending pp; double work; };
   Ctx cx{ &prof, Prof::Pending(), 2.0 * tl.n() * kls.n() * nw * (double)nn * ((c.B + 7) / 8) };
@@ -731,10 +757,22 @@ void Run::ww_excerpt(int trm, const Side& s1, const Side& s2, const IList& kls, 
     ww_valu(a, (double*)g_work.need(dev, wb), wb, st);
     hook.after(&cx, st);
   } else {
-    size_t wb = ww_mfma_work_bytes(a.nk, a.nt, nw, prb, c.B);
+    size_t wb = ww_mfma_work_bytes(a.nk, a.nt, nw, a.prb, c.B);
     ww_mfma(a, g_work.need(dev, wb), wb, st, &hook);
   }
   ops += (long long)tl.n() * kls.n() * nw * (c.B + 1) * (long long)prb * 8;
+}
+
+// Packed copies of one patient group (row list src, -1 = padding, 32 W entries):
+// SNP-major rows Atc (M x 4W bytes) through a row gather of An and a bit transpose,
+// covariate rows Ttc (T x 4W bytes) by column compaction of Tt.
+void Run::pack_rows(const int* src, int W, uint32_t* Atc, uint32_t* Ttc)
+{
+  if (W <= 0) return;
+  uint8_t* tmp = (uint8_t*)g_ctmp.need(dev, (size_t)32 * W * An.rb);
+  gather_rows(An.b(), src, 32 * W, tmp, st);
+  transpose_bits(Bits{ tmp, 32 * W, M, An.rb }, (uint8_t*)Atc, (size_t)W * 4, st);
+  compact_cols(Tt.b(), nullptr, T, src, W, Ttc, W, st);
 }
 
 // bcc_M_YnWt + bcc_lf_At_T_YnWt_S_Zn (bcc_lf.c:318-356, 636-709), initial pass
@@ -1067,12 +1105,19 @@ void Run::lrup()
   // QC a2 + a3
   double *t_knzt = mt[0], *t_hnvt = mt[1], *t_ankt = mt[2], *t_knkt = mt[3];
   Side s_rmv{ &At, Ar_rmv, &rmv, &Tt }, s_rtn{ &At, Ar_rtn, &rtn, &Tt }, s_Z{ &Zt, Zr_j, &zr, &St };
-  if (has_Z) {
-    ww_excerpt(trm_knzt, s_rmv, s_Z, crt, knzt, nrm, 0, t_knzt);
-    ww_excerpt(trm_hnvt, s_rmv, s_Z, crt, hnvt, nrm, 0, t_hnvt);
+  // the removed and the retained A rows packed to the front of the patient dimension
+  if (ww_kernel == 0) {
+    pack_rows(d_src_rmv, rmv_W, At_rmvc, Tt_rmvc);
+    pack_rows(d_src_rtn, rtn_W, At_rtnc, Tt_rtnc);
   }
-  ww_excerpt(0, s_rtn, s_rmv, crt, ankt, nrt, 0, t_ankt);
-  ww_excerpt(0, s_rmv, s_rmv, crt, knkt, nrm, 0, t_knkt);
+  const Packed pk_rmv{ (const uint8_t*)At_rmvc, (const uint8_t*)Tt_rmvc, (size_t)rmv_W * 4, d_cmask_rmv, 32 * rmv_W };
+  const Packed pk_rtn{ (const uint8_t*)At_rtnc, (const uint8_t*)Tt_rtnc, (size_t)rtn_W * 4, d_cmask_rtn, 32 * rtn_W };
+  if (has_Z) {
+    ww_excerpt(trm_knzt, s_rmv, s_Z, crt, knzt, nrm, 0, t_knzt, trm_knzt ? nullptr : &pk_rmv);
+    ww_excerpt(trm_hnvt, s_rmv, s_Z, crt, hnvt, nrm, 0, t_hnvt, trm_hnvt ? nullptr : &pk_rmv);
+  }
+  ww_excerpt(0, s_rtn, s_rmv, crt, ankt, nrt, 0, t_ankt, &pk_rtn);
+  ww_excerpt(0, s_rmv, s_rmv, crt, knkt, nrm, 0, t_knkt, &pk_rmv);
   // QC b0: plain +-1 products, [t*M + k]
   double *attn = mt[4], *jttn = mt[5], *ktrn = mt[6], *htrn = mt[7], *ztsn = mt[8], *vtsn = mt[9];
   mask_lf(Ar_rtn, sA, nullptr, mlf_nrtn, st);
@@ -1092,21 +1137,20 @@ void Run::lrup()
   double *vtszn = mt[10], *jttan = mt[11], *htrkn = mt[12], *hhtrkn = mt[13];
   if (ww_kernel == 0) {
     // QC b1 + b2 fused on the matrix cores; the removed rows are packed first
-    const int Wr = rmv_W;
-    {
-      uint8_t* tmp = (uint8_t*)g_ctmp.need(dev, (size_t)32 * std::max(Wr, 1) * An.rb);
-      gather_rows(An.b(), d_src_rmv, 32 * Wr, tmp, st);
-      transpose_bits(Bits{ tmp, 32 * Wr, M, An.rb }, (uint8_t*)At_rmvc, (size_t)Wr * 4, st);
-    }
-    compact_cols(Tt.b(), nullptr, T, d_src_rmv, Wr, Tt_rmvc, Wr, st);
+    // the A terms run over the packed removed / retained rows (packed above), the Z term
+    // over the controls' whole 128-bit groups
+    const int Wr = rmv_W, Wt = rtn_W;
     AttynArgs x;
     const uint8_t* Arc = (const uint8_t*)At_rmvc;
     const uint8_t* Trc = (const uint8_t*)Tt_rmvc;
-    x.term[0] = AttynTerm{ At.p, At.rb, Tt.p, Tt.rb, At.p, At.rb, (const uint8_t*)Ar_rtn, (long long)At.rb * 8, nrt,
-                           attn, jttn, M };
+    const uint8_t* Atc = (const uint8_t*)At_rtnc;
+    const uint8_t* Ttc = (const uint8_t*)Tt_rtnc;
+    x.term[0] = AttynTerm{ Atc, (size_t)Wt * 4, Ttc, (size_t)Wt * 4, Atc, (size_t)Wt * 4, (const uint8_t*)d_cmask_rtn,
+                           (long long)Wt * 32, nrt, attn, jttn, M };
     x.term[1] = AttynTerm{ Arc, (size_t)Wr * 4, Trc, (size_t)Wr * 4, Arc, (size_t)Wr * 4, (const uint8_t*)d_cmask_rmv,
                            (long long)Wr * 32, nrm, ktrn, htrn, M };
-    x.term[2] = AttynTerm{ Zt.p, Zt.rb, St.p, St.rb, Zt.p, Zt.rb, (const uint8_t*)Zr_j, (long long)Zt.rb * 8, nz,
+    const long long zbits = std::min((long long)Zt.rb * 8, (long long)(NZ + 127) / 128 * 128);
+    x.term[2] = AttynTerm{ Zt.p, Zt.rb, St.p, St.rb, Zt.p, Zt.rb, (const uint8_t*)Zr_j, zbits, nz,
                            ztsn, vtsn, M };
     x.nterm = has_Z ? 3 : 2;
     x.klist = crt.d; x.nk = crt.n(); x.ylist = crm.d; x.ny = crm.n(); x.tlist = tl.d; x.nt = tl.n();

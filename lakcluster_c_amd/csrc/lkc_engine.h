@@ -53,6 +53,9 @@ void run_batch(const Pool& P, const JobIn* jobs, int n, int max_conc, double bud
 struct IList { std::vector<int> h; int* d = nullptr; int n() const { return (int)h.size(); } };
 // one patient group of an At_T_YnWt_S_Zn term: SNP-major bits, active-row mask and list, covariates
 struct Side { const DMat* X; const uint32_t* mask; const IList* rows; const DMat* Cov; };
+// the same group with its patients packed to the front: SNP-major rows and covariate rows of
+// rb bytes whose first nbits positions are the group's rows in list order (mask: those in use)
+struct Packed { const uint8_t* X; const uint8_t* Cov; size_t rb; const uint32_t* mask; int nbits; };
 
 // HIP-event timing of named kernel groups on the run's stream
 struct Prof {
@@ -88,7 +91,7 @@ struct Run {
   int trm_AnZt = 0, trm_knzt = 0, trm_hnvt = 0;
   // control block
   uint8_t* hctl = nullptr; uint8_t* dctl = nullptr; size_t ctl_bytes = 0;
-  size_t ctl_mask_off[11] = {}, ctl_list_off[8] = {}, ctl_cmp_off[5] = {};
+  size_t ctl_mask_off[11] = {}, ctl_list_off[8] = {}, ctl_cmp_off[7] = {};
   // column compaction of the dropped columns (crm), block by block
   int cmp_W = 0, cmp_nseg = 0, cmp_Wmax = 0;
   int *d_seg = nullptr, *d_src = nullptr; uint32_t* d_cmask = nullptr;
@@ -97,6 +100,11 @@ struct Run {
   int rmv_W = 0, rmv_Wmax = 0;
   int* d_src_rmv = nullptr; uint32_t* d_cmask_rmv = nullptr;
   uint32_t *At_rmvc = nullptr, *Tt_rmvc = nullptr;
+  // and of the retained rows (rtn); d_iota[i] = i (positions in a packed group)
+  int rtn_W = 0;
+  int* d_src_rtn = nullptr; uint32_t* d_cmask_rtn = nullptr;
+  uint32_t *At_rtnc = nullptr, *Tt_rtnc = nullptr;
+  int* d_iota = nullptr;
   // ones per (1920-block segment, list position) of the matrix-core Grams' rows
   int* gcnt[6] = {};
   Grow g_attyn, g_qrs, g_qrs2, g_ztsn, g_split, g_ctmp;
@@ -138,7 +146,8 @@ struct Run {
   void st_ajdk(); void st_ZtSn(); void st_YnWt(); void st_At_T(); void st_QR();
   void st_single(); void st_double(); void st_flatten();
   void ww_excerpt(int trm, const Side& s1, const Side& s2, const IList& kls, const double* src, long long ld2,
-                  int full, double* out);
+                  int full, double* out, const Packed* pk = nullptr);
+  void pack_rows(const int* src, int W, uint32_t* Atc, uint32_t* Ttc);
   void init_lf();
   void sumscores();
   void lrup();

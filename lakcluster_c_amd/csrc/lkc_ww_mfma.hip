@@ -189,7 +189,11 @@ __global__ __launch_bounds__(256) void k_ww_mfma(WwArgs a, const int8_t* Dg, lon
         prv[t][0] = __builtin_bitreverse32((uint32_t)pos8[t]);
         prv[t][1] = __builtin_bitreverse32((uint32_t)(pos8[t] >> 32));
       }
-      if (c + 1 < nchunk) { WW_BLOAD(c + 1); WW_ALOAD(c + 1); }
+      // the next chunk (the last one again at the end: no branch in the loop body, so the
+      // scheduler can interleave the spreads and LDS reads with the MFMAs of the whole chunk)
+      const int cn = min(c + 1, nchunk - 1);
+      WW_BLOAD(cn); WW_ALOAD(cn);
+      __builtin_amdgcn_iglp_opt(0);
 #pragma unroll
       for (int s = 0; s < 4; s++) {
         v4i bf[ND];
@@ -202,7 +206,7 @@ __global__ __launch_bounds__(256) void k_ww_mfma(WwArgs a, const int8_t* Dg, lon
           for (int i = 0; i < ND; i++) acc[t][i] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af, bf[i], acc[t][i], 0, 0, 0);
         }
       }
-      if (c + 1 < nchunk) WW_BSTORE(buf ^ 1);
+      WW_BSTORE(buf ^ 1);
       __syncthreads();
     }
 #undef WW_BLOAD
