@@ -147,7 +147,19 @@ def main():
     ap.add_argument("--data-root", default=os.path.join(tempfile.gettempdir(), "lkc_bench_data"))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-null and serving legs")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 1 GiB buffer three times (the reference kernel for scaling HBM counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 1 << 28   # 1 GiB of float32
+        x = torch.ones(n, device="cuda"); y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 4 * n, "copies": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this path runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))

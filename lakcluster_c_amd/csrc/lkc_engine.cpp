@@ -459,7 +459,7 @@ void Run::setup_device(double t0)
   size_t o_l[8];
   for (int i = 0; i < 8; i++) o_l[i] = reserve(sizeof(int) * (size_t)(lcap[i] + 1));
   size_t o_zero = reserve(sizeof(int) * 4);
-  cmp_Wmax = 4 * pcols + (M + 31) / 32 + 4;   // segments padded to whole 4-word groups
+  cmp_Wmax = 4 * pcols + (M + 31) / 32 + 4;   // whole-word segments, rows padded to 4-word groups
   ctl_cmp_off[0] = reserve(sizeof(int) * 3 * (size_t)(pcols + 1));
   ctl_cmp_off[1] = reserve(sizeof(int) * 32 * (size_t)cmp_Wmax);
   ctl_cmp_off[2] = reserve(sizeof(uint32_t) * (size_t)cmp_Wmax);
@@ -599,9 +599,9 @@ void Run::push_ctl()
   };
   fill_list(ar, hAr_j, NA); fill_list(kl, hAc_j, M); fill_list(tl, hTc_j, T); fill_list(zr, hZr_j, NZ);
   fill_list(rtn, hAr_rtn, NA); fill_list(rmv, hAr_rmv, NA); fill_list(crt, hAc_rtn, M); fill_list(crm, hAc_rmv, M);
-  // compacted column layout: per non-empty 1920-block a segment of whole 4-word groups
-  // (offset and length in 32-column words, block), the source column of each position
-  // and the mask of the valid positions
+  // compacted column layout: per non-empty 1920-block a segment of whole words (offset
+  // and length in 32-column words, block), the source column of each position and the
+  // mask of the valid positions; the row length is padded to whole 4-word groups
   auto layout = [&](const std::vector<int>& cols, int* seg, int* src, uint32_t* cm, int& Wout, int& nsout) {
     int W = 0, ns = 0;
     size_t i = 0;
@@ -609,7 +609,7 @@ void Run::push_ctl()
       int b = cols[i] / kPopBits;
       size_t j = i;
       while (j < cols.size() && cols[j] / kPopBits == b) j++;
-      int cnt = (int)(j - i), nw = ((cnt + 31) / 32 + 3) / 4 * 4;
+      int cnt = (int)(j - i), nw = (cnt + 31) / 32;
       seg[3 * ns] = W; seg[3 * ns + 1] = nw; seg[3 * ns + 2] = b; ns++;
       for (int q = 0; q < 32 * nw; q++) src[32 * W + q] = q < cnt ? cols[i + q] : -1;
       if (cm)
@@ -620,7 +620,10 @@ void Run::push_ctl()
         }
       W += nw; i = j;
     }
-    Wout = W; nsout = ns;
+    const int Wp = (W + 3) / 4 * 4;
+    for (int q = 32 * W; q < 32 * Wp; q++) src[q] = -1;
+    if (cm) for (int w = W; w < Wp; w++) cm[w] = 0;
+    Wout = Wp; nsout = ns;
   };
   {
     layout(crm.h, (int*)(hctl + ctl_cmp_off[0]), (int*)(hctl + ctl_cmp_off[1]), (uint32_t*)(hctl + ctl_cmp_off[2]),

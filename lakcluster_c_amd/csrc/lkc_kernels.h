@@ -131,8 +131,9 @@ void attyn_mfma(const AttynArgs& a, void* work, size_t work_bytes, hipStream_t s
 // Centred Grams on the int8 matrix cores, over packed bit rows.  A column layout is
 // either the full one (seg == NULL: segment q = 1920-block q, words 60q .. 60q+59,
 // with the column mask applied) or a compacted one (segments of 3 ints: word offset,
-// word count, block; word counts multiples of 4; rows hold only the kept columns, the
-// rest of every word zero; mask NULL).  seg_counts: cnt[q*ld + i] = ones of
+// word count, block; segments of whole words, contiguous from word 0, the row stride a
+// whole number of 4-word groups; rows hold only the kept columns, the rest of every
+// word zero; mask NULL or of the same layout).  seg_counts: cnt[q*ld + i] = ones of
 // (row rows[i] & mask) in segment q, indexed by list position.  gram_mfma: output
 // and semantics as gram() with centred = 1; ca/cz are seg_counts of alist/zlist.
 struct GramMArgs {
