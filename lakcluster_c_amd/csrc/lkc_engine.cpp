@@ -679,7 +679,10 @@ void Run::st_ZtSn()
   }
 }
 
-// bcc_lf_YnWt (bcc_lf.c:223-302): centred Grams; lf_AnZt[z*NA + a], lf_AnAt[a2*NA + a1]
+// bcc_lf_YnWt (bcc_lf.c:223-302): centred Grams; lf_AnZt[a*NZ + z], lf_AnAt[a2*NA + a1].
+// lf_AnZt is stored with z contiguous (the controls are never dropped, so its rows stay
+// dense while A rows leave): the Grams run with the roles of A and Z swapped, which the
+// symmetric formula allows (pop((a ^ z) & m) and xa + xz are symmetric, bit for bit).
 void Run::st_YnWt()
 {
   if (ww_kernel == 0) {
@@ -687,8 +690,8 @@ void Run::st_YnWt()
     seg_counts(An.b(), ar.d, ar.n(), Ac_j, nullptr, pcols, cA, NA, st);
     if (has_Z) {
       seg_counts(Zn.b(), zr.d, zr.n(), Ac_j, nullptr, pcols, cZ, NZ, st);
-      GramMArgs g{ An.p, An.rb, ar.d, ar.n(), Zn.p, Zn.rb, zr.d, zr.n(), Ac_j, nullptr, pcols, cA, NA, cZ, NZ,
-                   ajdk, pcols, mlf, lf_An_ajdk + 6 * (size_t)NA, lf_Zn_ajdk + 6 * (size_t)NZ, lf_AnZt, NA, 1, 1, 0 };
+      GramMArgs g{ Zn.p, Zn.rb, zr.d, zr.n(), An.p, An.rb, ar.d, ar.n(), Ac_j, nullptr, pcols, cZ, NZ, cA, NA,
+                   ajdk, pcols, mlf, lf_Zn_ajdk + 6 * (size_t)NZ, lf_An_ajdk + 6 * (size_t)NA, lf_AnZt, NZ, 1, 1, 0 };
       gram_mfma(g, st);
       ops += (long long)ar.n() * zr.n() * (long long)bsize(M) * 8;
     }
@@ -699,8 +702,8 @@ void Run::st_YnWt()
     return;
   }
   if (has_Z) {
-    gram(An.b(), ar.d, ar.n(), Zn.b(), zr.d, zr.n(), Ac_j, pcols, ajdk, mlf,
-         lf_An_ajdk + 6 * (size_t)NA, lf_Zn_ajdk + 6 * (size_t)NZ, 1, lf_AnZt, NA, 1, 1, 0, st);
+    gram(Zn.b(), zr.d, zr.n(), An.b(), ar.d, ar.n(), Ac_j, pcols, ajdk, mlf,
+         lf_Zn_ajdk + 6 * (size_t)NZ, lf_An_ajdk + 6 * (size_t)NA, 1, lf_AnZt, NZ, 1, 1, 0, st);
     ops += (long long)ar.n() * zr.n() * (long long)bsize(M) * 8;
   }
   gram(An.b(), ar.d, ar.n(), An.b(), ar.d, ar.n(), Ac_j, pcols, ajdk, mlf,
@@ -709,16 +712,16 @@ void Run::st_YnWt()
 }
 
 // bcc_M_YnWt_excerpt + bcc_lf_At_T_YnWt_S_Zn_excerpt (bcc_lf.c:304-316, 615-634).
-// L(r1, r2) = src[idx1(r1) + idx2(r2)*ld2] with idx = the side's original index when full.
-void Run::ww_excerpt(int trm, const Side& s1, const Side& s2, const IList& kls, const double* src, long long ld2,
-                     int full, double* out, const Packed* pk)
+// L(r1, r2) = src[idx1(r1)*st1 + idx2(r2)*st2] with idx = the side's original index when full.
+void Run::ww_excerpt(int trm, const Side& s1, const Side& s2, const IList& kls, const double* src, long long st1,
+                     long long st2, int full, double* out, const Packed* pk)
 {
   const Side& pn = trm ? s2 : s1;
   const Side& pw = trm ? s1 : s2;
   int nn = pn.rows->n(), nw = pw.rows->n();
   long long sn, sw; const int *rl = nullptr, *cl = nullptr;
-  if (!trm) { sn = 1; sw = ld2; if (full) { rl = s1.rows->d; cl = s2.rows->d; } }
-  else { sn = ld2; sw = 1; if (full) { rl = s2.rows->d; cl = s1.rows->d; } }
+  if (!trm) { sn = st1; sw = st2; if (full) { rl = s1.rows->d; cl = s2.rows->d; } }
+  else { sn = st2; sw = st1; if (full) { rl = s2.rows->d; cl = s1.rows->d; } }
   int n_total = pn.X->nc;
   long long ldc = std::max(n_total, 1);
   uint32_t* cz = (uint32_t*)g_cz.need(dev, sizeof(uint32_t) * (size_t)std::max(nw, 1) * ldc);
@@ -782,8 +785,8 @@ void Run::pack_rows(const int* src, int W, uint32_t* Atc, uint32_t* Ttc)
 void Run::st_At_T()
 {
   Side sA{ &At, Ar_j, &ar, &Tt }, sZ{ &Zt, Zr_j, &zr, &St };
-  if (has_Z) ww_excerpt(trm_AnZt, sA, sZ, kl, lf_AnZt, NA, 1, lf_AtTAnZtSZn);
-  ww_excerpt(0, sA, sA, kl, lf_AnAt, NA, 1, lf_AtTAnAtTAn);
+  if (has_Z) ww_excerpt(trm_AnZt, sA, sZ, kl, lf_AnZt, NZ, 1, 1, lf_AtTAnZtSZn);
+  ww_excerpt(0, sA, sA, kl, lf_AnAt, 1, NA, 1, lf_AtTAnAtTAn);
 }
 
 // bcc_lf_AnZt_S_WnYt (bcc_lf.c:360-409)
@@ -791,7 +794,7 @@ void Run::st_QR()
 {
   if (has_Z) {
     signed char* sg = (signed char*)g_qrs.need(dev, qr_sign_bytes(zr.n(), tl.n()));
-    qr(lf_AnZt, NA, ar.d, ar.n(), zr.d, zr.n(), St.b(), tl.d, tl.n(), lf_AnZtSZnAt, NA, sg, st);
+    qr(lf_AnZt, NZ, ar.d, ar.n(), zr.d, zr.n(), St.b(), tl.d, tl.n(), lf_AnZtSZnAt, NA, sg, st, 1);
   }
   signed char* sg2 = (signed char*)g_qrs2.need(dev, qr_sign_bytes(ar.n(), tl.n()));
   qr(lf_AnAt, NA, ar.d, ar.n(), ar.d, ar.n(), Tt.b(), tl.d, tl.n(), lf_AnAtTAnAt, NA, sg2, st);
@@ -1036,8 +1039,8 @@ void Run::lrup()
     seg_counts(Anc, rmv.d, rmv.n(), nullptr, d_seg, cmp_nseg, c_mm, NA, st);
     if (has_Z) {
       seg_counts(Znc, zr.d, zr.n(), nullptr, d_seg, cmp_nseg, c_mz, NZ, st);
-      GramMArgs g{ Anc.p, Anc.rb, rtn.d, rtn.n(), Znc.p, Znc.rb, zr.d, zr.n(), nullptr, d_seg, cmp_nseg, c_mr, NA, c_mz, NZ,
-                   ajdk, pcols, mlf_rmv, jn_ajdk + 6 * (size_t)NA, vn_ajdk + 6 * (size_t)NZ, lf_AnZt, NA, 1, 1, 1 };
+      GramMArgs g{ Znc.p, Znc.rb, zr.d, zr.n(), Anc.p, Anc.rb, rtn.d, rtn.n(), nullptr, d_seg, cmp_nseg, c_mz, NZ, c_mr, NA,
+                   ajdk, pcols, mlf_rmv, vn_ajdk + 6 * (size_t)NZ, jn_ajdk + 6 * (size_t)NA, lf_AnZt, NZ, 1, 1, 1 };
       gram_mfma(g, st);
     }
     GramMArgs g{ Anc.p, Anc.rb, rtn.d, rtn.n(), Anc.p, Anc.rb, rtn.d, rtn.n(), nullptr, d_seg, cmp_nseg, c_mr, NA, c_mr, NA,
@@ -1045,8 +1048,8 @@ void Run::lrup()
     gram_mfma(g, st);
   } else {
     if (has_Z)
-      gram(Anc, rtn.d, rtn.n(), Znc, zr.d, zr.n(), d_cmask, pcols, ajdk, mlf_rmv,
-           jn_ajdk + 6 * (size_t)NA, vn_ajdk + 6 * (size_t)NZ, 1, lf_AnZt, NA, 1, 1, 1, st, d_seg, cmp_nseg);
+      gram(Znc, zr.d, zr.n(), Anc, rtn.d, rtn.n(), d_cmask, pcols, ajdk, mlf_rmv,
+           vn_ajdk + 6 * (size_t)NZ, jn_ajdk + 6 * (size_t)NA, 1, lf_AnZt, NZ, 1, 1, 1, st, d_seg, cmp_nseg);
     gram(Anc, rtn.d, rtn.n(), Anc, rtn.d, rtn.n(), d_cmask, pcols, ajdk, mlf_rmv,
          jn_ajdk + 6 * (size_t)NA, jn_ajdk + 6 * (size_t)NA, 1, lf_AnAt, NA, 1, 1, 1, st, d_seg, cmp_nseg);
   }
@@ -1116,11 +1119,11 @@ void Run::lrup()
   const Packed pk_rmv{ (const uint8_t*)At_rmvc, (const uint8_t*)Tt_rmvc, (size_t)rmv_W * 4, d_cmask_rmv, 32 * rmv_W };
   const Packed pk_rtn{ (const uint8_t*)At_rtnc, (const uint8_t*)Tt_rtnc, (size_t)rtn_W * 4, d_cmask_rtn, 32 * rtn_W };
   if (has_Z) {
-    ww_excerpt(trm_knzt, s_rmv, s_Z, crt, knzt, nrm, 0, t_knzt, trm_knzt ? nullptr : &pk_rmv);
-    ww_excerpt(trm_hnvt, s_rmv, s_Z, crt, hnvt, nrm, 0, t_hnvt, trm_hnvt ? nullptr : &pk_rmv);
+    ww_excerpt(trm_knzt, s_rmv, s_Z, crt, knzt, 1, nrm, 0, t_knzt, trm_knzt ? nullptr : &pk_rmv);
+    ww_excerpt(trm_hnvt, s_rmv, s_Z, crt, hnvt, 1, nrm, 0, t_hnvt, trm_hnvt ? nullptr : &pk_rmv);
   }
-  ww_excerpt(0, s_rtn, s_rmv, crt, ankt, nrt, 0, t_ankt, &pk_rtn);
-  ww_excerpt(0, s_rmv, s_rmv, crt, knkt, nrm, 0, t_knkt, &pk_rmv);
+  ww_excerpt(0, s_rtn, s_rmv, crt, ankt, 1, nrt, 0, t_ankt, &pk_rtn);
+  ww_excerpt(0, s_rmv, s_rmv, crt, knkt, 1, nrm, 0, t_knkt, &pk_rmv);
   // QC b0: plain +-1 products, [t*M + k]
   double *attn = mt[4], *jttn = mt[5], *ktrn = mt[6], *htrn = mt[7], *ztsn = mt[8], *vtsn = mt[9];
   mask_lf(Ar_rtn, sA, nullptr, mlf_nrtn, st);
@@ -1292,6 +1295,7 @@ long Run::debug_get(const std::string& name, std::vector<double>& out)
     { "et_An", et_An, nM, 1 }, { "T_AnAt_AnAt", T_AnAt_AnAt, nT * nA, 1 },
     { "QR_ZSZ", QR_ZSZ, nA, nT }, { "QR_TAT", QR_TAT, nA, nT }, { "QC_ZSZ", QC_ZSZ, nM, nT }, { "QC_TAT", QC_TAT, nM, nT },
   };
+  if (name == "lf_AnZt") { out = fetch(lf_AnZt, nA * nZ); return (long)out.size(); }   // [a*NZ + z] on the device
   for (auto& e : mt_tab) if (name == e.nm) { transpose(e.p, e.rows, e.cols); return (long)out.size(); }
   const char* scn[] = { "a1d2_AtTn", "a3d2_AtTn", "a1d2_ZtSn", "a3d2_ZtSn", "et_Tn", "et_Sn" };
   std::vector<double> h = fetch(sc, 6 * nT + 1);

@@ -145,7 +145,7 @@ struct Run {
   void push_ctl();
   void st_ajdk(); void st_ZtSn(); void st_YnWt(); void st_At_T(); void st_QR();
   void st_single(); void st_double(); void st_flatten();
-  void ww_excerpt(int trm, const Side& s1, const Side& s2, const IList& kls, const double* src, long long ld2,
+  void ww_excerpt(int trm, const Side& s1, const Side& s2, const IList& kls, const double* src, long long st1, long long st2,
                   int full, double* out, const Packed* pk = nullptr);
   void pack_rows(const int* src, int W, uint32_t* Atc, uint32_t* Ttc);
   void init_lf();

@@ -91,10 +91,12 @@ void ztsn_mfma(const Bits& A, const int* alist, int na, const uint32_t* mask, co
                int n_total, int ns, const int* slist, int B, const double* scal, double* out, long long ldo,
                void* work, size_t work_bytes, hipStream_t st);
 
-// AnZt_S_WnYt_vv with X==Y: out[tlist[t]*ldo + a] = Σ_z (X[z*ldx+a]*vS(z))*X[z*ldx+a]
+// AnZt_S_WnYt_vv with X==Y: out[tlist[t]*ldo + a] = Σ_z (X(a,z)*vS(z))*X(a,z) in list
+// order of z, with X(a,z) = X[z*ldx + a] (x_rows 0) or X[a*ldx + z] (x_rows 1).
 // sgbuf: qr_sign_bytes(nz, nt) bytes of device scratch
 void qr(const double* X, long long ldx, const int* alist, int na, const int* zlist, int nz,
-        const Bits& Sc, const int* tlist, int nt, double* out, long long ldo, signed char* sgbuf, hipStream_t st);
+        const Bits& Sc, const int* tlist, int nt, double* out, long long ldo, signed char* sgbuf, hipStream_t st,
+        int x_rows = 0);
 size_t qr_sign_bytes(int nz, int nt);
 
 // AtTYn_vv: out[(y*nt + t)*nk + k] (compact indices); lf_AtTn/lf_YtTn read at [t*ldT + row]

@@ -583,14 +583,73 @@ __global__ __launch_bounds__(256) void k_qr(const double* X, long long ldx, cons
   }
   if (chain) out[(long long)tlist[ct] * ldo + alist[i0 + cc]] = acc;
 }
+// The same sums over X[a*ldx + z] (z contiguous): thread z of a chunk loads its z for
+// the block's 16 rows a, so every load is a contiguous run along z whatever rows a
+// are still active; the chains are those of k_qr.
+__global__ __launch_bounds__(256) void k_qr_rows(const double* X, long long ldx, const int* alist, int na,
+                                                 const int* zlist, int nz, const signed char* sg, int nzp,
+                                                 const int* tlist, int nt, double* out, long long ldo)
+{
+  __shared__ double tile[QR_ZC][QR_AC + 1];
+  __shared__ signed char sgn[kMaxT][QR_ZC];
+  const int tid = threadIdx.x, i0 = blockIdx.x * QR_AC;
+  const int ct = tid / QR_AC, cc = tid & (QR_AC - 1);
+  const bool chain = ct < nt && i0 + cc < na;
+  const int nch = (nz + QR_ZC - 1) / QR_ZC;
+  long long arow[QR_AC];
+#pragma unroll
+  for (int j = 0; j < QR_AC; j++) arow[j] = (long long)alist[min(i0 + j, na - 1)] * ldx;
+  double reg[QR_AC];
+  signed char sreg[kMaxT];
+  // every address is valid (clamped z, clamped rows); entries past nz or na are never read
+  auto zat = [&](int c) { return (long long)zlist[min(c * QR_ZC + tid, nz - 1)]; };
+  auto load = [&](int c, long long z) {
+#pragma unroll
+    for (int j = 0; j < QR_AC; j++) reg[j] = X[arow[j] + z];
+#pragma unroll
+    for (int t = 0; t < kMaxT; t++) sreg[t] = sg[(long long)min(t, nt - 1) * nzp + c * QR_ZC + tid];
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int j = 0; j < QR_AC; j++) tile[tid][j] = reg[j];
+#pragma unroll
+    for (int t = 0; t < kMaxT; t++) sgn[t][tid] = sreg[t];
+  };
+  double acc = 0.0;
+  long long zn = 0;
+  if (nch > 0) { load(0, zat(0)); store(); zn = zat(1); }
+  __syncthreads();
+  for (int c = 0; c < nch; c++) {
+    long long zn2 = 0;
+    if (c + 1 < nch) { load(c + 1, zn); zn2 = zat(c + 2); }
+    const int zc = min(QR_ZC, nz - c * QR_ZC);
+    if (chain) {
+#pragma unroll 8
+      for (int zi = 0; zi < zc; zi++) {
+        const double x = tile[zi][cc];
+        const int vS = sgn[ct][zi];
+        acc += x * vS * x;
+      }
+    }
+    __syncthreads();
+    if (c + 1 < nch) { store(); zn = zn2; }
+    __syncthreads();
+  }
+  if (chain) out[(long long)tlist[ct] * ldo + alist[i0 + cc]] = acc;
+}
+
 void qr(const double* X, long long ldx, const int* alist, int na, const int* zlist, int nz,
-        const Bits& Sc, const int* tlist, int nt, double* out, long long ldo, signed char* sgbuf, hipStream_t st)
+        const Bits& Sc, const int* tlist, int nt, double* out, long long ldo, signed char* sgbuf, hipStream_t st,
+        int x_rows)
 {
   if (na <= 0) return;
   const int nzp = ((nz + QR_ZC - 1) / QR_ZC) * QR_ZC + QR_ZC;
   if (nt > 0 && nzp > 0)
     hipLaunchKernelGGL(k_qr_signs, dim3(cdiv((long long)nt * nzp, 256)), dim3(256), 0, st, Sc.p, Sc.rb, tlist, nt, zlist, nz, nzp, sgbuf);
-  hipLaunchKernelGGL(k_qr, dim3(cdiv(na, QR_AC)), dim3(256), 0, st, X, ldx, alist, na, zlist, nz, sgbuf, nzp, tlist, nt, out, ldo);
+  if (x_rows)
+    hipLaunchKernelGGL(k_qr_rows, dim3(cdiv(na, QR_AC)), dim3(256), 0, st, X, ldx, alist, na, zlist, nz, sgbuf, nzp, tlist, nt, out, ldo);
+  else
+    hipLaunchKernelGGL(k_qr, dim3(cdiv(na, QR_AC)), dim3(256), 0, st, X, ldx, alist, na, zlist, nz, sgbuf, nzp, tlist, nt, out, ldo);
 }
 int qr_signs(const Bits& Sc, const int* tlist, int nt, const int* zlist, int nz, signed char* sgbuf, hipStream_t st)
 {
