@@ -476,8 +476,10 @@ __global__ void k_gram_combine(GramMArgs g, const uint32_t* X)
 size_t gram_split_bytes(const GramMArgs& g)
 {
   if (g.seg || g.na <= 0 || g.nz <= 0 || g.nseg < 2) return 0;
+  // the split pays an int32 store and reload per segment and pair: only for Grams too
+  // small to fill the CUs with whole tiles
   const long long tiles = (long long)cdiv(g.na, 128) * cdiv(g.nz, 128);
-  if (tiles >= 1024) return 0;
+  if (tiles >= 128) return 0;
   const double b = 4.0 * g.nseg * (double)g.na * g.nz;
   return b <= 4.0e9 ? (size_t)b + 256 : 0;
 }

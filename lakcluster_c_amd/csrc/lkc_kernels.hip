@@ -262,12 +262,17 @@ __global__ void k_minmax(const double* src, long long sn, long long sw, const in
 }
 __global__ void k_minmax_final(const double* part, int np, long long D_MLT, double* scal)
 {
+  // the partials come in through LDS in one parallel load; one lane then folds them in
+  // order, as a single scan of the values would
+  __shared__ double pp[3 * 256];
+  for (int e = threadIdx.x; e < 3 * np; e += blockDim.x) pp[e] = part[e];
+  __syncthreads();
   if (threadIdx.x != 0) return;
   double mn = 0, mx = 0; int set = 0;
   for (int p = 0; p < np; p++) {
-    if (part[3 * p + 2] == 0) continue;
-    if (!set) { mn = part[3 * p]; mx = part[3 * p + 1]; set = 1; }
-    else { mn = part[3 * p] < mn ? part[3 * p] : mn; mx = part[3 * p + 1] > mx ? part[3 * p + 1] : mx; }
+    if (pp[3 * p + 2] == 0) continue;
+    if (!set) { mn = pp[3 * p]; mx = pp[3 * p + 1]; set = 1; }
+    else { mn = pp[3 * p] < mn ? pp[3 * p] : mn; mx = pp[3 * p + 1] > mx ? pp[3 * p + 1] : mx; }
   }
   if (mn >= mx) mx = mn + 1;
   double lo = mn - 0.0625 * (mx - mn);
@@ -299,7 +304,7 @@ void xcalc(const double* src, long long sn, long long sw, const int* rlist, cons
   long long tot = (long long)nn * nw;
   if (tot < 256LL * 64) np = (int)cdiv(tot > 0 ? tot : 1, 256);
   hipLaunchKernelGGL(k_minmax, dim3(np), dim3(256), 0, st, src, sn, sw, rlist, clist, nn, nw, work);
-  hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(64), 0, st, work, np, D_MLT, scal);
+  hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(256), 0, st, work, np, D_MLT, scal);
   if (tot > 0)
     hipLaunchKernelGGL(k_quantize, dim3(cdiv(tot, 256) < 65536 ? cdiv(tot, 256) : 65536), dim3(256), 0, st,
                        src, sn, sw, rlist, clist, nn, nw, nout, D_MLT, scal, cz, ldc);
@@ -503,7 +508,7 @@ void an_ztsn(const Bits& A, const int* alist, int na, const uint32_t* amask,
 }
 
 // ------------------------------------------------------------------
-// AnZt_S_WnYt_vv (AnZt_S_WnYt_vv.c:49-67) with X==Y; X[z*ldx + a] (a contiguous)
+// AnZt_S_WnYt_vv (AnZt_S_WnYt_vv.c:49-67) with X==Y
 // ------------------------------------------------------------------
 // Signs of the covariate columns at the listed z: sg[t*nzp + i] = S_t(zlist[i]) (+-1).
 __global__ void k_qr_signs(const uint8_t* Sc, size_t rbS, const int* tlist, int nt, const int* zlist, int nz,
@@ -515,150 +520,98 @@ __global__ void k_qr_signs(const uint8_t* Sc, size_t rbS, const int* tlist, int 
   sg[e] = i < nz ? (signed char)pm_at(Sc + (size_t)tlist[t] * rbS, zlist[i]) : 0;
 }
 
-// One block per 16 rows a (QR_AC): all 256 threads stream a 256(z) x 16(a) tile of X
-// through registers into LDS while thread (t, a) of the first waves adds the previous
-// tile into its ordered sum: one sequential chain per (a, t), in ascending z.  Small
-// blocks keep four of them on a compute unit, so enough loads are in flight.
-#define QR_ZC 256
+// One block per 16 rows a.  Per chunk of 128 z all 256 threads load X and form the
+// reference's products p = (X * S_t) * X (X == Y), one per (t, z, a), into LDS; the
+// chain threads (t, a) of the first wave then only add p in ascending z, so the ordered
+// chain costs one read and one add per z.  The next chunk's X and signs are loaded into
+// registers meanwhile.  x_rows selects the layout X[a*ldx + z] (thread (z, 8 rows a):
+// runs along z) or X[z*ldx + a] (thread (a, 8 consecutive z): runs along a).
+#define QR_ZC 128
 #define QR_AC 16
+#define QR_AP 17        // padded row of the product tile
+template <int XROWS>
 __global__ __launch_bounds__(256) void k_qr(const double* X, long long ldx, const int* alist, int na, const int* zlist,
                                             int nz, const signed char* sg, int nzp, const int* tlist, int nt,
                                             double* out, long long ldo)
 {
-  __shared__ double tile[QR_ZC][QR_AC];
-  __shared__ signed char sgn[kMaxT][QR_ZC];
-  __shared__ int zrow[2][QR_ZC];          // zlist of the next two chunks
+  extern __shared__ double pr[];          // [t][z][a], nt x QR_ZC x QR_AP
   const int tid = threadIdx.x, i0 = blockIdx.x * QR_AC;
-  // loader role: column lc, rows lr + 16 j
-  const int lc = tid & (QR_AC - 1), lr = tid / QR_AC;
-  const long long lcol = alist[min(i0 + lc, na - 1)];
-  // chain role: covariate ct, column cc
+  // load role: XROWS: z = tid % 128, rows a = 8 (tid / 128) + k;  else: a = tid % 16, z = 8 (tid / 16) + k
+  const int lz = XROWS ? (tid & 127) : 8 * (tid >> 4), la = XROWS ? 8 * (tid >> 7) : (tid & 15);
+  long long arow[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) arow[k] = (long long)alist[min(i0 + (XROWS ? la + k : la), na - 1)];
+  // chain role: thread (t, a) = (tid / 16, tid % 16)
   const int ct = tid / QR_AC, cc = tid & (QR_AC - 1);
   const bool chain = ct < nt && i0 + cc < na;
   const int nch = (nz + QR_ZC - 1) / QR_ZC;
-  double reg[QR_ZC / 16];
-  signed char sreg[kMaxT];
-  // The row indices come from LDS and every address is valid (clamped), so the 16
-  // loads of a thread go out back to back.  Entries past nz or na are never read.
+  double xr[8];
+  uint64_t sv[kMaxT];                     // the signs of the thread's z (8 consecutive z: one per byte)
+  // every address is valid (clamped); entries past nz or na are never added
   auto load = [&](int c) {
-    const int z0 = c * QR_ZC;
-    const int* zr = zrow[c & 1];
 #pragma unroll
-    for (int j = 0; j < QR_ZC / 16; j++) reg[j] = X[(long long)zr[lr + 16 * j] * ldx + lcol];
+    for (int k = 0; k < 8; k++) {
+      const int zp = c * QR_ZC + lz + (XROWS ? 0 : k);
+      const long long z = zlist[min(zp, nz - 1)];
+      xr[k] = XROWS ? X[arow[k] * ldx + z] : X[z * ldx + arow[0]];
+    }
 #pragma unroll
-    for (int t = 0; t < kMaxT; t++) sreg[t] = sg[(long long)min(t, nt - 1) * nzp + z0 + tid];
+    for (int t = 0; t < kMaxT; t++) {
+      const signed char* sr = sg + (long long)min(t, nt - 1) * nzp + c * QR_ZC + lz;
+      sv[t] = XROWS ? (uint64_t)(uint8_t)sr[0] : *(const uint64_t*)sr;
+    }
   };
-  auto zload = [&](int c) { return zlist[min(c * QR_ZC + tid, nz - 1)]; };
-  auto store = [&]() {
+  auto products = [&]() {
 #pragma unroll
-    for (int j = 0; j < QR_ZC / 16; j++) tile[lr + 16 * j][lc] = reg[j];
+    for (int t = 0; t < kMaxT; t++) {
+      if (t >= nt) break;
 #pragma unroll
-    for (int t = 0; t < kMaxT; t++) sgn[t][tid] = sreg[t];
-  };
-  double acc = 0.0;
-  if (nch > 0) {
-    zrow[0][tid] = zload(0);
-    __syncthreads();
-    load(0);
-    int zn1 = zload(1);
-    store();
-    zrow[1][tid] = zn1;
-  }
-  __syncthreads();
-  for (int c = 0; c < nch; c++) {
-    int zn2 = 0;
-    if (c + 1 < nch) { load(c + 1); zn2 = zload(c + 2); }
-    const int zn = min(QR_ZC, nz - c * QR_ZC);
-    if (chain) {
-#pragma unroll 8
-      for (int zi = 0; zi < zn; zi++) {
-        const double x = tile[zi][cc];
-        const int vS = sgn[ct][zi];
-        acc += x * vS * x;
+      for (int k = 0; k < 8; k++) {
+        const int z = XROWS ? lz : lz + k, a = XROWS ? la + k : la;
+        const int vS = (int)(signed char)(uint8_t)(sv[t] >> (XROWS ? 0 : 8 * k));
+        pr[((size_t)t * QR_ZC + z) * QR_AP + a] = xr[k] * vS * xr[k];
       }
     }
-    __syncthreads();
-    if (c + 1 < nch) { store(); zrow[c & 1][tid] = zn2; }
-    __syncthreads();
-  }
-  if (chain) out[(long long)tlist[ct] * ldo + alist[i0 + cc]] = acc;
-}
-// The same sums over X[a*ldx + z] (z contiguous): thread z of a chunk loads its z for
-// the block's 16 rows a, so every load is a contiguous run along z whatever rows a
-// are still active; the chains are those of k_qr.
-__global__ __launch_bounds__(256) void k_qr_rows(const double* X, long long ldx, const int* alist, int na,
-                                                 const int* zlist, int nz, const signed char* sg, int nzp,
-                                                 const int* tlist, int nt, double* out, long long ldo)
-{
-  __shared__ double tile[QR_ZC][QR_AC + 1];
-  __shared__ signed char sgn[kMaxT][QR_ZC];
-  const int tid = threadIdx.x, i0 = blockIdx.x * QR_AC;
-  const int ct = tid / QR_AC, cc = tid & (QR_AC - 1);
-  const bool chain = ct < nt && i0 + cc < na;
-  const int nch = (nz + QR_ZC - 1) / QR_ZC;
-  long long arow[QR_AC];
-#pragma unroll
-  for (int j = 0; j < QR_AC; j++) arow[j] = (long long)alist[min(i0 + j, na - 1)] * ldx;
-  double reg[QR_AC];
-  signed char sreg[kMaxT];
-  // every address is valid (clamped z, clamped rows); entries past nz or na are never read
-  auto zat = [&](int c) { return (long long)zlist[min(c * QR_ZC + tid, nz - 1)]; };
-  auto load = [&](int c, long long z) {
-#pragma unroll
-    for (int j = 0; j < QR_AC; j++) reg[j] = X[arow[j] + z];
-#pragma unroll
-    for (int t = 0; t < kMaxT; t++) sreg[t] = sg[(long long)min(t, nt - 1) * nzp + c * QR_ZC + tid];
-  };
-  auto store = [&]() {
-#pragma unroll
-    for (int j = 0; j < QR_AC; j++) tile[tid][j] = reg[j];
-#pragma unroll
-    for (int t = 0; t < kMaxT; t++) sgn[t][tid] = sreg[t];
   };
   double acc = 0.0;
-  long long zn = 0;
-  if (nch > 0) { load(0, zat(0)); store(); zn = zat(1); }
-  __syncthreads();
+  if (nch > 0) load(0);
   for (int c = 0; c < nch; c++) {
-    long long zn2 = 0;
-    if (c + 1 < nch) { load(c + 1, zn); zn2 = zat(c + 2); }
-    const int zc = min(QR_ZC, nz - c * QR_ZC);
+    __syncthreads();               // the previous chunk's chains are done with pr
+    products();
+    if (c + 1 < nch) load(c + 1);
+    __syncthreads();
     if (chain) {
-#pragma unroll 8
-      for (int zi = 0; zi < zc; zi++) {
-        const double x = tile[zi][cc];
-        const int vS = sgn[ct][zi];
-        acc += x * vS * x;
-      }
+      const double* pp = pr + (size_t)ct * QR_ZC * QR_AP + cc;
+      const int zc = min(QR_ZC, nz - c * QR_ZC);
+#pragma unroll 16
+      for (int zi = 0; zi < zc; zi++) acc += pp[zi * QR_AP];
     }
-    __syncthreads();
-    if (c + 1 < nch) { store(); zn = zn2; }
-    __syncthreads();
   }
   if (chain) out[(long long)tlist[ct] * ldo + alist[i0 + cc]] = acc;
 }
 
+static int qr_nzp(int nz) { return ((nz + QR_ZC - 1) / QR_ZC) * QR_ZC + QR_ZC; }
 void qr(const double* X, long long ldx, const int* alist, int na, const int* zlist, int nz,
         const Bits& Sc, const int* tlist, int nt, double* out, long long ldo, signed char* sgbuf, hipStream_t st,
         int x_rows)
 {
-  if (na <= 0) return;
-  const int nzp = ((nz + QR_ZC - 1) / QR_ZC) * QR_ZC + QR_ZC;
-  if (nt > 0 && nzp > 0)
-    hipLaunchKernelGGL(k_qr_signs, dim3(cdiv((long long)nt * nzp, 256)), dim3(256), 0, st, Sc.p, Sc.rb, tlist, nt, zlist, nz, nzp, sgbuf);
+  if (na <= 0 || nt <= 0) return;
+  const int nzp = qr_nzp(nz);
+  hipLaunchKernelGGL(k_qr_signs, dim3(cdiv((long long)nt * nzp, 256)), dim3(256), 0, st, Sc.p, Sc.rb, tlist, nt, zlist, nz, nzp, sgbuf);
+  const size_t lds = sizeof(double) * (size_t)nt * QR_ZC * QR_AP;
   if (x_rows)
-    hipLaunchKernelGGL(k_qr_rows, dim3(cdiv(na, QR_AC)), dim3(256), 0, st, X, ldx, alist, na, zlist, nz, sgbuf, nzp, tlist, nt, out, ldo);
+    hipLaunchKernelGGL((k_qr<1>), dim3(cdiv(na, QR_AC)), dim3(256), lds, st, X, ldx, alist, na, zlist, nz, sgbuf, nzp, tlist, nt, out, ldo);
   else
-    hipLaunchKernelGGL(k_qr, dim3(cdiv(na, QR_AC)), dim3(256), 0, st, X, ldx, alist, na, zlist, nz, sgbuf, nzp, tlist, nt, out, ldo);
+    hipLaunchKernelGGL((k_qr<0>), dim3(cdiv(na, QR_AC)), dim3(256), lds, st, X, ldx, alist, na, zlist, nz, sgbuf, nzp, tlist, nt, out, ldo);
 }
 int qr_signs(const Bits& Sc, const int* tlist, int nt, const int* zlist, int nz, signed char* sgbuf, hipStream_t st)
 {
-  const int nzp = ((nz + QR_ZC - 1) / QR_ZC) * QR_ZC + QR_ZC;
+  const int nzp = qr_nzp(nz);
   if (nt > 0)
     hipLaunchKernelGGL(k_qr_signs, dim3(cdiv((long long)nt * nzp, 256)), dim3(256), 0, st, Sc.p, Sc.rb, tlist, nt, zlist, nz, nzp, sgbuf);
   return nzp;
 }
-size_t qr_sign_bytes(int nz, int nt) { return (size_t)nt * (((nz + QR_ZC - 1) / QR_ZC) * QR_ZC + QR_ZC) + 256; }
+size_t qr_sign_bytes(int nz, int nt) { return (size_t)nt * qr_nzp(nz) + 256; }
 
 // ------------------------------------------------------------------
 // AtTYn_vv (AtTYn_vv.c:79-120)
