@@ -82,6 +82,9 @@ __global__ __launch_bounds__(256, 2) void k_attyn_mfma(AttynArgs a, const long l
   __shared__ __attribute__((aligned(16))) int8_t Ps[2][NTG][32][AT_PROW];
   __shared__ int Dm[NQ][NTG][32][33];
   __shared__ int cpos[NQ][NTG][32];
+  // per-y values of the sub-tile the chains walk (staged by all threads)
+  __shared__ double s_ay[32], s_Dy[32], s_haY[NQ][NTG][32];
+  __shared__ long long s_cy[NQ][32];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, g = lane >> 5, r = lane & 31;
   const int t0 = blockIdx.y * NTG;
   const int nts = min(NTG, a.nt - t0);
@@ -167,10 +170,26 @@ __global__ __launch_bounds__(256, 2) void k_attyn_mfma(AttynArgs a, const long l
 #pragma unroll
             for (int v = 0; v < 16; v++) Dm[q][t][8 * (v / 4) + 4 * g + (v % 4)][r] = acc[q][t][v];
       }
+      // the sub-tile's per-y inputs of the chain, loaded once by all threads
+      {
+        const double* alpha = a.ajdk + (size_t)1 * a.pcols;
+        const double* Dt = a.ajdk + (size_t)5 * a.pcols;
+        for (int e = tid; e < 32 * (1 + NQ + NQ * NTG); e += 256) {
+          const int j = e & 31, f = e >> 5, yi = min(ys + j, a.ny - 1);
+          const int yy = a.ylist[yi];
+          if (f == 0) { s_ay[j] = alpha[yy / kPopBits]; s_Dy[j] = Dt[yy / kPopBits]; }
+          else if (f <= NQ) s_cy[f - 1][j] = cy[(size_t)(f - 1) * a.ny + yi];
+          else {
+            const int qt = f - 1 - NQ, q = qt / NTG, t = qt % NTG;
+            const AttynTerm& tm = a.term[q];
+            const int tt = a.tlist[t0 + min(t, nts - 1)];
+            s_haY[q][t][j] = (double)(long long)tm.haY[(long long)tt * tm.ldh + yy];
+          }
+        }
+      }
       __syncthreads();
       if (chain) {
         const double* alpha = a.ajdk + (size_t)1 * a.pcols;
-        const double* Dt = a.ajdk + (size_t)5 * a.pcols;
         const int kk = a.klist[min(kb + ck, a.nk - 1)];
         const double aa = alpha[kk / kPopBits];
         const int tt = a.tlist[t0 + ct];
@@ -183,14 +202,13 @@ __global__ __launch_bounds__(256, 2) void k_attyn_mfma(AttynArgs a, const long l
         }
         const int yn = min(32, a.ny - ys);
         for (int j = 0; j < yn; j++) {
-          const int yy = a.ylist[ys + j];
-          const double ay = alpha[yy / kPopBits], Dy = Dt[yy / kPopBits];
+          const double ay = s_ay[j], Dy = s_Dy[j];
           double L[NQ];
 #pragma unroll
           for (int q = 0; q < NQ; q++) {
-            const long long hhAY = 4LL * Dm[q][ct][ck][j] - base[q] - cy[(size_t)q * a.ny + ys + j];
-            const long long hhaY = (long long)a.term[q].haY[(long long)tt * a.term[q].ldh + yy];
-            L[q] = (hhAY - hAy[q] * ay - aa * hhaY + aa * hhay_[q] * ay);
+            const long long hhAY = 4LL * Dm[q][ct][ck][j] - base[q] - s_cy[q][j];
+            // (aa * hhaY with the reference's integer hhaY converted to double)
+            L[q] = (hhAY - hAy[q] * ay - aa * s_haY[q][ct][j] + aa * hhay_[q] * ay);
           }
           // attjn = L[0], ktrhn = L[1], ztsvn = L[2]
           if (NQ > 2) o0 += L[0] * Dy * L[NQ - 1];
