@@ -468,6 +468,8 @@ void Run::setup_device(double t0)
   ctl_cmp_off[4] = reserve(sizeof(uint32_t) * (size_t)rmv_Wmax);
   ctl_cmp_off[5] = reserve(sizeof(int) * 32 * (size_t)rmv_Wmax);
   ctl_cmp_off[6] = reserve(sizeof(uint32_t) * (size_t)rmv_Wmax);
+  ctl_cmp_off[7] = reserve(sizeof(int) * 3 * (size_t)(pcols + 1));
+  ctl_cmp_off[8] = reserve(sizeof(int) * 32 * (size_t)cmp_Wmax);
   ctl_bytes = off;
   HIPCHK(hipHostMalloc((void**)&hctl, ctl_bytes, hipHostMallocDefault));
   memset(hctl, 0, ctl_bytes);
@@ -479,6 +481,7 @@ void Run::setup_device(double t0)
   d_seg = (int*)(dctl + ctl_cmp_off[0]); d_src = (int*)(dctl + ctl_cmp_off[1]); d_cmask = (uint32_t*)(dctl + ctl_cmp_off[2]);
   d_src_rmv = (int*)(dctl + ctl_cmp_off[3]); d_cmask_rmv = (uint32_t*)(dctl + ctl_cmp_off[4]);
   d_src_rtn = (int*)(dctl + ctl_cmp_off[5]); d_cmask_rtn = (uint32_t*)(dctl + ctl_cmp_off[6]);
+  d_seg_k = (int*)(dctl + ctl_cmp_off[7]); d_src_k = (int*)(dctl + ctl_cmp_off[8]);
   for (int i = 0; i < 6; i++) gcnt[i] = dev.alloc<int>((size_t)std::max(NA, NZ) * (pcols + 1));
   At_rmvc = dev.alloc<uint32_t>((size_t)M * rmv_Wmax);
   Tt_rmvc = dev.alloc<uint32_t>((size_t)T * rmv_Wmax);
@@ -628,6 +631,11 @@ void Run::push_ctl()
   {
     layout(crm.h, (int*)(hctl + ctl_cmp_off[0]), (int*)(hctl + ctl_cmp_off[1]), (uint32_t*)(hctl + ctl_cmp_off[2]),
            cmp_W, cmp_nseg);
+    // kept columns: compacted once fewer than half of the columns remain (the full layout
+    // would walk every column of every row under the mask)
+    kc_on = ww_kernel == 0 && 2LL * crt.n() < (long long)M && crt.n() > 0;
+    if (kc_on) layout(crt.h, (int*)(hctl + ctl_cmp_off[7]), (int*)(hctl + ctl_cmp_off[8]), nullptr, kc_W, kc_nseg);
+    else { kc_W = 0; kc_nseg = 0; }
     // removed and retained rows, each packed in whole 128-bit groups
     auto pack = [&](const IList& l, int* srr, uint32_t* cmr) {
       int nr = l.n(), W = (((nr + 31) / 32) + 3) / 4 * 4;
@@ -1074,7 +1082,42 @@ void Run::lrup()
     knzt = (double*)g_knzt.need(dev, sizeof(double) * ((size_t)nrm * nz + 1));
     hnvt = (double*)g_hnvt.need(dev, sizeof(double) * ((size_t)nrm * nz + 1));
   }
-  if (ww_kernel == 0) {
+  if (ww_kernel == 0 && kc_on) {
+    // over the compacted kept columns: gather those columns as rows of the SNP-major
+    // copies, transpose back; the segments are the kept columns' 1920-blocks in order
+    const int Wk = kc_W;
+    uint32_t* Ank = (uint32_t*)g_ank.need(dev, sizeof(uint32_t) * (size_t)NA * Wk);
+    uint32_t* Znk = has_Z ? (uint32_t*)g_znk.need(dev, sizeof(uint32_t) * (size_t)NZ * Wk) : nullptr;
+    {
+      uint8_t* tmp = (uint8_t*)g_ctmp.need(dev, (size_t)32 * Wk * std::max(At.rb, Zt.rb));
+      gather_rows(At.b(), d_src_k, 32 * Wk, tmp, st);
+      transpose_bits(Bits{ tmp, 32 * Wk, NA, At.rb }, (uint8_t*)Ank, (size_t)Wk * 4, st);
+      if (has_Z) {
+        gather_rows(Zt.b(), d_src_k, 32 * Wk, tmp, st);
+        transpose_bits(Bits{ tmp, 32 * Wk, NZ, Zt.rb }, (uint8_t*)Znk, (size_t)Wk * 4, st);
+      }
+    }
+    const Bits Ak{ (const uint8_t*)Ank, NA, Wk * 32, (size_t)Wk * 4 }, Zk{ (const uint8_t*)Znk, NZ, Wk * 32, (size_t)Wk * 4 };
+    int *c_kr = gcnt[3], *c_km = gcnt[4], *c_kz = gcnt[5];
+    seg_counts(Ak, rtn.d, nrt, nullptr, d_seg_k, kc_nseg, c_kr, NA, st);
+    seg_counts(Ak, rmv.d, nrm, nullptr, d_seg_k, kc_nseg, c_km, NA, st);
+    if (has_Z) {
+      seg_counts(Zk, zr.d, nz, nullptr, d_seg_k, kc_nseg, c_kz, NZ, st);
+      GramMArgs g1{ Ak.p, Ak.rb, rmv.d, nrm, Zk.p, Zk.rb, zr.d, nz, nullptr, d_seg_k, kc_nseg, c_km, NA, c_kz, NZ,
+                    ajdk, pcols, mlf_rtn, kn_ajdk + 6 * (size_t)NA, zn_ajdk + 6 * (size_t)NZ, knzt, nrm, 0, 0, 0 };
+      gram_mfma(g1, st);
+      GramMArgs g2{ Anc.p, Anc.rb, rmv.d, nrm, Znc.p, Znc.rb, zr.d, nz, nullptr, d_seg, cmp_nseg, c_mm, NA, c_mz, NZ,
+                    ajdk, pcols, mlf_rmv, hn_ajdk + 6 * (size_t)NA, vn_ajdk + 6 * (size_t)NZ, hnvt, nrm, 0, 0, 0 };
+      gram_mfma(g2, st);
+      ops += 2LL * nrm * nz * rbm;
+    }
+    GramMArgs g3{ Ak.p, Ak.rb, rtn.d, nrt, Ak.p, Ak.rb, rmv.d, nrm, nullptr, d_seg_k, kc_nseg, c_kr, NA, c_km, NA,
+                  ajdk, pcols, mlf_rtn, an_ajdk_ + 6 * (size_t)NA, kn_ajdk + 6 * (size_t)NA, ankt, nrt, 0, 0, 0 };
+    gram_mfma(g3, st);
+    GramMArgs g4{ Ak.p, Ak.rb, rmv.d, nrm, Ak.p, Ak.rb, rmv.d, nrm, nullptr, d_seg_k, kc_nseg, c_km, NA, c_km, NA,
+                  ajdk, pcols, mlf_rtn, kn_ajdk + 6 * (size_t)NA, kn_ajdk + 6 * (size_t)NA, knkt, nrm, 0, 0, 0 };
+    gram_mfma(g4, st);
+  } else if (ww_kernel == 0) {
     int *c_kr = gcnt[3], *c_km = gcnt[4], *c_kz = gcnt[5];
     seg_counts(An.b(), rtn.d, nrt, Ac_rtn, nullptr, pcols, c_kr, NA, st);
     seg_counts(An.b(), rmv.d, nrm, Ac_rtn, nullptr, pcols, c_km, NA, st);

@@ -91,11 +91,15 @@ struct Run {
   int trm_AnZt = 0, trm_knzt = 0, trm_hnvt = 0;
   // control block
   uint8_t* hctl = nullptr; uint8_t* dctl = nullptr; size_t ctl_bytes = 0;
-  size_t ctl_mask_off[11] = {}, ctl_list_off[8] = {}, ctl_cmp_off[7] = {};
+  size_t ctl_mask_off[11] = {}, ctl_list_off[8] = {}, ctl_cmp_off[9] = {};
   // column compaction of the dropped columns (crm), block by block
   int cmp_W = 0, cmp_nseg = 0, cmp_Wmax = 0;
   int *d_seg = nullptr, *d_src = nullptr; uint32_t* d_cmask = nullptr;
   uint32_t *An_c = nullptr, *Zn_c = nullptr;
+  // the same compaction of the kept columns (crt), used once fewer than half are kept
+  int kc_on = 0, kc_W = 0, kc_nseg = 0;
+  int *d_seg_k = nullptr, *d_src_k = nullptr;
+  Grow g_ank, g_znk;
   // compaction of the removed rows (rmv) for the patient dimension of At / Tt
   int rmv_W = 0, rmv_Wmax = 0;
   int* d_src_rmv = nullptr; uint32_t* d_cmask_rmv = nullptr;
