@@ -27,7 +27,13 @@ namespace lkc {
 static inline int bget(const uint8_t* b, long i) { return (b[i >> 3] >> (7 - (i & 7))) & 1; }
 static inline void bset(uint8_t* b, long i) { b[i >> 3] |= (uint8_t)(1u << (7 - (i & 7))); }
 static inline void bclr(uint8_t* b, long i) { b[i >> 3] &= (uint8_t)~(1u << (7 - (i & 7))); }
-static int popmask(const std::vector<uint8_t>& m, int n) { int s = 0; for (int i = 0; i < n; i++) s += bget(m.data(), i); return s; }
+static int popmask(const std::vector<uint8_t>& m, int n)
+{
+  int s = 0;
+  for (int b = 0; b < n / 8; b++) s += __builtin_popcount(m[b]);
+  for (int i = n / 8 * 8; i < n; i++) s += bget(m.data(), i);
+  return s;
+}
 static double now_s() { struct timeval tv; gettimeofday(&tv, NULL); return tv.tv_sec + 1e-6 * tv.tv_usec; }
 
 // ------------------------------------------------------------------
@@ -596,9 +602,20 @@ void Run::push_ctl()
   const std::vector<uint8_t>* hm[11] = { &hAr_b, &hAr_j, &hAr_rtn, &hAr_rmv, &hZr_b, &hZr_j, &hAc_b, &hAc_j, &hAc_rtn, &hAc_rmv, &hTc_j };
   int mn[11] = { NA, NA, NA, NA, NZ, NZ, M, M, M, M, T };
   for (int i = 0; i < 11; i++) memcpy(hctl + ctl_mask_off[i], hm[i]->data(), bsize(mn[i]));
+  // set positions of a mask in ascending order: zero bytes skipped, set bits taken
+  // MSB-first with a count of leading zeros
   auto fill_list = [&](IList& l, const std::vector<uint8_t>& m, int n) {
     l.h.clear();
-    for (int i = 0; i < n; i++) if (bget(m.data(), i)) l.h.push_back(i);
+    const int nb = (n + 7) / 8;
+    for (int b = 0; b < nb; b++) {
+      unsigned v = m[b];
+      while (v) {
+        const int k = __builtin_clz(v) - 24, i = 8 * b + k;
+        if (i >= n) break;
+        l.h.push_back(i);
+        v &= ~(0x80u >> k);
+      }
+    }
   };
   fill_list(ar, hAr_j, NA); fill_list(kl, hAc_j, M); fill_list(tl, hTc_j, T); fill_list(zr, hZr_j, NZ);
   fill_list(rtn, hAr_rtn, NA); fill_list(rmv, hAr_rmv, NA); fill_list(crt, hAc_rtn, M); fill_list(crm, hAc_rmv, M);
