@@ -1189,10 +1189,20 @@ void Run::lrup()
   mask_lf(Ar_rtn, sA, nullptr, mlf_nrtn, st);
   mask_lf(Ar_rmv, sA, nullptr, mlf_nrmv, st);
   const long long rbn = (long long)At.rb * 8;
-  pm_few(At.b(), crt.d, crt.n(), Tt.b(), tl.d, tl.n(), Ar_rtn, sA, mlf_nrtn, attn, M, st);
-  pm_few(At.b(), crm.d, crm.n(), Tt.b(), tl.d, tl.n(), Ar_rtn, sA, mlf_nrtn, jttn, M, st);
-  pm_few(At.b(), crt.d, crt.n(), Tt.b(), tl.d, tl.n(), Ar_rmv, sA, mlf_nrmv, ktrn, M, st);
-  pm_few(At.b(), crm.d, crm.n(), Tt.b(), tl.d, tl.n(), Ar_rmv, sA, mlf_nrmv, htrn, M, st);
+  if (ww_kernel == 0) {
+    // over the packed retained / removed patients (the counts are the same integers)
+    const Bits Ptn{ (const uint8_t*)At_rtnc, M, 32 * rtn_W, (size_t)rtn_W * 4 }, Ttn{ (const uint8_t*)Tt_rtnc, T, 32 * rtn_W, (size_t)rtn_W * 4 };
+    const Bits Pmv{ (const uint8_t*)At_rmvc, M, 32 * rmv_W, (size_t)rmv_W * 4 }, Tmv{ (const uint8_t*)Tt_rmvc, T, 32 * rmv_W, (size_t)rmv_W * 4 };
+    pm_few(Ptn, crt.d, crt.n(), Ttn, tl.d, tl.n(), d_cmask_rtn, sA, mlf_nrtn, attn, M, st, rtn_W);
+    pm_few(Ptn, crm.d, crm.n(), Ttn, tl.d, tl.n(), d_cmask_rtn, sA, mlf_nrtn, jttn, M, st, rtn_W);
+    pm_few(Pmv, crt.d, crt.n(), Tmv, tl.d, tl.n(), d_cmask_rmv, sA, mlf_nrmv, ktrn, M, st, rmv_W);
+    pm_few(Pmv, crm.d, crm.n(), Tmv, tl.d, tl.n(), d_cmask_rmv, sA, mlf_nrmv, htrn, M, st, rmv_W);
+  } else {
+    pm_few(At.b(), crt.d, crt.n(), Tt.b(), tl.d, tl.n(), Ar_rtn, sA, mlf_nrtn, attn, M, st);
+    pm_few(At.b(), crm.d, crm.n(), Tt.b(), tl.d, tl.n(), Ar_rtn, sA, mlf_nrtn, jttn, M, st);
+    pm_few(At.b(), crt.d, crt.n(), Tt.b(), tl.d, tl.n(), Ar_rmv, sA, mlf_nrmv, ktrn, M, st);
+    pm_few(At.b(), crm.d, crm.n(), Tt.b(), tl.d, tl.n(), Ar_rmv, sA, mlf_nrmv, htrn, M, st);
+  }
   ops += 2LL * (crt.n() + crm.n()) * tl.n() * rbn;
   if (has_Z) {
     mask_lf(Zr_j, sZ, nullptr, mlf_nz, st);

@@ -158,8 +158,10 @@ int qr_signs(const Bits& Sc, const int* tlist, int nt, const int* zlist, int nz,
 
 // plain +-1 products with a few (<= kMaxT) covariate rows, as gram() with centred = 0,
 // a_orig = z_orig = 1: out[tlist[t]*ldo + alist[i]] = base16[15] - 2 pop((A_i ^ T_t) & mask)
+// over pcols 1920-blocks, or over nwords 32-bit words when nwords >= 0 (packed rows)
 void pm_few(const Bits& A, const int* alist, int na, const Bits& T, const int* tlist, int nt,
-            const uint32_t* mask, int pcols, const double* base16, double* out, long long ldo, hipStream_t st);
+            const uint32_t* mask, int pcols, const double* base16, double* out, long long ldo, hipStream_t st,
+            int nwords = -1);
 
 // dst row i = src row rows[i] (rows[i] < 0: zeros); dst row t, bit i = src row t, bit idx[i] (rows of rbd bytes)
 void gather_rows(const Bits& src, const int* rows, int n, uint8_t* dst, hipStream_t st);

@@ -798,11 +798,12 @@ __global__ __launch_bounds__(256) void k_pm_few(const uint8_t* A, size_t rbA, co
   }
 }
 void pm_few(const Bits& A, const int* alist, int na, const Bits& T, const int* tlist, int nt,
-            const uint32_t* mask, int pcols, const double* base16, double* out, long long ldo, hipStream_t st)
+            const uint32_t* mask, int pcols, const double* base16, double* out, long long ldo, hipStream_t st,
+            int nwords)
 {
   if (na <= 0 || nt <= 0) return;
   hipLaunchKernelGGL(k_pm_few, dim3(cdiv(na, 4)), dim3(256), 0, st, A.p, A.rb, alist, na, T.p, T.rb, tlist, nt,
-                     mask, pcols * kPopWords, base16, out, ldo);
+                     mask, nwords >= 0 ? nwords : pcols * kPopWords, base16, out, ldo);
 }
 
 }  // namespace lkc

@@ -56,18 +56,33 @@ __global__ __launch_bounds__(256) void k_ww_hg(WwArgs a, double* H, double* bp, 
 {
   __shared__ long long red[4][kMaxT + 1];
   const int j = blockIdx.x, wave = threadIdx.x >> 6;
-  const uint8_t* pm = (const uint8_t*)a.pm;
-  const int n_total = (int)(a.prb * 8);
+  const uint32_t* pm = a.pm;
+  const int nwd = (int)(a.prb / 4);       // 32-column words of the n extent
   long long g = 0, h[kMaxT];
 #pragma unroll
   for (int t = 0; t < kMaxT; t++) h[t] = 0;
-  for (int n = threadIdx.x; n < n_total; n += 256) {
-    if (!bitv(pm, n)) continue;
-    const long long v = (j < a.nw) ? (long long)a.cz[(long long)j * a.ldc + n] : 1;
-    g += v;
+  const uint32_t* Tw[kMaxT];
 #pragma unroll
-    for (int t = 0; t < kMaxT; t++)
-      if (t < a.nt) h[t] += bitv(a.Tc.p + (size_t)a.tlist[t] * a.Tc.rb, n) ? v : -v;
+  for (int t = 0; t < kMaxT; t++) Tw[t] = (const uint32_t*)(a.Tc.p + (size_t)a.tlist[min(t, a.nt - 1)] * a.Tc.rb);
+  // thread per word: the mask word and the covariate words are loaded once, the 32
+  // values of the word in eight 16-byte loads (words of an all-zero mask are skipped)
+  for (int w = threadIdx.x; w < nwd; w += 256) {
+    const uint32_t m = pm[w];
+    if (!m) continue;
+    uint32_t tw[kMaxT];
+#pragma unroll
+    for (int t = 0; t < kMaxT; t++) tw[t] = Tw[t][w];
+#pragma unroll 8
+    for (int q = 0; q < 32; q++) {
+      // bit of column 32 w + q: byte q / 8 of the word, MSB first
+      const int sh = 8 * (q >> 3) + 7 - (q & 7);
+      if (!((m >> sh) & 1u)) continue;
+      const long long v = (j < a.nw) ? (long long)a.cz[(long long)j * a.ldc + 32 * w + q] : 1;
+      g += v;
+#pragma unroll
+      for (int t = 0; t < kMaxT; t++)
+        if (t < a.nt) h[t] += ((tw[t] >> sh) & 1u) ? v : -v;
+    }
   }
   // integer sums: the order of the partial sums does not matter
   g = wsum64(g);
