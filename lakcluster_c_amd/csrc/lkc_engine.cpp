@@ -6,6 +6,9 @@
 // on the device; per step only the two score columns that are sorted travel to
 // the host, and the new masks and index lists travel back.
 #include "lkc_engine.h"
+
+#include <algorithm>
+#include <thread>
 #include "lkc_stages.h"
 
 #include <math.h>
@@ -91,8 +94,32 @@ void parse_cfg(const char* text, Cfg& c)
     else if (name == "GLOBAL_S_n_name_") c.S_n = path(first);
     else if (name == "GLOBAL_S_t_name_") c.S_t = path(first);
     else if (name == "GLOBAL_scramble_num") c.scramble_num = atoi(val.c_str());
+    else if (name == "GLOBAL_scramble_out_xdrop_") c.scramble_files_raw = val;
+    else if (name == "GLOBAL_scramble_rseed_") c.scramble_seeds_raw = val;
     else if (name.compare(0, 7, "GLOBAL_") == 0) { /* accepted; no effect on this path */ }
     else throw Error(LKC_ERR_CONFIG, "unknown key " + name);
+  }
+  // the scramble lists are sized by GLOBAL_scramble_num (updateglobals.c:122-135)
+  if (c.scramble_num > 0) {
+    auto split = [](const std::string& v) {
+      std::vector<std::string> out; size_t q = 0;
+      while (q <= v.size()) {
+        size_t k = v.find(',', q); if (k == std::string::npos) k = v.size();
+        out.push_back(trim(v.substr(q, k - q))); q = k + 1;
+      }
+      return out;
+    };
+    c.scramble_files.assign(c.scramble_num, std::string());
+    c.scramble_seeds.resize(c.scramble_num);
+    for (int i = 0; i < c.scramble_num; i++) c.scramble_seeds[i] = (unsigned long)(i + 1);
+    if (!c.scramble_files_raw.empty()) {
+      std::vector<std::string> f = split(c.scramble_files_raw);
+      for (int i = 0; i < c.scramble_num && i < (int)f.size(); i++) c.scramble_files[i] = f[i];
+    }
+    if (!c.scramble_seeds_raw.empty()) {
+      std::vector<std::string> f = split(c.scramble_seeds_raw);
+      for (int i = 0; i < c.scramble_num && i < (int)f.size(); i++) if (!f[i].empty()) c.scramble_seeds[i] = (unsigned long)atol(f[i].c_str());
+    }
   }
 }
 
@@ -304,7 +331,8 @@ void Run::load(const char* in_text)
       c.QR.find("condense") != std::string::npos || c.QC.find("store one") != std::string::npos ||
       c.QR.find("ZtSWn") != std::string::npos || c.QC.find("ZtSWn") != std::string::npos)
     throw Error(LKC_ERR_CONFIG, "only the YnWt strategies are supported");
-  if (c.scramble_num > 0) throw Error(LKC_ERR_CONFIG, "scramble is not supported");
+  for (int i = 0; i < c.scramble_num; i++)
+    if (c.scramble_files[i].empty()) throw Error(LKC_ERR_CONFIG, "GLOBAL_scramble_out_xdrop_ names fewer files than GLOBAL_scramble_num");
   if (c.B < 1 || c.B > 32) throw Error(LKC_ERR_CONFIG, "GLOBAL_B_MLT must be in [1,32]");
   if (c.A_t.empty() && c.A_n.empty()) throw Error(LKC_ERR_CONFIG, "an A_n or A_t file is required");
   make_stream(&st, dev);
@@ -513,6 +541,13 @@ void Run::setup_device(double t0)
     std::vector<int> hc((size_t)M * 2, 0);
     HIPCHK(hipMemcpyAsync(hc.data(), cnt, sizeof(int) * (size_t)M * (has_Z ? 2 : 1), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    if (c.scramble_num > 0) {
+      // bcc_scramble runs after bcc_load, so A_p above is the loaded data's; the control
+      // frequency of column k is its count over the loaded Z rows (lf_Zt_rsum, An_v.c:5-57)
+      std::vector<int> zc(M, 0);
+      if (has_Z) for (int k = 0; k < M; k++) zc[k] = hc[(size_t)M + k];
+      scramble(zc, popmask(hZr_b, NZ));
+    }
     std::vector<double> AZ(M, 0.0);
     for (int k = 0; k < M; k++) {
       double a = 0, z = 0;
@@ -594,6 +629,102 @@ void Run::setup_device(double t0)
   trace.reserve((size_t)(M + NA + 2) * 6);
   HIPCHK(hipStreamSynchronize(st));
   load_s = now_s() - t0;
+}
+
+// ------------------------------------------------------------------
+// bcc_scramble (R_init.c:165-193 and 256-285, with R_handle_make R_init.c:5-29,
+// xcc_out_xdrop_load / _sort bcc_scorebox.c:163-196, bcc_out_xdrop_prune
+// R_init.c:195-238).  The rows and the columns named in an out_xdrop file (each in
+// ascending order, only those of the loaded masks) have the bits of every (row, column)
+// pair redrawn, row by row and column by column, as Bernoulli(p_k), p_k = the column's
+// count over the loaded Z rows / their number, from the reference's Lehmer generator
+// (R01GET, boxmuller.c:57-66: seed = seed * 35 mod 2^31 - 1, r = seed / (2^31 - 1) in
+// long double).  Both copies of A get the new bits.
+// ------------------------------------------------------------------
+static const unsigned long kRngMul = 35UL, kRngMod = 2147483647UL;
+static inline double r01_next(unsigned long* seed)
+{
+  *seed = (*seed * kRngMul) % kRngMod;
+  return (double)((long double)*seed / (long double)kRngMod);
+}
+// seed after n steps: seed * 35^n mod (2^31 - 1)
+static unsigned long rng_jump(unsigned long seed, unsigned long long n)
+{
+  unsigned long long r = seed % kRngMod, b = kRngMul;
+  while (n) { if (n & 1) r = r * b % kRngMod; b = b * b % kRngMod; n >>= 1; }
+  return (unsigned long)r;
+}
+
+void Run::scramble(const std::vector<int>& zcnt, int Zrb_)
+{
+  for (int s = 0; s < c.scramble_num; s++) {
+    const std::string& fn = c.scramble_files[s];
+    FILE* fp = fopen(fn.c_str(), "r");
+    if (!fp) throw Error(LKC_ERR_IO, "cannot open scramble file " + fn);
+    // the file holds one pair per line; rows are the first entries >= 0, columns the second
+    int nl = 0;
+    for (int ch; (ch = fgetc(fp)) != EOF;) nl += ch == '\n';
+    fseek(fp, 0, SEEK_SET);
+    std::vector<int> rows, cols;
+    for (int l = 0; l < nl; l++) {
+      int a = -1, b = -1;
+      if (fscanf(fp, "%d %d", &a, &b) != 2) break;
+      if (a >= 0) rows.push_back(a);
+      if (b >= 0) cols.push_back(b);
+    }
+    fclose(fp);
+    std::sort(rows.begin(), rows.end());
+    std::sort(cols.begin(), cols.end());
+    // bcc_out_xdrop_prune: keep the indices inside the loaded masks
+    std::vector<int> pr, pc;
+    for (int a : rows) if (a >= 0 && a < NA && bget(hAr_b.data(), a)) pr.push_back(a);
+    for (int k : cols) if (k >= 0 && k < M && bget(hAc_b.data(), k)) pc.push_back(k);
+    if (pr.empty() || pc.empty()) continue;
+    // A repeated index is redrawn in place, so its last occurrence decides: the unique
+    // rows and columns with the draw index of their last occurrence.
+    std::vector<int> ur, uc; std::vector<long long> ri, cj;
+    for (size_t i = 0; i < pr.size(); i++) {
+      if (!ur.empty() && ur.back() == pr[i]) ri.back() = (long long)i;
+      else { ur.push_back(pr[i]); ri.push_back((long long)i); }
+    }
+    for (size_t j = 0; j < pc.size(); j++) {
+      if (!uc.empty() && uc.back() == pc[j]) cj.back() = (long long)j;
+      else { uc.push_back(pc[j]); cj.push_back((long long)j); }
+    }
+    const long long ncol = (long long)pc.size();
+    std::vector<double> pk(uc.size());
+    for (size_t j = 0; j < uc.size(); j++) pk[j] = (double)zcnt[uc[j]] / (double)Zrb_;
+    const int wpr = (int)((uc.size() + 31) / 32);
+    std::vector<uint32_t> bits((size_t)ur.size() * wpr, 0u);
+    const unsigned long seed0 = c.scramble_seeds[s];
+    // rows in parallel: row i of the draw order starts i * ncol draws after the seed
+    const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    auto work = [&](int t) {
+      for (size_t u = t; u < ur.size(); u += nth) {
+        unsigned long sd = rng_jump(seed0, (unsigned long long)ri[u] * (unsigned long long)ncol);
+        size_t q = 0;
+        for (long long j = 0; j < ncol && q < uc.size(); j++) {
+          const double r = r01_next(&sd);
+          if (j == cj[q]) {
+            if (r < pk[q]) bits[u * wpr + q / 32] |= 1u << (q % 32);
+            q++;
+          }
+        }
+      }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nth; t++) th.emplace_back(work, t);
+    work(0);
+    for (auto& t : th) t.join();
+    int* dr = dev.alloc<int>(ur.size() + uc.size());
+    uint32_t* db = dev.alloc<uint32_t>(bits.size());
+    HIPCHK(hipMemcpyAsync(dr, ur.data(), sizeof(int) * ur.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dr + ur.size(), uc.data(), sizeof(int) * uc.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(db, bits.data(), sizeof(uint32_t) * bits.size(), hipMemcpyHostToDevice, st));
+    scramble_bits(An.p, An.rb, At.p, At.rb, dr, (int)ur.size(), dr + ur.size(), (int)uc.size(), db, st);
+    HIPCHK(hipStreamSynchronize(st));
+    dev.release(dr); dev.release(db);
+  }
 }
 
 // masks and lists: host -> pinned block -> device (one copy)

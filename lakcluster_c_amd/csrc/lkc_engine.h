@@ -19,6 +19,10 @@ struct Cfg {
   int sparse = 1; int nbins = 0; int Ireq = 0; int A_ncols = 0, Y_ncols = 0, T_ncols = 0; int scramble_num = 0;
   std::string QR, QC, A_n, A_t, A_rind, A_cind, A_p, Z_n, Z_t, Z_rind, T_n, T_t, T_cind, S_n, S_t;
   std::string xpre, dir_name, dir_base, out_name;
+  // bcc_scramble: per scramble an out_xdrop file and a seed (GLOBAL_scramble_out_xdrop_,
+  // GLOBAL_scramble_rseed_; the seed of scramble n defaults to n + 1)
+  std::vector<std::string> scramble_files; std::vector<unsigned long> scramble_seeds;
+  std::string scramble_files_raw, scramble_seeds_raw;
 };
 void parse_cfg(const char* text, Cfg& c);
 void get_xdrop(double lrij, double lcij, double gamma, int gamma_type, int* rdrop, int* cdrop);
@@ -146,6 +150,7 @@ struct Run {
   void masks_from_files();
   void finish_masks();
   void setup_device(double t0);
+  void scramble(const std::vector<int>& zcnt, int Zrb);
   void push_ctl();
   void st_ajdk(); void st_ZtSn(); void st_YnWt(); void st_At_T(); void st_QR();
   void st_single(); void st_double(); void st_flatten();

@@ -156,6 +156,11 @@ void gram_split(const GramMArgs& g, uint32_t* X, hipStream_t st);
 // sg[t*nzp + i] = S_t(zlist[i]) (+-1) for the QR sums; returns nzp.  sgbuf: qr_sign_bytes(nz, nt)
 int qr_signs(const Bits& Sc, const int* tlist, int nt, const int* zlist, int nz, signed char* sgbuf, hipStream_t st);
 
+// bcc_scramble's redrawn bits: bit q of bits[u*ceil(nc/32) + q/32] -> An(rows[u], cols[q]) and
+// At(cols[q], rows[u]); rows and cols unique
+void scramble_bits(uint8_t* An, size_t rbA, uint8_t* At, size_t rbT, const int* rows, int nr, const int* cols, int nc,
+                   const uint32_t* bits, hipStream_t st);
+
 // plain +-1 products with a few (<= kMaxT) covariate rows, as gram() with centred = 0,
 // a_orig = z_orig = 1: out[tlist[t]*ldo + alist[i]] = base16[15] - 2 pop((A_i ^ T_t) & mask)
 // over pcols 1920-blocks, or over nwords 32-bit words when nwords >= 0 (packed rows)

@@ -797,6 +797,36 @@ __global__ __launch_bounds__(256) void k_pm_few(const uint8_t* A, size_t rbA, co
       }
   }
 }
+// bcc_scramble's new bits: bit q of bits[u*wpr + q/32] becomes bit (rows[u], cols[q]) of
+// An and bit (cols[q], rows[u]) of At.  Rows and columns are unique, so every bit has one
+// writer; neighbours in a word are written with atomic and / or.
+__device__ __forceinline__ void set_bit_word(uint32_t* w, int i, bool v)
+{
+  const uint32_t m = 1u << (8 * ((i & 31) >> 3) + 7 - (i & 7));   // MSB first in each byte
+  if (v) atomicOr(w + (i >> 5), m); else atomicAnd(w + (i >> 5), ~m);
+}
+__global__ void k_scramble_bits(uint8_t* An, size_t rbA, uint8_t* At, size_t rbT, const int* rows, int nr,
+                                const int* cols, int nc, const uint32_t* bits)
+{
+  const int wpr = (nc + 31) / 32;
+  const long long tot = (long long)nr * nc;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
+    const int u = (int)(e / nc), q = (int)(e % nc);
+    const bool v = (bits[(size_t)u * wpr + q / 32] >> (q % 32)) & 1u;
+    const int a = rows[u], k = cols[q];
+    set_bit_word((uint32_t*)(An + (size_t)a * rbA), k, v);
+    set_bit_word((uint32_t*)(At + (size_t)k * rbT), a, v);
+  }
+}
+void scramble_bits(uint8_t* An, size_t rbA, uint8_t* At, size_t rbT, const int* rows, int nr, const int* cols, int nc,
+                   const uint32_t* bits, hipStream_t st)
+{
+  const long long tot = (long long)nr * nc;
+  if (tot <= 0) return;
+  hipLaunchKernelGGL(k_scramble_bits, dim3(cdiv(tot, 256) < 65536 ? cdiv(tot, 256) : 65536), dim3(256), 0, st,
+                     An, rbA, At, rbT, rows, nr, cols, nc, bits);
+}
+
 void pm_few(const Bits& A, const int* alist, int na, const Bits& T, const int* tlist, int nt,
             const uint32_t* mask, int pcols, const double* base16, double* out, long long ldo, hipStream_t st,
             int nwords)

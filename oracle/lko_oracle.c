@@ -499,6 +499,8 @@ typedef struct {
   char T_n[LKO_PATH], T_t[LKO_PATH], T_cind[LKO_PATH], S_n[LKO_PATH], S_t[LKO_PATH];
   char xpre[LKO_PATH], dir_name[LKO_PATH], dir_base[LKO_PATH], out_name[LKO_PATH];
   int scramble_num;
+  /* bcc_scramble inputs: out_xdrop files and seeds (seed n defaults to n + 1) */
+  char scr_files_raw[LKO_PATH], scr_seeds_raw[LKO_PATH];
 } lko_cfg;
 
 static void trim(char *s)
@@ -561,6 +563,8 @@ static int cfg_parse(const char *text, lko_cfg *c)
     else if (!strcmp(name, "GLOBAL_S_n_name_")) set_path(c, c->S_n, first);
     else if (!strcmp(name, "GLOBAL_S_t_name_")) set_path(c, c->S_t, first);
     else if (!strcmp(name, "GLOBAL_scramble_num")) c->scramble_num = atoi(val);
+    else if (!strcmp(name, "GLOBAL_scramble_out_xdrop_")) snprintf(c->scr_files_raw, LKO_PATH, "%s", val);
+    else if (!strcmp(name, "GLOBAL_scramble_rseed_")) snprintf(c->scr_seeds_raw, LKO_PATH, "%s", val);
     else if (!strncmp(name, "GLOBAL_", 7)) { /* accepted, no effect on this path */ }
     else { fprintf(stderr, "lko: unknown key %s\n", name); return -1; }
   }
@@ -1203,6 +1207,75 @@ typedef struct {
 
 static int fail(lko_result *R, const char *msg) { R->status = -1; snprintf(R->err, sizeof R->err, "%s", msg); return -1; }
 
+/* ------------------------------------------------------------------ */
+/* bcc_scramble (R_init.c:165-193, 256-285; R_handle_make R_init.c:5-29; xcc_out_xdrop_load
+   and _sort bcc_scorebox.c:163-196; bcc_out_xdrop_prune R_init.c:195-238): the rows and
+   columns named in an out_xdrop file, ascending and restricted to the loaded masks, get
+   every (row, column) bit redrawn in row-major order as (r < p_k), r from R01GET
+   (boxmuller.c:57-66: seed = seed * 35 % (2^31 - 1), r = seed / (2^31 - 1) in long
+   double), p_k = the column's count over the loaded Z rows / their number.  Parity
+   pinned by the closed form of the generator and the resampling invariants
+   (tests/test_scramble.py).                                                          */
+/* ------------------------------------------------------------------ */
+static double lko_r01(unsigned long *seed)
+{
+  *seed = (*seed * 35UL) % 2147483647UL;
+  return (double)((long double)*seed / (long double)2147483647UL);
+}
+static int icmp(const void *a, const void *b) { int x = *(const int *)a, y = *(const int *)b; return x < y ? -1 : x > y; }
+static void bclr_(uint8_t *b, long i) { b[i >> 3] &= (uint8_t)~(1u << (7 - (i & 7))); }
+
+static int list_item(const char *raw, int n, char *out, size_t cap)
+{
+  /* the n-th comma-separated entry, trimmed */
+  const char *p = raw;
+  for (int i = 0; i < n && p; i++) { p = strchr(p, ','); if (p) p++; }
+  if (!p) { out[0] = 0; return -1; }
+  const char *e = strchr(p, ','); size_t l = e ? (size_t)(e - p) : strlen(p); if (l >= cap) l = cap - 1;
+  memcpy(out, p, l); out[l] = 0; trim(out);
+  return out[0] ? 0 : -1;
+}
+
+static int st_scramble(lko_state *S, lko_result *R)
+{
+  lko_cfg *c = &S->c;
+  int Zrb = popmask(S->Zr_b, S->NZ);
+  for (int s = 0; s < c->scramble_num; s++) {
+    char fn[LKO_PATH], sd[64];
+    if (list_item(c->scr_files_raw, s, fn, sizeof fn)) return fail(R, "GLOBAL_scramble_out_xdrop_ names fewer files than GLOBAL_scramble_num");
+    unsigned long seed = (unsigned long)(s + 1);
+    if (!list_item(c->scr_seeds_raw, s, sd, sizeof sd)) seed = (unsigned long)atol(sd);
+    FILE *fp = fopen(fn, "r"); if (!fp) return fail(R, "cannot open a scramble file");
+    int nl = 0; for (int ch; (ch = fgetc(fp)) != EOF;) nl += ch == '\n';
+    fseek(fp, 0, SEEK_SET);
+    int *rows = (int *)xcalloc((size_t)nl + 1, sizeof(int)), *cols = (int *)xcalloc((size_t)nl + 1, sizeof(int));
+    int nr = 0, nc = 0;
+    for (int l = 0; l < nl; l++) {
+      int a = -1, b = -1;
+      if (fscanf(fp, "%d %d", &a, &b) != 2) break;
+      if (a >= 0) rows[nr++] = a;
+      if (b >= 0) cols[nc++] = b;
+    }
+    fclose(fp);
+    qsort(rows, nr, sizeof(int), icmp); qsort(cols, nc, sizeof(int), icmp);
+    int pr = 0, pc = 0;
+    for (int i = 0; i < nr; i++) if (rows[i] >= 0 && rows[i] < S->NA && bget(S->Ar_b, rows[i])) rows[pr++] = rows[i];
+    for (int j = 0; j < nc; j++) if (cols[j] >= 0 && cols[j] < S->M && bget(S->Ac_b, cols[j])) cols[pc++] = cols[j];
+    for (int i = 0; i < pr; i++) {
+      const int a = rows[i];
+      for (int j = 0; j < pc; j++) {
+        const int k = cols[j];
+        const double zs = S->has_Z ? (double)pop_and(bm_row(&S->Zt, k), S->Zr_b, S->Zt.rb) : 0.0;
+        const int v = lko_r01(&seed) < zs / (double)Zrb;
+        uint8_t *an = (uint8_t *)bm_row(&S->An, a), *at = (uint8_t *)bm_row(&S->At, k);
+        if (v) { bset(an, k); bset(at, a); } else { bclr_(an, k); bclr_(at, a); }
+      }
+    }
+    free(rows); free(cols);
+  }
+  return 0;
+}
+
 static int st_load(lko_state *S, lko_result *R)
 {
   lko_cfg *c = &S->c;
@@ -1210,7 +1283,6 @@ static int st_load(lko_state *S, lko_result *R)
   if (c->Y_ncols != 0) return fail(R, "Y/W genetic controls are not supported");
   if (!strstr(c->QR, "YnWt") || !strstr(c->QC, "YnWt") || strstr(c->QR, "condense") || strstr(c->QC, "store one")
       || strstr(c->QR, "ZtSWn") || strstr(c->QC, "ZtSWn")) return fail(R, "only the YnWt strategies are supported");
-  if (c->scramble_num > 0) return fail(R, "scramble is not supported");
   if (!c->A_t[0] || !c->A_n[0]) return fail(R, "A_n/A_t files required");
   if (b16_read(c->A_t, &S->An) || b16_read(c->A_n, &S->At)) return fail(R, "cannot read A");
   S->NA = S->An.nr; S->M = S->An.nc;
@@ -1276,6 +1348,8 @@ static int st_load(lko_state *S, lko_result *R)
   }
   free(AZ_rsum);
   lko_calc_ajdk(S->A_p, S->pcols, S->ajdk);
+  /* bcc_scramble runs after bcc_load (lakcluster_driver.c:56-62): A_p is the loaded data's */
+  if (c->scramble_num > 0 && st_scramble(S, R)) return -1;
   /* xcalc orientation flags (bcc_double_init_lf, bcc.c:1005-1150) */
   int rdrop0 = 0; lko_get_xdrop(Arb, popmask(S->Ac_b, S->M), c->gamma, c->gamma_type, &rdrop0, NULL);
   int rdrop2 = MAXI(2, rdrop0);
@@ -1451,6 +1525,13 @@ long lko_state_get(void *h, const char *name, double *out, long n)
   struct { const char *nm; const uint8_t *m; int n; } mt[] = {
     { "mask_Ar_j", S->Ar_j, S->NA }, { "mask_Zr_j", S->Zr_j, S->NZ }, { "mask_Ac_j", S->Ac_j, S->M },
   };
+  /* the bit matrices as 0/1 values, row-major (after any scramble) */
+  if (!strcmp(name, "An_bits") || !strcmp(name, "At_bits") || !strcmp(name, "Zn_bits")) {
+    const bm_t *B = name[0] == 'Z' ? &S->Zn : (name[1] == 'n' ? &S->An : &S->At);
+    const long len = (long)B->nr * B->nc;
+    if (out) for (long e = 0; e < MINI(len, n); e++) out[e] = bget(bm_row(B, (int)(e / B->nc)), e % B->nc);
+    return len;
+  }
   for (size_t i = 0; i < sizeof mt / sizeof mt[0]; i++) {
     if (strcmp(mt[i].nm, name)) continue;
     if (out) for (long j = 0; j < MINI((long)mt[i].n, n); j++) out[j] = bget(mt[i].m, j);
