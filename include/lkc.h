@@ -10,7 +10,8 @@
  *
  * Scope of the path (as in the reference driver with these settings): one bin
  * (GLOBAL_NBINS= 1), no Y/W genetic controls (GLOBAL_Y_n_cols= 0), QR/QC
- * strategies "YnWt", no scramble, at most 8 covariate columns.
+ * strategies "YnWt", at most 8 covariate columns; bcc_scramble of the loaded A
+ * (GLOBAL_scramble_num) and the lakcluster_scorebox replay are supported.
  */
 #ifndef LKC_H
 #define LKC_H
@@ -40,6 +41,17 @@ const char *lkc_bcc_error(const lkc_bcc *D);
    out_xdrop_a.txt, out_xdrop_b.txt into out_dir (NULL: GLOBAL_DIR_NAME). */
 int lkc_lakcluster_driver(const char *in_text, const char *out_dir);
 
+/* lakcluster_scorebox (lakcluster_scorebox.c:265-271, dispatched on GLOBAL_TEST_TYPE=
+   lakcluster_scorebox at lakcluster_ver18.c:338): replays the elimination order of
+   GLOBAL_scorebox_out_xdrop over the grid of GLOBAL_scorebox_row_num x _col_num
+   (rows kept, columns kept) points and writes the S_handle_dmp files
+   (S_init.c:79-89) scorebox_r<row_max>n<row_num>_c<col_max>n<col_num>_{A_rpop_j_total,
+   A_cpop_j,Irem,QR_avg,QC_avg}.mda into out_dir (NULL: GLOBAL_DIR_NAME). */
+int lkc_lakcluster_scorebox(const char *in_text, const char *out_dir);
+/* xcc_scorebox_xdrop_array (bcc_scorebox.c:282-308): xval[x_num] index values from
+   x_max down to 0, xdrop[x_num] the number of indices removed at each step */
+int lkc_scorebox_xdrop_array(int nxols, int x_max, int x_num, int *xval, int *xdrop);
+
 /* ---- elimination handle: the bcc stage API -------------------------------- */
 /* bcc_load + bcc_init (bcc.c:1482-1540, 1369-1438): reads the .b16 inputs and
    masks, computes A_p and the A_ajdk tables, uploads everything to device 0. */
@@ -58,6 +70,20 @@ int lkc_bcc_run(lkc_bcc *D, int max_iter);
 /* bcc_sumscores_dmp output files (bcc_sumscores.c:127-157 format of out_trace /
    out_xdrop_a / out_xdrop_b) */
 int lkc_bcc_write_outputs(const lkc_bcc *D, const char *out_dir);
+/* The scorebox on a freshly loaded handle (lakcluster_scorebox_rc / _cr,
+   lakcluster_scorebox.c:96-263): S_init_bcc, then per grid point bcc_scorebox_srt +
+   bcc_scorebox_mxA, the loop-subscores (recomputed at the first point, by the low-rank
+   update after it; the inner loop from a copy of the outer state, bcc_copy), then
+   bcc_sumscores_nrm/ifT/srt/cmb + bcc_scorebox_svalue + lakcluster_scorebox_excerpt_0. */
+int lkc_bcc_scorebox(lkc_bcc *D);
+/* its results: dims4 = {row_max, row_num, col_max, col_num}; each non-NULL array gets the
+   row_num*col_num grid entries at [nr + nc*row_num] (the S_handle arrays) */
+int lkc_bcc_get_scorebox(const lkc_bcc *D, int *dims4, int *rpop, int *cpop, int *irem, double *qr_avg,
+                         double *qc_avg);
+/* S_handle_dmp (S_init.c:79-89) into out_dir (NULL: GLOBAL_DIR_NAME) */
+int lkc_bcc_write_scorebox(const lkc_bcc *D, const char *out_dir);
+/* 1 when the handle's GLOBAL_TEST_TYPE is lakcluster_scorebox (lakcluster_ver18.c:337-338) */
+int lkc_bcc_is_scorebox(const lkc_bcc *D);
 
 /* The individual stages, in the order lakcluster_driver calls them.  They act on
    the device-resident state of D (lakcluster_header.h:394-502 names). */

@@ -7,7 +7,7 @@
 #include <string>
 #include <vector>
 
-struct lkc_bcc { lkc::Run run; };
+struct lkc_bcc { lkc::Run run; lkc::Scorebox sb; bool has_sb = false; };
 struct lkc_pool { lkc::Pool pool; };
 
 static thread_local std::string g_last_error;
@@ -85,6 +85,54 @@ int lkc_lakcluster_driver(const char* in_text, const char* out_dir)
   if (!rc) rc = lkc_bcc_write_outputs(D, out_dir);
   lkc_bcc_free(D);
   return rc;
+}
+
+int lkc_bcc_scorebox(lkc_bcc* D)
+{
+  if (!D) return LKC_ERR_STATE;
+  return guard(D, [&] { D->has_sb = false; D->run.scorebox(D->sb); D->has_sb = true; });
+}
+
+int lkc_bcc_get_scorebox(const lkc_bcc* D, int* dims4, int* rpop, int* cpop, int* irem, double* qr, double* qc)
+{
+  if (!D || !dims4 || !D->has_sb) return LKC_ERR_STATE;
+  const lkc::Scorebox& S = D->sb;
+  dims4[0] = S.row_max; dims4[1] = S.row_num; dims4[2] = S.col_max; dims4[3] = S.col_num;
+  const size_t n = (size_t)S.row_num * S.col_num;
+  if (rpop) memcpy(rpop, S.rpop.data(), sizeof(int) * n);
+  if (cpop) memcpy(cpop, S.cpop.data(), sizeof(int) * n);
+  if (irem) memcpy(irem, S.irem.data(), sizeof(int) * n);
+  if (qr) memcpy(qr, S.qr.data(), sizeof(double) * n);
+  if (qc) memcpy(qc, S.qc.data(), sizeof(double) * n);
+  return LKC_OK;
+}
+
+int lkc_bcc_write_scorebox(const lkc_bcc* D, const char* out_dir)
+{
+  if (!D || !D->has_sb) return LKC_ERR_STATE;
+  return guard(const_cast<lkc_bcc*>(D), [&] {
+    lkc::write_scorebox(D->sb, out_dir && out_dir[0] ? std::string(out_dir) : D->run.default_out_dir());
+  });
+}
+
+int lkc_bcc_is_scorebox(const lkc_bcc* D) { return D && D->run.c.test_type == "lakcluster_scorebox" ? 1 : 0; }
+
+int lkc_lakcluster_scorebox(const char* in_text, const char* out_dir)
+{
+  lkc_bcc* D = nullptr;
+  int rc = lkc_bcc_load(in_text, &D);
+  if (rc) return rc;
+  rc = lkc_bcc_scorebox(D);
+  if (!rc) rc = lkc_bcc_write_scorebox(D, out_dir);
+  lkc_bcc_free(D);
+  return rc;
+}
+
+int lkc_scorebox_xdrop_array(int nxols, int x_max, int x_num, int* xval, int* xdrop)
+{
+  if (x_num < 1 || !xval || !xdrop) { g_last_error = "x_num must be >= 1 with both outputs given"; return LKC_ERR_STATE; }
+  lkc::scorebox_xdrop_array(nxols, x_max, x_num, xval, xdrop);
+  return LKC_OK;
 }
 
 int lkc_bcc_n_iter(const lkc_bcc* D) { return D ? D->run.n_iter : -1; }

@@ -1,6 +1,7 @@
 // lkc_cli.cpp -- the .in-driven command line: reads the key/value text of an
-// .in file on stdin (as lakcluster_ver18 does, lakcluster_ver18.c:337) and runs
-// the elimination on device 0, writing out_trace.txt / out_xdrop_{a,b}.txt.
+// .in file on stdin (as lakcluster_ver18 does, lakcluster_ver18.c:337-338) and runs
+// the elimination on device 0, writing out_trace.txt / out_xdrop_{a,b}.txt, or with
+// GLOBAL_TEST_TYPE= lakcluster_scorebox the scorebox replay and its .mda files.
 #include "../../include/lkc.h"
 #include <stdio.h>
 #include <string>
@@ -15,6 +16,16 @@ int main(int argc, char** argv)
   lkc_bcc* D = nullptr;
   int rc = lkc_bcc_load(text.c_str(), &D);
   if (rc) { fprintf(stderr, "lkc: %s\n", lkc_last_error()); return 1; }
+  if (lkc_bcc_is_scorebox(D)) {
+    rc = lkc_bcc_scorebox(D);
+    if (!rc) rc = lkc_bcc_write_scorebox(D, out_dir);
+    if (rc) { fprintf(stderr, "lkc: %s\n", lkc_bcc_error(D)); lkc_bcc_free(D); return 1; }
+    int dims[4];
+    lkc_bcc_get_scorebox(D, dims, NULL, NULL, NULL, NULL, NULL);
+    printf("scorebox rows %d..0 in %d, cols %d..0 in %d\n", dims[0], dims[1], dims[2], dims[3]);
+    lkc_bcc_free(D);
+    return 0;
+  }
   rc = lkc_bcc_run(D, -1);
   if (!rc) rc = lkc_bcc_write_outputs(D, out_dir);
   if (rc) { fprintf(stderr, "lkc: %s\n", lkc_bcc_error(D)); lkc_bcc_free(D); return 1; }

@@ -96,6 +96,12 @@ void parse_cfg(const char* text, Cfg& c)
     else if (name == "GLOBAL_scramble_num") c.scramble_num = atoi(val.c_str());
     else if (name == "GLOBAL_scramble_out_xdrop_") c.scramble_files_raw = val;
     else if (name == "GLOBAL_scramble_rseed_") c.scramble_seeds_raw = val;
+    else if (name == "GLOBAL_TEST_TYPE") c.test_type = first;
+    else if (name == "GLOBAL_scorebox_out_xdrop") c.sb_file = first;
+    else if (name == "GLOBAL_scorebox_row_max") c.sb_row_max = atoi(val.c_str());
+    else if (name == "GLOBAL_scorebox_row_num") c.sb_row_num = atoi(val.c_str());
+    else if (name == "GLOBAL_scorebox_col_max") c.sb_col_max = atoi(val.c_str());
+    else if (name == "GLOBAL_scorebox_col_num") c.sb_col_num = atoi(val.c_str());
     else if (name.compare(0, 7, "GLOBAL_") == 0) { /* accepted; no effect on this path */ }
     else throw Error(LKC_ERR_CONFIG, "unknown key " + name);
   }
@@ -655,24 +661,30 @@ static unsigned long rng_jump(unsigned long seed, unsigned long long n)
   return (unsigned long)r;
 }
 
+// xcc_out_xdrop_load + xcc_out_xdrop_sort (bcc_scorebox.c:163-196): one pair per line;
+// the rows are the first entries >= 0 and the columns the second, each in file order
+static void read_out_xdrop(const std::string& fn, const char* what, std::vector<int>& rows, std::vector<int>& cols)
+{
+  FILE* fp = fopen(fn.c_str(), "r");
+  if (!fp) throw Error(LKC_ERR_IO, std::string("cannot open ") + what + " file " + fn);
+  int nl = 0;
+  for (int ch; (ch = fgetc(fp)) != EOF;) nl += ch == '\n';
+  fseek(fp, 0, SEEK_SET);
+  rows.clear(); cols.clear();
+  for (int l = 0; l < nl; l++) {
+    int a = -1, b = -1;
+    if (fscanf(fp, "%d %d", &a, &b) != 2) break;
+    if (a >= 0) rows.push_back(a);
+    if (b >= 0) cols.push_back(b);
+  }
+  fclose(fp);
+}
+
 void Run::scramble(const std::vector<int>& zcnt, int Zrb_)
 {
   for (int s = 0; s < c.scramble_num; s++) {
-    const std::string& fn = c.scramble_files[s];
-    FILE* fp = fopen(fn.c_str(), "r");
-    if (!fp) throw Error(LKC_ERR_IO, "cannot open scramble file " + fn);
-    // the file holds one pair per line; rows are the first entries >= 0, columns the second
-    int nl = 0;
-    for (int ch; (ch = fgetc(fp)) != EOF;) nl += ch == '\n';
-    fseek(fp, 0, SEEK_SET);
     std::vector<int> rows, cols;
-    for (int l = 0; l < nl; l++) {
-      int a = -1, b = -1;
-      if (fscanf(fp, "%d %d", &a, &b) != 2) break;
-      if (a >= 0) rows.push_back(a);
-      if (b >= 0) cols.push_back(b);
-    }
-    fclose(fp);
+    read_out_xdrop(c.scramble_files[s], "scramble", rows, cols);
     std::sort(rows.begin(), rows.end());
     std::sort(cols.begin(), cols.end());
     // bcc_out_xdrop_prune: keep the indices inside the loaded masks
@@ -879,9 +891,17 @@ void Run::ww_excerpt(int trm, const Side& s1, const Side& s2, const IList& kls, 
   if (!trm) { sn = st1; sw = st2; if (full) { rl = s1.rows->d; cl = s2.rows->d; } }
   else { sn = st2; sw = st1; if (full) { rl = s2.rows->d; cl = s1.rows->d; } }
   int n_total = pn.X->nc;
+  size_t prb = pn.X->rb;
+  if (nn == 0 || nw == 0) {
+    // no rows on one side: every sum of the reference's kernel is over an empty set, so
+    // each output is +0.0 (the low-rank update after a drop of only rows or only columns)
+    ops += (long long)nn * nw * (c.B + 1) * 8;
+    ops += (long long)tl.n() * kls.n() * nw * (c.B + 1) * (long long)prb * 8;
+    fill(out, (long long)M * T, 0.0, st);
+    return;
+  }
   long long ldc = std::max(n_total, 1);
   uint32_t* cz = (uint32_t*)g_cz.need(dev, sizeof(uint32_t) * (size_t)std::max(nw, 1) * ldc);
-  size_t prb = pn.X->rb;
   // The matrix-core contraction runs over the patients of the n side only: packed to the
   // front when a packed copy is given (the quantised values then go to list positions),
   // else the rows' whole 128-bit groups.  The integer sums over n do not depend on where
@@ -1062,8 +1082,10 @@ void Run::init_lf()
   init_s = now_s() - t0;
 }
 
-// bcc_sumscores_nrm/ifT/cmb/xij/mxA (bcc_sumscores.c:37-779), one bin, Ireq<=0
-void Run::sumscores()
+// bcc_sumscores_nrm/ifT/cmb (bcc_sumscores.c:324-356, 483-779), one bin: the first
+// covariate column's score of every active column into dsv[0, nk) and of every active
+// row into dsv[M, M + na), in list (index) order
+void Run::score_lists()
 {
   unsigned int Ar = (unsigned int)ar.n(), Zr = (unsigned int)zr.n();
   long long cpop = kl.n();
@@ -1084,6 +1106,12 @@ void Run::sumscores()
   p.dT = den((long long)(Ar) * (long long)(cpop - 1) * (long long)(Ar - 1));
   p.den = dmds; p.pow_form = 0; p.cmb = has_Z && Zrb > 0 && Arb > 0; p.out = dsv;
   scores(p, st);
+}
+
+// bcc_sumscores_nrm/ifT/cmb/xij/mxA (bcc_sumscores.c:37-779), one bin, Ireq<=0
+void Run::sumscores()
+{
+  score_lists();
   int nk = kl.n(), na = ar.n();
   int rpop = na, cpop_i = nk, rdrop = 0, cdrop = 0;
   get_xdrop(rpop, cpop_i, c.gamma, c.gamma_type, &rdrop, &cdrop);
@@ -1190,7 +1218,10 @@ void Run::lrup()
     }
   }
   int *c_mr = gcnt[0], *c_mm = gcnt[1], *c_mz = gcnt[2];
-  if (ww_kernel == 0) {
+  // with no column dropped (a drop of rows only) the QR updates subtract +0.0: skipped
+  const bool cols_dropped = cmp_nseg > 0;
+  if (!cols_dropped) {
+  } else if (ww_kernel == 0) {
     seg_counts(Anc, rtn.d, rtn.n(), nullptr, d_seg, cmp_nseg, c_mr, NA, st);
     seg_counts(Anc, rmv.d, rmv.n(), nullptr, d_seg, cmp_nseg, c_mm, NA, st);
     if (has_Z) {
@@ -1256,7 +1287,7 @@ void Run::lrup()
       gram_mfma(g1, st);
       GramMArgs g2{ Anc.p, Anc.rb, rmv.d, nrm, Znc.p, Znc.rb, zr.d, nz, nullptr, d_seg, cmp_nseg, c_mm, NA, c_mz, NZ,
                     ajdk, pcols, mlf_rmv, hn_ajdk + 6 * (size_t)NA, vn_ajdk + 6 * (size_t)NZ, hnvt, nrm, 0, 0, 0 };
-      gram_mfma(g2, st);
+      if (cols_dropped) gram_mfma(g2, st);
       ops += 2LL * nrm * nz * rbm;
     }
     GramMArgs g3{ Ak.p, Ak.rb, rtn.d, nrt, Ak.p, Ak.rb, rmv.d, nrm, nullptr, d_seg_k, kc_nseg, c_kr, NA, c_km, NA,
@@ -1276,7 +1307,7 @@ void Run::lrup()
       gram_any(g1);
       GramMArgs g2{ Anc.p, Anc.rb, rmv.d, nrm, Znc.p, Znc.rb, zr.d, nz, nullptr, d_seg, cmp_nseg, c_mm, NA, c_mz, NZ,
                     ajdk, pcols, mlf_rmv, hn_ajdk + 6 * (size_t)NA, vn_ajdk + 6 * (size_t)NZ, hnvt, nrm, 0, 0, 0 };
-      gram_mfma(g2, st);
+      if (cols_dropped) gram_mfma(g2, st);
       ops += 2LL * nrm * nz * rbm;
     }
     GramMArgs g3{ An.p, An.rb, rtn.d, nrt, An.p, An.rb, rmv.d, nrm, Ac_rtn, nullptr, pcols, c_kr, NA, c_km, NA,
@@ -1289,8 +1320,9 @@ void Run::lrup()
     if (has_Z) {
       gram(An.b(), rmv.d, nrm, Zn.b(), zr.d, nz, Ac_rtn, pcols, ajdk, mlf_rtn,
            kn_ajdk + 6 * (size_t)NA, zn_ajdk + 6 * (size_t)NZ, 1, knzt, nrm, 0, 0, 0, st);
-      gram(Anc, rmv.d, nrm, Znc, zr.d, nz, d_cmask, pcols, ajdk, mlf_rmv,
-           hn_ajdk + 6 * (size_t)NA, vn_ajdk + 6 * (size_t)NZ, 1, hnvt, nrm, 0, 0, 0, st, d_seg, cmp_nseg);
+      if (cols_dropped)
+        gram(Anc, rmv.d, nrm, Znc, zr.d, nz, d_cmask, pcols, ajdk, mlf_rmv,
+             hn_ajdk + 6 * (size_t)NA, vn_ajdk + 6 * (size_t)NZ, 1, hnvt, nrm, 0, 0, 0, st, d_seg, cmp_nseg);
       ops += 2LL * nrm * nz * rbm;
     }
     gram(An.b(), rtn.d, nrt, An.b(), rmv.d, nrm, Ac_rtn, pcols, ajdk, mlf_rtn,
@@ -1299,6 +1331,9 @@ void Run::lrup()
          kn_ajdk + 6 * (size_t)NA, kn_ajdk + 6 * (size_t)NA, 1, knkt, nrm, 0, 0, 0, st);
   }
   ops += ((long long)nrt * nrm + (long long)nrm * nrm) * rbm;
+  // the Gram of the removed rows and the controls over no dropped column: every entry
+  // is the reference's empty-mask value, +0.0
+  if (has_Z && !cols_dropped && nrm > 0 && nz > 0) fill(hnvt, (long long)nrm * nz, 0.0, st);
   // QC a2 + a3
   double *t_knzt = mt[0], *t_hnvt = mt[1], *t_ankt = mt[2], *t_knkt = mt[3];
   Side s_rmv{ &At, Ar_rmv, &rmv, &Tt }, s_rtn{ &At, Ar_rtn, &rtn, &Tt }, s_Z{ &Zt, Zr_j, &zr, &St };
@@ -1342,7 +1377,12 @@ void Run::lrup()
     ops += (long long)(crt.n() + crm.n()) * tl.n() * (long long)Zt.rb * 8;
   }
   double *vtszn = mt[10], *jttan = mt[11], *htrkn = mt[12], *hhtrkn = mt[13];
-  if (ww_kernel == 0) {
+  if (crm.n() == 0) {
+    // no dropped column: the stage-b2 sums over the dropped columns are empty, +0.0
+    ops += (long long)tl.n() * crt.n() * crm.n() * (rbn * 2 + (has_Z ? (long long)Zt.rb * 8 : 0));
+    double* zb[4] = { vtszn, jttan, htrkn, hhtrkn };
+    for (double* z : zb) fill(z, (long long)M * T, 0.0, st);
+  } else if (ww_kernel == 0) {
     // QC b1 + b2 fused on the matrix cores; the removed rows are packed first
     // the A terms run over the packed removed / retained rows (packed above), the Z term
     // over the controls' whole 128-bit groups
@@ -1397,6 +1437,166 @@ void Run::lrup()
 }
 
 bool Run::active() const { return !ar.h.empty() || !kl.h.empty(); }
+
+// ------------------------------------------------------------------
+// lakcluster_scorebox (lakcluster_scorebox.c:5-271; S_handle_make / S_init_bcc
+// S_init.c:5-89; bcc_scorebox_mxA / _svalue / _srt and xcc_scorebox_xdrop_array
+// bcc_scorebox.c:5-133, 282-308).  The elimination order of an out_xdrop file is
+// replayed over a grid of (rows kept, columns kept).  The first grid point drops its
+// rows and columns and recomputes every loop-subscore; the outer loop (rows, or columns
+// when row_max > col_max) then follows by the low-rank update, and from each outer
+// point the inner loop does the same over the other index, from a copy of the outer
+// state.  Here the copy is a snapshot of the device arrays and host masks that the
+// inner loop starts from and that is restored before the next outer step.
+// ------------------------------------------------------------------
+void scorebox_xdrop_array(int nxols, int x_max, int x_num, int* xval, int* xdrop)
+{
+  const double d = ((double)(x_max - 0)) / ((double)std::max(1, x_num - 1));
+  for (int nx = 0; nx < x_num; nx++) xval[nx] = (int)round(x_max - nx * d);
+  xdrop[0] = std::max(0, nxols - (x_max + 1));
+  for (int nx = 1; nx < x_num; nx++) xdrop[nx] = xval[nx - 1] - xval[nx];
+}
+
+// bcc_scorebox_srt + bcc_scorebox_mxA: the current order is the last rpop (cpop) entries of
+// the file's rows (columns); its first rdrop (cdrop) entries leave.  Then bcc_lrup_mxdup.
+void Run::sb_drop(const std::vector<int>& rord, const std::vector<int>& cord, int rdrop, int cdrop)
+{
+  const int rpop = ar.n(), cpop = kl.n(), nro = (int)rord.size(), nco = (int)cord.size();
+  const int nrd = std::min(rdrop, rpop), ncd = std::min(cdrop, cpop);
+  const char* msg = "the scorebox out_xdrop order does not cover the active rows and columns";
+  if ((nro < rpop && nrd > 0) || (nco < cpop && ncd > 0)) throw Error(LKC_ERR_CONFIG, msg);
+  hAc_rtn = hAc_j; std::fill(hAc_rmv.begin(), hAc_rmv.end(), 0);
+  for (int i = 0; i < ncd; i++) {
+    const int k = cord[nco - cpop + i];
+    if (k < 0 || k >= M || !bget(hAc_j.data(), k)) throw Error(LKC_ERR_CONFIG, msg);
+    bset(hAc_rmv.data(), k); bclr(hAc_rtn.data(), k);
+  }
+  hAr_rtn = hAr_j; std::fill(hAr_rmv.begin(), hAr_rmv.end(), 0);
+  for (int i = 0; i < nrd; i++) {
+    const int a = rord[nro - rpop + i];
+    if (a < 0 || a >= NA || !bget(hAr_j.data(), a)) throw Error(LKC_ERR_CONFIG, msg);
+    bset(hAr_rmv.data(), a); bclr(hAr_rtn.data(), a);
+  }
+  hAr_j = hAr_rtn; hAc_j = hAc_rtn;
+  push_ctl();
+}
+
+// lakcluster_scorebox_excerpt_0 with bcc_scorebox_svalue: the active counts, Irem and the
+// means of the first covariate column's row and column scores, in index order
+void Run::sb_record(Scorebox& S, int tab)
+{
+  score_lists();
+  const int nk = kl.n(), na = ar.n();
+  if (nk) HIPCHK(hipMemcpyAsync(hsv, dsv, sizeof(double) * nk, hipMemcpyDeviceToHost, st));
+  if (na) HIPCHK(hipMemcpyAsync(hsv + M, dsv + M, sizeof(double) * na, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipGetLastError());
+  prof.collect();
+  double m = 0;
+  for (int i = 0; i < na; i++) m += hsv[M + i];
+  m /= (double)na; if (!std::isfinite(m)) m = 1.0;
+  S.qr[tab] = m;
+  m = 0;
+  for (int j = 0; j < nk; j++) m += hsv[j];
+  m /= (double)nk; if (!std::isfinite(m)) m = 1.0;
+  S.qc[tab] = m;
+  S.rpop[tab] = na; S.cpop[tab] = nk; S.irem[tab] = na > 0 ? 1 : 0;
+}
+
+void Run::scorebox(Scorebox& S)
+{
+  if (inited || n_iter) throw Error(LKC_ERR_STATE, "the scorebox starts from a freshly loaded handle");
+  S.row_max = c.sb_row_max; S.row_num = c.sb_row_num; S.col_max = c.sb_col_max; S.col_num = c.sb_col_num;
+  const int row_num = S.row_num, col_num = S.col_num;
+  if (row_num < 1 || col_num < 1) throw Error(LKC_ERR_CONFIG, "GLOBAL_scorebox_row_num and _col_num must be >= 1");
+  std::vector<int> rord, cord;
+  read_out_xdrop(c.sb_file, "scorebox out_xdrop", rord, cord);
+  std::vector<int> rkeep(row_num), rdrop(row_num), ckeep(col_num), cdrop(col_num);
+  scorebox_xdrop_array(kl.n(), S.col_max, col_num, ckeep.data(), cdrop.data());
+  scorebox_xdrop_array(ar.n(), S.row_max, row_num, rkeep.data(), rdrop.data());
+  const size_t ng = (size_t)row_num * col_num;
+  S.rpop.assign(ng, 0); S.cpop.assign(ng, 0); S.irem.assign(ng, 0); S.qr.assign(ng, 0.0); S.qc.assign(ng, 0.0);
+  const bool rows_outer = !(S.row_max > S.col_max);     // lakcluster_scorebox_cr, else _rc
+  const int n_out = rows_outer ? row_num : col_num, n_in = rows_outer ? col_num : row_num;
+  // everything an update reads or accumulates into
+  size_t nA = NA, nZ = std::max(NZ, 1), nM = M, nT = T;
+  std::vector<std::pair<double*, size_t>> keep = {
+    { mlf, 16 * 8 }, { lf_An_ajdk, nA * kAjdk }, { lf_Zn_ajdk, nZ * kAjdk }, { jn_ajdk, nA * kAjdk },
+    { vn_ajdk, nZ * kAjdk }, { kn_ajdk, nA * kAjdk }, { hn_ajdk, nA * kAjdk }, { an_ajdk_, nA * kAjdk },
+    { zn_ajdk, nZ * kAjdk }, { lf_AtTn, nM * nT }, { lf_ZtSn, nM * nT }, { lf_AnZt, nA * nZ }, { lf_AnAt, nA * nA },
+    { lf_AtTAnZtSZn, nM * nT }, { lf_AtTAnAtTAn, nM * nT }, { lf_AnZtSZnAt, nA * nT }, { lf_AnAtTAnAt, nA * nT },
+    { At_An_a1d1, nM }, { et_An, nM }, { An_a2d2_ZtSn, nA * nT }, { An_a2d2_AtTn, nA * nT }, { sc, 6 * nT + 2 },
+    { T_AnAt_AnAt, nT * nA }, { QR_ZSZ, nA * nT }, { QR_TAT, nA * nT }, { QC_ZSZ, nM * nT }, { QC_TAT, nM * nT } };
+  for (int i = 0; i < 14; i++) keep.push_back({ mt[i], nM * nT });
+  std::vector<double*> snap;
+  std::vector<std::vector<uint8_t>> hsnap;
+  auto save = [&]() {
+    if (snap.empty()) for (auto& e : keep) snap.push_back(dev.alloc<double>(e.second));
+    for (size_t i = 0; i < keep.size(); i++)
+      HIPCHK(hipMemcpyAsync(snap[i], keep[i].first, sizeof(double) * keep[i].second, hipMemcpyDeviceToDevice, st));
+    hsnap = { hAr_j, hAr_rtn, hAr_rmv, hAc_j, hAc_rtn, hAc_rmv };
+  };
+  auto restore = [&]() {
+    for (size_t i = 0; i < keep.size(); i++)
+      HIPCHK(hipMemcpyAsync(keep[i].first, snap[i], sizeof(double) * keep[i].second, hipMemcpyDeviceToDevice, st));
+    hAr_j = hsnap[0]; hAr_rtn = hsnap[1]; hAr_rmv = hsnap[2]; hAc_j = hsnap[3]; hAc_rtn = hsnap[4]; hAc_rmv = hsnap[5];
+    push_ctl();
+  };
+  auto update = [&]() {
+    lrup();
+    st_QR(); st_ajdk(); st_ZtSn(); st_single(); st_double(); st_flatten();
+  };
+  double t0 = now_s();
+  for (int o = 0; o < n_out; o++) {
+    if (o == 0) {
+      sb_drop(rord, cord, rdrop[0], cdrop[0]);
+      st_ajdk(); st_ZtSn(); st_YnWt(); st_At_T(); st_QR();
+      st_single(); st_double(); st_flatten();
+      init_s = now_s() - t0;
+    } else {
+      sb_drop(rord, cord, rows_outer ? rdrop[o] : 0, rows_outer ? 0 : cdrop[o]);
+      update();
+    }
+    sb_record(S, rows_outer ? o : o * row_num);
+    if (n_in > 1) save();
+    for (int i = 1; i < n_in; i++) {
+      sb_drop(rord, cord, rows_outer ? 0 : rdrop[i], rows_outer ? cdrop[i] : 0);
+      update();
+      sb_record(S, rows_outer ? o + i * row_num : i + o * row_num);
+    }
+    if (n_in > 1 && o + 1 < n_out) restore();
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipGetLastError());
+  prof.collect();
+  for (double* p : snap) dev.release(p);
+  inited = true;
+  loop_s = now_s() - t0 - init_s;
+}
+
+// S_handle_dmp (S_init.c:79-89) with mda_write_d2_i4 / _r8 (mda_io.c:5-31): an int32 rank
+// 2, the dimensions row_num and col_num, then the entries [nr + nc*row_num]
+void write_scorebox(const Scorebox& S, const std::string& dir)
+{
+  mkdir(dir.c_str(), 0755);
+  char pre[160];
+  snprintf(pre, sizeof pre, "scorebox_r%dn%d_c%dn%d", S.row_max, S.row_num, S.col_max, S.col_num);
+  auto put = [&](const char* nm, const void* data, size_t el) {
+    std::string p = dir + "/" + pre + "_" + nm + ".mda";
+    FILE* fp = fopen(p.c_str(), "wb");
+    if (!fp) throw Error(LKC_ERR_IO, "cannot write " + p);
+    int hdr[3] = { 2, S.row_num, S.col_num };
+    const size_t n = (size_t)S.row_num * S.col_num;
+    bool ok = fwrite(hdr, sizeof(int), 3, fp) == 3 && fwrite(data, el, n, fp) == n;
+    ok = (fclose(fp) == 0) && ok;
+    if (!ok) throw Error(LKC_ERR_IO, "cannot write " + p);
+  };
+  put("A_rpop_j_total", S.rpop.data(), sizeof(int));
+  put("A_cpop_j", S.cpop.data(), sizeof(int));
+  put("Irem", S.irem.data(), sizeof(int));
+  put("QR_avg", S.qr.data(), sizeof(double));
+  put("QC_avg", S.qc.data(), sizeof(double));
+}
 
 void Run::iterate(int* done)
 {

@@ -23,7 +23,16 @@ struct Cfg {
   // GLOBAL_scramble_rseed_; the seed of scramble n defaults to n + 1)
   std::vector<std::string> scramble_files; std::vector<unsigned long> scramble_seeds;
   std::string scramble_files_raw, scramble_seeds_raw;
+  // GLOBAL_TEST_TYPE and the lakcluster_scorebox grid (updateglobals.c:18, 136-140)
+  std::string test_type, sb_file; int sb_row_max = 0, sb_row_num = 0, sb_col_max = 0, sb_col_num = 0;
 };
+// lakcluster_scorebox output (struct S_handle): row_num x col_num entries at [nr + nc*row_num]
+struct Scorebox {
+  int row_max = 0, row_num = 0, col_max = 0, col_num = 0;
+  std::vector<int> rpop, cpop, irem; std::vector<double> qr, qc;
+};
+void scorebox_xdrop_array(int nxols, int x_max, int x_num, int* xval, int* xdrop);
+void write_scorebox(const Scorebox& S, const std::string& dir);
 void parse_cfg(const char* text, Cfg& c);
 void get_xdrop(double lrij, double lcij, double gamma, int gamma_type, int* rdrop, int* cdrop);
 void sort_scores(double* d, int* perm, int n);
@@ -158,8 +167,12 @@ struct Run {
                   int full, double* out, const Packed* pk = nullptr);
   void pack_rows(const int* src, int W, uint32_t* Atc, uint32_t* Ttc);
   void init_lf();
+  void score_lists();
   void sumscores();
   void lrup();
+  void sb_drop(const std::vector<int>& rord, const std::vector<int>& cord, int rdrop, int cdrop);
+  void sb_record(Scorebox& S, int tab);
+  void scorebox(Scorebox& S);
   void gram_any(const GramMArgs& g);
   bool active() const;
   void iterate(int* done);

@@ -84,6 +84,12 @@ def lib():
         "lkc_batch_run": (C.c_int, [vp, C.POINTER(Job), C.c_int, C.c_int, C.c_double, C.POINTER(JobResult)]),
         "lkc_batch_results_free": (None, [C.POINTER(JobResult), C.c_int]),
         "lkc_job_bytes": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_int]),
+        "lkc_lakcluster_scorebox": (C.c_int, [cp, cp]),
+        "lkc_scorebox_xdrop_array": (C.c_int, [C.c_int, C.c_int, C.c_int, ip, ip]),
+        "lkc_bcc_scorebox": (C.c_int, [vp]),
+        "lkc_bcc_get_scorebox": (C.c_int, [vp, ip, ip, ip, ip, dp, dp]),
+        "lkc_bcc_write_scorebox": (C.c_int, [vp, cp]),
+        "lkc_bcc_is_scorebox": (C.c_int, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -213,6 +219,62 @@ class Bcc:
         s = np.zeros(4, dtype=np.int32)
         _check(lib().lkc_bcc_shape(self._h, _ptr(s, C.c_int)), self._h)
         return tuple(int(x) for x in s)
+
+    @property
+    def is_scorebox(self) -> bool:
+        """GLOBAL_TEST_TYPE is lakcluster_scorebox."""
+        return bool(lib().lkc_bcc_is_scorebox(self._h))
+
+    def scorebox(self) -> "Scorebox":
+        """The scorebox replay of GLOBAL_scorebox_out_xdrop (lakcluster_scorebox) on this
+        freshly loaded handle; the grids are (row_num, col_num) arrays."""
+        L = lib()
+        _check(L.lkc_bcc_scorebox(self._h), self._h)
+        dims = np.zeros(4, dtype=np.int32)
+        _check(L.lkc_bcc_get_scorebox(self._h, _ptr(dims, C.c_int), None, None, None, None, None), self._h)
+        nr, nc = int(dims[1]), int(dims[3])
+        ia = [np.zeros(nr * nc, dtype=np.int32) for _ in range(3)]
+        da = [np.zeros(nr * nc, dtype=np.float64) for _ in range(2)]
+        _check(L.lkc_bcc_get_scorebox(self._h, _ptr(dims, C.c_int), *[_ptr(a, C.c_int) for a in ia],
+                                      *[_ptr(a, C.c_double) for a in da]), self._h)
+        grid = [a.reshape(nc, nr).T.copy() for a in ia + da]     # entries at [nr + nc*row_num]
+        return Scorebox(tuple(int(x) for x in dims), *grid)
+
+    def write_scorebox(self, out_dir: Optional[str] = None) -> None:
+        _check(lib().lkc_bcc_write_scorebox(self._h, out_dir.encode() if out_dir else None), self._h)
+
+
+class Scorebox:
+    """struct S_handle outputs: A_rpop_j_total, A_cpop_j, Irem, QR_avg, QC_avg as
+    (row_num, col_num) grids; dims = (row_max, row_num, col_max, col_num)."""
+
+    def __init__(self, dims, rpop, cpop, irem, qr, qc):
+        self.dims = dims
+        self.A_rpop_j_total, self.A_cpop_j, self.Irem, self.QR_avg, self.QC_avg = rpop, cpop, irem, qr, qc
+
+
+def lakcluster_scorebox(in_text: str, out_dir: Optional[str] = None) -> None:
+    """Whole scorebox run with its five .mda files written (lakcluster_scorebox)."""
+    _check(lib().lkc_lakcluster_scorebox(in_text.encode(), out_dir.encode() if out_dir else None))
+
+
+def scorebox_xdrop_array(nxols: int, x_max: int, x_num: int) -> Tuple[np.ndarray, np.ndarray]:
+    xval = np.zeros(max(x_num, 1), dtype=np.int32)
+    xdrop = np.zeros(max(x_num, 1), dtype=np.int32)
+    _check(lib().lkc_scorebox_xdrop_array(nxols, x_max, x_num, _ptr(xval, C.c_int), _ptr(xdrop, C.c_int)))
+    return xval[:x_num], xdrop[:x_num]
+
+
+def read_mda(path: str) -> np.ndarray:
+    """An .mda file of rank 2 (int32 rank, dimensions, then column-major entries); the
+    entry type follows from the file size (int32 or float64)."""
+    raw = open(path, "rb").read()
+    rank, nr, nc = np.frombuffer(raw[:12], dtype=np.int32)
+    if rank != 2:
+        raise LkcError(f"{path}: rank {rank}")
+    body = raw[12:]
+    dt = np.int32 if len(body) == 4 * nr * nc else np.float64
+    return np.frombuffer(body, dtype=dt).reshape(nc, nr).T.copy()
 
 
 def lakcluster_driver(in_text: str, out_dir: Optional[str] = None) -> None:

@@ -56,6 +56,8 @@ def lib():
         "lko_state_iterate": (C.c_int, [vp]),
         "lko_state_get": (C.c_long, [vp, cp, dp, C.c_long]),
         "lko_state_free": (None, [vp]),
+        "lko_scorebox": (C.c_int, [cp, C.c_int, ip, ip, ip, ip, dp, dp, cp, C.c_int]),
+        "lko_scorebox_xdrop_array": (None, [C.c_int, C.c_int, C.c_int, ip, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -176,6 +178,36 @@ def an_ztsn_ww(A, alist, amask, L, nlist, n_total, B, D_MLT=None):
     lib().lko_an_ztsn_ww_L(_p(A, C.c_uint8), A.shape[1], A.shape[0], _p(alist, C.c_int), alist.size, _p(amask, C.c_uint8),
                            _p(L, C.c_double), _p(nlist, C.c_int), nlist.size, n_total, ns, B, D, _p(out, C.c_double))
     return out[:, :ns]
+
+
+def scorebox_xdrop_array(nxols: int, x_max: int, x_num: int):
+    xval, xdrop = np.zeros(max(x_num, 1), np.int32), np.zeros(max(x_num, 1), np.int32)
+    lib().lko_scorebox_xdrop_array(nxols, x_max, x_num, _p(xval, C.c_int), _p(xdrop, C.c_int))
+    return xval[:x_num], xdrop[:x_num]
+
+
+class Scorebox:
+    """Scorebox grid (row_num x col_num): active counts, Irem and the mean row and column scores."""
+
+    def __init__(self, rpop, cpop, irem, qr, qc):
+        self.A_rpop_j_total, self.A_cpop_j, self.Irem, self.QR_avg, self.QC_avg = rpop, cpop, irem, qr, qc
+
+
+def scorebox(in_text: str, recompute: bool = False) -> Scorebox:
+    L = lib()
+    dims = np.zeros(4, np.int32)
+    err = C.create_string_buffer(256)
+    if L.lko_scorebox(in_text.encode(), 0, _p(dims, C.c_int), None, None, None, None, None, err, 256) != 0:
+        raise RuntimeError(f"oracle: {err.value.decode()}")
+    nr, nc = int(dims[1]), int(dims[3])
+    ia = [np.zeros(nr * nc, np.int32) for _ in range(3)]
+    da = [np.zeros(nr * nc) for _ in range(2)]
+    if L.lko_scorebox(in_text.encode(), 1 if recompute else 0, _p(dims, C.c_int), *[_p(a, C.c_int) for a in ia],
+                      *[_p(a, C.c_double) for a in da], err, 256) != 0:
+        raise RuntimeError(f"oracle: {err.value.decode()}")
+    # [nr + nc*row_num] -> (row_num, col_num)
+    sh = lambda a: a.reshape(nc, nr).T.copy()
+    return Scorebox(*[sh(a) for a in ia + da])
 
 
 class State:

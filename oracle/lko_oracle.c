@@ -501,6 +501,9 @@ typedef struct {
   int scramble_num;
   /* bcc_scramble inputs: out_xdrop files and seeds (seed n defaults to n + 1) */
   char scr_files_raw[LKO_PATH], scr_seeds_raw[LKO_PATH];
+  /* lakcluster_scorebox inputs (updateglobals.c:136-140) */
+  char test_type[LKO_PATH], sb_file[LKO_PATH];
+  int sb_row_max, sb_row_num, sb_col_max, sb_col_num;
 } lko_cfg;
 
 static void trim(char *s)
@@ -565,6 +568,12 @@ static int cfg_parse(const char *text, lko_cfg *c)
     else if (!strcmp(name, "GLOBAL_scramble_num")) c->scramble_num = atoi(val);
     else if (!strcmp(name, "GLOBAL_scramble_out_xdrop_")) snprintf(c->scr_files_raw, LKO_PATH, "%s", val);
     else if (!strcmp(name, "GLOBAL_scramble_rseed_")) snprintf(c->scr_seeds_raw, LKO_PATH, "%s", val);
+    else if (!strcmp(name, "GLOBAL_TEST_TYPE")) snprintf(c->test_type, LKO_PATH, "%s", val);
+    else if (!strcmp(name, "GLOBAL_scorebox_out_xdrop")) snprintf(c->sb_file, LKO_PATH, "%s", first);
+    else if (!strcmp(name, "GLOBAL_scorebox_row_max")) c->sb_row_max = atoi(val);
+    else if (!strcmp(name, "GLOBAL_scorebox_row_num")) c->sb_row_num = atoi(val);
+    else if (!strcmp(name, "GLOBAL_scorebox_col_max")) c->sb_col_max = atoi(val);
+    else if (!strcmp(name, "GLOBAL_scorebox_col_num")) c->sb_col_num = atoi(val);
     else if (!strncmp(name, "GLOBAL_", 7)) { /* accepted, no effect on this path */ }
     else { fprintf(stderr, "lko: unknown key %s\n", name); return -1; }
   }
@@ -1478,6 +1487,166 @@ void lko_times(void *h, double *t3) { lko_result *R = (lko_result *)h; t3[0] = R
 void lko_get_trace(void *h, double *out) { lko_result *R = (lko_result *)h; memcpy(out, R->trace, sizeof(double) * 6 * R->n_trace); }
 void lko_get_drops(void *h, int *a, int *b) { lko_result *R = (lko_result *)h; memcpy(a, R->xdrop_a, sizeof(int) * 2 * R->n_xdrop); memcpy(b, R->xdrop_b, sizeof(int) * 2 * R->n_xdrop); }
 void lko_free(void *h) { lko_result *R = (lko_result *)h; if (R) { free(R->trace); free(R->xdrop_a); free(R->xdrop_b); free(R); } }
+
+/* ------------------------------------------------------------------ */
+/* lakcluster_scorebox (lakcluster_scorebox.c:5-271; S_handle S_init.c:5-89;
+   bcc_scorebox_mxA/_svalue/_srt, xcc_scorebox_xdrop_array bcc_scorebox.c:5-133,
+   282-308): replays the elimination order of an out_xdrop file over a grid of
+   (rows kept, columns kept).  The first grid point recomputes every loop-subscore
+   after its drop; the others follow by the low-rank update, rows along the outer
+   loop (columns when row_max > col_max) and columns from a copy of that state along
+   the inner one.  Each point records the active counts, Irem and the mean row and
+   column scores in index order.  Parity pinned by the reference's worked example of
+   the schedule and by full recomputation at each grid point
+   (tests/test_scorebox.py).                                                       */
+/* ------------------------------------------------------------------ */
+void lko_scorebox_xdrop_array(int nxols, int x_max, int x_num, int *xval, int *xdrop)
+{
+  double d = ((double)(x_max - 0)) / ((double)MAXI(1, x_num - 1));
+  for (int nx = 0; nx < x_num; nx++) xval[nx] = (int)round(x_max - nx * d);
+  xdrop[0] = MAXI(0, nxols - (x_max + 1));
+  for (int nx = 1; nx < x_num; nx++) xdrop[nx] = xval[nx - 1] - xval[nx];
+}
+
+typedef struct { int *rpop, *cpop, *irem; double *qr, *qc; } lko_sbout;
+
+/* bcc_scorebox_srt + bcc_scorebox_mxA: the current order is the last rpop (cpop) entries
+   of the file's row (column) order; its first rdrop (cdrop) entries are removed */
+static int sb_drop(lko_state *S, const int *rord, int nro, const int *cord, int nco, int rdrop, int cdrop)
+{
+  int rpop = popmask(S->Ar_j, S->NA), cpop = popmask(S->Ac_j, S->M);
+  if ((nro < rpop && MINI(rdrop, rpop) > 0) || (nco < cpop && MINI(cdrop, cpop) > 0)) return -1;
+  size_t nbc = lko_bsize(S->M), nbr = lko_bsize(S->NA);
+  memcpy(S->Ac_rtn, S->Ac_j, nbc); memset(S->Ac_rmv, 0, nbc);
+  for (int nc = 0; nc < MINI(cdrop, cpop); nc++) {
+    int k = cord[nco - cpop + nc];
+    if (k < 0 || k >= S->M || !bget(S->Ac_j, k)) return -1;
+    bset(S->Ac_rmv, k); S->Ac_rtn[k >> 3] &= (uint8_t)~(1u << (7 - (k & 7)));
+  }
+  memcpy(S->Ar_rtn, S->Ar_j, nbr); memset(S->Ar_rmv, 0, nbr);
+  for (int nr = 0; nr < MINI(rdrop, rpop); nr++) {
+    int a = rord[nro - rpop + nr];
+    if (a < 0 || a >= S->NA || !bget(S->Ar_j, a)) return -1;
+    bset(S->Ar_rmv, a); S->Ar_rtn[a >> 3] &= (uint8_t)~(1u << (7 - (a & 7)));
+  }
+  return 0;
+}
+/* lakcluster_scorebox_excerpt_0 with bcc_scorebox_svalue: scores of the active rows and
+   columns in index order, first covariate column */
+static void sb_record(lko_state *S, lko_sbout *B, int tab)
+{
+  st_nrm_ifT_cmb(S);
+  ilist ar = il_from_mask(S->Ar_j, S->NA), kl = il_from_mask(S->Ac_j, S->M);
+  double *v = dalloc((size_t)MAXI(ar.n, kl.n));
+  for (int i = 0; i < ar.n; i++) v[i] = S->QR_TAT_nrm[(size_t)ar.ix[i] * S->T + 0];
+  double m = ra_mean(v, (unsigned long long)ar.n); if (!isfinite(m)) m = 1.0;
+  B->qr[tab] = m;
+  for (int j = 0; j < kl.n; j++) v[j] = S->QC_TAT_nrm[(size_t)kl.ix[j] * S->T + 0];
+  m = ra_mean(v, (unsigned long long)kl.n); if (!isfinite(m)) m = 1.0;
+  B->qc[tab] = m;
+  B->rpop[tab] = ar.n; B->cpop[tab] = kl.n; B->irem[tab] = ar.n > 0 ? 1 : 0;
+  free(v); free(ar.ix); free(kl.ix);
+}
+static void sb_full(lko_state *S)
+{
+  memcpy(S->Ar_j, S->Ar_rtn, lko_bsize(S->NA)); memcpy(S->Ac_j, S->Ac_rtn, lko_bsize(S->M));
+  st_ajdk(S); st_ZtSn(S); st_YnWt(S); st_At_T(S); st_QR(S);
+  st_single(S); st_double(S); st_flatten(S);
+}
+static void sb_update(lko_state *S) { st_lrup(S); st_rescore_tail(S); }
+
+/* bcc_copy: every array the loop carries */
+static void sb_copy(lko_state *D, const lko_state *S)
+{
+  size_t NA = S->NA, NZ = S->NZ, M = S->M, T = S->T;
+  uint8_t *dm[] = { D->Ar_j, D->Ar_rtn, D->Ar_rmv, D->Zr_j, D->Ac_j, D->Ac_rtn, D->Ac_rmv, D->Tc_j };
+  const uint8_t *sm[] = { S->Ar_j, S->Ar_rtn, S->Ar_rmv, S->Zr_j, S->Ac_j, S->Ac_rtn, S->Ac_rmv, S->Tc_j };
+  size_t mn[] = { NA, NA, NA, NZ, M, M, M, T };
+  for (int i = 0; i < 8; i++) memcpy(dm[i], sm[i], lko_bsize((int)mn[i]));
+  double *dd[] = { D->lf_An_ajdk, D->lf_Zn_ajdk, D->lf_AtTn, D->lf_ZtSn, D->lf_AnZt, D->lf_AnAt, D->lf_AtTAnZtSZn,
+                   D->lf_AtTAnAtTAn, D->lf_AnZtSZnAt, D->lf_AnAtTAnAt, D->At_An_a1d1, D->An_a2d2_ZtSn, D->An_a2d2_AtTn,
+                   D->a1d2_ZtSn, D->a3d2_ZtSn, D->a1d2_AtTn, D->a3d2_AtTn, D->et_Tn, D->et_Sn, D->et_An, D->T_AnAt_AnAt,
+                   D->QR_ZSZ, D->QR_TAT, D->QC_ZSZ, D->QC_TAT, D->QR_ZSZ_nrm, D->QR_TAT_nrm, D->QC_ZSZ_nrm, D->QC_TAT_nrm };
+  const double *sd[] = { S->lf_An_ajdk, S->lf_Zn_ajdk, S->lf_AtTn, S->lf_ZtSn, S->lf_AnZt, S->lf_AnAt, S->lf_AtTAnZtSZn,
+                         S->lf_AtTAnAtTAn, S->lf_AnZtSZnAt, S->lf_AnAtTAnAt, S->At_An_a1d1, S->An_a2d2_ZtSn, S->An_a2d2_AtTn,
+                         S->a1d2_ZtSn, S->a3d2_ZtSn, S->a1d2_AtTn, S->a3d2_AtTn, S->et_Tn, S->et_Sn, S->et_An, S->T_AnAt_AnAt,
+                         S->QR_ZSZ, S->QR_TAT, S->QC_ZSZ, S->QC_TAT, S->QR_ZSZ_nrm, S->QR_TAT_nrm, S->QC_ZSZ_nrm, S->QC_TAT_nrm };
+  size_t dn[] = { NA * LKO_AJDK, NZ * LKO_AJDK, M * T, M * T, NA * NZ, NA * NA, M * T, M * T, NA * T, NA * T, M,
+                  NA * T, NA * T, T, T, T, T, T, T, M, T * NA, NA * T, NA * T, M * T, M * T, NA * T, NA * T, M * T, M * T };
+  for (size_t i = 0; i < sizeof dd / sizeof dd[0]; i++) memcpy(dd[i], sd[i], sizeof(double) * dn[i]);
+  D->et_An_a1d1 = S->et_An_a1d1;
+}
+
+/* Runs the scorebox of an .in text.  out gets row_num x col_num entries at
+   [nr + nc*row_num]; returns 0, or -1 with a message in err. */
+static int sb_run(const char *in_text, lko_sbout *B, char *err, int nerr, int *dims, int recompute)
+{
+  lko_result R0; memset(&R0, 0, sizeof R0);
+  lko_state *S = (lko_state *)xcalloc(1, sizeof(lko_state)), *U = (lko_state *)xcalloc(1, sizeof(lko_state));
+  int rc = -1;
+  int *rord = NULL, *cord = NULL, nro = 0, nco = 0;
+  if (cfg_parse(in_text, &S->c) || cfg_parse(in_text, &U->c)) { snprintf(err, nerr, "bad .in"); goto done; }
+  {
+    lko_cfg *c = &S->c;
+    const int row_num = c->sb_row_num, col_num = c->sb_col_num;
+    if (dims) { dims[0] = c->sb_row_max; dims[1] = row_num; dims[2] = c->sb_col_max; dims[3] = col_num; }
+    if (row_num < 1 || col_num < 1) { snprintf(err, nerr, "scorebox grid needs row_num, col_num >= 1"); goto done; }
+    if (!B) { rc = 0; goto done; }
+    if (st_load(S, &R0) || st_load(U, &R0)) { snprintf(err, nerr, "%s", R0.err); goto done; }
+    FILE *fp = fopen(c->sb_file, "r");
+    if (!fp) { snprintf(err, nerr, "cannot open the scorebox out_xdrop file"); goto done; }
+    int nl = 0; for (int ch; (ch = fgetc(fp)) != EOF;) nl += ch == '\n';
+    fseek(fp, 0, SEEK_SET);
+    rord = (int *)xcalloc((size_t)nl + 1, sizeof(int)); cord = (int *)xcalloc((size_t)nl + 1, sizeof(int));
+    for (int l = 0; l < nl; l++) {
+      int a = -1, b = -1;
+      if (fscanf(fp, "%d %d", &a, &b) != 2) break;
+      if (a >= 0) rord[nro++] = a;
+      if (b >= 0) cord[nco++] = b;
+    }
+    fclose(fp);
+    int *rkeep = (int *)xcalloc(row_num, sizeof(int)), *rdrop = (int *)xcalloc(row_num, sizeof(int));
+    int *ckeep = (int *)xcalloc(col_num, sizeof(int)), *cdrop = (int *)xcalloc(col_num, sizeof(int));
+    lko_scorebox_xdrop_array(popmask(S->Ac_j, S->M), c->sb_col_max, col_num, ckeep, cdrop);
+    lko_scorebox_xdrop_array(popmask(S->Ar_j, S->NA), c->sb_row_max, row_num, rkeep, rdrop);
+    const int rows_outer = !(c->sb_row_max > c->sb_col_max);   /* lakcluster_scorebox_cr, else _rc */
+    const int n_out = rows_outer ? row_num : col_num, n_in = rows_outer ? col_num : row_num;
+    int bad = 0;
+    for (int o = 0; o < n_out && !bad; o++) {
+      const int r_o = rows_outer ? rdrop[o] : 0, c_o = rows_outer ? 0 : cdrop[o];
+      if (o == 0) {
+        if (sb_drop(S, rord, nro, cord, nco, rdrop[0], cdrop[0])) { bad = 1; break; }
+        sb_full(S);
+      } else {
+        if (sb_drop(S, rord, nro, cord, nco, r_o, c_o)) { bad = 1; break; }
+        if (recompute) sb_full(S); else sb_update(S);
+      }
+      sb_record(S, B, rows_outer ? o : o * row_num);
+      sb_copy(U, S);
+      for (int i = 1; i < n_in; i++) {
+        const int r_i = rows_outer ? 0 : rdrop[i], c_i = rows_outer ? cdrop[i] : 0;
+        if (sb_drop(U, rord, nro, cord, nco, r_i, c_i)) { bad = 1; break; }
+        if (recompute) sb_full(U); else sb_update(U);
+        sb_record(U, B, rows_outer ? o + i * row_num : i + o * row_num);
+      }
+    }
+    free(rkeep); free(rdrop); free(ckeep); free(cdrop);
+    if (bad) { snprintf(err, nerr, "the out_xdrop order does not cover the active rows and columns"); goto done; }
+    rc = 0;
+  }
+done:
+  free(rord); free(cord);
+  st_free(S); st_free(U); free(S); free(U);
+  return rc;
+}
+
+int lko_scorebox(const char *in_text, int recompute, int *dims, int *rpop, int *cpop, int *irem, double *qr,
+                 double *qc, char *err, int nerr)
+{
+  if (!rpop || !cpop || !irem || !qr || !qc) return sb_run(in_text, NULL, err, nerr, dims, recompute);
+  lko_sbout B = { rpop, cpop, irem, qr, qc };
+  return sb_run(in_text, &B, err, nerr, dims, recompute);
+}
 
 /* ------------------------------------------------------------------ */
 /* stepwise access to the state, for stage-level comparisons in tests  */

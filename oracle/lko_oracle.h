@@ -90,6 +90,14 @@ int lko_state_iterate(void *S);
 long lko_state_get(void *S, const char *name, double *out, long n);
 void lko_state_free(void *S);
 
+/* lakcluster_scorebox (lakcluster_scorebox.c:5-271): grid entries at [nr + nc*row_num];
+   dims = [row_max, row_num, col_max, col_num] (with NULL outputs only dims are read);
+   recompute=1 recomputes every loop-subscore at each grid point instead of the
+   low-rank update.  xval/xdrop: xcc_scorebox_xdrop_array (bcc_scorebox.c:282-308). */
+int lko_scorebox(const char *in_text, int recompute, int *dims, int *rpop, int *cpop, int *irem, double *qr,
+                 double *qc, char *err, int nerr);
+void lko_scorebox_xdrop_array(int nxols, int x_max, int x_num, int *xval, int *xdrop);
+
 #ifdef __cplusplus
 }
 #endif
