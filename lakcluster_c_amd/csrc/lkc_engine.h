@@ -130,7 +130,7 @@ struct Run {
   double *lf_An_ajdk = nullptr, *lf_Zn_ajdk = nullptr;              // [jk*rows + r]
   double *jn_ajdk = nullptr, *vn_ajdk = nullptr, *kn_ajdk = nullptr, *hn_ajdk = nullptr, *an_ajdk_ = nullptr, *zn_ajdk = nullptr;
   double *lf_AtTn = nullptr, *lf_ZtSn = nullptr;
-  double *lf_AnZt = nullptr, *lf_AnAt = nullptr;                      // [z*NA + a]
+  double *lf_AnZt = nullptr, *lf_AnAt = nullptr;                      // [a*NZ + z], [a2*NA + a1] (symmetric)
   double *lf_AtTAnZtSZn = nullptr, *lf_AtTAnAtTAn = nullptr;
   double *lf_AnZtSZnAt = nullptr, *lf_AnAtTAnAt = nullptr;
   double *At_An_a1d1 = nullptr, *et_An = nullptr, *An_a2d2_ZtSn = nullptr, *An_a2d2_AtTn = nullptr;
