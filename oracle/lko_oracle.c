@@ -976,7 +976,11 @@ static void mds_nrm(double *x, const ilist *rows, const ilist *cols, int ld, dou
     for (int c = 0; c < cols->n; c++) if (cols->ix[c] >= 1) v[0] -= v[cols->ix[c]] / den;
   }
 }
-static void st_nrm_ifT_cmb(lko_state *S)
+/* dcc: the half-loop normalisation of dcc_sumscores_nrm (dcc_sumscores.c:478-699, one bin,
+   Ireq<=0); the scores are the same arrays (QR_TAnAtT / QR_TAnZtS / QC_TAnAtT / QC_TAnZtS in
+   QR_TAT / QR_ZSZ / QC_TAT / QC_ZSZ), ifT and cmb as dcc_sumscores_ifT / _cmb
+   (dcc_sumscores.c:248-280, 411-477) */
+static void st_nrm_ifT_cmb_x(lko_state *S, int dcc)
 {
   ilist ar = il_from_mask(S->Ar_j, S->NA), kl = il_from_mask(S->Ac_j, S->M), tl = il_from_mask(S->Tc_j, S->T);
   unsigned int Ar = (unsigned int)ar.n, Zr = (unsigned int)popmask(S->Zr_j, S->NZ);
@@ -985,19 +989,19 @@ static void st_nrm_ifT_cmb(lko_state *S)
   long long lld;
   if (S->has_Z) {
     memset(S->QR_ZSZ_nrm, 0, sizeof(double) * nqr);
-    lld = (long long)(Zr - 0) * (long long)(cpop) * (long long)(cpop - 1);
+    lld = dcc ? (long long)(Zr - 0) * (long long)(cpop) : (long long)(Zr - 0) * (long long)(cpop) * (long long)(cpop - 1);
     dra_plusdiv(S->QR_ZSZ_nrm, S->QR_ZSZ, (double)lld, &ar, &tl, S->T);
   }
   memset(S->QR_TAT_nrm, 0, sizeof(double) * nqr);
-  lld = (long long)(Ar - 1) * (long long)(cpop) * (long long)(cpop - 1);
+  lld = dcc ? (long long)(Ar - 1) * (long long)(cpop) : (long long)(Ar - 1) * (long long)(cpop) * (long long)(cpop - 1);
   dra_plusdiv(S->QR_TAT_nrm, S->QR_TAT, (double)lld, &ar, &tl, S->T);
   if (S->has_Z) {
     memset(S->QC_ZSZ_nrm, 0, sizeof(double) * nqc);
-    lld = (long long)(Ar) * (long long)(cpop - 1) * (long long)(Zr);
+    lld = dcc ? (long long)(Ar) * (long long)(Zr) : (long long)(Ar) * (long long)(cpop - 1) * (long long)(Zr);
     dra_plusdiv(S->QC_ZSZ_nrm, S->QC_ZSZ, (double)lld, &kl, &tl, S->T);
   }
   memset(S->QC_TAT_nrm, 0, sizeof(double) * nqc);
-  lld = (long long)(Ar) * (long long)(cpop - 1) * (long long)(Ar - 1);
+  lld = dcc ? (long long)(Ar) * (long long)(Ar - 1) : (long long)(Ar) * (long long)(cpop - 1) * (long long)(Ar - 1);
   dra_plusdiv(S->QC_TAT_nrm, S->QC_TAT, (double)lld, &kl, &tl, S->T);
   /* ifT: with the reference's stray ';' the scale factor is always kappa_squared */
   int n_mds = MAXI(0, tl.n - 1); if (n_mds > 6) n_mds = 6;
@@ -1005,7 +1009,7 @@ static void st_nrm_ifT_cmb(lko_state *S)
     double den = n_mds * S->c.kappa2;
     if (S->has_Z) mds_pow(S->QR_ZSZ_nrm, &ar, &tl, S->T, den);
     mds_pow(S->QR_TAT_nrm, &ar, &tl, S->T, den);
-    if (S->has_Z) mds_nrm(S->QC_ZSZ_nrm, &kl, &tl, S->T, den);
+    if (S->has_Z && (!dcc || popmask(S->Zr_b, S->NZ) > 0)) mds_nrm(S->QC_ZSZ_nrm, &kl, &tl, S->T, den);
     mds_nrm(S->QC_TAT_nrm, &kl, &tl, S->T, den);
   }
   /* cmb, first T column only */
@@ -1017,6 +1021,7 @@ static void st_nrm_ifT_cmb(lko_state *S)
   }
   free(ar.ix); free(kl.ix); free(tl.ix);
 }
+static void st_nrm_ifT_cmb(lko_state *S) { st_nrm_ifT_cmb_x(S, 0); }
 
 /* bcc_sumscores_xij (bcc_sumscores.c:225-322), one bin, Ireq<=0 */
 static int st_xij(lko_state *S, int *Irem)
@@ -1084,6 +1089,93 @@ static void st_mxA(lko_state *S, int iter, int Irem, int *rdrop_p, int *cdrop_p)
 }
 
 static double *dalloc(size_t n) { return (double *)xcalloc(n + 1, sizeof(double)); }
+
+/* ------------------------------------------------------------------ */
+/* dexcluster_driver's half-loop subscores (dexcluster_driver.c:22-25, 44-47), one bin,
+   no Y: dcc_lf_D_AtTn_ZtSn_vv (dcc_lf.c:102-163; get_D_AtTn_ZtSn_vv and
+   get_D_AtTn_AtTn_vv, D_AtTn_ZtSn.c:5-310), dcc_lf_TAnZtS_ww (dcc_lf.c:267-288;
+   TAnZtS.c stages 0-2, 5-540) and dcc_wrap_dcc_halfloop (dcc_lf.c:379-530).  Writes
+   QR_TAnAtT / QR_TAnZtS [a*T+t] into QR_TAT / QR_ZSZ and QC_TAnAtT / QC_TAnZtS [k*T+t]
+   into QC_TAT / QC_ZSZ; needs st_ajdk and st_ZtSn of the same masks.                  */
+/* ------------------------------------------------------------------ */
+static void st_dcc(lko_state *S)
+{
+  ilist ar = il_from_mask(S->Ar_j, S->NA), kl = il_from_mask(S->Ac_j, S->M), tl = il_from_mask(S->Tc_j, S->T);
+  ilist zr = il_from_mask(S->Zr_j, S->NZ);
+  const int P = S->pcols, T = S->T; const double *J = S->ajdk;
+  const double *aA = J + 1 * (size_t)P, *DA = J + 5 * (size_t)P;        /* AJDK_1_0, AJDK_0_1 */
+  memset(S->QR_ZSZ, 0, sizeof(double) * (size_t)S->NA * T); memset(S->QR_TAT, 0, sizeof(double) * (size_t)S->NA * T);
+  memset(S->QC_ZSZ, 0, sizeof(double) * (size_t)S->M * T); memset(S->QC_TAT, 0, sizeof(double) * (size_t)S->M * T);
+  /* -cpop_j + 2 popcount of the covariate rows over the active patients */
+  double *etT = dalloc(T), *etS = dalloc(T), *a1T = dalloc(T), *a1S = dalloc(T);
+  for (int t = 0; t < tl.n; t++) {
+    int tt = tl.ix[t];
+    etT[tt] = (double)(-(long long)ar.n + 2 * pop_and(bm_row(&S->Tt, tt), S->Ar_j, S->Tt.rb));
+    if (S->has_Z) etS[tt] = (double)(-(long long)zr.n + 2 * pop_and(bm_row(&S->St, tt), S->Zr_j, S->St.rb));
+  }
+  /* QC: get_D_AtTn_ZtSn_vv / get_D_AtTn_AtTn_vv (one bin: nb1 == nb2) */
+  for (int t = 0; t < tl.n; t++) {
+    int tt = tl.ix[t];
+    for (int k = 0; k < kl.n; k++) {
+      int kk = kl.ix[k]; size_t o = (size_t)kk * T + tt;
+      double a = aA[kk / LKO_POP], D = DA[kk / LKO_POP], At = S->lf_AtTn[o];
+      if (S->has_Z) S->QC_ZSZ[o] = (At - etT[tt] * a) * D * (S->lf_ZtSn[o] - a * etS[tt]);
+      double v = (At - etT[tt] * a) * D * (At - a * etT[tt]);
+      double etA = (double)(-(long long)ar.n + 2 * pop_and(bm_row(&S->At, kk), S->Ar_j, S->At.rb));
+      v -= D * (ar.n - 2 * etA * a + ar.n * a * a);
+      S->QC_TAT[o] = v;
+    }
+  }
+  /* stage 0 (get_dcc_TAnZtS_ww_stage_0): a0d1, a2d1, a1d1_AtTn, a1d1_ZtSn */
+  double a0d1 = mask_lf(S->Ac_j, P, J + 5 * (size_t)P), a2d1 = mask_lf(S->Ac_j, P, J + 7 * (size_t)P);
+  for (int t = 0; t < tl.n; t++) {
+    int tt = tl.ix[t];
+    const uint8_t *Tr = bm_row(&S->Tt, tt);
+    for (int i = 0; i < ar.n; i++) a1T[tt] += bpm(Tr, ar.ix[i]) * S->lf_An_ajdk[(size_t)ar.ix[i] * LKO_AJDK + 6];
+    if (S->has_Z) {
+      const uint8_t *Sr = bm_row(&S->St, tt);
+      for (int i = 0; i < zr.n; i++) a1S[tt] += bpm(Sr, zr.ix[i]) * S->lf_Zn_ajdk[(size_t)zr.ix[i] * LKO_AJDK + 6];
+    }
+  }
+  /* stage 1 (dcc_TAnZtS_ww_stage_1): An_a0d1_ZtSn_excerpt (An_ZtSn_ww.c:243-265), xcalc and
+     An_ZtSn_ww of the a0d1-weighted ZtSn / AtTn */
+  double *L = (double *)xcalloc((size_t)kl.n * tl.n + 1, sizeof(double));
+  double *xZ = dalloc((size_t)S->NA * T), *xA = dalloc((size_t)S->NA * T);
+  lko_planes Pl;
+  for (int pass = 0; pass < 2; pass++) {
+    if (pass == 0 && !S->has_Z) continue;
+    const double *src = pass == 0 ? S->lf_ZtSn : S->lf_AtTn;
+    for (int k = 0; k < kl.n; k++) for (int t = 0; t < tl.n; t++)
+      L[(size_t)k * tl.n + t] = DA[kl.ix[k] / LKO_POP] * src[(size_t)kl.ix[k] * T + tl.ix[t]];
+    memset(&Pl, 0, sizeof Pl);
+    lko_xcalc(L, kl.ix, kl.n, S->M, tl.n, S->c.B, S->c.D_MLT, &Pl, NULL);
+    an_ztsn_ww(&S->An, ar.ix, ar.n, S->Ac_j, &Pl, tl.ix, tl.n, pass == 0 ? xZ : xA, T);
+    S->ops += (long long)ar.n * tl.n * (S->c.B + 1) * (long long)Pl.rb * 8;
+    lko_planes_free(&Pl);
+  }
+  /* stage 2 (get_dcc_TAnZtS_ww_stage_2) and dcc_halfloop */
+  for (int t = 0; t < tl.n; t++) {
+    int tt = tl.ix[t];
+    for (int i = 0; i < ar.n; i++) {
+      int a = ar.ix[i]; size_t o = (size_t)a * T + tt;
+      double x6 = S->lf_An_ajdk[(size_t)a * LKO_AJDK + 6];
+      if (S->has_Z) S->QR_ZSZ[o] += +(xZ[o] - a1S[tt] - x6 * etS[tt] + a2d1 * etS[tt]);
+      S->QR_TAT[o] += +(xA[o] - a1T[tt] - x6 * etT[tt] + a2d1 * etT[tt]);
+    }
+  }
+  for (int t = 0; t < tl.n; t++) {
+    int tt = tl.ix[t];
+    const uint8_t *Tr = bm_row(&S->Tt, tt);
+    for (int i = 0; i < ar.n; i++) {
+      int a = ar.ix[i]; size_t o = (size_t)a * T + tt;
+      double x6 = S->lf_An_ajdk[(size_t)a * LKO_AJDK + 6];
+      double l = bpm(Tr, a) * (a0d1 - 2 * x6 + a2d1);
+      S->QR_TAT[o] += -l;
+    }
+  }
+  free(L); free(xZ); free(xA); free(etT); free(etS); free(a1T); free(a1S);
+  free(ar.ix); free(kl.ix); free(tl.ix); free(zr.ix);
+}
 
 /* low-rank update after a drop: bcc_lrup_QR_YnWt_stage_0/1/3 and
    bcc_lrup_QC_YnWt_stage_a0..a3, b0..b2, c (bcc_lrup.c:5-655, 659-892) */
@@ -1434,9 +1526,13 @@ static void *run_impl(const char *in_text, const char *out_dir, int max_iter, in
   double t0 = now_s();
   if (st_load(S, R)) { st_free(S); free(S); return R; }
   double t1 = now_s();
-  /* initial loop-subscores (lakcluster_driver.c:73-106) */
-  st_ajdk(S); st_ZtSn(S); st_YnWt(S); st_At_T(S); st_QR(S);
-  st_single(S); st_double(S); st_flatten(S);
+  const int dcc = !strcmp(S->c.test_type, "dexcluster_driver");
+  /* initial loop-subscores (lakcluster_driver.c:73-106; dexcluster_driver.c:22-25) */
+  if (dcc) { st_ajdk(S); st_ZtSn(S); st_dcc(S); }
+  else {
+    st_ajdk(S); st_ZtSn(S); st_YnWt(S); st_At_T(S); st_QR(S);
+    st_single(S); st_double(S); st_flatten(S);
+  }
   double t2 = now_s();
   int iter = 0;
   char dir[LKO_PATH];
@@ -1447,9 +1543,17 @@ static void *run_impl(const char *in_text, const char *out_dir, int max_iter, in
   if (write_files) mkdir(dir, 0755);
   while ((popmask(S->Ar_j, S->NA) > 0 || popmask(S->Ac_j, S->M) > 0) && (max_iter < 0 || iter < max_iter)) {
     int Irem = 0, rdrop = 0, cdrop = 0;
-    st_nrm_ifT_cmb(S);
+    st_nrm_ifT_cmb_x(S, dcc);
     st_xij(S, &Irem);
     st_mxA(S, iter, Irem, &rdrop, &cdrop);
+    if (dcc) {
+      /* dcc_lrup_mxdup + dcc_M_mxset, then the half-loop subscores from scratch
+         (dexcluster_driver.c:43-47) */
+      memcpy(S->Ar_j, S->Ar_rtn, lko_bsize(S->NA)); memcpy(S->Ac_j, S->Ac_rtn, lko_bsize(S->M));
+      st_ajdk(S); st_ZtSn(S); st_dcc(S);
+      iter++;
+      continue;
+    }
     if (!recompute) st_lrup(S);
     else {
       /* no low-rank update: drop, then every loop-subscore from scratch on the new masks */
@@ -1657,6 +1761,7 @@ void *lko_state_new(const char *in_text)
 {
   lko_stepper *P = (lko_stepper *)xcalloc(1, sizeof(lko_stepper));
   if (cfg_parse(in_text, &P->S.c) || st_load(&P->S, &P->R)) { st_free(&P->S); free(P); return NULL; }
+  if (!strcmp(P->S.c.test_type, "dexcluster_driver")) { st_ajdk(&P->S); st_ZtSn(&P->S); st_dcc(&P->S); return P; }
   st_ajdk(&P->S); st_ZtSn(&P->S); st_YnWt(&P->S); st_At_T(&P->S); st_QR(&P->S);
   st_single(&P->S); st_double(&P->S); st_flatten(&P->S);
   return P;
@@ -1667,8 +1772,12 @@ int lko_state_iterate(void *h)
   lko_stepper *P = (lko_stepper *)h; lko_state *S = &P->S;
   if (!(popmask(S->Ar_j, S->NA) > 0 || popmask(S->Ac_j, S->M) > 0)) return 1;
   int Irem = 0, rdrop = 0, cdrop = 0;
-  st_nrm_ifT_cmb(S); st_xij(S, &Irem); st_mxA(S, P->iter, Irem, &rdrop, &cdrop);
-  st_lrup(S); st_rescore_tail(S);
+  const int dcc = !strcmp(S->c.test_type, "dexcluster_driver");
+  st_nrm_ifT_cmb_x(S, dcc); st_xij(S, &Irem); st_mxA(S, P->iter, Irem, &rdrop, &cdrop);
+  if (dcc) {
+    memcpy(S->Ar_j, S->Ar_rtn, lko_bsize(S->NA)); memcpy(S->Ac_j, S->Ac_rtn, lko_bsize(S->M));
+    st_ajdk(S); st_ZtSn(S); st_dcc(S);
+  } else { st_lrup(S); st_rescore_tail(S); }
   P->iter++;
   return (popmask(S->Ar_j, S->NA) > 0 || popmask(S->Ac_j, S->M) > 0) ? 0 : 1;
 }
@@ -1695,8 +1804,10 @@ long lko_state_get(void *h, const char *name, double *out, long n)
     { "mask_Ar_j", S->Ar_j, S->NA }, { "mask_Zr_j", S->Zr_j, S->NZ }, { "mask_Ac_j", S->Ac_j, S->M },
   };
   /* the bit matrices as 0/1 values, row-major (after any scramble) */
-  if (!strcmp(name, "An_bits") || !strcmp(name, "At_bits") || !strcmp(name, "Zn_bits")) {
-    const bm_t *B = name[0] == 'Z' ? &S->Zn : (name[1] == 'n' ? &S->An : &S->At);
+  if (!strcmp(name, "An_bits") || !strcmp(name, "At_bits") || !strcmp(name, "Zn_bits") || !strcmp(name, "Tt_bits") ||
+      !strcmp(name, "St_bits")) {
+    const bm_t *B = name[0] == 'Z' ? &S->Zn : name[0] == 'T' ? &S->Tt : name[0] == 'S' ? &S->St
+                  : (name[1] == 'n' ? &S->An : &S->At);
     const long len = (long)B->nr * B->nc;
     if (out) for (long e = 0; e < MINI(len, n); e++) out[e] = bget(bm_row(B, (int)(e / B->nc)), e % B->nc);
     return len;
