@@ -38,8 +38,15 @@ const char *lkc_bcc_error(const lkc_bcc *D);
 /* ---- whole run ------------------------------------------------------------ */
 /* lakcluster_driver (lakcluster_driver.c:14-251) driven by the key/value text of
    an .in file (updateglobals, updateglobals.c:5-199).  Writes out_trace.txt,
-   out_xdrop_a.txt, out_xdrop_b.txt into out_dir (NULL: GLOBAL_DIR_NAME). */
+   out_xdrop_a.txt, out_xdrop_b.txt into out_dir (NULL: GLOBAL_DIR_NAME).  The bcc
+   elimination whatever GLOBAL_TEST_TYPE says, as the reference function. */
 int lkc_lakcluster_driver(const char *in_text, const char *out_dir);
+
+/* dexcluster_driver (dexcluster_driver.c:5-57, GLOBAL_TEST_TYPE= dexcluster_driver at
+   lakcluster_ver18.c:345): the dcc half-loop elimination, recomputing its subscores
+   (dcc_An_ajdk, dcc_lf_ZtSn, dcc_lf_D_AtTn_ZtSn_vv, dcc_lf_TAnZtS_ww,
+   dcc_wrap_dcc_halfloop) after each drop; the same three output files. */
+int lkc_dexcluster_driver(const char *in_text, const char *out_dir);
 
 /* lakcluster_scorebox (lakcluster_scorebox.c:265-271, dispatched on GLOBAL_TEST_TYPE=
    lakcluster_scorebox at lakcluster_ver18.c:338): replays the elimination order of
@@ -84,6 +91,9 @@ int lkc_bcc_get_scorebox(const lkc_bcc *D, int *dims4, int *rpop, int *cpop, int
 int lkc_bcc_write_scorebox(const lkc_bcc *D, const char *out_dir);
 /* 1 when the handle's GLOBAL_TEST_TYPE is lakcluster_scorebox (lakcluster_ver18.c:337-338) */
 int lkc_bcc_is_scorebox(const lkc_bcc *D);
+/* 1 when the handle runs the dcc elimination (GLOBAL_TEST_TYPE= dexcluster_driver): init,
+   iterate and run then follow dexcluster_driver (dexcluster_driver.c:22-53) */
+int lkc_bcc_is_dcc(const lkc_bcc *D);
 
 /* The individual stages, in the order lakcluster_driver calls them.  They act on
    the device-resident state of D (lakcluster_header.h:394-502 names). */

@@ -81,6 +81,7 @@ int lkc_lakcluster_driver(const char* in_text, const char* out_dir)
   lkc_bcc* D = nullptr;
   int rc = lkc_bcc_load(in_text, &D);
   if (rc) return rc;
+  D->run.dcc = 0;
   rc = lkc_bcc_run(D, -1);
   if (!rc) rc = lkc_bcc_write_outputs(D, out_dir);
   lkc_bcc_free(D);
@@ -134,6 +135,20 @@ int lkc_scorebox_xdrop_array(int nxols, int x_max, int x_num, int* xval, int* xd
   lkc::scorebox_xdrop_array(nxols, x_max, x_num, xval, xdrop);
   return LKC_OK;
 }
+
+int lkc_dexcluster_driver(const char* in_text, const char* out_dir)
+{
+  lkc_bcc* D = nullptr;
+  int rc = lkc_bcc_load(in_text, &D);
+  if (rc) return rc;
+  D->run.dcc = 1;
+  rc = lkc_bcc_run(D, -1);
+  if (!rc) rc = lkc_bcc_write_outputs(D, out_dir);
+  lkc_bcc_free(D);
+  return rc;
+}
+
+int lkc_bcc_is_dcc(const lkc_bcc* D) { return D && D->run.dcc ? 1 : 0; }
 
 int lkc_bcc_n_iter(const lkc_bcc* D) { return D ? D->run.n_iter : -1; }
 int lkc_bcc_n_trace(const lkc_bcc* D) { return D ? (int)(D->run.trace.size() / 6) : -1; }

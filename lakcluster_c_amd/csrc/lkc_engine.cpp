@@ -333,7 +333,10 @@ void Run::load(const char* in_text)
   parse_cfg(in_text, c);
   if (c.nbins != 1) throw Error(LKC_ERR_CONFIG, "only GLOBAL_NBINS= 1 is supported");
   if (c.Y_ncols != 0) throw Error(LKC_ERR_CONFIG, "Y/W genetic controls are not supported");
-  if (c.QR.find("YnWt") == std::string::npos || c.QC.find("YnWt") == std::string::npos ||
+  // GLOBAL_TEST_TYPE selects the elimination (lakcluster_ver18.c:337-345); the dcc one has
+  // no low-rank update, so no QR/QC strategy
+  dcc = c.test_type == "dexcluster_driver" ? 1 : 0;
+  if (!dcc && c.QR.find("YnWt") == std::string::npos || c.QC.find("YnWt") == std::string::npos ||
       c.QR.find("condense") != std::string::npos || c.QC.find("store one") != std::string::npos ||
       c.QR.find("ZtSWn") != std::string::npos || c.QC.find("ZtSWn") != std::string::npos)
     throw Error(LKC_ERR_CONFIG, "only the YnWt strategies are supported");
@@ -1025,9 +1028,12 @@ void Run::st_single()
 }
 
 // bcc_doublestudy_ww (bcc_flattenloop.c:977-1029) with An_a2d2_ZtSn_excerpt (An_ZtSn_ww.c:267-290)
-void Run::st_double()
+void Run::st_double() { ztsn_weighted(ajdk + 12 * (size_t)pcols, An_a2d2_ZtSn, An_a2d2_AtTn); }
+
+// An_ZtSn_ww of the block-weighted ZtSn and AtTn (An_a2d2_ZtSn_excerpt / An_a0d1_ZtSn_excerpt,
+// An_ZtSn_ww.c:243-290, then xcalc): out[t*NA + a] over the active rows and columns
+void Run::ztsn_weighted(const double* a2d2, double* outZ, double* outA)
 {
-  const double* a2d2 = ajdk + 12 * (size_t)pcols;
   int nk = kl.n(), nt = tl.n();
   double* L = (double*)g_L.need(dev, sizeof(double) * ((size_t)std::max(nk, 1) * std::max(nt, 1)));
   size_t prb = An.rb;
@@ -1036,7 +1042,7 @@ void Run::st_double()
   for (int pass = 0; pass < 2; pass++) {
     if (pass == 0 && !has_Z) continue;
     const double* src = pass == 0 ? lf_ZtSn : lf_AtTn;
-    double* out = pass == 0 ? An_a2d2_ZtSn : An_a2d2_AtTn;
+    double* out = pass == 0 ? outZ : outA;
     scale_rows(src, M, kl.d, nk, tl.d, nt, a2d2, L, st);
     xcalc(L, 1, nk, nullptr, nullptr, nk, nt, kl.d, c.B, c.D_MLT, cz, M, scal, xwork, st);
     if (ww_kernel == 0) {
@@ -1069,12 +1075,57 @@ void Run::st_flatten()
   flat_qc(g, st);
 }
 
+// ------------------------------------------------------------------
+// dexcluster_driver's half-loop subscores (dexcluster_driver.c:22-25, 44-47), one bin, no
+// Y/W: dcc_lf_D_AtTn_ZtSn_vv (dcc_lf.c:102-163), dcc_lf_TAnZtS_ww stages 0-2 (dcc_lf.c:267-288,
+// TAnZtS.c:5-540) and dcc_wrap_dcc_halfloop (dcc_lf.c:379-530).  The scores land in the bcc
+// score arrays: QR_TAT / QR_ZSZ hold QR_TAnAtT / QR_TAnZtS, QC_TAT / QC_ZSZ hold
+// QC_TAnAtT / QC_TAnZtS.  Runs after st_ajdk and st_ZtSn of the same masks.
+// ------------------------------------------------------------------
+void Run::st_dcc()
+{
+  const int nk = kl.n(), nt = tl.n();
+  // etTn, etSn (-|rows| + 2 pop of the covariate rows over the active patients) into
+  // sc + 4T, sc + 5T; etAn of the active SNP rows into et_An
+  int* ct = cnt_buf + std::max(M, NA);
+  popc_rows(Tt.b(), tl.d, nt, Ar_j, (int)(Tt.rb / 4), ct, st);
+  et_from_counts(ct, tl.d, nt, ar.n(), sc + 4 * T, 1, st);
+  if (has_Z) {
+    popc_rows(St.b(), tl.d, nt, Zr_j, (int)(St.rb / 4), ct, st);
+    et_from_counts(ct, tl.d, nt, zr.n(), sc + 5 * T, 1, st);
+  }
+  popc_rows(At.b(), kl.d, nk, Ar_j, (int)(At.rb / 4), cnt_buf, st);
+  et_from_counts(cnt_buf, kl.d, nk, ar.n(), et_An, 1, st);
+  // stage 0: a1d1_AtTn, a1d1_ZtSn (sc, sc + 2T) as ordered sums over the active rows
+  OrderedJobs J; J.n = 0;
+  for (int t : tl.h) {
+    OrderedJob& q = J.job[J.n++];
+    q.list = ar.d; q.n = ar.n(); q.sign = Tt.p + (size_t)t * Tt.rb; q.vals = lf_An_ajdk + 6 * (size_t)NA; q.out = sc + t;
+    if (has_Z) {
+      OrderedJob& z = J.job[J.n++];
+      z.list = zr.d; z.n = zr.n(); z.sign = St.p + (size_t)t * St.rb; z.vals = lf_Zn_ajdk + 6 * (size_t)NZ; z.out = sc + 2 * T + t;
+    }
+  }
+  ordered_sums(J, st);
+  DccQC q{ kl.d, nk, tl.d, nt, M, lf_AtTn, lf_ZtSn, sc + 4 * T, sc + 5 * T, et_An, ar.n(), ajdk, pcols, has_Z, QC_ZSZ, QC_TAT };
+  dcc_qc(q, st);
+  // stage 1: the a0d1-weighted products (AJDK_0_1) through xcalc and An_ZtSn_ww
+  ztsn_weighted(ajdk + 5 * (size_t)pcols, An_a2d2_ZtSn, An_a2d2_AtTn);
+  // stage 2 and the half-loop copy
+  DccQR r{ ar.d, ar.n(), tl.d, nt, NA, An_a2d2_ZtSn, An_a2d2_AtTn, sc, sc + 2 * T, sc + 4 * T, sc + 5 * T,
+           lf_An_ajdk + 6 * (size_t)NA, mlf, Tt.p, Tt.rb, has_Z, QR_ZSZ, QR_TAT };
+  dcc_qr(r, st);
+}
+
 void Run::init_lf()
 {
   if (inited) throw Error(LKC_ERR_STATE, "already initialised");
   double t0 = now_s();
-  st_ajdk(); st_ZtSn(); st_YnWt(); st_At_T(); st_QR();
-  st_single(); st_double(); st_flatten();
+  if (dcc) { st_ajdk(); st_ZtSn(); st_dcc(); }
+  else {
+    st_ajdk(); st_ZtSn(); st_YnWt(); st_At_T(); st_QR();
+    st_single(); st_double(); st_flatten();
+  }
   HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipGetLastError());
   prof.collect();
@@ -1095,15 +1146,16 @@ void Run::score_lists()
   Scores q;
   q.list = ar.d; q.n = ar.n(); q.tl = tl.d; q.nt = tl.n(); q.ld = NA; q.has_Z = has_Z;
   q.ZSZ = QR_ZSZ; q.TAT = QR_TAT;
-  q.dZ = den((long long)(Zr - 0) * (long long)(cpop) * (long long)(cpop - 1));
-  q.dT = den((long long)(Ar - 1) * (long long)(cpop) * (long long)(cpop - 1));
+  // dcc: the half-loop counts of dcc_sumscores_nrm (dcc_sumscores.c:478-699)
+  q.dZ = den(dcc ? (long long)(Zr - 0) * (long long)(cpop) : (long long)(Zr - 0) * (long long)(cpop) * (long long)(cpop - 1));
+  q.dT = den(dcc ? (long long)(Ar - 1) * (long long)(cpop) : (long long)(Ar - 1) * (long long)(cpop) * (long long)(cpop - 1));
   q.den = dmds; q.pow_form = 1; q.cmb = has_Z; q.out = dsv + M;
   scores(q, st);
   Scores p;
   p.list = kl.d; p.n = kl.n(); p.tl = tl.d; p.nt = tl.n(); p.ld = M; p.has_Z = has_Z;
   p.ZSZ = QC_ZSZ; p.TAT = QC_TAT;
-  p.dZ = den((long long)(Ar) * (long long)(cpop - 1) * (long long)(Zr));
-  p.dT = den((long long)(Ar) * (long long)(cpop - 1) * (long long)(Ar - 1));
+  p.dZ = den(dcc ? (long long)(Ar) * (long long)(Zr) : (long long)(Ar) * (long long)(cpop - 1) * (long long)(Zr));
+  p.dT = den(dcc ? (long long)(Ar) * (long long)(Ar - 1) : (long long)(Ar) * (long long)(cpop - 1) * (long long)(Ar - 1));
   p.den = dmds; p.pow_form = 0; p.cmb = has_Z && Zrb > 0 && Arb > 0; p.out = dsv;
   scores(p, st);
 }
@@ -1604,8 +1656,11 @@ void Run::iterate(int* done)
   if (!active()) { if (done) *done = 1; return; }
   double t0 = now_s();
   sumscores();
-  lrup();
-  st_QR(); st_ajdk(); st_ZtSn(); st_single(); st_double(); st_flatten();
+  if (dcc) { st_ajdk(); st_ZtSn(); st_dcc(); }     // dcc: recomputed, no low-rank update
+  else {
+    lrup();
+    st_QR(); st_ajdk(); st_ZtSn(); st_single(); st_double(); st_flatten();
+  }
   HIPCHK(hipGetLastError());
   n_iter++;
   if (done) *done = active() ? 0 : 1;
@@ -1621,8 +1676,11 @@ void Run::run(int max_iter)
   loop_s = 0;
   while (!done && (max_iter < 0 || n_iter < max_iter)) {
     sumscores();
-    lrup();
-    st_QR(); st_ajdk(); st_ZtSn(); st_single(); st_double(); st_flatten();
+    if (dcc) { st_ajdk(); st_ZtSn(); st_dcc(); }
+    else {
+      lrup();
+      st_QR(); st_ajdk(); st_ZtSn(); st_single(); st_double(); st_flatten();
+    }
     n_iter++;
     done = active() ? 0 : 1;
   }

@@ -148,6 +148,7 @@ struct Run {
   std::vector<double> trace; std::vector<int> xdrop_a, xdrop_b;
   int n_iter = 0; bool inited = false;
   int ww_kernel = 0;                        // contractions: 0 int8 matrix cores, 1 VALU popcount
+  int dcc = 0;                              // the dcc half-loop elimination (dexcluster_driver)
   long long ops = 0;
   double load_s = 0, init_s = 0, loop_s = 0;
   std::string err;
@@ -162,7 +163,8 @@ struct Run {
   void scramble(const std::vector<int>& zcnt, int Zrb);
   void push_ctl();
   void st_ajdk(); void st_ZtSn(); void st_YnWt(); void st_At_T(); void st_QR();
-  void st_single(); void st_double(); void st_flatten();
+  void st_single(); void st_double(); void st_flatten(); void st_dcc();
+  void ztsn_weighted(const double* w, double* outZ, double* outA);
   void ww_excerpt(int trm, const Side& s1, const Side& s2, const IList& kls, const double* src, long long st1, long long st2,
                   int full, double* out, const Packed* pk = nullptr);
   void pack_rows(const int* src, int W, uint32_t* Atc, uint32_t* Ttc);

@@ -54,4 +54,20 @@ void stage_c(const StageC& c, hipStream_t st);
 
 void fill(double* p, long long n, double v, hipStream_t st);
 
+// dcc half-loop subscores of one bin (get_D_AtTn_ZtSn_vv / get_D_AtTn_AtTn_vv,
+// D_AtTn_ZtSn.c:5-310; get_dcc_TAnZtS_ww_stage_2 and get_dcc_halfloop, TAnZtS.c:380-495,
+// dcc_lf.c:379-488).  (row, covariate) arrays are [t*rows + row].
+struct DccQC {
+  const int* kl; int nk; const int* tl; int nt; long long ld;
+  const double* AtTn; const double* ZtSn; const double* etT; const double* etS; const double* etA; int rpop;
+  const double* ajdk; int pcols; int has_Z; double* QC_ZSZ; double* QC_TAT;
+};
+void dcc_qc(const DccQC& a, hipStream_t st);
+struct DccQR {
+  const int* ar; int na; const int* tl; int nt; long long ld;
+  const double* xZ; const double* xA; const double* a1T; const double* a1S; const double* etT; const double* etS;
+  const double* x6; const double* mlf; const uint8_t* Tt; size_t rbT; int has_Z; double* QR_ZSZ; double* QR_TAT;
+};
+void dcc_qr(const DccQR& a, hipStream_t st);
+
 }  // namespace lkc

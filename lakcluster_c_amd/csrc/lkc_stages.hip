@@ -291,4 +291,62 @@ void fill(double* p, long long n, double v, hipStream_t st)
   hipLaunchKernelGGL(k_fill, dim3(cdiv(n, 256)), dim3(256), 0, st, p, n, v);
 }
 
+// ------------------------------------------------------------------
+// dcc QC terms, one thread per (k, t): get_D_AtTn_ZtSn_vv and, with Z = A, the
+// get_D_AtTn_AtTn_vv form that removes the a = a' terms (D_AtTn_ZtSn.c:5-310):
+//   QC_TAnZtS = (AtTn - etTn a) D (ZtSn - a etSn)
+//   QC_TAnAtT = (AtTn - etTn a) D (AtTn - a etTn) - D (|rows| - 2 etAn a + |rows| a a)
+// with a, D the column's block values of A_ajdk (AJDK_1_0, AJDK_0_1).
+// ------------------------------------------------------------------
+__global__ void k_dcc_qc(DccQC a)
+{
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long long)a.nk * a.nt) return;
+  const int k = a.kl[i % a.nk], tt = a.tl[i / a.nk];
+  const long long o = (long long)tt * a.ld + k;
+  const double al = a.ajdk[(size_t)1 * a.pcols + k / kPopBits], D = a.ajdk[(size_t)5 * a.pcols + k / kPopBits];
+  const double At = a.AtTn[o], eT = a.etT[tt];
+  if (a.has_Z) a.QC_ZSZ[o] = (At - eT * al) * D * (a.ZtSn[o] - al * a.etS[tt]);
+  double v = (At - eT * al) * D * (At - al * eT);
+  const double r = (double)a.rpop;
+  v -= D * (r - 2 * a.etA[k] * al + r * al * al);
+  a.QC_TAT[o] = v;
+}
+void dcc_qc(const DccQC& a, hipStream_t st)
+{
+  const long long n = (long long)a.nk * a.nt;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_dcc_qc, dim3(cdiv(n, 256)), dim3(256), 0, st, a);
+}
+
+// dcc QR terms, one thread per (a, t) (stage 2 and the half-loop copy):
+//   QR_TAnZtS = 0 + (An_a0d1_ZtSn - a1d1_ZtSn - An_a1d1 etSn + A_a2d1 etSn)
+//   QR_TAnAtT = 0 + (An_a0d1_AtTn - a1d1_AtTn - An_a1d1 etTn + A_a2d1 etTn)
+//               - T(a,t) (A_a0d1 - 2 An_a1d1 + A_a2d1)
+// A_a0d1 = mlf[5], A_a2d1 = mlf[7] (the weighted counts of the active columns).
+__global__ void k_dcc_qr(DccQR a)
+{
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long long)a.na * a.nt) return;
+  const int r = a.ar[i % a.na], tt = a.tl[i / a.na];
+  const long long o = (long long)tt * a.ld + r;
+  const double x6 = a.x6[r], a0d1 = a.mlf[5], a2d1 = a.mlf[7];
+  if (a.has_Z) {
+    double z = 0.0;
+    z += +(a.xZ[o] - a.a1S[tt] - x6 * a.etS[tt] + a2d1 * a.etS[tt]);
+    a.QR_ZSZ[o] = z;
+  }
+  double v = 0.0;
+  v += +(a.xA[o] - a.a1T[tt] - x6 * a.etT[tt] + a2d1 * a.etT[tt]);
+  const double l = pm_of(a.Tt + (size_t)tt * a.rbT, r) * (a0d1 - 2 * x6 + a2d1);
+  v += -l;
+  a.QR_TAT[o] = v;
+}
+void dcc_qr(const DccQR& a, hipStream_t st)
+{
+  const long long n = (long long)a.na * a.nt;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_dcc_qr, dim3(cdiv(n, 256)), dim3(256), 0, st, a);
+}
+
 }  // namespace lkc

@@ -90,6 +90,8 @@ def lib():
         "lkc_bcc_get_scorebox": (C.c_int, [vp, ip, ip, ip, ip, dp, dp]),
         "lkc_bcc_write_scorebox": (C.c_int, [vp, cp]),
         "lkc_bcc_is_scorebox": (C.c_int, [vp]),
+        "lkc_dexcluster_driver": (C.c_int, [cp, cp]),
+        "lkc_bcc_is_dcc": (C.c_int, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -221,6 +223,11 @@ class Bcc:
         return tuple(int(x) for x in s)
 
     @property
+    def is_dcc(self) -> bool:
+        """The handle runs the dcc half-loop elimination (GLOBAL_TEST_TYPE= dexcluster_driver)."""
+        return bool(lib().lkc_bcc_is_dcc(self._h))
+
+    @property
     def is_scorebox(self) -> bool:
         """GLOBAL_TEST_TYPE is lakcluster_scorebox."""
         return bool(lib().lkc_bcc_is_scorebox(self._h))
@@ -280,6 +287,11 @@ def read_mda(path: str) -> np.ndarray:
 def lakcluster_driver(in_text: str, out_dir: Optional[str] = None) -> None:
     """Whole run with the three output files written (lakcluster_driver)."""
     _check(lib().lkc_lakcluster_driver(in_text.encode(), out_dir.encode() if out_dir else None))
+
+
+def dexcluster_driver(in_text: str, out_dir: Optional[str] = None) -> None:
+    """Whole dcc run with the three output files written (dexcluster_driver)."""
+    _check(lib().lkc_dexcluster_driver(in_text.encode(), out_dir.encode() if out_dir else None))
 
 
 def bsize(ncols: int) -> int:

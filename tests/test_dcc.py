@@ -92,3 +92,39 @@ def test_dcc_run_invariants(oracle, datasets, name):
     # a different path from bcc's on the same data
     b = oracle.run_text(open(ds.in_path).read())
     assert not np.array_equal(b.trace[:, 3:5], r.trace[:, 3:5])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["caseonly", "casectrl", "masked", "blocks", "b16_g1", "kappa", "tiny", "wide_z"])
+def test_engine_dcc_matches_oracle(datasets, oracle, name):
+    from lakcluster_c_amd import lkc
+    ds = datasets[name]
+    text = _dcc_text(ds)
+    print(f"dcc {name}", flush=True)
+    ref = oracle.run_text(text)
+    with lkc.Bcc(text) as run:
+        assert run.is_dcc
+        run.run()
+        tr = run.trace()
+        xa, xb = run.xdrop()
+        ops = run.ops
+    assert tr.tobytes() == ref.trace.tobytes()
+    np.testing.assert_array_equal(xa, ref.xdrop_a)
+    np.testing.assert_array_equal(xb, ref.xdrop_b)
+    assert ops == ref.ops
+
+
+@pytest.mark.gpu
+def test_engine_dcc_output_files(tmp_path, datasets, oracle):
+    from lakcluster_c_amd import lkc
+    ds = datasets["casectrl"]
+    text = _dcc_text(ds)
+    a, b = str(tmp_path / "eng"), str(tmp_path / "ora")
+    lkc.dexcluster_driver(text, a)
+    oracle.run_text(text, out_dir=b, write_files=True)
+    for f in ("out_trace.txt", "out_xdrop_a.txt", "out_xdrop_b.txt"):
+        assert open(f"{a}/{f}").read() == open(f"{b}/{f}").read(), f
+    # the entry point runs dcc whatever the .in names; lkc_lakcluster_driver runs bcc
+    c = str(tmp_path / "bcc")
+    lkc.lakcluster_driver(text, c)
+    assert open(f"{c}/out_trace.txt").read() != open(f"{a}/out_trace.txt").read()
