@@ -137,6 +137,32 @@ def batched_legs(n_null=16, k_null=8, n_serve=24, k_serve=8, seed=7):
     }
 
 
+def dcc_leg(text, max_iter=-1):
+    """The dcc half-loop elimination (dexcluster_driver) on the same problem: one untimed
+    run, one timed run; its dominant kernel is the a0d1-weighted An_ZtSn_ww contraction."""
+    from lakcluster_c_amd import lkc
+    dtext = text.replace("GLOBAL_TEST_TYPE= lakcluster_driver;", "GLOBAL_TEST_TYPE= dexcluster_driver;")
+    run = lkc.Bcc(dtext)
+    run.set_profiling(True)
+    run.run(max_iter)
+    run.reset()
+    t0 = time.perf_counter()
+    run.run(max_iter)
+    el = time.perf_counter() - t0
+    launches, ms, work = run.kernel_stats("ztsn")
+    out = {"elim_run_s": el, "passes": run.n_iter, "bit_ops_per_run": run.ops, "value": run.ops / el / 1e12,
+           "unit": "Tbit-op/s"}
+    run.close()
+    # the contraction streams every active packed row once per launch: HBM-bound
+    ach = work / (ms / 1e3) / 1e9 if ms > 0 else None
+    out["roofline"] = {"kernel": "An_ZtSn_ww contraction", "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS,
+                       "unit": "GB/s", "frac": ach / PEAK_HBM_GBS if ach else None, "launches": launches,
+                       "avg_launch_ms": ms / launches if launches else None}
+    print(f"[bench] dcc: {el:.3f} s, {out['passes']} passes, ztsn {launches} launches {ms:.1f} ms",
+          file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +173,7 @@ def main():
     ap.add_argument("--data-root", default=os.path.join(tempfile.gettempdir(), "lkc_bench_data"))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-null and serving legs")
+    ap.add_argument("--no-dcc", action="store_true", help="skip the dcc half-loop elimination leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 1 GiB buffer three times (the reference kernel for scaling HBM counters)")
     args = ap.parse_args()
@@ -229,6 +256,9 @@ def main():
             "avg_launch_ms": kms / kl if kl else None,
         },
     }
+    if not args.no_dcc:
+        print("[bench] dcc half-loop elimination ...", file=sys.stderr, flush=True)
+        line["dcc"] = dcc_leg(text, args.max_iter)
     if not args.no_batch:
         print("[bench] batched nulls and serving ...", file=sys.stderr, flush=True)
         line.update(batched_legs())

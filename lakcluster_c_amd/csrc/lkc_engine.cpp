@@ -1047,7 +1047,11 @@ void Run::ztsn_weighted(const double* a2d2, double* outZ, double* outA)
     xcalc(L, 1, nk, nullptr, nullptr, nk, nt, kl.d, c.B, c.D_MLT, cz, M, scal, xwork, st);
     if (ww_kernel == 0) {
       size_t wb = ztsn_mfma_work_bytes(nt, prb, c.B);
+      Prof::Pending pz;
+      // timed with its HBM bytes: the active rows (whole packed rows) and the quantised values
+      prof.begin("ztsn", (double)ar.n() * prb + 4.0 * nt * (double)M, st, pz);
       ztsn_mfma(An.b(), ar.d, ar.n(), Ac_j, cz, M, M, nt, tl.d, c.B, scal, out, NA, g_ztsn.need(dev, wb), wb, st);
+      prof.end(pz, st);
     } else {
       cz_planes(cz, M, nt, M, Ac_j, c.B, planes, prb, st);
       an_ztsn(An.b(), ar.d, ar.n(), Ac_j, planes, prb, Ac_j, nt, tl.d, c.B, scal, out, NA, st);
