@@ -128,3 +128,23 @@ def test_engine_dcc_output_files(tmp_path, datasets, oracle):
     c = str(tmp_path / "bcc")
     lkc.lakcluster_driver(text, c)
     assert open(f"{c}/out_trace.txt").read() != open(f"{a}/out_trace.txt").read()
+
+
+@pytest.mark.gpu
+def test_engine_dcc_scramble_matches_oracle(tmp_path, datasets, oracle):
+    """wrap_dcc_scramble and dcc_scramble (R_init.c:90-163) are bcc_scramble's twins: the same load-time redraw."""
+    from lakcluster_c_amd import lkc
+    ds = datasets["casectrl"]
+    r = oracle.run_text(_dcc_text(ds))
+    f1 = str(tmp_path / "tail.txt")
+    with open(f1, "w") as f:
+        for a, b in r.xdrop_a.reshape(-1, 2)[len(r.xdrop_a) // 2:]:
+            f.write(f"{int(a)} {int(b)}\n")
+    text = _dcc_text(ds).replace("END= 0;", "").rstrip()
+    text += f"\nGLOBAL_scramble_num= 1;\nGLOBAL_scramble_out_xdrop_= {f1};\nGLOBAL_scramble_rseed_= 5;\nEND= 0;\n"
+    ref = oracle.run_text(text)
+    assert not np.array_equal(ref.trace, r.trace)
+    with lkc.Bcc(text) as run:
+        run.run()
+        tr = run.trace()
+    assert tr.tobytes() == ref.trace.tobytes()
