@@ -174,6 +174,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-null and serving legs")
     ap.add_argument("--no-dcc", action="store_true", help="skip the dcc half-loop elimination leg")
+    ap.add_argument("--dcc-only", action="store_true", help="run only the dcc leg and print its JSON object")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 1 GiB buffer three times (the reference kernel for scaling HBM counters)")
     args = ap.parse_args()
@@ -199,6 +200,9 @@ def main():
     print(f"[bench] generating {args.config} ...", file=sys.stderr, flush=True)
     in_path = make_data(args.data_root, args.config, kw)
     text = open(in_path).read()
+    if args.dcc_only:
+        print(json.dumps({"workload": args.config, "dcc": dcc_leg(text, args.max_iter)}), flush=True)
+        return
     print("[bench] loading ...", file=sys.stderr, flush=True)
     run = lkc.Bcc(text)
     run.set_profiling(True)
