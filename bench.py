@@ -163,6 +163,80 @@ def dcc_leg(text, max_iter=-1):
     return out
 
 
+def large_leg(rows=65536, cols=524288, passes=20, gamma=0.01, B=24, seed=1, check_valu=False):
+    """The large single run (A and Z rows x cols each, p_j ~ U[0.01, 0.5], biclusters
+    of rows**0.5, rows**0.55 and rows**0.6 rows planted in A, T/S 3 columns, gamma 0.01,
+    B_MLT 24): the initial loop-subscores and the first ``passes`` elimination passes,
+    each timed, inputs drawn on the device and resident in HBM.  With check_valu the
+    same passes are repeated with the popcount (VALU) kernels, whose indexing is
+    separate code, and the traces must agree bit for bit."""
+    import numpy as np
+    import torch
+    from lakcluster_c_amd import gen, lkc
+    t0 = time.time()
+    pool = gen.make_pool_device(rows, rows, cols, T=3, pmin=0.01, pmax=0.5, planted=(0.5, 0.55, 0.6), seed=seed)
+    planted_rows = pool.planted_rows
+    torch.cuda.empty_cache()
+    print(f"[bench] large: pool {rows}+{rows} x {cols} drawn in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    out = {"workload": f"large_{rows}x{cols}", "rows_A": rows, "rows_Z": rows, "cols": cols, "covariates": 3,
+           "gamma": gamma, "B_MLT": B}
+    with lkc.Pool(pool.rows, pool.nrows, pool.ncols, pool.cov, pool.ncov) as P:
+        del pool
+        ra = np.arange(rows, dtype=np.int32); rz = np.arange(rows, 2 * rows, dtype=np.int32)
+        traces = []
+        for kern in (["mfma", "valu"] if check_valu else ["mfma"]):
+            t0 = time.time()
+            run = P.job(ra, rz, None, gamma, B)
+            run.set_profiling(True)
+            if kern != "mfma":
+                run.set_kernels(kern)
+            torch.cuda.synchronize()
+            t_load = time.time() - t0
+            t0 = time.perf_counter()
+            run.init_lf()
+            t_init = time.perf_counter() - t0
+            ops_init = run.ops
+            print(f"[bench] large ({kern}): loaded in {t_load:.1f} s, initial subscores {t_init:.2f} s, "
+                  f"{ops_init:.3e} bit-ops", file=sys.stderr, flush=True)
+            t0 = time.perf_counter(); tl = t0
+            for k in range(passes):
+                if run.iterate():
+                    break
+                if time.perf_counter() - tl > 20 or k + 1 == passes:
+                    tl = time.perf_counter()
+                    print(f"[bench] large ({kern}): {run.n_iter} passes, {tl - t0:.2f} s", file=sys.stderr, flush=True)
+            t_loop = time.perf_counter() - t0
+            tr = run.trace()
+            traces.append(tr)
+            if kern == "mfma":
+                launches, ms, work = run.kernel_stats("ww")
+                ach = work / (ms / 1e3) / 1e12 if ms > 0 else None
+                out.update({
+                    "load_s": t_load, "init_s": t_init, "passes": run.n_iter, "passes_s": t_loop,
+                    "bit_ops_init": ops_init, "bit_ops": run.ops,
+                    "value": run.ops / (t_init + t_loop) / 1e12, "unit": "Tbit-op/s",
+                    "trace_finite": bool(np.isfinite(tr).all()),
+                    "finished": bool(run.n_iter > 0 and tr[-1, 1] <= 2),
+                    "rows_cols_after": [int(tr[-1, 1]), int(tr[-1, 2])] if len(tr) else None,
+                    "roofline": {"kernel": "At_T_YnWt_S_Zn_ww contraction", "bound": "mfma", "achieved": ach,
+                                 "peak": PEAK_INT8_TOPS, "unit": "TOP/s",
+                                 "frac": ach / PEAK_INT8_TOPS if ach else None, "launches": launches,
+                                 "avg_launch_ms": ms / launches if launches else None},
+                })
+            if kern == "mfma" and out["finished"]:
+                # the planted biclusters should be the last A rows eliminated
+                xa, _ = run.xdrop()
+                order = xa[xa[:, 0] >= 0, 0]
+                planted = np.unique(np.concatenate(planted_rows))
+                for m in sorted(len(r) for r in planted_rows):
+                    out.setdefault("planted_recovery", []).append(
+                        {"last_rows": m, "planted_frac": float(np.isin(order[-m:], planted).mean())})
+            run.close()
+        if check_valu:
+            out["valu_trace_identical"] = bool(traces[0].tobytes() == traces[1].tobytes())
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,6 +249,11 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-null and serving legs")
     ap.add_argument("--no-dcc", action="store_true", help="skip the dcc half-loop elimination leg")
     ap.add_argument("--dcc-only", action="store_true", help="run only the dcc leg and print its JSON object")
+    ap.add_argument("--large", type=int, nargs=2, metavar=("ROWS", "COLS"),
+                    help="run only the large single-run leg (e.g. 65536 524288) for --max-iter passes "
+                         "(default: to the end) and print its JSON object")
+    ap.add_argument("--check-valu", action="store_true",
+                    help="with --large: repeat the passes with the popcount kernels and compare traces")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 1 GiB buffer three times (the reference kernel for scaling HBM counters)")
     args = ap.parse_args()
@@ -193,6 +272,11 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.large:
+        passes = args.max_iter if args.max_iter > 0 else 1 << 30
+        print(json.dumps({"large": large_leg(args.large[0], args.large[1], passes, check_valu=args.check_valu)}),
+              flush=True)
+        return
     from lakcluster_c_amd import gen, lkc
     os.makedirs(args.data_root, exist_ok=True)
     kw = dict(gen.CONFIGS[args.config])

@@ -280,15 +280,17 @@ void stage_c(const StageC& c, hipStream_t st)
   hipLaunchKernelGGL(k_stage_c, dim3(cdiv(n, 256)), dim3(256), 0, st, c);
 }
 
+// grid-stride: a dispatch counts its work-items in 32 bits, and the square score
+// arrays reach 2^32 entries (65536 x 65536)
 __global__ void k_fill(double* p, long long n, double v)
 {
-  long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) p[i] = v;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) p[i] = v;
 }
 void fill(double* p, long long n, double v, hipStream_t st)
 {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_fill, dim3(cdiv(n, 256)), dim3(256), 0, st, p, n, v);
+  const unsigned nb = cdiv(n, 256);
+  hipLaunchKernelGGL(k_fill, dim3(nb < 65536 ? nb : 65536), dim3(256), 0, st, p, n, v);
 }
 
 // ------------------------------------------------------------------

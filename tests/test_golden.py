@@ -52,3 +52,55 @@ def test_gpu_reproduces_golden(name):
     assert np.array_equal(xa, g[f"{name}/xdrop_a"])
     assert np.array_equal(xb, g[f"{name}/xdrop_b"])
     assert ops == int(g[f"{name}/ops"][0])
+
+
+# ---------------------------------------------------------------------------
+# The 4096 x 16384 case-only problem at full size (tests/golden/make_c1_golden.py)
+# ---------------------------------------------------------------------------
+
+def _c1():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_c1_golden", os.path.join(HERE, "golden", "make_c1_golden.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.C1, np.load(os.path.join(HERE, "golden", "oracle_c1_4096x16384.npz"))
+
+
+def test_c1_fixture_follows_reference_run(oracle):
+    """The oracle's full-size run agrees with the reference's own run of this problem
+    (recorded in xdrop_schedules.json: 115 trace lines, last 114 2 2, 20480 xdrop
+    lines), every count follows get_xdrop, and every row and column leaves once."""
+    import json
+    kw, g = _c1()
+    ref = json.load(open(os.path.join(HERE, "golden", "xdrop_schedules.json")))["c1_4096x16384"]
+    tr = g["trace"]
+    assert tr.shape[0] == ref["n_trace"]
+    assert [int(x) for x in tr[-1, :3]] == ref["last"]
+    assert g["xdrop_a"].shape[0] == ref["xdrop_lines"]
+    assert np.isfinite(tr).all()
+    for i in range(tr.shape[0] - 1):
+        rd, cd = oracle.get_xdrop(tr[i, 1], tr[i, 2], kw["gamma"])
+        assert tr[i + 1, 1] == tr[i, 1] - min(rd, tr[i, 1]) and tr[i + 1, 2] == tr[i, 2] - min(cd, tr[i, 2])
+    xa = g["xdrop_a"].astype(np.int64)
+    rows, cols = xa[xa[:, 0] >= 0, 0], xa[xa[:, 1] >= 0, 1]
+    assert np.array_equal(np.sort(rows), np.arange(kw["nA"]))
+    assert np.array_equal(np.sort(cols), np.arange(kw["ncols"]))
+
+
+@pytest.mark.gpu
+def test_gpu_c1_full_size_matches_oracle():
+    """The HIP path on the 4096 x 16384 case-only problem: trace (bit patterns), xdrop
+    lists and bit-op count equal the oracle's committed full-size run."""
+    from lakcluster_c_amd import gen, lkc
+    kw, g = _c1()
+    with tempfile.TemporaryDirectory() as root:
+        ds = gen.make_dataset(root, "c1", **kw)
+        with lkc.Bcc(open(ds.in_path).read()) as run:
+            run.run()
+            tr = run.trace()
+            xa, xb = run.xdrop()
+            ops = run.ops
+    assert tr.tobytes() == g["trace"].tobytes()
+    assert np.array_equal(xa, g["xdrop_a"].astype(xa.dtype))
+    assert np.array_equal(xb, g["xdrop_b"].astype(xb.dtype))
+    assert ops == int(g["ops"][0])
