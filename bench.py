@@ -137,6 +137,31 @@ def batched_legs(n_null=16, k_null=8, n_serve=24, k_serve=8, seed=7):
     }
 
 
+def nulls_leg(n_null, k_null, seed=7):
+    """The permutation-null jobs of batched_legs alone: n_null jobs one at a time (the
+    plain GPU path) and, for k_null > 1, k_null at a time, traces compared."""
+    import numpy as np
+    from lakcluster_c_amd import gen, lkc
+    pool = gen.make_pool(16384, 16384, 65536, T=3, seed=1)
+    jobs = gen.null_jobs(pool, n_null, seed=seed, rows=(2048, 16384), cols=(16384, 65536))
+    spec = [(j.rowsA, j.rowsZ, j.colmask, 0.05, 32) for j in jobs]
+    out = {"runs": n_null}
+    with lkc.Pool(pool.rows, pool.nrows, pool.ncols, pool.cov, pool.ncov) as P:
+        P.batch(spec[:1], max_concurrent=1)                      # warm-up
+        t = time.time(); plain = P.batch(spec, max_concurrent=1); tp = time.time() - t
+        print(f"[bench] nulls: {n_null} runs one at a time in {tp:.2f} s", file=sys.stderr, flush=True)
+        out.update({"plain_s": tp, "plain_runs_per_hour": n_null / tp * 3600,
+                    "passes_mean": float(np.mean([r["n_iter"] for r in plain]))})
+        if k_null > 1:
+            t = time.time(); bat = P.batch(spec, max_concurrent=k_null); tb = time.time() - t
+            print(f"[bench] nulls: {n_null} runs {k_null} at a time in {tb:.2f} s", file=sys.stderr, flush=True)
+            out.update({"concurrent": k_null, "batched_s": tb, "batched_runs_per_hour": n_null / tb * 3600,
+                        "ratio": tp / tb,
+                        "identical_to_plain": all(a["status"] == 0 and a["trace"].tobytes() == b["trace"].tobytes()
+                                                  for a, b in zip(plain, bat))})
+    return out
+
+
 def dcc_leg(text, max_iter=-1):
     """The dcc half-loop elimination (dexcluster_driver) on the same problem: one untimed
     run, one timed run; its dominant kernel is the a0d1-weighted An_ZtSn_ww contraction."""
@@ -237,6 +262,58 @@ def large_leg(rows=65536, cols=524288, passes=20, gamma=0.01, B=24, seed=1, chec
     return out
 
 
+def serve_large_leg(n_jobs=24, k=8, seed=11, load=0.7):
+    """Serving at the full job-size range: a device-drawn pool of 65536 cases + 65536
+    controls x 524288 SNPs (p_j ~ U[0.05, 0.5], a planted bicluster in A, T/S 3 columns);
+    jobs draw rows log-uniform in [1024, 65536] and a column subset log-uniform in
+    [8192, 524288] (a column mask), half case-only, gamma ~ U[0.01, 0.1], B 32.  The jobs
+    first run as one closed batch (all present at time 0, k at a time, admitted by the
+    HBM budget) for the capacity in jobs/hour, then again as a Poisson stream at
+    ``load`` times that capacity for p50/p99 latency; every job's trace must be the same
+    in both runs."""
+    import numpy as np
+    import torch
+    from lakcluster_c_amd import gen, lkc
+    t0 = time.time()
+    pool = gen.make_pool_device(65536, 65536, 524288, T=3, pmin=0.05, pmax=0.5, planted=(0.55,), seed=seed)
+    torch.cuda.empty_cache()
+    print(f"[bench] serve-large: pool drawn in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(seed + 1)
+    sjobs = gen.null_jobs(pool, n_jobs, seed=seed + 2, rows=(1024, 65536), cols=(8192, 524288))
+    spec, sizes = [], []
+    for i, j in enumerate(sjobs):
+        g = float(rng.uniform(0.01, 0.1))
+        if i % 2:
+            spec.append((np.sort(np.r_[j.rowsA, j.rowsZ]).astype(np.int32), np.zeros(0, np.int32), j.colmask, g, 32))
+        else:
+            spec.append((j.rowsA, j.rowsZ, j.colmask, g, 32))
+        sizes.append((int(spec[-1][0].size), int(spec[-1][1].size), int(j.ncols_active)))
+    with lkc.Pool(pool.rows, pool.nrows, pool.ncols, pool.cov, pool.ncov) as P:
+        del pool
+        t = time.time(); closed = P.batch(spec, max_concurrent=k); tc = time.time() - t
+        print(f"[bench] serve-large: {n_jobs} jobs as one batch in {tc:.2f} s", file=sys.stderr, flush=True)
+        rate = load * n_jobs / tc
+        arr = np.cumsum(rng.exponential(1.0 / rate, size=n_jobs))
+        arr -= arr[0]
+        t = time.time(); srv = P.batch(spec, max_concurrent=k, arrivals=arr); ts = time.time() - t
+        print(f"[bench] serve-large: {n_jobs} Poisson arrivals in {ts:.2f} s", file=sys.stderr, flush=True)
+    lat = np.array([r["end_s"] - r["arrival_s"] for r in srv])
+    svc = np.array([r["end_s"] - r["start_s"] for r in srv])
+    same = all(a["status"] == 0 and b["status"] == 0 and a["trace"].tobytes() == b["trace"].tobytes()
+               for a, b in zip(closed, srv))
+    return {
+        "jobs": n_jobs, "concurrent": k, "capacity_jobs_per_hour": n_jobs / tc * 3600,
+        "arrival_rate_per_s": rate, "jobs_per_hour": n_jobs / ts * 3600,
+        "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+        "service_p50_s": float(np.percentile(svc, 50)), "service_max_s": float(svc.max()),
+        "identical_batch_and_stream": bool(same),
+        "job_sizes": sizes,
+        "sample": f"{n_jobs} jobs: rows log-U[1024,65536] of a 65536+65536 x 524288 pool, column masks log-U"
+                  "[8192,524288], half case-only, gamma ~ U[0.01,0.1], B 32; Poisson arrivals at "
+                  f"{load:.0%} of the closed-batch capacity",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -252,6 +329,11 @@ def main():
     ap.add_argument("--large", type=int, nargs=2, metavar=("ROWS", "COLS"),
                     help="run only the large single-run leg (e.g. 65536 524288) for --max-iter passes "
                          "(default: to the end) and print its JSON object")
+    ap.add_argument("--serve-large", type=int, metavar="JOBS",
+                    help="run only the serving leg at the full job-size range with JOBS jobs and print its JSON object")
+    ap.add_argument("--nulls", type=int, metavar="RUNS",
+                    help="run only the permutation-null leg with RUNS jobs (plain, then --concurrent at a time)")
+    ap.add_argument("--concurrent", type=int, default=8, help="with --nulls: jobs at a time (1: plain only)")
     ap.add_argument("--check-valu", action="store_true",
                     help="with --large: repeat the passes with the popcount kernels and compare traces")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -272,6 +354,12 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.nulls:
+        print(json.dumps({"batched_nulls": nulls_leg(args.nulls, args.concurrent)}), flush=True)
+        return
+    if args.serve_large:
+        print(json.dumps({"serve_large": serve_large_leg(args.serve_large)}), flush=True)
+        return
     if args.large:
         passes = args.max_iter if args.max_iter > 0 else 1 << 30
         print(json.dumps({"large": large_leg(args.large[0], args.large[1], passes, check_valu=args.check_valu)}),
