@@ -334,6 +334,8 @@ def main():
     ap.add_argument("--nulls", type=int, metavar="RUNS",
                     help="run only the permutation-null leg with RUNS jobs (plain, then --concurrent at a time)")
     ap.add_argument("--concurrent", type=int, default=8, help="with --nulls: jobs at a time (1: plain only)")
+    ap.add_argument("--kernels", choices=["mfma", "valu"], default="mfma",
+                    help="contraction kernels of the main leg: int8 matrix cores (default) or VALU popcount")
     ap.add_argument("--check-valu", action="store_true",
                     help="with --large: repeat the passes with the popcount kernels and compare traces")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -378,6 +380,8 @@ def main():
     print("[bench] loading ...", file=sys.stderr, flush=True)
     run = lkc.Bcc(text)
     run.set_profiling(True)
+    if args.kernels != "mfma":
+        run.set_kernels(args.kernels)
     for w in range(args.warmup):
         t0 = time.perf_counter()
         run.run(args.max_iter)
@@ -417,7 +421,7 @@ def main():
         "vs_baseline": None,
         "dtype": "int8/int32 popcount, f64",
         "data": "synthetic (seeded generator: Bernoulli columns, planted rank-1 bicluster)",
-        "config": {"workload": args.config, "rows_A": shape[0], "rows_Z": shape[1], "cols": shape[2],
+        "config": {"workload": args.config, "kernels": args.kernels, "rows_A": shape[0], "rows_Z": shape[1], "cols": shape[2],
                    "covariates": shape[3], "gamma": kw["gamma"], "B_MLT": kw["B"],
                    "passes": n_iter, "bit_ops_per_run": ops[-1] if ops else None},
         "roofline": {
