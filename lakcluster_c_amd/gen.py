@@ -359,4 +359,5 @@ def make_pool_device(nA: int, nZ: int, ncols: int, T: int = 3, pmin: float = 0.0
     cov = pack_rows(cov_rows.T, bsize(n))
     pool = Pool(rows, cov, n, ncols, T, nA)
     pool.planted_rows = [r for (r, _, _) in plants]    # A rows of each bicluster
+    pool.planted_cols = [c.cpu().numpy() for (_, c, _) in plants]
     return pool
