@@ -388,6 +388,7 @@ def main():
         print(f"[bench] warmup {w}: {time.perf_counter() - t0:.2f} s, {run.n_iter} passes", file=sys.stderr, flush=True)
         run.reset()
     elim, ops, kms, kwork, kl = [], [], 0.0, 0.0, 0
+    qr_l, qr_ms, qr_work = 0, 0.0, 0.0
     n_iter = 0
     for k in range(args.steps):
         t0 = time.perf_counter()
@@ -398,6 +399,8 @@ def main():
         n_iter = run.n_iter
         launches, ms, work = run.kernel_stats("ww")
         kl += launches; kms += ms; kwork += work
+        ql, qms, qwork = run.kernel_stats("qr")
+        qr_l += ql; qr_ms += qms; qr_work += qwork
         print(f"[bench] step {k}: {t1 - t0:.3f} s, {run.n_iter} passes, {run.ops:.3e} bit-ops, "
               f"ww {launches} launches {ms:.1f} ms", file=sys.stderr, flush=True)
         if k + 1 < args.steps:
@@ -436,6 +439,13 @@ def main():
             "avg_launch_ms": kms / kl if kl else None,
         },
     }
+    # the QR row sums (AnZt_S_WnYt_vv) stream the loop-subscores: HBM-bound, 8 bytes per
+    # (row, column) pair read, HIP events around each launch on the engine's stream
+    qach = qr_work / (qr_ms / 1e3) / 1e9 if qr_ms > 0 else None
+    line["qr_roofline"] = {"kernel": "AnZt_S_WnYt_vv row sums", "bound": "hbm", "achieved": qach,
+                           "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": qach / PEAK_HBM_GBS if qach else None,
+                           "bytes_per_launch": qr_work / qr_l if qr_l else None, "launches": qr_l,
+                           "avg_launch_ms": qr_ms / qr_l if qr_l else None}
     if not args.no_dcc:
         print("[bench] dcc half-loop elimination ...", file=sys.stderr, flush=True)
         line["dcc"] = dcc_leg(text, args.max_iter)

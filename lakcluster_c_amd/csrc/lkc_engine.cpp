@@ -969,14 +969,20 @@ void Run::st_At_T()
 }
 
 // bcc_lf_AnZt_S_WnYt (bcc_lf.c:360-409)
+// timed as "qr": work = the bytes of the loop-subscores the row sums read (8 per pair)
 void Run::st_QR()
 {
+  Prof::Pending pq;
   if (has_Z) {
     signed char* sg = (signed char*)g_qrs.need(dev, qr_sign_bytes(zr.n(), tl.n()));
+    prof.begin("qr", 8.0 * ar.n() * (double)zr.n(), st, pq);
     qr(lf_AnZt, NZ, ar.d, ar.n(), zr.d, zr.n(), St.b(), tl.d, tl.n(), lf_AnZtSZnAt, NA, sg, st, 1);
+    prof.end(pq, st);
   }
   signed char* sg2 = (signed char*)g_qrs2.need(dev, qr_sign_bytes(ar.n(), tl.n()));
+  prof.begin("qr", 8.0 * ar.n() * (double)ar.n(), st, pq);
   qr(lf_AnAt, NA, ar.d, ar.n(), ar.d, ar.n(), Tt.b(), tl.d, tl.n(), lf_AnAtTAnAt, NA, sg2, st);
+  prof.end(pq, st);
 }
 
 // bcc_singlestudy_ww (bcc_flattenloop.c:753-831) and get_singlestudy_vv (bcc_flattenloop.c:359-731)
