@@ -77,7 +77,7 @@ def _traffic(workload):
 
 
 def batched_legs(n_null=16, k_null=8, n_serve=24, k_serve=8, seed=7):
-    """Many eliminations on one GPU, from a device-resident pool of the C2 patients
+    """Many eliminations on one GPU, from a device-resident pool of the 16384 x 65536 case-control problem's patients
     (16384 cases + 16384 controls x 65536 SNPs, T/S 3 columns):
     * permutation nulls: row subsamples log-uniform in [2048, 16384] with re-shuffled
       case/control labels, column subsets log-uniform in [16384, 65536], gamma 0.05,
@@ -124,15 +124,15 @@ def batched_legs(n_null=16, k_null=8, n_serve=24, k_serve=8, seed=7):
             "runs": n_null, "plain_runs_per_hour": n_null / tp * 3600, "batched_runs_per_hour": n_null / tb * 3600,
             "ratio": tp / tb, "concurrent": k_null, "identical_to_plain": bool(same),
             "passes_mean": float(np.mean([r["n_iter"] for r in plain])),
-            "sample": f"{n_null} permutation-null jobs of the C4 family (rows log-U[2048,16384] of the pooled "
-                      "16384+16384 C2 patients, labels re-shuffled, columns log-U[16384,65536], gamma 0.05, B 32)",
+            "sample": f"{n_null} permutation-null jobs (rows log-U[2048,16384] of the pooled "
+                      "16384+16384 patients x 65536 SNPs, labels re-shuffled, columns log-U[16384,65536], gamma 0.05, B 32)",
         },
         "serving": {
             "jobs": len(sspec), "concurrent": k_serve, "arrival_rate_per_s": rate, "jobs_per_hour": len(sspec) / tsrv * 3600,
             "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
             "all_ok": bool(ok),
-            "sample": "Poisson arrivals; rows log-U[1024,32768], columns log-U[8192,65536] from the C2 pool, "
-                      "half case-only, gamma ~ U[0.01,0.1], B 32 (C5 sizes capped at the pool)",
+            "sample": "Poisson arrivals; rows log-U[1024,32768], columns log-U[8192,65536] from the 16384+16384 x 65536 "
+                      "pool, half case-only, gamma ~ U[0.01,0.1], B 32 (job sizes capped at the pool)",
         },
     }
 

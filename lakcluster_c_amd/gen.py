@@ -264,7 +264,7 @@ def null_jobs(pool: Pool, n_jobs: int, seed: int, rows: Tuple[int, int] = (2048,
               cols: Tuple[int, int] = (16384, 65536)) -> List[NullJob]:
     """Permutation nulls: each job takes a uniform row subsample of the pooled patients,
     log-uniform in size, whose case/control labels are a fresh shuffle of the pooled
-    labels (SURVEY C4 / dir_m/lakcluster_uADZSZDA_shuffle_1.m), and a uniform column
+    labels (dir_m/lakcluster_uADZSZDA_shuffle_1.m:1-2), and a uniform column
     subset, log-uniform in size, as a column mask."""
     rng = np.random.default_rng(seed)
     out = []

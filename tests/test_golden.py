@@ -55,23 +55,23 @@ def test_gpu_reproduces_golden(name):
 
 
 # ---------------------------------------------------------------------------
-# The 4096 x 16384 case-only problem at full size (tests/golden/make_c1_golden.py)
+# The 4096 x 16384 case-only problem at full size (tests/golden/make_caseonly_golden.py)
 # ---------------------------------------------------------------------------
 
-def _c1():
+def _caseonly_4096():
     import importlib.util
-    spec = importlib.util.spec_from_file_location("make_c1_golden", os.path.join(HERE, "golden", "make_c1_golden.py"))
+    spec = importlib.util.spec_from_file_location("make_caseonly_golden", os.path.join(HERE, "golden", "make_caseonly_golden.py"))
     mod = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(mod)
-    return mod.C1, np.load(os.path.join(HERE, "golden", "oracle_c1_4096x16384.npz"))
+    return mod.CASE_ONLY_4096, np.load(os.path.join(HERE, "golden", "oracle_caseonly_4096x16384.npz"))
 
 
-def test_c1_fixture_follows_reference_run(oracle):
+def test_caseonly_4096_fixture_follows_reference_run(oracle):
     """The oracle's full-size run agrees with the reference's own run of this problem
     (recorded in xdrop_schedules.json: 115 trace lines, last 114 2 2, 20480 xdrop
     lines), every count follows get_xdrop, and every row and column leaves once."""
     import json
-    kw, g = _c1()
+    kw, g = _caseonly_4096()
     ref = json.load(open(os.path.join(HERE, "golden", "xdrop_schedules.json")))["c1_4096x16384"]
     tr = g["trace"]
     assert tr.shape[0] == ref["n_trace"]
@@ -88,13 +88,13 @@ def test_c1_fixture_follows_reference_run(oracle):
 
 
 @pytest.mark.gpu
-def test_gpu_c1_full_size_matches_oracle():
+def test_gpu_caseonly_4096_full_size_matches_oracle():
     """The HIP path on the 4096 x 16384 case-only problem: trace (bit patterns), xdrop
     lists and bit-op count equal the oracle's committed full-size run."""
     from lakcluster_c_amd import gen, lkc
-    kw, g = _c1()
+    kw, g = _caseonly_4096()
     with tempfile.TemporaryDirectory() as root:
-        ds = gen.make_dataset(root, "c1", **kw)
+        ds = gen.make_dataset(root, "caseonly_4096x16384", **kw)
         with lkc.Bcc(open(ds.in_path).read()) as run:
             run.run()
             tr = run.trace()
