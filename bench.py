@@ -188,7 +188,7 @@ def dcc_leg(text, max_iter=-1):
     return out
 
 
-def large_leg(rows=65536, cols=524288, passes=20, gamma=0.01, B=24, seed=1, check_valu=False):
+def large_leg(rows=65536, cols=524288, passes=20, gamma=0.01, B=24, seed=1, check_valu=False, repeat=False):
     """The large single run (A and Z rows x cols each, p_j ~ U[0.01, 0.5], biclusters
     of rows**0.5, rows**0.55 and rows**0.6 rows planted in A, T/S 3 columns, gamma 0.01,
     B_MLT 24): the initial loop-subscores and the first ``passes`` elimination passes,
@@ -209,7 +209,9 @@ def large_leg(rows=65536, cols=524288, passes=20, gamma=0.01, B=24, seed=1, chec
         del pool
         ra = np.arange(rows, dtype=np.int32); rz = np.arange(rows, 2 * rows, dtype=np.int32)
         traces = []
-        for kern in (["mfma", "valu"] if check_valu else ["mfma"]):
+        kerns = ["mfma", "valu"] if check_valu else ["mfma", "mfma"] if repeat else ["mfma"]
+        xdrops = []
+        for rep, kern in enumerate(kerns):
             t0 = time.time()
             run = P.job(ra, rz, None, gamma, B)
             run.set_profiling(True)
@@ -233,7 +235,8 @@ def large_leg(rows=65536, cols=524288, passes=20, gamma=0.01, B=24, seed=1, chec
             t_loop = time.perf_counter() - t0
             tr = run.trace()
             traces.append(tr)
-            if kern == "mfma":
+            xdrops.append(run.xdrop())
+            if kern == "mfma" and rep == 0:
                 launches, ms, work = run.kernel_stats("ww")
                 ach = work / (ms / 1e3) / 1e12 if ms > 0 else None
                 out.update({
@@ -248,7 +251,7 @@ def large_leg(rows=65536, cols=524288, passes=20, gamma=0.01, B=24, seed=1, chec
                                  "frac": ach / PEAK_INT8_TOPS if ach else None, "launches": launches,
                                  "avg_launch_ms": ms / launches if launches else None},
                 })
-            if kern == "mfma" and out["finished"]:
+            if kern == "mfma" and rep == 0 and out["finished"]:
                 # the planted biclusters should be the last A rows eliminated
                 xa, _ = run.xdrop()
                 order = xa[xa[:, 0] >= 0, 0]
@@ -259,6 +262,11 @@ def large_leg(rows=65536, cols=524288, passes=20, gamma=0.01, B=24, seed=1, chec
             run.close()
         if check_valu:
             out["valu_trace_identical"] = bool(traces[0].tobytes() == traces[1].tobytes())
+        if repeat and not check_valu:
+            # a second whole run on a fresh handle: the trace (bit patterns) and the xdrop
+            # lists must repeat exactly
+            out["repeat_identical"] = bool(traces[0].tobytes() == traces[1].tobytes()
+                                           and all(np.array_equal(a, b) for a, b in zip(xdrops[0], xdrops[1])))
     return out
 
 
@@ -336,6 +344,8 @@ def main():
     ap.add_argument("--concurrent", type=int, default=8, help="with --nulls: jobs at a time (1: plain only)")
     ap.add_argument("--kernels", choices=["mfma", "valu"], default="mfma",
                     help="contraction kernels of the main leg: int8 matrix cores (default) or VALU popcount")
+    ap.add_argument("--repeat", action="store_true",
+                    help="with --large: run the elimination a second time on a fresh handle and compare")
     ap.add_argument("--check-valu", action="store_true",
                     help="with --large: repeat the passes with the popcount kernels and compare traces")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -364,7 +374,8 @@ def main():
         return
     if args.large:
         passes = args.max_iter if args.max_iter > 0 else 1 << 30
-        print(json.dumps({"large": large_leg(args.large[0], args.large[1], passes, check_valu=args.check_valu)}),
+        print(json.dumps({"large": large_leg(args.large[0], args.large[1], passes, check_valu=args.check_valu,
+                                               repeat=args.repeat)}),
               flush=True)
         return
     from lakcluster_c_amd import gen, lkc
