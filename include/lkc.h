@@ -118,6 +118,12 @@ int lkc_bcc_n_xdrop(const lkc_bcc *D);
 int lkc_bcc_get_xdrop(const lkc_bcc *D, int *a, int *b);
 /* the reference's bit-operation accounting, summed over all stage calls */
 long long lkc_bcc_ops(const lkc_bcc *D);
+/* Score sorts (of a column or a row list) that ran the reference's quicksort on the host
+   (dQuickSort_xij, Quicksort.c:83-141) instead of taking the device radix sort's order:
+   ties or NaN in the dropped prefix, or ties anywhere in a list longer than the
+   quicksort's recursion limit (GLOBAL_recursion_limit = 32768, lakcluster_ver18.c:68).
+   Counted since load or the last reset. */
+long lkc_bcc_sort_fallbacks(const lkc_bcc *D);
 /* seconds: [load, init, loop] */
 int lkc_bcc_times(const lkc_bcc *D, double *t3);
 /* problem size: [rows of A, rows of Z, columns, covariate columns] */
@@ -156,8 +162,10 @@ int lkc_AnZt_vv(const uint8_t *A, size_t rbA, int na_total, const int *arows, in
                 const uint8_t *Z, size_t rbZ, int nz_total, const int *zrows, int nz,
                 const uint8_t *colmask, int ncols, const double *ajdk,
                 const double *xa, const double *xz, double *out);
-/* get_xcalc (xcalc.c:41-246): L[i*nw + j] for n-row nrows_list[i] and w j.  Returns the
-   quantised integers cz[j*nn + i] and [min, mlt]. */
+/* get_xcalc (xcalc.c:41-246): L[i*nw + j] for n-row nrows_list[i] (in [0, n_total)) and w j.
+   Returns the quantised integers in the reference's n_total-long layout, cz[j*n_total +
+   nrows_list[i]] (0 at positions no list entry names: the planes' clear bits), and
+   [min, mlt]. */
 int lkc_xcalc(const double *L, const int *nrows_list, int nn, int n_total, int nw, int B,
               long long D_MLT, unsigned long long *cz, double *min_mlt);
 /* get_An_ZtSn_ww (An_ZtSn_ww.c:5-155) with its xcalc: L[i*ns + s] holds the (n-row

@@ -234,10 +234,10 @@ size_t attyn_mfma_work_bytes(int ny, int nt)
   return sizeof(long long) * (3 * (size_t)(ny + 1) + 3 * kMaxT + 16) + 256;
 }
 
-void attyn_mfma(const AttynArgs& a, void* work, size_t work_bytes, hipStream_t st)
+hipError_t attyn_mfma(const AttynArgs& a, void* work, size_t work_bytes, hipStream_t st)
 {
-  if (a.nk <= 0 || a.nt <= 0) return;
-  if (work_bytes < attyn_mfma_work_bytes(a.ny, a.nt)) { fprintf(stderr, "lkc: attyn work buffer too small\n"); return; }
+  if (a.nk <= 0 || a.nt <= 0) return hipSuccess;
+  if (work_bytes < attyn_mfma_work_bytes(a.ny, a.nt)) return hipErrorInvalidValue;
   long long* cy = (long long*)work;
   long long* hhay = cy + 3 * (size_t)(a.ny + 1);
   hipLaunchKernelGGL(k_attyn_pre, dim3(a.ny + 1, a.nterm), dim3(64), 0, st, a, cy, hhay);
@@ -250,6 +250,7 @@ void attyn_mfma(const AttynArgs& a, void* work, size_t work_bytes, hipStream_t s
     if (ntg == 3) AT_LAUNCH(3, 2); else if (ntg == 2) AT_LAUNCH(2, 2); else AT_LAUNCH(1, 2);
   }
 #undef AT_LAUNCH
+  return hipGetLastError();
 }
 
 }  // namespace lkc

@@ -169,6 +169,7 @@ int lkc_bcc_get_xdrop(const lkc_bcc* D, int* a, int* b)
   return LKC_OK;
 }
 long long lkc_bcc_ops(const lkc_bcc* D) { return D ? D->run.ops : -1; }
+long lkc_bcc_sort_fallbacks(const lkc_bcc* D) { return D ? D->run.n_sort_fallback : -1; }
 int lkc_bcc_times(const lkc_bcc* D, double* t3)
 {
   if (!D || !t3) return LKC_ERR_STATE;
@@ -328,23 +329,25 @@ int lkc_xcalc(const double* L, const int* nrows_list, int nn, int n_total, int n
               long long D_MLT, unsigned long long* cz, double* min_mlt)
 {
   return guard(nullptr, [&] {
-    if (nn < 0 || nw < 0 || B < 1 || B > 32 || !min_mlt || (nn * (long long)nw && (!L || !cz))) throw lkc::Error(LKC_ERR_STATE, "bad argument");
-    (void)nrows_list; (void)n_total;
+    if (nn < 0 || nw < 0 || B < 1 || B > 32 || !min_mlt || n_total < nn || (nn * (long long)nw && (!L || !cz || !nrows_list)))
+      throw lkc::Error(LKC_ERR_STATE, "bad argument");
+    for (int i = 0; i < nn; i++)
+      if (nrows_list[i] < 0 || nrows_list[i] >= n_total) throw lkc::Error(LKC_ERR_STATE, "nrows_list entry outside [0, n_total)");
     Scoped s;
     double* dL = s.put(L, (size_t)nn * nw);
-    std::vector<int> ident(std::max(nn, 1));
-    for (int i = 0; i < nn; i++) ident[i] = i;
-    int* dn = s.put(ident.data(), ident.size());
-    uint32_t* dcz = s.dev.alloc<uint32_t>((size_t)std::max(nn, 1) * std::max(nw, 1));
+    int* dn = nn ? s.put(nrows_list, (size_t)nn) : nullptr;
+    const size_t ncz = (size_t)std::max(n_total, 1) * std::max(nw, 1);
+    uint32_t* dcz = s.dev.alloc<uint32_t>(ncz);
+    HIPCHK(hipMemsetAsync(dcz, 0, sizeof(uint32_t) * ncz, s.st));
     double* scal = s.dev.alloc<double>(16);
     double* work = s.dev.alloc<double>(1024);
-    lkc::xcalc(dL, nw, 1, nullptr, nullptr, nn, nw, dn, B, D_MLT, dcz, std::max(nn, 1), scal, work, s.st);
-    std::vector<uint32_t> h((size_t)std::max(nn, 1) * std::max(nw, 1));
+    lkc::xcalc(dL, nw, 1, nullptr, nullptr, nn, nw, dn, B, D_MLT, dcz, std::max(n_total, 1), scal, work, s.st);
+    std::vector<uint32_t> h(ncz);
     double hs[16];
     HIPCHK(hipMemcpyAsync(h.data(), dcz, sizeof(uint32_t) * h.size(), hipMemcpyDeviceToHost, s.st));
     HIPCHK(hipMemcpyAsync(hs, scal, sizeof(hs), hipMemcpyDeviceToHost, s.st));
     HIPCHK(hipStreamSynchronize(s.st));
-    for (int j = 0; j < nw; j++) for (int i = 0; i < nn; i++) cz[(size_t)j * nn + i] = h[(size_t)j * std::max(nn, 1) + i];
+    for (int j = 0; j < nw; j++) for (int n = 0; n < n_total; n++) cz[(size_t)j * n_total + n] = h[(size_t)j * std::max(n_total, 1) + n];
     min_mlt[0] = hs[2]; min_mlt[1] = hs[4];
   });
 }

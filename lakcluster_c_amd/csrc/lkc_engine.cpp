@@ -48,6 +48,31 @@ static std::string trim(const std::string& s)
   return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
 }
 
+// Keys that updateglobals reads (updateglobals.c:5-186) but that the bcc / dcc elimination,
+// the scorebox and bcc_scramble never consult: thread and print settings, the self-tests'
+// synthetic sizes (GLOBAL_TEST_*), the row counts the loader takes from the files, the Y/W
+// file names (GLOBAL_Y_n_cols > 0 is what turns those on, and is rejected), the dcg / PCA /
+// PLINK-conversion settings.  Any other key is an error, as the reference's exit() on an
+// unknown name (updateglobals.c:185).
+static bool reference_key_without_effect(const std::string& n)
+{
+  static const char* keys[] = {
+    "GLOBAL_verbose", "GLOBAL_thread_count", "GLOBAL_omp_type", "GLOBAL_LBITS", "GLOBAL_TEST_TYP2",
+    "GLOBAL_TEST_mrand", "GLOBAL_TEST_A_n_rows", "GLOBAL_TEST_A_n_cols", "GLOBAL_TEST_Z_n_rows",
+    "GLOBAL_TEST_Y_n_cols", "GLOBAL_TEST_T_n_cols", "GLOBAL_TEST_niter", "GLOBAL_A_n_rows_", "GLOBAL_Z_n_rows_",
+    "GLOBAL_Y_n_name_", "GLOBAL_Y_t_name_", "GLOBAL_Y_n_cind", "GLOBAL_Y_p_name", "GLOBAL_W_n_name_",
+    "GLOBAL_W_t_name_", "GLOBAL_J_n_rind_", "GLOBAL_K_n_cind", "GLOBAL_pca_out_xdrop", "GLOBAL_pca_V_",
+    "GLOBAL_pca_V_pre_", "GLOBAL_pca_infix", "GLOBAL_pca_iteration_num", "GLOBAL_pca_iteration_max",
+    "GLOBAL_pca_iteration_min", "GLOBAL_pca_rank", "GLOBAL_pca_tolerance", "GLOBAL_fname_bed_0in",
+    "GLOBAL_fname_b16_out", "GLOBAL_fname_bim_0in", "GLOBAL_fname_bim_out", "GLOBAL_fname_fam_0in",
+    "GLOBAL_fname_fam_out", "GLOBAL_fname_flip_flag", "GLOBAL_snp_mss_threshold", "GLOBAL_snp_maf_threshold",
+    "GLOBAL_snp_I_opt_threshold", "GLOBAL_pat_mss_threshold", "GLOBAL_n_fam_char_max", "GLOBAL_n_bim_char_max",
+    "GLOBAL_n_file", "GLOBAL_fname_b16_0in_", "GLOBAL_fname_bim_0in_", "GLOBAL_fname_fam_0in_",
+    "GLOBAL_fname_b16_0in", "GLOBAL_n_bytes_per_read" };
+  for (const char* k : keys) if (n == k) return true;
+  return false;
+}
+
 void parse_cfg(const char* text, Cfg& c)
 {
   c = Cfg();
@@ -102,7 +127,7 @@ void parse_cfg(const char* text, Cfg& c)
     else if (name == "GLOBAL_scorebox_row_num") c.sb_row_num = atoi(val.c_str());
     else if (name == "GLOBAL_scorebox_col_max") c.sb_col_max = atoi(val.c_str());
     else if (name == "GLOBAL_scorebox_col_num") c.sb_col_num = atoi(val.c_str());
-    else if (name.compare(0, 7, "GLOBAL_") == 0) { /* accepted; no effect on this path */ }
+    else if (reference_key_without_effect(name)) { /* a key of updateglobals that this path does not read */ }
     else throw Error(LKC_ERR_CONFIG, "unknown key " + name);
   }
   // the scramble lists are sized by GLOBAL_scramble_num (updateglobals.c:122-135)
@@ -336,10 +361,12 @@ void Run::load(const char* in_text)
   // GLOBAL_TEST_TYPE selects the elimination (lakcluster_ver18.c:337-345); the dcc one has
   // no low-rank update, so no QR/QC strategy
   dcc = c.test_type == "dexcluster_driver" ? 1 : 0;
-  if (!dcc && c.QR.find("YnWt") == std::string::npos || c.QC.find("YnWt") == std::string::npos ||
-      c.QR.find("condense") != std::string::npos || c.QC.find("store one") != std::string::npos ||
-      c.QR.find("ZtSWn") != std::string::npos || c.QC.find("ZtSWn") != std::string::npos)
+  if (!dcc && (c.QR.find("YnWt") == std::string::npos || c.QC.find("YnWt") == std::string::npos ||
+               c.QR.find("condense") != std::string::npos || c.QC.find("store one") != std::string::npos ||
+               c.QR.find("ZtSWn") != std::string::npos || c.QC.find("ZtSWn") != std::string::npos))
     throw Error(LKC_ERR_CONFIG, "only the YnWt strategies are supported");
+  // GLOBAL_Ireq > 0 sorts and drops per bin (bcc_sumscores_srt, bcc_sumscores.c:358-481)
+  if (c.Ireq > 0) throw Error(LKC_ERR_CONFIG, "GLOBAL_Ireq > 0 (per-bin sorting) is not supported");
   for (int i = 0; i < c.scramble_num; i++)
     if (c.scramble_files[i].empty()) throw Error(LKC_ERR_CONFIG, "GLOBAL_scramble_out_xdrop_ names fewer files than GLOBAL_scramble_num");
   if (c.B < 1 || c.B > 32) throw Error(LKC_ERR_CONFIG, "GLOBAL_B_MLT must be in [1,32]");
@@ -939,11 +966,11 @@ void Run::ww_excerpt(int trm, const Side& s1, const Side& s2, const IList& kls, 
   if (ww_kernel == 1) {
     size_t wb = ww_work_bytes(a.nk, a.nt, nw);
     hook.before(&cx, st);
-    ww_valu(a, (double*)g_work.need(dev, wb), wb, st);
+    HIPCHK(ww_valu(a, (double*)g_work.need(dev, wb), wb, st));
     hook.after(&cx, st);
   } else {
     size_t wb = ww_mfma_work_bytes(a.nk, a.nt, nw, a.prb, c.B);
-    ww_mfma(a, g_work.need(dev, wb), wb, st, &hook);
+    HIPCHK(ww_mfma(a, g_work.need(dev, wb), wb, st, &hook));
   }
   ops += (long long)tl.n() * kls.n() * nw * (c.B + 1) * (long long)prb * 8;
 }
@@ -997,7 +1024,7 @@ void Run::st_single()
   ops += (long long)ar.n() * (c.B + 1) * 8;
   if (ww_kernel == 0) {
     size_t wb = ztsn_mfma_work_bytes(1, prb, c.B);
-    ztsn_mfma(At.b(), kl.d, kl.n(), Ar_j, cz, NA, NA, 1, d_zero, c.B, scal, At_An_a1d1, 0, g_ztsn.need(dev, wb), wb, st);
+    HIPCHK(ztsn_mfma(At.b(), kl.d, kl.n(), Ar_j, cz, NA, NA, 1, d_zero, c.B, scal, At_An_a1d1, 0, g_ztsn.need(dev, wb), wb, st));
   } else {
     cz_planes(cz, NA, 1, NA, Ar_j, c.B, planes, prb, st);
     an_ztsn(At.b(), kl.d, kl.n(), Ar_j, planes, prb, Ar_j, 1, d_zero, c.B, scal, At_An_a1d1, 0, st);
@@ -1056,7 +1083,7 @@ void Run::ztsn_weighted(const double* a2d2, double* outZ, double* outA)
       Prof::Pending pz;
       // timed with its HBM bytes: the active rows (whole packed rows) and the quantised values
       prof.begin("ztsn", (double)ar.n() * prb + 4.0 * nt * (double)M, st, pz);
-      ztsn_mfma(An.b(), ar.d, ar.n(), Ac_j, cz, M, M, nt, tl.d, c.B, scal, out, NA, g_ztsn.need(dev, wb), wb, st);
+      HIPCHK(ztsn_mfma(An.b(), ar.d, ar.n(), Ac_j, cz, M, M, nt, tl.d, c.B, scal, out, NA, g_ztsn.need(dev, wb), wb, st));
       prof.end(pz, st);
     } else {
       cz_planes(cz, M, nt, M, Ac_j, c.B, planes, prb, st);
@@ -1195,9 +1222,18 @@ void Run::sumscores()
   int Irem = na > 0 ? 1 : 0;
   QC_sv.assign(hsv, hsv + nk); QC_perm.resize(nk);
   QR_sv.assign(hsv + M, hsv + M + na); QR_perm.resize(na);
+  // The device order is the reference quicksort's where the sorted values of the dropped
+  // prefix are strictly increasing and below the rest, no value is NaN (a NaN moves the
+  // quicksort's pivots), and the quicksort sorts the whole list.  It stops recursing at
+  // depth 32768 (GLOBAL_recursion_limit, Quicksort.c:33-40), which a list of at most 32769
+  // entries cannot reach; in a longer one a run of equal scores peels one entry per level,
+  // so there any tie sends the list to the host quicksort too.  (A longer list of distinct
+  // values reaches that depth only when tens of thousands of its entries arrive in sorted
+  // order, which scores in column or row order do not.)
   auto strict = [](const double* v, int n, int k) {
-    for (int i = 0; i < n; i++) if (v[i] != v[i]) return false;   // a NaN anywhere moves quicksort's pivots
-    for (int i = 0; i < k && i + 1 < n; i++) if (!(v[i] < v[i + 1])) return false;
+    for (int i = 0; i < n; i++) if (v[i] != v[i]) return false;
+    const int upto = n > 32769 ? n - 1 : k;
+    for (int i = 0; i < upto && i + 1 < n; i++) if (!(v[i] < v[i + 1])) return false;
     return true;
   };
   auto exact_order = [&](int off, int n, std::vector<double>& sv, std::vector<int>& perm) {
@@ -1468,7 +1504,7 @@ void Run::lrup()
     size_t wb = attyn_mfma_work_bytes(x.ny, x.nt);
     Prof::Pending pa;
     prof.begin("attyn", 2.0 * crt.n() * crm.n() * tl.n() * ((double)nrt + nrm + (has_Z ? nz : 0)), st, pa);
-    attyn_mfma(x, g_attyn.need(dev, wb), wb, st);
+    HIPCHK(attyn_mfma(x, g_attyn.need(dev, wb), wb, st));
     prof.end(pa, st);
     ops += (long long)tl.n() * crt.n() * crm.n() * (rbn * 2 + (has_Z ? (long long)Zt.rb * 8 : 0));
   } else {
@@ -1711,7 +1747,7 @@ void Run::reset()
   push_ctl();
   HIPCHK(hipStreamSynchronize(st));
   trace.clear(); xdrop_a.clear(); xdrop_b.clear();
-  n_iter = 0; inited = false; ops = 0; init_s = 0; loop_s = 0;
+  n_iter = 0; inited = false; ops = 0; init_s = 0; loop_s = 0; n_sort_fallback = 0;
   for (auto& k : prof.stats) { k.launches = 0; k.ms = 0; k.work = 0; }
 }
 

@@ -70,13 +70,13 @@ struct WwArgs {
 };
 // bit-plane VALU formulation (popcount of plane bit-rows)
 size_t ww_work_bytes(int nk, int nt, int nw);
-void ww_valu(const WwArgs& a, double* work, size_t work_bytes, hipStream_t st);
+hipError_t ww_valu(const WwArgs& a, double* work, size_t work_bytes, hipStream_t st);
 
 // int8 matrix-core formulation (cz split into signed bytes); planes are not used
 size_t ww_mfma_work_bytes(int nk, int nt, int nw, size_t prb, int B);
 // hook: called on the stream right before and after the matrix-core kernel (timing)
 struct WwHook { void (*before)(void*, hipStream_t); void (*after)(void*, hipStream_t); void* ctx; };
-void ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t st, const WwHook* hook = nullptr);
+hipError_t ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t st, const WwHook* hook = nullptr);
 
 // An_ZtSn_ww: out[slist[s]*ldo + alist[i]] = (base + Σ 2^nb pm0(A_a, plane(s,nb)))/mlt
 void an_ztsn(const Bits& A, const int* alist, int na, const uint32_t* amask,
@@ -87,7 +87,7 @@ void an_ztsn(const Bits& A, const int* alist, int na, const uint32_t* amask,
 // cz[s*ldc + n] for n < n_total, mask = the active n (the planes' row0 and amask, equal
 // at every call site).  Same output as an_ztsn.
 size_t ztsn_mfma_work_bytes(int ns, size_t prb, int B);
-void ztsn_mfma(const Bits& A, const int* alist, int na, const uint32_t* mask, const uint32_t* cz, long long ldc,
+hipError_t ztsn_mfma(const Bits& A, const int* alist, int na, const uint32_t* mask, const uint32_t* cz, long long ldc,
                int n_total, int ns, const int* slist, int B, const double* scal, double* out, long long ldo,
                void* work, size_t work_bytes, hipStream_t st);
 
@@ -128,7 +128,7 @@ struct AttynArgs {
   double* vtszn; double* jttan; double* htrkn; double* hhtrkn; long long ldo;
 };
 size_t attyn_mfma_work_bytes(int ny, int nt);
-void attyn_mfma(const AttynArgs& a, void* work, size_t work_bytes, hipStream_t st);
+hipError_t attyn_mfma(const AttynArgs& a, void* work, size_t work_bytes, hipStream_t st);
 
 // Centred Grams on the int8 matrix cores, over packed bit rows.  A column layout is
 // either the full one (seg == NULL: segment q = 1920-block q, words 60q .. 60q+59,

@@ -444,9 +444,9 @@ size_t ww_work_bytes(int nk, int nt, int nw)
 {
   return sizeof(double) * ((size_t)nt * nw + nt + 2 * (size_t)nt * nk + (size_t)64 * (nk > 0 ? nk : 1));
 }
-void ww_valu(const WwArgs& a, double* work, size_t work_bytes, hipStream_t st)
+hipError_t ww_valu(const WwArgs& a, double* work, size_t work_bytes, hipStream_t st)
 {
-  if (a.nk <= 0 || a.nt <= 0) return;
+  if (a.nk <= 0 || a.nt <= 0) return hipSuccess;
   // work layout: H[nt*nw] bp[nt] bk[nt*nk] acc[nt*nk] term[chunk*nk]
   double* H = work;
   double* bp = H + (size_t)a.nt * a.nw;
@@ -456,7 +456,7 @@ void ww_valu(const WwArgs& a, double* work, size_t work_bytes, hipStream_t st)
   size_t used = (size_t)(term - work) * sizeof(double);
   long long chunk = (long long)((work_bytes - used) / sizeof(double)) / (a.nk > 0 ? a.nk : 1);
   if (chunk > 65535) chunk = 65535;
-  if (chunk < 1) { fprintf(stderr, "lkc: ww work buffer too small\n"); return; }
+  if (chunk < 1) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_ww_H, dim3(a.nw + 1, a.nt), dim3(64), 0, st, a, H, bp);
   hipLaunchKernelGGL(k_ww_bk, dim3(a.nk, a.nt), dim3(64), 0, st, a, bk);
   hipLaunchKernelGGL(k_fill0, dim3(cdiv((long long)a.nt * a.nk, 256)), dim3(256), 0, st, acc, (long long)a.nt * a.nk);
@@ -468,6 +468,7 @@ void ww_valu(const WwArgs& a, double* work, size_t work_bytes, hipStream_t st)
     }
   }
   hipLaunchKernelGGL(k_ww_store, dim3(cdiv(a.nk, 256), a.nt), dim3(256), 0, st, acc, a.klist, a.tlist, a.nk, a.nt, a.out, a.ldo);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------

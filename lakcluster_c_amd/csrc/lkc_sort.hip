@@ -5,7 +5,7 @@
 // permutation there (dQuickSort_xij, Quicksort.c:83-141).  The host checks that and
 // falls back to the reference's quicksort otherwise (ties or NaN).
 #include "lkc_kernels.h"
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 namespace lkc {
 
@@ -19,8 +19,8 @@ size_t sort_pairs_temp_bytes(int n)
 {
   size_t b = 0;
   if (n <= 0) return 256;
-  if (hipcub::DeviceRadixSort::SortPairs(nullptr, b, (const double*)nullptr, (double*)nullptr,
-                                         (const int*)nullptr, (int*)nullptr, n) != hipSuccess) return 0;
+  if (rocprim::radix_sort_pairs(nullptr, b, (const double*)nullptr, (double*)nullptr, (const int*)nullptr,
+                                (int*)nullptr, (unsigned int)n) != hipSuccess) return 0;
   return b + 256;
 }
 
@@ -30,7 +30,7 @@ hipError_t sort_pairs(const double* keys, double* keys_out, int* idx_in, int* id
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, st, idx_in, n);
   size_t b = temp_bytes;
-  return hipcub::DeviceRadixSort::SortPairs(temp, b, keys, keys_out, idx_in, idx_out, n, 0, 64, st);
+  return rocprim::radix_sort_pairs(temp, b, keys, keys_out, (const int*)idx_in, idx_out, (unsigned int)n, 0u, 64u, st);
 }
 
 }  // namespace lkc

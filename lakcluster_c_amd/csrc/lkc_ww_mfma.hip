@@ -289,13 +289,13 @@ size_t ww_mfma_work_bytes(int nk, int nt, int nw, size_t prb, int B)
   return b + 1024;
 }
 
-void ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t st, const WwHook* hook)
+hipError_t ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t st, const WwHook* hook)
 {
-  if (a.nk <= 0 || a.nt <= 0) return;
+  if (a.nk <= 0 || a.nt <= 0) return hipSuccess;
   int ndig = (a.B + 7) / 8;
   int nwp = ((a.nw + 31) / 32) * 32;
   long long ldn = (long long)a.prb * 8;
-  if (work_bytes < ww_mfma_work_bytes(a.nk, a.nt, a.nw, a.prb, a.B)) { fprintf(stderr, "lkc: ww work buffer too small\n"); return; }
+  if (work_bytes < ww_mfma_work_bytes(a.nk, a.nt, a.nw, a.prb, a.B)) return hipErrorInvalidValue;   // the caller sizes work with ww_mfma_work_bytes
   uint8_t* p = (uint8_t*)work;
   int8_t* Dg = (int8_t*)p;
   size_t off = ((size_t)ndig * nwp * ldn + 255) & ~(size_t)255;
@@ -321,6 +321,7 @@ void ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t st, con
 #undef WW_ND
 #undef WW_LAUNCH
   if (hook && hook->after) hook->after(hook->ctx, st);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------
@@ -415,14 +416,14 @@ size_t ztsn_mfma_work_bytes(int ns, size_t prb, int B)
   return (((size_t)ndig * ns * prb * 8 + 255) & ~(size_t)255) + sizeof(long long) * ((size_t)ns * ZT_PARTS + 8) + 256;
 }
 
-void ztsn_mfma(const Bits& A, const int* alist, int na, const uint32_t* mask, const uint32_t* cz, long long ldc,
+hipError_t ztsn_mfma(const Bits& A, const int* alist, int na, const uint32_t* mask, const uint32_t* cz, long long ldc,
                int n_total, int ns, const int* slist, int B, const double* scal, double* out, long long ldo,
                void* work, size_t work_bytes, hipStream_t st)
 {
-  if (na <= 0 || ns <= 0) return;
+  if (na <= 0 || ns <= 0) return hipSuccess;
   const int ndig = (B + 7) / 8;
   const long long ldn = (long long)A.rb * 8;
-  if (ndig * ns > 32 || work_bytes < ztsn_mfma_work_bytes(ns, A.rb, B)) { fprintf(stderr, "lkc: ztsn work buffer too small\n"); return; }
+  if (ndig * ns > 32 || work_bytes < ztsn_mfma_work_bytes(ns, A.rb, B)) return hipErrorInvalidValue;
   int8_t* Dg = (int8_t*)work;
   long long* Gsum = (long long*)((uint8_t*)work + (((size_t)ndig * ns * ldn + 255) & ~(size_t)255));
   const int nt_ = n_total < ldn ? n_total : (int)ldn;
@@ -434,6 +435,7 @@ void ztsn_mfma(const Bits& A, const int* alist, int na, const uint32_t* mask, co
                                         (const uint8_t*)mask, Dg, ldn, ns, Gsum, slist, scal, out, ldo)
   switch (ndig) { case 1: ZT_LAUNCH(1); break; case 2: ZT_LAUNCH(2); break; case 3: ZT_LAUNCH(3); break; default: ZT_LAUNCH(4); break; }
 #undef ZT_LAUNCH
+  return hipGetLastError();
 }
 
 }  // namespace lkc

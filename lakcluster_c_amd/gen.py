@@ -98,13 +98,15 @@ def make_dataset(root: str, name: str, nA: int, ncols: int, nZ: int = 0, T: int 
                  flip: float = 0.1, gamma: float = 0.05, B: int = 32, seed: int = 1,
                  gamma_type: Optional[int] = None, kappa_squared: Optional[float] = None,
                  row_mask_frac: float = 0.0, col_mask_frac: float = 0.0,
-                 write_T_file: Optional[bool] = None) -> Dataset:
+                 write_T_file: Optional[bool] = None, dup_cols: float = 0.0) -> Dataset:
     """Generate A (and optionally Z, T, S) and the .in file that drives the elimination.
 
     ``planted`` lists bicluster exponents e: m = round(nA**e) rows by
     round(ncols**e) columns, planted in A only.  ``row_mask_frac`` /
     ``col_mask_frac`` > 0 write GLOBAL_A_n_rind_/GLOBAL_A_n_cind masks with that
-    fraction of entries switched off.
+    fraction of entries switched off.  ``dup_cols`` > 0 makes that fraction of the
+    columns copies of their left neighbour in the same 1920-column block, in A and in Z
+    (duplicated SNPs: their column scores tie exactly).
     """
     rng = np.random.default_rng(seed)
     d_in = os.path.join(root, f"dir_{name}_in")
@@ -119,11 +121,19 @@ def make_dataset(root: str, name: str, nA: int, ncols: int, nZ: int = 0, T: int 
         r, c = _plant(rng, A, min(m, nA), min(mc, ncols), flip)
         prow.append(r)
         pcol.append(c)
+    dup = np.zeros(0, np.int64)
+    if dup_cols > 0:
+        j = np.arange(1, ncols)
+        dup = j[(j // 1920 == (j - 1) // 1920) & (rng.random(ncols - 1) < dup_cols)]
+    for j in dup:
+        A[:, j] = A[:, j - 1]
     write_b16(os.path.join(d_in, "A_n.b16"), A)
     write_b16(os.path.join(d_in, "A_t.b16"), A.T)
     del A
     if nZ > 0:
         Z = _bernoulli_columns(rng, nZ, p)
+        for j in dup:
+            Z[:, j] = Z[:, j - 1]
         write_b16(os.path.join(d_in, "Z_n.b16"), Z)
         write_b16(os.path.join(d_in, "Z_t.b16"), Z.T)
         del Z

@@ -62,6 +62,7 @@ def lib():
         "lkc_bcc_n_xdrop": (C.c_int, [vp]),
         "lkc_bcc_get_xdrop": (C.c_int, [vp, ip, ip]),
         "lkc_bcc_ops": (C.c_longlong, [vp]),
+        "lkc_bcc_sort_fallbacks": (C.c_long, [vp]),
         "lkc_bcc_times": (C.c_int, [vp, dp]),
         "lkc_bcc_shape": (C.c_int, [vp, ip]),
         "lkc_bcc_free": (None, [vp]),
@@ -186,6 +187,11 @@ class Bcc:
     @property
     def ops(self) -> int:
         return int(lib().lkc_bcc_ops(self._h))
+
+    @property
+    def sort_fallbacks(self) -> int:
+        """Score sorts that took the reference quicksort on the host (ties / NaN)."""
+        return int(lib().lkc_bcc_sort_fallbacks(self._h))
 
     def times(self) -> Tuple[float, float, float]:
         t = np.zeros(3, dtype=np.float64)
@@ -339,17 +345,20 @@ def anzt_vv(A: np.ndarray, arows, Z: np.ndarray, zrows, colmask: np.ndarray, nco
     return out[:arows.size, :zrows.size]
 
 
-def xcalc(L: np.ndarray, B: int, D_MLT: Optional[int] = None) -> Tuple[np.ndarray, float, float]:
-    """get_xcalc quantiser on L (nn x nw); returns cz (nw x nn), min, mlt."""
+def xcalc(L: np.ndarray, B: int, D_MLT: Optional[int] = None, nrows_list=None,
+          n_total: Optional[int] = None) -> Tuple[np.ndarray, float, float]:
+    """get_xcalc quantiser on L (nn x nw), row i of L placed at nrows_list[i] (default i) of an
+    n_total-long layout (default nn); returns cz (nw x n_total), min, mlt."""
     L = np.ascontiguousarray(L, dtype=np.float64)
     nn, nw = L.shape
     D = (1 << B) if D_MLT is None else D_MLT
-    cz = np.zeros((max(nw, 1), max(nn, 1)), dtype=np.uint64)
+    rows = np.arange(nn, dtype=np.int32) if nrows_list is None else np.ascontiguousarray(nrows_list, dtype=np.int32)
+    nt = nn if n_total is None else int(n_total)
+    cz = np.zeros((max(nw, 1), max(nt, 1)), dtype=np.uint64)
     mm = np.zeros(2, dtype=np.float64)
-    rows = np.arange(nn, dtype=np.int32)
-    _check(lib().lkc_xcalc(_ptr(L, C.c_double), _ptr(rows, C.c_int), nn, nn, nw, B, D,
+    _check(lib().lkc_xcalc(_ptr(L, C.c_double), _ptr(rows, C.c_int), nn, nt, nw, B, D,
                            _ptr(cz, C.c_ulonglong), _ptr(mm, C.c_double)))
-    return cz[:nw, :nn], float(mm[0]), float(mm[1])
+    return cz[:nw, :nt], float(mm[0]), float(mm[1])
 
 
 def an_ztsn_ww(A: np.ndarray, alist, amask: np.ndarray, L: np.ndarray, nlist, n_total: int, B: int,

@@ -56,6 +56,38 @@ def test_errors_are_status_codes(tmp_path):
     assert L.lkc_bcc_n_trace(None) == -1
 
 
+def _load_status(text):
+    from lakcluster_c_amd import lkc
+    L = lkc.lib()
+    h = ctypes.c_void_p()
+    rc = L.lkc_bcc_load(text.encode(), ctypes.byref(h))
+    if h:
+        L.lkc_bcc_free(h)
+    return rc, L.lkc_last_error().decode()
+
+
+def test_config_errors_are_rejected_before_any_device_work():
+    """LKC_ERR_CONFIG (-1) for a key updateglobals does not know (the reference exits,
+    updateglobals.c:185), for settings this path does not honour, and for the YnWt check of
+    the bcc elimination; the dcc elimination reads no strategy (dcc.c:1167)."""
+    base = "GLOBAL_NBINS= 1; GLOBAL_QR_strategy= YnWt; GLOBAL_QC_strategy= YnWt; GLOBAL_Y_n_cols= 0; "
+    for text, why in ((base + "GLOBAL_gama= 0.1; END= 0;", "unknown key GLOBAL_gama"),
+                      (base + "GLOBAL_Ireq= 1; END= 0;", "GLOBAL_Ireq"),
+                      ("GLOBAL_NBINS= 1; GLOBAL_QR_strategy= YnWt; GLOBAL_QC_strategy= YnWt store one; END= 0;", "YnWt"),
+                      ("GLOBAL_NBINS= 1; END= 0;", "YnWt"),
+                      (base.replace("GLOBAL_NBINS= 1", "GLOBAL_NBINS= 2") + "END= 0;", "NBINS")):
+        rc, msg = _load_status(text)
+        assert rc == -1 and why in msg, (text, rc, msg)
+    # keys the reference reads that do not touch this path are accepted: the load gets past
+    # the configuration and fails on the missing A file instead
+    rc, msg = _load_status(base + "GLOBAL_verbose= 1; GLOBAL_thread_count= 8; GLOBAL_TEST_niter= 3; END= 0;")
+    assert rc == -1 and "A_n or A_t" in msg
+    # dexcluster_driver with no strategy lines, or with the dcc files' "YnWt store one": past the check
+    for qc in ("", "GLOBAL_QC_strategy= YnWt store one; "):
+        rc, msg = _load_status("GLOBAL_TEST_TYPE= dexcluster_driver; GLOBAL_NBINS= 1; " + qc + "END= 0;")
+        assert rc == -1 and "A_n or A_t" in msg, msg
+
+
 def test_driver_binary_built():
     from lakcluster_c_amd import lkc
     assert os.access(lkc.DRIVER_PATH, os.X_OK)

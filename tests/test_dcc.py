@@ -11,7 +11,10 @@ at the initial pass and after elimination steps:
     QC_TAnAtT[k,t] = D_k sum_{a!=a'} Ah(a,k) T(a,t) Ah(a',k) T(a',t)
 with Ah = A(+-1) - alpha, Zh = Z(+-1) - alpha (alpha, D per 1920-column block); the QR
 terms go through the B_MLT-bit quantisation (relative 1e-8 at 32 bits), the QC terms are
-exact products.  Whole runs drop every row and column once with the get_xdrop counts.
+exact products.  Whole runs: at every step the trace means equal the means of the
+normalised (dcc_sumscores_nrm), covariate-corrected (_ifT) and combined (_cmb) brute-force
+half-loop scores, the dropped rows and columns are the lowest-scoring ones, and every row
+and column is dropped once with the get_xdrop counts.
 The engine is held to the oracle bit for bit.
 
 The reference's committed dcc fixture (dir_test/, from dir_in/dcc_sumscores_error.in)
@@ -20,28 +23,11 @@ outside this path, so it pins only the drop schedule (tests/test_golden.py)."""
 import numpy as np
 import pytest
 
+import loopcount
+
 
 def _dcc_text(ds):
     return open(ds.in_path).read().replace("GLOBAL_TEST_TYPE= lakcluster_driver;", "GLOBAL_TEST_TYPE= dexcluster_driver;")
-
-
-def _halfloop_scores(A, Z, Tm, Sm, tab, ar, zr, kl, nt):
-    b = np.arange(A.shape[1]) // 1920
-    al, D = tab[1][b], tab[5][b]
-    Ah = (np.where(A, 1.0, -1.0) - al)[np.ix_(ar, kl)]
-    D = D[kl]
-    T = np.where(Tm, 1.0, -1.0)[ar][:, :nt]
-    G = (Ah * D) @ Ah.T
-    np.fill_diagonal(G, 0.0)
-    out = {"QR_TAT": G @ T}
-    u = Ah.T @ T                                    # [k, t]
-    out["QC_TAT"] = D[:, None] * (u * u - (Ah ** 2).sum(0)[:, None])
-    if Z is not None:
-        Zh = (np.where(Z, 1.0, -1.0) - al)[np.ix_(zr, kl)]
-        S = np.where(Sm, 1.0, -1.0)[zr][:, :nt]
-        out["QR_ZSZ"] = ((Ah * D) @ Zh.T) @ S
-        out["QC_ZSZ"] = D[:, None] * u * (Zh.T @ S)
-    return out
 
 
 @pytest.mark.parametrize("kw,steps", [
@@ -66,7 +52,7 @@ def test_dcc_scores_match_halfloop_definition(kw, steps, oracle, tmp_path):
     zr = np.flatnonzero(S.get("mask_Zr_j")) if Z is not None else None
     tab = S.get("ajdk").reshape(15, -1)
     nt = kw["T"]
-    bf = _halfloop_scores(A, Z, Tm, Sm, tab, ar, zr, kl, nt)
+    bf = loopcount.halfloop_counts(A, Z, Tm, Sm, tab, ar, zr, kl, nt)
     for name, want in bf.items():
         rows = kl if name.startswith("QC") else ar
         got = S.get(name).reshape(-1, nt)[rows]
@@ -74,6 +60,22 @@ def test_dcc_scores_match_halfloop_definition(kw, steps, oracle, tmp_path):
         tol = 1e-12 if name.startswith("QC") else 1e-8
         np.testing.assert_allclose(got, want, rtol=0, atol=tol * scale, err_msg=name)
     S.close()
+
+
+@pytest.mark.parametrize("kw", [
+    dict(nA=14, ncols=36, nZ=11, T=3, seed=31, gamma=0.1),
+    dict(nA=18, ncols=40, nZ=0, T=1, seed=32, gamma=0.1),
+    dict(nA=16, ncols=30, nZ=12, T=2, seed=33, gamma=0.12, row_mask_frac=0.15, col_mask_frac=0.15),
+    dict(nA=15, ncols=34, nZ=13, T=3, seed=34, gamma=0.1, kappa_squared=0.5),
+    dict(nA=14, ncols=3900, nZ=10, T=3, seed=36, gamma=0.2, B=24),
+])
+def test_dcc_trace_and_drops_match_halfloop_scores(kw, oracle, tmp_path):
+    from lakcluster_c_amd import gen
+    ds = gen.make_dataset(str(tmp_path), "ns", **kw)
+    n, compared = loopcount.check_run(oracle, _dcc_text(ds), str(tmp_path / "dir_ns_in"), kw["nZ"], kw["T"],
+                                      kw.get("kappa_squared", 0.0), dcc=True,
+                                      rtol=1e-6 if kw.get("B", 32) == 32 else 1e-4)
+    assert n >= 5 and compared >= n // 2
 
 
 @pytest.mark.parametrize("name", ["caseonly", "casectrl", "masked", "kappa"])
@@ -112,6 +114,37 @@ def test_engine_dcc_matches_oracle(datasets, oracle, name):
     np.testing.assert_array_equal(xa, ref.xdrop_a)
     np.testing.assert_array_equal(xb, ref.xdrop_b)
     assert ops == ref.ops
+
+
+def _no_strategy(text, qc=None):
+    lines = [l for l in text.splitlines() if "_strategy" not in l]
+    if qc:
+        lines.insert(0, f"GLOBAL_QC_strategy= {qc};")
+    return "\n".join(lines) + "\n"
+
+
+def test_oracle_dcc_reads_no_strategy(datasets, oracle):
+    """dexcluster_driver reads no QR/QC strategy (dcc.c:1167): a .in without the strategy
+    lines, or with the dcc files' "YnWt store one", runs the same elimination."""
+    text = _dcc_text(datasets["casectrl"])
+    ref = oracle.run_text(text)
+    for t in (_no_strategy(text), _no_strategy(text, "YnWt store one")):
+        r = oracle.run_text(t)
+        assert r.trace.tobytes() == ref.trace.tobytes()
+
+
+@pytest.mark.gpu
+def test_engine_dcc_reads_no_strategy(datasets, oracle):
+    from lakcluster_c_amd import lkc
+    text = _no_strategy(_dcc_text(datasets["casectrl"]), "YnWt store one")
+    print("dcc without strategy lines", flush=True)
+    ref = oracle.run_text(text)
+    with lkc.Bcc(text) as run:
+        run.run()
+        tr = run.trace()
+        xa, _ = run.xdrop()
+    assert tr.tobytes() == ref.trace.tobytes()
+    np.testing.assert_array_equal(xa, ref.xdrop_a)
 
 
 @pytest.mark.gpu

@@ -13,6 +13,8 @@ import os
 import numpy as np
 import pytest
 
+import loopcount
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
@@ -220,44 +222,6 @@ def test_run_invariants(oracle, datasets):
         assert np.all(np.isfinite(r.trace[:, 3:5]))
 
 
-def _loop_count_scores(A, Z, Tm, Sm, tab, ar, zr, kl, nt):
-    """The reference's brute-force loop counts (get_QR_AnZtSWnYt_uu, bcc_flattenloop.c:1033-1131;
-    get_QC_AtTYnWtSZn_uu, bcc_flattenloop.c:1226-1328) over the active rows ar, zr and columns kl:
-      QR_TAT[a,t] = sum_{a'!=a} sum_{j!=j'} Ah(a,j)D_j Ah(a',j) T(a',t) Ah(a',j')D_j' Ah(a,j')
-      QR_ZSZ[a,t] = sum_z      sum_{j!=j'} Ah(a,j)D_j Zh(z,j)  S(z,t)  Zh(z,j')D_j' Ah(a,j')
-      QC_TAT[k,t] = D_k sum_{a!=a'} Ah(a,k)T(a,t) Ah(a',k)T(a',t) sum_{y!=k} Ah(a,y)D_y Ah(a',y)
-      QC_ZSZ[k,t] = D_k sum_{a,z}   Ah(a,k)T(a,t) Zh(z,k)S(z,t)   sum_{y!=k} Ah(a,y)D_y Zh(z,y)
-    with Ah = A(+-1) - alpha, Zh = Z(+-1) - alpha (alpha, D per 1920-column block)."""
-    b = np.arange(A.shape[1]) // 1920
-    al, D = tab[1][b], tab[5][b]
-    Ah = (np.where(A, 1.0, -1.0) - al)[np.ix_(ar, kl)]
-    Zh = (np.where(Z, 1.0, -1.0) - al)[np.ix_(zr, kl)] if Z is not None else None
-    D = D[kl]
-    T = np.where(Tm, 1.0, -1.0)[ar][:, :nt]
-    S = np.where(Sm, 1.0, -1.0)[zr][:, :nt] if Z is not None else None
-    G = (Ah * D) @ Ah.T
-    G2 = (Ah * D) ** 2 @ (Ah ** 2).T
-    W = G ** 2 - G2
-    np.fill_diagonal(W, 0.0)
-    out = {"QR_TAT": W @ T}
-    if Z is not None:
-        out["QR_ZSZ"] = (((Ah * D) @ Zh.T) ** 2 - (Ah * D) ** 2 @ (Zh ** 2).T) @ S
-    qct = np.zeros((len(kl), nt)); qcz = np.zeros((len(kl), nt))
-    for i in range(len(kl)):
-        m = np.ones(len(kl), bool); m[i] = False
-        Gk = (Ah[:, m] * D[m]) @ Ah[:, m].T
-        np.fill_diagonal(Gk, 0.0)
-        u = Ah[:, i:i + 1] * T
-        qct[i] = D[i] * np.einsum("at,ab,bt->t", u, Gk, u)
-        if Z is not None:
-            v = Zh[:, i:i + 1] * S
-            qcz[i] = D[i] * np.einsum("at,az,zt->t", u, (Ah[:, m] * D[m]) @ Zh[:, m].T, v)
-    out["QC_TAT"] = qct
-    if Z is not None:
-        out["QC_ZSZ"] = qcz
-    return out
-
-
 @pytest.mark.parametrize("kw,steps", [
     (dict(nA=12, ncols=20, nZ=10, T=3, seed=5, gamma=0.1, planted=()), 0),
     (dict(nA=14, ncols=30, nZ=0, T=1, seed=6, gamma=0.1), 0),
@@ -281,10 +245,56 @@ def test_scores_match_loop_count_definition(kw, steps, oracle, tmp_path):
     zr = np.flatnonzero(S.get("mask_Zr_j")) if Z is not None else None
     tab = S.get("ajdk").reshape(15, -1)
     nt = kw["T"]
-    bf = _loop_count_scores(A, Z, Tm, Sm, tab, ar, zr, kl, nt)
+    bf = loopcount.loop_counts(A, Z, Tm, Sm, tab, ar, zr, kl, nt)
     for name, want in bf.items():
         rows = kl if name.startswith("QC") else ar
         got = S.get(name).reshape(-1, nt)[rows]
         scale = np.abs(want).max()
         np.testing.assert_allclose(got, want, rtol=0, atol=1e-8 * scale, err_msg=name)
     S.close()
+
+
+@pytest.mark.parametrize("kw", [
+    dict(nA=14, ncols=36, nZ=11, T=3, seed=21, gamma=0.1),
+    dict(nA=18, ncols=40, nZ=0, T=1, seed=22, gamma=0.1),
+    dict(nA=16, ncols=30, nZ=12, T=2, seed=23, gamma=0.12, row_mask_frac=0.15, col_mask_frac=0.15),
+    dict(nA=15, ncols=34, nZ=13, T=3, seed=24, gamma=0.1, kappa_squared=0.5),
+    dict(nA=16, ncols=30, nZ=0, T=4, seed=25, gamma=0.1, kappa_squared=0.25),
+    dict(nA=14, ncols=3900, nZ=10, T=3, seed=26, gamma=0.2, B=24),
+])
+def test_trace_and_drops_match_loop_count_scores(kw, oracle, tmp_path):
+    """The whole elimination against the loop-count definition: at every step the trace's
+    mean row/column scores equal the means of the normalised (bcc_sumscores_nrm), covariate-
+    corrected (_ifT) and combined (_cmb) brute-force loop counts, and the rows and columns
+    dropped are the lowest-scoring ones.  The kappa_squared = 0 cases with several covariates
+    pin the reference's no-op ifT (its scale is kappa_squared itself, not the table of
+    GLOBAL_kappa_squared_loop_scale_factor_); the kappa_squared > 0 cases pin the pow (QR) and
+    linear (QC) corrections."""
+    from lakcluster_c_amd import gen
+    ds = gen.make_dataset(str(tmp_path), "ns", **kw)
+    text = open(ds.in_path).read()
+    n, compared = loopcount.check_run(oracle, text, str(tmp_path / "dir_ns_in"), kw["nZ"], kw["T"],
+                                      kw.get("kappa_squared", 0.0), rtol=1e-6 if kw.get("B", 32) == 32 else 1e-4)
+    assert n >= 5 and compared >= n // 2
+
+
+def test_duplicate_columns_tie_exactly(oracle, tmp_path):
+    """Duplicated SNP columns (copies of their left neighbour in the same 1920-column block)
+    get bit-identical column subscores, so their scores tie exactly and the drop order among
+    them is the reference quicksort's tie order; the run still drops every row and column
+    once with the get_xdrop counts."""
+    from lakcluster_c_amd import gen
+    ds = gen.make_dataset(str(tmp_path), "dup", nA=60, ncols=300, nZ=40, T=3, gamma=0.05, seed=41, dup_cols=0.5)
+    text = open(ds.in_path).read()
+    A = gen.read_b16(str(tmp_path / "dir_dup_in" / "A_n.b16"))
+    dup = [j for j in range(1, A.shape[1]) if np.array_equal(A[:, j], A[:, j - 1])]
+    assert len(dup) > 100
+    S = oracle.State(text)
+    for name in ("QC_TAT", "QC_ZSZ"):
+        q = S.get(name).reshape(-1, 3)
+        for j in dup:
+            assert q[j].tobytes() == q[j - 1].tobytes(), (name, j)
+    S.close()
+    r = oracle.run_text(text)
+    cols = r.xdrop_a[r.xdrop_a[:, 1] >= 0, 1]
+    assert sorted(cols.tolist()) == list(range(300))

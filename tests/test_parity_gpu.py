@@ -49,6 +49,29 @@ def test_elimination_matches_oracle(case, kern, datasets, oracle):
     assert len(set(cols.tolist())) == len(cols)
 
 
+@pytest.mark.parametrize("kw", [
+    dict(nA=120, ncols=700, nZ=90, T=3, gamma=0.05, seed=41, dup_cols=0.5),
+    dict(nA=150, ncols=600, nZ=0, T=1, gamma=0.05, seed=42, dup_cols=0.6),
+])
+def test_tied_scores_take_the_reference_quicksort(kw, oracle, tmp_path):
+    """Duplicated SNP columns tie exactly; where ties reach the dropped prefix the engine
+    sorts with the reference's quicksort on the host, and the run stays bit-exact."""
+    from lakcluster_c_amd import gen, lkc
+    ds = gen.make_dataset(str(tmp_path), "dup", **kw)
+    text = _read(ds.in_path)
+    print(f"tied scores: {kw}", flush=True)
+    ref = oracle.run_text(text)
+    with lkc.Bcc(text) as run:
+        run.run()
+        tr = run.trace()
+        xa, xb = run.xdrop()
+        fb = run.sort_fallbacks
+    assert fb > 0
+    assert tr.tobytes() == ref.trace.tobytes()
+    np.testing.assert_array_equal(xa, ref.xdrop_a)
+    np.testing.assert_array_equal(xb, ref.xdrop_b)
+
+
 def test_stepwise_api_matches_whole_run(datasets, oracle):
     """lkc_bcc_init_lf + lkc_bcc_iterate reproduce lkc_bcc_run."""
     from lakcluster_c_amd import lkc
@@ -156,6 +179,23 @@ def test_kernel_xcalc_quantiser(B):
     order = np.argsort(v, kind="stable")
     assert np.all(np.diff(c[order]) >= 0)
     assert np.all(np.abs(mn + c / mlt - v) <= 1.0 / mlt * (1 + 1e-9))
+
+
+@pytest.mark.parametrize("B", [16, 32])
+def test_kernel_xcalc_sparse_rows_matches_oracle(B, oracle):
+    """lkc_xcalc places row i of L at nrows_list[i] of an n_total-long layout, as get_xcalc
+    does (xcalc.c:135-159), with zeros where no list entry points; bit-exact with the oracle."""
+    from lakcluster_c_amd import lkc
+    rng = np.random.default_rng(100 + B)
+    nlist = np.sort(rng.permutation(500)[:300]).astype(np.int32)
+    L = rng.normal(size=(300, 5)) * 3
+    cz, mn, mlt = lkc.xcalc(L, B, nrows_list=nlist, n_total=500)
+    rcz, rmn, rmlt, _ = oracle.xcalc(L, nlist, 500, B)
+    assert cz.shape == (5, 500)
+    np.testing.assert_array_equal(cz[:, nlist], rcz)
+    off = np.ones(500, bool); off[nlist] = False
+    assert not cz[:, off].any()
+    assert (mn, mlt) == (rmn, rmlt)
 
 
 @pytest.mark.parametrize("case", ["caseonly", "casectrl", "wide_z"])
