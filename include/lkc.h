@@ -173,6 +173,20 @@ int lkc_xcalc(const double *L, const int *nrows_list, int nn, int n_total, int n
    are contracted with the planes under amask; out[a*ns + s] for a in alist, 0 elsewhere. */
 int lkc_An_ZtSn_ww(const uint8_t *A, size_t rbA, int nr_total, const int *alist, int na, const uint8_t *amask,
                    const double *L, const int *nlist, int nn, int n_total, int ns, int B, long long D_MLT, double *out);
+/* No counterpart in the reference: lkc_An_ZtSn_ww for callers that cannot allocate device
+   memory (code under HIP graph capture, serving loops that keep their buffers, harnesses that
+   forbid hipMalloc).  Every pointer is device memory: A holds nr_total rows of
+   lkc_bsize(n_total) bytes, amask lkc_bsize(n_total) bytes, alist/nlist na/nn row indices
+   (which must lie in [0, nr_total) / [0, n_total): they are not read on the host), L the
+   nn x ns values, out nr_total*ns doubles, out[a*ns + s] as lkc_An_ZtSn_ww (0 for rows not in
+   alist).  The intermediates live in a fixed 32 MiB device array per GPU, held until the
+   process ends; a call needing more (lkc's estimate: ns*n_total*4 + ns*B*bsize(n_total) +
+   ns*nr_total*8 bytes plus about 10 KB) fails with LKC_ERR_STATE.  The work is enqueued on
+   `stream` (a hipStream_t; NULL: the null stream) and the call returns without waiting.
+   Calls that overlap on one GPU share the array and race; calls on one stream are ordered. */
+int lkc_An_ZtSn_ww_noalloc(const uint8_t *A, int nr_total, const int *alist, int na, const uint8_t *amask,
+                           const double *L, const int *nlist, int nn, int n_total, int ns, int B, long long D_MLT,
+                           double *out, void *stream);
 /* wrap_transpose (binary_read.c:174-269), on the device: Y (ncols rows of rbY >=
    bsize(nrows) bytes) receives the transpose of X (nrows rows of ncols bits), padding
    bits zero.  lkc_bcc_load does the same when a .in names only one of *_n / *_t. */

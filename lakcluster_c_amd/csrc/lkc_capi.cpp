@@ -512,4 +512,22 @@ int lkc_An_ZtSn_ww(const uint8_t* A, size_t rbA, int nr_total, const int* alist,
   });
 }
 
+int lkc_An_ZtSn_ww_noalloc(const uint8_t* A, int nr_total, const int* alist, int na, const uint8_t* amask,
+                           const double* L, const int* nlist, int nn, int n_total, int ns, int B, long long D_MLT,
+                           double* out, void* stream)
+{
+  return guard(nullptr, [&] {
+    if (na < 0 || nn < 0 || ns < 0 || nr_total < 0 || n_total <= 0 || nn > n_total || B < 1 || B > 32 || D_MLT < 2 ||
+        !A || !amask || (ns && nr_total && !out) || (na && !alist) || (nn && (!nlist || !L)))
+      throw lkc::Error(LKC_ERR_STATE, "bad argument");
+    const size_t need = lkc::an_ztsn_noalloc_bytes(nr_total, n_total, ns, B);
+    if (need > lkc::kNoallocScratchBytes)
+      throw lkc::Error(LKC_ERR_STATE, "lkc_An_ZtSn_ww_noalloc: the problem needs " + std::to_string(need) +
+                                      " bytes of scratch, more than its fixed " +
+                                      std::to_string(lkc::kNoallocScratchBytes) + " (use lkc_An_ZtSn_ww)");
+    HIPCHK(lkc::an_ztsn_noalloc(A, nr_total, alist, na, amask, L, nlist, nn, n_total, ns, B, D_MLT, out,
+                                (hipStream_t)stream));
+  });
+}
+
 }  // extern "C"

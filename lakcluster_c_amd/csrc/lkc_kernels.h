@@ -83,6 +83,16 @@ void an_ztsn(const Bits& A, const int* alist, int na, const uint32_t* amask,
              const uint8_t* planes, size_t prb, const uint32_t* row0, int ns, const int* slist, int B,
              const double* scal, double* out, long long ldo, hipStream_t st);
 
+// The same An_ZtSn_ww with its xcalc on caller device buffers, every intermediate in a
+// file-scope device array of kNoallocScratchBytes (lkc_noalloc.hip): no device allocation.
+// A: nr_total rows of bsize(n_total) bytes; out[a*ns + s] (zero for rows not in alist).
+constexpr size_t kNoallocScratchBytes = (size_t)32 << 20;
+size_t an_ztsn_noalloc_bytes(int nr_total, int n_total, int ns, int B);
+hipError_t noalloc_scratch(void** ptr);   // the current device's copy of the array
+hipError_t an_ztsn_noalloc(const uint8_t* A, int nr_total, const int* alist, int na, const uint8_t* amask,
+                           const double* L, const int* nlist, int nn, int n_total, int ns, int B, long long D_MLT,
+                           double* out, hipStream_t st);
+
 // An_ZtSn_ww on the int8 matrix cores from the quantised values themselves (no planes):
 // cz[s*ldc + n] for n < n_total, mask = the active n (the planes' row0 and amask, equal
 // at every call site).  Same output as an_ztsn.

@@ -79,6 +79,8 @@ def lib():
         "lkc_xcalc": (C.c_int, [dp, ip, C.c_int, C.c_int, C.c_int, C.c_int, C.c_longlong,
                                 C.POINTER(C.c_ulonglong), dp]),
         "lkc_transpose_bits": (C.c_int, [u8p, C.c_size_t, C.c_int, C.c_int, u8p, C.c_size_t]),
+        "lkc_An_ZtSn_ww_noalloc": (C.c_int, [vp, C.c_int, vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_int,
+                                             C.c_int, C.c_longlong, vp, vp]),
         "lkc_pool_create": (C.c_int, [u8p, C.c_size_t, C.c_int, C.c_int, u8p, C.c_size_t, C.c_int, C.POINTER(vp)]),
         "lkc_pool_free": (None, [vp]),
         "lkc_bcc_load_job": (C.c_int, [vp, ip, C.c_int, ip, C.c_int, u8p, C.POINTER(JobParams), C.POINTER(vp)]),
