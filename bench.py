@@ -196,17 +196,14 @@ def large_leg(rows=65536, cols=524288, passes=20, gamma=0.01, B=24, seed=1, chec
     same passes are repeated with the popcount (VALU) kernels, whose indexing is
     separate code, and the traces must agree bit for bit."""
     import numpy as np
-    import torch
-    from lakcluster_c_amd import gen, lkc
+    from lakcluster_c_amd import gen
     t0 = time.time()
-    pool = gen.make_pool_device(rows, rows, cols, T=3, pmin=0.01, pmax=0.5, planted=(0.5, 0.55, 0.6), seed=seed)
+    pool = gen.make_pool_random(rows, rows, cols, T=3, pmin=0.01, pmax=0.5, planted=(0.5, 0.55, 0.6), seed=seed)
     planted_rows = pool.planted_rows
-    torch.cuda.empty_cache()
     print(f"[bench] large: pool {rows}+{rows} x {cols} drawn in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     out = {"workload": f"large_{rows}x{cols}", "rows_A": rows, "rows_Z": rows, "cols": cols, "covariates": 3,
            "gamma": gamma, "B_MLT": B}
-    with lkc.Pool(pool.rows, pool.nrows, pool.ncols, pool.cov, pool.ncov) as P:
-        del pool
+    with pool.pool as P:
         ra = np.arange(rows, dtype=np.int32); rz = np.arange(rows, 2 * rows, dtype=np.int32)
         traces = []
         kerns = ["mfma", "valu"] if check_valu else ["mfma", "mfma"] if repeat else ["mfma"]
@@ -217,7 +214,6 @@ def large_leg(rows=65536, cols=524288, passes=20, gamma=0.01, B=24, seed=1, chec
             run.set_profiling(True)
             if kern != "mfma":
                 run.set_kernels(kern)
-            torch.cuda.synchronize()
             t_load = time.time() - t0
             t0 = time.perf_counter()
             run.init_lf()
@@ -280,11 +276,9 @@ def serve_large_leg(n_jobs=24, k=8, seed=11, load=0.7):
     ``load`` times that capacity for p50/p99 latency; every job's trace must be the same
     in both runs."""
     import numpy as np
-    import torch
-    from lakcluster_c_amd import gen, lkc
+    from lakcluster_c_amd import gen
     t0 = time.time()
-    pool = gen.make_pool_device(65536, 65536, 524288, T=3, pmin=0.05, pmax=0.5, planted=(0.55,), seed=seed)
-    torch.cuda.empty_cache()
+    pool = gen.make_pool_random(65536, 65536, 524288, T=3, pmin=0.05, pmax=0.5, planted=(0.55,), seed=seed)
     print(f"[bench] serve-large: pool drawn in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     rng = np.random.default_rng(seed + 1)
     sjobs = gen.null_jobs(pool, n_jobs, seed=seed + 2, rows=(1024, 65536), cols=(8192, 524288))
@@ -296,8 +290,7 @@ def serve_large_leg(n_jobs=24, k=8, seed=11, load=0.7):
         else:
             spec.append((j.rowsA, j.rowsZ, j.colmask, g, 32))
         sizes.append((int(spec[-1][0].size), int(spec[-1][1].size), int(j.ncols_active)))
-    with lkc.Pool(pool.rows, pool.nrows, pool.ncols, pool.cov, pool.ncov) as P:
-        del pool
+    with pool.pool as P:
         t = time.time(); closed = P.batch(spec, max_concurrent=k); tc = time.time() - t
         print(f"[bench] serve-large: {n_jobs} jobs as one batch in {tc:.2f} s", file=sys.stderr, flush=True)
         rate = load * n_jobs / tc

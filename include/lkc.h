@@ -203,6 +203,17 @@ typedef struct lkc_pool lkc_pool;
    columns as rows of nrows bits (stride rbc >= bsize(nrows)) */
 int lkc_pool_create(const uint8_t *rows, size_t rb, int nrows, int ncols,
                     const uint8_t *cov, size_t rbc, int ncov, lkc_pool **out);
+/* A synthetic pool drawn on the device: entry (r, j) of nrows x ncols bits is 1 with
+   probability p_j = pmin + (pmax - pmin) U(seed, ~0, j), drawn as U(seed, r, j) < p_j, where U is
+   a counter-based uniform (the splitmix64 finaliser of seed, row and column; every entry can be
+   recomputed on its own); then block k (plant_m[k] rows listed in plant_rows, plant_mc[k]
+   columns in plant_cols, both consecutive over k) is overwritten with plant_bits (row-major
+   0/1 bytes).  cov as in lkc_pool_create. */
+int lkc_pool_create_random(int nrows, int ncols, double pmin, double pmax, unsigned long long seed, int nplant,
+                           const int *plant_m, const int *plant_mc, const int *plant_rows, const int *plant_cols,
+                           const uint8_t *plant_bits, const uint8_t *cov, size_t rbc, int ncov, lkc_pool **out);
+/* rows r0 .. r0+n-1 of the pool, lkc_bsize(ncols) bytes each, into out */
+int lkc_pool_read(const lkc_pool *P, int r0, int n, uint8_t *out);
 void lkc_pool_free(lkc_pool *P);
 typedef struct lkc_job_params {
   double gamma; int gamma_type; int B_MLT; double kappa_squared; int sparse;

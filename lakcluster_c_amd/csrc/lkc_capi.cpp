@@ -389,6 +389,62 @@ int lkc_pool_create(const uint8_t* rows, size_t rb, int nrows, int ncols, const 
   return LKC_OK;
 }
 
+int lkc_pool_create_random(int nrows, int ncols, double pmin, double pmax, unsigned long long seed, int nplant,
+                           const int* plant_m, const int* plant_mc, const int* plant_rows, const int* plant_cols,
+                           const uint8_t* plant_bits, const uint8_t* cov, size_t rbc, int ncov, lkc_pool** out)
+{
+  if (!out) { g_last_error = "null output pointer"; return LKC_ERR_STATE; }
+  *out = nullptr;
+  lkc_pool* P = new lkc_pool();
+  int rc = guard(nullptr, [&] {
+    if (nrows <= 0 || ncols <= 0 || ncov < 1 || ncov > lkc::kMaxT || !cov || nplant < 0 || !(pmin <= pmax) ||
+        (nplant && (!plant_m || !plant_mc || !plant_rows || !plant_cols || !plant_bits)))
+      throw lkc::Error(LKC_ERR_STATE, "bad argument");
+    const size_t b = lkc::bsize(ncols), bc = lkc::bsize(nrows);
+    if (rbc < bc) throw lkc::Error(LKC_ERR_STATE, "row stride smaller than bsize");
+    size_t nr = 0, nc = 0;
+    for (int k = 0; k < nplant; k++) {
+      if (plant_m[k] < 0 || plant_mc[k] < 0) throw lkc::Error(LKC_ERR_STATE, "bad block size");
+      nr += plant_m[k]; nc += plant_mc[k];
+    }
+    for (size_t i = 0; i < nr; i++) if (plant_rows[i] < 0 || plant_rows[i] >= nrows) throw lkc::Error(LKC_ERR_STATE, "block row out of range");
+    for (size_t i = 0; i < nc; i++) if (plant_cols[i] < 0 || plant_cols[i] >= ncols) throw lkc::Error(LKC_ERR_STATE, "block column out of range");
+    HIPCHK(hipSetDevice(0));
+    lkc::Pool& p = P->pool;
+    p.P = nrows; p.M = ncols; p.T = ncov;
+    p.Pn.nr = nrows; p.Pn.nc = ncols; p.Pn.rb = b; p.Pn.p = p.dev.alloc<uint8_t>((size_t)nrows * b);
+    Scoped s;
+    double* pb = s.dev.alloc<double>((size_t)ncols);
+    lkc::pool_draw(p.Pn.p, b, nrows, ncols, pmin, pmax, seed, pb, s.st);
+    size_t orr = 0, oc = 0, ob = 0;
+    for (int k = 0; k < nplant; k++) {
+      int* dr = s.put(plant_rows + orr, (size_t)plant_m[k]);
+      int* dc = s.put(plant_cols + oc, (size_t)plant_mc[k]);
+      uint8_t* db = s.put(plant_bits + ob, (size_t)plant_m[k] * plant_mc[k]);
+      lkc::pool_plant(p.Pn.p, b, dr, dc, db, plant_m[k], plant_mc[k], s.st);
+      orr += plant_m[k]; oc += plant_mc[k]; ob += (size_t)plant_m[k] * plant_mc[k];
+    }
+    HIPCHK(hipStreamSynchronize(s.st));
+    HIPCHK(hipGetLastError());
+    std::vector<uint8_t> hc((size_t)ncov * bc);
+    for (int t = 0; t < ncov; t++) memcpy(hc.data() + (size_t)t * bc, cov + (size_t)t * rbc, bc);
+    p.Tp.nr = ncov; p.Tp.nc = nrows; p.Tp.rb = bc; p.Tp.p = p.dev.alloc<uint8_t>(hc.size());
+    HIPCHK(hipMemcpy(p.Tp.p, hc.data(), hc.size(), hipMemcpyHostToDevice));
+  });
+  if (rc != LKC_OK) { delete P; return rc; }
+  *out = P;
+  return LKC_OK;
+}
+
+int lkc_pool_read(const lkc_pool* P, int r0, int n, uint8_t* out)
+{
+  return guard(nullptr, [&] {
+    if (!P || !out || r0 < 0 || n < 0 || r0 + (long long)n > P->pool.P) throw lkc::Error(LKC_ERR_STATE, "bad argument");
+    const lkc::DMat& m = P->pool.Pn;
+    if (n) HIPCHK(hipMemcpy(out, m.p + (size_t)r0 * m.rb, (size_t)n * m.rb, hipMemcpyDeviceToHost));
+  });
+}
+
 void lkc_pool_free(lkc_pool* P) { delete P; }
 
 int lkc_bcc_load_job(const lkc_pool* P, const int* rowsA, int nA, const int* rowsZ, int nZ, const uint8_t* colmask,

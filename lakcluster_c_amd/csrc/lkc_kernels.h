@@ -184,6 +184,13 @@ void gather_bits(const Bits& src, const int* idx, int n, uint8_t* dst, size_t rb
 
 // Y (X.nc rows of rbY = bsize(X.nr) bytes, zero padded by the caller) = transpose of X
 void transpose_bits(const Bits& X, uint8_t* Y, size_t rbY, hipStream_t st);
+// synthetic pools: rows (nrows x rb bytes, rb % 4 == 0) with entry (r, j) = 1 with probability
+// p_j = pmin + (pmax - pmin) U(seed, ~0, j), drawn as U(seed, r, j) < p_j (pbuf: ncols doubles);
+// then blocks overwritten with given bits
+void pool_draw(uint8_t* rows, size_t rb, int nrows, int ncols, double pmin, double pmax, unsigned long long seed,
+               double* pbuf, hipStream_t st);
+void pool_plant(uint8_t* rows, size_t rb, const int* prow, const int* pcol, const uint8_t* bits, int m, int mc,
+                hipStream_t st);
 
 // stable ascending radix sort of (keys[i], i): keys_out, idx_out; temp of sort_pairs_temp_bytes(n)
 size_t sort_pairs_temp_bytes(int n);

@@ -837,4 +837,72 @@ void pm_few(const Bits& A, const int* alist, int na, const Bits& T, const int* t
                      mask, nwords >= 0 ? nwords : pcols * kPopWords, base16, out, ldo);
 }
 
+// ------------------------------------------------------------------
+// Synthetic patient pools drawn on the device.  A counter-based generator: the uniform of
+// (seed, row, column) is the splitmix64 finaliser (Steele, Lea, Flood 2014) of a Weyl-style
+// combination of the three, so any entry can be recomputed on its own and a pool does not
+// depend on launch shape.  Row ~0u gives the column frequencies.
+// ------------------------------------------------------------------
+__device__ __host__ inline double pool_uniform(unsigned long long seed, unsigned int row, unsigned int col)
+{
+  unsigned long long x = seed * 0x9E3779B97F4A7C15ull + (unsigned long long)row * 0xD1B54A32D192ED03ull +
+                         ((unsigned long long)col + 1ull) * 0xABC98388FB8FAC03ull;
+  x ^= x >> 30; x *= 0xBF58476D1CE4E5B9ull;
+  x ^= x >> 27; x *= 0x94D049BB133111EBull;
+  x ^= x >> 31;
+  return (double)(x >> 11) * (1.0 / 9007199254740992.0);
+}
+__global__ void k_pool_p(double* p, int ncols, double pmin, double pmax, unsigned long long seed)
+{
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < ncols; j += gridDim.x * blockDim.x)
+    p[j] = pmin + (pmax - pmin) * pool_uniform(seed, 0xFFFFFFFFu, (unsigned)j);
+}
+// one 32-bit word (32 columns, MSB-first bytes) per thread; padding bits stay 0
+__global__ void k_pool_draw(uint32_t* rows, long long words_per_row, int nrows, int ncols, const double* p,
+                            unsigned long long seed)
+{
+  const long long tot = words_per_row * nrows;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
+    const int r = (int)(e / words_per_row);
+    const int w = (int)(e % words_per_row);
+    uint32_t v = 0;
+    for (int q = 0; q < 32; q++) {
+      const int j = w * 32 + q;
+      if (j >= ncols) break;
+      if (pool_uniform(seed, (unsigned)r, (unsigned)j) < p[j]) v |= 1u << (((q >> 3) << 3) + (7 - (q & 7)));
+    }
+    rows[e] = v;
+  }
+}
+// overwrite an m x mc block: bit (prow[i], pcol[c]) = bits[i*mc + c]
+__global__ void k_pool_plant(uint32_t* rows, long long words_per_row, const int* prow, const int* pcol,
+                             const uint8_t* bits, int m, int mc)
+{
+  const long long tot = (long long)m * mc;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(e / mc), c = (int)(e % mc);
+    const int j = pcol[c];
+    uint32_t* wd = rows + (long long)prow[i] * words_per_row + (j >> 5);
+    const uint32_t b = 1u << (((j >> 3) & 3) * 8 + (7 - (j & 7)));
+    if (bits[e]) atomicOr(wd, b); else atomicAnd(wd, ~b);
+  }
+}
+void pool_draw(uint8_t* rows, size_t rb, int nrows, int ncols, double pmin, double pmax, unsigned long long seed,
+               double* pbuf, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_pool_p, dim3(cdiv(ncols, 256) < 4096 ? cdiv(ncols, 256) : 4096), dim3(256), 0, st,
+                     pbuf, ncols, pmin, pmax, seed);
+  const long long tot = (long long)(rb / 4) * nrows;
+  hipLaunchKernelGGL(k_pool_draw, dim3(cdiv(tot, 256) < 65536 ? cdiv(tot, 256) : 65536), dim3(256), 0, st,
+                     (uint32_t*)rows, (long long)(rb / 4), nrows, ncols, pbuf, seed);
+}
+void pool_plant(uint8_t* rows, size_t rb, const int* prow, const int* pcol, const uint8_t* bits, int m, int mc,
+                hipStream_t st)
+{
+  const long long tot = (long long)m * mc;
+  if (tot <= 0) return;
+  hipLaunchKernelGGL(k_pool_plant, dim3(cdiv(tot, 256) < 65536 ? cdiv(tot, 256) : 65536), dim3(256), 0, st,
+                     (uint32_t*)rows, (long long)(rb / 4), prow, pcol, bits, m, mc);
+}
+
 }  // namespace lkc

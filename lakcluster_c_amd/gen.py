@@ -326,48 +326,41 @@ def write_job(root: str, name: str, pool: Pool, job: NullJob, gamma: float, B: i
     return Dataset(in_path, d_out, A.shape[0], Z.shape[0], pool.ncols, pool.ncov, gamma, B)
 
 
-def make_pool_device(nA: int, nZ: int, ncols: int, T: int = 3, pmin: float = 0.01, pmax: float = 0.5,
-                     planted: Sequence[float] = (0.5, 0.55, 0.6), flip: float = 0.1, seed: int = 1,
-                     chunk: int = 1024, device: str = "cuda:0") -> Pool:
-    """make_pool's column model drawn on the device, for problems too large for the
-    host generator (65536 + 65536 patients x 524288 SNPs is 6.9e10 draws): column
-    frequencies p_j ~ U[pmin, pmax], Bernoulli(p_j) entries, biclusters planted in A
-    (m = round(nA**e) rows by round(ncols**e) columns, rank-1 sign pattern u_i v_j
-    with each entry flipped with probability ``flip``), covariates T then S with an
-    all-ones first column and i.i.d. +-1 columns.  The draws come from torch's device
-    generator seeded with ``seed``, so a problem is the same on every run on the same
-    software stack (it is not the host generator's problem)."""
-    import torch
-    g = torch.Generator(device=device)
-    g.manual_seed(seed)
+@dataclass
+class DevicePool:
+    """A pool drawn and kept on the device (``pool``: an lkc.Pool); the first nA rows are
+    the cases.  ``cov`` holds the covariate columns as rows (ncov x bsize(nrows))."""
+    pool: object
+    cov: np.ndarray
+    nrows: int
+    ncols: int
+    ncov: int
+    nA: int
+    planted_rows: List[np.ndarray]
+    planted_cols: List[np.ndarray]
+
+
+def make_pool_random(nA: int, nZ: int, ncols: int, T: int = 3, pmin: float = 0.01, pmax: float = 0.5,
+                     planted: Sequence[float] = (0.5, 0.55, 0.6), flip: float = 0.1, seed: int = 1) -> DevicePool:
+    """make_pool's column model drawn on the device by the library (lkc_pool_create_random),
+    for problems too large for the host generator (65536 + 65536 patients x 524288 SNPs is
+    6.9e10 draws): column frequencies p_j ~ U[pmin, pmax] and Bernoulli(p_j) entries from
+    the library's counter-based generator keyed by ``seed``; biclusters planted in A
+    (m = round(nA**e) rows by round(ncols**e) columns, rank-1 sign pattern u_i v_j with each
+    entry flipped with probability ``flip``) and covariates T then S (an all-ones first
+    column and i.i.d. +-1 columns) from the host generator seeded with ``seed``.  The
+    problem is the same on every run and device (it is not the host generator's problem)."""
+    from . import lkc
     rng = np.random.default_rng(seed)
-    rb = bsize(ncols)
     n = nA + nZ
-    p = torch.rand(ncols, generator=g, device=device) * (pmax - pmin) + pmin
     plants = []
     for e in planted:
         m, mc = min(int(round(nA ** e)), nA), min(int(round(ncols ** e)), ncols)
         r = np.sort(rng.permutation(nA)[:m]); cidx = np.sort(rng.permutation(ncols)[:mc])
         u = rng.random(m) < 0.5; v = rng.random(mc) < 0.5
         blk = ~(u[:, None] ^ v[None, :]); blk ^= rng.random((m, mc)) < flip
-        plants.append((r, torch.as_tensor(cidx, device=device), torch.as_tensor(blk, device=device)))
-    rows = np.zeros((n, rb), dtype=np.uint8)
-    wts = torch.tensor([128, 64, 32, 16, 8, 4, 2, 1], dtype=torch.uint8, device=device)
-    for r0 in range(0, n, chunk):
-        r1 = min(n, r0 + chunk)
-        bits = torch.zeros((r1 - r0, rb * 8), dtype=torch.bool, device=device)
-        bits[:, :ncols] = torch.rand((r1 - r0, ncols), generator=g, device=device) < p[None, :]
-        for (r, cidx, blk) in plants:
-            sel = np.nonzero((r >= r0) & (r < r1))[0]
-            if sel.size:
-                loc = torch.as_tensor(r[sel] - r0, device=device)
-                bits[loc[:, None], cidx[None, :]] = blk[torch.as_tensor(sel, device=device)]
-        packed = (bits.view(r1 - r0, rb, 8).to(torch.uint8) * wts).sum(dim=2, dtype=torch.uint8)
-        rows[r0:r1] = packed.cpu().numpy()
-        del bits, packed
+        plants.append((r, cidx, blk))
     cov_rows = np.concatenate([_covariates(rng, nA, T), _covariates(rng, nZ, T)], axis=0) if nZ else _covariates(rng, nA, T)
     cov = pack_rows(cov_rows.T, bsize(n))
-    pool = Pool(rows, cov, n, ncols, T, nA)
-    pool.planted_rows = [r for (r, _, _) in plants]    # A rows of each bicluster
-    pool.planted_cols = [c.cpu().numpy() for (_, c, _) in plants]
-    return pool
+    P = lkc.Pool.random(n, ncols, pmin, pmax, seed, plants, cov, T)
+    return DevicePool(P, cov, n, ncols, T, nA, [r for r, _, _ in plants], [c for _, c, _ in plants])

@@ -79,6 +79,9 @@ def lib():
         "lkc_xcalc": (C.c_int, [dp, ip, C.c_int, C.c_int, C.c_int, C.c_int, C.c_longlong,
                                 C.POINTER(C.c_ulonglong), dp]),
         "lkc_transpose_bits": (C.c_int, [u8p, C.c_size_t, C.c_int, C.c_int, u8p, C.c_size_t]),
+        "lkc_pool_create_random": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_ulonglong, C.c_int, ip, ip,
+                                             ip, ip, u8p, u8p, C.c_size_t, C.c_int, C.POINTER(vp)]),
+        "lkc_pool_read": (C.c_int, [vp, C.c_int, C.c_int, u8p]),
         "lkc_An_ZtSn_ww_noalloc": (C.c_int, [vp, C.c_int, vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_int,
                                              C.c_int, C.c_longlong, vp, vp]),
         "lkc_pool_create": (C.c_int, [u8p, C.c_size_t, C.c_int, C.c_int, u8p, C.c_size_t, C.c_int, C.POINTER(vp)]),
@@ -395,6 +398,39 @@ class Pool:
         _check(lib().lkc_pool_create(_ptr(r, C.c_uint8), r.shape[1], nrows, ncols, _ptr(c, C.c_uint8), c.shape[1],
                                      ncov, C.byref(self._h)))
         self.nrows, self.ncols, self.ncov = nrows, ncols, ncov
+
+    @classmethod
+    def random(cls, nrows: int, ncols: int, pmin: float, pmax: float, seed: int, plants, cov: np.ndarray,
+               ncov: int) -> "Pool":
+        """A pool drawn on the device (lkc_pool_create_random): Bernoulli(p_j) entries with
+        p_j ~ U[pmin, pmax] from the library's counter-based generator keyed by seed, then the
+        planted blocks, a list of (rows, cols, bits m x mc bool), written over them."""
+        self = cls.__new__(cls)
+        c = np.ascontiguousarray(cov, dtype=np.uint8)
+        pm = np.array([len(r) for r, _, _ in plants], np.int32)
+        pmc = np.array([len(cc) for _, cc, _ in plants], np.int32)
+        pr = np.ascontiguousarray(np.concatenate([np.asarray(r, np.int32) for r, _, _ in plants]) if plants
+                                  else np.zeros(1, np.int32), dtype=np.int32)
+        pc = np.ascontiguousarray(np.concatenate([np.asarray(cc, np.int32) for _, cc, _ in plants]) if plants
+                                  else np.zeros(1, np.int32), dtype=np.int32)
+        pb = np.ascontiguousarray(np.concatenate([np.asarray(b, np.uint8).ravel() for _, _, b in plants]) if plants
+                                  else np.zeros(1, np.uint8), dtype=np.uint8)
+        if not plants:
+            pm = pmc = np.zeros(1, np.int32)
+        self._keep = (c,)
+        self._h = C.c_void_p()
+        _check(lib().lkc_pool_create_random(nrows, ncols, pmin, pmax, C.c_ulonglong(seed), len(plants),
+                                            _ptr(pm, C.c_int), _ptr(pmc, C.c_int), _ptr(pr, C.c_int),
+                                            _ptr(pc, C.c_int), _ptr(pb, C.c_uint8), _ptr(c, C.c_uint8), c.shape[1],
+                                            ncov, C.byref(self._h)))
+        self.nrows, self.ncols, self.ncov = nrows, ncols, ncov
+        return self
+
+    def read(self, r0: int, n: int) -> np.ndarray:
+        """Rows r0 .. r0+n-1 of the device-resident pool, packed (n x bsize(ncols))."""
+        out = np.zeros((max(n, 1), bsize(self.ncols)), dtype=np.uint8)
+        _check(lib().lkc_pool_read(self._h, r0, n, _ptr(out, C.c_uint8)))
+        return out[:n]
 
     def close(self) -> None:
         if self._h:

@@ -1,5 +1,6 @@
 """The seeded .b16 generator and the .in files it writes."""
 import numpy as np
+import pytest
 
 from lakcluster_c_amd import gen
 
@@ -36,34 +37,59 @@ def test_planted_bicluster_is_rank_one(tmp_path):
     assert np.linalg.matrix_rank(s) == 1
 
 
-def test_device_pool_generator_on_cpu():
-    """make_pool_device (the large problems' generator) run on the CPU: the packed rows
-    hold exactly the drawn bits with zero padding, the column frequencies follow
-    U[pmin, pmax], the planted blocks are rank-1 sign patterns up to the flips, and the
+def _pool_uniform(seed, rows, cols):
+    """The library's documented counter-based uniform (lkc.h, lkc_pool_create_random): the
+    splitmix64 finaliser of seed * 0x9E3779B97F4A7C15 + row * 0xD1B54A32D192ED03 +
+    (col + 1) * 0xABC98388FB8FAC03, top 53 bits as a fraction; arrays broadcast."""
+    u64 = np.uint64
+    with np.errstate(over="ignore"):
+        x = (u64(seed) * u64(0x9E3779B97F4A7C15) + np.asarray(rows, u64) * u64(0xD1B54A32D192ED03)
+             + (np.asarray(cols, u64) + u64(1)) * u64(0xABC98388FB8FAC03))
+        x ^= x >> u64(30); x *= u64(0xBF58476D1CE4E5B9)
+        x ^= x >> u64(27); x *= u64(0x94D049BB133111EB)
+        x ^= x >> u64(31)
+    return (x >> u64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def test_pool_uniform_restatement():
+    """The restatement of the generator: uniform on [0, 1), rows and columns independent."""
+    u = _pool_uniform(5, np.arange(400)[:, None], np.arange(300)[None, :])
+    assert u.min() >= 0 and u.max() < 1 and abs(u.mean() - 0.5) < 0.01
+    assert abs(np.corrcoef(u[:, :-1].ravel(), u[:, 1:].ravel())[0, 1]) < 0.02
+    assert abs(np.corrcoef(u[:-1].ravel(), u[1:].ravel())[0, 1]) < 0.02
+    assert not np.array_equal(_pool_uniform(5, 0, np.arange(10)), _pool_uniform(6, 0, np.arange(10)))
+
+
+@pytest.mark.gpu
+def test_device_pool_generator_matches_its_definition():
+    """make_pool_random (the large problems' generator, drawn by the library on the device):
+    every bit equals the documented generator's draw (U(seed, r, j) < p_j with p_j = pmin +
+    (pmax - pmin) U(seed, ~0, j)) except the planted blocks, which hold their rank-1 sign
+    patterns; padding bits are zero; the column frequencies follow U[pmin, pmax]; the
     covariates have an all-ones first column."""
-    import numpy as np
-    from lakcluster_c_amd import gen
-    nA, nZ, m = 300, 200, 5000
-    P = gen.make_pool_device(nA, nZ, m, T=3, pmin=0.05, pmax=0.5, planted=(0.5, 0.6), flip=0.1, seed=3,
-                             chunk=128, device="cpu")
-    assert P.rows.shape == (nA + nZ, gen.bsize(m)) and P.nA == nA and P.nrows == nA + nZ
-    bits = np.unpackbits(P.rows, axis=1, bitorder="big")
+    nA, nZ, m, seed, pmin, pmax = 300, 200, 5000, 3, 0.05, 0.5
+    P = gen.make_pool_random(nA, nZ, m, T=3, pmin=pmin, pmax=pmax, planted=(0.5, 0.6), flip=0.1, seed=seed)
+    with P.pool as pool:
+        rows = pool.read(0, nA + nZ)
+    assert rows.shape == (nA + nZ, gen.bsize(m)) and P.nA == nA and P.nrows == nA + nZ
+    bits = np.unpackbits(rows, axis=1, bitorder="big")
     assert not bits[:, m:].any()
-    freq = bits[:, :m].mean(axis=0)
+    p = pmin + (pmax - pmin) * _pool_uniform(seed, 0xFFFFFFFF, np.arange(m))
+    want = (_pool_uniform(seed, np.arange(nA + nZ)[:, None], np.arange(m)[None, :]) < p[None, :])
+    planted = np.zeros_like(want)
+    for r, c in zip(P.planted_rows, P.planted_cols):
+        planted[np.ix_(r, c)] = True
+    assert np.array_equal(bits[:, :m][~planted], want[~planted])
+    freq = want.mean(axis=0)
     assert abs(freq.mean() - 0.275) < 0.01 and freq.min() > 0.0 and freq.max() < 0.75
     assert [len(r) for r in P.planted_rows] == [round(nA ** 0.5), round(nA ** 0.6)]
     assert all(r.max() < nA for r in P.planted_rows)          # planted in A only
     cov = np.unpackbits(P.cov, axis=1, bitorder="big")[:, : nA + nZ]
     assert cov.shape[0] == 3 and cov[0].all() and 0.4 < cov[1].mean() < 0.6
     # a planted block is a rank-1 sign pattern u_i v_j with 10% flips: two of its rows agree
-    # on about 0.9^2 + 0.1^2 = 82% of its columns or on about 18% (opposite signs); two
-    # unplanted rows on the same columns agree on p^2 + (1-p)^2 of them (0.5 to 0.9)
+    # on about 0.9^2 + 0.1^2 = 82% of its columns or on about 18% (opposite signs)
     r, c = P.planted_rows[1], P.planted_cols[1]
     assert len(c) == round(m ** 0.6)
     sub = bits[r][:, c]
     agree = (sub[:, None, :] == sub[None, :, :]).mean(axis=2)[np.triu_indices(len(r), 1)]
     assert np.abs(agree - 0.5).mean() > 0.25
-    others = np.setdiff1d(np.arange(nA), np.concatenate(P.planted_rows))[:len(r)]
-    sub0 = bits[others][:, c]
-    agree0 = (sub0[:, None, :] == sub0[None, :, :]).mean(axis=2)[np.triu_indices(len(r), 1)]
-    assert np.abs(agree0 - 0.5).mean() < 0.15

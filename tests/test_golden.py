@@ -104,3 +104,52 @@ def test_gpu_caseonly_4096_full_size_matches_oracle():
     assert np.array_equal(xa, g["xdrop_a"].astype(xa.dtype))
     assert np.array_equal(xb, g["xdrop_b"].astype(xb.dtype))
     assert ops == int(g["ops"][0])
+
+
+# ---------------------------------------------------------------------------
+# The 16384 x 65536 case-control problem's family at 2048 x 8192
+# (tests/golden/make_casecontrol_golden.py): the benchmark's own parameters (T/S 3
+# covariates, gamma 0.02, B_MLT 32, planted bicluster in A) at a size the oracle finishes
+# ---------------------------------------------------------------------------
+
+def _casecontrol_2048():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_casecontrol_golden",
+                                                  os.path.join(HERE, "golden", "make_casecontrol_golden.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.CASE_CONTROL_2048, np.load(os.path.join(HERE, "golden", "oracle_casecontrol_2048x8192.npz"))
+
+
+def test_casecontrol_2048_fixture_invariants(oracle):
+    """The committed oracle run: counts follow get_xdrop at every line down to 2 x 2, and
+    every row and column leaves once."""
+    kw, g = _casecontrol_2048()
+    tr = g["trace"]
+    assert np.isfinite(tr).all() and [int(x) for x in tr[0, :3]] == [0, kw["nA"], kw["ncols"]] and tr[-1, 1] <= 2
+    for i in range(tr.shape[0] - 1):
+        rd, cd = oracle.get_xdrop(tr[i, 1], tr[i, 2], kw["gamma"])
+        assert tr[i + 1, 1] == tr[i, 1] - min(rd, tr[i, 1]) and tr[i + 1, 2] == tr[i, 2] - min(cd, tr[i, 2])
+    xa = g["xdrop_a"].astype(np.int64)
+    rows, cols = xa[xa[:, 0] >= 0, 0], xa[xa[:, 1] >= 0, 1]
+    assert np.array_equal(np.sort(rows), np.arange(kw["nA"]))
+    assert np.array_equal(np.sort(cols), np.arange(kw["ncols"]))
+
+
+@pytest.mark.gpu
+def test_gpu_casecontrol_2048x8192_matches_oracle():
+    """The HIP path on the 2048 x 8192 case-control problem: trace (bit patterns), xdrop
+    lists and bit-op count equal the oracle's committed run."""
+    from lakcluster_c_amd import gen, lkc
+    kw, g = _casecontrol_2048()
+    with tempfile.TemporaryDirectory() as root:
+        ds = gen.make_dataset(root, "casecontrol_2048x8192", **kw)
+        with lkc.Bcc(open(ds.in_path).read()) as run:
+            run.run()
+            tr = run.trace()
+            xa, xb = run.xdrop()
+            ops = run.ops
+    assert tr.tobytes() == g["trace"].tobytes()
+    assert np.array_equal(xa, g["xdrop_a"].astype(xa.dtype))
+    assert np.array_equal(xb, g["xdrop_b"].astype(xb.dtype))
+    assert ops == int(g["ops"][0])
