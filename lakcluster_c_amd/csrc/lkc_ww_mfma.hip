@@ -12,6 +12,7 @@
 // the reference's order (ascending w), per (k, t).
 #include "lkc_kernels.h"
 #include <stdio.h>
+#include <algorithm>
 
 namespace lkc {
 
@@ -140,17 +141,49 @@ __device__ __forceinline__ v4i spread16_cols(uint32_t wr, int h)
 
 #define WW_BROW 144   // LDS row of one w: 128 digit bytes + 16 bytes of padding (no bank conflicts)
 
+// the compiler keeps LDS accesses of one wave in program order across this point; the
+// LDS operations of a wave are performed in order, so lanes of the same wave then see
+// each other's writes
+__device__ __forceinline__ void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Block: 4 waves x 32 rows k; the 32 w of a step are shared, their digits staged in LDS
 // per 128-column chunk of n (double buffered, the next chunk loaded while the current
 // one is used).  Lane (r, g) of a step s feeds columns n0 + 64g + 16s .. +15: the
 // packed bits of P, T and the mask come as one 8-byte word per lane and chunk.
-template <int NTG, int ND>
+//
+// After the last chunk of a w step, each lane holds the integer sums of 16 rows k at one
+// w.  Every lane turns its 16 into the reference's double terms (the per-w inputs -- Gm,
+// H, the S signs -- and the per-k ones -- alpha, the counts, the Q signs -- come from LDS,
+// staged once per block or per w step), writes them to LDS, and lane k of each wave then
+// only adds its row's terms in ascending w, as the reference's loop over w does.
+//
+// MODE 0 is that fused form.  For grids too small to fill the device (few rows k), the
+// w steps, and parts of the n chunks, spread over blocks (blockIdx.z = w step * nsplit +
+// part) and the terms go through a buffer Pbuf[(t * nwp + w) * nk + k]: MODE 1 (one part)
+// forms the terms in the block and stores them; MODE 2 (several parts) adds its integer
+// sums there with 64-bit atomics (integer sums in any order are exact) and k_ww_terms
+// forms the terms in place.  k_ww_sum then adds each row's terms in ascending w.
+template <int NTG, int ND, int MODE>
 __global__ __launch_bounds__(256) void k_ww_mfma(WwArgs a, const int8_t* Dg, long long ldn, int nwp,
                                                  const double* H, const double* bp, const long long* Gm,
-                                                 const int* cntp, const int* llk)
+                                                 const int* cntp, const int* llk, int nsplit,
+                                                 unsigned long long* Pbuf)
 {
   __shared__ __attribute__((aligned(16))) int8_t Bs[2][ND][32][WW_BROW];
-  __shared__ long long Gs[4][32][33];
+  __shared__ double Tm[4][32][33];                    // terms [wave][k][w]
+  __shared__ double KcA[128], KcQ[128];               // alpha of k's column block, of q's
+  __shared__ int Kcnt[NTG][128], Kllk[NTG][128];      // cntp, llk of the block's rows k
+  __shared__ double Cs[2 + NTG];                      // min, mlt, bp[t]
+  __shared__ long long WGm[32];                       // the w step's Gm, H, S signs, w indices
+  __shared__ double WHt[NTG][32];
+  __shared__ int WvS[NTG][32];
+  __shared__ int Wws[32];
+  __shared__ uint32_t Zb[128];                        // bit j: Q(q_k, w0 + j) == +1
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, g = lane >> 5, r = lane & 31;
   const int t0 = blockIdx.y * NTG;
   const int nts = min(NTG, a.nt - t0);
@@ -161,15 +194,45 @@ __global__ __launch_bounds__(256) void k_ww_mfma(WwArgs a, const int8_t* Dg, lon
   const uint8_t* Trow[NTG];
 #pragma unroll
   for (int t = 0; t < NTG; t++) Trow[t] = a.Tc.p + (size_t)a.tlist[t0 + min(t, nts - 1)] * a.Tc.rb + 8 * g;
+  constexpr bool SPLIT = MODE != 0;
+  // per-block constants (visible after the first barrier of the w loop)
+  if (MODE == 2) {
+  } else if (tid < 128) {
+    const int myk = min(blockIdx.x * 128 + tid, a.nk - 1);
+    const double* alpha = a.ajdk + (size_t)1 * a.pcols;
+    KcA[tid] = alpha[a.klist[myk] / kPopBits];
+    KcQ[tid] = alpha[a.qlist[myk] / kPopBits];
+#pragma unroll
+    for (int t = 0; t < NTG; t++) {
+      const int tt = t0 + min(t, nts - 1);
+      Kcnt[t][tid] = cntp[(size_t)tt * a.nk + myk];
+      Kllk[t][tid] = llk[(size_t)tt * a.nk + myk];
+    }
+  } else if (tid == 128) {
+    Cs[0] = a.scal[2]; Cs[1] = a.scal[4];
+#pragma unroll
+    for (int t = 0; t < NTG; t++) Cs[2 + t] = bp[t0 + min(t, nts - 1)];
+  }
   // ordered sums over w of row kbase + lane (lane < 32), per covariate
   double o[NTG];
 #pragma unroll
   for (int t = 0; t < NTG; t++) o[t] = 0.0;
-  const int nchunk = (int)(ldn / 128);
+  const int nchunk_all = (int)(ldn / 128);
+  int c_lo = 0, c_hi = nchunk_all, w_first = 0, w_end = a.nw;
+  if (SPLIT) {
+    const int ws = blockIdx.z / nsplit, sp = blockIdx.z % nsplit;
+    c_lo = (int)((long long)sp * nchunk_all / nsplit);
+    c_hi = (int)((long long)(sp + 1) * nchunk_all / nsplit);
+    w_first = 32 * ws; w_end = min(a.nw, w_first + 32);
+  }
   // digit loader: piece e = tid + 256 j -> digit plane j, w row tid / 8, 16 bytes (tid % 8)
   const int lw = tid >> 3, lp = tid & 7;
+  long long S = 0;
+#pragma unroll
+  for (int i = 0; i < ND; i++) S += 1LL << (8 * i);
 
-  for (int w0 = 0; w0 < a.nw; w0 += 32) {
+  for (int w0 = w_first; w0 < w_end; w0 += 32) {
+    const int wn = min(32, a.nw - w0);
     v16i acc[NTG][ND];
 #pragma unroll
     for (int t = 0; t < NTG; t++)
@@ -189,11 +252,24 @@ __global__ __launch_bounds__(256) void k_ww_mfma(WwArgs a, const int8_t* Dg, lon
 #define WW_ALOAD(c) do { const long long bo_ = 16LL * (c); pn = *(const uint64_t*)(Prow + bo_); \
       tn0 = *(const uint64_t*)(Trow[0] + bo_); if (NTG > 1) tn1 = *(const uint64_t*)(Trow[NTG > 1 ? 1 : 0] + bo_); \
       if (NTG > 2) tn2 = *(const uint64_t*)(Trow[NTG > 2 ? 2 : 0] + bo_); mk = *(const uint64_t*)(pm + bo_); } while (0)
-    __syncthreads();   // the previous w step's readers are done with Bs
-    if (nchunk > 0) { WW_BLOAD(0); WW_ALOAD(0); WW_BSTORE(0); }
+    __syncthreads();   // the previous w step's readers are done with Bs and the staged inputs
+    // the step's per-w inputs
+    if (MODE != 2 && tid < 32) {
+      const int w = w0 + min(tid, wn - 1);
+      const int ww = a.wlist[w];
+      Wws[tid] = ww;
+      WGm[tid] = Gm[w];
+#pragma unroll
+      for (int t = 0; t < NTG; t++) {
+        const int tt = t0 + min(t, nts - 1);
+        WHt[t][tid] = H[(size_t)tt * a.nw + w];
+        WvS[t][tid] = bitv(a.Sc.p + (size_t)a.slist[tt] * a.Sc.rb, ww) ? 1 : -1;
+      }
+    }
+    if (c_hi > c_lo) { WW_BLOAD(c_lo); WW_ALOAD(c_lo); WW_BSTORE(0); }
     __syncthreads();
-    for (int c = 0; c < nchunk; c++) {
-      const int buf = c & 1;
+    for (int c = c_lo; c < c_hi; c++) {
+      const int buf = (c - c_lo) & 1;
       uint64_t pos8[NTG];
       pos8[0] = ~(pn ^ tn0) & mk;
       if (NTG > 1) pos8[NTG > 1 ? 1 : 0] = ~(pn ^ tn1) & mk;
@@ -206,7 +282,7 @@ __global__ __launch_bounds__(256) void k_ww_mfma(WwArgs a, const int8_t* Dg, lon
       }
       // the next chunk (the last one again at the end: no branch in the loop body, so the
       // scheduler can interleave the spreads and LDS reads with the MFMAs of the whole chunk)
-      const int cn = min(c + 1, nchunk - 1);
+      const int cn = min(c + 1, c_hi - 1);
       WW_BLOAD(cn); WW_ALOAD(cn);
       __builtin_amdgcn_iglp_opt(0);
 #pragma unroll
@@ -227,54 +303,150 @@ __global__ __launch_bounds__(256) void k_ww_mfma(WwArgs a, const int8_t* Dg, lon
 #undef WW_BLOAD
 #undef WW_BSTORE
 #undef WW_ALOAD
-    // epilogue: each lane < 32 walks its row's 32 w in ascending order
-    const int wn = min(32, a.nw - w0);
+    if (MODE == 2) {
+      // this block's part of the integer sums of rows kbase .. kbase + 31 at w = w0 + r
+      if (w0 + r < a.nw) {
+#pragma unroll
+        for (int t = 0; t < NTG; t++) {
+          if (t >= nts) break;
+#pragma unroll
+          for (int v = 0; v < 16; v++) {
+            const int k = kbase + 8 * (v / 4) + 4 * g + (v % 4);
+            long long spc = 0;
+#pragma unroll
+            for (int i = ND - 1; i >= 0; i--) spc = spc * 256 + (long long)acc[t][i][v];
+            if (k < a.nk) atomicAdd(Pbuf + ((size_t)(t0 + t) * nwp + w0 + r) * a.nk + k, (unsigned long long)spc);
+          }
+        }
+      }
+      continue;
+    }
+    // the Q signs of the block's 128 rows k at the step's w: two threads per k, 16 w each
+    {
+      const int kq = tid >> 1, half = tid & 1;
+      const int myk = min(blockIdx.x * 128 + kq, a.nk - 1);
+      const uint8_t* Qrow = a.Q.p + (size_t)a.qlist[myk] * a.Q.rb;
+      uint32_t bits = 0;
+#pragma unroll 4
+      for (int jj = 0; jj < 16; jj++) {
+        const int j = 16 * half + jj;
+        if (j < wn && bitv(Qrow, Wws[j])) bits |= 1u << j;
+      }
+      bits |= __shfl_xor(bits, 1, 64);
+      if (!half) Zb[kq] = bits;
+    }
+    __syncthreads();
+    // terms: lane (g, r) holds rows k = 8 (v / 4) + 4 g + v % 4 of its wave at w = w0 + r
 #pragma unroll
     for (int t = 0; t < NTG; t++) {
       if (t >= nts) break;
+      const double mn = Cs[0], mlt = Cs[1], bpt = Cs[2 + t];
+      if (r < wn) {
+        const long long gm = WGm[r];
+        const double ht = WHt[t][r];
+        const double vS = (double)WvS[t][r];
 #pragma unroll
-      for (int v = 0; v < 16; v++) {
-        long long spc = 0;
+        for (int v = 0; v < 16; v++) {
+          const int kw = 8 * (v / 4) + 4 * g + (v % 4);        // row within the wave
+          const int kl = 32 * wave + kw;                        // row within the block
+          long long spc = 0;
 #pragma unroll
-        for (int i = ND - 1; i >= 0; i--) spc = spc * 256 + (long long)acc[t][i][v];
-        Gs[wave][8 * (v / 4) + 4 * g + (v % 4)][r] = spc;
-      }
-      __syncthreads();
-      if (lane < 32) {
-        // per-row constants (recomputed here to keep them out of the main loop's registers)
-        const double mn = a.scal[2], mlt = a.scal[4];
-        const double* alpha = a.ajdk + (size_t)1 * a.pcols;
-        const int myk = min(kbase + lane, a.nk - 1);
-        const int kk = a.klist[myk], qq = a.qlist[myk];
-        const double alpha_k = alpha[kk / kPopBits], alpha_q = alpha[qq / kPopBits];
-        const uint8_t* Qrow = a.Q.p + (size_t)qq * a.Q.rb;
-        long long S = 0;
-#pragma unroll
-        for (int i = 0; i < ND; i++) S += 1LL << (8 * i);
-        const int tt = t0 + t;
-        const double bkk = (double)llk[(size_t)tt * a.nk + myk] * mn * mlt;
-        const long long off = 128 * S * (long long)cntp[(size_t)tt * a.nk + myk];
-        const double at_base = bp[tt] * alpha_k;
-        const uint8_t* Srow = a.Sc.p + (size_t)a.slist[tt] * a.Sc.rb;
-        const double* Ht = H + (size_t)tt * a.nw;
-        double acc_o = o[t];
-        for (int j = 0; j < wn; j++) {
-          const int w = w0 + j, ww = a.wlist[w];
-          const long long G = 2 * (Gs[wave][lane][j] + off) - Gm[w];
-          const double at_tmp = Ht[w] * alpha_k;
+          for (int i = ND - 1; i >= 0; i--) spc = spc * 256 + (long long)acc[t][i][v];
+          const long long G = 2 * (spc + 128 * S * (long long)Kcnt[t][kl]) - gm;
+          const double alpha_k = KcA[kl], alpha_q = KcQ[kl];
+          const double bkk = (double)Kllk[t][kl] * mn * mlt;
+          const double at_base = bpt * alpha_k;
+          const double at_tmp = ht * alpha_k;
           const double gd = (double)G;
           const double tmp = (double)(bkk + gd) / mlt - (double)(at_base + at_tmp) / mlt;
-          const int vS = bitv(Srow, ww) ? 1 : -1, vZ = bitv(Qrow, ww) ? 1 : -1;
-          acc_o += tmp * vS * (vZ - alpha_q);
+          const int vZ = ((Zb[kl] >> r) & 1u) ? 1 : -1;
+          const double term = tmp * vS * (vZ - alpha_q);
+          if (MODE == 1) {
+            const int k = kbase + kw;
+            if (k < a.nk) ((double*)Pbuf)[((size_t)(t0 + t) * nwp + w0 + r) * a.nk + k] = term;
+          } else {
+            Tm[wave][kw][r] = term;
+          }
         }
+      }
+      if (MODE == 1) continue;
+      wave_lds_sync();
+      if (lane < 32) {
+        double acc_o = o[t];
+        for (int j = 0; j < wn; j++) acc_o += Tm[wave][lane][j];
         o[t] = acc_o;
       }
-      __syncthreads();
+      wave_lds_sync();
     }
   }
-  if (lane < 32 && kbase + lane < a.nk)
+  if (!SPLIT && lane < 32 && kbase + lane < a.nk)
 #pragma unroll
     for (int t = 0; t < NTG; t++) if (t < nts) a.out[(long long)a.tlist[t0 + t] * a.ldo + a.klist[kbase + lane]] = o[t];
+}
+
+// The terms of the split form with several parts, in place: one thread per (t, w, k), the
+// same double expression as the fused kernel's.
+__global__ void k_ww_terms(WwArgs a, int nwp, unsigned long long* Pbuf, const double* H, const double* bp,
+                           const long long* Gm, const int* cntp, const int* llk, long long S)
+{
+  const long long tot = (long long)a.nt * a.nw * a.nk;
+  const double* alpha = a.ajdk + (size_t)1 * a.pcols;
+  const double mn = a.scal[2], mlt = a.scal[4];
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
+    const int k = (int)(e % a.nk);
+    const long long tw = e / a.nk;
+    const int w = (int)(tw % a.nw), tt = (int)(tw / a.nw);
+    const int kk = a.klist[k], qq = a.qlist[k], ww = a.wlist[w];
+    const double alpha_k = alpha[kk / kPopBits], alpha_q = alpha[qq / kPopBits];
+    const double bkk = (double)llk[(size_t)tt * a.nk + k] * mn * mlt;
+    const long long off = 128 * S * (long long)cntp[(size_t)tt * a.nk + k];
+    const double at_base = bp[tt] * alpha_k;
+    unsigned long long* P = Pbuf + ((size_t)tt * nwp + w) * a.nk + k;
+    const long long G = 2 * ((long long)*P + off) - Gm[w];
+    const double at_tmp = H[(size_t)tt * a.nw + w] * alpha_k;
+    const double gd = (double)G;
+    const double tmp = (double)(bkk + gd) / mlt - (double)(at_base + at_tmp) / mlt;
+    const int vS = bitv(a.Sc.p + (size_t)a.slist[tt] * a.Sc.rb, ww) ? 1 : -1;
+    const int vZ = bitv(a.Q.p + (size_t)qq * a.Q.rb, ww) ? 1 : -1;
+    *(double*)P = tmp * vS * (vZ - alpha_q);
+  }
+}
+// the ordered sums over w of the terms, one thread per (k, t): loads along k coalesce and
+// do not wait on the sum
+__global__ void k_ww_sum(WwArgs a, int nwp, const unsigned long long* Pbuf)
+{
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)a.nk * a.nt) return;
+  const int k = (int)(e % a.nk), tt = (int)(e / a.nk);
+  const double* T = (const double*)Pbuf + (size_t)tt * nwp * a.nk + k;
+  double acc_o = 0.0;
+  for (int w = 0; w < a.nw; w++) acc_o += T[(size_t)w * a.nk];
+  a.out[(long long)a.tlist[tt] * a.ldo + a.klist[k]] = acc_o;
+}
+__global__ void k_ww_zero(unsigned long long* p, long long n)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = 0ull;
+}
+
+// The split form is taken when the fused grid (128 rows k per block, covariates in groups of
+// 3) leaves CUs idle and the w steps can fill them: then one block per (rows, covariates,
+// w step), and the n chunks of a step in parts (at least 4 chunks each) until about one
+// block per CU.  (One block of this kernel fills a CU: its registers allow one wave per SIMD.)
+static const int kWwCUs = 256;
+static int ww_split_parts(int nk, int nt, int nw, size_t prb)
+{
+  if (nk <= 0 || nt <= 0 || nw <= 0) return 0;
+  const long long fused = (long long)cdiv(nk, 128) * cdiv(nt, 3);
+  const int nchunk = (int)(prb * 8 / 128);
+  if (fused >= kWwCUs || nchunk < 1 || nw < 1) return 0;
+  // the terms buffer (8 bytes per k, t and w) stays under 1 GiB
+  if ((double)nt * (double)(cdiv(nw, 32) * 32) * (double)nk * 8.0 > (double)(1ll << 30)) return 0;
+  const long long per = fused * cdiv(nw, 32);
+  if (per >= 2 * fused && per >= kWwCUs) return 1;
+  long long parts = (kWwCUs + per - 1) / per;
+  parts = std::min<long long>(parts, std::max(1, nchunk / 4));
+  if (parts <= 1 && per < 2 * fused) return 0;       // nothing to spread
+  return (int)std::max<long long>(parts, 1);
 }
 
 size_t ww_mfma_work_bytes(int nk, int nt, int nw, size_t prb, int B)
@@ -286,6 +458,7 @@ size_t ww_mfma_work_bytes(int nk, int nt, int nw, size_t prb, int B)
   b += sizeof(double) * ((size_t)nt * nw + nt + 8);
   b += sizeof(long long) * ((size_t)nw + 8);
   b += 2 * sizeof(int) * ((size_t)nt * nk + 8);
+  if (ww_split_parts(nk, nt, nw, prb) > 0) b += 256 + sizeof(unsigned long long) * (size_t)nt * nwp * nk;
   return b + 1024;
 }
 
@@ -303,7 +476,9 @@ hipError_t ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t s
   double* bp = (double*)(p + off); off += sizeof(double) * ((size_t)a.nt + 8);
   long long* Gm = (long long*)(p + off); off += sizeof(long long) * ((size_t)a.nw + 8);
   int* cntp = (int*)(p + off); off += sizeof(int) * ((size_t)a.nt * a.nk + 8);
-  int* llk = (int*)(p + off);
+  int* llk = (int*)(p + off); off += sizeof(int) * ((size_t)a.nt * a.nk + 8);
+  const int nsplit = ww_split_parts(a.nk, a.nt, a.nw, a.prb);
+  unsigned long long* Pbuf = (unsigned long long*)(((uintptr_t)(p + off) + 255) & ~(uintptr_t)255);
   if (nwp > 0) {
     long long tot = (long long)nwp * (ldn / 4);
     hipLaunchKernelGGL(k_ww_digits, dim3(cdiv(tot, 256) < 65536 ? cdiv(tot, 256) : 65536), dim3(256), 0, st,
@@ -312,14 +487,32 @@ hipError_t ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t s
   }
   hipLaunchKernelGGL(k_ww_hg, dim3(a.nw + 1), dim3(256), 0, st, a, H, bp, Gm);
   hipLaunchKernelGGL(k_ww_cntp, dim3(a.nk), dim3(64), 0, st, a, cntp, llk);
-  dim3 grid(cdiv(a.nk, 128), cdiv(a.nt, a.nt >= 3 ? 3 : a.nt));
+  long long S = 0;
+  for (int i = 0; i < ndig; i++) S += 1LL << (8 * i);
+  const long long np = (long long)a.nt * nwp * a.nk;
+  if (nsplit > 1)
+    hipLaunchKernelGGL(k_ww_zero, dim3(cdiv(np, 256) < 65536 ? cdiv(np, 256) : 65536), dim3(256), 0, st, Pbuf, np);
+  dim3 grid(cdiv(a.nk, 128), cdiv(a.nt, a.nt >= 3 ? 3 : a.nt), nsplit > 0 ? cdiv(a.nw, 32) * nsplit : 1);
   if (hook && hook->before) hook->before(hook->ctx, st);
-#define WW_LAUNCH(NTG, ND) hipLaunchKernelGGL((k_ww_mfma<NTG, ND>), grid, dim3(256), 0, st, a, Dg, ldn, nwp, H, bp, Gm, cntp, llk)
+#define WW_LAUNCH(NTG, ND) do { if (nsplit > 1) \
+      hipLaunchKernelGGL((k_ww_mfma<NTG, ND, 2>), grid, dim3(256), 0, st, a, Dg, ldn, nwp, H, bp, Gm, cntp, llk, nsplit, Pbuf); \
+    else if (nsplit == 1) \
+      hipLaunchKernelGGL((k_ww_mfma<NTG, ND, 1>), grid, dim3(256), 0, st, a, Dg, ldn, nwp, H, bp, Gm, cntp, llk, 1, Pbuf); \
+    else hipLaunchKernelGGL((k_ww_mfma<NTG, ND, 0>), grid, dim3(256), 0, st, a, Dg, ldn, nwp, H, bp, Gm, cntp, llk, 1, Pbuf); } while (0)
 #define WW_ND(NTG) switch (ndig) { case 1: WW_LAUNCH(NTG, 1); break; case 2: WW_LAUNCH(NTG, 2); break; \
                                    case 3: WW_LAUNCH(NTG, 3); break; default: WW_LAUNCH(NTG, 4); break; }
   if (a.nt >= 3) { WW_ND(3) } else if (a.nt == 2) { WW_ND(2) } else { WW_ND(1) }
 #undef WW_ND
 #undef WW_LAUNCH
+  if (nsplit > 1) {
+    const long long nte = (long long)a.nt * a.nw * a.nk;
+    hipLaunchKernelGGL(k_ww_terms, dim3(cdiv(nte, 256) < 65536 ? cdiv(nte, 256) : 65536), dim3(256), 0, st,
+                       a, nwp, Pbuf, H, bp, Gm, cntp, llk, S);
+  }
+  if (nsplit > 0) {
+    const long long nc = (long long)a.nk * a.nt;
+    hipLaunchKernelGGL(k_ww_sum, dim3(cdiv(nc, 64)), dim3(64), 0, st, a, nwp, Pbuf);
+  }
   if (hook && hook->after) hook->after(hook->ctx, st);
   return hipGetLastError();
 }
