@@ -641,6 +641,8 @@ void Run::setup_device(double t0)
   T_AnAt_AnAt = dev.alloc<double>(nT * nA);
   QR_ZSZ = dev.alloc<double>(nA * nT); QR_TAT = dev.alloc<double>(nA * nT);
   QC_ZSZ = dev.alloc<double>(nM * nT); QC_TAT = dev.alloc<double>(nM * nT);
+  QR_ZSZ_nrm = dev.alloc<double>(nA * nT); QR_TAT_nrm = dev.alloc<double>(nA * nT);
+  QC_ZSZ_nrm = dev.alloc<double>(nM * nT); QC_TAT_nrm = dev.alloc<double>(nM * nT);
   for (int i = 0; i < 14; i++) mt[i] = dev.alloc<double>(nM * nT);
   {
     // zero the persistent arrays once: entries outside the active sets then read as 0
@@ -654,6 +656,7 @@ void Run::setup_device(double t0)
   cnt_buf = dev.alloc<int>(std::max(nM, nA) + nT + 8);
   xwork = dev.alloc<double>(1024);
   scal = dev.alloc<double>(16);
+  scal_s = dev.alloc<double>(4 * 16);
   dsv = dev.alloc<double>(nM + nA);
   HIPCHK(hipHostMalloc((void**)&hsv, sizeof(double) * (nM + nA), hipHostMallocDefault));
   skey = dev.alloc<double>(nM + nA);
@@ -911,56 +914,69 @@ void Run::st_YnWt()
 
 // bcc_M_YnWt_excerpt + bcc_lf_At_T_YnWt_S_Zn_excerpt (bcc_lf.c:304-316, 615-634).
 // L(r1, r2) = src[idx1(r1)*st1 + idx2(r2)*st2] with idx = the side's original index when full.
-void Run::ww_excerpt(int trm, const Side& s1, const Side& s2, const IList& kls, const double* src, long long st1,
-                     long long st2, int full, double* out, const Packed* pk)
+// The quantisation (xcalc into the slot's integers and scalars) and the contraction are
+// separate steps: the low-rank update quantises its four terms (stage a2) before it
+// contracts them (stage a3).
+void Run::ww_quantise(int slot, const WwTerm& w)
 {
-  const Side& pn = trm ? s2 : s1;
-  const Side& pw = trm ? s1 : s2;
+  const Side& pn = w.trm ? w.s2 : w.s1;
+  const Side& pw = w.trm ? w.s1 : w.s2;
   int nn = pn.rows->n(), nw = pw.rows->n();
+  ops += (long long)nn * nw * (c.B + 1) * 8;
+  if (nn == 0 || nw == 0) return;
   long long sn, sw; const int *rl = nullptr, *cl = nullptr;
-  if (!trm) { sn = st1; sw = st2; if (full) { rl = s1.rows->d; cl = s2.rows->d; } }
-  else { sn = st2; sw = st1; if (full) { rl = s2.rows->d; cl = s1.rows->d; } }
+  if (!w.trm) { sn = w.st1; sw = w.st2; if (w.full) { rl = w.s1.rows->d; cl = w.s2.rows->d; } }
+  else { sn = w.st2; sw = w.st1; if (w.full) { rl = w.s2.rows->d; cl = w.s1.rows->d; } }
   int n_total = pn.X->nc;
   size_t prb = pn.X->rb;
-  if (nn == 0 || nw == 0) {
-    // no rows on one side: every sum of the reference's kernel is over an empty set, so
-    // each output is +0.0 (the low-rank update after a drop of only rows or only columns)
-    ops += (long long)nn * nw * (c.B + 1) * 8;
-    ops += (long long)tl.n() * kls.n() * nw * (c.B + 1) * (long long)prb * 8;
-    fill(out, (long long)M * T, 0.0, st);
-    return;
-  }
   long long ldc = std::max(n_total, 1);
-  uint32_t* cz = (uint32_t*)g_cz.need(dev, sizeof(uint32_t) * (size_t)std::max(nw, 1) * ldc);
+  uint32_t* cz = (uint32_t*)g_czs[slot].need(dev, sizeof(uint32_t) * (size_t)std::max(nw, 1) * ldc);
   // The matrix-core contraction runs over the patients of the n side only: packed to the
   // front when a packed copy is given (the quantised values then go to list positions),
   // else the rows' whole 128-bit groups.  The integer sums over n do not depend on where
   // each patient sits, so both forms give the same result.
-  const bool packed = ww_kernel == 0 && pk && pk->nbits > 0;
+  const bool packed = ww_kernel == 0 && w.pk.nbits > 0;
   const int* nout = packed ? d_iota : pn.rows->d;
-  uint8_t* planes = nullptr;
-  xcalc(src, sn, sw, rl, cl, nn, nw, nout, c.B, c.D_MLT, cz, ldc, scal, xwork, st);
+  xcalc(w.src, sn, sw, rl, cl, nn, nw, nout, c.B, c.D_MLT, cz, ldc, scal_s + 16 * slot, xwork, st);
   if (ww_kernel == 1) {
-    planes = (uint8_t*)g_planes.need(dev, (size_t)std::max(nw, 1) * c.B * prb);
+    uint8_t* planes = (uint8_t*)g_planess[slot].need(dev, (size_t)std::max(nw, 1) * c.B * prb);
     cz_planes(cz, ldc, nw, n_total, pn.mask, c.B, planes, prb, st);
   }
-  ops += (long long)nn * nw * (c.B + 1) * 8;
+}
+
+void Run::ww_contract(int slot, const WwTerm& w)
+{
+  const Side& pn = w.trm ? w.s2 : w.s1;
+  const Side& pw = w.trm ? w.s1 : w.s2;
+  int nn = pn.rows->n(), nw = pw.rows->n();
+  int n_total = pn.X->nc;
+  size_t prb = pn.X->rb;
+  ops += (long long)tl.n() * w.kls->n() * nw * (c.B + 1) * (long long)prb * 8;
+  if (nn == 0 || nw == 0) {
+    // no rows on one side: every sum of the reference's kernel is over an empty set, so
+    // each output is +0.0 (the low-rank update after a drop of only rows or only columns)
+    fill(w.out, (long long)M * T, 0.0, st);
+    return;
+  }
+  long long ldc = std::max(n_total, 1);
+  const bool packed = ww_kernel == 0 && w.pk.nbits > 0;
   WwArgs a;
-  a.P = pn.X->b(); a.klist = kls.d; a.qlist = kls.d; a.nk = kls.n(); a.pm = pn.mask;
+  a.P = pn.X->b(); a.klist = w.kls->d; a.qlist = w.kls->d; a.nk = w.kls->n(); a.pm = pn.mask;
   a.Tc = pn.Cov->b(); a.tlist = tl.d; a.slist = tl.d; a.nt = tl.n();
-  a.planes = planes; a.prb = prb; a.row0 = pn.mask; a.nw = nw; a.wlist = pw.rows->d; a.B = c.B;
-  a.cz = cz; a.ldc = ldc;
-  a.Sc = pw.Cov->b(); a.Q = pw.X->b(); a.ajdk = ajdk; a.pcols = pcols; a.scal = scal;
-  a.out = out; a.ldo = M;
+  a.planes = ww_kernel == 1 ? (const uint8_t*)g_planess[slot].p : nullptr; a.prb = prb; a.row0 = pn.mask;
+  a.nw = nw; a.wlist = pw.rows->d; a.B = c.B;
+  a.cz = (const uint32_t*)g_czs[slot].p; a.ldc = ldc;
+  a.Sc = pw.Cov->b(); a.Q = pw.X->b(); a.ajdk = ajdk; a.pcols = pcols; a.scal = scal_s + 16 * slot;
+  a.out = w.out; a.ldo = M;
   if (packed) {
-    a.P = Bits{ pk->X, pn.X->nr, pk->nbits, pk->rb }; a.Tc = Bits{ pk->Cov, pn.Cov->nr, pk->nbits, pk->rb };
-    a.pm = pk->mask; a.row0 = pk->mask; a.prb = (size_t)pk->nbits / 8;
+    a.P = Bits{ w.pk.X, pn.X->nr, w.pk.nbits, w.pk.rb }; a.Tc = Bits{ w.pk.Cov, pn.Cov->nr, w.pk.nbits, w.pk.rb };
+    a.pm = w.pk.mask; a.row0 = w.pk.mask; a.prb = (size_t)w.pk.nbits / 8;
   } else if (ww_kernel == 0) {
     a.prb = std::min(prb, (size_t)((n_total + 127) / 128) * 16);
   }
   // timed: the contraction kernel itself (the quantised-value preparation is not included)
   struct Ctx { Prof* prof; Prof::Pending pp; double work; };
-  Ctx cx{ &prof, Prof::Pending(), 2.0 * tl.n() * kls.n() * nw * (double)nn * ((c.B + 7) / 8) };
+  Ctx cx{ &prof, Prof::Pending(), 2.0 * tl.n() * w.kls->n() * nw * (double)nn * ((c.B + 7) / 8) };
   WwHook hook{ [](void* p, hipStream_t s) { Ctx* x = (Ctx*)p; x->prof->begin("ww", x->work, s, x->pp); },
                [](void* p, hipStream_t s) { Ctx* x = (Ctx*)p; x->prof->end(x->pp, s); }, &cx };
   if (ww_kernel == 1) {
@@ -972,7 +988,14 @@ void Run::ww_excerpt(int trm, const Side& s1, const Side& s2, const IList& kls, 
     size_t wb = ww_mfma_work_bytes(a.nk, a.nt, nw, a.prb, c.B);
     HIPCHK(ww_mfma(a, g_work.need(dev, wb), wb, st, &hook));
   }
-  ops += (long long)tl.n() * kls.n() * nw * (c.B + 1) * (long long)prb * 8;
+}
+
+void Run::ww_excerpt(int trm, const Side& s1, const Side& s2, const IList& kls, const double* src, long long st1,
+                     long long st2, int full, double* out, const Packed* pk)
+{
+  const WwTerm w{ trm, s1, s2, &kls, src, st1, st2, full, out, pk ? *pk : Packed{} };
+  ww_quantise(0, w);
+  ww_contract(0, w);
 }
 
 // Packed copies of one patient group (row list src, -1 = padding, 32 W entries):
@@ -1170,42 +1193,73 @@ void Run::init_lf()
   init_s = now_s() - t0;
 }
 
-// bcc_sumscores_nrm/ifT/cmb (bcc_sumscores.c:324-356, 483-779), one bin: the first
-// covariate column's score of every active column into dsv[0, nk) and of every active
-// row into dsv[M, M + na), in list (index) order
-void Run::score_lists()
+// bcc_sumscores_nrm (bcc_sumscores.c:549-779; dcc_sumscores.c:478-699), one bin,
+// GLOBAL_Ireq <= 0: the four subscores over the active rows and columns divided by their
+// counts of summands, into zeroed arrays
+void Run::st_nrm()
 {
   unsigned int Ar = (unsigned int)ar.n(), Zr = (unsigned int)zr.n();
   long long cpop = kl.n();
   auto den = [](long long lld) { double d = (double)lld; if (fabs(d) < 0.000000000001) d = 1; return d; };
+  fill(QR_ZSZ_nrm, (long long)NA * T, 0.0, st); fill(QR_TAT_nrm, (long long)NA * T, 0.0, st);
+  fill(QC_ZSZ_nrm, (long long)M * T, 0.0, st); fill(QC_TAT_nrm, (long long)M * T, 0.0, st);
+  const ScoreStage qr{ ar.d, ar.n(), tl.d, tl.n(), NA }, qc{ kl.d, kl.n(), tl.d, tl.n(), M };
+  if (has_Z)
+    score_nrm(qr, QR_ZSZ, den(dcc ? (long long)(Zr - 0) * (long long)(cpop) : (long long)(Zr - 0) * (long long)(cpop) * (long long)(cpop - 1)),
+              QR_ZSZ_nrm, st);
+  score_nrm(qr, QR_TAT, den(dcc ? (long long)(Ar - 1) * (long long)(cpop) : (long long)(Ar - 1) * (long long)(cpop) * (long long)(cpop - 1)),
+            QR_TAT_nrm, st);
+  if (has_Z)
+    score_nrm(qc, QC_ZSZ, den(dcc ? (long long)(Ar) * (long long)(Zr) : (long long)(Ar) * (long long)(cpop - 1) * (long long)(Zr)),
+              QC_ZSZ_nrm, st);
+  score_nrm(qc, QC_TAT, den(dcc ? (long long)(Ar) * (long long)(Ar - 1) : (long long)(Ar) * (long long)(cpop - 1) * (long long)(Ar - 1)),
+            QC_TAT_nrm, st);
+}
+// bcc_sumscores_ifT (bcc_sumscores.c:483-548): n_mds = min(6, T - 1); the scale is
+// GLOBAL_kappa_squared whatever its sign (the reference's `if (...);{ ... }`), so nothing
+// changes when it is 0
+void Run::st_ifT()
+{
   int n_mds = std::max(0, tl.n() - 1); if (n_mds > 6) n_mds = 6;
-  double dmds = n_mds > 0 ? n_mds * c.kappa2 : 0.0;
-  Scores q;
-  q.list = ar.d; q.n = ar.n(); q.tl = tl.d; q.nt = tl.n(); q.ld = NA; q.has_Z = has_Z;
-  q.ZSZ = QR_ZSZ; q.TAT = QR_TAT;
-  // dcc: the half-loop counts of dcc_sumscores_nrm (dcc_sumscores.c:478-699)
-  q.dZ = den(dcc ? (long long)(Zr - 0) * (long long)(cpop) : (long long)(Zr - 0) * (long long)(cpop) * (long long)(cpop - 1));
-  q.dT = den(dcc ? (long long)(Ar - 1) * (long long)(cpop) : (long long)(Ar - 1) * (long long)(cpop) * (long long)(cpop - 1));
-  q.den = dmds; q.pow_form = 1; q.cmb = has_Z; q.out = dsv + M;
-  scores(q, st);
-  Scores p;
-  p.list = kl.d; p.n = kl.n(); p.tl = tl.d; p.nt = tl.n(); p.ld = M; p.has_Z = has_Z;
-  p.ZSZ = QC_ZSZ; p.TAT = QC_TAT;
-  p.dZ = den(dcc ? (long long)(Ar) * (long long)(Zr) : (long long)(Ar) * (long long)(cpop - 1) * (long long)(Zr));
-  p.dT = den(dcc ? (long long)(Ar) * (long long)(Ar - 1) : (long long)(Ar) * (long long)(cpop - 1) * (long long)(Ar - 1));
-  p.den = dmds; p.pow_form = 0; p.cmb = has_Z && Zrb > 0 && Arb > 0; p.out = dsv;
-  scores(p, st);
+  const double dmds = n_mds > 0 ? n_mds * c.kappa2 : 0.0;
+  if (!(dmds > 0)) return;
+  const ScoreStage qr{ ar.d, ar.n(), tl.d, tl.n(), NA }, qc{ kl.d, kl.n(), tl.d, tl.n(), M };
+  if (has_Z) score_ifT(qr, QR_ZSZ_nrm, dmds, 1, st);
+  score_ifT(qr, QR_TAT_nrm, dmds, 1, st);
+  if (has_Z) score_ifT(qc, QC_ZSZ_nrm, dmds, 0, st);
+  score_ifT(qc, QC_TAT_nrm, dmds, 0, st);
+}
+// bcc_sumscores_srt (bcc_sumscores.c:358-481): with GLOBAL_Ireq <= 0 the scores summed over
+// the bins are used as they are (one bin here): nothing to do
+void Run::st_srt() {}
+// bcc_sumscores_cmb (bcc_sumscores.c:324-356): QC_AtTAnAtTAn -= QC_AtTAnZtSZn and
+// QR_AnAtTAnAt -= QR_AnZtSZnAt, first covariate column
+void Run::st_cmb()
+{
+  const ScoreStage qr{ ar.d, ar.n(), tl.d, tl.n(), NA }, qc{ kl.d, kl.n(), tl.d, tl.n(), M };
+  if (has_Z && Zrb > 0 && Arb > 0) score_cmb(qc, QC_TAT_nrm, QC_ZSZ_nrm, st);
+  if (has_Z) score_cmb(qr, QR_TAT_nrm, QR_ZSZ_nrm, st);
+}
+// the combined scores of the active columns into dsv[0, nk) and of the active rows into
+// dsv[M, M + na), in list (index) order
+void Run::score_lists()
+{
+  st_nrm(); st_ifT(); st_srt(); st_cmb();
+  score_gather0(ScoreStage{ kl.d, kl.n(), tl.d, tl.n(), M }, QC_TAT_nrm, dsv, st);
+  score_gather0(ScoreStage{ ar.d, ar.n(), tl.d, tl.n(), NA }, QR_TAT_nrm, dsv + M, st);
 }
 
-// bcc_sumscores_nrm/ifT/cmb/xij/mxA (bcc_sumscores.c:37-779), one bin, Ireq<=0
-void Run::sumscores()
+// bcc_sumscores_xij (bcc_sumscores.c:225-322): the active column scores then row scores in
+// ascending order (dQuickSort_xij, Quicksort.c:83-141)
+void Run::st_xij()
 {
-  score_lists();
+  score_gather0(ScoreStage{ kl.d, kl.n(), tl.d, tl.n(), M }, QC_TAT_nrm, dsv, st);
+  score_gather0(ScoreStage{ ar.d, ar.n(), tl.d, tl.n(), NA }, QR_TAT_nrm, dsv + M, st);
   int nk = kl.n(), na = ar.n();
-  int rpop = na, cpop_i = nk, rdrop = 0, cdrop = 0;
-  get_xdrop(rpop, cpop_i, c.gamma, c.gamma_type, &rdrop, &cdrop);
-  int nc_drop = std::min(cdrop, cpop_i), nr_drop = std::min(rdrop, rpop);
-  // xij: ascending sort of the column then the row scores.  The device sorts both
+  int rdrop = 0, cdrop = 0;
+  get_xdrop(na, nk, c.gamma, c.gamma_type, &rdrop, &cdrop);
+  int nc_drop = std::min(cdrop, nk), nr_drop = std::min(rdrop, na);
+  // The device sorts both
   // columns (stable radix sort of (score, position)); the sorted values are unique, and
   // where the dropped prefix is strictly increasing its positions are the reference
   // quicksort's as well.  Otherwise (ties or NaN in the prefix) the host runs the
@@ -1219,7 +1273,6 @@ void Run::sumscores()
   HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipGetLastError());
   prof.collect();
-  int Irem = na > 0 ? 1 : 0;
   QC_sv.assign(hsv, hsv + nk); QC_perm.resize(nk);
   QR_sv.assign(hsv + M, hsv + M + na); QR_perm.resize(na);
   // The device order is the reference quicksort's where the sorted values of the dropped
@@ -1248,7 +1301,23 @@ void Run::sumscores()
   else { exact_order(0, nk, QC_sv, QC_perm); n_sort_fallback++; }
   if (strict(QR_sv.data(), na, nr_drop)) { for (int i = 0; i < nr_drop; i++) QR_perm[i] = hidx[M + i]; }
   else { exact_order(M, na, QR_sv, QR_perm); n_sort_fallback++; }
-  // mxA: drop counts, _rtn/_rmv masks, trace and xdrop lines
+  xij_done = true;
+}
+
+// bcc_sumscores_mxA / _mxB / _mxC (bcc_sumscores.c:37-205): get_xdrop's counts, the
+// first cdrop columns and rdrop rows of the sorted order leave (_rmv / _rtn masks), the
+// trace line and the xdrop lines; bcc_lrup_mxset's masks reach the device.  With dup, the
+// retained sets also become the active ones at once (bcc_lrup_mxdup, which the update
+// itself does not read: it uses the _rtn / _rmv sets).
+void Run::st_mxA(bool dup)
+{
+  if (!xij_done) throw Error(LKC_ERR_STATE, "bcc_sumscores_mxA needs bcc_sumscores_xij of the same pass");
+  xij_done = false;
+  int nk = kl.n(), na = ar.n();
+  int rpop = na, cpop_i = nk, rdrop = 0, cdrop = 0;
+  get_xdrop(rpop, cpop_i, c.gamma, c.gamma_type, &rdrop, &cdrop);
+  int nc_drop = std::min(cdrop, cpop_i), nr_drop = std::min(rdrop, rpop);
+  int Irem = na > 0 ? 1 : 0;
   hAc_rtn = hAc_j; std::fill(hAc_rmv.begin(), hAc_rmv.end(), 0);
   hAr_rtn = hAr_j; std::fill(hAr_rmv.begin(), hAr_rmv.end(), 0);
   for (int i = 0; i < nc_drop; i++) { int k = kl.h[QC_perm[i]]; bset(hAc_rmv.data(), k); bclr(hAc_rtn.data(), k); }
@@ -1268,9 +1337,26 @@ void Run::sumscores()
     int k = kl.h[QC_perm[i]];
     xdrop_a.push_back(-1); xdrop_a.push_back(k); xdrop_b.push_back(-1); xdrop_b.push_back(Ac_bidx[k]);
   }
-  // bcc_lrup_mxdup takes effect for the re-scoring; the update itself uses _rtn/_rmv
+  if (dup) { hAr_j = hAr_rtn; hAc_j = hAc_rtn; }
+  push_ctl();
+  pending_dup = !dup;
+}
+
+// bcc_lrup_mxdup (bcc_lrup.c:1128-1138) + bcc_M_mxset (bcc.c:1414-1438): the retained rows
+// and columns become the active sets, and the masks and index lists reach the device
+void Run::st_mxdup()
+{
+  if (!pending_dup) return;
   hAr_j = hAr_rtn; hAc_j = hAc_rtn;
   push_ctl();
+  pending_dup = false;
+}
+
+// one pass's bcc_sumscores_nrm/ifT/srt/cmb/xij/mxA (bcc_sumscores.c:37-779), one bin,
+// Ireq <= 0, with the active sets moved to the retained ones at once
+void Run::sumscores()
+{
+  st_nrm(); st_ifT(); st_srt(); st_cmb(); st_xij(); st_mxA(true);
 }
 
 // a Gram over the full column layout: split over the 1920-blocks when it has few tiles
@@ -1281,28 +1367,50 @@ void Run::gram_any(const GramMArgs& g)
   else gram_mfma(g, st);
 }
 
-// low-rank update after a drop (bcc_lrup.c:5-1138)
-void Run::lrup()
+// the low-rank update's shared quantities (mask_lf slots, strides, the compacted dropped
+// columns, the list sizes of the pass)
+#define LRUP_LOCALS \
+  [[maybe_unused]] const int sA = psize(NA), sZ = psize(NZ); \
+  [[maybe_unused]] double* mlf_rmv = mlf + 1 * 16;     /* column masks */ \
+  [[maybe_unused]] double* mlf_rtn = mlf + 2 * 16; \
+  [[maybe_unused]] double* mlf_nrtn = mlf + 5 * 16;    /* row masks */ \
+  [[maybe_unused]] double* mlf_nrmv = mlf + 6 * 16; \
+  [[maybe_unused]] double* mlf_nz = mlf + 7 * 16; \
+  [[maybe_unused]] const long long rbm = (long long)An.rb * 8, rbn = (long long)At.rb * 8; \
+  [[maybe_unused]] const int W = cmp_W; \
+  [[maybe_unused]] Bits Anc{ (const uint8_t*)An_c, NA, W * 32, (size_t)W * 4 }, Znc{ (const uint8_t*)Zn_c, NZ, W * 32, (size_t)W * 4 }; \
+  [[maybe_unused]] const bool cols_dropped = cmp_nseg > 0; \
+  [[maybe_unused]] int *c_mr = gcnt[0], *c_mm = gcnt[1], *c_mz = gcnt[2]; \
+  [[maybe_unused]] const int nrm = rmv.n(), nrt = rtn.n(), nz = zr.n(); \
+  [[maybe_unused]] double *t_knzt = mt[0], *t_hnvt = mt[1], *t_ankt = mt[2], *t_knkt = mt[3]; \
+  [[maybe_unused]] double *attn = mt[4], *jttn = mt[5], *ktrn = mt[6], *htrn = mt[7], *ztsn = mt[8], *vtsn = mt[9]; \
+  [[maybe_unused]] double *vtszn = mt[10], *jttan = mt[11], *htrkn = mt[12], *hhtrkn = mt[13]; \
+  [[maybe_unused]] double *knzt = (double*)g_knzt.p, *hnvt = (double*)g_hnvt.p, *ankt = (double*)g_ankt.p, *knkt = (double*)g_knkt.p; \
+  [[maybe_unused]] Side s_rmv{ &At, Ar_rmv, &rmv, &Tt }, s_rtn{ &At, Ar_rtn, &rtn, &Tt }, s_Z{ &Zt, Zr_j, &zr, &St }; \
+  [[maybe_unused]] const Packed pk_rmv{ (const uint8_t*)At_rmvc, (const uint8_t*)Tt_rmvc, (size_t)rmv_W * 4, d_cmask_rmv, 32 * rmv_W }; \
+  [[maybe_unused]] const Packed pk_rtn{ (const uint8_t*)At_rtnc, (const uint8_t*)Tt_rtnc, (size_t)rtn_W * 4, d_cmask_rtn, 32 * rtn_W }
+
+// bcc_lrup_QR_YnWt_stage_0 (bcc_lrup.c:659-723): the mask weights of the dropped and the
+// retained columns, the An_ajdk of the retained rows and the controls over the dropped columns
+void Run::lrup_QR_0()
 {
-  const int sA = psize(NA), sZ = psize(NZ);
-  double* mlf_rmv = mlf + 1 * 16;     // column masks
-  double* mlf_rtn = mlf + 2 * 16;
-  double* mlf_nrtn = mlf + 5 * 16;    // row masks
-  double* mlf_nrmv = mlf + 6 * 16;
-  double* mlf_nz = mlf + 7 * 16;
+  LRUP_LOCALS;
   mask_lf(Ac_rmv, pcols, ajdk, mlf_rmv, st);
   mask_lf(Ac_rtn, pcols, ajdk, mlf_rtn, st);
-  const long long rbm = (long long)An.rb * 8;
-  // QR stages 0/1/3: lf_AnZt -= jn.vt, lf_AnAt -= jn.jt over the dropped columns
   an_ajdk(An.b(), rtn.d, rtn.n(), Ac_rmv, pcols, ajdk, mlf_rmv, jn_ajdk, NA, st);
   ops += 15LL * rtn.n() * rbm;
-  // the Grams over the dropped columns run on the column-compacted rows
-  const int W = cmp_W;
-  Bits Anc{ (const uint8_t*)An_c, NA, W * 32, (size_t)W * 4 }, Znc{ (const uint8_t*)Zn_c, NZ, W * 32, (size_t)W * 4 };
   if (has_Z) {
     an_ajdk(Zn.b(), zr.d, zr.n(), Ac_rmv, pcols, ajdk, mlf_rmv, vn_ajdk, NZ, st);
     ops += 15LL * zr.n() * (long long)Zn.rb * 8;
   }
+}
+
+// bcc_lrup_QR_YnWt_stage_1 + _stage_3 (bcc_lrup.c:724-770, 863-892): the Grams of the retained
+// rows with the controls and with themselves over the dropped columns, subtracted from
+// lf_AnZt and lf_AnAt as they are formed (the Grams are not kept: stage 3 has nothing left)
+void Run::lrup_QR_1()
+{
+  LRUP_LOCALS;
   // compacted rows over the dropped columns: gather those columns as rows of the
   // transposed copies (padding positions give zero rows), transpose back
   {
@@ -1315,9 +1423,7 @@ void Run::lrup()
       transpose_bits(Bits{ tmp, 32 * W, NZ, Zt.rb }, (uint8_t*)Zn_c, (size_t)W * 4, st);
     }
   }
-  int *c_mr = gcnt[0], *c_mm = gcnt[1], *c_mz = gcnt[2];
   // with no column dropped (a drop of rows only) the QR updates subtract +0.0: skipped
-  const bool cols_dropped = cmp_nseg > 0;
   if (!cols_dropped) {
   } else if (ww_kernel == 0) {
     seg_counts(Anc, rtn.d, rtn.n(), nullptr, d_seg, cmp_nseg, c_mr, NA, st);
@@ -1340,7 +1446,16 @@ void Run::lrup()
   }
   if (has_Z) ops += (long long)rtn.n() * zr.n() * rbm;
   ops += (long long)rtn.n() * rtn.n() * rbm;
-  // QC a0
+}
+
+// bcc_lrup_QR_YnWt_stage_3 (bcc_lrup.c:863-892): done inside stage 1
+void Run::lrup_QR_3() {}
+
+// bcc_lrup_QC_YnWt_stage_a0 (bcc_lrup.c:5-75): An_ajdk of the removed, the retained rows and
+// the controls over the retained and the dropped columns
+void Run::lrup_QC_a0()
+{
+  LRUP_LOCALS;
   an_ajdk(An.b(), rmv.d, rmv.n(), Ac_rtn, pcols, ajdk, mlf_rtn, kn_ajdk, NA, st);
   an_ajdk(An.b(), rmv.d, rmv.n(), Ac_rmv, pcols, ajdk, mlf_rmv, hn_ajdk, NA, st);
   an_ajdk(An.b(), rtn.d, rtn.n(), Ac_rtn, pcols, ajdk, mlf_rtn, an_ajdk_, NA, st);
@@ -1350,11 +1465,14 @@ void Run::lrup()
     an_ajdk(Zn.b(), zr.d, zr.n(), Ac_rmv, pcols, ajdk, mlf_rmv, vn_ajdk, NZ, st);
     ops += 30LL * zr.n() * (long long)Zn.rb * 8;
   }
-  // QC a1: compact Grams G[j*n1 + i]
-  int nrm = rmv.n(), nrt = rtn.n(), nz = zr.n();
-  double *knzt = nullptr, *hnvt = nullptr;
-  double* ankt = (double*)g_ankt.need(dev, sizeof(double) * ((size_t)nrt * nrm + 1));
-  double* knkt = (double*)g_knkt.need(dev, sizeof(double) * ((size_t)nrm * nrm + 1));
+}
+
+// bcc_lrup_QC_YnWt_stage_a1 (bcc_lrup.c:76-180): the compact Grams knzt, hnvt, ankt, knkt
+void Run::lrup_QC_a1()
+{
+  LRUP_LOCALS;
+  ankt = (double*)g_ankt.need(dev, sizeof(double) * ((size_t)nrt * nrm + 1));
+  knkt = (double*)g_knkt.need(dev, sizeof(double) * ((size_t)nrm * nrm + 1));
   if (has_Z) {
     knzt = (double*)g_knzt.need(dev, sizeof(double) * ((size_t)nrm * nz + 1));
     hnvt = (double*)g_hnvt.need(dev, sizeof(double) * ((size_t)nrm * nz + 1));
@@ -1432,27 +1550,49 @@ void Run::lrup()
   // the Gram of the removed rows and the controls over no dropped column: every entry
   // is the reference's empty-mask value, +0.0
   if (has_Z && !cols_dropped && nrm > 0 && nz > 0) fill(hnvt, (long long)nrm * nz, 0.0, st);
-  // QC a2 + a3
-  double *t_knzt = mt[0], *t_hnvt = mt[1], *t_ankt = mt[2], *t_knkt = mt[3];
-  Side s_rmv{ &At, Ar_rmv, &rmv, &Tt }, s_rtn{ &At, Ar_rtn, &rtn, &Tt }, s_Z{ &Zt, Zr_j, &zr, &St };
+}
+
+// the update's four At_T_YnWt_S_Zn terms (bcc_lrup.c:238-303): knzt and hnvt (with Z),
+// ankt and knkt, each over the retained columns
+const WwTerm* Run::lrup_terms()
+{
+  LRUP_LOCALS;
+  lrup_w[0] = WwTerm{ trm_knzt, s_rmv, s_Z, &crt, knzt, 1, nrm, 0, t_knzt, trm_knzt ? Packed{} : pk_rmv };
+  lrup_w[1] = WwTerm{ trm_hnvt, s_rmv, s_Z, &crt, hnvt, 1, nrm, 0, t_hnvt, trm_hnvt ? Packed{} : pk_rmv };
+  lrup_w[2] = WwTerm{ 0, s_rtn, s_rmv, &crt, ankt, 1, nrt, 0, t_ankt, pk_rtn };
+  lrup_w[3] = WwTerm{ 0, s_rmv, s_rmv, &crt, knkt, 1, nrm, 0, t_knkt, pk_rmv };
+  return lrup_w;
+}
+
+// bcc_lrup_QC_YnWt_stage_a2 (bcc_lrup.c:181-237): the four Grams quantised (xcalc), each into
+// its own slot of integers and scalars
+void Run::lrup_QC_a2()
+{
+  LRUP_LOCALS;
   // the removed and the retained A rows packed to the front of the patient dimension
   if (ww_kernel == 0) {
     pack_rows(d_src_rmv, rmv_W, At_rmvc, Tt_rmvc);
     pack_rows(d_src_rtn, rtn_W, At_rtnc, Tt_rtnc);
   }
-  const Packed pk_rmv{ (const uint8_t*)At_rmvc, (const uint8_t*)Tt_rmvc, (size_t)rmv_W * 4, d_cmask_rmv, 32 * rmv_W };
-  const Packed pk_rtn{ (const uint8_t*)At_rtnc, (const uint8_t*)Tt_rtnc, (size_t)rtn_W * 4, d_cmask_rtn, 32 * rtn_W };
-  if (has_Z) {
-    ww_excerpt(trm_knzt, s_rmv, s_Z, crt, knzt, 1, nrm, 0, t_knzt, trm_knzt ? nullptr : &pk_rmv);
-    ww_excerpt(trm_hnvt, s_rmv, s_Z, crt, hnvt, 1, nrm, 0, t_hnvt, trm_hnvt ? nullptr : &pk_rmv);
-  }
-  ww_excerpt(0, s_rtn, s_rmv, crt, ankt, 1, nrt, 0, t_ankt, &pk_rtn);
-  ww_excerpt(0, s_rmv, s_rmv, crt, knkt, 1, nrm, 0, t_knkt, &pk_rmv);
-  // QC b0: plain +-1 products, [t*M + k]
-  double *attn = mt[4], *jttn = mt[5], *ktrn = mt[6], *htrn = mt[7], *ztsn = mt[8], *vtsn = mt[9];
+  const WwTerm* w = lrup_terms();
+  for (int i = 0; i < 4; i++) if (has_Z || i >= 2) ww_quantise(i, w[i]);
+}
+
+// bcc_lrup_QC_YnWt_stage_a3 (bcc_lrup.c:238-303): the four At_T_YnWt_S_Zn_ww contractions
+void Run::lrup_QC_a3()
+{
+  LRUP_LOCALS;
+  const WwTerm* w = lrup_terms();
+  for (int i = 0; i < 4; i++) if (has_Z || i >= 2) ww_contract(i, w[i]);
+}
+
+// bcc_lrup_QC_YnWt_stage_b0 (bcc_lrup.c:304-379): plain +-1 products of the retained and the
+// dropped columns with the covariates, [t*M + k]
+void Run::lrup_QC_b0()
+{
+  LRUP_LOCALS;
   mask_lf(Ar_rtn, sA, nullptr, mlf_nrtn, st);
   mask_lf(Ar_rmv, sA, nullptr, mlf_nrmv, st);
-  const long long rbn = (long long)At.rb * 8;
   if (ww_kernel == 0) {
     // over the packed retained / removed patients (the counts are the same integers)
     const Bits Ptn{ (const uint8_t*)At_rtnc, M, 32 * rtn_W, (size_t)rtn_W * 4 }, Ttn{ (const uint8_t*)Tt_rtnc, T, 32 * rtn_W, (size_t)rtn_W * 4 };
@@ -1474,7 +1614,13 @@ void Run::lrup()
     pm_few(Zt.b(), crm.d, crm.n(), St.b(), tl.d, tl.n(), Zr_j, sZ, mlf_nz, vtsn, M, st);
     ops += (long long)(crt.n() + crm.n()) * tl.n() * (long long)Zt.rb * 8;
   }
-  double *vtszn = mt[10], *jttan = mt[11], *htrkn = mt[12], *hhtrkn = mt[13];
+}
+
+// bcc_lrup_QC_YnWt_stage_b1 (bcc_lrup.c:380-480): the AtTYn_vv triples; on the matrix cores
+// fused with stage b2's ordered sums (the triples are then not kept)
+void Run::lrup_QC_b1()
+{
+  LRUP_LOCALS;
   if (crm.n() == 0) {
     // no dropped column: the stage-b2 sums over the dropped columns are empty, +0.0
     ops += (long long)tl.n() * crt.n() * crm.n() * (rbn * 2 + (has_Z ? (long long)Zt.rb * 8 : 0));
@@ -1520,18 +1666,45 @@ void Run::lrup()
   attyn(At.b(), crt.d, crt.n(), Ar_rmv, nrm, Tt.b(), tl.d, tl.n(), Ar_rmv, nrm, At.b(), crm.d, crm.n(),
         ktrn, htrn, M, ajdk, pcols, ktrhn, st);
   ops += (long long)tl.n() * crt.n() * crm.n() * (rbn * 2 + (has_Z ? (long long)Zt.rb * 8 : 0));
-  // QC b2
+  }
+}
+
+// bcc_lrup_QC_YnWt_stage_b2 (bcc_lrup.c:481-590): the ordered WtSZn sums of the triples
+void Run::lrup_QC_b2()
+{
+  LRUP_LOCALS;
+  // with the matrix-core kernels, and with no dropped column, stage b1 has formed the sums
+  if (ww_kernel == 0 || crm.n() == 0) return;
+  const double* ztsvn = (const double*)g_ztsvn.p;
+  const double* attjn = (const double*)g_attjn.p;
+  const double* ktrhn = (const double*)g_ktrhn.p;
   if (has_Z) attyn_wtszn(attjn, ztsvn, crt.d, crt.n(), crm.d, crm.n(), tl.d, tl.n(), ajdk, pcols, vtszn, M, st);
   attyn_wtszn(attjn, attjn, crt.d, crt.n(), crm.d, crm.n(), tl.d, tl.n(), ajdk, pcols, jttan, M, st);
   attyn_wtszn(attjn, ktrhn, crt.d, crt.n(), crm.d, crm.n(), tl.d, tl.n(), ajdk, pcols, htrkn, M, st);
   attyn_wtszn(ktrhn, ktrhn, crt.d, crt.n(), crm.d, crm.n(), tl.d, tl.n(), ajdk, pcols, hhtrkn, M, st);
-  }
-  // QC stage c
+}
+
+// bcc_lrup_QC_YnWt_stage_c (bcc_lrup.c:592-655): L2_clean of the column scores, then the
+// low-rank terms subtracted
+void Run::lrup_QC_c()
+{
+  LRUP_LOCALS;
   StageC s;
   s.M = M; s.T = T; s.has_Z = has_Z; s.Ac_rtn = Ac_rtn; s.Ac_b = Ac_b; s.Tc_j = Tc_j;
   s.vtszn = vtszn; s.t_knzt = t_knzt; s.t_hnvt = t_hnvt; s.jttan = jttan; s.htrkn = htrkn; s.hhtrkn = hhtrkn;
   s.t_ankt = t_ankt; s.t_knkt = t_knkt; s.ZSZ = lf_AtTAnZtSZn; s.TAT = lf_AtTAnAtTAn;
   stage_c(s, st);
+}
+
+// the whole low-rank update after a drop (bcc_lrup.c:5-1138), in the driver's order
+// (lakcluster_driver.c:185-203); stage c, which the driver calls after the re-quantisation
+// of the Grams (bcc_M_YnWt), reads nothing that re-quantisation changes
+void Run::lrup()
+{
+  lrup_QR_0(); lrup_QR_1(); lrup_QR_3();
+  lrup_QC_a0(); lrup_QC_a1(); lrup_QC_a2(); lrup_QC_a3();
+  lrup_QC_b0(); lrup_QC_b1(); lrup_QC_b2();
+  lrup_QC_c();
 }
 
 bool Run::active() const { return !ar.h.empty() || !kl.h.empty(); }
@@ -1748,6 +1921,7 @@ void Run::reset()
   HIPCHK(hipStreamSynchronize(st));
   trace.clear(); xdrop_a.clear(); xdrop_b.clear();
   n_iter = 0; inited = false; ops = 0; init_s = 0; loop_s = 0; n_sort_fallback = 0;
+  xij_done = false; pending_dup = false;
   for (auto& k : prof.stats) { k.launches = 0; k.ms = 0; k.work = 0; }
 }
 

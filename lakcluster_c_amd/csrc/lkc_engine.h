@@ -68,7 +68,10 @@ struct IList { std::vector<int> h; int* d = nullptr; int n() const { return (int
 struct Side { const DMat* X; const uint32_t* mask; const IList* rows; const DMat* Cov; };
 // the same group with its patients packed to the front: SNP-major rows and covariate rows of
 // rb bytes whose first nbits positions are the group's rows in list order (mask: those in use)
-struct Packed { const uint8_t* X; const uint8_t* Cov; size_t rb; const uint32_t* mask; int nbits; };
+struct Packed { const uint8_t* X = nullptr; const uint8_t* Cov = nullptr; size_t rb = 0; const uint32_t* mask = nullptr; int nbits = 0; };
+// one At_T_YnWt_S_Zn term: L(r1, r2) = src[idx1(r1)*st1 + idx2(r2)*st2] over the sides' rows, the
+// n side (s2 when trm, else s1) contracted, the result into out[t*M + k] for k in kls
+struct WwTerm { int trm; Side s1, s2; const IList* kls; const double* src; long long st1, st2; int full; double* out; Packed pk; };
 
 // HIP-event timing of named kernel groups on the run's stream
 struct Prof {
@@ -136,6 +139,8 @@ struct Run {
   double *At_An_a1d1 = nullptr, *et_An = nullptr, *An_a2d2_ZtSn = nullptr, *An_a2d2_AtTn = nullptr;
   double *sc = nullptr, *T_AnAt_AnAt = nullptr;
   double *QR_ZSZ = nullptr, *QR_TAT = nullptr, *QC_ZSZ = nullptr, *QC_TAT = nullptr;
+  double *QR_ZSZ_nrm = nullptr, *QR_TAT_nrm = nullptr, *QC_ZSZ_nrm = nullptr, *QC_TAT_nrm = nullptr;   // [t*rows + r]
+  bool xij_done = false, pending_dup = false;   // stage order within a pass
   double *mt[14] = {};
   int* cnt_buf = nullptr;
   double *xwork = nullptr, *scal = nullptr, *dsv = nullptr, *hsv = nullptr;
@@ -143,6 +148,8 @@ struct Run {
   void* sort_tmp = nullptr; size_t sort_tmp_bytes = 0;
   long n_sort_fallback = 0;                 // passes that needed the host quicksort
   Grow g_cz, g_planes, g_work, g_L, g_knzt, g_hnvt, g_ankt, g_knkt, g_ztsvn, g_attjn, g_ktrhn;
+  Grow g_czs[4], g_planess[4];            // the quantised values of the update's four ww terms
+  double* scal_s = nullptr;               // and their [min, mlt] scalars, 16 doubles each
   // host sort state and outputs
   std::vector<double> QC_sv, QR_sv; std::vector<int> QC_perm, QR_perm;
   std::vector<double> trace; std::vector<int> xdrop_a, xdrop_b;
@@ -167,11 +174,19 @@ struct Run {
   void ztsn_weighted(const double* w, double* outZ, double* outA);
   void ww_excerpt(int trm, const Side& s1, const Side& s2, const IList& kls, const double* src, long long st1, long long st2,
                   int full, double* out, const Packed* pk = nullptr);
+  void ww_quantise(int slot, const WwTerm& w);
+  void ww_contract(int slot, const WwTerm& w);
   void pack_rows(const int* src, int W, uint32_t* Atc, uint32_t* Ttc);
   void init_lf();
   void score_lists();
+  void st_nrm(); void st_ifT(); void st_srt(); void st_cmb(); void st_xij(); void st_mxA(bool dup); void st_mxdup();
   void sumscores();
   void lrup();
+  void lrup_QR_0(); void lrup_QR_1(); void lrup_QR_3();
+  void lrup_QC_a0(); void lrup_QC_a1(); void lrup_QC_a2(); void lrup_QC_a3();
+  void lrup_QC_b0(); void lrup_QC_b1(); void lrup_QC_b2(); void lrup_QC_c();
+  const WwTerm* lrup_terms();
+  WwTerm lrup_w[4] = {};
   void sb_drop(const std::vector<int>& rord, const std::vector<int>& cord, int rdrop, int cdrop);
   void sb_record(Scorebox& S, int tab);
   void scorebox(Scorebox& S);

@@ -37,12 +37,12 @@ struct FlatQC {
 };
 void flat_qc(const FlatQC& f, hipStream_t st);
 
-struct Scores {
-  const int* list; int n; const int* tl; int nt; long long ld; int has_Z;
-  const double* ZSZ; const double* TAT; double dZ; double dT; double den; int pow_form; int cmb;
-  double* out;
-};
-void scores(const Scores& s, hipStream_t st);
+// the score stages over the listed rows (or columns) and active covariates; arrays [t*ld + r]
+struct ScoreStage { const int* list; int n; const int* tl; int nt; long long ld; };
+void score_nrm(const ScoreStage& s, const double* src, double den, double* dst, hipStream_t st);
+void score_ifT(const ScoreStage& s, double* v, double den, int pow_form, hipStream_t st);
+void score_cmb(const ScoreStage& s, double* x, const double* z, hipStream_t st);
+void score_gather0(const ScoreStage& s, const double* x, double* out, hipStream_t st);
 
 struct StageC {
   int M; int T; int has_Z; const uint32_t* Ac_rtn; const uint32_t* Ac_b; const uint32_t* Tc_j;

@@ -206,44 +206,70 @@ void flat_qc(const FlatQC& f, hipStream_t st)
 }
 
 // ------------------------------------------------------------------
-// bcc_sumscores_nrm / _ifT / _cmb (bcc_sumscores.c:324-356, 483-548, 550-779) for one
-// bin, followed by the xij gather of column 0: out[i] = score of list[i].
-// pow_form: the QR rows' root-of-squares form; else the QC rows' subtractive form.
+// bcc_sumscores_nrm / _ifT / _cmb (bcc_sumscores.c:324-779; dcc_sumscores.c:248-699), one
+// bin, GLOBAL_Ireq <= 0, over the listed rows (or columns) r and the active covariates:
+//   nrm: dst[t*ld + r] = 0.0 + src[t*ld + r] / den (dra_plusdivequals into a zeroed array)
+//   ifT: column 0 corrected by the others (dra_mds_pow / dra_mds_nrm, ra_stats.c:77-104)
+//   cmb: x[0*ld + r] += -1.0 * z[0*ld + r]
+// then xij's gather of column 0 into list order.
 // ------------------------------------------------------------------
-__global__ void k_scores(Scores s)
+__global__ void k_nrm(ScoreStage s, const double* src, double den, double* dst)
 {
-  int i = blockIdx.x * 256 + threadIdx.x;
+  const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= s.n) return;
-  int r = s.list[i];
-  double z[kMaxT], x[kMaxT];
-  for (int t = 0; t < kMaxT; t++) { z[t] = 0.0; x[t] = 0.0; }
+  const int r = s.list[i];
   for (int t = 0; t < s.nt; t++) {
-    int tt = s.tl[t];
-    if (s.has_Z) z[tt] = 0.0 + s.ZSZ[(long long)tt * s.ld + r] / s.dZ;
-    x[tt] = 0.0 + s.TAT[(long long)tt * s.ld + r] / s.dT;
+    const long long o = (long long)s.tl[t] * s.ld + r;
+    dst[o] = 0.0 + src[o] / den;
   }
-  if (s.den > 0) {
-    for (int pass = 0; pass < 2; pass++) {
-      double* v = pass == 0 ? z : x;
-      if (pass == 0 && !s.has_Z) continue;
-      if (s.pow_form) {
-        if (v[0] >= 0) {
-          v[0] *= v[0];
-          for (int c = 0; c < s.nt; c++) if (s.tl[c] >= 1) v[0] -= v[s.tl[c]] * v[s.tl[c]] / s.den;
-          v[0] = v[0] >= 0 ? sqrt(v[0]) : -sqrt(-v[0]);
-        }
-      } else {
-        for (int c = 0; c < s.nt; c++) if (s.tl[c] >= 1) v[0] -= v[s.tl[c]] / s.den;
-      }
-    }
-  }
-  if (s.cmb) x[0] += -1.0 * z[0];
-  s.out[i] = x[0];
 }
-void scores(const Scores& s, hipStream_t st)
+__global__ void k_ifT(ScoreStage s, double* v_, double den, int pow_form)
 {
-  if (s.n <= 0) return;
-  hipLaunchKernelGGL(k_scores, dim3(cdiv(s.n, 256)), dim3(256), 0, st, s);
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= s.n) return;
+  const int r = s.list[i];
+  double* v = v_ + r;                      // v[t * ld]
+  const long long ld = s.ld;
+  if (pow_form) {
+    if (v[0] >= 0) {
+      double v0 = v[0] * v[0];
+      for (int c = 0; c < s.nt; c++) if (s.tl[c] >= 1) v0 -= v[s.tl[c] * ld] * v[s.tl[c] * ld] / den;
+      v[0] = v0 >= 0 ? sqrt(v0) : -sqrt(-v0);
+    }
+  } else {
+    double v0 = v[0];
+    for (int c = 0; c < s.nt; c++) if (s.tl[c] >= 1) v0 -= v[s.tl[c] * ld] / den;
+    v[0] = v0;
+  }
+}
+__global__ void k_cmb(ScoreStage s, double* x, const double* z)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= s.n) return;
+  const int r = s.list[i];
+  x[r] += -1.0 * z[r];
+}
+__global__ void k_gather0(ScoreStage s, const double* x, double* out)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= s.n) return;
+  out[i] = x[s.list[i]];
+}
+void score_nrm(const ScoreStage& s, const double* src, double den, double* dst, hipStream_t st)
+{
+  if (s.n > 0) hipLaunchKernelGGL(k_nrm, dim3(cdiv(s.n, 256)), dim3(256), 0, st, s, src, den, dst);
+}
+void score_ifT(const ScoreStage& s, double* v, double den, int pow_form, hipStream_t st)
+{
+  if (s.n > 0) hipLaunchKernelGGL(k_ifT, dim3(cdiv(s.n, 256)), dim3(256), 0, st, s, v, den, pow_form);
+}
+void score_cmb(const ScoreStage& s, double* x, const double* z, hipStream_t st)
+{
+  if (s.n > 0) hipLaunchKernelGGL(k_cmb, dim3(cdiv(s.n, 256)), dim3(256), 0, st, s, x, z);
+}
+void score_gather0(const ScoreStage& s, const double* x, double* out, hipStream_t st)
+{
+  if (s.n > 0) hipLaunchKernelGGL(k_gather0, dim3(cdiv(s.n, 256)), dim3(256), 0, st, s, x, out);
 }
 
 // ------------------------------------------------------------------
