@@ -95,18 +95,73 @@ int lkc_bcc_is_scorebox(const lkc_bcc *D);
    iterate and run then follow dexcluster_driver (dexcluster_driver.c:22-53) */
 int lkc_bcc_is_dcc(const lkc_bcc *D);
 
-/* The individual stages, in the order lakcluster_driver calls them.  They act on
-   the device-resident state of D (lakcluster_header.h:394-502 names). */
+/* The individual stages, in the order lakcluster_driver calls them
+   (lakcluster_driver.c:73-246; declared at lakcluster_header.h:413-502).  They act on
+   the device-resident state of D.  The initial pass, in this order: */
 int lkc_bcc_An_ajdk(lkc_bcc *D);                 /* bcc_lf.c:7-51 */
 int lkc_bcc_lf_ZtSn(lkc_bcc *D);                 /* bcc_lf.c:53-101 */
 int lkc_bcc_lf_YnWt(lkc_bcc *D);                 /* bcc_lf.c:223-302 */
-int lkc_bcc_lf_At_T_YnWt_S_Zn(lkc_bcc *D);       /* bcc_lf.c:318-356 + 636-709 */
+int lkc_bcc_lf_At_T_YnWt_S_Zn(lkc_bcc *D);       /* bcc_lf.c:318-356 + 636-709 (bcc_M_YnWt inside) */
 int lkc_bcc_lf_AnZt_S_WnYt(lkc_bcc *D);          /* bcc_lf.c:360-409 */
 int lkc_bcc_singlestudy_ww(lkc_bcc *D);          /* bcc_flattenloop.c:753-831 */
 int lkc_bcc_doublestudy_ww(lkc_bcc *D);          /* bcc_flattenloop.c:977-1029 */
-int lkc_bcc_flattenloop(lkc_bcc *D);             /* bcc_flattenloop.c:1431-1889 */
-int lkc_bcc_sumscores(lkc_bcc *D);               /* nrm, ifT, cmb, xij, mxA: bcc_sumscores.c:37-779 */
-int lkc_bcc_lrup(lkc_bcc *D);                    /* bcc_lrup.c:5-1138 incl. bcc_lrup_mxdup */
+int lkc_bcc_flattenloop(lkc_bcc *D);             /* bcc_flattenloop.c:1431-1889; ends the initial pass */
+
+/* One pass of the loop, while lkc_bcc_active(D) (the driver's
+   `while (D->A_rpop_j_total>0 || D->A_cpop_j>0)`), in this order; each returns
+   LKC_ERR_STATE before the initial pass.  With one bin and GLOBAL_Ireq <= 0: */
+int lkc_bcc_active(const lkc_bcc *D);            /* 1 while a row or a column is active */
+int lkc_bcc_sumscores_nrm(lkc_bcc *D);           /* bcc_sumscores.c:550-779 */
+int lkc_bcc_sumscores_ifT(lkc_bcc *D);           /* bcc_sumscores.c:483-548 */
+int lkc_bcc_sumscores_srt(lkc_bcc *D);           /* bcc_sumscores.c:358-481: nothing to do with Ireq <= 0 */
+int lkc_bcc_sumscores_cmb(lkc_bcc *D);           /* bcc_sumscores.c:324-356 */
+int lkc_bcc_sumscores_xij(lkc_bcc *D);           /* bcc_sumscores.c:225-322 with dQuickSort_xij */
+int lkc_bcc_sumscores_mxA(lkc_bcc *D);           /* bcc_sumscores.c:159-224 with _mxB, _mxC (37-125), bcc_lrup_mxset;
+                                                    LKC_ERR_STATE unless _xij ran in this pass */
+int lkc_bcc_sumscores_dmp(const lkc_bcc *D, const char *out_dir);   /* bcc_sumscores.c:127-157 */
+int lkc_bcc_lrup_QR_YnWt_stage_0(lkc_bcc *D);    /* bcc_lrup.c:659-701 */
+int lkc_bcc_lrup_QR_YnWt_stage_1(lkc_bcc *D);    /* bcc_lrup.c:702-771, with stage 3's subtraction */
+int lkc_bcc_lrup_QR_YnWt_stage_3(lkc_bcc *D);    /* bcc_lrup.c:863-895: done inside stage 1 */
+int lkc_bcc_lrup_QC_ZtSWn_stage_0(lkc_bcc *D);   /* bcc_lrup.c:896-938: no-op for QC_strategy YnWt */
+int lkc_bcc_lrup_QC_ZtSWn_stage_1(lkc_bcc *D);   /* bcc_lrup.c:939-978: no-op for QC_strategy YnWt */
+int lkc_bcc_lrup_QC_ZtSWn_stage_2(lkc_bcc *D);   /* bcc_lrup.c:979-1058: no-op for QC_strategy YnWt */
+int lkc_bcc_lrup_QC_YnWt_stage_a0(lkc_bcc *D);   /* bcc_lrup.c:5-66 */
+int lkc_bcc_lrup_QC_YnWt_stage_a1(lkc_bcc *D);   /* bcc_lrup.c:67-196 */
+int lkc_bcc_lrup_QC_YnWt_stage_a2(lkc_bcc *D);   /* bcc_lrup.c:197-237 (the Grams quantised) */
+int lkc_bcc_lrup_QC_YnWt_stage_a3(lkc_bcc *D);   /* bcc_lrup.c:238-304 */
+int lkc_bcc_lrup_QC_YnWt_stage_b0(lkc_bcc *D);   /* bcc_lrup.c:305-362 */
+int lkc_bcc_lrup_QC_YnWt_stage_b1(lkc_bcc *D);   /* bcc_lrup.c:363-408; on the matrix cores it also
+                                                    forms stage b2's sums (the triples are not kept) */
+int lkc_bcc_lrup_QC_YnWt_stage_b2(lkc_bcc *D);   /* bcc_lrup.c:409-482 */
+int lkc_bcc_lrup_mxdup(lkc_bcc *D);              /* bcc_lrup.c:1128-1139 */
+int lkc_bcc_M_mxset(lkc_bcc *D);                 /* bcc.c:1414-1438: the counts follow the masks
+                                                    that lkc_bcc_lrup_mxdup uploads; no-op */
+int lkc_bcc_M_ZtSWn(lkc_bcc *D);                 /* bcc_lf.c:176-222: no-op for QC_strategy YnWt */
+int lkc_bcc_M_YnWt(lkc_bcc *D);                  /* bcc_lf.c:318-359: each Gram is quantised where it
+                                                    is contracted (stage a2, lf_AnZt_S_WnYt); no-op */
+int lkc_bcc_lf_AtTYn____WtSZn(lkc_bcc *D);       /* bcc_lf.c:566-635: no-op for QC_strategy YnWt */
+int lkc_bcc_lrup_QC_YnWt_stage_c(lkc_bcc *D);    /* bcc_lrup.c:592-655 */
+/* then lkc_bcc_lf_AnZt_S_WnYt, lkc_bcc_An_ajdk, lkc_bcc_lf_ZtSn, lkc_bcc_singlestudy_ww,
+   lkc_bcc_doublestudy_ww, lkc_bcc_flattenloop (lakcluster_driver.c:225-245). */
+
+/* Grouped: lkc_bcc_sumscores = nrm, ifT, srt, cmb, xij, mxA and bcc_lrup_mxdup;
+   lkc_bcc_lrup = the QR and QC stages above through stage c, then bcc_lrup_mxdup if
+   still pending (the update reads the _rtn / _rmv sets, not the active ones). */
+int lkc_bcc_sumscores(lkc_bcc *D);
+int lkc_bcc_lrup(lkc_bcc *D);
+
+/* get_xdrop (xdrop.c:5-64): the rows and columns one pass drops from lrij rows and
+   lcij columns, with gamma and gamma_type passed in (GLOBAL_gamma, GLOBAL_gamma_type:
+   0 log, 1 sqrt, 2 lin, 3 rows); either output may be NULL. */
+int lkc_get_xdrop(double lrij, double lcij, double gamma, int gamma_type, int *rdrop, int *cdrop);
+/* get_xdrop_length (xdrop.c:66-83; declared get_xdrop_total at lakcluster_header.h:748):
+   the number of passes; negative on a non-finite count */
+int lkc_get_xdrop_length(double lrij, double lcij, double gamma, int gamma_type);
+/* get_xdrop_array (xdrop.c:85-111) into caller arrays of cap entries (each may be NULL):
+   the drops and the counts kept after each pass; *length = the number of passes.  With
+   every array NULL only *length is set; LKC_ERR_STATE when cap < *length. */
+int lkc_get_xdrop_array(double lrij, double lcij, double gamma, int gamma_type, int cap, int *length,
+                        int *rdrop, int *cdrop, int *rkeep, int *ckeep);
 
 /* ---- results ------------------------------------------------------------- */
 int lkc_bcc_n_iter(const lkc_bcc *D);

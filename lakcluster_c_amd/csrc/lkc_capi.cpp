@@ -51,10 +51,81 @@ STAGE(lkc_bcc_lf_At_T_YnWt_S_Zn, D->run.st_At_T())
 STAGE(lkc_bcc_lf_AnZt_S_WnYt, D->run.st_QR())
 STAGE(lkc_bcc_singlestudy_ww, D->run.st_single())
 STAGE(lkc_bcc_doublestudy_ww, D->run.st_double())
-STAGE(lkc_bcc_flattenloop, D->run.st_flatten())
-STAGE(lkc_bcc_sumscores, D->run.sumscores())
-STAGE(lkc_bcc_lrup, D->run.lrup())
+// the initial pass is complete once bcc_flattenloop has run (lakcluster_driver.c:104)
+STAGE(lkc_bcc_flattenloop, D->run.st_flatten(); D->run.inited = true)
 #undef STAGE
+
+// the stages of one pass of the elimination loop (lakcluster_driver.c:107-246); they need
+// the initial pass
+#define LOOP_STAGE(fn, call) int fn(lkc_bcc* D) { if (!D) return LKC_ERR_STATE; return guard(D, [&] { \
+  if (!D->run.inited) throw lkc::Error(LKC_ERR_STATE, #fn " needs the initial pass (through bcc_flattenloop)"); \
+  call; }); }
+LOOP_STAGE(lkc_bcc_sumscores, D->run.sumscores())
+LOOP_STAGE(lkc_bcc_lrup, D->run.lrup(); D->run.st_mxdup())
+LOOP_STAGE(lkc_bcc_sumscores_nrm, D->run.st_nrm())
+LOOP_STAGE(lkc_bcc_sumscores_ifT, D->run.st_ifT())
+LOOP_STAGE(lkc_bcc_sumscores_srt, D->run.st_srt())
+LOOP_STAGE(lkc_bcc_sumscores_cmb, D->run.st_cmb())
+LOOP_STAGE(lkc_bcc_sumscores_xij, D->run.st_xij())
+LOOP_STAGE(lkc_bcc_sumscores_mxA, D->run.st_mxA(false))
+LOOP_STAGE(lkc_bcc_lrup_QR_YnWt_stage_0, D->run.lrup_QR_0())
+LOOP_STAGE(lkc_bcc_lrup_QR_YnWt_stage_1, D->run.lrup_QR_1())
+LOOP_STAGE(lkc_bcc_lrup_QR_YnWt_stage_3, D->run.lrup_QR_3())
+LOOP_STAGE(lkc_bcc_lrup_QC_ZtSWn_stage_0, (void)0)
+LOOP_STAGE(lkc_bcc_lrup_QC_ZtSWn_stage_1, (void)0)
+LOOP_STAGE(lkc_bcc_lrup_QC_ZtSWn_stage_2, (void)0)
+LOOP_STAGE(lkc_bcc_lrup_QC_YnWt_stage_a0, D->run.lrup_QC_a0())
+LOOP_STAGE(lkc_bcc_lrup_QC_YnWt_stage_a1, D->run.lrup_QC_a1())
+LOOP_STAGE(lkc_bcc_lrup_QC_YnWt_stage_a2, D->run.lrup_QC_a2())
+LOOP_STAGE(lkc_bcc_lrup_QC_YnWt_stage_a3, D->run.lrup_QC_a3())
+LOOP_STAGE(lkc_bcc_lrup_QC_YnWt_stage_b0, D->run.lrup_QC_b0())
+LOOP_STAGE(lkc_bcc_lrup_QC_YnWt_stage_b1, D->run.lrup_QC_b1())
+LOOP_STAGE(lkc_bcc_lrup_QC_YnWt_stage_b2, D->run.lrup_QC_b2())
+LOOP_STAGE(lkc_bcc_lrup_QC_YnWt_stage_c, D->run.lrup_QC_c())
+LOOP_STAGE(lkc_bcc_lrup_mxdup, D->run.st_mxdup())
+LOOP_STAGE(lkc_bcc_M_mxset, (void)0)
+LOOP_STAGE(lkc_bcc_M_ZtSWn, (void)0)
+LOOP_STAGE(lkc_bcc_M_YnWt, (void)0)
+LOOP_STAGE(lkc_bcc_lf_AtTYn____WtSZn, (void)0)
+#undef LOOP_STAGE
+
+int lkc_bcc_sumscores_dmp(const lkc_bcc* D, const char* out_dir) { return lkc_bcc_write_outputs(D, out_dir); }
+
+int lkc_bcc_active(const lkc_bcc* D) { return D && D->run.active() ? 1 : 0; }
+
+int lkc_get_xdrop(double lrij, double lcij, double gamma, int gamma_type, int* rdrop, int* cdrop)
+{
+  if (!std::isfinite(lrij) || !std::isfinite(lcij)) { g_last_error = "get_xdrop: non-finite row or column count"; return LKC_ERR_STATE; }
+  lkc::get_xdrop(lrij, lcij, gamma, gamma_type, rdrop, cdrop);
+  return LKC_OK;
+}
+
+int lkc_get_xdrop_length(double lrij, double lcij, double gamma, int gamma_type)
+{
+  if (!std::isfinite(lrij) || !std::isfinite(lcij)) { g_last_error = "get_xdrop_length: non-finite row or column count"; return LKC_ERR_STATE; }
+  return lkc::get_xdrop_length(lrij, lcij, gamma, gamma_type);
+}
+
+int lkc_get_xdrop_array(double lrij, double lcij, double gamma, int gamma_type, int cap, int* length, int* rdrop,
+                        int* cdrop, int* rkeep, int* ckeep)
+{
+  const int n = lkc_get_xdrop_length(lrij, lcij, gamma, gamma_type);
+  if (n < 0) return n;
+  if (length) *length = n;
+  if (!rdrop && !cdrop && !rkeep && !ckeep) return LKC_OK;
+  if (cap < n) { g_last_error = "get_xdrop_array: the arrays hold fewer entries than the schedule"; return LKC_ERR_STATE; }
+  double r = lrij, c = lcij;
+  for (int i = 0; i < n; i++) {
+    int rd = 0, cd = 0;
+    lkc::get_xdrop(r, c, gamma, gamma_type, &rd, &cd);
+    r -= rd; c -= cd;
+    if (rdrop) rdrop[i] = rd;
+    if (cdrop) cdrop[i] = cd;
+    if (rkeep) rkeep[i] = (int)std::round(r);
+    if (ckeep) ckeep[i] = (int)std::round(c);
+  }
+  return LKC_OK;
+}
 
 int lkc_bcc_iterate(lkc_bcc* D, int* done)
 {

@@ -215,6 +215,20 @@ void get_xdrop(double lrij, double lcij, double gamma, int gamma_type, int* rdro
   if (cdrop_p) *cdrop_p = cdrop;
 }
 
+// get_xdrop_length (xdrop.c:66-83): passes until the rows or the columns run out
+int get_xdrop_length(double lrij, double lcij, double gamma, int gamma_type)
+{
+  int length = 0;
+  for (bool more = true; more;) {
+    length++;
+    int rdrop = 0, cdrop = 0;
+    get_xdrop(lrij, lcij, gamma, gamma_type, &rdrop, &cdrop);
+    lrij -= rdrop; lcij -= cdrop;
+    more = lrij > 0 && lcij > 0;
+  }
+  return length;
+}
+
 // Hoare partition around the first element; equal keys stay left of the pivot.
 static int partition_first(double* d, int* perm, int l, int r)
 {
@@ -1304,7 +1318,7 @@ void Run::st_xij()
   xij_done = true;
 }
 
-// bcc_sumscores_mxA / _mxB / _mxC (bcc_sumscores.c:37-205): get_xdrop's counts, the
+// bcc_sumscores_mxA (bcc_sumscores.c:159-224) with _mxB / _mxC (37-125): get_xdrop's counts, the
 // first cdrop columns and rdrop rows of the sorted order leave (_rmv / _rtn masks), the
 // trace line and the xdrop lines; bcc_lrup_mxset's masks reach the device.  With dup, the
 // retained sets also become the active ones at once (bcc_lrup_mxdup, which the update
@@ -1329,6 +1343,7 @@ void Run::st_mxA(bool dup)
   double mc = mean(QC_sv, (unsigned long long)cpop_i); if (!std::isfinite(mc)) mc = 1.0;
   double tr[6] = { (double)n_iter, (double)rpop, (double)cpop_i, mr, mc, (double)Irem };
   trace.insert(trace.end(), tr, tr + 6);
+  n_iter++;                        // D->out_iteration (bcc_sumscores_mxC)
   for (int i = 0; i < nr_drop; i++) {
     int a = ar.h[QR_perm[i]];
     xdrop_a.push_back(a); xdrop_a.push_back(-1); xdrop_b.push_back(Ar_bidx[a]); xdrop_b.push_back(-1);
@@ -1342,7 +1357,7 @@ void Run::st_mxA(bool dup)
   pending_dup = !dup;
 }
 
-// bcc_lrup_mxdup (bcc_lrup.c:1128-1138) + bcc_M_mxset (bcc.c:1414-1438): the retained rows
+// bcc_lrup_mxdup (bcc_lrup.c:1128-1139) + bcc_M_mxset (bcc.c:1414-1438): the retained rows
 // and columns become the active sets, and the masks and index lists reach the device
 void Run::st_mxdup()
 {
@@ -1390,7 +1405,7 @@ void Run::gram_any(const GramMArgs& g)
   [[maybe_unused]] const Packed pk_rmv{ (const uint8_t*)At_rmvc, (const uint8_t*)Tt_rmvc, (size_t)rmv_W * 4, d_cmask_rmv, 32 * rmv_W }; \
   [[maybe_unused]] const Packed pk_rtn{ (const uint8_t*)At_rtnc, (const uint8_t*)Tt_rtnc, (size_t)rtn_W * 4, d_cmask_rtn, 32 * rtn_W }
 
-// bcc_lrup_QR_YnWt_stage_0 (bcc_lrup.c:659-723): the mask weights of the dropped and the
+// bcc_lrup_QR_YnWt_stage_0 (bcc_lrup.c:659-701): the mask weights of the dropped and the
 // retained columns, the An_ajdk of the retained rows and the controls over the dropped columns
 void Run::lrup_QR_0()
 {
@@ -1405,7 +1420,7 @@ void Run::lrup_QR_0()
   }
 }
 
-// bcc_lrup_QR_YnWt_stage_1 + _stage_3 (bcc_lrup.c:724-770, 863-892): the Grams of the retained
+// bcc_lrup_QR_YnWt_stage_1 + _stage_3 (bcc_lrup.c:702-771, 863-895): the Grams of the retained
 // rows with the controls and with themselves over the dropped columns, subtracted from
 // lf_AnZt and lf_AnAt as they are formed (the Grams are not kept: stage 3 has nothing left)
 void Run::lrup_QR_1()
@@ -1448,10 +1463,10 @@ void Run::lrup_QR_1()
   ops += (long long)rtn.n() * rtn.n() * rbm;
 }
 
-// bcc_lrup_QR_YnWt_stage_3 (bcc_lrup.c:863-892): done inside stage 1
+// bcc_lrup_QR_YnWt_stage_3 (bcc_lrup.c:863-895): done inside stage 1
 void Run::lrup_QR_3() {}
 
-// bcc_lrup_QC_YnWt_stage_a0 (bcc_lrup.c:5-75): An_ajdk of the removed, the retained rows and
+// bcc_lrup_QC_YnWt_stage_a0 (bcc_lrup.c:5-66): An_ajdk of the removed, the retained rows and
 // the controls over the retained and the dropped columns
 void Run::lrup_QC_a0()
 {
@@ -1467,7 +1482,7 @@ void Run::lrup_QC_a0()
   }
 }
 
-// bcc_lrup_QC_YnWt_stage_a1 (bcc_lrup.c:76-180): the compact Grams knzt, hnvt, ankt, knkt
+// bcc_lrup_QC_YnWt_stage_a1 (bcc_lrup.c:67-196): the compact Grams knzt, hnvt, ankt, knkt
 void Run::lrup_QC_a1()
 {
   LRUP_LOCALS;
@@ -1552,7 +1567,7 @@ void Run::lrup_QC_a1()
   if (has_Z && !cols_dropped && nrm > 0 && nz > 0) fill(hnvt, (long long)nrm * nz, 0.0, st);
 }
 
-// the update's four At_T_YnWt_S_Zn terms (bcc_lrup.c:238-303): knzt and hnvt (with Z),
+// the update's four At_T_YnWt_S_Zn terms (bcc_lrup.c:238-304): knzt and hnvt (with Z),
 // ankt and knkt, each over the retained columns
 const WwTerm* Run::lrup_terms()
 {
@@ -1564,7 +1579,7 @@ const WwTerm* Run::lrup_terms()
   return lrup_w;
 }
 
-// bcc_lrup_QC_YnWt_stage_a2 (bcc_lrup.c:181-237): the four Grams quantised (xcalc), each into
+// bcc_lrup_QC_YnWt_stage_a2 (bcc_lrup.c:197-237): the four Grams quantised (xcalc), each into
 // its own slot of integers and scalars
 void Run::lrup_QC_a2()
 {
@@ -1578,7 +1593,7 @@ void Run::lrup_QC_a2()
   for (int i = 0; i < 4; i++) if (has_Z || i >= 2) ww_quantise(i, w[i]);
 }
 
-// bcc_lrup_QC_YnWt_stage_a3 (bcc_lrup.c:238-303): the four At_T_YnWt_S_Zn_ww contractions
+// bcc_lrup_QC_YnWt_stage_a3 (bcc_lrup.c:238-304): the four At_T_YnWt_S_Zn_ww contractions
 void Run::lrup_QC_a3()
 {
   LRUP_LOCALS;
@@ -1586,7 +1601,7 @@ void Run::lrup_QC_a3()
   for (int i = 0; i < 4; i++) if (has_Z || i >= 2) ww_contract(i, w[i]);
 }
 
-// bcc_lrup_QC_YnWt_stage_b0 (bcc_lrup.c:304-379): plain +-1 products of the retained and the
+// bcc_lrup_QC_YnWt_stage_b0 (bcc_lrup.c:305-362): plain +-1 products of the retained and the
 // dropped columns with the covariates, [t*M + k]
 void Run::lrup_QC_b0()
 {
@@ -1616,7 +1631,7 @@ void Run::lrup_QC_b0()
   }
 }
 
-// bcc_lrup_QC_YnWt_stage_b1 (bcc_lrup.c:380-480): the AtTYn_vv triples; on the matrix cores
+// bcc_lrup_QC_YnWt_stage_b1 (bcc_lrup.c:363-408): the AtTYn_vv triples; on the matrix cores
 // fused with stage b2's ordered sums (the triples are then not kept)
 void Run::lrup_QC_b1()
 {
@@ -1669,7 +1684,7 @@ void Run::lrup_QC_b1()
   }
 }
 
-// bcc_lrup_QC_YnWt_stage_b2 (bcc_lrup.c:481-590): the ordered WtSZn sums of the triples
+// bcc_lrup_QC_YnWt_stage_b2 (bcc_lrup.c:409-482): the ordered WtSZn sums of the triples
 void Run::lrup_QC_b2()
 {
   LRUP_LOCALS;
@@ -1881,7 +1896,6 @@ void Run::iterate(int* done)
     st_QR(); st_ajdk(); st_ZtSn(); st_single(); st_double(); st_flatten();
   }
   HIPCHK(hipGetLastError());
-  n_iter++;
   if (done) *done = active() ? 0 : 1;
   if (!active()) HIPCHK(hipStreamSynchronize(st));
   loop_s += now_s() - t0;
@@ -1900,7 +1914,6 @@ void Run::run(int max_iter)
       lrup();
       st_QR(); st_ajdk(); st_ZtSn(); st_single(); st_double(); st_flatten();
     }
-    n_iter++;
     done = active() ? 0 : 1;
   }
   HIPCHK(hipStreamSynchronize(st));

@@ -35,6 +35,7 @@ void scorebox_xdrop_array(int nxols, int x_max, int x_num, int* xval, int* xdrop
 void write_scorebox(const Scorebox& S, const std::string& dir);
 void parse_cfg(const char* text, Cfg& c);
 void get_xdrop(double lrij, double lcij, double gamma, int gamma_type, int* rdrop, int* cdrop);
+int get_xdrop_length(double lrij, double lcij, double gamma, int gamma_type);
 void sort_scores(double* d, int* perm, int n);
 
 // Device allocations of one owner.  With a stream set, they are stream-ordered

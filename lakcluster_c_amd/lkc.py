@@ -26,6 +26,21 @@ class LkcError(RuntimeError):
     pass
 
 
+# the stages of one pass of lakcluster_driver's loop, in its order (lakcluster_driver.c:121-246),
+# after which lf_AnZt_S_WnYt, An_ajdk, lf_ZtSn, singlestudy_ww, doublestudy_ww and
+# flattenloop run again; sumscores_dmp (the output files) may follow sumscores_mxA
+INITIAL_STAGES = ("An_ajdk", "lf_ZtSn", "lf_YnWt", "lf_At_T_YnWt_S_Zn", "lf_AnZt_S_WnYt",
+                  "singlestudy_ww", "doublestudy_ww", "flattenloop")
+LOOP_STAGES = ("sumscores_nrm", "sumscores_ifT", "sumscores_srt", "sumscores_cmb", "sumscores_xij",
+               "sumscores_mxA", "lrup_QR_YnWt_stage_0", "lrup_QR_YnWt_stage_1", "lrup_QR_YnWt_stage_3",
+               "lrup_QC_ZtSWn_stage_0", "lrup_QC_ZtSWn_stage_1", "lrup_QC_ZtSWn_stage_2",
+               "lrup_QC_YnWt_stage_a0", "lrup_QC_YnWt_stage_a1", "lrup_QC_YnWt_stage_a2",
+               "lrup_QC_YnWt_stage_a3", "lrup_QC_YnWt_stage_b0", "lrup_QC_YnWt_stage_b1",
+               "lrup_QC_YnWt_stage_b2", "lrup_mxdup", "M_mxset", "M_ZtSWn", "M_YnWt",
+               "lf_AtTYn____WtSZn", "lrup_QC_YnWt_stage_c")
+RESCORE_STAGES = ("lf_AnZt_S_WnYt", "An_ajdk", "lf_ZtSn", "singlestudy_ww", "doublestudy_ww", "flattenloop")
+
+
 def lib():
     """Load liblkc.so (built by ``make -C lakcluster_c_amd/csrc``)."""
     global _lib
@@ -98,7 +113,13 @@ def lib():
         "lkc_bcc_is_scorebox": (C.c_int, [vp]),
         "lkc_dexcluster_driver": (C.c_int, [cp, cp]),
         "lkc_bcc_is_dcc": (C.c_int, [vp]),
+        "lkc_bcc_active": (C.c_int, [vp]),
+        "lkc_bcc_sumscores_dmp": (C.c_int, [vp, cp]),
+        "lkc_get_xdrop": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int, ip, ip]),
+        "lkc_get_xdrop_length": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int]),
+        "lkc_get_xdrop_array": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, ip, ip, ip, ip, ip]),
     }
+    sig.update({"lkc_bcc_" + name: (C.c_int, [vp]) for name in LOOP_STAGES})
     for name, (res, args) in sig.items():
         f = getattr(L, name)
         f.restype = res
@@ -168,6 +189,20 @@ class Bcc:
 
     def run(self, max_iter: int = -1) -> None:
         _check(lib().lkc_bcc_run(self._h, max_iter), self._h)
+
+    def stage(self, name: str) -> None:
+        """One stage of the driver by its reference name without the bcc_ prefix
+        (INITIAL_STAGES, LOOP_STAGES, RESCORE_STAGES, "sumscores", "lrup")."""
+        if name not in INITIAL_STAGES + LOOP_STAGES + ("sumscores", "lrup"):
+            raise LkcError(f"unknown stage {name}")
+        _check(getattr(lib(), "lkc_bcc_" + name)(self._h), self._h)
+
+    def active(self) -> bool:
+        """True while a row or a column is active (the driver's loop condition)."""
+        return bool(lib().lkc_bcc_active(self._h))
+
+    def sumscores_dmp(self, out_dir: Optional[str] = None) -> None:
+        _check(lib().lkc_bcc_sumscores_dmp(self._h, out_dir.encode() if out_dir else None), self._h)
 
     def write_outputs(self, out_dir: Optional[str] = None) -> None:
         _check(lib().lkc_bcc_write_outputs(self._h, out_dir.encode() if out_dir else None), self._h)
@@ -303,6 +338,24 @@ def lakcluster_driver(in_text: str, out_dir: Optional[str] = None) -> None:
 def dexcluster_driver(in_text: str, out_dir: Optional[str] = None) -> None:
     """Whole dcc run with the three output files written (dexcluster_driver)."""
     _check(lib().lkc_dexcluster_driver(in_text.encode(), out_dir.encode() if out_dir else None))
+
+
+def get_xdrop(lrij: float, lcij: float, gamma: float, gamma_type: int = 0) -> Tuple[int, int]:
+    """get_xdrop (xdrop.c:5-64): (rows, columns) one pass drops."""
+    r, c = C.c_int(0), C.c_int(0)
+    _check(lib().lkc_get_xdrop(lrij, lcij, gamma, gamma_type, C.byref(r), C.byref(c)))
+    return r.value, c.value
+
+
+def get_xdrop_array(lrij: float, lcij: float, gamma: float, gamma_type: int = 0) -> np.ndarray:
+    """get_xdrop_array (xdrop.c:85-111): rows [rdrop, cdrop, rkeep, ckeep] per pass."""
+    n = lib().lkc_get_xdrop_length(lrij, lcij, gamma, gamma_type)
+    if n < 0:
+        _check(n)
+    out = np.zeros((4, max(n, 1)), dtype=np.int32)
+    ln = C.c_int(0)
+    _check(lib().lkc_get_xdrop_array(lrij, lcij, gamma, gamma_type, n, C.byref(ln), *(_ptr(out[i], C.c_int) for i in range(4))))
+    return out[:, :ln.value].T.copy()
 
 
 def bsize(ncols: int) -> int:

@@ -91,3 +91,85 @@ def test_config_errors_are_rejected_before_any_device_work():
 def test_driver_binary_built():
     from lakcluster_c_amd import lkc
     assert os.access(lkc.DRIVER_PATH, os.X_OK)
+
+
+def test_get_xdrop_entry_points(oracle):
+    """lkc_get_xdrop / _length / _array (xdrop.c:5-111) against the oracle's get_xdrop, and the
+    schedule's own invariants: each pass keeps what the previous kept minus its drops, the
+    last pass empties the rows or the columns, no earlier pass does."""
+    from lakcluster_c_amd import lkc
+    L = lkc.lib()
+    for gamma, gt in ((0.1, 0), (0.02, 1), (0.05, 2), (0.2, 3), (0.0, 0), (0.01, 0)):
+        for r, c in ((64, 400), (2048, 8192), (3, 3), (500, 100), (1, 7), (16384, 65536), (2, 1)):
+            assert lkc.get_xdrop(r, c, gamma, gt) == oracle.get_xdrop(r, c, gamma, gt)
+            a = lkc.get_xdrop_array(r, c, gamma, gt)
+            assert len(a) == L.lkc_get_xdrop_length(r, c, gamma, gt) >= 1
+            rr, cc = r, c
+            for i, (rd, cd, rk, ck) in enumerate(a):
+                assert (rd, cd) == oracle.get_xdrop(rr, cc, gamma, gt)
+                rr, cc = rr - rd, cc - cd
+                assert (rk, ck) == (rr, cc)
+                assert (rr > 0 and cc > 0) == (i + 1 < len(a)), (gamma, gt, r, c, i)
+    n = ctypes.c_int(0)
+    buf = (ctypes.c_int * 4)()
+    assert L.lkc_get_xdrop_array(64.0, 400.0, 0.1, 0, 4, ctypes.byref(n), buf, None, None, None) == -5
+    assert n.value == 23
+    assert L.lkc_get_xdrop_length(float("nan"), 4.0, 0.1, 0) == -5
+
+
+def test_stage_entries_are_exported_in_driver_order():
+    from lakcluster_c_amd import lkc
+    names = _declared()
+    for st in lkc.INITIAL_STAGES + lkc.LOOP_STAGES + ("sumscores_dmp", "active"):
+        assert "lkc_bcc_" + st in names, st
+    # the header lists the loop's stages in the driver's order
+    text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "lkc.h")).read(), flags=re.S)
+    pos = [text.index("lkc_bcc_" + st + "(") for st in lkc.LOOP_STAGES]
+    assert pos == sorted(pos)
+
+
+def test_stage_loop_program_built():
+    assert os.access(os.path.join(ROOT, "tests", "c", "stage_loop"), os.X_OK)
+
+
+def _oracle_files(oracle, text, out):
+    oracle.run_text(text, out_dir=str(out), write_files=True)
+    return {f: open(os.path.join(out, f), "rb").read() for f in ("out_trace.txt", "out_xdrop_a.txt", "out_xdrop_b.txt")}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["casectrl", "caseonly", "masked", "kappa"])
+def test_c_stage_loop_matches_whole_run_and_oracle(datasets, oracle, tmp_path, name):
+    """A C program built against include/lkc.h walks the driver's loop one stage entry at a
+    time, writing the output files with bcc_sumscores_dmp after each pass; its trace and
+    drops equal lkc_bcc_run's byte for byte, and its files equal the oracle's."""
+    import subprocess
+    ds = datasets[name]
+    out = tmp_path / "stages"
+    out.mkdir()
+    print(f"stage_loop {name}", flush=True)
+    p = subprocess.run([os.path.join(ROOT, "tests", "c", "stage_loop"), ds.in_path, str(out)],
+                       capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "identical" in p.stdout
+    ref = _oracle_files(oracle, open(ds.in_path).read(), tmp_path / "oracle")
+    for f, b in ref.items():
+        assert open(out / f, "rb").read() == b, f
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["casectrl", "b16_g1", "blocks"])
+def test_lkc_driver_files_match_oracle(datasets, oracle, tmp_path, name):
+    """`lkc_driver out_dir < file.in`, as `lakcluster_ver18 < file.in`: its three output
+    files equal the oracle's byte for byte."""
+    import subprocess
+    from lakcluster_c_amd import lkc
+    ds = datasets[name]
+    out = tmp_path / "driver"
+    print(f"lkc_driver {name}", flush=True)
+    with open(ds.in_path) as fin:
+        p = subprocess.run([lkc.DRIVER_PATH, str(out)], stdin=fin, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout + p.stderr
+    ref = _oracle_files(oracle, open(ds.in_path).read(), tmp_path / "oracle")
+    for f, b in ref.items():
+        assert open(out / f, "rb").read() == b, f
