@@ -76,7 +76,7 @@ def _traffic(workload):
     return t["fetch_kib_per_launch_raw"] * 1024 * t["fetch_scale"] + t["write_kib_per_launch_raw"] * 1024 * t["write_scale"]
 
 
-def batched_legs(n_null=16, k_null=8, n_serve=24, k_serve=8, seed=7):
+def batched_legs(n_null=256, k_null=8, n_serve=24, k_serve=8, seed=7, c2_tbitops=None):
     """Many eliminations on one GPU, from a device-resident pool of the 16384 x 65536 case-control problem's patients
     (16384 cases + 16384 controls x 65536 SNPs, T/S 3 columns):
     * permutation nulls: row subsamples log-uniform in [2048, 16384] with re-shuffled
@@ -84,7 +84,9 @@ def batched_legs(n_null=16, k_null=8, n_serve=24, k_serve=8, seed=7):
       B 32; the same jobs run one after another (the plain GPU path) and k at a time;
     * serving: a Poisson stream of jobs (rows log-uniform in [1024, 32768], columns in
       [8192, 65536], half case-only, gamma ~ U[0.01, 0.1]) admitted by an HBM budget.
-    Traces of the batched runs are compared with the sequential ones (bit for bit)."""
+    Traces of the batched runs are compared with the sequential ones (bit for bit).  The
+    reference's bit-operation count of the null jobs gives their Tbit-op/s, next to the
+    16384 x 65536 single run's (c2_tbitops) as the rate the same kernels reach on a large job."""
     import numpy as np
     from lakcluster_c_amd import gen, lkc
     t0 = time.time()
@@ -119,11 +121,15 @@ def batched_legs(n_null=16, k_null=8, n_serve=24, k_serve=8, seed=7):
     lat = np.array([r["end_s"] - r["arrival_s"] for r in srv])
     ok = all(r["status"] == 0 for r in srv)
     print(f"[bench] serving: {len(sspec)} jobs in {tsrv:.2f} s", file=sys.stderr, flush=True)
+    null_ops = float(sum(r["ops"] for r in plain))
     return {
         "batched_nulls": {
             "runs": n_null, "plain_runs_per_hour": n_null / tp * 3600, "batched_runs_per_hour": n_null / tb * 3600,
             "ratio": tp / tb, "concurrent": k_null, "identical_to_plain": bool(same),
             "passes_mean": float(np.mean([r["n_iter"] for r in plain])),
+            "bit_ops_per_run_mean": null_ops / n_null,
+            "plain_Tbitops": null_ops / tp / 1e12, "batched_Tbitops": null_ops / tb / 1e12,
+            "batched_vs_single_run_rate": (null_ops / tb / 1e12) / c2_tbitops if c2_tbitops else None,
             "sample": f"{n_null} permutation-null jobs (rows log-U[2048,16384] of the pooled "
                       "16384+16384 patients x 65536 SNPs, labels re-shuffled, columns log-U[16384,65536], gamma 0.05, B 32)",
         },
@@ -150,13 +156,15 @@ def nulls_leg(n_null, k_null, seed=7):
         P.batch(spec[:1], max_concurrent=1)                      # warm-up
         t = time.time(); plain = P.batch(spec, max_concurrent=1); tp = time.time() - t
         print(f"[bench] nulls: {n_null} runs one at a time in {tp:.2f} s", file=sys.stderr, flush=True)
+        null_ops = float(sum(r["ops"] for r in plain))
         out.update({"plain_s": tp, "plain_runs_per_hour": n_null / tp * 3600,
-                    "passes_mean": float(np.mean([r["n_iter"] for r in plain]))})
+                    "passes_mean": float(np.mean([r["n_iter"] for r in plain])),
+                    "bit_ops_per_run_mean": null_ops / n_null, "plain_Tbitops": null_ops / tp / 1e12})
         if k_null > 1:
             t = time.time(); bat = P.batch(spec, max_concurrent=k_null); tb = time.time() - t
             print(f"[bench] nulls: {n_null} runs {k_null} at a time in {tb:.2f} s", file=sys.stderr, flush=True)
             out.update({"concurrent": k_null, "batched_s": tb, "batched_runs_per_hour": n_null / tb * 3600,
-                        "ratio": tp / tb,
+                        "ratio": tp / tb, "batched_Tbitops": null_ops / tb / 1e12,
                         "identical_to_plain": all(a["status"] == 0 and a["trace"].tobytes() == b["trace"].tobytes()
                                                   for a, b in zip(plain, bat))})
     return out
@@ -176,7 +184,7 @@ def dcc_leg(text, max_iter=-1):
     el = time.perf_counter() - t0
     launches, ms, work = run.kernel_stats("ztsn")
     out = {"elim_run_s": el, "passes": run.n_iter, "bit_ops_per_run": run.ops, "value": run.ops / el / 1e12,
-           "unit": "Tbit-op/s"}
+           "unit": "Tbit-op/s", "host_quicksorts": run.sort_fallbacks}
     run.close()
     # the contraction streams every active packed row once per launch: HBM-bound
     ach = work / (ms / 1e3) / 1e9 if ms > 0 else None
@@ -335,6 +343,8 @@ def main():
     ap.add_argument("--nulls", type=int, metavar="RUNS",
                     help="run only the permutation-null leg with RUNS jobs (plain, then --concurrent at a time)")
     ap.add_argument("--concurrent", type=int, default=8, help="with --nulls: jobs at a time (1: plain only)")
+    ap.add_argument("--null-runs", type=int, default=256,
+                    help="permutation-null jobs of the batched leg (the 256-run batch by default)")
     ap.add_argument("--kernels", choices=["mfma", "valu"], default="mfma",
                     help="contraction kernels of the main leg: int8 matrix cores (default) or VALU popcount")
     ap.add_argument("--repeat", action="store_true",
@@ -392,6 +402,7 @@ def main():
         print(f"[bench] warmup {w}: {time.perf_counter() - t0:.2f} s, {run.n_iter} passes", file=sys.stderr, flush=True)
         run.reset()
     elim, ops, kms, kwork, kl = [], [], 0.0, 0.0, 0
+    fallbacks = 0
     qr_l, qr_ms, qr_work = 0, 0.0, 0.0
     n_iter = 0
     for k in range(args.steps):
@@ -401,6 +412,7 @@ def main():
         elim.append(t1 - t0)
         ops.append(run.ops)
         n_iter = run.n_iter
+        fallbacks = run.sort_fallbacks
         launches, ms, work = run.kernel_stats("ww")
         kl += launches; kms += ms; kwork += work
         ql, qms, qwork = run.kernel_stats("qr")
@@ -430,7 +442,8 @@ def main():
         "data": "synthetic (seeded generator: Bernoulli columns, planted rank-1 bicluster)",
         "config": {"workload": args.config, "kernels": args.kernels, "rows_A": shape[0], "rows_Z": shape[1], "cols": shape[2],
                    "covariates": shape[3], "gamma": kw["gamma"], "B_MLT": kw["B"],
-                   "passes": n_iter, "bit_ops_per_run": ops[-1] if ops else None},
+                   "passes": n_iter, "bit_ops_per_run": ops[-1] if ops else None,
+                   "host_quicksorts": fallbacks if args.steps else None},
         "roofline": {
             "kernel": "At_T_YnWt_S_Zn_ww contraction",
             "bound": "mfma",
@@ -455,7 +468,7 @@ def main():
         line["dcc"] = dcc_leg(text, args.max_iter)
     if not args.no_batch:
         print("[bench] batched nulls and serving ...", file=sys.stderr, flush=True)
-        line.update(batched_legs())
+        line.update(batched_legs(n_null=args.null_runs, c2_tbitops=value))
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(args.data_root)

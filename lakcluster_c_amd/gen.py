@@ -293,37 +293,49 @@ def null_jobs(pool: Pool, n_jobs: int, seed: int, rows: Tuple[int, int] = (2048,
     return out
 
 
-def write_job(root: str, name: str, pool: Pool, job: NullJob, gamma: float, B: int) -> Dataset:
+def write_job(root: str, name: str, pool: Pool, job: NullJob, gamma: float, B: int, gamma_type: int = 0,
+              kappa_squared: float = 0.0) -> Dataset:
     """The same job as files and a .in (rows as A/Z, covariates, the column mask as
-    mc_A.b16), for the oracle and for the file-driven path."""
+    mc_A.b16), for the oracle and for the file-driven path; a job with no Z rows is
+    written as a case-only problem."""
     d_in = os.path.join(root, f"dir_{name}_in")
     d_out = os.path.join(root, f"dir_{name}_out")
     os.makedirs(d_in, exist_ok=True)
     os.makedirs(d_out, exist_ok=True)
     unpack = lambda R: np.unpackbits(R, axis=1, bitorder="big")[:, :pool.ncols].astype(bool)
-    A = unpack(pool.rows[job.rowsA]); Z = unpack(pool.rows[job.rowsZ])
+    A = unpack(pool.rows[job.rowsA])
+    nz = int(job.rowsZ.size)
     covb = np.unpackbits(pool.cov, axis=1, bitorder="big")[:, :pool.nrows].astype(bool).T   # nrows x T
     write_b16(os.path.join(d_in, "A_n.b16"), A); write_b16(os.path.join(d_in, "A_t.b16"), A.T)
-    write_b16(os.path.join(d_in, "Z_n.b16"), Z); write_b16(os.path.join(d_in, "Z_t.b16"), Z.T)
-    Tm, Sm = covb[job.rowsA], covb[job.rowsZ]
+    Tm = covb[job.rowsA]
     write_b16(os.path.join(d_in, "T_n.b16"), Tm); write_b16(os.path.join(d_in, "T_t.b16"), Tm.T)
-    write_b16(os.path.join(d_in, "S_n.b16"), Sm); write_b16(os.path.join(d_in, "S_t.b16"), Sm.T)
+    if nz:
+        Z = unpack(pool.rows[job.rowsZ])
+        write_b16(os.path.join(d_in, "Z_n.b16"), Z); write_b16(os.path.join(d_in, "Z_t.b16"), Z.T)
+        Sm = covb[job.rowsZ]
+        write_b16(os.path.join(d_in, "S_n.b16"), Sm); write_b16(os.path.join(d_in, "S_t.b16"), Sm.T)
     cm = np.unpackbits(job.colmask, bitorder="big")[:pool.ncols].astype(bool)
     write_b16(os.path.join(d_in, "mc_A.b16"), cm[:, None])
+    # gamma in full (shortest round-trip) precision, so that the file's run is the job's
     lines = ["GLOBAL_verbose= 0;", "GLOBAL_thread_count= 8;", "GLOBAL_omp_type= 1;",
              "GLOBAL_TEST_TYPE= lakcluster_driver;", "GLOBAL_QR_strategy= YnWt;", "GLOBAL_QC_strategy= YnWt;",
-             "GLOBAL_NBINS= 1;", f"GLOBAL_B_MLT= {B};", f"GLOBAL_gamma= {gamma:.4f};",
+             "GLOBAL_NBINS= 1;", f"GLOBAL_B_MLT= {B};", f"GLOBAL_gamma= {float(gamma)!r};",
+             f"GLOBAL_gamma_type= {gamma_type};", f"GLOBAL_kappa_squared= {float(kappa_squared)!r};",
              f"GLOBAL_DIR_XPRE= {d_in};", "GLOBAL_A_n_name_= A_n.b16;", "GLOBAL_A_t_name_= A_t.b16;",
-             f"GLOBAL_A_n_rows_= {A.shape[0]};", f"GLOBAL_A_n_cols= {pool.ncols};", "GLOBAL_A_n_cind= mc_A.b16;",
-             "GLOBAL_Z_n_name_= Z_n.b16;", "GLOBAL_Z_t_name_= Z_t.b16;", f"GLOBAL_Z_n_rows_= {Z.shape[0]};",
-             "GLOBAL_Y_n_cols= 0;", f"GLOBAL_T_n_cols= {pool.ncov};",
-             "GLOBAL_T_n_name_= T_n.b16;", "GLOBAL_T_t_name_= T_t.b16;",
-             "GLOBAL_S_n_name_= S_n.b16;", "GLOBAL_S_t_name_= S_t.b16;",
-             f"GLOBAL_DIR_NAME= {d_out};", "END= 0;"]
+             f"GLOBAL_A_n_rows_= {A.shape[0]};", f"GLOBAL_A_n_cols= {pool.ncols};", "GLOBAL_A_n_cind= mc_A.b16;"]
+    if nz:
+        lines += ["GLOBAL_Z_n_name_= Z_n.b16;", "GLOBAL_Z_t_name_= Z_t.b16;", f"GLOBAL_Z_n_rows_= {nz};"]
+    else:
+        lines.append("GLOBAL_Z_n_rows_= 0;")
+    lines += ["GLOBAL_Y_n_cols= 0;", f"GLOBAL_T_n_cols= {pool.ncov};",
+              "GLOBAL_T_n_name_= T_n.b16;", "GLOBAL_T_t_name_= T_t.b16;"]
+    if nz:
+        lines += ["GLOBAL_S_n_name_= S_n.b16;", "GLOBAL_S_t_name_= S_t.b16;"]
+    lines += [f"GLOBAL_DIR_NAME= {d_out};", "END= 0;"]
     in_path = os.path.join(d_in, f"{name}.in")
     with open(in_path, "w") as fp:
         fp.write("\n".join(lines) + "\n")
-    return Dataset(in_path, d_out, A.shape[0], Z.shape[0], pool.ncols, pool.ncov, gamma, B)
+    return Dataset(in_path, d_out, A.shape[0], nz, pool.ncols, pool.ncov, gamma, B)
 
 
 @dataclass

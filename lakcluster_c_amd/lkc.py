@@ -510,13 +510,15 @@ class Pool:
         return b
 
     def batch(self, jobs, max_concurrent: int, budget_bytes: float = 0.0, arrivals=None):
-        """Run jobs [(rowsA, rowsZ, colmask, gamma, B)] with at most max_concurrent at a time
-        (lkc_batch_run).  Returns a list of dicts: status, n_iter, ops, arrival/start/end
-        seconds, trace (n x 6), xdrop_a, xdrop_b (n x 2)."""
+        """Run jobs [(rowsA, rowsZ, colmask, gamma, B[, {gamma_type, kappa_squared}])] with at
+        most max_concurrent at a time (lkc_batch_run).  Returns a list of dicts: status,
+        n_iter, ops, arrival/start/end seconds, trace (n x 6), xdrop_a, xdrop_b (n x 2)."""
         n = len(jobs)
         keep = []
         arr = (Job * max(n, 1))()
-        for j, (ra, rz, cm, gamma, B) in enumerate(jobs):
+        for j, job in enumerate(jobs):
+            ra, rz, cm, gamma, B = job[:5]
+            kw = job[5] if len(job) > 5 else {}
             a = np.ascontiguousarray(ra, dtype=np.int32); z = np.ascontiguousarray(rz, dtype=np.int32)
             keep += [a, z]
             arr[j].rowsA = _ptr(a, C.c_int); arr[j].nA = a.size
@@ -524,7 +526,7 @@ class Pool:
             if cm is not None:
                 m = np.ascontiguousarray(cm, dtype=np.uint8); keep.append(m)
                 arr[j].colmask = _ptr(m, C.c_uint8)
-            arr[j].params = _params(gamma, B)
+            arr[j].params = _params(gamma, B, **kw)
             arr[j].arrival_s = float(arrivals[j]) if arrivals is not None else 0.0
         res = (JobResult * max(n, 1))()
         _check(lib().lkc_batch_run(self._h, arr, n, max_concurrent, budget_bytes, res))

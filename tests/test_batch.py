@@ -82,3 +82,50 @@ def test_batch_and_serving_match_solo_runs():
     # one at a time: no two jobs overlap
     iv = sorted((r["start_s"], r["end_s"]) for r in one_at_a_time)
     assert all(b[0] >= a[1] - 1e-9 for a, b in zip(iv, iv[1:]))
+
+
+def _mixed_jobs(pool, n, seed):
+    """Jobs as a served stream draws them: half case-only (every picked row a case, no
+    controls), half case-control, gamma ~ U[0.01, 0.1] per job, one with gamma_type 1 and one
+    with kappa_squared 0.5."""
+    rng = np.random.default_rng(seed)
+    jobs = gen.null_jobs(pool, n, seed=seed, rows=(60, 260), cols=(1000, 2600))
+    out = []
+    for i, j in enumerate(jobs):
+        if i % 2:
+            j = gen.NullJob(np.sort(np.r_[j.rowsA, j.rowsZ]).astype(np.int32), np.zeros(0, np.int32), j.colmask,
+                            j.ncols_active)
+        kw = {"gamma_type": 1 if i == 2 else 0, "kappa_squared": 0.5 if i == 3 else 0.0}
+        out.append((j, float(rng.uniform(0.01, 0.1)), kw))
+    return out
+
+
+@pytest.mark.gpu
+def test_batch_and_served_mixed_jobs_match_oracle(tmp_path, oracle):
+    """Every job of a concurrent batch and of a served stream (Poisson-like arrivals, a
+    budget that holds two jobs) against its own oracle run of the job written as files:
+    case-only and case-control jobs, per-job gamma, gamma_type and kappa_squared."""
+    from lakcluster_c_amd import lkc
+    pool = _pool(seed=13)
+    mixed = _mixed_jobs(pool, 8, seed=21)
+    refs = []
+    for i, (j, g, kw) in enumerate(mixed):
+        ds = gen.write_job(str(tmp_path), f"m{i}", pool, j, gamma=g, B=32, **kw)
+        refs.append(oracle.run_text(open(ds.in_path).read()))
+        assert refs[-1].trace[0, 1] == j.rowsA.size and refs[-1].trace[0, 2] == j.ncols_active
+    spec = [(j.rowsA, j.rowsZ, j.colmask, g, 32, kw) for j, g, kw in mixed]
+    arr = np.cumsum(np.random.default_rng(5).exponential(0.03, size=len(spec)))
+    with lkc.Pool(pool.rows, pool.nrows, pool.ncols, pool.cov, pool.ncov) as P:
+        print("mixed batch", flush=True)
+        batched = P.batch(spec, max_concurrent=4)
+        two = 2.05 * max(lkc.job_bytes(j.rowsA.size, j.rowsZ.size, pool.ncols, 32) for j, _, _ in mixed)
+        print("mixed served stream", flush=True)
+        served = P.batch(spec, max_concurrent=4, budget_bytes=two, arrivals=arr - arr[0])
+    for out in (batched, served):
+        for i, (r, ref) in enumerate(zip(out, refs)):
+            assert r["status"] == 0, i
+            assert r["trace"].tobytes() == ref.trace.tobytes(), i
+            np.testing.assert_array_equal(r["xdrop_a"], ref.xdrop_a)
+            np.testing.assert_array_equal(r["xdrop_b"], ref.xdrop_b)
+    for r in served:
+        assert r["end_s"] >= r["start_s"] >= r["arrival_s"] - 1e-9
