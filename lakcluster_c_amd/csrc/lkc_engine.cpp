@@ -1060,7 +1060,7 @@ void Run::st_single()
   xcalc(x6, 1, 0, ar.d, nullptr, ar.n(), 1, ar.d, c.B, c.D_MLT, cz, NA, scal, xwork, st);
   ops += (long long)ar.n() * (c.B + 1) * 8;
   if (ww_kernel == 0) {
-    size_t wb = ztsn_mfma_work_bytes(1, prb, c.B);
+    size_t wb = ztsn_mfma_work_bytes(kl.n(), 1, prb, c.B);
     HIPCHK(ztsn_mfma(At.b(), kl.d, kl.n(), Ar_j, cz, NA, NA, 1, d_zero, c.B, scal, At_An_a1d1, 0, g_ztsn.need(dev, wb), wb, st));
   } else {
     cz_planes(cz, NA, 1, NA, Ar_j, c.B, planes, prb, st);
@@ -1116,7 +1116,7 @@ void Run::ztsn_weighted(const double* a2d2, double* outZ, double* outA)
     scale_rows(src, M, kl.d, nk, tl.d, nt, a2d2, L, st);
     xcalc(L, 1, nk, nullptr, nullptr, nk, nt, kl.d, c.B, c.D_MLT, cz, M, scal, xwork, st);
     if (ww_kernel == 0) {
-      size_t wb = ztsn_mfma_work_bytes(nt, prb, c.B);
+      size_t wb = ztsn_mfma_work_bytes(ar.n(), nt, prb, c.B);
       Prof::Pending pz;
       // timed with its HBM bytes: the active rows (whole packed rows) and the quantised values
       prof.begin("ztsn", (double)ar.n() * prb + 4.0 * nt * (double)M, st, pz);

@@ -96,7 +96,7 @@ hipError_t an_ztsn_noalloc(const uint8_t* A, int nr_total, const int* alist, int
 // An_ZtSn_ww on the int8 matrix cores from the quantised values themselves (no planes):
 // cz[s*ldc + n] for n < n_total, mask = the active n (the planes' row0 and amask, equal
 // at every call site).  Same output as an_ztsn.
-size_t ztsn_mfma_work_bytes(int ns, size_t prb, int B);
+size_t ztsn_mfma_work_bytes(int na, int ns, size_t prb, int B);
 hipError_t ztsn_mfma(const Bits& A, const int* alist, int na, const uint32_t* mask, const uint32_t* cz, long long ldc,
                int n_total, int ns, const int* slist, int B, const double* scal, double* out, long long ldo,
                void* work, size_t work_bytes, hipStream_t st);

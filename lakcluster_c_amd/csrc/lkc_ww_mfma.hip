@@ -432,16 +432,30 @@ __global__ void k_ww_zero(unsigned long long* p, long long n)
 // 3) leaves CUs idle and the w steps can fill them: then one block per (rows, covariates,
 // w step), and the n chunks of a step in parts (at least 4 chunks each) until about one
 // block per CU.  (One block of this kernel fills a CU: its registers allow one wave per SIMD.)
+// A fused grid of one to a few rounds of blocks with a ragged last round (257 to 511
+// blocks: the last round leaves up to half the CUs idle for a whole block's w loop) also
+// takes the one-part split when its estimated time is lower: rounds x w steps x chunks of
+// per-chunk block time (about ndig / 3 us on MI355X, measured at 258-500 blocks) against
+// the split's rounds x chunks plus writing and reading the terms (8 bytes per k, t, w at
+// about 2.3 TB/s, plus the sum kernel's launch).
 static const int kWwCUs = 256;
-static int ww_split_parts(int nk, int nt, int nw, size_t prb)
+static int ww_split_parts(int nk, int nt, int nw, size_t prb, int ndig)
 {
   if (nk <= 0 || nt <= 0 || nw <= 0) return 0;
   const long long fused = (long long)cdiv(nk, 128) * cdiv(nt, 3);
   const int nchunk = (int)(prb * 8 / 128);
-  if (fused >= kWwCUs || nchunk < 1 || nw < 1) return 0;
+  if (nchunk < 1 || nw < 1) return 0;
   // the terms buffer (8 bytes per k, t and w) stays under 1 GiB
-  if ((double)nt * (double)(cdiv(nw, 32) * 32) * (double)nk * 8.0 > (double)(1ll << 30)) return 0;
-  const long long per = fused * cdiv(nw, 32);
+  const double pbytes = (double)nt * (double)(cdiv(nw, 32) * 32) * (double)nk * 8.0;
+  if (pbytes > (double)(1ll << 30)) return 0;
+  const long long nws = cdiv(nw, 32);
+  if (fused >= kWwCUs) {
+    const double c_us = ndig / 3.0;
+    const double t_fused = (double)cdiv(fused, kWwCUs) * nws * nchunk * c_us;
+    const double t_split = (double)cdiv(fused * nws, kWwCUs) * nchunk * c_us + 2.0 * pbytes / 2.3e6 + 20.0;
+    return t_split < 0.9 * t_fused ? 1 : 0;
+  }
+  const long long per = fused * nws;
   if (per >= 2 * fused && per >= kWwCUs) return 1;
   long long parts = (kWwCUs + per - 1) / per;
   parts = std::min<long long>(parts, std::max(1, nchunk / 4));
@@ -458,7 +472,7 @@ size_t ww_mfma_work_bytes(int nk, int nt, int nw, size_t prb, int B)
   b += sizeof(double) * ((size_t)nt * nw + nt + 8);
   b += sizeof(long long) * ((size_t)nw + 8);
   b += 2 * sizeof(int) * ((size_t)nt * nk + 8);
-  if (ww_split_parts(nk, nt, nw, prb) > 0) b += 256 + sizeof(unsigned long long) * (size_t)nt * nwp * nk;
+  if (ww_split_parts(nk, nt, nw, prb, ndig) > 0) b += 256 + sizeof(unsigned long long) * (size_t)nt * nwp * nk;
   return b + 1024;
 }
 
@@ -477,7 +491,7 @@ hipError_t ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t s
   long long* Gm = (long long*)(p + off); off += sizeof(long long) * ((size_t)a.nw + 8);
   int* cntp = (int*)(p + off); off += sizeof(int) * ((size_t)a.nt * a.nk + 8);
   int* llk = (int*)(p + off); off += sizeof(int) * ((size_t)a.nt * a.nk + 8);
-  const int nsplit = ww_split_parts(a.nk, a.nt, a.nw, a.prb);
+  const int nsplit = ww_split_parts(a.nk, a.nt, a.nw, a.prb, ndig);
   unsigned long long* Pbuf = (unsigned long long*)(((uintptr_t)(p + off) + 255) & ~(uintptr_t)255);
   if (nwp > 0) {
     long long tot = (long long)nwp * (ldn / 4);
@@ -543,11 +557,17 @@ __global__ __launch_bounds__(256) void k_cz_sums(const uint32_t* cz, long long l
   if (threadIdx.x == 0) Gpart[s * ZT_PARTS + b] = part[0] + part[1] + part[2] + part[3];
 }
 
-template <int ND>
+// The rows' packed bits stream from HBM once per launch; the kernel keeps several chunks
+// of them in flight per wave (loads issued one iteration ahead) and, when the rows alone
+// give too few blocks to fill the device, also splits n over blockIdx.y.  Each block adds
+// its waves' integer sums in LDS; with one part it forms the outputs itself, else it adds
+// them to Pg / Cg / Mg with 64-bit integer atomics (exact in any order) and k_ztsn_fin
+// forms the outputs.
+template <int ND, bool PARTS>
 __global__ __launch_bounds__(256) void k_ztsn_mfma(const uint8_t* A, size_t rbA, const int* alist, int na,
                                                    const uint8_t* mask, const int8_t* Dg, long long ldn, int ns,
                                                    const long long* Gsum, const int* slist, const double* scal,
-                                                   double* out, long long ldo)
+                                                   double* out, long long ldo, unsigned long long* Pg)
 {
   __shared__ int Ps[32][33];
   __shared__ int Cs[32], Ms[32];
@@ -564,12 +584,25 @@ __global__ __launch_bounds__(256) void k_ztsn_mfma(const uint8_t* A, size_t rbA,
   for (int v = 0; v < 16; v++) acc[v] = 0;
   int cnt = 0, mcnt = 0;
   const int nchunk = (int)(ldn / 128);
-  for (int c = wave; c < nchunk; c += 4) {
-    const uint64_t m8 = *(const uint64_t*)(mrow + 16LL * c);
-    const uint64_t a8 = *(const uint64_t*)(Arow + 16LL * c) & m8;
+  const int c_lo = (int)((long long)blockIdx.y * nchunk / gridDim.y);
+  const int c_hi = (int)((long long)(blockIdx.y + 1) * nchunk / gridDim.y);
+  // chunk c of this wave: its A and mask words and its 4 digit fragments
+  uint64_t a_n = 0, m_n = 0;
+  v4i b_n[4];
+  auto load = [&](int c) {
+    m_n = *(const uint64_t*)(mrow + 16LL * c);
+    a_n = *(const uint64_t*)(Arow + 16LL * c);
+#pragma unroll
+    for (int s = 0; s < 4; s++) b_n[s] = *(const v4i*)(Bcol + 128LL * c + 16 * s);
+  };
+  int c = c_lo + wave;
+  if (c < c_hi) load(c);
+  for (; c < c_hi; c += 4) {
+    const uint64_t m8 = m_n, a8 = a_n & m_n;
     v4i bf[4];
 #pragma unroll
-    for (int s = 0; s < 4; s++) bf[s] = *(const v4i*)(Bcol + 128LL * c + 16 * s);
+    for (int s = 0; s < 4; s++) bf[s] = b_n[s];
+    if (c + 4 < c_hi) load(c + 4);
     cnt += __popcll(a8); mcnt += __popcll(m8);
     const uint32_t wr0 = __builtin_bitreverse32((uint32_t)a8), wr1 = __builtin_bitreverse32((uint32_t)(a8 >> 32));
 #pragma unroll
@@ -583,6 +616,16 @@ __global__ __launch_bounds__(256) void k_ztsn_mfma(const uint8_t* A, size_t rbA,
   atomicAdd(&Cs[r], cnt);
   atomicAdd(&Ms[r], mcnt);
   __syncthreads();
+  if (PARTS) {
+    // rows i0 .. i0 + 31: 32 digit columns, then the two counts
+    for (int e = tid; e < 32 * 34; e += 256) {
+      const int rr = e / 34, f = e % 34;
+      if (i0 + rr >= na) continue;
+      const long long v = f < 32 ? (long long)Ps[rr][f] : (f == 32 ? (long long)Cs[rr] : (long long)Ms[rr]);
+      if (v) atomicAdd(Pg + (size_t)(i0 + rr) * 34 + f, (unsigned long long)v);
+    }
+    return;
+  }
   const double mn = scal[2], mlt = scal[4];
   long long S = 0;
 #pragma unroll
@@ -603,10 +646,55 @@ __global__ __launch_bounds__(256) void k_ztsn_mfma(const uint8_t* A, size_t rbA,
   }
 }
 
-size_t ztsn_mfma_work_bytes(int ns, size_t prb, int B)
+// the outputs of the split form from the summed integers, one thread per (row, covariate):
+// the same expressions as the one-part kernel's
+template <int ND>
+__global__ void k_ztsn_fin(const unsigned long long* Pg, int na, int ns, const long long* Gsum, const int* alist,
+                           const int* slist, const double* scal, double* out, long long ldo)
+{
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= na * ns) return;
+  const int rr = e % na, s = e / na;
+  const long long* row = (const long long*)Pg + (size_t)rr * 34;
+  const double mn = scal[2], mlt = scal[4];
+  long long S = 0;
+#pragma unroll
+  for (int i = 0; i < ND; i++) S += 1LL << (8 * i);
+  long long P = 0;
+#pragma unroll
+  for (int i = ND - 1; i >= 0; i--) P = P * 256 + row[i * ns + s];
+  long long G = 0;
+  for (int b = 0; b < ZT_PARTS; b++) G += Gsum[s * ZT_PARTS + b];
+  const long long cnt = row[32], mcnt = row[33];
+  const long long x = 2 * (P + 128 * S * cnt) - G;
+  const double ll = (double)(2LL * cnt - mcnt);
+  const double base = ll * mn * mlt;
+  const double tmp = (double)x;
+  out[(long long)slist[s] * ldo + alist[rr]] = (double)(base + tmp) / mlt;
+}
+
+__global__ void k_ztsn_zero(unsigned long long* p, long long n)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = 0ull;
+}
+
+// n parts: enough blocks for about four per CU, each wave keeping at least 8 chunks
+static int ztsn_parts(int na, size_t prb)
+{
+  if (na <= 0) return 1;
+  const long long rows_blocks = cdiv(na, 32);
+  const int nchunk = (int)(prb * 8 / 128);
+  long long parts = (1024 + rows_blocks - 1) / rows_blocks;
+  parts = std::min<long long>(parts, std::max(1, nchunk / 32));
+  return (int)std::max<long long>(parts, 1);
+}
+
+size_t ztsn_mfma_work_bytes(int na, int ns, size_t prb, int B)
 {
   const int ndig = (B + 7) / 8;
-  return (((size_t)ndig * ns * prb * 8 + 255) & ~(size_t)255) + sizeof(long long) * ((size_t)ns * ZT_PARTS + 8) + 256;
+  size_t b = (((size_t)ndig * ns * prb * 8 + 255) & ~(size_t)255) + sizeof(long long) * ((size_t)ns * ZT_PARTS + 8) + 256;
+  if (ztsn_parts(na, prb) > 1) b += sizeof(unsigned long long) * 34 * (size_t)na + 256;
+  return b;
 }
 
 hipError_t ztsn_mfma(const Bits& A, const int* alist, int na, const uint32_t* mask, const uint32_t* cz, long long ldc,
@@ -616,16 +704,28 @@ hipError_t ztsn_mfma(const Bits& A, const int* alist, int na, const uint32_t* ma
   if (na <= 0 || ns <= 0) return hipSuccess;
   const int ndig = (B + 7) / 8;
   const long long ldn = (long long)A.rb * 8;
-  if (ndig * ns > 32 || work_bytes < ztsn_mfma_work_bytes(ns, A.rb, B)) return hipErrorInvalidValue;
+  if (ndig * ns > 32 || work_bytes < ztsn_mfma_work_bytes(na, ns, A.rb, B)) return hipErrorInvalidValue;
   int8_t* Dg = (int8_t*)work;
   long long* Gsum = (long long*)((uint8_t*)work + (((size_t)ndig * ns * ldn + 255) & ~(size_t)255));
+  unsigned long long* Pg = (unsigned long long*)((((uintptr_t)(Gsum + (size_t)ns * ZT_PARTS + 8)) + 255) & ~(uintptr_t)255);
+  const int parts = ztsn_parts(na, A.rb);
   const int nt_ = n_total < ldn ? n_total : (int)ldn;
   const long long tot = (long long)ns * (ldn / 4);
   hipLaunchKernelGGL(k_ww_digits, dim3(cdiv(tot, 256) < 65536 ? cdiv(tot, 256) : 65536), dim3(256), 0, st,
                      cz, ldc, ns, ns, nt_, (const uint8_t*)mask, ndig, ldn, Dg);
   hipLaunchKernelGGL(k_cz_sums, dim3(ZT_PARTS, ns), dim3(256), 0, st, cz, ldc, nt_, mask, Gsum);
-#define ZT_LAUNCH(N) hipLaunchKernelGGL((k_ztsn_mfma<N>), dim3(cdiv(na, 32)), dim3(256), 0, st, A.p, A.rb, alist, na, \
-                                        (const uint8_t*)mask, Dg, ldn, ns, Gsum, slist, scal, out, ldo)
+  if (parts > 1) {
+    const long long np = 34LL * na;
+    hipLaunchKernelGGL(k_ztsn_zero, dim3(cdiv(np, 256) < 65536 ? cdiv(np, 256) : 65536), dim3(256), 0, st, Pg, np);
+  }
+  const dim3 grid(cdiv(na, 32), parts);
+#define ZT_LAUNCH(N) do { if (parts > 1) { \
+      hipLaunchKernelGGL((k_ztsn_mfma<N, true>), grid, dim3(256), 0, st, A.p, A.rb, alist, na, (const uint8_t*)mask, Dg, \
+                         ldn, ns, Gsum, slist, scal, out, ldo, Pg); \
+      hipLaunchKernelGGL((k_ztsn_fin<N>), dim3(cdiv((long long)na * ns, 256)), dim3(256), 0, st, Pg, na, ns, Gsum, alist, \
+                         slist, scal, out, ldo); \
+    } else hipLaunchKernelGGL((k_ztsn_mfma<N, false>), grid, dim3(256), 0, st, A.p, A.rb, alist, na, (const uint8_t*)mask, \
+                              Dg, ldn, ns, Gsum, slist, scal, out, ldo, Pg); } while (0)
   switch (ndig) { case 1: ZT_LAUNCH(1); break; case 2: ZT_LAUNCH(2); break; case 3: ZT_LAUNCH(3); break; default: ZT_LAUNCH(4); break; }
 #undef ZT_LAUNCH
   return hipGetLastError();
