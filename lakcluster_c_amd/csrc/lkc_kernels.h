@@ -12,6 +12,16 @@
 
 namespace lkc {
 
+// A block barrier that orders LDS only: the waves' LDS writes before it are visible to
+// LDS reads after it, while global loads issued before it stay in flight (__syncthreads
+// would also wait for them).  For kernels that prefetch from memory across barriers.
+__device__ __forceinline__ void block_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 constexpr int kPopBits = 1920;   // weighting block (lakcluster_header.h:268)
 constexpr int kPopBytes = 240;
 constexpr int kPopWords = 60;    // 32-bit words per block

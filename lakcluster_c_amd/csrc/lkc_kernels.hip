@@ -576,16 +576,38 @@ __global__ __launch_bounds__(256) void k_qr(const double* X, long long ldx, cons
   };
   double acc = 0.0;
   if (nch > 0) load(0);
+  // the barriers order LDS only: the next chunk's loads stay in flight while the chains add
   for (int c = 0; c < nch; c++) {
-    __syncthreads();               // the previous chunk's chains are done with pr
+    block_lds_sync();              // the previous chunk's chains are done with pr
     products();
     if (c + 1 < nch) load(c + 1);
-    __syncthreads();
+    block_lds_sync();
     if (chain) {
       const double* pp = pr + (size_t)ct * QR_ZC * QR_AP + cc;
       const int zc = min(QR_ZC, nz - c * QR_ZC);
+      if (zc == QR_ZC) {
+        // a whole chunk: the next 16 values are read while the current 16 are added, so
+        // the adds do not wait for LDS (same order, one add per z)
+        double u[16], v[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) u[k] = pp[k * QR_AP];
+#pragma unroll
+        for (int g = 0; g < QR_ZC / 16; g += 2) {
+#pragma unroll
+          for (int k = 0; k < 16; k++) v[k] = pp[(16 * (g + 1) + k) * QR_AP];
+#pragma unroll
+          for (int k = 0; k < 16; k++) acc += u[k];
+          if (g + 2 < QR_ZC / 16) {
+#pragma unroll
+            for (int k = 0; k < 16; k++) u[k] = pp[(16 * (g + 2) + k) * QR_AP];
+          }
+#pragma unroll
+          for (int k = 0; k < 16; k++) acc += v[k];
+        }
+      } else {
 #pragma unroll 16
-      for (int zi = 0; zi < zc; zi++) acc += pp[zi * QR_AP];
+        for (int zi = 0; zi < zc; zi++) acc += pp[zi * QR_AP];
+      }
     }
   }
   if (chain) out[(long long)tlist[ct] * ldo + alist[i0 + cc]] = acc;
