@@ -287,7 +287,7 @@ typedef struct lkc_job_result {
 } lkc_job_result;
 /* Runs the jobs, at most max_concurrent at a time, each admitted in order once it has
    arrived and its device-memory estimate fits hbm_budget_bytes (<= 0: 80% of the free
-   device memory).  Results are allocated by the library; free them with
+   device memory plus what the library's stream-ordered memory pool holds idle).  Results are allocated by the library; free them with
    lkc_batch_results_free.  Returns LKC_OK if the batch ran (per-job status inside). */
 int lkc_batch_run(const lkc_pool *P, const lkc_job *jobs, int njobs, int max_concurrent,
                   double hbm_budget_bytes, lkc_job_result *results);

@@ -538,10 +538,21 @@ int lkc_batch_run(const lkc_pool* P, const lkc_job* jobs, int njobs, int max_con
   return guard(nullptr, [&] {
     if (!P || njobs < 0 || (njobs && (!jobs || !results))) throw lkc::Error(LKC_ERR_STATE, "bad argument");
     if (hbm_budget_bytes <= 0) {
+      // free device memory plus what the stream-ordered pool holds idle from earlier runs
+      // (its release threshold keeps freed blocks for reuse, so hipMemGetInfo counts them
+      // as used although every job can take them)
       size_t fr = 0, tot = 0;
       HIPCHK(hipSetDevice(0));
       HIPCHK(hipMemGetInfo(&fr, &tot));
-      hbm_budget_bytes = 0.8 * (double)fr;
+      double idle = 0.0;
+      hipMemPool_t mp;
+      if (hipDeviceGetDefaultMemPool(&mp, 0) == hipSuccess) {
+        uint64_t reserved = 0, used = 0;
+        if (hipMemPoolGetAttribute(mp, hipMemPoolAttrReservedMemCurrent, &reserved) == hipSuccess &&
+            hipMemPoolGetAttribute(mp, hipMemPoolAttrUsedMemCurrent, &used) == hipSuccess && reserved > used)
+          idle = (double)(reserved - used);
+      }
+      hbm_budget_bytes = 0.8 * ((double)fr + idle);
     }
     std::vector<lkc::JobIn> in(njobs);
     for (int j = 0; j < njobs; j++) {
