@@ -150,6 +150,12 @@ int lkc_bcc_lrup_QC_YnWt_stage_c(lkc_bcc *D);    /* bcc_lrup.c:592-655 */
 int lkc_bcc_sumscores(lkc_bcc *D);
 int lkc_bcc_lrup(lkc_bcc *D);
 
+/* dQuickSort_xij (Quicksort.c:124-141) with one companion index array: sorts d[l..r] in
+   place by the reference's first-element-pivot partition (idx permuted alongside), stops
+   recursing at depth GLOBAL_recursion_limit = 32768 as the reference does, and returns the
+   deepest level reached.  The order the elimination's score sort must reproduce; host only. */
+unsigned int lkc_dquicksort_xij(double *d, int *idx, int l, int r);
+
 /* get_xdrop (xdrop.c:5-64): the rows and columns one pass drops from lrij rows and
    lcij columns, with gamma and gamma_type passed in (GLOBAL_gamma, GLOBAL_gamma_type:
    0 log, 1 sqrt, 2 lin, 3 rows); either output may be NULL. */

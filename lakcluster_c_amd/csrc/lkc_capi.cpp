@@ -93,6 +93,12 @@ int lkc_bcc_sumscores_dmp(const lkc_bcc* D, const char* out_dir) { return lkc_bc
 
 int lkc_bcc_active(const lkc_bcc* D) { return D && D->run.active() ? 1 : 0; }
 
+unsigned int lkc_dquicksort_xij(double* d, int* idx, int l, int r)
+{
+  if (!d || !idx || l > r) return 0;
+  return lkc::quicksort_xij(d, idx, l, r);
+}
+
 int lkc_get_xdrop(double lrij, double lcij, double gamma, int gamma_type, int* rdrop, int* cdrop)
 {
   if (!std::isfinite(lrij) || !std::isfinite(lcij)) { g_last_error = "get_xdrop: non-finite row or column count"; return LKC_ERR_STATE; }

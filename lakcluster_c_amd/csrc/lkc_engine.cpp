@@ -243,13 +243,35 @@ static int partition_first(double* d, int* perm, int l, int r)
   std::swap(d[l], d[j]); std::swap(perm[l], perm[j]);
   return j;
 }
-static void quicksort_first(double* d, int* perm, int l, int r, unsigned depth)
+// dQuickSort_xij (Quicksort.c:124-141): returns the deepest level reached (the recursion
+// stops below GLOBAL_recursion_limit = 32768, leaving such a part as partitioned so far)
+static unsigned quicksort_first(double* d, int* perm, int l, int r, unsigned depth)
 {
-  if (l >= r) return;
-  int j = partition_first(d, perm, l, r);
-  if (depth < 32768u) { quicksort_first(d, perm, l, j - 1, depth + 1); quicksort_first(d, perm, j + 1, r, depth + 1); }
+  unsigned n1 = depth, n2 = depth;
+  if (l < r) {
+    const int j = partition_first(d, perm, l, r);
+    if (depth < 32768u) { n1 = quicksort_first(d, perm, l, j - 1, depth + 1); n2 = quicksort_first(d, perm, j + 1, r, depth + 1); }
+  }
+  return std::max(n1, n2);
 }
-void sort_scores(double* d, int* perm, int n) { quicksort_first(d, perm, 0, n - 1, 0); }
+// The same recursion with its two halves sorted on two host threads near the top: the
+// halves are disjoint and each keeps its own depth count, so the result is the sequential
+// one's exactly (a list of tied scores longer than 32769 entries is common in the dcc
+// elimination, whose column scores are functions of a few per-column counts).
+static unsigned quicksort_par(double* d, int* perm, int l, int r, unsigned depth, int spawn)
+{
+  if (spawn <= 0 || r - l < 16384) return quicksort_first(d, perm, l, r, depth);
+  unsigned n1 = depth, n2 = depth;
+  const int j = partition_first(d, perm, l, r);
+  if (depth < 32768u) {
+    std::thread left([&n1, d, perm, l, j, depth, spawn] { n1 = quicksort_par(d, perm, l, j - 1, depth + 1, spawn - 1); });
+    n2 = quicksort_par(d, perm, j + 1, r, depth + 1, spawn - 1);
+    left.join();
+  }
+  return std::max(n1, n2);
+}
+unsigned sort_scores(double* d, int* perm, int n) { return n > 0 ? quicksort_par(d, perm, 0, n - 1, 0, 2) : 0; }
+unsigned quicksort_xij(double* d, int* perm, int l, int r) { return quicksort_par(d, perm, l, r, 0, 2); }
 
 // ------------------------------------------------------------------
 // device memory
@@ -538,11 +560,12 @@ void Run::setup_device(double t0)
   auto reserve = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
   size_t o_m[11]; int mn[11] = { NA, NA, NA, NA, NZ, NZ, M, M, M, M, T };
   for (int i = 0; i < 11; i++) o_m[i] = reserve(bsize(mn[i]) + 8);
+  size_t o_zero = reserve(sizeof(int) * 4);
+  ctl_head = off;                             // masks and the constant zero: fixed offsets
   IList* L[8] = { &ar, &kl, &tl, &zr, &rtn, &rmv, &crt, &crm };
   int lcap[8] = { NA, M, T, NZ, NA, NA, M, M };
   size_t o_l[8];
   for (int i = 0; i < 8; i++) o_l[i] = reserve(sizeof(int) * (size_t)(lcap[i] + 1));
-  size_t o_zero = reserve(sizeof(int) * 4);
   cmp_Wmax = 4 * pcols + (M + 31) / 32 + 4;   // whole-word segments, rows padded to 4-word groups
   ctl_cmp_off[0] = reserve(sizeof(int) * 3 * (size_t)(pcols + 1));
   ctl_cmp_off[1] = reserve(sizeof(int) * 32 * (size_t)cmp_Wmax);
@@ -857,13 +880,34 @@ void Run::push_ctl()
     rmv_W = pack(rmv, (int*)(hctl + ctl_cmp_off[3]), (uint32_t*)(hctl + ctl_cmp_off[4]));
     rtn_W = pack(rtn, (int*)(hctl + ctl_cmp_off[5]), (uint32_t*)(hctl + ctl_cmp_off[6]));
   }
+  // Only what this pass uses travels: after the fixed head, the lists and tables are
+  // packed in order at their used sizes (each at most its reserved size, so a table moved
+  // down never overwrites one not yet moved), their device pointers set to match, and the
+  // packed block goes up in one copy.
+  size_t off = ctl_head;
+  auto next = [&](size_t bytes) { const size_t o = off; off += std::max<size_t>(256, (bytes + 255) & ~(size_t)255); return o; };
   IList* L[8] = { &ar, &kl, &tl, &zr, &rtn, &rmv, &crt, &crm };
-  size_t last = 0;
   for (int i = 0; i < 8; i++) {
-    if (!L[i]->h.empty()) memcpy(hctl + ctl_list_off[i], L[i]->h.data(), sizeof(int) * L[i]->h.size());
-    last = std::max(last, ctl_list_off[i] + sizeof(int) * L[i]->h.size());
+    const size_t n = L[i]->h.size(), o = next(sizeof(int) * (n + 1));
+    if (n) memcpy(hctl + o, L[i]->h.data(), sizeof(int) * n);
+    L[i]->d = (int*)(dctl + o);
   }
-  HIPCHK(hipMemcpyAsync(dctl, hctl, ctl_bytes, hipMemcpyHostToDevice, st));
+  auto move = [&](int k, size_t bytes) {
+    const size_t o = next(bytes);
+    if (bytes && o != ctl_cmp_off[k]) memmove(hctl + o, hctl + ctl_cmp_off[k], bytes);
+    return dctl + o;
+  };
+  d_seg = (int*)move(0, sizeof(int) * 3 * (size_t)cmp_nseg);
+  d_src = (int*)move(1, sizeof(int) * 32 * (size_t)cmp_W);
+  d_cmask = (uint32_t*)move(2, sizeof(uint32_t) * (size_t)cmp_W);
+  d_src_rmv = (int*)move(3, sizeof(int) * 32 * (size_t)rmv_W);
+  d_cmask_rmv = (uint32_t*)move(4, sizeof(uint32_t) * (size_t)rmv_W);
+  d_src_rtn = (int*)move(5, sizeof(int) * 32 * (size_t)rtn_W);
+  d_cmask_rtn = (uint32_t*)move(6, sizeof(uint32_t) * (size_t)rtn_W);
+  d_seg_k = (int*)move(7, sizeof(int) * 3 * (size_t)kc_nseg);
+  d_src_k = (int*)move(8, sizeof(int) * 32 * (size_t)kc_W);
+  if (off > ctl_bytes) throw Error(LKC_ERR_STATE, "control block overflow");
+  HIPCHK(hipMemcpyAsync(dctl, hctl, off, hipMemcpyHostToDevice, st));
 }
 
 // ------------------------------------------------------------------

@@ -36,7 +36,8 @@ void write_scorebox(const Scorebox& S, const std::string& dir);
 void parse_cfg(const char* text, Cfg& c);
 void get_xdrop(double lrij, double lcij, double gamma, int gamma_type, int* rdrop, int* cdrop);
 int get_xdrop_length(double lrij, double lcij, double gamma, int gamma_type);
-void sort_scores(double* d, int* perm, int n);
+unsigned sort_scores(double* d, int* perm, int n);
+unsigned quicksort_xij(double* d, int* perm, int l, int r);
 
 // Device allocations of one owner.  With a stream set, they are stream-ordered
 // (hipMallocAsync / hipFreeAsync from the device's memory pool): growing or freeing a
@@ -107,7 +108,7 @@ struct Run {
   double* ajdk = nullptr;
   int trm_AnZt = 0, trm_knzt = 0, trm_hnvt = 0;
   // control block
-  uint8_t* hctl = nullptr; uint8_t* dctl = nullptr; size_t ctl_bytes = 0;
+  uint8_t* hctl = nullptr; uint8_t* dctl = nullptr; size_t ctl_bytes = 0, ctl_head = 0;
   size_t ctl_mask_off[11] = {}, ctl_list_off[8] = {}, ctl_cmp_off[9] = {};
   // column compaction of the dropped columns (crm), block by block
   int cmp_W = 0, cmp_nseg = 0, cmp_Wmax = 0;

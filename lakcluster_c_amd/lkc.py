@@ -117,6 +117,7 @@ def lib():
         "lkc_bcc_sumscores_dmp": (C.c_int, [vp, cp]),
         "lkc_get_xdrop": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int, ip, ip]),
         "lkc_get_xdrop_length": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int]),
+        "lkc_dquicksort_xij": (C.c_uint, [dp, ip, C.c_int, C.c_int]),
         "lkc_get_xdrop_array": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, ip, ip, ip, ip, ip]),
     }
     sig.update({"lkc_bcc_" + name: (C.c_int, [vp]) for name in LOOP_STAGES})
@@ -338,6 +339,15 @@ def lakcluster_driver(in_text: str, out_dir: Optional[str] = None) -> None:
 def dexcluster_driver(in_text: str, out_dir: Optional[str] = None) -> None:
     """Whole dcc run with the three output files written (dexcluster_driver)."""
     _check(lib().lkc_dexcluster_driver(in_text.encode(), out_dir.encode() if out_dir else None))
+
+
+def dquicksort_xij(d: np.ndarray) -> Tuple[np.ndarray, np.ndarray, int]:
+    """dQuickSort_xij (Quicksort.c:124-141) on a copy of d: (sorted values, permutation,
+    deepest recursion level)."""
+    d = np.ascontiguousarray(d, dtype=np.float64).copy()
+    ix = np.arange(d.size, dtype=np.int32)
+    depth = lib().lkc_dquicksort_xij(_ptr(d, C.c_double), _ptr(ix, C.c_int), 0, d.size - 1) if d.size else 0
+    return d, ix, int(depth)
 
 
 def get_xdrop(lrij: float, lcij: float, gamma: float, gamma_type: int = 0) -> Tuple[int, int]:

@@ -139,7 +139,9 @@ def quicksort(d: np.ndarray):
     d = np.ascontiguousarray(d, dtype=np.float64).copy()
     ix = np.arange(d.size, dtype=np.int32)
     null = C.POINTER(C.c_int)()
-    lib().lko_quicksort_xij(_p(d, C.c_double), _p(ix, C.c_int), null, null, null, null, null, 0, d.size - 1)
+    depth = lib().lko_quicksort_xij(_p(d, C.c_double), _p(ix, C.c_int), null, null, null, null, null, 0, d.size - 1) \
+        if d.size else 0
+    quicksort.depth = int(depth)
     return d, ix
 
 

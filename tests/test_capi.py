@@ -173,3 +173,29 @@ def test_lkc_driver_files_match_oracle(datasets, oracle, tmp_path, name):
     ref = _oracle_files(oracle, open(ds.in_path).read(), tmp_path / "oracle")
     for f, b in ref.items():
         assert open(out / f, "rb").read() == b, f
+
+
+def test_dquicksort_xij_matches_oracle(oracle):
+    """lkc_dquicksort_xij, the host sort the elimination falls back to, against the oracle's
+    dQuickSort_xij (Quicksort.c:83-141): values, permutation and deepest recursion level
+    equal, on lists that take the two-thread path (longer than 16384) with many ties, on
+    short ones, and on inputs that reach the 32768-level recursion limit (sorted, reversed
+    and constant lists longer than 32769, which the reference leaves partly unsorted)."""
+    import numpy as np
+    from lakcluster_c_amd import lkc
+    rng = np.random.default_rng(3)
+    cases = [rng.integers(0, 400, 70000).astype(float),          # ties everywhere, parallel path
+             rng.normal(size=70000),                             # distinct, parallel path
+             rng.integers(0, 9, 5000).astype(float),             # short, ties
+             np.arange(40000, dtype=float),                      # sorted: depth limit reached
+             np.arange(40000, 0, -1, dtype=float),               # reversed
+             np.full(40000, 2.5),                                # constant
+             np.r_[rng.normal(size=30000), np.full(20000, 0.0)],  # a long run of equal scores
+             np.array([3.0, 1.0, 3.0, 2.0, 3.0])]
+    for v in cases:
+        d, ix, depth = lkc.dquicksort_xij(v)
+        od, oix = oracle.quicksort(v)
+        assert d.tobytes() == od.tobytes()
+        np.testing.assert_array_equal(ix, oix)
+        assert depth == oracle.quicksort.depth
+    assert lkc.dquicksort_xij(np.arange(40000, dtype=float))[2] == 32768
