@@ -664,6 +664,7 @@ void Run::setup_device(double t0)
   // persistent device arrays
   size_t nA = NA, nZ = std::max(NZ, 1), nM = M, nT = T;
   mlf = dev.alloc<double>(16 * 8);
+  mask_lfs();
   lf_An_ajdk = dev.alloc<double>(nA * kAjdk); lf_Zn_ajdk = dev.alloc<double>(nZ * kAjdk);
   jn_ajdk = dev.alloc<double>(nA * kAjdk); vn_ajdk = dev.alloc<double>(nZ * kAjdk);
   kn_ajdk = dev.alloc<double>(nA * kAjdk); hn_ajdk = dev.alloc<double>(nA * kAjdk);
@@ -908,6 +909,21 @@ void Run::push_ctl()
   d_src_k = (int*)move(8, sizeof(int) * 32 * (size_t)kc_W);
   if (off > ctl_bytes) throw Error(LKC_ERR_STATE, "control block overflow");
   HIPCHK(hipMemcpyAsync(dctl, hctl, off, hipMemcpyHostToDevice, st));
+  mask_lfs();
+}
+
+// the mask_lf slots of every mask a pass reads (each a function of its mask only), in
+// one launch after each upload: 0 active columns, 1 / 2 removed / retained columns (with
+// the ajdk weights), 3 active A rows, 4 and 7 active Z rows, 5 / 6 retained / removed A rows
+void Run::mask_lfs()
+{
+  if (!mlf || !ajdk) return;     // before the tables exist (the first upload at load)
+  MaskLfJobs J{};
+  auto job = [&](const uint32_t* m, int pc, const double* w, int slot) { J.job[J.n++] = MaskLfJob{ m, pc, w, mlf + 16 * slot }; };
+  job(Ac_j, pcols, ajdk, 0); job(Ac_rmv, pcols, ajdk, 1); job(Ac_rtn, pcols, ajdk, 2);
+  job(Ar_j, psize(NA), nullptr, 3); job(Ar_rtn, psize(NA), nullptr, 5); job(Ar_rmv, psize(NA), nullptr, 6);
+  if (has_Z) { job(Zr_j, psize(NZ), nullptr, 4); job(Zr_j, psize(NZ), nullptr, 7); }
+  mask_lf_jobs(J, st);
 }
 
 // ------------------------------------------------------------------
@@ -916,7 +932,6 @@ void Run::push_ctl()
 // bcc_An_ajdk (bcc_lf.c:7-51)
 void Run::st_ajdk()
 {
-  mask_lf(Ac_j, pcols, ajdk, mlf + 0 * 16, st);
   an_ajdk(An.b(), ar.d, ar.n(), Ac_j, pcols, ajdk, mlf, lf_An_ajdk, NA, st);
   ops += 15LL * ar.n() * (long long)An.rb * 8;
   if (has_Z) {
@@ -928,11 +943,9 @@ void Run::st_ajdk()
 // bcc_lf_ZtSn (bcc_lf.c:53-101): plain +-1 products At_k . T_t over the active rows
 void Run::st_ZtSn()
 {
-  mask_lf(Ar_j, psize(NA), nullptr, mlf + 3 * 16, st);
   pm_few(At.b(), kl.d, kl.n(), Tt.b(), tl.d, tl.n(), Ar_j, psize(NA), mlf + 3 * 16, lf_AtTn, M, st);
   ops += (long long)kl.n() * tl.n() * (long long)bsize(NA) * 8;
   if (has_Z) {
-    mask_lf(Zr_j, psize(NZ), nullptr, mlf + 4 * 16, st);
     pm_few(Zt.b(), kl.d, kl.n(), St.b(), tl.d, tl.n(), Zr_j, psize(NZ), mlf + 4 * 16, lf_ZtSn, M, st);
     ops += (long long)kl.n() * tl.n() * (long long)bsize(NZ) * 8;
   }
@@ -1454,8 +1467,6 @@ void Run::gram_any(const GramMArgs& g)
 void Run::lrup_QR_0()
 {
   LRUP_LOCALS;
-  mask_lf(Ac_rmv, pcols, ajdk, mlf_rmv, st);
-  mask_lf(Ac_rtn, pcols, ajdk, mlf_rtn, st);
   an_ajdk(An.b(), rtn.d, rtn.n(), Ac_rmv, pcols, ajdk, mlf_rmv, jn_ajdk, NA, st);
   ops += 15LL * rtn.n() * rbm;
   if (has_Z) {
@@ -1650,8 +1661,6 @@ void Run::lrup_QC_a3()
 void Run::lrup_QC_b0()
 {
   LRUP_LOCALS;
-  mask_lf(Ar_rtn, sA, nullptr, mlf_nrtn, st);
-  mask_lf(Ar_rmv, sA, nullptr, mlf_nrmv, st);
   if (ww_kernel == 0) {
     // over the packed retained / removed patients (the counts are the same integers)
     const Bits Ptn{ (const uint8_t*)At_rtnc, M, 32 * rtn_W, (size_t)rtn_W * 4 }, Ttn{ (const uint8_t*)Tt_rtnc, T, 32 * rtn_W, (size_t)rtn_W * 4 };
@@ -1668,7 +1677,6 @@ void Run::lrup_QC_b0()
   }
   ops += 2LL * (crt.n() + crm.n()) * tl.n() * rbn;
   if (has_Z) {
-    mask_lf(Zr_j, sZ, nullptr, mlf_nz, st);
     pm_few(Zt.b(), crt.d, crt.n(), St.b(), tl.d, tl.n(), Zr_j, sZ, mlf_nz, ztsn, M, st);
     pm_few(Zt.b(), crm.d, crm.n(), St.b(), tl.d, tl.n(), Zr_j, sZ, mlf_nz, vtsn, M, st);
     ops += (long long)(crt.n() + crm.n()) * tl.n() * (long long)Zt.rb * 8;

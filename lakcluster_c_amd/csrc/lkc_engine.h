@@ -179,6 +179,7 @@ struct Run {
   void ww_quantise(int slot, const WwTerm& w);
   void ww_contract(int slot, const WwTerm& w);
   void pack_rows(const int* src, int W, uint32_t* Atc, uint32_t* Ttc);
+  void mask_lfs();
   void init_lf();
   void score_lists();
   void st_nrm(); void st_ifT(); void st_srt(); void st_cmb(); void st_xij(); void st_mxA(bool dup); void st_mxdup();

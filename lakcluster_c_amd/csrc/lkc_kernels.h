@@ -36,6 +36,9 @@ struct Bits { const uint8_t* p; int nr; int nc; size_t rb; };
 // Σ_b w_jk[b]·|mask ∩ block b| for the 15 tables (out[0..14], only when ajdk
 // is non-null) and the plain count Σ_b 1.0·|mask ∩ block b| (out[15]); ordered over b.
 void mask_lf(const uint32_t* mask, int pcols, const double* ajdk, double* out16, hipStream_t st);
+struct MaskLfJob { const uint32_t* mask; int pcols; const double* ajdk; double* out16; };
+struct MaskLfJobs { MaskLfJob job[8]; int n; };
+void mask_lf_jobs(const MaskLfJobs& J, hipStream_t st);
 
 // An_ajdk_v: out[jk*ld + rows[i]] = -base[jk] + 2 Σ_b ajdk[jk][b]·pop(X_row ∩ mask ∩ b)
 void an_ajdk(const Bits& X, const int* rows, int nrows, const uint32_t* mask, int pcols,

@@ -65,6 +65,36 @@ void mask_lf(const uint32_t* mask, int pcols, const double* ajdk, double* out16,
 {
   hipLaunchKernelGGL(k_mask_lf, dim3(1), dim3(64), sizeof(int) * (pcols + 1), st, mask, pcols, ajdk, out16);
 }
+// several masks in one launch, one block each
+__global__ void k_mask_lf_jobs(MaskLfJobs J)
+{
+  const MaskLfJob& j = J.job[blockIdx.x];
+  extern __shared__ int cnt[];
+  for (int b = threadIdx.x; b < j.pcols; b += blockDim.x) {
+    int c = 0;
+    for (int w = 0; w < kPopWords; w++) c += popc(j.mask[(size_t)b * kPopWords + w]);
+    cnt[b] = c;
+  }
+  __syncthreads();
+  int jk = threadIdx.x;
+  if (jk < kAjdk) {
+    if (!j.ajdk) return;
+    double s = 0;
+    for (int b = 0; b < j.pcols; b++) s += (double)j.ajdk[(size_t)jk * j.pcols + b] * (double)cnt[b];
+    j.out16[jk] = s;
+  } else if (jk == kAjdk) {
+    double s = 0;
+    for (int b = 0; b < j.pcols; b++) s += (double)1 * (double)cnt[b];
+    j.out16[kAjdk] = s;
+  }
+}
+void mask_lf_jobs(const MaskLfJobs& J, hipStream_t st)
+{
+  if (J.n <= 0) return;
+  int pmax = 0;
+  for (int i = 0; i < J.n; i++) pmax = J.job[i].pcols > pmax ? J.job[i].pcols : pmax;
+  hipLaunchKernelGGL(k_mask_lf_jobs, dim3(J.n), dim3(64), sizeof(int) * (pmax + 1), st, J);
+}
 
 // ------------------------------------------------------------------
 // An_ajdk_v (An_ajdk_v.c:272-363): one wave per row
@@ -260,20 +290,28 @@ __global__ void k_minmax(const double* src, long long sn, long long sw, const in
   }
   if (threadIdx.x == 0) { part[3 * blockIdx.x] = smin[0]; part[3 * blockIdx.x + 1] = smax[0]; part[3 * blockIdx.x + 2] = sset[0]; }
 }
+// The partials (at most 256, one per k_minmax block, each with a flag for "saw a value")
+// fold in a tree over the block's threads: a min / max of the values is the same in any
+// order (as within the k_minmax blocks), then one lane derives the quantiser's range.
 __global__ void k_minmax_final(const double* part, int np, long long D_MLT, double* scal)
 {
-  // the partials come in through LDS in one parallel load; one lane then folds them in
-  // order, as a single scan of the values would
-  __shared__ double pp[3 * 256];
-  for (int e = threadIdx.x; e < 3 * np; e += blockDim.x) pp[e] = part[e];
+  __shared__ double smin[256], smax[256]; __shared__ int sset[256];
+  const int t = threadIdx.x;
+  smin[t] = t < np ? part[3 * t] : 0.0; smax[t] = t < np ? part[3 * t + 1] : 0.0;
+  sset[t] = t < np ? (part[3 * t + 2] != 0) : 0;
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  double mn = 0, mx = 0; int set = 0;
-  for (int p = 0; p < np; p++) {
-    if (pp[3 * p + 2] == 0) continue;
-    if (!set) { mn = pp[3 * p]; mx = pp[3 * p + 1]; set = 1; }
-    else { mn = pp[3 * p] < mn ? pp[3 * p] : mn; mx = pp[3 * p + 1] > mx ? pp[3 * p + 1] : mx; }
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      const int o = t + s;
+      if (sset[o]) {
+        if (!sset[t]) { smin[t] = smin[o]; smax[t] = smax[o]; sset[t] = 1; }
+        else { smin[t] = smin[o] < smin[t] ? smin[o] : smin[t]; smax[t] = smax[o] > smax[t] ? smax[o] : smax[t]; }
+      }
+    }
+    __syncthreads();
   }
+  if (t != 0) return;
+  double mn = sset[0] ? smin[0] : 0.0, mx = sset[0] ? smax[0] : 0.0;
   if (mn >= mx) mx = mn + 1;
   double lo = mn - 0.0625 * (mx - mn);
   double hi = mx + 0.0625 * (mx - mn);

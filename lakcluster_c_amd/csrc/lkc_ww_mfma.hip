@@ -51,14 +51,34 @@ __global__ void k_ww_digits(const uint32_t* cz, long long ldc, int nw, int nwp, 
   }
 }
 
-// per w: Gm[w] = sum_{n in mask} cz(w,n);  H[t][w] = (double) sum_{n in mask} T_t(n)+-1 cz(w,n)
-// block nw: bp[t] = (double)(sum_{n in mask} T_t(n)+-1) * min * mlt
-__global__ __launch_bounds__(256) void k_ww_hg(WwArgs a, double* H, double* bp, long long* Gm)
+// The per-w and per-k inputs of the contraction, in one launch: blocks 0 .. nw - 1 the
+// per-w sums, block nw the covariate sums, then four rows k per block (a wave each).
+//   per w: Gm[w] = sum_{n in mask} cz(w,n);  H[t][w] = (double) sum_{n in mask} T_t(n)+-1 cz(w,n)
+//   block nw: bp[t] = (double)(sum_{n in mask} T_t(n)+-1) * min * mlt
+//   per k: cntp[t][k] = |mask & (P_k xnor T_t)|, llk[t][k] = pmpm0(P_k, T_t, mask, mask)
+__global__ __launch_bounds__(256) void k_ww_pre(WwArgs a, double* H, double* bp, long long* Gm, int* cntp, int* llk)
 {
-  __shared__ long long red[4][kMaxT + 1];
-  const int j = blockIdx.x, wave = threadIdx.x >> 6;
-  const uint32_t* pm = a.pm;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nwd = (int)(a.prb / 4);       // 32-column words of the n extent
+  if ((int)blockIdx.x > a.nw) {
+    const int k = ((int)blockIdx.x - a.nw - 1) * 4 + wave;
+    if (k >= a.nk) return;
+    const uint32_t* Pr = (const uint32_t*)(a.P.p + (size_t)a.klist[k] * a.P.rb);
+    for (int t = 0; t < a.nt; t++) {
+      const uint32_t* Tr = (const uint32_t*)(a.Tc.p + (size_t)a.tlist[t] * a.Tc.rb);
+      long long p = 0, q = 0;
+      for (int w = lane; w < nwd; w += 64) {
+        uint32_t e = Pr[w] ^ Tr[w], m = a.pm[w];
+        p += popc32(~e & m); q += popc32(e & m);
+      }
+      p = wsum64(p); q = wsum64(q);
+      if (lane == 0) { cntp[(size_t)t * a.nk + k] = (int)p; llk[(size_t)t * a.nk + k] = (int)(p - q); }
+    }
+    return;
+  }
+  __shared__ long long red[4][kMaxT + 1];
+  const int j = blockIdx.x;
+  const uint32_t* pm = a.pm;
   long long g = 0, h[kMaxT];
 #pragma unroll
   for (int t = 0; t < kMaxT; t++) h[t] = 0;
@@ -89,7 +109,7 @@ __global__ __launch_bounds__(256) void k_ww_hg(WwArgs a, double* H, double* bp, 
   g = wsum64(g);
 #pragma unroll
   for (int t = 0; t < kMaxT; t++) if (t < a.nt) h[t] = wsum64(h[t]);
-  if ((threadIdx.x & 63) == 0) {
+  if (lane == 0) {
     red[wave][kMaxT] = g;
 #pragma unroll
     for (int t = 0; t < kMaxT; t++) red[wave][t] = h[t];
@@ -103,24 +123,6 @@ __global__ __launch_bounds__(256) void k_ww_hg(WwArgs a, double* H, double* bp, 
     for (int t = 0; t < a.nt; t++) H[(size_t)t * a.nw + j] = (double)h[t];
   } else {
     for (int t = 0; t < a.nt; t++) { double d = h[t] * 1.0; bp[t] = d * a.scal[2] * a.scal[4]; }
-  }
-}
-
-// per k: cntp[t][k] = |mask & (P_k xnor T_t)|, llk[t][k] = pmpm0(P_k, T_t, mask, mask)
-__global__ void k_ww_cntp(WwArgs a, int* cntp, int* llk)
-{
-  int k = blockIdx.x;
-  const uint32_t* Pr = (const uint32_t*)(a.P.p + (size_t)a.klist[k] * a.P.rb);
-  int nwd = (int)(a.prb / 4);
-  for (int t = 0; t < a.nt; t++) {
-    const uint32_t* Tr = (const uint32_t*)(a.Tc.p + (size_t)a.tlist[t] * a.Tc.rb);
-    long long p = 0, q = 0;
-    for (int w = threadIdx.x; w < nwd; w += 64) {
-      uint32_t e = Pr[w] ^ Tr[w], m = a.pm[w];
-      p += popc32(~e & m); q += popc32(e & m);
-    }
-    p = wsum64(p); q = wsum64(q);
-    if (threadIdx.x == 0) { cntp[(size_t)t * a.nk + k] = (int)p; llk[(size_t)t * a.nk + k] = (int)(p - q); }
   }
 }
 
@@ -499,8 +501,7 @@ hipError_t ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t s
                        a.cz, a.ldc, a.nw, nwp, (int)(a.prb * 8) < a.ldc ? (int)(a.prb * 8) : (int)a.ldc,
                        (const uint8_t*)a.pm, ndig, ldn, Dg);
   }
-  hipLaunchKernelGGL(k_ww_hg, dim3(a.nw + 1), dim3(256), 0, st, a, H, bp, Gm);
-  hipLaunchKernelGGL(k_ww_cntp, dim3(a.nk), dim3(64), 0, st, a, cntp, llk);
+  hipLaunchKernelGGL(k_ww_pre, dim3(a.nw + 1 + cdiv(a.nk, 4)), dim3(256), 0, st, a, H, bp, Gm, cntp, llk);
   long long S = 0;
   for (int i = 0; i < ndig; i++) S += 1LL << (8 * i);
   const long long np = (long long)a.nt * nwp * a.nk;
