@@ -909,6 +909,7 @@ void Run::push_ctl()
   d_src_k = (int*)move(8, sizeof(int) * 32 * (size_t)kc_W);
   if (off > ctl_bytes) throw Error(LKC_ERR_STATE, "control block overflow");
   HIPCHK(hipMemcpyAsync(dctl, hctl, off, hipMemcpyHostToDevice, st));
+  ctl_epoch++;
   mask_lfs();
 }
 
@@ -932,10 +933,17 @@ void Run::mask_lfs()
 // bcc_An_ajdk (bcc_lf.c:7-51)
 void Run::st_ajdk()
 {
-  an_ajdk(An.b(), ar.d, ar.n(), Ac_j, pcols, ajdk, mlf, lf_An_ajdk, NA, st);
+  // After the update the active rows and columns are the retained ones, whose An_ajdk
+  // stage a0 has just formed with the same kernel on the same inputs (its an_ajdk /
+  // zn_ajdk): those are copied instead of recomputed.
+  const bool reuse = a0_valid && hAr_j == a0_Ar && hAc_j == a0_Ac && hZr_j == a0_Zr;
+  a0_valid = false;
+  if (reuse) HIPCHK(hipMemcpyAsync(lf_An_ajdk, an_ajdk_, sizeof(double) * kAjdk * (size_t)NA, hipMemcpyDeviceToDevice, st));
+  else an_ajdk(An.b(), ar.d, ar.n(), Ac_j, pcols, ajdk, mlf, lf_An_ajdk, NA, st);
   ops += 15LL * ar.n() * (long long)An.rb * 8;
   if (has_Z) {
-    an_ajdk(Zn.b(), zr.d, zr.n(), Ac_j, pcols, ajdk, mlf, lf_Zn_ajdk, NZ, st);
+    if (reuse) HIPCHK(hipMemcpyAsync(lf_Zn_ajdk, zn_ajdk, sizeof(double) * kAjdk * (size_t)NZ, hipMemcpyDeviceToDevice, st));
+    else an_ajdk(Zn.b(), zr.d, zr.n(), Ac_j, pcols, ajdk, mlf, lf_Zn_ajdk, NZ, st);
     ops += 15LL * zr.n() * (long long)Zn.rb * 8;
   }
 }
@@ -1471,6 +1479,7 @@ void Run::lrup_QR_0()
   ops += 15LL * rtn.n() * rbm;
   if (has_Z) {
     an_ajdk(Zn.b(), zr.d, zr.n(), Ac_rmv, pcols, ajdk, mlf_rmv, vn_ajdk, NZ, st);
+    vn_epoch = ctl_epoch;
     ops += 15LL * zr.n() * (long long)Zn.rb * 8;
   }
 }
@@ -1532,9 +1541,13 @@ void Run::lrup_QC_a0()
   ops += 15LL * (2LL * rmv.n() + rtn.n()) * rbm;
   if (has_Z) {
     an_ajdk(Zn.b(), zr.d, zr.n(), Ac_rtn, pcols, ajdk, mlf_rtn, zn_ajdk, NZ, st);
-    an_ajdk(Zn.b(), zr.d, zr.n(), Ac_rmv, pcols, ajdk, mlf_rmv, vn_ajdk, NZ, st);
+    // the controls' An_ajdk over the dropped columns: stage 0 of the QR update formed it
+    // since the masks last changed (the same call)
+    if (vn_epoch != ctl_epoch) an_ajdk(Zn.b(), zr.d, zr.n(), Ac_rmv, pcols, ajdk, mlf_rmv, vn_ajdk, NZ, st);
     ops += 30LL * zr.n() * (long long)Zn.rb * 8;
   }
+  // what the next An_ajdk stage may copy
+  a0_valid = true; a0_Ar = hAr_rtn; a0_Ac = hAc_rtn; a0_Zr = hZr_j;
 }
 
 // bcc_lrup_QC_YnWt_stage_a1 (bcc_lrup.c:67-196): the compact Grams knzt, hnvt, ankt, knkt
@@ -1986,7 +1999,7 @@ void Run::reset()
   HIPCHK(hipStreamSynchronize(st));
   trace.clear(); xdrop_a.clear(); xdrop_b.clear();
   n_iter = 0; inited = false; ops = 0; init_s = 0; loop_s = 0; n_sort_fallback = 0;
-  xij_done = false; pending_dup = false;
+  xij_done = false; pending_dup = false; a0_valid = false; vn_epoch = -1;
   for (auto& k : prof.stats) { k.launches = 0; k.ms = 0; k.work = 0; }
 }
 

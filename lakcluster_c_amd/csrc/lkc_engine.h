@@ -143,6 +143,8 @@ struct Run {
   double *QR_ZSZ = nullptr, *QR_TAT = nullptr, *QC_ZSZ = nullptr, *QC_TAT = nullptr;
   double *QR_ZSZ_nrm = nullptr, *QR_TAT_nrm = nullptr, *QC_ZSZ_nrm = nullptr, *QC_TAT_nrm = nullptr;   // [t*rows + r]
   bool xij_done = false, pending_dup = false;   // stage order within a pass
+  long ctl_epoch = 0, vn_epoch = -1;            // uploads of the masks; the one vn_ajdk was formed under
+  bool a0_valid = false; std::vector<uint8_t> a0_Ar, a0_Ac, a0_Zr;   // the masks of stage a0's An_ajdk
   double *mt[14] = {};
   int* cnt_buf = nullptr;
   double *xwork = nullptr, *scal = nullptr, *dsv = nullptr, *hsv = nullptr;
