@@ -951,10 +951,17 @@ void Run::st_ajdk()
 // bcc_lf_ZtSn (bcc_lf.c:53-101): plain +-1 products At_k . T_t over the active rows
 void Run::st_ZtSn()
 {
-  pm_few(At.b(), kl.d, kl.n(), Tt.b(), tl.d, tl.n(), Ar_j, psize(NA), mlf + 3 * 16, lf_AtTn, M, st);
+  // After the update the active rows and columns are the retained ones, whose products
+  // stage b0 has just formed (attn, ztsn: the same integers and mask weights): copied.
+  const bool reuse = b0_valid && hAr_j == b0_Ar && hAc_j == b0_Ac && hZr_j == b0_Zr;
+  b0_valid = false;
+  const size_t bytes = sizeof(double) * (size_t)M * T;
+  if (reuse) HIPCHK(hipMemcpyAsync(lf_AtTn, mt[4], bytes, hipMemcpyDeviceToDevice, st));
+  else pm_few(At.b(), kl.d, kl.n(), Tt.b(), tl.d, tl.n(), Ar_j, psize(NA), mlf + 3 * 16, lf_AtTn, M, st);
   ops += (long long)kl.n() * tl.n() * (long long)bsize(NA) * 8;
   if (has_Z) {
-    pm_few(Zt.b(), kl.d, kl.n(), St.b(), tl.d, tl.n(), Zr_j, psize(NZ), mlf + 4 * 16, lf_ZtSn, M, st);
+    if (reuse) HIPCHK(hipMemcpyAsync(lf_ZtSn, mt[8], bytes, hipMemcpyDeviceToDevice, st));
+    else pm_few(Zt.b(), kl.d, kl.n(), St.b(), tl.d, tl.n(), Zr_j, psize(NZ), mlf + 4 * 16, lf_ZtSn, M, st);
     ops += (long long)kl.n() * tl.n() * (long long)bsize(NZ) * 8;
   }
 }
@@ -1694,6 +1701,8 @@ void Run::lrup_QC_b0()
     pm_few(Zt.b(), crm.d, crm.n(), St.b(), tl.d, tl.n(), Zr_j, sZ, mlf_nz, vtsn, M, st);
     ops += (long long)(crt.n() + crm.n()) * tl.n() * (long long)Zt.rb * 8;
   }
+  // what the next ZtSn stage may copy
+  b0_valid = true; b0_Ar = hAr_rtn; b0_Ac = hAc_rtn; b0_Zr = hZr_j;
 }
 
 // bcc_lrup_QC_YnWt_stage_b1 (bcc_lrup.c:363-408): the AtTYn_vv triples; on the matrix cores
@@ -1999,7 +2008,7 @@ void Run::reset()
   HIPCHK(hipStreamSynchronize(st));
   trace.clear(); xdrop_a.clear(); xdrop_b.clear();
   n_iter = 0; inited = false; ops = 0; init_s = 0; loop_s = 0; n_sort_fallback = 0;
-  xij_done = false; pending_dup = false; a0_valid = false; vn_epoch = -1;
+  xij_done = false; pending_dup = false; a0_valid = false; b0_valid = false; vn_epoch = -1;
   for (auto& k : prof.stats) { k.launches = 0; k.ms = 0; k.work = 0; }
 }
 

@@ -145,6 +145,7 @@ struct Run {
   bool xij_done = false, pending_dup = false;   // stage order within a pass
   long ctl_epoch = 0, vn_epoch = -1;            // uploads of the masks; the one vn_ajdk was formed under
   bool a0_valid = false; std::vector<uint8_t> a0_Ar, a0_Ac, a0_Zr;   // the masks of stage a0's An_ajdk
+  bool b0_valid = false; std::vector<uint8_t> b0_Ar, b0_Ac, b0_Zr;   // and of stage b0's products
   double *mt[14] = {};
   int* cnt_buf = nullptr;
   double *xwork = nullptr, *scal = nullptr, *dsv = nullptr, *hsv = nullptr;
