@@ -58,6 +58,7 @@ def lib():
         "lko_state_free": (None, [vp]),
         "lko_scorebox": (C.c_int, [cp, C.c_int, ip, ip, ip, ip, dp, dp, cp, C.c_int]),
         "lko_scorebox_xdrop_array": (None, [C.c_int, C.c_int, C.c_int, ip, ip]),
+        "lko_dcc_sumscores_test": (C.c_int, [vp, cp, dp, ip, ip, ip, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -210,6 +211,29 @@ def scorebox(in_text: str, recompute: bool = False) -> Scorebox:
     # [nr + nc*row_num] -> (row_num, col_num)
     sh = lambda a: a.reshape(nc, nr).T.copy()
     return Scorebox(*[sh(a) for a in ia + da])
+
+
+class DccParams(C.Structure):
+    """lko_dcc_params: the GLOBAL_TEST_* settings of a dcc_sumscores test input."""
+    _fields_ = [(n, C.c_int) for n in ("nbins", "A_n_rows", "A_n_cols", "Z_n_rows", "Y_n_cols", "T_n_cols", "B_MLT",
+                                      "niter", "sparse", "gamma_type")] + \
+               [(n, C.c_double) for n in ("mrand", "gamma", "kappa2")]
+
+
+def dcc_sumscores_test(out_dir: Optional[str] = None, **kw):
+    """dcc_sumscores_test on dcc_init's synthetic problem (several bins, Y/W controls).
+    Returns (trace (n x 6), xdrop_a, xdrop_b); writes the three output files into out_dir."""
+    p = DccParams(**{**dict(nbins=1, A_n_rows=0, A_n_cols=0, Z_n_rows=0, Y_n_cols=0, T_n_cols=1, B_MLT=32, niter=0,
+                            sparse=1, gamma_type=0, mrand=0.5, gamma=0.125, kappa2=0.0), **kw})
+    L = lib()
+    nt, nx = C.c_int(0), C.c_int(0)
+    if L.lko_dcc_sumscores_test(C.byref(p), None, None, C.byref(nt), None, None, C.byref(nx)) != 0:
+        raise RuntimeError("oracle: dcc_sumscores_test failed")
+    tr = np.zeros((max(nt.value, 1), 6)); xa = np.zeros((max(nx.value, 1), 2), np.int32); xb = np.zeros_like(xa)
+    if L.lko_dcc_sumscores_test(C.byref(p), out_dir.encode() if out_dir else None, _p(tr, C.c_double), C.byref(nt),
+                                _p(xa, C.c_int), _p(xb, C.c_int), C.byref(nx)) != 0:
+        raise RuntimeError("oracle: dcc_sumscores_test failed")
+    return tr[:nt.value], xa[:nx.value], xb[:nx.value]
 
 
 class State:

@@ -98,6 +98,18 @@ int lko_scorebox(const char *in_text, int recompute, int *dims, int *rpop, int *
                  double *qc, char *err, int nerr);
 void lko_scorebox_xdrop_array(int nxols, int x_max, int x_num, int *xval, int *xdrop);
 
+/* dcc_sumscores_test (dcc_sumscores.c:700-747) on dcc_init's synthetic problem
+   (dcc.c:1217-1275; glibc rand() from its initial state): several bins, Y/W genetic
+   controls, GLOBAL_Ireq = 0.  Writes out_trace.txt / out_xdrop_a.txt / out_xdrop_b.txt
+   into out_dir when it is not NULL; trace holds 6 doubles per line, the xdrop
+   arrays 2 ints per line (lko_dcc_bins.c). */
+typedef struct {
+  int nbins, A_n_rows, A_n_cols, Z_n_rows, Y_n_cols, T_n_cols, B_MLT, niter, sparse, gamma_type;
+  double mrand, gamma, kappa2;
+} lko_dcc_params;
+int lko_dcc_sumscores_test(const lko_dcc_params *p, const char *out_dir, double *trace, int *n_trace,
+                           int *xdrop_a, int *xdrop_b, int *n_xdrop);
+
 #ifdef __cplusplus
 }
 #endif
