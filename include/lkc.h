@@ -11,7 +11,9 @@
  * Scope of the path (as in the reference driver with these settings): one bin
  * (GLOBAL_NBINS= 1), no Y/W genetic controls (GLOBAL_Y_n_cols= 0), QR/QC
  * strategies "YnWt", at most 8 covariate columns; bcc_scramble of the loaded A
- * (GLOBAL_scramble_num) and the lakcluster_scorebox replay are supported.
+ * (GLOBAL_scramble_num) and the lakcluster_scorebox replay are supported.  The
+ * reference's dcc self-test (lkc_dcc_sumscores_test) runs several bins with Y/W
+ * genetic controls on its synthetic problem.
  */
 #ifndef LKC_H
 #define LKC_H
@@ -47,6 +49,15 @@ int lkc_lakcluster_driver(const char *in_text, const char *out_dir);
    (dcc_An_ajdk, dcc_lf_ZtSn, dcc_lf_D_AtTn_ZtSn_vv, dcc_lf_TAnZtS_ww,
    dcc_wrap_dcc_halfloop) after each drop; the same three output files. */
 int lkc_dexcluster_driver(const char *in_text, const char *out_dir);
+
+/* dcc_sumscores_test (dcc_sumscores.c:700-747; GLOBAL_TEST_TYPE= dcc_sumscores at
+   lakcluster_ver18.c:343): the synthetic problem of dcc_init (dcc.c:1217-1275) drawn from
+   the GLOBAL_TEST_* keys (GLOBAL_TEST_TYP2= error) with glibc rand()'s sequence from its
+   initial state, GLOBAL_NBINS bins, Y/W genetic controls of GLOBAL_TEST_Y_n_cols columns;
+   GLOBAL_TEST_niter passes of the dcc half-loop elimination (to the end when <= 0); writes
+   out_trace.txt, out_xdrop_a.txt, out_xdrop_b.txt into out_dir (NULL: GLOBAL_DIR_NAME, else
+   GLOBAL_DIR_BASE/dir_<GLOBAL_out_name>). */
+int lkc_dcc_sumscores_test(const char *in_text, const char *out_dir);
 
 /* lakcluster_scorebox (lakcluster_scorebox.c:265-271, dispatched on GLOBAL_TEST_TYPE=
    lakcluster_scorebox at lakcluster_ver18.c:338): replays the elimination order of

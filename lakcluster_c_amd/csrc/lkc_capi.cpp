@@ -1,5 +1,6 @@
 // lkc_capi.cpp -- the extern "C" boundary declared in include/lkc.h.
 #include "lkc_engine.h"
+#include "lkc_dcc_bins.h"
 #include <cmath>
 #include "lkc_stages.h"
 
@@ -226,6 +227,16 @@ int lkc_dexcluster_driver(const char* in_text, const char* out_dir)
 }
 
 int lkc_bcc_is_dcc(const lkc_bcc* D) { return D && D->run.dcc ? 1 : 0; }
+
+int lkc_dcc_sumscores_test(const char* in_text, const char* out_dir)
+{
+  return guard(nullptr, [&] {
+    lkc::Cfg c;
+    lkc::parse_cfg(in_text, c);
+    lkc::DccTestOut out;
+    lkc::dcc_sumscores_test(c, out, out_dir && out_dir[0] ? std::string(out_dir) : lkc::out_dir_of(c));
+  });
+}
 
 int lkc_bcc_n_iter(const lkc_bcc* D) { return D ? D->run.n_iter : -1; }
 int lkc_bcc_n_trace(const lkc_bcc* D) { return D ? (int)(D->run.trace.size() / 6) : -1; }

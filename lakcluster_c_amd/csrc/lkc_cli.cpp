@@ -1,7 +1,8 @@
 // lkc_cli.cpp -- the .in-driven command line: reads the key/value text of an
 // .in file on stdin (as lakcluster_ver18 does, lakcluster_ver18.c:337-338) and runs
 // the elimination on device 0, writing out_trace.txt / out_xdrop_{a,b}.txt, or with
-// GLOBAL_TEST_TYPE= lakcluster_scorebox the scorebox replay and its .mda files.
+// GLOBAL_TEST_TYPE= lakcluster_scorebox the scorebox replay and its .mda files, or with
+// GLOBAL_TEST_TYPE= dcc_sumscores the reference's dcc self-test on its synthetic problem.
 #include "../../include/lkc.h"
 #include <stdio.h>
 #include <string>
@@ -13,6 +14,19 @@ int main(int argc, char** argv)
   size_t n;
   while ((n = fread(buf, 1, sizeof buf, stdin)) > 0) text.append(buf, n);
   const char* out_dir = argc > 1 ? argv[1] : nullptr;
+  // GLOBAL_TEST_TYPE dispatch (lakcluster_ver18.c:337-345)
+  size_t k = text.find("GLOBAL_TEST_TYPE");
+  if (k != std::string::npos) {
+    size_t e = text.find('=', k), s = text.find(';', k);
+    std::string v = e != std::string::npos ? text.substr(e + 1, s == std::string::npos ? std::string::npos : s - e - 1) : "";
+    v.erase(0, v.find_first_not_of(" \t\r\n"));
+    v.erase(v.find_last_not_of(" \t\r\n") + 1);
+    if (v == "dcc_sumscores") {
+      if (lkc_dcc_sumscores_test(text.c_str(), out_dir)) { fprintf(stderr, "lkc: %s\n", lkc_last_error()); return 1; }
+      printf("dcc_sumscores done\n");
+      return 0;
+    }
+  }
   lkc_bcc* D = nullptr;
   int rc = lkc_bcc_load(text.c_str(), &D);
   if (rc) { fprintf(stderr, "lkc: %s\n", lkc_last_error()); return 1; }

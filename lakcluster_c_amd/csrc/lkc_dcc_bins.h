@@ -1,0 +1,45 @@
+// lkc_dcc_bins.h -- launchers of the multi-bin dcc stage kernels (lkc_dcc_pairs.hip) and the
+// host side of dcc_sumscores_test on the device (lkc_dcc_bins.cpp).
+#pragma once
+#include "lkc_kernels.h"
+#include <string>
+#include <vector>
+
+namespace lkc {
+
+// one QC half-loop term of a bin pair: out[tb[t]*ld + k] for the listed columns k and
+// active covariates t; L1/L2 [t*ld + k]; etA (with rpop) adds the same-bin correction
+struct PairQC {
+  const int* kl; int nk; const int* tl; int nt; long long ld;
+  const double* L1; const double* L2; const double* etT; const double* etS;
+  const double* etA; int rpop;
+  const double* ajdk; int pcols;
+  int tb[kMaxT];
+  double* out;
+};
+void pair_qc(const PairQC& a, hipStream_t st);
+
+// one QR half-loop term of a bin pair: out[tb[t]*ld + r] over the first bin's listed rows;
+// x the An_ZtSn_ww product [t*ld + r], a1/et of the second bin [t], x6 = lf_*_ajdk[AJDK_1_1],
+// mlf the weighted counts of the active columns ([5] a0d1, [7] a2d1); Tt (rows of rbT
+// bytes) adds the same-bin diagonal term
+struct PairQR {
+  const int* ar; int na; const int* tl; int nt; long long ld;
+  const double* x; const double* a1; const double* et; const double* x6; const double* mlf;
+  const uint8_t* Tt; size_t rbT;
+  int tb[kMaxT];
+  double* out;
+};
+void pair_qr(const PairQR& a, hipStream_t st);
+
+void bins_sumx(const int* list, int n, const int* tl, int nt, long long ld, double* x, long long stride, int nsum,
+               hipStream_t st);
+void bins_cmb_rows(const int* list, int n, int nblk, long long blk, double* x, const double* yw, const double* az,
+                   const double* yy, hipStream_t st);
+
+// dcc_sumscores_test (dcc_sumscores.c:700-747) on dcc_init's synthetic problem; outputs
+struct DccTestOut { std::vector<double> trace; std::vector<int> xdrop_a, xdrop_b; int n_iter = 0; long long ops = 0; };
+struct Cfg;
+void dcc_sumscores_test(const Cfg& c, DccTestOut& out, const std::string& out_dir);
+
+}  // namespace lkc

@@ -49,17 +49,14 @@ static std::string trim(const std::string& s)
 }
 
 // Keys that updateglobals reads (updateglobals.c:5-186) but that the bcc / dcc elimination,
-// the scorebox and bcc_scramble never consult: thread and print settings, the self-tests'
-// synthetic sizes (GLOBAL_TEST_*), the row counts the loader takes from the files, the Y/W
+// the scorebox, bcc_scramble and dcc_sumscores_test never consult: thread and print settings, the row counts the loader takes from the files, the Y/W
 // file names (GLOBAL_Y_n_cols > 0 is what turns those on, and is rejected), the dcg / PCA /
 // PLINK-conversion settings.  Any other key is an error, as the reference's exit() on an
 // unknown name (updateglobals.c:185).
 static bool reference_key_without_effect(const std::string& n)
 {
   static const char* keys[] = {
-    "GLOBAL_verbose", "GLOBAL_thread_count", "GLOBAL_omp_type", "GLOBAL_LBITS", "GLOBAL_TEST_TYP2",
-    "GLOBAL_TEST_mrand", "GLOBAL_TEST_A_n_rows", "GLOBAL_TEST_A_n_cols", "GLOBAL_TEST_Z_n_rows",
-    "GLOBAL_TEST_Y_n_cols", "GLOBAL_TEST_T_n_cols", "GLOBAL_TEST_niter", "GLOBAL_A_n_rows_", "GLOBAL_Z_n_rows_",
+    "GLOBAL_verbose", "GLOBAL_thread_count", "GLOBAL_omp_type", "GLOBAL_LBITS", "GLOBAL_A_n_rows_", "GLOBAL_Z_n_rows_",
     "GLOBAL_Y_n_name_", "GLOBAL_Y_t_name_", "GLOBAL_Y_n_cind", "GLOBAL_Y_p_name", "GLOBAL_W_n_name_",
     "GLOBAL_W_t_name_", "GLOBAL_J_n_rind_", "GLOBAL_K_n_cind", "GLOBAL_pca_out_xdrop", "GLOBAL_pca_V_",
     "GLOBAL_pca_V_pre_", "GLOBAL_pca_infix", "GLOBAL_pca_iteration_num", "GLOBAL_pca_iteration_max",
@@ -122,6 +119,14 @@ void parse_cfg(const char* text, Cfg& c)
     else if (name == "GLOBAL_scramble_out_xdrop_") c.scramble_files_raw = val;
     else if (name == "GLOBAL_scramble_rseed_") c.scramble_seeds_raw = val;
     else if (name == "GLOBAL_TEST_TYPE") c.test_type = first;
+    else if (name == "GLOBAL_TEST_TYP2") c.test_typ2 = first;
+    else if (name == "GLOBAL_TEST_mrand") c.test_mrand = atof(val.c_str());
+    else if (name == "GLOBAL_TEST_A_n_rows") c.test_A_rows = atoi(val.c_str());
+    else if (name == "GLOBAL_TEST_A_n_cols") c.test_A_cols = atoi(val.c_str());
+    else if (name == "GLOBAL_TEST_Z_n_rows") c.test_Z_rows = atoi(val.c_str());
+    else if (name == "GLOBAL_TEST_Y_n_cols") c.test_Y_cols = atoi(val.c_str());
+    else if (name == "GLOBAL_TEST_T_n_cols") c.test_T_cols = atoi(val.c_str());
+    else if (name == "GLOBAL_TEST_niter") c.test_niter = atoi(val.c_str());
     else if (name == "GLOBAL_scorebox_out_xdrop") c.sb_file = first;
     else if (name == "GLOBAL_scorebox_row_max") c.sb_row_max = atoi(val.c_str());
     else if (name == "GLOBAL_scorebox_row_num") c.sb_row_num = atoi(val.c_str());
@@ -2070,11 +2075,12 @@ long Run::debug_get(const std::string& name, std::vector<double>& out)
   return -1;
 }
 
-std::string Run::default_out_dir() const
+std::string out_dir_of(const Cfg& c)
 {
   if (!c.dir_name.empty()) return c.dir_name;
   if (!c.out_name.empty()) return (c.dir_base.empty() ? std::string(".") : c.dir_base) + "/dir_" + c.out_name;
   return ".";
 }
+std::string Run::default_out_dir() const { return out_dir_of(c); }
 
 }  // namespace lkc

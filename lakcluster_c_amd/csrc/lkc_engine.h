@@ -25,6 +25,10 @@ struct Cfg {
   std::string scramble_files_raw, scramble_seeds_raw;
   // GLOBAL_TEST_TYPE and the lakcluster_scorebox grid (updateglobals.c:18, 136-140)
   std::string test_type, sb_file; int sb_row_max = 0, sb_row_num = 0, sb_col_max = 0, sb_col_num = 0;
+  // the self-tests' synthetic problem (GLOBAL_TEST_*, defaults of lakcluster_ver18.c:70-79):
+  // read by dcc_sumscores_test, ignored by the file-driven eliminations
+  std::string test_typ2; double test_mrand = 0.0;
+  int test_A_rows = 12, test_A_cols = 13, test_Z_rows = 15, test_Y_cols = 11, test_T_cols = 3, test_niter = 2;
 };
 // lakcluster_scorebox output (struct S_handle): row_num x col_num entries at [nr + nc*row_num]
 struct Scorebox {
@@ -34,6 +38,7 @@ struct Scorebox {
 void scorebox_xdrop_array(int nxols, int x_max, int x_num, int* xval, int* xdrop);
 void write_scorebox(const Scorebox& S, const std::string& dir);
 void parse_cfg(const char* text, Cfg& c);
+std::string out_dir_of(const Cfg& c);   // GLOBAL_DIR_NAME, else GLOBAL_DIR_BASE/dir_<GLOBAL_out_name>
 void get_xdrop(double lrij, double lcij, double gamma, int gamma_type, int* rdrop, int* cdrop);
 int get_xdrop_length(double lrij, double lcij, double gamma, int gamma_type);
 unsigned sort_scores(double* d, int* perm, int n);

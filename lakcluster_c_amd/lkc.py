@@ -112,6 +112,7 @@ def lib():
         "lkc_bcc_write_scorebox": (C.c_int, [vp, cp]),
         "lkc_bcc_is_scorebox": (C.c_int, [vp]),
         "lkc_dexcluster_driver": (C.c_int, [cp, cp]),
+        "lkc_dcc_sumscores_test": (C.c_int, [cp, cp]),
         "lkc_bcc_is_dcc": (C.c_int, [vp]),
         "lkc_bcc_active": (C.c_int, [vp]),
         "lkc_bcc_sumscores_dmp": (C.c_int, [vp, cp]),
@@ -339,6 +340,12 @@ def lakcluster_driver(in_text: str, out_dir: Optional[str] = None) -> None:
 def dexcluster_driver(in_text: str, out_dir: Optional[str] = None) -> None:
     """Whole dcc run with the three output files written (dexcluster_driver)."""
     _check(lib().lkc_dexcluster_driver(in_text.encode(), out_dir.encode() if out_dir else None))
+
+
+def dcc_sumscores_test(in_text: str, out_dir: Optional[str] = None) -> None:
+    """The reference's dcc self-test (dcc_sumscores_test) on dcc_init's synthetic problem from
+    the GLOBAL_TEST_* keys: several bins, Y/W controls; the three output files written."""
+    _check(lib().lkc_dcc_sumscores_test(in_text.encode(), out_dir.encode() if out_dir else None))
 
 
 def dquicksort_xij(d: np.ndarray) -> Tuple[np.ndarray, np.ndarray, int]:

@@ -95,6 +95,13 @@ typedef struct {
 } dcc_t;
 
 static int qcols(const dcc_t *D, int q) { return (q == Q_YW || q == Q_YY) ? D->NY : D->M; }
+/* the `bother` flag of a bin pair's term (dcc_lf.c:118-154, TAnZtS.c:409-416, dcc_sumscores.c:503-559) */
+static int on(const dcc_t *D, int q, int nb1, int nb2)
+{
+  if (D->E[nb1].nA <= 0) return 0;
+  switch (q) { case Q_YW: return D->NY > 0 && D->E[nb2].nZ > 0; case Q_AZ: return D->M > 0 && D->E[nb2].nZ > 0;
+               case Q_YY: return D->NY > 0 && D->E[nb2].nA > 0; default: return D->M > 0 && D->E[nb2].nA > 0; }
+}
 
 /* wrap_M_setup_test_excerpt_0, "error" (An_ajdk_v.c:33-52): column-major draws, bit = draw > 1/2 */
 static void gen_bits(int nrows, int ncols, dm *Xn, dm *Xt)
@@ -287,10 +294,11 @@ static void st_D(dcc_t *D)
 {
   for (int nb1 = 0; nb1 < D->nbins; nb1++) for (int nb2 = 0; nb2 < D->nbins; nb2++) {
     dbin *E1 = &D->E[nb1], *E2 = &D->E[nb2]; dpair *F = &D->F[nb1 + nb2 * D->nbins];
-    d_term(D, &E1->Yt, E1->Ar_j, E1->nA, E1->YtTn, &E2->Wt, &E2->St, E2->Zr_j, E2->nZ, E2->WtSn, &E1->Tt, D->Yc_j, D->NY, D->Y_ajdk, D->Ypc, 0, F->qc[Q_YW]);
-    d_term(D, &E1->Yt, E1->Ar_j, E1->nA, E1->YtTn, &E2->Yt, &E2->Tt, E2->Ar_j, E2->nA, E2->YtTn, &E1->Tt, D->Yc_j, D->NY, D->Y_ajdk, D->Ypc, nb1 == nb2, F->qc[Q_YY]);
-    d_term(D, &E1->At, E1->Ar_j, E1->nA, E1->AtTn, &E2->Zt, &E2->St, E2->Zr_j, E2->nZ, E2->ZtSn, &E1->Tt, D->Ac_j, D->M, D->A_ajdk, D->Apc, 0, F->qc[Q_AZ]);
-    d_term(D, &E1->At, E1->Ar_j, E1->nA, E1->AtTn, &E2->At, &E2->Tt, E2->Ar_j, E2->nA, E2->AtTn, &E1->Tt, D->Ac_j, D->M, D->A_ajdk, D->Apc, nb1 == nb2, F->qc[Q_AA]);
+    for (int q = 0; q < NQ; q++) memset(F->qc[q], 0, sizeof(double) * (size_t)qcols(D, q) * D->T);
+    if (on(D, Q_YW, nb1, nb2)) d_term(D, &E1->Yt, E1->Ar_j, E1->nA, E1->YtTn, &E2->Wt, &E2->St, E2->Zr_j, E2->nZ, E2->WtSn, &E1->Tt, D->Yc_j, D->NY, D->Y_ajdk, D->Ypc, 0, F->qc[Q_YW]);
+    if (on(D, Q_YY, nb1, nb2)) d_term(D, &E1->Yt, E1->Ar_j, E1->nA, E1->YtTn, &E2->Yt, &E2->Tt, E2->Ar_j, E2->nA, E2->YtTn, &E1->Tt, D->Yc_j, D->NY, D->Y_ajdk, D->Ypc, nb1 == nb2, F->qc[Q_YY]);
+    if (on(D, Q_AZ, nb1, nb2)) d_term(D, &E1->At, E1->Ar_j, E1->nA, E1->AtTn, &E2->Zt, &E2->St, E2->Zr_j, E2->nZ, E2->ZtSn, &E1->Tt, D->Ac_j, D->M, D->A_ajdk, D->Apc, 0, F->qc[Q_AZ]);
+    if (on(D, Q_AA, nb1, nb2)) d_term(D, &E1->At, E1->Ar_j, E1->nA, E1->AtTn, &E2->At, &E2->Tt, E2->Ar_j, E2->nA, E2->AtTn, &E1->Tt, D->Ac_j, D->M, D->A_ajdk, D->Apc, nb1 == nb2, F->qc[Q_AA]);
   }
 }
 
@@ -419,9 +427,11 @@ static void st_TAnZtS(dcc_t *D)
         F->ww[Q_YY][o] += -l;
       }
     }
-    /* get_dcc_halfloop: the covariate index becomes the loaded one */
+    /* get_dcc_halfloop: the covariate index becomes the loaded one (terms without their
+       rows or columns stay zero) */
     for (int q = 0; q < NQ; q++) {
       memset(F->qr[q], 0, sizeof(double) * (size_t)nA * T);
+      if (!on(D, q, nb1, nb2)) continue;
       for (int t = 0; t < tl.n; t++) for (int i = 0; i < rl.n; i++)
         F->qr[q][(size_t)rl.ix[i] * T + D->tb[tl.ix[t]]] = F->ww[q][(size_t)rl.ix[i] * T + tl.ix[t]];
     }
@@ -442,6 +452,7 @@ static void st_nrm(dcc_t *D)
     long long lldc[NQ] = { At * Zt, At * Zt, At * (At - 1), At * (At - 1) };
     for (int q = 0; q < NQ; q++) {   /* QR rows, in the order TYnWtS, TYnYtT, TAnZtS, TAnAtT */
       int qq = q == 1 ? Q_YY : q == 2 ? Q_AZ : q;
+      if (!on(D, qq, nb1, nb2)) continue;
       double *dst = E->QRn[qq] + (size_t)nb2 * E->nA * T;
       memset(dst, 0, sizeof(double) * (size_t)E->nA * T);
       double d = (double)lldr[qq]; if (fabs(d) < 1e-12) d = 1;
@@ -450,6 +461,7 @@ static void st_nrm(dcc_t *D)
     }
     for (int q = 0; q < NQ; q++) {
       int qq = q == 1 ? Q_YY : q == 2 ? Q_AZ : q;
+      if (!on(D, qq, nb1, nb2)) continue;
       int X = qcols(D, qq); const uint8_t *km = X == D->NY && (qq == Q_YW || qq == Q_YY) ? D->Yc_j : D->Ac_j;
       double *dst = D->QCn[qq] + (size_t)nbx * X * T;
       memset(dst, 0, sizeof(double) * (size_t)X * T);
@@ -464,14 +476,19 @@ static void st_nrm(dcc_t *D)
     for (int nb1 = 0; nb1 < nbins; nb1++) {
       dbin *E = &D->E[nb1]; size_t stride = (size_t)E->nA * T;
       const int order[NQ] = { Q_YW, Q_YY, Q_AZ, Q_AA };
-      for (int q = 0; q < NQ; q++) for (int r = 0; r < E->nA; r++) if (bg(E->Ar_b, r) && bg(E->Ar_j, r)) {
+      const int use[NQ] = { E->nZ > 0 && D->NY > 0, E->nZ > 0 && D->M > 0, D->NY > 0, D->M > 0 };   /* by term */
+      if (E->nA <= 0) continue;
+      for (int q = 0; q < NQ; q++) if (use[order[q]]) for (int r = 0; r < E->nA; r++) if (bg(E->Ar_b, r) && bg(E->Ar_j, r)) {
         double *x = E->QRn[order[q]] + (size_t)c * E->nA + r;
         for (int k = 1; k < nbins; k++) x[0] += x[k * stride];
       }
     }
     const int order[NQ] = { Q_YW, Q_YY, Q_AZ, Q_AA };
+    const int ar = D->A_rpop_b_total > 0, zr = D->Z_rpop_b_total > 0;
+    const int usec[NQ] = { ar && D->NY > 0 && zr, ar && D->M > 0 && zr, ar && D->NY > 0 && zr, ar && D->M > 0 };   /* by term */
     for (int q = 0; q < NQ; q++) {
       int qq = order[q], X = qcols(D, qq);
+      if (!usec[qq]) continue;
       const uint8_t *mb = (qq == Q_YW || qq == Q_YY) ? D->Yc_b : D->Ac_b, *mj = (qq == Q_YW || qq == Q_YY) ? D->Yc_j : D->Ac_j;
       size_t stride = (size_t)X * T;
       for (int k = 0; k < X; k++) if (bg(mb, k) && bg(mj, k)) {
@@ -510,9 +527,13 @@ static void st_ifT(dcc_t *D)
   for (int nb1 = 0; nb1 < D->nbins; nb1++) {
     int nb2 = 0, nbx = nb1 + nb2 * D->nbins; dbin *E = &D->E[nb1];
     const int order[NQ] = { Q_YW, Q_YY, Q_AZ, Q_AA };
-    for (int q = 0; q < NQ; q++) mds_pow(E->QRn[order[q]] + (size_t)nb2 * E->nA * D->T, E->nA, D->T, den, E->Ar_j, D->Tc_j);
+    const int use[NQ] = { E->nZ > 0 && D->NY > 0, E->nZ > 0 && D->M > 0, D->NY > 0, D->M > 0 };
+    const int arb = D->A_rpop_b_total > 0, zrb = D->Z_rpop_b_total > 0;
+    const int usec[NQ] = { arb && D->NY > 0 && zrb, arb && D->M > 0 && zrb, arb && D->NY > 0, arb && D->M > 0 };
+    if (E->nA > 0) for (int q = 0; q < NQ; q++) if (use[order[q]]) mds_pow(E->QRn[order[q]] + (size_t)nb2 * E->nA * D->T, E->nA, D->T, den, E->Ar_j, D->Tc_j);
     for (int q = 0; q < NQ; q++) {
       int qq = order[q], X = qcols(D, qq);
+      if (!usec[qq]) continue;
       mds_nrm(D->QCn[qq] + (size_t)nbx * X * D->T, X, D->T, den, (qq == Q_YW || qq == Q_YY) ? D->Yc_j : D->Ac_j, D->Tc_j);
     }
   }
@@ -520,15 +541,16 @@ static void st_ifT(dcc_t *D)
 /* dcc_sumscores_cmb (dcc_sumscores.c:243-278): first covariate column only */
 static void st_cmb(dcc_t *D)
 {
-  for (int nbx = 0; nbx < D->nbins * D->nbins; nbx++) {
+  if (D->M > 0 && D->A_rpop_b_total > 0 && D->Z_rpop_b_total > 0) for (int nbx = 0; nbx < D->nbins * D->nbins; nbx++) {
     double *a = D->QCn[Q_AA] + (size_t)nbx * D->M * D->T, *z = D->QCn[Q_AZ] + (size_t)nbx * D->M * D->T;
     for (int k = 0; k < D->M; k++) if (bg(D->Ac_j, k)) a[k] += -1.0 * z[k];
   }
   for (int nb1 = 0; nb1 < D->nbins; nb1++) {
     dbin *E = &D->E[nb1]; size_t blk = (size_t)E->nA * D->T;
-    for (int nb2 = 0; nb2 < D->nbins; nb2++) for (int r = 0; r < E->nA; r++) if (bg(E->Ar_j, r)) E->QRn[Q_AA][nb2 * blk + r] += +1.0 * E->QRn[Q_YW][nb2 * blk + r];
-    for (int nb2 = 0; nb2 < D->nbins; nb2++) for (int r = 0; r < E->nA; r++) if (bg(E->Ar_j, r)) E->QRn[Q_AA][nb2 * blk + r] += -1.0 * E->QRn[Q_AZ][nb2 * blk + r];
-    for (int nb2 = 0; nb2 < D->nbins; nb2++) for (int r = 0; r < E->nA; r++) if (bg(E->Ar_j, r)) E->QRn[Q_AA][nb2 * blk + r] += -1.0 * E->QRn[Q_YY][nb2 * blk + r];
+    if (E->nA <= 0 || D->M <= 0) continue;
+    if (E->nZ > 0 && D->NY > 0) for (int nb2 = 0; nb2 < D->nbins; nb2++) for (int r = 0; r < E->nA; r++) if (bg(E->Ar_j, r)) E->QRn[Q_AA][nb2 * blk + r] += +1.0 * E->QRn[Q_YW][nb2 * blk + r];
+    if (E->nZ > 0) for (int nb2 = 0; nb2 < D->nbins; nb2++) for (int r = 0; r < E->nA; r++) if (bg(E->Ar_j, r)) E->QRn[Q_AA][nb2 * blk + r] += -1.0 * E->QRn[Q_AZ][nb2 * blk + r];
+    if (D->NY > 0) for (int nb2 = 0; nb2 < D->nbins; nb2++) for (int r = 0; r < E->nA; r++) if (bg(E->Ar_j, r)) E->QRn[Q_AA][nb2 * blk + r] += -1.0 * E->QRn[Q_YY][nb2 * blk + r];
   }
 }
 /* dcc_sumscores_xij (dcc_sumscores.c:158-241) with Ireq = 0 ("empty low": Irow = Icol = 0) */

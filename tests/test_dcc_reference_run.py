@@ -14,8 +14,15 @@ D-weighted QC products, xcalc and An_ZtSn_ww at B_MLT 32, the half-loop, the bin
 Y/W combination of dcc_sumscores_nrm / _cmb, the first-element-pivot quicksort and
 get_xdrop; the trace prints 16 digits of each mean, so a changed summation order, a
 dropped term or a wrong normaliser shows in the files.
+The HIP path (lkc_dcc_sumscores_test, and the lkc_driver command line on the same .in
+keys) draws the same problem on the host, runs every stage on the device and must write
+the same three files; on other bin counts, sizes, B_MLT, kappa_squared and missing Y or Z
+rows it is held to the oracle's files byte for byte.
 """
 import os
+import subprocess
+
+import pytest
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "dcc_sumscores_2bins")
 
@@ -23,6 +30,24 @@ GOLD = os.path.join(os.path.dirname(__file__), "golden", "dcc_sumscores_2bins")
 # their defaults, lakcluster_ver18.c:43,85-86)
 PARAMS = dict(nbins=2, A_n_rows=380, A_n_cols=3100, Z_n_rows=379, Y_n_cols=4800, T_n_cols=3, B_MLT=32,
               niter=3, sparse=1, gamma_type=0, mrand=0.90, gamma=0.125, kappa2=0.0)
+
+
+def in_text(**kw):
+    """The .in keys of a dcc_sumscores run with these GLOBAL_TEST_* settings."""
+    p = {**PARAMS, **kw}
+    keys = [("GLOBAL_TEST_TYPE", "dcc_sumscores"), ("GLOBAL_TEST_TYP2", "error"), ("GLOBAL_NBINS", p["nbins"]),
+            ("GLOBAL_B_MLT", p["B_MLT"]), ("GLOBAL_TEST_sparse", p["sparse"]), ("GLOBAL_TEST_mrand", repr(p["mrand"])),
+            ("GLOBAL_TEST_A_n_rows", p["A_n_rows"]), ("GLOBAL_TEST_A_n_cols", p["A_n_cols"]),
+            ("GLOBAL_TEST_Z_n_rows", p["Z_n_rows"]), ("GLOBAL_TEST_Y_n_cols", p["Y_n_cols"]),
+            ("GLOBAL_TEST_T_n_cols", p["T_n_cols"]), ("GLOBAL_TEST_niter", p["niter"]),
+            ("GLOBAL_gamma", repr(p["gamma"])), ("GLOBAL_gamma_type", p["gamma_type"]),
+            ("GLOBAL_kappa_squared", repr(p["kappa2"])), ("GLOBAL_out_name", "test")]
+    return "".join(f"{k}= {v};\n" for k, v in keys) + "END= 0;\n"
+
+
+def _same_files(a, b):
+    for f in ("out_trace.txt", "out_xdrop_a.txt", "out_xdrop_b.txt"):
+        assert open(os.path.join(a, f), "rb").read() == open(os.path.join(b, f), "rb").read(), f
 
 
 def test_oracle_reproduces_reference_dcc_run(oracle, tmp_path):
@@ -48,3 +73,47 @@ def test_reference_dcc_run_is_sensitive(oracle, tmp_path):
     assert open(os.path.join(tmp_path, "out_trace.txt"), "rb").read() == want[: want.index(b"\n2 ") + 1]
     tr2, _, _ = oracle.dcc_sumscores_test(None, **{**PARAMS, "Y_n_cols": 4799})
     assert tr2.shape == (3, 6) and tr2[0, 3] != tr[0, 3] and tr2[0, 4] != tr[0, 4]
+
+
+@pytest.mark.gpu
+def test_gpu_reproduces_reference_dcc_run(tmp_path):
+    from lakcluster_c_amd import lkc
+    lkc.dcc_sumscores_test(in_text(), str(tmp_path))
+    _same_files(str(tmp_path), GOLD)
+
+
+@pytest.mark.gpu
+def test_gpu_driver_reproduces_reference_dcc_run(tmp_path):
+    """The .in text on the command line's stdin, as the reference binary reads it."""
+    from lakcluster_c_amd import lkc
+    exe = os.path.join(os.path.dirname(lkc.LIB_PATH), "lkc_driver")
+    r = subprocess.run([exe, str(tmp_path)], input=in_text().encode(), capture_output=True, timeout=300)
+    assert r.returncode == 0, r.stderr.decode()
+    _same_files(str(tmp_path), GOLD)
+
+
+CONFIGS = {
+    # three bins run to the end
+    "bins3_to_end": dict(nbins=3, A_n_rows=120, A_n_cols=2500, Z_n_rows=100, Y_n_cols=1500, T_n_cols=3, niter=0, mrand=0.5),
+    # no Y/W columns, one covariate, B_MLT 24, gamma_type 1
+    "noY_T1_B24": dict(nbins=2, A_n_rows=150, A_n_cols=3000, Z_n_rows=140, Y_n_cols=0, T_n_cols=1, B_MLT=24,
+                       niter=0, mrand=0.3, gamma=0.1, gamma_type=1),
+    # one bin, four covariates, kappa_squared > 0 (the ifT correction acts), B_MLT 16
+    "bin1_kappa_B16": dict(nbins=1, A_n_rows=200, A_n_cols=2100, Z_n_rows=180, Y_n_cols=800, T_n_cols=4, B_MLT=16,
+                           niter=6, mrand=0.4, kappa2=0.4),
+    # four small bins, the first without controls (its Z terms absent), several 1920-column blocks
+    "bins4_small": dict(nbins=4, A_n_rows=3, A_n_cols=4200, Z_n_rows=2, Y_n_cols=300, T_n_cols=2, niter=0, mrand=0.2),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(CONFIGS))
+def test_gpu_dcc_bins_match_oracle(name, oracle, tmp_path):
+    from lakcluster_c_amd import lkc
+    kw = CONFIGS[name]
+    ref, got = tmp_path / "oracle", tmp_path / "gpu"
+    ref.mkdir(); got.mkdir()
+    tr, xa, _ = oracle.dcc_sumscores_test(str(ref), **{**PARAMS, **kw})
+    assert tr.shape[0] >= 2 and xa.shape[0] > 0
+    lkc.dcc_sumscores_test(in_text(**kw), str(got))
+    _same_files(str(got), str(ref))
