@@ -215,6 +215,22 @@ int lkc_bcc_set_profiling(lkc_bcc *D, int on);
    popcount over packed bits (and the B_MLT bit planes).  Both are bit-exact. */
 int lkc_bcc_set_kernels(lkc_bcc *D, int which);
 int lkc_bcc_kernel_stats(const lkc_bcc *D, const char *name, long long *launches, double *ms, double *work);
+/* Launch-shape autotuner (no counterpart in the reference, whose SSE loops have one form;
+   for callers that run many problems of varying shape, such as the batch and serving
+   front-ends).  The ww contraction (fused, or split over w steps and n parts) and the
+   full-layout Grams (one block per tile, or split over the 1920-blocks) have forms with
+   bit-identical results; with tuning on, the first launch of each shape class (sizes to
+   half an octave) times every form with HIP events on the handle's stream and keeps the
+   fastest; off, a cost model picks.  mode 0 off, 1 measure new classes (the default, or
+   LKC_TUNE=0/1/2 in the environment), 2 use stored classes only.  The store is shared by
+   the process; lkc_tune_save / lkc_tune_load write and read it as text (return the number
+   of classes).  lkc_tune_get: mode, stored classes, classes measured, and the summed
+   device milliseconds the tuned forms saved over the cost model's at measurement. */
+int lkc_tune_set(int mode);
+int lkc_tune_get(int *mode, int *classes, long long *measured, double *saved_ms);
+int lkc_tune_save(const char *path);
+int lkc_tune_load(const char *path);
+int lkc_tune_clear(void);
 /* State inspection (no counterpart in the reference, which keeps these arrays in
    struct bcc_ajdk/bcc_single/bcc_double): copies the named score array, e.g.
    "lf_AnZt", "QC_TAT", "et_An", in row-major (row, column) order with the

@@ -291,6 +291,47 @@ int lkc_bcc_set_kernels(lkc_bcc* D, int which)
   return LKC_OK;
 }
 
+int lkc_tune_set(int mode)
+{
+  if (mode < 0 || mode > 2) { g_last_error = "tuning mode must be 0, 1 or 2"; return LKC_ERR_STATE; }
+  lkc::tune_set_mode(mode);
+  return LKC_OK;
+}
+
+int lkc_tune_get(int* mode, int* classes, long long* measured, double* saved_ms)
+{
+  long long m = 0;
+  double s = 0;
+  lkc::tune_stats(&m, &s);
+  if (mode) *mode = lkc::tune_mode();
+  if (classes) *classes = lkc::tune_entries();
+  if (measured) *measured = m;
+  if (saved_ms) *saved_ms = s;
+  return LKC_OK;
+}
+
+int lkc_tune_save(const char* path)
+{
+  if (!path) { g_last_error = "null path"; return LKC_ERR_STATE; }
+  const int n = lkc::tune_save(path);
+  if (n < 0) { g_last_error = std::string("cannot write ") + path; return LKC_ERR_IO; }
+  return n;
+}
+
+int lkc_tune_load(const char* path)
+{
+  if (!path) { g_last_error = "null path"; return LKC_ERR_STATE; }
+  const int n = lkc::tune_load(path);
+  if (n < 0) { g_last_error = std::string("cannot read ") + path; return LKC_ERR_IO; }
+  return n;
+}
+
+int lkc_tune_clear(void)
+{
+  lkc::tune_clear();
+  return LKC_OK;
+}
+
 int lkc_bcc_kernel_stats(const lkc_bcc* D, const char* name, long long* launches, double* ms, double* work)
 {
   if (!D || !name) return LKC_ERR_STATE;

@@ -1452,9 +1452,30 @@ void Run::sumscores()
 }
 
 // a Gram over the full column layout: split over the 1920-blocks when it has few tiles
+// (the cost model), or the form the autotuner measured faster for its shape class.  A
+// Gram that subtracts in place (op_sub) runs once, so only the others are measured.
 void Run::gram_any(const GramMArgs& g)
 {
-  const size_t b = gram_split_bytes(g);
+  size_t b = gram_split_bytes(g);
+  const size_t bany = gram_split_bytes_any(g);
+  const int tm = g.op_sub ? 0 : tune_mode();
+  if (tm != 0 && bany > 0 && bany <= ((size_t)256 << 20)) {
+    const TuneKey key{ kTuneGram, tune_bucket(g.na), tune_bucket(g.nz), tune_bucket(g.nseg), 0, 0 };
+    int ch = -1;
+    if (tune_get(key, &ch)) b = ch == 1 ? bany : 0;
+    else if (tm == 1) {
+      struct Ctx { GramMArgs g; uint32_t* X; hipStream_t st; };
+      Ctx cx{ g, (uint32_t*)g_split.need(dev, bany), st };
+      const int cand[2] = { 0, 1 };
+      float best_ms = 0, model_ms = 0;
+      const int bi = tune_measure(st, cand, 2, b ? 1 : 0, [](void* p, int v) {
+        const Ctx* c = (const Ctx*)p;
+        if (v) gram_split(c->g, c->X, c->st); else gram_mfma(c->g, c->st);
+      }, &cx, &best_ms, &model_ms);
+      tune_put(key, bi, best_ms, model_ms);
+      return;   // every measured launch wrote the same Gram
+    }
+  }
   if (b) gram_split(g, (uint32_t*)g_split.need(dev, b), st);
   else gram_mfma(g, st);
 }

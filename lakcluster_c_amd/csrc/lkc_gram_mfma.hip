@@ -473,15 +473,20 @@ __global__ void k_gram_combine(GramMArgs g, const uint32_t* X)
   if (g.op_sub) *dst -= o; else *dst = o;
 }
 
-size_t gram_split_bytes(const GramMArgs& g)
+size_t gram_split_bytes_any(const GramMArgs& g)
 {
   if (g.seg || g.na <= 0 || g.nz <= 0 || g.nseg < 2) return 0;
+  const double b = 4.0 * g.nseg * (double)g.na * g.nz;
+  return b <= 4.0e9 ? (size_t)b + 256 : 0;
+}
+
+size_t gram_split_bytes(const GramMArgs& g)
+{
   // the split pays an int32 store and reload per segment and pair: only for Grams too
   // small to fill the CUs with whole tiles
   const long long tiles = (long long)cdiv(g.na, 128) * cdiv(g.nz, 128);
   if (tiles >= 128) return 0;
-  const double b = 4.0 * g.nseg * (double)g.na * g.nz;
-  return b <= 4.0e9 ? (size_t)b + 256 : 0;
+  return gram_split_bytes_any(g);
 }
 
 void gram_split(const GramMArgs& g, uint32_t* X, hipStream_t st)

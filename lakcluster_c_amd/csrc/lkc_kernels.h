@@ -175,6 +175,7 @@ void gram_mfma(const GramMArgs& g, hipStream_t st);
 // The same Gram split over the 1920-blocks for few output tiles (full layout only):
 // gram_split_bytes(g) = scratch bytes it needs, 0 when the split does not apply.
 size_t gram_split_bytes(const GramMArgs& g);
+size_t gram_split_bytes_any(const GramMArgs& g);   // the same without the tile-count rule
 void gram_split(const GramMArgs& g, uint32_t* X, hipStream_t st);
 // sg[t*nzp + i] = S_t(zlist[i]) (+-1) for the QR sums; returns nzp.  sgbuf: qr_sign_bytes(nz, nt)
 int qr_signs(const Bits& Sc, const int* tlist, int nt, const int* zlist, int nz, signed char* sgbuf, hipStream_t st);
@@ -212,5 +213,32 @@ hipError_t sort_pairs(const double* keys, double* keys_out, int* idx_in, int* id
 
 // out[i] = pop(X[rows[i]] & mask) over nwords 32-bit words (rows==NULL: row i)
 void popc_rows(const Bits& X, const int* rows, int nrows, const uint32_t* mask, int nwords, int* out, hipStream_t st);
+
+// launch-shape autotuner (lkc_tune.cpp): per shape class, the fastest of a kernel's
+// bit-identical forms, measured on first use.  mode 0 off, 1 measure new classes, 2 frozen.
+struct TuneKey {
+  int op, a, b, c, d, e;
+  bool operator<(const TuneKey& o) const
+  {
+    const int x[6] = { op, a, b, c, d, e }, y[6] = { o.op, o.a, o.b, o.c, o.d, o.e };
+    for (int i = 0; i < 6; i++) if (x[i] != y[i]) return x[i] < y[i];
+    return false;
+  }
+};
+enum { kTuneWw = 1, kTuneGram = 2 };
+int tune_mode();
+void tune_set_mode(int m);
+int tune_bucket(long long x);            // round(2 log2 x): half-octave size classes
+bool tune_get(const TuneKey& k, int* choice);
+void tune_put(const TuneKey& k, int choice, float best_ms, float model_ms);
+// runs launch(ctx, cand[i]) for every candidate twice on st between HIP events, waits, and
+// returns the index of the fastest; model_idx is the cost model's candidate
+int tune_measure(hipStream_t st, const int* cand, int n, int model_idx, void (*launch)(void*, int), void* ctx,
+                 float* best_ms, float* model_ms);
+int tune_entries();
+void tune_stats(long long* measured, double* saved_ms);
+void tune_clear();
+int tune_save(const char* path);
+int tune_load(const char* path);
 
 }  // namespace lkc

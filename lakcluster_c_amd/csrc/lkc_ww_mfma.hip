@@ -465,6 +465,10 @@ static int ww_split_parts(int nk, int nt, int nw, size_t prb, int ndig)
   return (int)std::max<long long>(parts, 1);
 }
 
+// the split forms the tuner may try besides the cost model's: terms buffers up to 256 MiB
+static const double kTunePbufMax = 256.0 * 1024 * 1024;
+static double ww_pbytes(int nk, int nt, int nw) { return (double)nt * (double)(cdiv(nw, 32) * 32) * (double)nk * 8.0; }
+
 size_t ww_mfma_work_bytes(int nk, int nt, int nw, size_t prb, int B)
 {
   int ndig = (B + 7) / 8;
@@ -474,9 +478,48 @@ size_t ww_mfma_work_bytes(int nk, int nt, int nw, size_t prb, int B)
   b += sizeof(double) * ((size_t)nt * nw + nt + 8);
   b += sizeof(long long) * ((size_t)nw + 8);
   b += 2 * sizeof(int) * ((size_t)nt * nk + 8);
-  if (ww_split_parts(nk, nt, nw, prb, ndig) > 0) b += 256 + sizeof(unsigned long long) * (size_t)nt * nwp * nk;
+  const bool terms = ww_split_parts(nk, nt, nw, prb, ndig) > 0 || (tune_mode() != 0 && ww_pbytes(nk, nt, nw) <= kTunePbufMax);
+  if (terms) b += 256 + sizeof(unsigned long long) * (size_t)nt * nwp * nk;
   return b + 1024;
 }
+
+namespace {
+// one form of the contraction after the digit and per-w / per-k kernels: nsplit 0 the fused
+// kernel, 1 one block per (rows, w step) with k_ww_sum, >1 also n parts with k_ww_terms
+struct WwLaunch {
+  WwArgs a; const int8_t* Dg; long long ldn; int nwp, ndig; const double* H; const double* bp; const long long* Gm;
+  const int* cntp; const int* llk; unsigned long long* Pbuf; long long S; hipStream_t st;
+};
+void ww_launch(void* ctx, int nsplit)
+{
+  const WwLaunch& L = *(const WwLaunch*)ctx;
+  const WwArgs& a = L.a;
+  hipStream_t st = L.st;
+  const long long np = (long long)a.nt * L.nwp * a.nk;
+  if (nsplit > 1)
+    hipLaunchKernelGGL(k_ww_zero, dim3(cdiv(np, 256) < 65536 ? cdiv(np, 256) : 65536), dim3(256), 0, st, L.Pbuf, np);
+  dim3 grid(cdiv(a.nk, 128), cdiv(a.nt, a.nt >= 3 ? 3 : a.nt), nsplit > 0 ? cdiv(a.nw, 32) * nsplit : 1);
+#define WW_LAUNCH(NTG, ND) do { if (nsplit > 1) \
+      hipLaunchKernelGGL((k_ww_mfma<NTG, ND, 2>), grid, dim3(256), 0, st, a, L.Dg, L.ldn, L.nwp, L.H, L.bp, L.Gm, L.cntp, L.llk, nsplit, L.Pbuf); \
+    else if (nsplit == 1) \
+      hipLaunchKernelGGL((k_ww_mfma<NTG, ND, 1>), grid, dim3(256), 0, st, a, L.Dg, L.ldn, L.nwp, L.H, L.bp, L.Gm, L.cntp, L.llk, 1, L.Pbuf); \
+    else hipLaunchKernelGGL((k_ww_mfma<NTG, ND, 0>), grid, dim3(256), 0, st, a, L.Dg, L.ldn, L.nwp, L.H, L.bp, L.Gm, L.cntp, L.llk, 1, L.Pbuf); } while (0)
+#define WW_ND(NTG) switch (L.ndig) { case 1: WW_LAUNCH(NTG, 1); break; case 2: WW_LAUNCH(NTG, 2); break; \
+                                     case 3: WW_LAUNCH(NTG, 3); break; default: WW_LAUNCH(NTG, 4); break; }
+  if (a.nt >= 3) { WW_ND(3) } else if (a.nt == 2) { WW_ND(2) } else { WW_ND(1) }
+#undef WW_ND
+#undef WW_LAUNCH
+  if (nsplit > 1) {
+    const long long nte = (long long)a.nt * a.nw * a.nk;
+    hipLaunchKernelGGL(k_ww_terms, dim3(cdiv(nte, 256) < 65536 ? cdiv(nte, 256) : 65536), dim3(256), 0, st,
+                       a, L.nwp, L.Pbuf, L.H, L.bp, L.Gm, L.cntp, L.llk, L.S);
+  }
+  if (nsplit > 0) {
+    const long long nc = (long long)a.nk * a.nt;
+    hipLaunchKernelGGL(k_ww_sum, dim3(cdiv(nc, 64)), dim3(64), 0, st, a, L.nwp, L.Pbuf);
+  }
+}
+}  // namespace
 
 hipError_t ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t st, const WwHook* hook)
 {
@@ -484,7 +527,6 @@ hipError_t ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t s
   int ndig = (a.B + 7) / 8;
   int nwp = ((a.nw + 31) / 32) * 32;
   long long ldn = (long long)a.prb * 8;
-  if (work_bytes < ww_mfma_work_bytes(a.nk, a.nt, a.nw, a.prb, a.B)) return hipErrorInvalidValue;   // the caller sizes work with ww_mfma_work_bytes
   uint8_t* p = (uint8_t*)work;
   int8_t* Dg = (int8_t*)p;
   size_t off = ((size_t)ndig * nwp * ldn + 255) & ~(size_t)255;
@@ -493,8 +535,13 @@ hipError_t ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t s
   long long* Gm = (long long*)(p + off); off += sizeof(long long) * ((size_t)a.nw + 8);
   int* cntp = (int*)(p + off); off += sizeof(int) * ((size_t)a.nt * a.nk + 8);
   int* llk = (int*)(p + off); off += sizeof(int) * ((size_t)a.nt * a.nk + 8);
-  const int nsplit = ww_split_parts(a.nk, a.nt, a.nw, a.prb, ndig);
   unsigned long long* Pbuf = (unsigned long long*)(((uintptr_t)(p + off) + 255) & ~(uintptr_t)255);
+  const long long np = (long long)a.nt * nwp * a.nk;
+  // the caller sizes work with ww_mfma_work_bytes; the split forms need the terms buffer
+  if (work_bytes < off) return hipErrorInvalidValue;
+  const bool terms_fit = (size_t)((uint8_t*)(Pbuf + np) - p) <= work_bytes;
+  int nsplit = ww_split_parts(a.nk, a.nt, a.nw, a.prb, ndig);
+  if (nsplit > 0 && !terms_fit) return hipErrorInvalidValue;
   if (nwp > 0) {
     long long tot = (long long)nwp * (ldn / 4);
     hipLaunchKernelGGL(k_ww_digits, dim3(cdiv(tot, 256) < 65536 ? cdiv(tot, 256) : 65536), dim3(256), 0, st,
@@ -504,30 +551,31 @@ hipError_t ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t s
   hipLaunchKernelGGL(k_ww_pre, dim3(a.nw + 1 + cdiv(a.nk, 4)), dim3(256), 0, st, a, H, bp, Gm, cntp, llk);
   long long S = 0;
   for (int i = 0; i < ndig; i++) S += 1LL << (8 * i);
-  const long long np = (long long)a.nt * nwp * a.nk;
-  if (nsplit > 1)
-    hipLaunchKernelGGL(k_ww_zero, dim3(cdiv(np, 256) < 65536 ? cdiv(np, 256) : 65536), dim3(256), 0, st, Pbuf, np);
-  dim3 grid(cdiv(a.nk, 128), cdiv(a.nt, a.nt >= 3 ? 3 : a.nt), nsplit > 0 ? cdiv(a.nw, 32) * nsplit : 1);
+  WwLaunch L{ a, Dg, ldn, nwp, ndig, H, bp, Gm, cntp, llk, Pbuf, S, st };
+  // the tuned form of this shape class, measured here on first use (every form writes the
+  // same outputs, so the measured launches also leave the result)
+  const int nchunk = (int)(ldn / 128);
+  const int max_parts = std::min(16, nchunk);
+  const int tmode = terms_fit ? tune_mode() : 0;
+  if (tmode != 0 && a.nw > 0 && nchunk > 0) {
+    const TuneKey key{ kTuneWw, tune_bucket(a.nk), tune_bucket(a.nw), tune_bucket(nchunk), a.nt, ndig };
+    int ch = -1;
+    if (tune_get(key, &ch)) {
+      if (ch == 0 || ch == 1 || (ch > 1 && ch <= max_parts)) nsplit = ch;
+    } else if (tmode == 1) {
+      int cand[8], nc = 0, mi = -1;
+      cand[nc++] = 0; cand[nc++] = 1;
+      for (int q = 2; q <= max_parts; q *= 2) cand[nc++] = q;
+      for (int i = 0; i < nc; i++) if (cand[i] == nsplit) mi = i;
+      if (mi < 0) { mi = nc; cand[nc++] = nsplit; }
+      float best_ms = 0, model_ms = 0;
+      const int b = tune_measure(st, cand, nc, mi, ww_launch, &L, &best_ms, &model_ms);
+      tune_put(key, cand[b], best_ms, model_ms);
+      nsplit = cand[b];
+    }
+  }
   if (hook && hook->before) hook->before(hook->ctx, st);
-#define WW_LAUNCH(NTG, ND) do { if (nsplit > 1) \
-      hipLaunchKernelGGL((k_ww_mfma<NTG, ND, 2>), grid, dim3(256), 0, st, a, Dg, ldn, nwp, H, bp, Gm, cntp, llk, nsplit, Pbuf); \
-    else if (nsplit == 1) \
-      hipLaunchKernelGGL((k_ww_mfma<NTG, ND, 1>), grid, dim3(256), 0, st, a, Dg, ldn, nwp, H, bp, Gm, cntp, llk, 1, Pbuf); \
-    else hipLaunchKernelGGL((k_ww_mfma<NTG, ND, 0>), grid, dim3(256), 0, st, a, Dg, ldn, nwp, H, bp, Gm, cntp, llk, 1, Pbuf); } while (0)
-#define WW_ND(NTG) switch (ndig) { case 1: WW_LAUNCH(NTG, 1); break; case 2: WW_LAUNCH(NTG, 2); break; \
-                                   case 3: WW_LAUNCH(NTG, 3); break; default: WW_LAUNCH(NTG, 4); break; }
-  if (a.nt >= 3) { WW_ND(3) } else if (a.nt == 2) { WW_ND(2) } else { WW_ND(1) }
-#undef WW_ND
-#undef WW_LAUNCH
-  if (nsplit > 1) {
-    const long long nte = (long long)a.nt * a.nw * a.nk;
-    hipLaunchKernelGGL(k_ww_terms, dim3(cdiv(nte, 256) < 65536 ? cdiv(nte, 256) : 65536), dim3(256), 0, st,
-                       a, nwp, Pbuf, H, bp, Gm, cntp, llk, S);
-  }
-  if (nsplit > 0) {
-    const long long nc = (long long)a.nk * a.nt;
-    hipLaunchKernelGGL(k_ww_sum, dim3(cdiv(nc, 64)), dim3(64), 0, st, a, nwp, Pbuf);
-  }
+  ww_launch(&L, nsplit);
   if (hook && hook->after) hook->after(hook->ctx, st);
   return hipGetLastError();
 }

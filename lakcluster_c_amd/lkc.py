@@ -86,6 +86,11 @@ def lib():
         "lkc_bcc_set_profiling": (C.c_int, [vp, C.c_int]),
         "lkc_bcc_set_kernels": (C.c_int, [vp, C.c_int]),
         "lkc_bcc_kernel_stats": (C.c_int, [vp, cp, C.POINTER(C.c_longlong), dp, dp]),
+        "lkc_tune_set": (C.c_int, [C.c_int]),
+        "lkc_tune_get": (C.c_int, [ip, ip, C.POINTER(C.c_longlong), dp]),
+        "lkc_tune_save": (C.c_int, [cp]),
+        "lkc_tune_load": (C.c_int, [cp]),
+        "lkc_tune_clear": (C.c_int, []),
         "lkc_An_ajdk_v": (C.c_int, [u8p, C.c_size_t, C.c_int, ip, C.c_int, u8p, C.c_int, dp, dp]),
         "lkc_AnZt_vv": (C.c_int, [u8p, C.c_size_t, C.c_int, ip, C.c_int, u8p, C.c_size_t, C.c_int, ip, C.c_int,
                                   u8p, C.c_int, dp, dp, dp, dp]),
@@ -346,6 +351,36 @@ def dcc_sumscores_test(in_text: str, out_dir: Optional[str] = None) -> None:
     """The reference's dcc self-test (dcc_sumscores_test) on dcc_init's synthetic problem from
     the GLOBAL_TEST_* keys: several bins, Y/W controls; the three output files written."""
     _check(lib().lkc_dcc_sumscores_test(in_text.encode(), out_dir.encode() if out_dir else None))
+
+
+def tune_set(mode: int) -> None:
+    """Launch-shape autotuner mode: 0 off (cost model), 1 measure new shape classes, 2 frozen."""
+    _check(lib().lkc_tune_set(int(mode)))
+
+
+def tune_get() -> dict:
+    """The autotuner's mode, stored shape classes, classes measured, and the device
+    milliseconds its choices saved over the cost model's at measurement."""
+    m, n = C.c_int(), C.c_int()
+    meas, saved = C.c_longlong(), C.c_double()
+    _check(lib().lkc_tune_get(C.byref(m), C.byref(n), C.byref(meas), C.byref(saved)))
+    return {"mode": m.value, "classes": n.value, "measured": meas.value, "saved_ms": saved.value}
+
+
+def tune_save(path: str) -> int:
+    rc = lib().lkc_tune_save(path.encode())
+    _check(min(rc, 0))
+    return rc
+
+
+def tune_load(path: str) -> int:
+    rc = lib().lkc_tune_load(path.encode())
+    _check(min(rc, 0))
+    return rc
+
+
+def tune_clear() -> None:
+    _check(lib().lkc_tune_clear())
 
 
 def dquicksort_xij(d: np.ndarray) -> Tuple[np.ndarray, np.ndarray, int]:
