@@ -823,21 +823,45 @@ void Run::push_ctl()
   for (int i = 0; i < 11; i++) memcpy(hctl + ctl_mask_off[i], hm[i]->data(), bsize(mn[i]));
   // set positions of a mask in ascending order: zero bytes skipped, set bits taken
   // MSB-first with a count of leading zeros
+  // (written into a list sized for every position, then cut: full bytes take eight
+  // consecutive positions at once, empty 8-byte groups are skipped)
   auto fill_list = [&](IList& l, const std::vector<uint8_t>& m, int n) {
-    l.h.clear();
+    l.h.resize((size_t)std::max(n, 0));
+    int* o = l.h.data();
+    int c = 0;
     const int nb = (n + 7) / 8;
-    for (int b = 0; b < nb; b++) {
-      unsigned v = m[b];
-      while (v) {
-        const int k = __builtin_clz(v) - 24, i = 8 * b + k;
-        if (i >= n) break;
-        l.h.push_back(i);
-        v &= ~(0x80u >> k);
+    int b = 0;
+    while (b < nb) {
+      if (b + 8 <= nb) {
+        uint64_t w;
+        memcpy(&w, m.data() + b, 8);
+        if (!w) { b += 8; continue; }
       }
+      const unsigned v0 = m[b];
+      if (v0 == 0xFFu && 8 * b + 8 <= n) {
+        for (int k = 0; k < 8; k++) o[c++] = 8 * b + k;
+      } else {
+        unsigned v = v0;
+        while (v) {
+          const int k = __builtin_clz(v) - 24, i = 8 * b + k;
+          if (i >= n) break;
+          o[c++] = i;
+          v &= ~(0x80u >> k);
+        }
+      }
+      b++;
     }
+    l.h.resize(c);
+  };
+  // the retained sets equal the active ones after the drop (bcc_lrup_mxdup): copied
+  auto same = [](const std::vector<uint8_t>& x, const std::vector<uint8_t>& y, int n) {
+    return memcmp(x.data(), y.data(), (size_t)(n + 7) / 8) == 0;
   };
   fill_list(ar, hAr_j, NA); fill_list(kl, hAc_j, M); fill_list(tl, hTc_j, T); fill_list(zr, hZr_j, NZ);
-  fill_list(rtn, hAr_rtn, NA); fill_list(rmv, hAr_rmv, NA); fill_list(crt, hAc_rtn, M); fill_list(crm, hAc_rmv, M);
+  if (same(hAr_rtn, hAr_j, NA)) rtn.h = ar.h; else fill_list(rtn, hAr_rtn, NA);
+  fill_list(rmv, hAr_rmv, NA);
+  if (same(hAc_rtn, hAc_j, M)) crt.h = kl.h; else fill_list(crt, hAc_rtn, M);
+  fill_list(crm, hAc_rmv, M);
   // compacted column layout: per non-empty 1920-block a segment of whole words (offset
   // and length in 32-column words, block), the source column of each position and the
   // mask of the valid positions; the row length is padded to whole 4-word groups
@@ -1361,6 +1385,7 @@ void Run::st_xij()
   if (na) HIPCHK(hipMemcpyAsync(hsv + M, skey + M, sizeof(double) * na, hipMemcpyDeviceToHost, st));
   if (nc_drop) HIPCHK(hipMemcpyAsync(hidx, sidx, sizeof(int) * nc_drop, hipMemcpyDeviceToHost, st));
   if (nr_drop) HIPCHK(hipMemcpyAsync(hidx + M, sidx + M, sizeof(int) * nr_drop, hipMemcpyDeviceToHost, st));
+  finish_trace();
   HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipGetLastError());
   prof.collect();
@@ -1413,13 +1438,12 @@ void Run::st_mxA(bool dup)
   hAr_rtn = hAr_j; std::fill(hAr_rmv.begin(), hAr_rmv.end(), 0);
   for (int i = 0; i < nc_drop; i++) { int k = kl.h[QC_perm[i]]; bset(hAc_rmv.data(), k); bclr(hAc_rtn.data(), k); }
   for (int i = 0; i < nr_drop; i++) { int a = ar.h[QR_perm[i]]; bset(hAr_rmv.data(), a); bclr(hAr_rtn.data(), a); }
-  auto mean = [](const std::vector<double>& v, unsigned long long n) {
-    double m = 0; for (unsigned long long i = 0; i < n; i++) m += v[i]; m /= (double)n; return m;
-  };
-  double mr = mean(QR_sv, (unsigned long long)rpop); if (!std::isfinite(mr)) mr = 1.0;
-  double mc = mean(QC_sv, (unsigned long long)cpop_i); if (!std::isfinite(mc)) mc = 1.0;
-  double tr[6] = { (double)n_iter, (double)rpop, (double)cpop_i, mr, mc, (double)Irem };
+  double tr[6] = { (double)n_iter, (double)rpop, (double)cpop_i, 0.0, 0.0, (double)Irem };
   trace.insert(trace.end(), tr, tr + 6);
+  // the trace line's mean scores (ordered sums over the sorted values); inside the
+  // engine's own loop they are formed while the device runs the update
+  trace_pending = true; trace_at = trace.size() - 6; trace_rpop = rpop; trace_cpop = cpop_i;
+  if (!defer_means) finish_trace();
   n_iter++;                        // D->out_iteration (bcc_sumscores_mxC)
   for (int i = 0; i < nr_drop; i++) {
     int a = ar.h[QR_perm[i]];
@@ -1445,10 +1469,30 @@ void Run::st_mxdup()
 }
 
 // one pass's bcc_sumscores_nrm/ifT/srt/cmb/xij/mxA (bcc_sumscores.c:37-779), one bin,
-// Ireq <= 0, with the active sets moved to the retained ones at once
-void Run::sumscores()
+// Ireq <= 0, with the active sets moved to the retained ones at once.  With defer, the
+// trace line's means wait for finish_trace (the engine's loops call it once the pass's
+// update is enqueued).
+void Run::sumscores(bool defer)
 {
-  st_nrm(); st_ifT(); st_srt(); st_cmb(); st_xij(); st_mxA(true);
+  st_nrm(); st_ifT(); st_srt(); st_cmb(); st_xij();
+  defer_means = defer;
+  st_mxA(true);
+  defer_means = false;
+}
+
+// the mean row and column scores of the last trace line: the sorted values summed in
+// order (bcc_sumscores_mxB, bcc_sumscores.c:37-125), a non-finite mean written as 1
+void Run::finish_trace()
+{
+  if (!trace_pending) return;
+  trace_pending = false;
+  auto mean = [](const std::vector<double>& v, unsigned long long n) {
+    double m = 0; for (unsigned long long i = 0; i < n; i++) m += v[i]; m /= (double)n; return m;
+  };
+  double mr = mean(QR_sv, (unsigned long long)trace_rpop); if (!std::isfinite(mr)) mr = 1.0;
+  double mc = mean(QC_sv, (unsigned long long)trace_cpop); if (!std::isfinite(mc)) mc = 1.0;
+  trace[trace_at + 3] = mr;
+  trace[trace_at + 4] = mc;
 }
 
 // a Gram over the full column layout: split over the 1920-blocks when it has few tiles
@@ -1989,12 +2033,13 @@ void Run::iterate(int* done)
   if (!inited) throw Error(LKC_ERR_STATE, "call init first");
   if (!active()) { if (done) *done = 1; return; }
   double t0 = now_s();
-  sumscores();
+  sumscores(true);
   if (dcc) { st_ajdk(); st_ZtSn(); st_dcc(); }     // dcc: recomputed, no low-rank update
   else {
     lrup();
     st_QR(); st_ajdk(); st_ZtSn(); st_single(); st_double(); st_flatten();
   }
+  finish_trace();
   HIPCHK(hipGetLastError());
   if (done) *done = active() ? 0 : 1;
   if (!active()) HIPCHK(hipStreamSynchronize(st));
@@ -2008,12 +2053,13 @@ void Run::run(int max_iter)
   double t0 = now_s();
   loop_s = 0;
   while (!done && (max_iter < 0 || n_iter < max_iter)) {
-    sumscores();
+    sumscores(true);
     if (dcc) { st_ajdk(); st_ZtSn(); st_dcc(); }
     else {
       lrup();
       st_QR(); st_ajdk(); st_ZtSn(); st_single(); st_double(); st_flatten();
     }
+    finish_trace();
     done = active() ? 0 : 1;
   }
   HIPCHK(hipStreamSynchronize(st));
@@ -2032,7 +2078,7 @@ void Run::reset()
   std::fill(hAc_rtn.begin(), hAc_rtn.end(), 0); std::fill(hAc_rmv.begin(), hAc_rmv.end(), 0);
   push_ctl();
   HIPCHK(hipStreamSynchronize(st));
-  trace.clear(); xdrop_a.clear(); xdrop_b.clear();
+  trace.clear(); xdrop_a.clear(); xdrop_b.clear(); trace_pending = false;
   n_iter = 0; inited = false; ops = 0; init_s = 0; loop_s = 0; n_sort_fallback = 0;
   xij_done = false; pending_dup = false; a0_valid = false; b0_valid = false; vn_epoch = -1;
   for (auto& k : prof.stats) { k.launches = 0; k.ms = 0; k.work = 0; }

@@ -191,7 +191,11 @@ struct Run {
   void init_lf();
   void score_lists();
   void st_nrm(); void st_ifT(); void st_srt(); void st_cmb(); void st_xij(); void st_mxA(bool dup); void st_mxdup();
-  void sumscores();
+  void sumscores(bool defer = false);
+  void finish_trace();
+  bool defer_means = false, trace_pending = false;
+  size_t trace_at = 0;
+  int trace_rpop = 0, trace_cpop = 0;
   void lrup();
   void lrup_QR_0(); void lrup_QR_1(); void lrup_QR_3();
   void lrup_QC_a0(); void lrup_QC_a1(); void lrup_QC_a2(); void lrup_QC_a3();
