@@ -2,12 +2,14 @@
 //
 // Every job is an ordinary elimination (Run) whose inputs are drawn from a device-resident
 // pool; jobs share nothing but the pool, so a job's trace is the same alone or in a mix.
-// Jobs are admitted in arrival order while their memory estimate fits the HBM budget;
+// Jobs are admitted in arrival order (jobs arriving together largest first) while their
+// memory estimate fits the HBM budget;
 // each running job drives its own HIP stream from its own host thread, so the small
 // kernels of concurrent jobs overlap on the device.  Arrival times make the same loop a
 // server: a job is admitted no earlier than its arrival, latency = end - arrival.
 #include "lkc_engine.h"
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <mutex>
@@ -36,18 +38,33 @@ void run_batch(const Pool& P, const JobIn* jobs, int n, int max_conc, double bud
   std::condition_variable cv;
   int next = 0, running = 0;
   double reserved = 0;
+  // admission order: by arrival; jobs that arrive together (a batch) largest first, so the
+  // last jobs to finish are short ones and the device stays full to the end
+  std::vector<int> order(n);
+  std::vector<double> work(n);
+  for (int i = 0; i < n; i++) {
+    order[i] = i;
+    int cols = 0;
+    if (jobs[i].colmask) { for (int k = 0; k < P.M; k++) cols += (jobs[i].colmask[k >> 3] >> (7 - (k & 7))) & 1; }
+    else cols = P.M;
+    work[i] = (double)jobs[i].nA * (double)(jobs[i].nA + jobs[i].nZ) * (double)cols;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    if (jobs[x].arrival_s != jobs[y].arrival_s) return jobs[x].arrival_s < jobs[y].arrival_s;
+    return work[x] > work[y];
+  });
   auto worker = [&]() {
     for (;;) {
       std::unique_lock<std::mutex> lk(mu);
       if (next >= n) return;
-      // FIFO: wait for the head job's arrival, then for room in the budget
-      const int j = next;
+      // in admission order: wait for the head job's arrival, then for room in the budget
+      const int j = order[next];
       const auto arrive = t0 + std::chrono::duration_cast<clk::duration>(std::chrono::duration<double>(jobs[j].arrival_s));
-      while (clk::now() < arrive && next == j) cv.wait_until(lk, arrive);
-      if (next != j) continue;
+      while (next < n && order[next] == j && clk::now() < arrive) cv.wait_until(lk, arrive);
+      if (next >= n || order[next] != j) continue;
       const double need = job_bytes(jobs[j].nA, jobs[j].nZ, P.M, jobs[j].cfg.B);
-      while (next == j && running > 0 && reserved + need > budget) cv.wait(lk);
-      if (next != j) continue;
+      while (next < n && order[next] == j && running > 0 && reserved + need > budget) cv.wait(lk);
+      if (next >= n || order[next] != j) continue;
       next++; reserved += need; running++;
       lk.unlock();
       cv.notify_all();

@@ -8,6 +8,7 @@
 #include "lkc_engine.h"
 
 #include <algorithm>
+#include <mutex>
 #include <thread>
 #include "lkc_stages.h"
 
@@ -311,11 +312,81 @@ void* Grow::need(Dev& d, size_t bytes)
   return p;
 }
 
+// sizes of the pinned host buffers handed out (freed ones are kept with their size)
+namespace {
+std::mutex g_pin_sz_mu;
+std::vector<std::pair<void*, size_t>> g_pin_sz;
+}
+static void pinned_size_note(void* p, size_t bytes)
+{
+  std::lock_guard<std::mutex> lk(g_pin_sz_mu);
+  g_pin_sz.push_back({ p, bytes });
+}
+static size_t pinned_size_of(void* p)
+{
+  std::lock_guard<std::mutex> lk(g_pin_sz_mu);
+  for (const auto& e : g_pin_sz) if (e.first == p) return e.second;
+  return 0;
+}
+
+// Streams and pinned host buffers outlive the runs that used them: a finished run hands
+// them back and the next one takes them, so jobs that start and end while others run
+// create and free neither (freeing pinned memory waits for the whole device).
+namespace {
+std::mutex g_res_mu;
+std::vector<hipStream_t> g_free_streams;
+std::vector<std::pair<size_t, void*>> g_free_pinned;
+}
+static hipStream_t stream_take()
+{
+  {
+    std::lock_guard<std::mutex> lk(g_res_mu);
+    if (!g_free_streams.empty()) { hipStream_t s = g_free_streams.back(); g_free_streams.pop_back(); return s; }
+  }
+  hipStream_t s = nullptr;
+  HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  return s;
+}
+static void stream_give(hipStream_t s)
+{
+  if (!s) return;
+  std::lock_guard<std::mutex> lk(g_res_mu);
+  g_free_streams.push_back(s);
+}
+// the smallest free buffer of at least bytes (and at most four times that), else a new one
+static void* pinned_take(size_t bytes)
+{
+  {
+    std::lock_guard<std::mutex> lk(g_res_mu);
+    size_t best = (size_t)-1;
+    for (size_t i = 0; i < g_free_pinned.size(); i++) {
+      const size_t b = g_free_pinned[i].first;
+      if (b >= bytes && b / 4 <= bytes && (best == (size_t)-1 || b < g_free_pinned[best].first)) best = i;
+    }
+    if (best != (size_t)-1) {
+      void* p = g_free_pinned[best].second;
+      g_free_pinned.erase(g_free_pinned.begin() + best);
+      return p;
+    }
+  }
+  void* p = nullptr;
+  HIPCHK(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+  pinned_size_note(p, bytes);
+  return p;
+}
+static void pinned_give(void* p)
+{
+  if (!p) return;
+  const size_t bytes = pinned_size_of(p);
+  std::lock_guard<std::mutex> lk(g_res_mu);
+  g_free_pinned.push_back({ bytes, p });
+}
+
 // the run's stream; device memory of the run comes from the pool, kept for reuse
 static void make_stream(hipStream_t* st, Dev& dev)
 {
   HIPCHK(hipSetDevice(0));
-  HIPCHK(hipStreamCreateWithFlags(st, hipStreamNonBlocking));
+  *st = stream_take();
   static const bool pool_set = [] {   // once per process, thread-safe
     hipMemPool_t mp;
     if (hipDeviceGetDefaultMemPool(&mp, 0) == hipSuccess) {
@@ -387,10 +458,10 @@ Run::~Run()
 {
   dev.release_all();
   if (st) (void)hipStreamSynchronize(st);
-  if (hctl) (void)hipHostFree(hctl);
-  if (hsv) (void)hipHostFree(hsv);
-  if (hidx) (void)hipHostFree(hidx);
-  if (st) (void)hipStreamDestroy(st);
+  pinned_give(hctl);
+  pinned_give(hsv);
+  pinned_give(hidx);
+  stream_give(st);
 }
 
 void Run::load(const char* in_text)
@@ -583,7 +654,7 @@ void Run::setup_device(double t0)
   ctl_cmp_off[7] = reserve(sizeof(int) * 3 * (size_t)(pcols + 1));
   ctl_cmp_off[8] = reserve(sizeof(int) * 32 * (size_t)cmp_Wmax);
   ctl_bytes = off;
-  HIPCHK(hipHostMalloc((void**)&hctl, ctl_bytes, hipHostMallocDefault));
+  hctl = (uint8_t*)pinned_take(ctl_bytes);
   memset(hctl, 0, ctl_bytes);
   dctl = dev.alloc<uint8_t>(ctl_bytes);
   uint32_t** dm[11] = { &Ar_b, &Ar_j, &Ar_rtn, &Ar_rmv, &Zr_b, &Zr_j, &Ac_b, &Ac_j, &Ac_rtn, &Ac_rmv, &Tc_j };
@@ -701,10 +772,10 @@ void Run::setup_device(double t0)
   scal = dev.alloc<double>(16);
   scal_s = dev.alloc<double>(4 * 16);
   dsv = dev.alloc<double>(nM + nA);
-  HIPCHK(hipHostMalloc((void**)&hsv, sizeof(double) * (nM + nA), hipHostMallocDefault));
+  hsv = (double*)pinned_take(sizeof(double) * (nM + nA));
   skey = dev.alloc<double>(nM + nA);
   sidx_in = dev.alloc<int>(nM + nA); sidx = dev.alloc<int>(nM + nA);
-  HIPCHK(hipHostMalloc((void**)&hidx, sizeof(int) * (nM + nA), hipHostMallocDefault));
+  hidx = (int*)pinned_take(sizeof(int) * (nM + nA));
   sort_tmp_bytes = std::max(sort_pairs_temp_bytes(M), sort_pairs_temp_bytes(NA));
   if (!sort_tmp_bytes) throw Error(LKC_ERR_HIP, "radix sort set-up failed");
   sort_tmp = dev.raw(sort_tmp_bytes);
@@ -1100,7 +1171,7 @@ void Run::ww_contract(int slot, const WwTerm& w)
     HIPCHK(ww_valu(a, (double*)g_work.need(dev, wb), wb, st));
     hook.after(&cx, st);
   } else {
-    size_t wb = ww_mfma_work_bytes(a.nk, a.nt, nw, a.prb, c.B);
+    size_t wb = ww_mfma_work_bytes(a.nk, a.nt, nw, a.prb, c.B, a.pcols);
     HIPCHK(ww_mfma(a, g_work.need(dev, wb), wb, st, &hook));
   }
 }

@@ -86,7 +86,7 @@ size_t ww_work_bytes(int nk, int nt, int nw);
 hipError_t ww_valu(const WwArgs& a, double* work, size_t work_bytes, hipStream_t st);
 
 // int8 matrix-core formulation (cz split into signed bytes); planes are not used
-size_t ww_mfma_work_bytes(int nk, int nt, int nw, size_t prb, int B);
+size_t ww_mfma_work_bytes(int nk, int nt, int nw, size_t prb, int B, int pcols);
 // hook: called on the stream right before and after the matrix-core kernel (timing)
 struct WwHook { void (*before)(void*, hipStream_t); void (*after)(void*, hipStream_t); void* ctx; };
 hipError_t ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t st, const WwHook* hook = nullptr);

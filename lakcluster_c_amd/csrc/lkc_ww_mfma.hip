@@ -171,18 +171,20 @@ __device__ __forceinline__ void wave_lds_sync()
 // sums there with 64-bit atomics (integer sums in any order are exact) and k_ww_terms
 // forms the terms in place.  k_ww_sum then adds each row's terms in ascending w.
 template <int NTG, int ND, int MODE>
-__global__ __launch_bounds__(256) void k_ww_mfma(WwArgs a, const int8_t* Dg, long long ldn, int nwp,
-                                                 const double* H, const double* bp, const long long* Gm,
+__global__ __launch_bounds__(256, NTG == 1 ? 2 : 1) void k_ww_mfma(WwArgs a, const int8_t* Dg, long long ldn, int nwp,
+                                                 const double* X, const double* bp, const long long* Gm,
                                                  const int* cntp, const int* llk, int nsplit,
                                                  unsigned long long* Pbuf)
 {
   __shared__ __attribute__((aligned(16))) int8_t Bs[2][ND][32][WW_BROW];
   __shared__ double Tm[4][32][33];                    // terms [wave][k][w]
-  __shared__ double KcA[128], KcQ[128];               // alpha of k's column block, of q's
+  __shared__ double KcQ[128];                         // alpha of q's column block
+  __shared__ int KcB[128];                            // k's column block
+  __shared__ int WvB[4];                              // the wave's rows' one column block, or -1
   __shared__ int Kcnt[NTG][128], Kllk[NTG][128];      // cntp, llk of the block's rows k
   __shared__ double Cs[2 + NTG];                      // min, mlt, bp[t]
-  __shared__ long long WGm[32];                       // the w step's Gm, H, S signs, w indices
-  __shared__ double WHt[NTG][32];
+  __shared__ long long WGm[32];                       // the w step's Gm, X, S signs, w indices
+  __shared__ double WX[4][NTG][32];
   __shared__ int WvS[NTG][32];
   __shared__ int Wws[32];
   __shared__ uint32_t Zb[128];                        // bit j: Q(q_k, w0 + j) == +1
@@ -202,7 +204,7 @@ __global__ __launch_bounds__(256) void k_ww_mfma(WwArgs a, const int8_t* Dg, lon
   } else if (tid < 128) {
     const int myk = min(blockIdx.x * 128 + tid, a.nk - 1);
     const double* alpha = a.ajdk + (size_t)1 * a.pcols;
-    KcA[tid] = alpha[a.klist[myk] / kPopBits];
+    KcB[tid] = a.klist[myk] / kPopBits;
     KcQ[tid] = alpha[a.qlist[myk] / kPopBits];
 #pragma unroll
     for (int t = 0; t < NTG; t++) {
@@ -214,6 +216,15 @@ __global__ __launch_bounds__(256) void k_ww_mfma(WwArgs a, const int8_t* Dg, lon
     Cs[0] = a.scal[2]; Cs[1] = a.scal[4];
 #pragma unroll
     for (int t = 0; t < NTG; t++) Cs[2 + t] = bp[t0 + min(t, nts - 1)];
+  }
+  __syncthreads();
+  // a wave whose 32 rows share one column block reads the per-(block, covariate, w) part of
+  // its terms once per w (staged below); otherwise each row reads its own
+  if (MODE != 2 && tid < 4) {
+    const int b0 = KcB[32 * tid];
+    bool one = true;
+    for (int j = 1; j < 32; j++) one = one && KcB[32 * tid + j] == b0;
+    WvB[tid] = one ? b0 : -1;
   }
   // ordered sums over w of row kbase + lane (lane < 32), per covariate
   double o[NTG];
@@ -264,8 +275,15 @@ __global__ __launch_bounds__(256) void k_ww_mfma(WwArgs a, const int8_t* Dg, lon
 #pragma unroll
       for (int t = 0; t < NTG; t++) {
         const int tt = t0 + min(t, nts - 1);
-        WHt[t][tid] = H[(size_t)tt * a.nw + w];
         WvS[t][tid] = bitv(a.Sc.p + (size_t)a.slist[tt] * a.Sc.rb, ww) ? 1 : -1;
+      }
+    }
+    if (MODE != 2 && tid < 128) {
+      const int wv = tid >> 5, j = tid & 31, b = WvB[wv];
+      if (b >= 0) {
+        const int w = w0 + min(j, wn - 1);
+#pragma unroll
+        for (int t = 0; t < NTG; t++) WX[wv][t][j] = X[((size_t)b * a.nt + t0 + min(t, nts - 1)) * a.nw + w];
       }
     }
     if (c_hi > c_lo) { WW_BLOAD(c_lo); WW_ALOAD(c_lo); WW_BSTORE(0); }
@@ -342,11 +360,13 @@ __global__ __launch_bounds__(256) void k_ww_mfma(WwArgs a, const int8_t* Dg, lon
 #pragma unroll
     for (int t = 0; t < NTG; t++) {
       if (t >= nts) break;
-      const double mn = Cs[0], mlt = Cs[1], bpt = Cs[2 + t];
+      const double mn = Cs[0], mlt = Cs[1];
+      const int tt = t0 + min(t, nts - 1);
       if (r < wn) {
         const long long gm = WGm[r];
-        const double ht = WHt[t][r];
         const double vS = (double)WvS[t][r];
+        const int bw = WvB[wave];
+        const double xw = bw >= 0 ? WX[wave][t][r] : 0.0;
 #pragma unroll
         for (int v = 0; v < 16; v++) {
           const int kw = 8 * (v / 4) + 4 * g + (v % 4);        // row within the wave
@@ -355,12 +375,12 @@ __global__ __launch_bounds__(256) void k_ww_mfma(WwArgs a, const int8_t* Dg, lon
 #pragma unroll
           for (int i = ND - 1; i >= 0; i--) spc = spc * 256 + (long long)acc[t][i][v];
           const long long G = 2 * (spc + 128 * S * (long long)Kcnt[t][kl]) - gm;
-          const double alpha_k = KcA[kl], alpha_q = KcQ[kl];
+          const double alpha_q = KcQ[kl];
           const double bkk = (double)Kllk[t][kl] * mn * mlt;
-          const double at_base = bpt * alpha_k;
-          const double at_tmp = ht * alpha_k;
           const double gd = (double)G;
-          const double tmp = (double)(bkk + gd) / mlt - (double)(at_base + at_tmp) / mlt;
+          // X = (bp alpha_k + H alpha_k) / mlt of the row's column block, covariate and w
+          const double xk = bw >= 0 ? xw : X[((size_t)KcB[kl] * a.nt + tt) * a.nw + w0 + r];
+          const double tmp = (double)(bkk + gd) / mlt - xk;
           const int vZ = ((Zb[kl] >> r) & 1u) ? 1 : -1;
           const double term = tmp * vS * (vZ - alpha_q);
           if (MODE == 1) {
@@ -388,7 +408,7 @@ __global__ __launch_bounds__(256) void k_ww_mfma(WwArgs a, const int8_t* Dg, lon
 
 // The terms of the split form with several parts, in place: one thread per (t, w, k), the
 // same double expression as the fused kernel's.
-__global__ void k_ww_terms(WwArgs a, int nwp, unsigned long long* Pbuf, const double* H, const double* bp,
+__global__ void k_ww_terms(WwArgs a, int nwp, unsigned long long* Pbuf, const double* X,
                            const long long* Gm, const int* cntp, const int* llk, long long S)
 {
   const long long tot = (long long)a.nt * a.nw * a.nk;
@@ -399,15 +419,13 @@ __global__ void k_ww_terms(WwArgs a, int nwp, unsigned long long* Pbuf, const do
     const long long tw = e / a.nk;
     const int w = (int)(tw % a.nw), tt = (int)(tw / a.nw);
     const int kk = a.klist[k], qq = a.qlist[k], ww = a.wlist[w];
-    const double alpha_k = alpha[kk / kPopBits], alpha_q = alpha[qq / kPopBits];
+    const double alpha_q = alpha[qq / kPopBits];
     const double bkk = (double)llk[(size_t)tt * a.nk + k] * mn * mlt;
     const long long off = 128 * S * (long long)cntp[(size_t)tt * a.nk + k];
-    const double at_base = bp[tt] * alpha_k;
     unsigned long long* P = Pbuf + ((size_t)tt * nwp + w) * a.nk + k;
     const long long G = 2 * ((long long)*P + off) - Gm[w];
-    const double at_tmp = H[(size_t)tt * a.nw + w] * alpha_k;
     const double gd = (double)G;
-    const double tmp = (double)(bkk + gd) / mlt - (double)(at_base + at_tmp) / mlt;
+    const double tmp = (double)(bkk + gd) / mlt - X[((size_t)(kk / kPopBits) * a.nt + tt) * a.nw + w];
     const int vS = bitv(a.Sc.p + (size_t)a.slist[tt] * a.Sc.rb, ww) ? 1 : -1;
     const int vZ = bitv(a.Q.p + (size_t)qq * a.Q.rb, ww) ? 1 : -1;
     *(double*)P = tmp * vS * (vZ - alpha_q);
@@ -465,11 +483,32 @@ static int ww_split_parts(int nk, int nt, int nw, size_t prb, int ndig)
   return (int)std::max<long long>(parts, 1);
 }
 
+// X[(b*nt + t)*nw + w] = (bp[t] alpha_b + H[t][w] alpha_b) / mlt for every column block b:
+// the part of the reference's term that depends on k only through its column block
+// (At_T_YnWt_S_Zn_ww.c:95-104), formed once per (block, covariate, w) with the same
+// operations in the same order
+__global__ void k_ww_xtab(const double* H, const double* bp, const double* ajdk, int pcols, int nt, int nw,
+                          const double* scal, double* X)
+{
+  const long long tot = (long long)pcols * nt * nw;
+  const double* alpha = ajdk + (size_t)1 * pcols;
+  const double mlt = scal[4];
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
+    const int w = (int)(e % nw);
+    const long long bt = e / nw;
+    const int t = (int)(bt % nt), b = (int)(bt / nt);
+    const double alpha_k = alpha[b];
+    const double at_base = bp[t] * alpha_k;
+    const double at_tmp = H[(size_t)t * nw + w] * alpha_k;
+    X[e] = (double)(at_base + at_tmp) / mlt;
+  }
+}
+
 // the split forms the tuner may try besides the cost model's: terms buffers up to 256 MiB
 static const double kTunePbufMax = 256.0 * 1024 * 1024;
 static double ww_pbytes(int nk, int nt, int nw) { return (double)nt * (double)(cdiv(nw, 32) * 32) * (double)nk * 8.0; }
 
-size_t ww_mfma_work_bytes(int nk, int nt, int nw, size_t prb, int B)
+size_t ww_mfma_work_bytes(int nk, int nt, int nw, size_t prb, int B, int pcols)
 {
   int ndig = (B + 7) / 8;
   size_t nwp = (size_t)((nw + 31) / 32) * 32, ldn = prb * 8;
@@ -478,6 +517,7 @@ size_t ww_mfma_work_bytes(int nk, int nt, int nw, size_t prb, int B)
   b += sizeof(double) * ((size_t)nt * nw + nt + 8);
   b += sizeof(long long) * ((size_t)nw + 8);
   b += 2 * sizeof(int) * ((size_t)nt * nk + 8);
+  b += sizeof(double) * ((size_t)pcols * nt * nw + 8) + 256;
   const bool terms = ww_split_parts(nk, nt, nw, prb, ndig) > 0 || (tune_mode() != 0 && ww_pbytes(nk, nt, nw) <= kTunePbufMax);
   if (terms) b += 256 + sizeof(unsigned long long) * (size_t)nt * nwp * nk;
   return b + 1024;
@@ -485,34 +525,40 @@ size_t ww_mfma_work_bytes(int nk, int nt, int nw, size_t prb, int B)
 
 namespace {
 // one form of the contraction after the digit and per-w / per-k kernels: nsplit 0 the fused
-// kernel, 1 one block per (rows, w step) with k_ww_sum, >1 also n parts with k_ww_terms
+// kernel, 1 one block per (rows, w step) with k_ww_sum, >1 also n parts with k_ww_terms;
+// kWwOneT + nsplit the same with one covariate per block (fewer registers: two blocks per
+// CU, which hides the double epilogue of contractions over few n)
+constexpr int kWwOneT = 100;
 struct WwLaunch {
-  WwArgs a; const int8_t* Dg; long long ldn; int nwp, ndig; const double* H; const double* bp; const long long* Gm;
+  WwArgs a; const int8_t* Dg; long long ldn; int nwp, ndig; const double* X; const double* bp; const long long* Gm;
   const int* cntp; const int* llk; unsigned long long* Pbuf; long long S; hipStream_t st;
 };
-void ww_launch(void* ctx, int nsplit)
+void ww_launch(void* ctx, int choice)
 {
   const WwLaunch& L = *(const WwLaunch*)ctx;
   const WwArgs& a = L.a;
   hipStream_t st = L.st;
+  const bool one_t = choice >= kWwOneT;
+  const int nsplit = one_t ? choice - kWwOneT : choice;
   const long long np = (long long)a.nt * L.nwp * a.nk;
   if (nsplit > 1)
     hipLaunchKernelGGL(k_ww_zero, dim3(cdiv(np, 256) < 65536 ? cdiv(np, 256) : 65536), dim3(256), 0, st, L.Pbuf, np);
-  dim3 grid(cdiv(a.nk, 128), cdiv(a.nt, a.nt >= 3 ? 3 : a.nt), nsplit > 0 ? cdiv(a.nw, 32) * nsplit : 1);
+  const int ntg = one_t ? 1 : (a.nt >= 3 ? 3 : a.nt);
+  dim3 grid(cdiv(a.nk, 128), cdiv(a.nt, ntg), nsplit > 0 ? cdiv(a.nw, 32) * nsplit : 1);
 #define WW_LAUNCH(NTG, ND) do { if (nsplit > 1) \
-      hipLaunchKernelGGL((k_ww_mfma<NTG, ND, 2>), grid, dim3(256), 0, st, a, L.Dg, L.ldn, L.nwp, L.H, L.bp, L.Gm, L.cntp, L.llk, nsplit, L.Pbuf); \
+      hipLaunchKernelGGL((k_ww_mfma<NTG, ND, 2>), grid, dim3(256), 0, st, a, L.Dg, L.ldn, L.nwp, L.X, L.bp, L.Gm, L.cntp, L.llk, nsplit, L.Pbuf); \
     else if (nsplit == 1) \
-      hipLaunchKernelGGL((k_ww_mfma<NTG, ND, 1>), grid, dim3(256), 0, st, a, L.Dg, L.ldn, L.nwp, L.H, L.bp, L.Gm, L.cntp, L.llk, 1, L.Pbuf); \
-    else hipLaunchKernelGGL((k_ww_mfma<NTG, ND, 0>), grid, dim3(256), 0, st, a, L.Dg, L.ldn, L.nwp, L.H, L.bp, L.Gm, L.cntp, L.llk, 1, L.Pbuf); } while (0)
+      hipLaunchKernelGGL((k_ww_mfma<NTG, ND, 1>), grid, dim3(256), 0, st, a, L.Dg, L.ldn, L.nwp, L.X, L.bp, L.Gm, L.cntp, L.llk, 1, L.Pbuf); \
+    else hipLaunchKernelGGL((k_ww_mfma<NTG, ND, 0>), grid, dim3(256), 0, st, a, L.Dg, L.ldn, L.nwp, L.X, L.bp, L.Gm, L.cntp, L.llk, 1, L.Pbuf); } while (0)
 #define WW_ND(NTG) switch (L.ndig) { case 1: WW_LAUNCH(NTG, 1); break; case 2: WW_LAUNCH(NTG, 2); break; \
                                      case 3: WW_LAUNCH(NTG, 3); break; default: WW_LAUNCH(NTG, 4); break; }
-  if (a.nt >= 3) { WW_ND(3) } else if (a.nt == 2) { WW_ND(2) } else { WW_ND(1) }
+  if (ntg == 3) { WW_ND(3) } else if (ntg == 2) { WW_ND(2) } else { WW_ND(1) }
 #undef WW_ND
 #undef WW_LAUNCH
   if (nsplit > 1) {
     const long long nte = (long long)a.nt * a.nw * a.nk;
     hipLaunchKernelGGL(k_ww_terms, dim3(cdiv(nte, 256) < 65536 ? cdiv(nte, 256) : 65536), dim3(256), 0, st,
-                       a, L.nwp, L.Pbuf, L.H, L.bp, L.Gm, L.cntp, L.llk, L.S);
+                       a, L.nwp, L.Pbuf, L.X, L.Gm, L.cntp, L.llk, L.S);
   }
   if (nsplit > 0) {
     const long long nc = (long long)a.nk * a.nt;
@@ -535,6 +581,8 @@ hipError_t ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t s
   long long* Gm = (long long*)(p + off); off += sizeof(long long) * ((size_t)a.nw + 8);
   int* cntp = (int*)(p + off); off += sizeof(int) * ((size_t)a.nt * a.nk + 8);
   int* llk = (int*)(p + off); off += sizeof(int) * ((size_t)a.nt * a.nk + 8);
+  off = (off + 255) & ~(size_t)255;
+  double* X = (double*)(p + off); off += sizeof(double) * ((size_t)a.pcols * a.nt * a.nw + 8);
   unsigned long long* Pbuf = (unsigned long long*)(((uintptr_t)(p + off) + 255) & ~(uintptr_t)255);
   const long long np = (long long)a.nt * nwp * a.nk;
   // the caller sizes work with ww_mfma_work_bytes; the split forms need the terms buffer
@@ -549,9 +597,14 @@ hipError_t ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t s
                        (const uint8_t*)a.pm, ndig, ldn, Dg);
   }
   hipLaunchKernelGGL(k_ww_pre, dim3(a.nw + 1 + cdiv(a.nk, 4)), dim3(256), 0, st, a, H, bp, Gm, cntp, llk);
+  {
+    const long long nx = (long long)a.pcols * a.nt * a.nw;
+    hipLaunchKernelGGL(k_ww_xtab, dim3(cdiv(nx, 256) < 65536 ? cdiv(nx, 256) : 65536), dim3(256), 0, st,
+                       H, bp, a.ajdk, a.pcols, a.nt, a.nw, a.scal, X);
+  }
   long long S = 0;
   for (int i = 0; i < ndig; i++) S += 1LL << (8 * i);
-  WwLaunch L{ a, Dg, ldn, nwp, ndig, H, bp, Gm, cntp, llk, Pbuf, S, st };
+  WwLaunch L{ a, Dg, ldn, nwp, ndig, X, bp, Gm, cntp, llk, Pbuf, S, st };
   // the tuned form of this shape class, measured here on first use (every form writes the
   // same outputs, so the measured launches also leave the result)
   const int nchunk = (int)(ldn / 128);
@@ -561,11 +614,13 @@ hipError_t ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t s
     const TuneKey key{ kTuneWw, tune_bucket(a.nk), tune_bucket(a.nw), tune_bucket(nchunk), a.nt, ndig };
     int ch = -1;
     if (tune_get(key, &ch)) {
-      if (ch == 0 || ch == 1 || (ch > 1 && ch <= max_parts)) nsplit = ch;
+      const int c = ch >= kWwOneT ? ch - kWwOneT : ch;
+      if ((ch < kWwOneT || a.nt > 1) && (c == 0 || c == 1 || (c > 1 && c <= max_parts))) nsplit = ch;
     } else if (tmode == 1) {
-      int cand[8], nc = 0, mi = -1;
+      int cand[12], nc = 0, mi = -1;
       cand[nc++] = 0; cand[nc++] = 1;
       for (int q = 2; q <= max_parts; q *= 2) cand[nc++] = q;
+      if (a.nt > 1) { cand[nc++] = kWwOneT; cand[nc++] = kWwOneT + 1; }
       for (int i = 0; i < nc; i++) if (cand[i] == nsplit) mi = i;
       if (mi < 0) { mi = nc; cand[nc++] = nsplit; }
       float best_ms = 0, model_ms = 0;
