@@ -176,7 +176,10 @@ def nulls_leg(n_null, k_null, seed=7):
     out = {"runs": n_null}
     with lkc.Pool(pool.rows, pool.nrows, pool.ncols, pool.cov, pool.ncov) as P:
         tune = tune_warmup(P, pool, seed)
+        # phase bounds on the monotonic clock (the profiler's kernel timestamps use it too)
+        m0 = time.monotonic_ns()
         t = time.time(); plain = P.batch(spec, max_concurrent=1); tp = time.time() - t
+        out["phase_ns"] = {"plain": [m0, time.monotonic_ns()]}
         print(f"[bench] nulls: {n_null} runs one at a time in {tp:.2f} s", file=sys.stderr, flush=True)
         null_ops = float(sum(r["ops"] for r in plain))
         out["tuning"] = tune
@@ -184,7 +187,9 @@ def nulls_leg(n_null, k_null, seed=7):
                     "passes_mean": float(np.mean([r["n_iter"] for r in plain])),
                     "bit_ops_per_run_mean": null_ops / n_null, "plain_Tbitops": null_ops / tp / 1e12})
         if k_null > 1:
+            m0 = time.monotonic_ns()
             t = time.time(); bat = P.batch(spec, max_concurrent=k_null); tb = time.time() - t
+            out["phase_ns"]["batched"] = [m0, time.monotonic_ns()]
             print(f"[bench] nulls: {n_null} runs {k_null} at a time in {tb:.2f} s", file=sys.stderr, flush=True)
             out.update({"concurrent": k_null, "batched_s": tb, "batched_runs_per_hour": n_null / tb * 3600,
                         "ratio": tp / tb, "batched_Tbitops": null_ops / tb / 1e12,
