@@ -222,12 +222,15 @@ int lkc_bcc_kernel_stats(const lkc_bcc *D, const char *name, long long *launches
    bit-identical results; with tuning on, the first launch of each shape class (sizes to
    half an octave) times every form with HIP events on the handle's stream and keeps the
    fastest; off, a cost model picks.  mode 0 off, 1 measure new classes (the default, or
-   LKC_TUNE=0/1/2 in the environment), 2 use stored classes only.  The store is shared by
+   LKC_TUNE=0..3 in the environment), 2 use stored classes only, 3 measure and check that
+   every form wrote the same bits as the first (lkc_tune_verify: launches compared and
+   8-byte words that differed, 0 when the forms agree).  The store is shared by
    the process; lkc_tune_save / lkc_tune_load write and read it as text (return the number
    of classes).  lkc_tune_get: mode, stored classes, classes measured, and the summed
    device milliseconds the tuned forms saved over the cost model's at measurement. */
 int lkc_tune_set(int mode);
 int lkc_tune_get(int *mode, int *classes, long long *measured, double *saved_ms);
+int lkc_tune_verify(long long *compared, long long *mismatched);
 int lkc_tune_save(const char *path);
 int lkc_tune_load(const char *path);
 int lkc_tune_clear(void);

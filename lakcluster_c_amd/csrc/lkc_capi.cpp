@@ -293,7 +293,7 @@ int lkc_bcc_set_kernels(lkc_bcc* D, int which)
 
 int lkc_tune_set(int mode)
 {
-  if (mode < 0 || mode > 2) { g_last_error = "tuning mode must be 0, 1 or 2"; return LKC_ERR_STATE; }
+  if (mode < 0 || mode > 3) { g_last_error = "tuning mode must be 0, 1, 2 or 3"; return LKC_ERR_STATE; }
   lkc::tune_set_mode(mode);
   return LKC_OK;
 }
@@ -307,6 +307,15 @@ int lkc_tune_get(int* mode, int* classes, long long* measured, double* saved_ms)
   if (classes) *classes = lkc::tune_entries();
   if (measured) *measured = m;
   if (saved_ms) *saved_ms = s;
+  return LKC_OK;
+}
+
+int lkc_tune_verify(long long* compared, long long* mismatched)
+{
+  long long c = 0, m = 0;
+  lkc::tune_verify_stats(&c, &m);
+  if (compared) *compared = c;
+  if (mismatched) *mismatched = m;
   return LKC_OK;
 }
 

@@ -1153,7 +1153,7 @@ void Run::ww_contract(int slot, const WwTerm& w)
   a.nw = nw; a.wlist = pw.rows->d; a.B = c.B;
   a.cz = (const uint32_t*)g_czs[slot].p; a.ldc = ldc;
   a.Sc = pw.Cov->b(); a.Q = pw.X->b(); a.ajdk = ajdk; a.pcols = pcols; a.scal = scal_s + 16 * slot;
-  a.out = w.out; a.ldo = M;
+  a.out = w.out; a.ldo = M; a.out_elems = (long long)M * T;
   if (packed) {
     a.P = Bits{ w.pk.X, pn.X->nr, w.pk.nbits, w.pk.rb }; a.Tc = Bits{ w.pk.Cov, pn.Cov->nr, w.pk.nbits, w.pk.rb };
     a.pm = w.pk.mask; a.row0 = w.pk.mask; a.prb = (size_t)w.pk.nbits / 8;
@@ -1578,7 +1578,7 @@ void Run::gram_any(const GramMArgs& g)
     const TuneKey key{ kTuneGram, tune_bucket(g.na), tune_bucket(g.nz), tune_bucket(g.nseg), 0, 0 };
     int ch = -1;
     if (tune_get(key, &ch)) b = ch == 1 ? bany : 0;
-    else if (tm == 1) {
+    else if (tune_measuring(tm)) {
       struct Ctx { GramMArgs g; uint32_t* X; hipStream_t st; };
       Ctx cx{ g, (uint32_t*)g_split.need(dev, bany), st };
       const int cand[2] = { 0, 1 };
@@ -1586,7 +1586,8 @@ void Run::gram_any(const GramMArgs& g)
       const int bi = tune_measure(st, cand, 2, b ? 1 : 0, [](void* p, int v) {
         const Ctx* c = (const Ctx*)p;
         if (v) gram_split(c->g, c->X, c->st); else gram_mfma(c->g, c->st);
-      }, &cx, &best_ms, &model_ms);
+      }, &cx, &best_ms, &model_ms, g.out,
+         (!g.a_orig && !g.z_orig) ? sizeof(double) * (size_t)g.nz * (size_t)g.ldo : 0);
       tune_put(key, bi, best_ms, model_ms);
       return;   // every measured launch wrote the same Gram
     }

@@ -80,6 +80,7 @@ struct WwArgs {
   const uint32_t* cz; long long ldc;      // cz[j*ldc + n]
   Bits Sc; Bits Q; const double* ajdk; int pcols; const double* scal;
   double* out; long long ldo;
+  long long out_elems = 0;                // doubles of out a call may write (0: unknown)
 };
 // bit-plane VALU formulation (popcount of plane bit-rows)
 size_t ww_work_bytes(int nk, int nt, int nw);
@@ -215,7 +216,8 @@ hipError_t sort_pairs(const double* keys, double* keys_out, int* idx_in, int* id
 void popc_rows(const Bits& X, const int* rows, int nrows, const uint32_t* mask, int nwords, int* out, hipStream_t st);
 
 // launch-shape autotuner (lkc_tune.cpp): per shape class, the fastest of a kernel's
-// bit-identical forms, measured on first use.  mode 0 off, 1 measure new classes, 2 frozen.
+// bit-identical forms, measured on first use.  mode 0 off, 1 measure new classes, 2 frozen,
+// 3 measure and check that every form wrote the same bits as the first.
 struct TuneKey {
   int op, a, b, c, d, e;
   bool operator<(const TuneKey& o) const
@@ -228,13 +230,17 @@ struct TuneKey {
 enum { kTuneWw = 1, kTuneGram = 2 };
 int tune_mode();
 void tune_set_mode(int m);
+inline bool tune_measuring(int mode) { return mode == 1 || mode == 3; }
 int tune_bucket(long long x);            // round(2 log2 x): half-octave size classes
 bool tune_get(const TuneKey& k, int* choice);
 void tune_put(const TuneKey& k, int choice, float best_ms, float model_ms);
 // runs launch(ctx, cand[i]) for every candidate twice on st between HIP events, waits, and
-// returns the index of the fastest; model_idx is the cost model's candidate
+// returns the index of the fastest; model_idx is the cost model's candidate.  In mode 3 the
+// out_bytes at out (what every form writes) are compared after each launch with the first
+// form's, the mismatching 8-byte words counted (tune_verify_stats)
 int tune_measure(hipStream_t st, const int* cand, int n, int model_idx, void (*launch)(void*, int), void* ctx,
-                 float* best_ms, float* model_ms);
+                 float* best_ms, float* model_ms, const void* out = nullptr, size_t out_bytes = 0);
+void tune_verify_stats(long long* compared, long long* mismatched);
 int tune_entries();
 void tune_stats(long long* measured, double* saved_ms);
 void tune_clear();

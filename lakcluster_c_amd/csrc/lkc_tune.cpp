@@ -12,7 +12,8 @@
 // parameters (covariates, digit count) exactly.
 //
 // Modes: 0 off (the cost model), 1 on (measure new classes), 2 frozen (use what is stored,
-// the cost model elsewhere).  The initial mode comes from LKC_TUNE (0/1/2, default 1).
+// the cost model elsewhere), 3 as 1 and checking that every form wrote the same bits (the
+// property the choice rests on).  The initial mode comes from LKC_TUNE (0-3, default 1).
 // The store is one per process, shared by every handle and thread; it can be written to a
 // text file and read back, so a later process starts tuned.
 #include "lkc_kernels.h"
@@ -27,6 +28,15 @@
 
 namespace lkc {
 
+// counts the 8-byte words where a and b differ
+__global__ void k_tune_cmp(const unsigned long long* a, const unsigned long long* b, long long n, unsigned long long* bad)
+{
+  unsigned long long c = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    c += a[i] != b[i];
+  if (c) atomicAdd(bad, c);
+}
+
 namespace {
 struct Entry { int choice; float best_ms; float model_ms; };
 std::mutex g_mu;
@@ -34,13 +44,14 @@ std::map<TuneKey, Entry> g_store;
 int g_mode = -1;
 long long g_measured = 0;
 double g_saved_ms = 0.0;
+long long g_verify_compared = 0, g_verify_mismatched = 0;
 
 int mode_locked()
 {
   if (g_mode < 0) {
     const char* e = getenv("LKC_TUNE");
     g_mode = 1;
-    if (e && (e[0] == '0' || e[0] == '1' || e[0] == '2') && e[1] == 0) g_mode = e[0] - '0';
+    if (e && e[0] >= '0' && e[0] <= '3' && e[1] == 0) g_mode = e[0] - '0';
   }
   return g_mode;
 }
@@ -82,10 +93,22 @@ void tune_put(const TuneKey& k, int choice, float best_ms, float model_ms)
 }
 
 int tune_measure(hipStream_t st, const int* cand, int n, int model_idx, void (*launch)(void*, int), void* ctx,
-                 float* best_ms, float* model_ms)
+                 float* best_ms, float* model_ms, const void* out, size_t out_bytes)
 {
   std::vector<hipEvent_t> ev(4 * (size_t)n);
   for (auto& e : ev) if (hipEventCreate(&e) != hipSuccess) e = nullptr;
+  // mode 3: a copy of the first form's output, and a device count of differing words
+  const bool verify = tune_mode() == 3 && out && out_bytes >= 8 && out_bytes <= ((size_t)512 << 20);
+  void* snap = nullptr;
+  unsigned long long* bad = nullptr;
+  if (verify && (hipMallocAsync(&snap, out_bytes, st) != hipSuccess ||
+                 hipMallocAsync((void**)&bad, sizeof(unsigned long long), st) != hipSuccess ||
+                 hipMemsetAsync(bad, 0, sizeof(unsigned long long), st) != hipSuccess)) {
+    if (snap) (void)hipFreeAsync(snap, st);
+    if (bad) (void)hipFreeAsync(bad, st);
+    snap = nullptr; bad = nullptr;
+  }
+  const long long words = (long long)(out_bytes / 8);
   // two rounds: forward, then reverse order (the first form to run also warms the caches)
   for (int r = 0; r < 2; r++)
     for (int j = 0; j < n; j++) {
@@ -94,6 +117,14 @@ int tune_measure(hipStream_t st, const int* cand, int n, int model_idx, void (*l
       if (a) (void)hipEventRecord(a, st);
       launch(ctx, cand[i]);
       if (b) (void)hipEventRecord(b, st);
+      if (bad) {
+        if (r == 0 && j == 0) (void)hipMemcpyAsync(snap, out, out_bytes, hipMemcpyDeviceToDevice, st);
+        else {
+          const long long blocks = std::min<long long>((words + 255) / 256, 4096);
+          hipLaunchKernelGGL(k_tune_cmp, dim3((unsigned)blocks), dim3(256), 0, st, (const unsigned long long*)out,
+                             (const unsigned long long*)snap, words, bad);
+        }
+      }
     }
   int best = model_idx;
   float bms = 1e30f;
@@ -107,9 +138,26 @@ int tune_measure(hipStream_t st, const int* cand, int n, int model_idx, void (*l
     }
   for (int i = 0; i < n; i++) if (t[i] < bms) { bms = t[i]; best = i; }
   for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+  if (bad) {
+    unsigned long long h = 0;
+    const bool ok = hipMemcpyAsync(&h, bad, sizeof h, hipMemcpyDeviceToHost, st) == hipSuccess &&
+                    hipStreamSynchronize(st) == hipSuccess;
+    (void)hipFreeAsync(snap, st);
+    (void)hipFreeAsync(bad, st);
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_verify_compared += (long long)(2 * n - 1);
+    g_verify_mismatched += ok ? (long long)h : 1;
+  }
   *best_ms = bms;
   *model_ms = t[model_idx];
   return best;
+}
+
+void tune_verify_stats(long long* compared, long long* mismatched)
+{
+  std::lock_guard<std::mutex> lk(g_mu);
+  *compared = g_verify_compared;
+  *mismatched = g_verify_mismatched;
 }
 
 int tune_entries()
@@ -131,6 +179,8 @@ void tune_clear()
   g_store.clear();
   g_measured = 0;
   g_saved_ms = 0.0;
+  g_verify_compared = 0;
+  g_verify_mismatched = 0;
 }
 
 // text: one class per line, "lkc-tune op a b c d e choice best_ms model_ms"

@@ -616,7 +616,7 @@ hipError_t ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t s
     if (tune_get(key, &ch)) {
       const int c = ch >= kWwOneT ? ch - kWwOneT : ch;
       if ((ch < kWwOneT || a.nt > 1) && (c == 0 || c == 1 || (c > 1 && c <= max_parts))) nsplit = ch;
-    } else if (tmode == 1) {
+    } else if (tune_measuring(tmode)) {
       int cand[12], nc = 0, mi = -1;
       cand[nc++] = 0; cand[nc++] = 1;
       for (int q = 2; q <= max_parts; q *= 2) cand[nc++] = q;
@@ -624,7 +624,8 @@ hipError_t ww_mfma(const WwArgs& a, void* work, size_t work_bytes, hipStream_t s
       for (int i = 0; i < nc; i++) if (cand[i] == nsplit) mi = i;
       if (mi < 0) { mi = nc; cand[nc++] = nsplit; }
       float best_ms = 0, model_ms = 0;
-      const int b = tune_measure(st, cand, nc, mi, ww_launch, &L, &best_ms, &model_ms);
+      const int b = tune_measure(st, cand, nc, mi, ww_launch, &L, &best_ms, &model_ms, a.out,
+                                 (size_t)std::max(a.out_elems, 0LL) * sizeof(double));
       tune_put(key, cand[b], best_ms, model_ms);
       nsplit = cand[b];
     }

@@ -89,6 +89,7 @@ def lib():
         "lkc_tune_set": (C.c_int, [C.c_int]),
         "lkc_tune_get": (C.c_int, [ip, ip, C.POINTER(C.c_longlong), dp]),
         "lkc_tune_save": (C.c_int, [cp]),
+        "lkc_tune_verify": (C.c_int, [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
         "lkc_tune_load": (C.c_int, [cp]),
         "lkc_tune_clear": (C.c_int, []),
         "lkc_An_ajdk_v": (C.c_int, [u8p, C.c_size_t, C.c_int, ip, C.c_int, u8p, C.c_int, dp, dp]),
@@ -365,6 +366,14 @@ def tune_get() -> dict:
     meas, saved = C.c_longlong(), C.c_double()
     _check(lib().lkc_tune_get(C.byref(m), C.byref(n), C.byref(meas), C.byref(saved)))
     return {"mode": m.value, "classes": n.value, "measured": meas.value, "saved_ms": saved.value}
+
+
+def tune_verify() -> Tuple[int, int]:
+    """(launches compared, 8-byte words that differed) over the tuning measurements made in
+    mode 3, where every form's output is compared with the first form's."""
+    c, m = C.c_longlong(), C.c_longlong()
+    _check(lib().lkc_tune_verify(C.byref(c), C.byref(m)))
+    return c.value, m.value
 
 
 def tune_save(path: str) -> int:

@@ -1,6 +1,7 @@
 """The launch-shape autotuner: its store, modes and text file on the CPU; on the GPU, runs
-with the cost model, with tuning (every valid form of each new shape class measured) and
-with the frozen store give the same bits as the oracle."""
+with the cost model, with tuning (every valid form of each new shape class measured), with
+the frozen store, and with every measured form's output compared with the first form's,
+give the same bits as the oracle."""
 import numpy as np
 import pytest
 
@@ -25,11 +26,11 @@ def test_tune_store_round_trip(tmp_path):
         assert lkc.tune_save(str(g)) == 2
         lines = sorted(g.read_text().split("\n"))
         assert lines[1:] == ["lkc-tune 1 20 18 12 3 4 2 0.5 0.75", "lkc-tune 2 14 16 5 0 0 1 0.1 0.1"]
-        for m in (0, 1, 2):
+        for m in (0, 1, 2, 3):
             lkc.tune_set(m)
             assert lkc.tune_get()["mode"] == m
         with pytest.raises(lkc.LkcError):
-            lkc.tune_set(3)
+            lkc.tune_set(4)
         with pytest.raises(lkc.LkcError):
             lkc.tune_load(str(tmp_path / "missing.txt"))
         lkc.tune_clear()
@@ -52,8 +53,8 @@ def test_tuned_forms_match_oracle(kw, oracle, tmp_path):
     before = lkc.tune_get()["mode"]
     runs = {}
     try:
-        for label, mode in (("model", 0), ("measure", 1), ("frozen", 2)):
-            if mode == 1:
+        for label, mode in (("model", 0), ("measure", 1), ("frozen", 2), ("verify", 3)):
+            if mode in (1, 3):
                 lkc.tune_clear()
             lkc.tune_set(mode)
             with lkc.Bcc(text) as run:
@@ -67,6 +68,12 @@ def test_tuned_forms_match_oracle(kw, oracle, tmp_path):
                 n = lkc.tune_save(f)
                 lkc.tune_clear()
                 assert lkc.tune_load(f) == n
+            if mode == 3:
+                # every measured form wrote the same bits as the first form of its launch
+                compared, mismatched = lkc.tune_verify()
+                print(f"  verify: {compared} launches compared, {mismatched} words differed", flush=True)
+                assert compared > 0
+                assert mismatched == 0
     finally:
         lkc.tune_set(before)
     for label, (tr, (xa, xb)) in runs.items():
