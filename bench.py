@@ -434,10 +434,13 @@ def main():
     fallbacks = 0
     qr_l, qr_ms, qr_work = 0, 0.0, 0.0
     n_iter = 0
+    step_ns = []       # the timed steps on the monotonic clock (profiler timestamps use it too)
     for k in range(args.steps):
+        m0 = time.monotonic_ns()
         t0 = time.perf_counter()
         run.run(args.max_iter)
         t1 = time.perf_counter()
+        step_ns.append([m0, time.monotonic_ns()])
         elim.append(t1 - t0)
         ops.append(run.ops)
         n_iter = run.n_iter
@@ -485,6 +488,7 @@ def main():
             "avg_launch_ms": kms / kl if kl else None,
         },
     }
+    line["step_ns"] = step_ns
     # the QR row sums (AnZt_S_WnYt_vv) stream the loop-subscores: HBM-bound, 8 bytes per
     # (row, column) pair read, HIP events around each launch on the engine's stream
     qach = qr_work / (qr_ms / 1e3) / 1e9 if qr_ms > 0 else None
