@@ -47,8 +47,20 @@ int lkc_lakcluster_driver(const char *in_text, const char *out_dir);
 /* dexcluster_driver (dexcluster_driver.c:5-57, GLOBAL_TEST_TYPE= dexcluster_driver at
    lakcluster_ver18.c:345): the dcc half-loop elimination, recomputing its subscores
    (dcc_An_ajdk, dcc_lf_ZtSn, dcc_lf_D_AtTn_ZtSn_vv, dcc_lf_TAnZtS_ww,
-   dcc_wrap_dcc_halfloop) after each drop; the same three output files. */
+   dcc_wrap_dcc_halfloop) after each drop; the same three output files.  With
+   GLOBAL_NBINS > 1 or Y/W files it runs lkc_dcc_driver_bins. */
 int lkc_dexcluster_driver(const char *in_text, const char *out_dir);
+
+/* dexcluster_driver with several bins and Y/W genetic controls: dcc_load (dcc.c:1157-1215)
+   reads bin n's files from item n of the comma-separated GLOBAL_A_n_name_, A_t_name_,
+   A_n_rind_, Z_n_name_, Z_t_name_, Z_n_rind_, Y_n_name_, Y_t_name_, W_n_name_, W_t_name_,
+   T_n_name_, T_t_name_, S_n_name_, S_t_name_ lists (updateglobals.c:76-117) with the
+   shared GLOBAL_A_n_cind / Y_n_cind / T_n_cind masks; A_p and Y_p come from the data
+   (dcc_load_A_p, dcc.c:875-991).  The subscores of every bin pair are recomputed after
+   each drop (dexcluster_driver.c:22-53); the same three output files, row indices over
+   the bins in bin order.  Every bin names the same kinds of file.  Any number of bins
+   from 1 to 64 (1 bin without Y gives what lkc_dexcluster_driver gives). */
+int lkc_dcc_driver_bins(const char *in_text, const char *out_dir);
 
 /* dcc_sumscores_test (dcc_sumscores.c:700-747; GLOBAL_TEST_TYPE= dcc_sumscores at
    lakcluster_ver18.c:343): the synthetic problem of dcc_init (dcc.c:1217-1275) drawn from

@@ -214,10 +214,30 @@ int lkc_scorebox_xdrop_array(int nxols, int x_max, int x_num, int* xval, int* xd
   return LKC_OK;
 }
 
+int lkc_dcc_driver_bins(const char* in_text, const char* out_dir)
+{
+  return guard(nullptr, [&] {
+    lkc::Cfg c;
+    lkc::parse_cfg(in_text, c);
+    lkc::DccTestOut out;
+    lkc::dcc_driver_bins(c, out, out_dir && out_dir[0] ? std::string(out_dir) : lkc::out_dir_of(c));
+  });
+}
+
 int lkc_dexcluster_driver(const char* in_text, const char* out_dir)
 {
+  // several bins or Y/W genetic controls: the multi-bin elimination; else the one-bin engine
+  bool bins = false;
+  int rc = guard(nullptr, [&] {
+    lkc::Cfg c;
+    lkc::parse_cfg(in_text, c);
+    bins = c.nbins > 1 || c.Y_ncols > 0;
+    for (const std::string& f : c.Y_t_) bins = bins || !f.empty();
+  });
+  if (rc) return rc;
+  if (bins) return lkc_dcc_driver_bins(in_text, out_dir);
   lkc_bcc* D = nullptr;
-  int rc = lkc_bcc_load(in_text, &D);
+  rc = lkc_bcc_load(in_text, &D);
   if (rc) return rc;
   D->run.dcc = 1;
   rc = lkc_bcc_run(D, -1);

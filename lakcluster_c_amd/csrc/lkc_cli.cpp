@@ -1,8 +1,10 @@
 // lkc_cli.cpp -- the .in-driven command line: reads the key/value text of an
 // .in file on stdin (as lakcluster_ver18 does, lakcluster_ver18.c:337-338) and runs
 // the elimination on device 0, writing out_trace.txt / out_xdrop_{a,b}.txt, or with
-// GLOBAL_TEST_TYPE= lakcluster_scorebox the scorebox replay and its .mda files, or with
-// GLOBAL_TEST_TYPE= dcc_sumscores the reference's dcc self-test on its synthetic problem.
+// GLOBAL_TEST_TYPE= lakcluster_scorebox the scorebox replay and its .mda files, with
+// GLOBAL_TEST_TYPE= dexcluster_driver the dcc elimination (several bins and Y/W controls
+// included), or with GLOBAL_TEST_TYPE= dcc_sumscores the reference's dcc self-test on its
+// synthetic problem.
 #include "../../include/lkc.h"
 #include <stdio.h>
 #include <string>
@@ -21,6 +23,11 @@ int main(int argc, char** argv)
     std::string v = e != std::string::npos ? text.substr(e + 1, s == std::string::npos ? std::string::npos : s - e - 1) : "";
     v.erase(0, v.find_first_not_of(" \t\r\n"));
     v.erase(v.find_last_not_of(" \t\r\n") + 1);
+    if (v == "dexcluster_driver") {
+      if (lkc_dexcluster_driver(text.c_str(), out_dir)) { fprintf(stderr, "lkc: %s\n", lkc_last_error()); return 1; }
+      printf("dexcluster_driver done\n");
+      return 0;
+    }
     if (v == "dcc_sumscores") {
       if (lkc_dcc_sumscores_test(text.c_str(), out_dir)) { fprintf(stderr, "lkc: %s\n", lkc_last_error()); return 1; }
       printf("dcc_sumscores done\n");

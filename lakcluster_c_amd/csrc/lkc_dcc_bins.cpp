@@ -1,6 +1,7 @@
-// lkc_dcc_bins.cpp -- dcc_sumscores_test (dcc_sumscores.c:700-747) on the device: the
-// synthetic problem of dcc_init (dcc.c:1217-1275) with several bins and Y/W genetic
-// controls, the dcc half-loop subscores of every bin pair (dcc_An_ajdk, dcc_lf_ZtSn,
+// lkc_dcc_bins.cpp -- the dcc elimination with several bins and Y/W genetic controls on the
+// device: dcc_sumscores_test (dcc_sumscores.c:700-747) on the synthetic problem of dcc_init
+// (dcc.c:1217-1275), and dexcluster_driver (dexcluster_driver.c:5-57) on the problem that
+// dcc_load (dcc.c:1157-1215) reads from the .in's per-bin files.  Both run the dcc half-loop subscores of every bin pair (dcc_An_ajdk, dcc_lf_ZtSn,
 // dcc_lf_D_AtTn_ZtSn_vv, dcc_lf_TAnZtS_ww, dcc_wrap_dcc_halfloop; dcc_lf.c, TAnZtS.c,
 // D_AtTn_ZtSn.c), the score stages that sum the bins (dcc_sumscores.c:5-699) and the three
 // output files.
@@ -115,8 +116,13 @@ struct DccDev {
                  case YY: return NY > 0 && e2.nA > 0; default: return M > 0 && e2.nA > 0; }
   }
 
+  bool files = false;   // the problem came from the .in's files: A_p / Y_p from the data
+
   void generate();
+  void load();
+  void finish_host();
   void setup();
+  void data_freqs();
   void upload_list(IList& l, const std::vector<uint8_t>& m, int n);
   void mxset();
   void subscores();
@@ -169,14 +175,96 @@ void DccDev::generate()
   if (c.sparse == 0) std::fill(hA_p.begin(), hA_p.end(), 0.5);
   for (int i = 0; i < Ypc; i++) hY_p[i] = crop_draw(g);
   if (c.sparse == 0) std::fill(hY_p.begin(), hY_p.end(), 0.5);
-  // calc_A_ajdk (An_ajdk_v.c:5-30), libm pow on the host
-  auto ajdk_of = [](const std::vector<double>& p, std::vector<double>& out) {
-    const int pc = (int)p.size(); out.assign((size_t)kAjdk * std::max(pc, 1), 0.0);
-    for (int j = 0; j < 5; j++) for (int k = 0; k < 3; k++) for (int n = 0; n < pc; n++)
-      out[(size_t)(j + 5 * k) * pc + n] = pow(p[n] - (1 - p[n]), j) * pow(1.0 / std::max(0.01, (4.0 * p[n] * (1 - p[n]))), k);
+  finish_host();
+}
+
+// calc_A_ajdk (An_ajdk_v.c:5-30), libm pow on the host
+static void ajdk_of(const std::vector<double>& p, std::vector<double>& out)
+{
+  const int pc = (int)p.size(); out.assign((size_t)kAjdk * std::max(pc, 1), 0.0);
+  for (int j = 0; j < 5; j++) for (int k = 0; k < 3; k++) for (int n = 0; n < pc; n++)
+    out[(size_t)(j + 5 * k) * pc + n] = pow(p[n] - (1 - p[n]), j) * pow(1.0 / std::max(0.01, (4.0 * p[n] * (1 - p[n]))), k);
+}
+
+// dcc_load (dcc.c:1157-1215): dcc_ajdk_load's column masks (dcc.c:54-115), per bin
+// dcc_single_load_M_An (dcc.c:251-419): A from A_t (patients) and A_n (SNPs), Z, Y, W, T
+// and S likewise, each pair optional as there (T and S default to the intercept column),
+// the row masks from the A_n_rind_ / Z_n_rind_ lists.  Every bin names the same kinds of
+// file.  The covariates in use are the T_n_cind mask, else the first GLOBAL_T_n_cols of
+// them (all when 0), as in the one-bin loader.  A_p and Y_p follow from the data once it
+// is on the device (data_freqs).
+void DccDev::load()
+{
+  files = true;
+  nbins = c.nbins;
+  auto at = [](const std::vector<std::string>& v, int nb) { return nb < (int)v.size() ? v[nb] : std::string(); };
+  for (int nb = 0; nb < nbins; nb++) {
+    if (at(c.A_t_, nb).empty() || at(c.A_n_, nb).empty())
+      throw Error(LKC_ERR_CONFIG, "bin " + std::to_string(nb) + ": GLOBAL_A_n_name_ and GLOBAL_A_t_name_ must name its files");
+    if (at(c.Y_t_, nb).empty() != at(c.Y_t_, 0).empty() || at(c.T_t_, nb).empty() != at(c.T_t_, 0).empty() ||
+        at(c.Z_t_, nb).empty() != at(c.Z_n_, nb).empty())
+      throw Error(LKC_ERR_CONFIG, "bin " + std::to_string(nb) + " names a different set of files");
+  }
+  int fnr = 0, fnc = 0;
+  b16_size(c.A_t_[0], &fnr, &fnc); M = fnr;
+  NY = 0; if (!at(c.Y_t_, 0).empty()) { b16_size(c.Y_t_[0], &fnr, &fnc); NY = fnr; }
+  T = 1; if (!at(c.T_t_, 0).empty()) { b16_size(c.T_t_[0], &fnr, &fnc); T = fnr; }
+  if ((c.A_ncols && c.A_ncols != M) || (c.Y_ncols && c.Y_ncols != NY))
+    throw Error(LKC_ERR_SHAPE, "GLOBAL_A_n_cols / GLOBAL_Y_n_cols do not match the files");
+  if (T < 1 || T > kMaxT) throw Error(LKC_ERR_CONFIG, "between 1 and 8 covariate columns are supported");
+  auto all = [](int n) { std::vector<uint8_t> m(bsize(n) + 8, 0); for (int i = 0; i < n; i++) bs(m.data(), i); return m; };
+  auto mask = [&](const std::string& f, int n) {
+    if (f.empty()) return all(n);
+    int r = 0, k = 0; b16_size(f, &r, &k);
+    if (r != n || k != 1) throw Error(LKC_ERR_SHAPE, "mask " + f + " has " + std::to_string(r) + " entries, expected " + std::to_string(n));
+    return read_mask(f, n);
   };
+  hAc_b = mask(c.A_cind, M); hYc_b = mask(c.Y_cind, NY);
+  if (!c.T_cind.empty()) hTc_b = mask(c.T_cind, T);
+  else { hTc_b.assign(bsize(T) + 8, 0); const int nt = c.T_ncols > 0 ? std::min(c.T_ncols, T) : T; for (int t = 0; t < nt; t++) bs(hTc_b.data(), t); }
+  hAc_j = hAc_b; hYc_j = hYc_b; hTc_j = hTc_b;
+  auto take = [](HBits& d, HostBits&& h) { d.nr = h.nr; d.nc = h.nc; d.rb = h.rb; d.b.swap(h.b); };
+  // X (rows x cols) from its patient-major file pt and SNP-major file pn
+  auto pair = [&](const std::string& pt, const std::string& pn, int rows, int cols, HBits& Xn, HBits& Xt) {
+    take(Xn, read_b16(pt)); take(Xt, read_b16(pn));
+    if (Xn.nr != rows || Xn.nc != cols || Xt.nr != cols || Xt.nc != rows)
+      throw Error(LKC_ERR_SHAPE, pt + " / " + pn + " do not hold a " + std::to_string(rows) + " x " + std::to_string(cols) + " matrix");
+  };
+  auto intercept = [](int rows, HBits& Xt) { Xt.init(1, rows); for (int r = 0; r < rows; r++) bs(Xt.b.data(), r); };
+  E.assign(nbins, Bin());
+  for (int nb = 0; nb < nbins; nb++) {
+    Bin& e = E[nb];
+    HBits Tn, Sn;
+    b16_size(c.A_t_[nb], &fnr, &fnc); e.nA = fnc;
+    pair(c.A_t_[nb], c.A_n_[nb], e.nA, M, e.hAn, e.hAt);
+    e.nZ = 0;
+    if (!at(c.Z_t_, nb).empty()) { b16_size(c.Z_t_[nb], &fnr, &fnc); e.nZ = fnc; }
+    if (e.nZ > 0) pair(c.Z_t_[nb], c.Z_n_[nb], e.nZ, M, e.hZn, e.hZt);
+    else { e.hZn.init(0, M); e.hZt.init(M, 0); }
+    if (NY > 0) pair(c.Y_t_[nb], at(c.Y_n_, nb), e.nA, NY, e.hYn, e.hYt);
+    else { e.hYn.init(e.nA, 0); e.hYt.init(0, e.nA); }
+    if (NY > 0 && e.nZ > 0) {
+      if (at(c.W_t_, nb).empty() || at(c.W_n_, nb).empty()) throw Error(LKC_ERR_CONFIG, "bin " + std::to_string(nb) + ": Y without W");
+      pair(c.W_t_[nb], c.W_n_[nb], e.nZ, NY, e.hWn, e.hWt);
+    } else { e.hWn.init(e.nZ, NY); e.hWt.init(NY, e.nZ); }
+    if (!at(c.T_t_, nb).empty()) pair(c.T_t_[nb], at(c.T_n_, nb), e.nA, T, Tn, e.hTt);
+    else intercept(e.nA, e.hTt);
+    if (e.nZ > 0 && !at(c.S_t_, nb).empty()) pair(c.S_t_[nb], at(c.S_n_, nb), e.nZ, T, Sn, e.hSt);
+    else if (T == 1 || e.nZ == 0) intercept(e.nZ, e.hSt);
+    else throw Error(LKC_ERR_CONFIG, "bin " + std::to_string(nb) + ": T without S");
+    e.hAr_b = mask(at(c.A_rind_, nb), e.nA); e.hAr_j = e.hAr_b;
+    e.hZr_b = mask(e.nZ > 0 ? at(c.Z_rind_, nb) : std::string(), e.nZ); e.hZr_j = e.hZr_b;
+  }
+  Apc = psize(M); Ypc = psize(NY);
+  hA_p.assign(Apc, 0.5); hY_p.assign(Ypc, 0.5);
+  finish_host();
+}
+
+// dcc_X_nrows_total and dcc_init_QX (dcc.c:1115-1131, 1047-1090): row indices over the
+// bins, the totals, the covariates' compacted indices
+void DccDev::finish_host()
+{
   ajdk_of(hA_p, hA_ajdk); ajdk_of(hY_p, hY_ajdk);
-  // dcc_X_nrows_total and dcc_init_QX: row indices over the bins
   int mx_a = 0, mx_b = 0;
   for (Bin& e : E) {
     e.g_a.assign(e.nA, 0); e.g_b.assign(e.nA, 0);
@@ -247,6 +335,49 @@ void DccDev::setup()
   L = dbl((size_t)T * std::max(M, NY)); xwork = dbl(768);
   dsv = dbl((size_t)M + A_nrows_total);
   cnt = dev.alloc<int>(std::max(std::max(M, NY), T) + 1);
+  if (files) data_freqs();
+  HIPCHK(hipStreamSynchronize(st));
+}
+
+// dcc_load_A_p (dcc.c:875-991): per column the counts of the loaded (mr_b) rows summed over
+// the bins, A with Z and Y with W; per 1920-column block the mean count over the block's
+// loaded columns divided by the loaded rows, cropped to [0.01, 0.99] (0.5 everywhere when
+// GLOBAL_TEST_sparse is 0); then the ajdk tables and the Y columns' weights again
+void DccDev::data_freqs()
+{
+  std::vector<double> AZ(M + 1, 0.0), YW(NY + 1, 0.0);
+  std::vector<int> h(std::max(M, NY) + 1, 0);
+  auto count = [&](const DMat& Xt, int rows, const uint32_t* mask, const std::vector<uint8_t>& cm, std::vector<double>& acc) {
+    if (rows <= 0 || Xt.nr <= 0) return;
+    popc_rows(Xt.b(), nullptr, Xt.nr, mask, (int)(Xt.rb / 4), cnt, st);
+    HIPCHK(hipMemcpyAsync(h.data(), cnt, sizeof(int) * Xt.nr, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int k = 0; k < Xt.nr; k++) if (bg(cm.data(), k)) acc[k] += (double)h[k];
+  };
+  for (Bin& e : E) {
+    // the row masks start as loaded (mr_j = mr_b), so the working masks hold mr_b here
+    HIPCHK(hipMemcpyAsync(e.Ar_j, e.hAr_b.data(), bsize(e.nA), hipMemcpyHostToDevice, st));
+    count(e.At, e.nA, e.Ar_j, hAc_b, AZ);
+    count(e.Zt, e.nZ, e.Zr_j, hAc_b, AZ);
+    count(e.Yt, e.nA, e.Ar_j, hYc_b, YW);
+    count(e.Wt, e.nZ, e.Zr_j, hYc_b, YW);
+  }
+  auto freq = [&](const std::vector<double>& rs, const std::vector<uint8_t>& cm, int n, std::vector<double>& p) {
+    for (int b = 0; b < (int)p.size(); b++) {
+      p[b] = 0.5;
+      if (!c.sparse) continue;
+      const int s0 = std::min(n - 1, std::max(0, b * kPopBits)), s1 = std::min(n - 1, std::max(0, (b + 1) * kPopBits - 1));
+      double tsum = 0, tnum = 0;
+      for (int k = s0; k <= s1; k++) if (bg(cm.data(), k)) { tsum += rs[k]; tnum += 1; }
+      if (tnum == 0) p[b] = 0.5;
+      else { const double v = (tsum) / (tnum * (A_rpop_b_total + Z_rpop_b_total)); p[b] = v < 0.01 ? 0.01 : (v > 0.99 ? 0.99 : v); }
+    }
+  };
+  freq(AZ, hAc_b, M, hA_p); freq(YW, hYc_b, NY, hY_p);
+  ajdk_of(hA_p, hA_ajdk); ajdk_of(hY_p, hY_ajdk);
+  HIPCHK(hipMemcpyAsync(A_ajdk, hA_ajdk.data(), sizeof(double) * hA_ajdk.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(Y_ajdk, hY_ajdk.data(), sizeof(double) * hY_ajdk.size(), hipMemcpyHostToDevice, st));
+  mask_lf(Yc_j, Ypc, Y_ajdk, mlfY, st);
   HIPCHK(hipStreamSynchronize(st));
 }
 
@@ -525,7 +656,41 @@ void DccDev::write(const std::string& dir) const
   fclose(fa); fclose(fb);
 }
 
+// the dexcluster_driver loop (dexcluster_driver.c:22-53) from the first subscores to the end
+void run_loop(DccDev& D, int max_iter)
+{
+  D.mxset();
+  D.subscores();
+  int nl = 0;
+  while ((max_iter <= 0 || nl < max_iter) && (D.A_rpop_j_total > 0 || D.A_cpop_j > 0)) {
+    D.nrm(); D.ifT(); /* dcc_sumscores_srt: nothing with GLOBAL_Ireq <= 0 */ D.cmb();
+    D.xij(); D.mxA();
+    D.mxdup(); D.mxset();
+    D.subscores();
+    nl++;
+  }
+  HIPCHK(hipStreamSynchronize(D.st));
+  HIPCHK(hipGetLastError());
+}
+
 }  // namespace
+
+void dcc_driver_bins(const Cfg& c, DccTestOut& out, const std::string& out_dir)
+{
+  if (c.nbins < 1 || c.nbins > 64) throw Error(LKC_ERR_CONFIG, "GLOBAL_NBINS must be in [1,64]");
+  if (c.Ireq > 0) throw Error(LKC_ERR_CONFIG, "GLOBAL_Ireq > 0 (per-bin sorting) is not supported");
+  if (c.scramble_num > 0) throw Error(LKC_ERR_CONFIG, "bcc_scramble is not part of dexcluster_driver");
+  if (!c.A_p.empty() || !c.Y_p.empty()) throw Error(LKC_ERR_CONFIG, "GLOBAL_A_p_name / GLOBAL_Y_p_name are not supported with several bins");
+  if (c.B < 1 || c.B > 32) throw Error(LKC_ERR_CONFIG, "GLOBAL_B_MLT must be in [1,32]");
+  DccDev D;
+  D.c = c; D.out = &out; D.B = c.B;
+  out = DccTestOut();
+  D.load();
+  D.setup();
+  run_loop(D, 0);
+  out.ops = D.ops;
+  if (!out_dir.empty()) D.write(out_dir);
+}
 
 void dcc_sumscores_test(const Cfg& c, DccTestOut& out, const std::string& out_dir)
 {
@@ -541,18 +706,7 @@ void dcc_sumscores_test(const Cfg& c, DccTestOut& out, const std::string& out_di
   out = DccTestOut();
   D.generate();
   D.setup();
-  D.mxset();
-  D.subscores();
-  int nl = 0;
-  while ((c.test_niter <= 0 || nl < c.test_niter) && (D.A_rpop_j_total > 0 || D.A_cpop_j > 0)) {
-    D.nrm(); D.ifT(); /* dcc_sumscores_srt: nothing with GLOBAL_Ireq <= 0 */ D.cmb();
-    D.xij(); D.mxA();
-    D.mxdup(); D.mxset();
-    D.subscores();
-    nl++;
-  }
-  HIPCHK(hipStreamSynchronize(D.st));
-  HIPCHK(hipGetLastError());
+  run_loop(D, c.test_niter);
   out.ops = D.ops;
   if (!out_dir.empty()) D.write(out_dir);
 }

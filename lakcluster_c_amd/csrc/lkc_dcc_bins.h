@@ -1,5 +1,6 @@
 // lkc_dcc_bins.h -- launchers of the multi-bin dcc stage kernels (lkc_dcc_pairs.hip) and the
-// host side of dcc_sumscores_test on the device (lkc_dcc_bins.cpp).
+// host side of the multi-bin dcc eliminations on the device (lkc_dcc_bins.cpp):
+// dcc_sumscores_test on its synthetic problem and dexcluster_driver on files.
 #pragma once
 #include "lkc_kernels.h"
 #include <string>
@@ -41,5 +42,8 @@ void bins_cmb_rows(const int* list, int n, int nblk, long long blk, double* x, c
 struct DccTestOut { std::vector<double> trace; std::vector<int> xdrop_a, xdrop_b; int n_iter = 0; long long ops = 0; };
 struct Cfg;
 void dcc_sumscores_test(const Cfg& c, DccTestOut& out, const std::string& out_dir);
+// dexcluster_driver (dexcluster_driver.c:5-57) on the .in's per-bin files with Y/W controls,
+// run to the end; the three output files into out_dir when it is not empty
+void dcc_driver_bins(const Cfg& c, DccTestOut& out, const std::string& out_dir);
 
 }  // namespace lkc

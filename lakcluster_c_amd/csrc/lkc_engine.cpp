@@ -50,16 +50,15 @@ static std::string trim(const std::string& s)
 }
 
 // Keys that updateglobals reads (updateglobals.c:5-186) but that the bcc / dcc elimination,
-// the scorebox, bcc_scramble and dcc_sumscores_test never consult: thread and print settings, the row counts the loader takes from the files, the Y/W
-// file names (GLOBAL_Y_n_cols > 0 is what turns those on, and is rejected), the dcg / PCA /
+// the scorebox, bcc_scramble and dcc_sumscores_test never consult: thread and print
+// settings, the row counts the loader takes from the files, the dcg / PCA /
 // PLINK-conversion settings.  Any other key is an error, as the reference's exit() on an
 // unknown name (updateglobals.c:185).
 static bool reference_key_without_effect(const std::string& n)
 {
   static const char* keys[] = {
     "GLOBAL_verbose", "GLOBAL_thread_count", "GLOBAL_omp_type", "GLOBAL_LBITS", "GLOBAL_A_n_rows_", "GLOBAL_Z_n_rows_",
-    "GLOBAL_Y_n_name_", "GLOBAL_Y_t_name_", "GLOBAL_Y_n_cind", "GLOBAL_Y_p_name", "GLOBAL_W_n_name_",
-    "GLOBAL_W_t_name_", "GLOBAL_J_n_rind_", "GLOBAL_K_n_cind", "GLOBAL_pca_out_xdrop", "GLOBAL_pca_V_",
+    "GLOBAL_J_n_rind_", "GLOBAL_K_n_cind", "GLOBAL_pca_out_xdrop", "GLOBAL_pca_V_",
     "GLOBAL_pca_V_pre_", "GLOBAL_pca_infix", "GLOBAL_pca_iteration_num", "GLOBAL_pca_iteration_max",
     "GLOBAL_pca_iteration_min", "GLOBAL_pca_rank", "GLOBAL_pca_tolerance", "GLOBAL_fname_bed_0in",
     "GLOBAL_fname_b16_out", "GLOBAL_fname_bim_0in", "GLOBAL_fname_bim_out", "GLOBAL_fname_fam_0in",
@@ -86,6 +85,17 @@ void parse_cfg(const char* text, Cfg& c)
     if (name == "END") break;
     std::string first = trim(val.substr(0, val.find(',')));   // list-valued keys: the first bin
     auto path = [&](const std::string& v) { return c.xpre.empty() ? v : c.xpre + "/" + v; };
+    auto list = [&](std::vector<std::string>& out) {
+      out.clear();
+      size_t q = 0;
+      while (q <= val.size()) {
+        size_t k = val.find(',', q); if (k == std::string::npos) k = val.size();
+        const std::string item = trim(val.substr(q, k - q));
+        out.push_back(item.empty() ? item : path(item));
+        q = k + 1;
+      }
+      return out.empty() ? std::string() : out[0];
+    };
     if (name == "GLOBAL_B_MLT") { c.B = atoi(val.c_str()); c.D_MLT = (long long)pow(2.0, (double)c.B); }
     else if (name == "GLOBAL_D_MLT") { c.D_MLT = atoll(val.c_str()); c.B = (int)llround(log2((double)c.D_MLT)); }
     else if (name == "GLOBAL_gamma") c.gamma = atof(val.c_str());
@@ -103,19 +113,25 @@ void parse_cfg(const char* text, Cfg& c)
     else if (name == "GLOBAL_DIR_NAME") c.dir_name = val;
     else if (name == "GLOBAL_DIR_BASE") c.dir_base = val;
     else if (name == "GLOBAL_out_name") c.out_name = val;
-    else if (name == "GLOBAL_A_n_name_") c.A_n = path(first);
-    else if (name == "GLOBAL_A_t_name_") c.A_t = path(first);
-    else if (name == "GLOBAL_A_n_rind_") c.A_rind = path(first);
+    else if (name == "GLOBAL_A_n_name_") c.A_n = list(c.A_n_);
+    else if (name == "GLOBAL_A_t_name_") c.A_t = list(c.A_t_);
+    else if (name == "GLOBAL_A_n_rind_") c.A_rind = list(c.A_rind_);
     else if (name == "GLOBAL_A_n_cind") c.A_cind = path(val);
     else if (name == "GLOBAL_A_p_name") c.A_p = path(val);
-    else if (name == "GLOBAL_Z_n_name_") c.Z_n = path(first);
-    else if (name == "GLOBAL_Z_t_name_") c.Z_t = path(first);
-    else if (name == "GLOBAL_Z_n_rind_") c.Z_rind = path(first);
-    else if (name == "GLOBAL_T_n_name_") c.T_n = path(first);
-    else if (name == "GLOBAL_T_t_name_") c.T_t = path(first);
+    else if (name == "GLOBAL_Z_n_name_") c.Z_n = list(c.Z_n_);
+    else if (name == "GLOBAL_Z_t_name_") c.Z_t = list(c.Z_t_);
+    else if (name == "GLOBAL_Z_n_rind_") c.Z_rind = list(c.Z_rind_);
+    else if (name == "GLOBAL_T_n_name_") c.T_n = list(c.T_n_);
+    else if (name == "GLOBAL_T_t_name_") c.T_t = list(c.T_t_);
     else if (name == "GLOBAL_T_n_cind") c.T_cind = path(val);
-    else if (name == "GLOBAL_S_n_name_") c.S_n = path(first);
-    else if (name == "GLOBAL_S_t_name_") c.S_t = path(first);
+    else if (name == "GLOBAL_S_n_name_") c.S_n = list(c.S_n_);
+    else if (name == "GLOBAL_S_t_name_") c.S_t = list(c.S_t_);
+    else if (name == "GLOBAL_Y_n_name_") list(c.Y_n_);
+    else if (name == "GLOBAL_Y_t_name_") list(c.Y_t_);
+    else if (name == "GLOBAL_W_n_name_") list(c.W_n_);
+    else if (name == "GLOBAL_W_t_name_") list(c.W_t_);
+    else if (name == "GLOBAL_Y_n_cind") c.Y_cind = path(val);
+    else if (name == "GLOBAL_Y_p_name") c.Y_p = path(val);
     else if (name == "GLOBAL_scramble_num") c.scramble_num = atoi(val.c_str());
     else if (name == "GLOBAL_scramble_out_xdrop_") c.scramble_files_raw = val;
     else if (name == "GLOBAL_scramble_rseed_") c.scramble_seeds_raw = val;
@@ -164,9 +180,7 @@ void parse_cfg(const char* text, Cfg& c)
 // .b16 input (binary_read.c:5-55; M_load, M_init.c:229-242): a file holding an
 // (fnr x fnc) matrix column by column yields fnc rows of fnr bits.
 // ------------------------------------------------------------------
-struct HostBits { std::vector<uint8_t> b; int nr = 0, nc = 0; size_t rb = 0; };
-
-static HostBits read_b16(const std::string& path)
+HostBits read_b16(const std::string& path)
 {
   FILE* fp = fopen(path.c_str(), "rb");
   if (!fp) throw Error(LKC_ERR_IO, "cannot open " + path);
@@ -187,7 +201,18 @@ static HostBits read_b16(const std::string& path)
   return h;
 }
 
-static std::vector<uint8_t> read_mask(const std::string& path, int n)
+void b16_size(const std::string& path, int* fnr, int* fnc)
+{
+  FILE* fp = fopen(path.c_str(), "rb");
+  if (!fp) throw Error(LKC_ERR_IO, "cannot open " + path);
+  int hdr[3];
+  const bool ok = fread(hdr, sizeof(int), 3, fp) == 3;
+  fclose(fp);
+  if (!ok || hdr[1] < 0 || hdr[2] < 0) throw Error(LKC_ERR_IO, "bad header in " + path);
+  *fnr = hdr[1]; *fnc = hdr[2];
+}
+
+std::vector<uint8_t> read_mask(const std::string& path, int n)
 {
   HostBits m = read_b16(path);
   std::vector<uint8_t> out(bsize(n) + 8, 0);
@@ -469,7 +494,9 @@ void Run::load(const char* in_text)
   double t0 = now_s();
   parse_cfg(in_text, c);
   if (c.nbins != 1) throw Error(LKC_ERR_CONFIG, "only GLOBAL_NBINS= 1 is supported");
-  if (c.Y_ncols != 0) throw Error(LKC_ERR_CONFIG, "Y/W genetic controls are not supported");
+  bool y_files = false;
+  for (const std::string& f : c.Y_t_) y_files = y_files || !f.empty();
+  if (c.Y_ncols != 0 || y_files) throw Error(LKC_ERR_CONFIG, "Y/W genetic controls are not supported here (the file-driven dexcluster_driver takes them)");
   // GLOBAL_TEST_TYPE selects the elimination (lakcluster_ver18.c:337-345); the dcc one has
   // no low-rank update, so no QR/QC strategy
   dcc = c.test_type == "dexcluster_driver" ? 1 : 0;

@@ -19,6 +19,11 @@ struct Cfg {
   int sparse = 1; int nbins = 0; int Ireq = 0; int A_ncols = 0, Y_ncols = 0, T_ncols = 0; int scramble_num = 0;
   std::string QR, QC, A_n, A_t, A_rind, A_cind, A_p, Z_n, Z_t, Z_rind, T_n, T_t, T_cind, S_n, S_t;
   std::string xpre, dir_name, dir_base, out_name;
+  // dexcluster_driver with several bins: bin n's files (item n of each comma-separated list,
+  // updateglobals.c:76-117, GLOBAL_DIR_XPRE applied; empty when not named) and the Y column
+  // mask; the fields above hold bin 0's
+  std::vector<std::string> A_n_, A_t_, A_rind_, Z_n_, Z_t_, Z_rind_, Y_n_, Y_t_, W_n_, W_t_, T_n_, T_t_, S_n_, S_t_;
+  std::string Y_cind, Y_p;
   // bcc_scramble: per scramble an out_xdrop file and a seed (GLOBAL_scramble_out_xdrop_,
   // GLOBAL_scramble_rseed_; the seed of scramble n defaults to n + 1)
   std::vector<std::string> scramble_files; std::vector<unsigned long> scramble_seeds;
@@ -38,6 +43,11 @@ struct Scorebox {
 void scorebox_xdrop_array(int nxols, int x_max, int x_num, int* xval, int* xdrop);
 void write_scorebox(const Scorebox& S, const std::string& dir);
 void parse_cfg(const char* text, Cfg& c);
+// .b16 input (binary_read.c:5-55): the file's (fnr x fnc) matrix as fnc rows of fnr bits
+struct HostBits { std::vector<uint8_t> b; int nr = 0, nc = 0; size_t rb = 0; };
+HostBits read_b16(const std::string& path);
+void b16_size(const std::string& path, int* fnr, int* fnc);   // the header's sizes only
+std::vector<uint8_t> read_mask(const std::string& path, int n);
 std::string out_dir_of(const Cfg& c);   // GLOBAL_DIR_NAME, else GLOBAL_DIR_BASE/dir_<GLOBAL_out_name>
 void get_xdrop(double lrij, double lcij, double gamma, int gamma_type, int* rdrop, int* cdrop);
 int get_xdrop_length(double lrij, double lcij, double gamma, int gamma_type);

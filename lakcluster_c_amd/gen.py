@@ -192,6 +192,116 @@ def make_dataset(root: str, name: str, nA: int, ncols: int, nZ: int = 0, T: int 
     return Dataset(in_path, d_out, nA, nZ, ncols, T, gamma, B, prow, pcol)
 
 
+def make_bins_dataset(root: str, name: str, bins: Sequence[Tuple[int, int]], ncols: int, ny: int = 0, T: int = 1,
+                      pmin: float = 0.05, pmax: float = 0.5, planted: Sequence[float] = (0.55,), flip: float = 0.1,
+                      gamma: float = 0.05, B: int = 32, seed: int = 1, gamma_type: Optional[int] = None,
+                      kappa_squared: Optional[float] = None, row_mask_frac: float = 0.0, col_mask_frac: float = 0.0,
+                      ycol_mask_frac: float = 0.0, tcol_mask: Optional[Sequence[bool]] = None,
+                      write_T_file: Optional[bool] = None) -> Dataset:
+    """A dexcluster_driver problem with several bins and Y/W genetic controls.
+
+    ``bins`` lists each bin's (case rows, control rows); a bin may have no controls.  The data are drawn for all rows at
+    once (A and Y over every bin's cases, Z and W over every bin's controls, the planted
+    bicluster in A) and bin b takes the next block of rows, so the same seed with the rows
+    split differently gives the same patients.  Per bin the .in names A_n/A_t, Z_n/Z_t,
+    Y_n/Y_t, W_n/W_t, T_n/T_t, S_n/S_t and the row masks as comma-separated lists
+    (updateglobals.c:76-117); the column masks of A, Y and T are shared.
+    """
+    rng = np.random.default_rng(seed)
+    d_in = os.path.join(root, f"dir_{name}_in")
+    d_out = os.path.join(root, f"dir_{name}_out")
+    os.makedirs(d_in, exist_ok=True)
+    os.makedirs(d_out, exist_ok=True)
+    nb = len(bins)
+    nA = int(sum(b[0] for b in bins))
+    nZ = int(sum(b[1] for b in bins))
+    p = rng.uniform(pmin, pmax, size=ncols).astype(np.float32)
+    A = _bernoulli_columns(rng, nA, p)
+    prow, pcol = [], []
+    for e in planted:
+        m, mc = int(round(nA ** e)), int(round(ncols ** e))
+        r, c = _plant(rng, A, min(m, nA), min(mc, ncols), flip)
+        prow.append(r)
+        pcol.append(c)
+    Z = _bernoulli_columns(rng, nZ, p)
+    if write_T_file is None:
+        write_T_file = T > 1
+    Tm = _covariates(rng, nA, T)
+    Sm = _covariates(rng, nZ, T)
+    mrA = rng.random(nA) >= row_mask_frac
+    mrZ = rng.random(nZ) >= row_mask_frac
+    mcA = rng.random(ncols) >= col_mask_frac
+    # the Y/W controls last: the same seed with and without them gives the same A, Z, T, S
+    py = rng.uniform(pmin, pmax, size=ny).astype(np.float32)
+    Y = _bernoulli_columns(rng, nA, py)
+    W = _bernoulli_columns(rng, nZ, py)
+    mcY = rng.random(ny) >= ycol_mask_frac
+    # per key one item per bin, empty where the bin has no such file
+    keys = {k: [""] * nb for k in ("A_n", "A_t", "Z_n", "Z_t", "Y_n", "Y_t", "W_n", "W_t", "T_n", "T_t", "S_n",
+                                   "S_t", "A_rind", "Z_rind")}
+    a0 = z0 = 0
+    for b, (na, nz) in enumerate(bins):
+        ra, rz = slice(a0, a0 + na), slice(z0, z0 + nz)
+        a0 += na
+        z0 += nz
+        mats = [("A", A[ra]), ("Y", Y[ra])]
+        if nz > 0:
+            mats += [("Z", Z[rz]), ("W", W[rz])]
+        if write_T_file:
+            mats.append(("T", Tm[ra]))
+            if nz > 0:
+                mats.append(("S", Sm[rz]))
+        for key, X in mats:
+            if key in ("Y", "W") and ny <= 0:
+                continue
+            write_b16(os.path.join(d_in, f"{key}_n_{b}.b16"), X)
+            write_b16(os.path.join(d_in, f"{key}_t_{b}.b16"), X.T)
+            keys[f"{key}_n"][b] = f"{key}_n_{b}.b16"
+            keys[f"{key}_t"][b] = f"{key}_t_{b}.b16"
+        if row_mask_frac > 0:
+            write_b16(os.path.join(d_in, f"mr_A_{b}.b16"), mrA[ra][:, None])
+            keys["A_rind"][b] = f"mr_A_{b}.b16"
+            if nz > 0:
+                write_b16(os.path.join(d_in, f"mr_Z_{b}.b16"), mrZ[rz][:, None])
+                keys["Z_rind"][b] = f"mr_Z_{b}.b16"
+    lines = [
+        "GLOBAL_verbose= 0;",
+        "GLOBAL_thread_count= 8;",
+        "GLOBAL_TEST_TYPE= dexcluster_driver;",
+        f"GLOBAL_NBINS= {nb};",
+        f"GLOBAL_B_MLT= {B};",
+        f"GLOBAL_gamma= {gamma:.4f};",
+    ]
+    if gamma_type is not None:
+        lines.append(f"GLOBAL_gamma_type= {gamma_type};")
+    if kappa_squared is not None:
+        lines.append(f"GLOBAL_kappa_squared= {kappa_squared};")
+    lines += [f"GLOBAL_DIR_XPRE= {d_in};", f"GLOBAL_A_n_cols= {ncols};", f"GLOBAL_Y_n_cols= {max(ny, 0)};",
+              f"GLOBAL_T_n_cols= {T};"]
+    names = {"A_n": "GLOBAL_A_n_name_", "A_t": "GLOBAL_A_t_name_", "Z_n": "GLOBAL_Z_n_name_",
+             "Z_t": "GLOBAL_Z_t_name_", "Y_n": "GLOBAL_Y_n_name_", "Y_t": "GLOBAL_Y_t_name_",
+             "W_n": "GLOBAL_W_n_name_", "W_t": "GLOBAL_W_t_name_", "T_n": "GLOBAL_T_n_name_",
+             "T_t": "GLOBAL_T_t_name_", "S_n": "GLOBAL_S_n_name_", "S_t": "GLOBAL_S_t_name_",
+             "A_rind": "GLOBAL_A_n_rind_", "Z_rind": "GLOBAL_Z_n_rind_"}
+    for k, key in names.items():
+        if any(keys[k]):
+            lines.append(f"{key}= {', '.join(keys[k])};")
+    if col_mask_frac > 0:
+        write_b16(os.path.join(d_in, "mc_A.b16"), mcA[:, None])
+        lines.append("GLOBAL_A_n_cind= mc_A.b16;")
+    if ycol_mask_frac > 0 and ny > 0:
+        write_b16(os.path.join(d_in, "mc_Y.b16"), mcY[:, None])
+        lines.append("GLOBAL_Y_n_cind= mc_Y.b16;")
+    if tcol_mask is not None:
+        write_b16(os.path.join(d_in, "mc_T.b16"), np.asarray(tcol_mask, dtype=bool)[:, None])
+        lines.append("GLOBAL_T_n_cind= mc_T.b16;")
+    lines += [f"GLOBAL_DIR_NAME= {d_out};", "END= 0;"]
+    in_path = os.path.join(d_in, f"{name}.in")
+    with open(in_path, "w") as fp:
+        fp.write("\n".join(lines) + "\n")
+    return Dataset(in_path, d_out, nA, nZ, ncols, T, gamma, B, prow, pcol)
+
+
 # the benchmark-sized problems (rows x SNP columns)
 CONFIGS = {
     "case_only_4096x16384": dict(nA=4096, ncols=16384, nZ=0, T=1, pmin=0.05, pmax=0.5,

@@ -119,6 +119,7 @@ def lib():
         "lkc_bcc_is_scorebox": (C.c_int, [vp]),
         "lkc_dexcluster_driver": (C.c_int, [cp, cp]),
         "lkc_dcc_sumscores_test": (C.c_int, [cp, cp]),
+        "lkc_dcc_driver_bins": (C.c_int, [cp, cp]),
         "lkc_bcc_is_dcc": (C.c_int, [vp]),
         "lkc_bcc_active": (C.c_int, [vp]),
         "lkc_bcc_sumscores_dmp": (C.c_int, [vp, cp]),
@@ -344,8 +345,15 @@ def lakcluster_driver(in_text: str, out_dir: Optional[str] = None) -> None:
 
 
 def dexcluster_driver(in_text: str, out_dir: Optional[str] = None) -> None:
-    """Whole dcc run with the three output files written (dexcluster_driver)."""
+    """Whole dcc run with the three output files written (dexcluster_driver); several bins
+    or Y/W files take the multi-bin elimination."""
     _check(lib().lkc_dexcluster_driver(in_text.encode(), out_dir.encode() if out_dir else None))
+
+
+def dcc_driver_bins(in_text: str, out_dir: Optional[str] = None) -> None:
+    """dexcluster_driver with several bins and Y/W controls from the .in's per-bin file lists
+    (any bin count, one included); the three output files written."""
+    _check(lib().lkc_dcc_driver_bins(in_text.encode(), out_dir.encode() if out_dir else None))
 
 
 def dcc_sumscores_test(in_text: str, out_dir: Optional[str] = None) -> None:

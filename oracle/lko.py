@@ -59,6 +59,7 @@ def lib():
         "lko_scorebox": (C.c_int, [cp, C.c_int, ip, ip, ip, ip, dp, dp, cp, C.c_int]),
         "lko_scorebox_xdrop_array": (None, [C.c_int, C.c_int, C.c_int, ip, ip]),
         "lko_dcc_sumscores_test": (C.c_int, [vp, cp, dp, ip, ip, ip, ip]),
+        "lko_dcc_driver_bins": (C.c_int, [cp, cp, ip, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -234,6 +235,25 @@ def dcc_sumscores_test(out_dir: Optional[str] = None, **kw):
                                 _p(xa, C.c_int), _p(xb, C.c_int), C.byref(nx)) != 0:
         raise RuntimeError("oracle: dcc_sumscores_test failed")
     return tr[:nt.value], xa[:nx.value], xb[:nx.value]
+
+
+def read_outputs(out_dir: str):
+    """(trace (n x 6), xdrop_a (n x 2), xdrop_b (n x 2)) from the three output files."""
+    def rd(name, dt, w):
+        a = np.loadtxt(os.path.join(out_dir, name), dtype=dt, ndmin=2)
+        return a.reshape(-1, w)
+    return rd("out_trace.txt", np.float64, 6), rd("out_xdrop_a.txt", np.int64, 2), rd("out_xdrop_b.txt", np.int64, 2)
+
+
+def dcc_driver_bins(in_text: str, out_dir: str):
+    """dexcluster_driver on the .in's per-bin files (several bins, Y/W controls) to the end;
+    writes the three output files into out_dir and returns them as read_outputs does."""
+    os.makedirs(out_dir, exist_ok=True)
+    nt, nx = C.c_int(0), C.c_int(0)
+    rc = lib().lko_dcc_driver_bins(in_text.encode(), out_dir.encode(), C.byref(nt), C.byref(nx))
+    if rc != 0:
+        raise RuntimeError(f"oracle: dcc driver over bins failed ({rc})")
+    return read_outputs(out_dir)
 
 
 class State:

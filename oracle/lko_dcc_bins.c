@@ -119,6 +119,8 @@ static void gen_mask(int n, double mrnd, uint8_t *mb, uint8_t *mj)
   for (int nx = 0; nx < n; nx++) { double dtmp = rand01; if (dtmp > mrnd * mrnd) bs(mb, nx); if (dtmp > mrnd) bs(mj, nx); }
 }
 
+static void dcc_alloc(dcc_t *D);
+
 /* dcc_init (dcc.c:1217-1275) */
 static void dcc_init(dcc_t *D, const lko_dcc_params *p)
 {
@@ -164,7 +166,13 @@ static void dcc_init(dcc_t *D, const lko_dcc_params *p)
   for (int nc = 0; nc < D->Ypc; nc++) D->Y_p[nc] = maximum(0.1, minimum(0.9, rand01));
   if (p->sparse == 0) for (int nc = 0; nc < D->Ypc; nc++) D->Y_p[nc] = 0.5;
   D->Y_ajdk = (double *)zalloc((size_t)LKO_AJDK * D->Ypc, sizeof(double)); lko_calc_ajdk(D->Y_p, D->Ypc, D->Y_ajdk);
-  /* dcc_X_nrows_total and dcc_init_QX: global row indices over the bins */
+  dcc_alloc(D);
+}
+
+/* dcc_X_nrows_total (dcc.c:1115-1131) and dcc_init_QX (dcc.c:1047-1090): global row
+   indices over the bins, then every score and work array */
+static void dcc_alloc(dcc_t *D)
+{
   int mx_a = 0, mx_b = 0;
   for (int nb = 0; nb < D->nbins; nb++) {
     dbin *E = &D->E[nb];
@@ -200,6 +208,255 @@ static void dcc_init(dcc_t *D, const lko_dcc_params *p)
   D->QC_a = (int *)zalloc(D->M + 1, sizeof(int)); D->QC_b = (int *)zalloc(D->M + 1, sizeof(int)); D->QC_j = (int *)zalloc(D->M + 1, sizeof(int));
   int *q7[7]; for (int i = 0; i < 7; i++) q7[i] = (int *)zalloc(D->A_nrows_total + 1, sizeof(int));
   D->QR_a = q7[0]; D->QR_b = q7[1]; D->QR_j = q7[2]; D->QR_nb = q7[3]; D->QR_ga = q7[4]; D->QR_gb = q7[5]; free(q7[6]);
+}
+
+/* ------------------------------------------------------------------ */
+/* dcc_load (dcc.c:1157-1215): the problem from the .in's per-bin files  */
+/* ------------------------------------------------------------------ */
+#define DL_MAXBINS 64
+typedef struct { char *v[DL_MAXBINS]; } dlist;           /* per-bin paths, NULL: none */
+typedef struct {
+  int nbins, B, gamma_type, sparse, Ireq, A_ncols, Y_ncols, T_ncols, scramble_num;
+  double gamma, kappa2;
+  char xpre[4096];
+  dlist A_n, A_t, A_rind, Z_n, Z_t, Z_rind, Y_n, Y_t, W_n, W_t, T_n, T_t, S_n, S_t;
+  char *A_cind, *Y_cind, *T_cind, *A_p, *Y_p;
+} dcfg;
+
+static void dl_trim(char *s)
+{
+  size_t n = strlen(s);
+  while (n && (s[n - 1] == ' ' || s[n - 1] == '\t' || s[n - 1] == '\n' || s[n - 1] == '\r')) s[--n] = 0;
+  size_t i = 0; while (s[i] == ' ' || s[i] == '\t' || s[i] == '\n' || s[i] == '\r') i++;
+  if (i) memmove(s, s + i, strlen(s + i) + 1);
+}
+/* a name as updateglobals keeps it: GLOBAL_DIR_XPRE/name when the prefix was set before */
+static char *dl_path(const dcfg *c, const char *v)
+{
+  if (!v[0]) return NULL;
+  size_t n = strlen(c->xpre) + strlen(v) + 2;
+  char *p = (char *)zalloc(n, 1);
+  if (c->xpre[0]) snprintf(p, n, "%s/%s", c->xpre, v); else snprintf(p, n, "%s", v);
+  return p;
+}
+static void dl_list(const dcfg *c, char *val, dlist *L)
+{
+  int nb = 0; char *q = val;
+  while (nb < DL_MAXBINS) {
+    char *cm = strchr(q, ',');
+    if (cm) *cm = 0;
+    char item[4096]; snprintf(item, sizeof item, "%s", q); dl_trim(item);
+    L->v[nb++] = dl_path(c, item);
+    if (!cm) break;
+    q = cm + 1;
+  }
+}
+/* the keys dexcluster_driver reads (updateglobals.c:5-186); others are accepted and ignored */
+static int dcfg_parse(const char *text, dcfg *c)
+{
+  memset(c, 0, sizeof *c);
+  c->B = 32; c->gamma = 0.125; c->sparse = 1;
+  const char *p = text;
+  while (*p) {
+    const char *eq = strchr(p, '='); if (!eq) break;
+    char name[256]; size_t nl = (size_t)(eq - p); if (nl > 255) nl = 255;
+    memcpy(name, p, nl); name[nl] = 0; dl_trim(name);
+    const char *v0 = eq + 1; const char *semi = strchr(v0, ';'); if (!semi) semi = v0 + strlen(v0);
+    size_t vl = (size_t)(semi - v0);
+    char *val = (char *)zalloc(vl + 1, 1); memcpy(val, v0, vl); dl_trim(val);
+    p = (*semi) ? semi + 1 : semi;
+    if (!strcmp(name, "END")) { free(val); break; }
+    if (!strcmp(name, "GLOBAL_NBINS")) c->nbins = atoi(val);
+    else if (!strcmp(name, "GLOBAL_B_MLT")) c->B = atoi(val);
+    else if (!strcmp(name, "GLOBAL_gamma")) c->gamma = atof(val);
+    else if (!strcmp(name, "GLOBAL_gamma_type")) c->gamma_type = atoi(val);
+    else if (!strcmp(name, "GLOBAL_kappa_squared")) c->kappa2 = atof(val);
+    else if (!strcmp(name, "GLOBAL_TEST_sparse")) c->sparse = atoi(val);
+    else if (!strcmp(name, "GLOBAL_Ireq")) c->Ireq = atoi(val);
+    else if (!strcmp(name, "GLOBAL_scramble_num")) c->scramble_num = atoi(val);
+    else if (!strcmp(name, "GLOBAL_A_n_cols")) c->A_ncols = atoi(val);
+    else if (!strcmp(name, "GLOBAL_Y_n_cols")) c->Y_ncols = atoi(val);
+    else if (!strcmp(name, "GLOBAL_T_n_cols")) c->T_ncols = atoi(val);
+    else if (!strcmp(name, "GLOBAL_DIR_XPRE")) snprintf(c->xpre, sizeof c->xpre, "%s", val);
+    else if (!strcmp(name, "GLOBAL_A_n_name_")) dl_list(c, val, &c->A_n);
+    else if (!strcmp(name, "GLOBAL_A_t_name_")) dl_list(c, val, &c->A_t);
+    else if (!strcmp(name, "GLOBAL_A_n_rind_")) dl_list(c, val, &c->A_rind);
+    else if (!strcmp(name, "GLOBAL_Z_n_name_")) dl_list(c, val, &c->Z_n);
+    else if (!strcmp(name, "GLOBAL_Z_t_name_")) dl_list(c, val, &c->Z_t);
+    else if (!strcmp(name, "GLOBAL_Z_n_rind_")) dl_list(c, val, &c->Z_rind);
+    else if (!strcmp(name, "GLOBAL_Y_n_name_")) dl_list(c, val, &c->Y_n);
+    else if (!strcmp(name, "GLOBAL_Y_t_name_")) dl_list(c, val, &c->Y_t);
+    else if (!strcmp(name, "GLOBAL_W_n_name_")) dl_list(c, val, &c->W_n);
+    else if (!strcmp(name, "GLOBAL_W_t_name_")) dl_list(c, val, &c->W_t);
+    else if (!strcmp(name, "GLOBAL_T_n_name_")) dl_list(c, val, &c->T_n);
+    else if (!strcmp(name, "GLOBAL_T_t_name_")) dl_list(c, val, &c->T_t);
+    else if (!strcmp(name, "GLOBAL_S_n_name_")) dl_list(c, val, &c->S_n);
+    else if (!strcmp(name, "GLOBAL_S_t_name_")) dl_list(c, val, &c->S_t);
+    else if (!strcmp(name, "GLOBAL_A_n_cind")) c->A_cind = dl_path(c, val);
+    else if (!strcmp(name, "GLOBAL_Y_n_cind")) c->Y_cind = dl_path(c, val);
+    else if (!strcmp(name, "GLOBAL_T_n_cind")) c->T_cind = dl_path(c, val);
+    else if (!strcmp(name, "GLOBAL_A_p_name")) c->A_p = dl_path(c, val);
+    else if (!strcmp(name, "GLOBAL_Y_p_name")) c->Y_p = dl_path(c, val);
+    else if (strncmp(name, "GLOBAL_", 7)) { fprintf(stderr, "lko_dcc_bins: unknown key %s\n", name); free(val); return -1; }
+    free(val);
+  }
+  return 0;
+}
+static void dcfg_free(dcfg *c)
+{
+  dlist *L[] = { &c->A_n, &c->A_t, &c->A_rind, &c->Z_n, &c->Z_t, &c->Z_rind, &c->Y_n, &c->Y_t, &c->W_n, &c->W_t,
+                 &c->T_n, &c->T_t, &c->S_n, &c->S_t };
+  for (size_t i = 0; i < sizeof L / sizeof L[0]; i++) for (int nb = 0; nb < DL_MAXBINS; nb++) free(L[i]->v[nb]);
+  free(c->A_cind); free(c->Y_cind); free(c->T_cind); free(c->A_p); free(c->Y_p);
+}
+
+/* .b16 (binary_read.c:5-55): a file holding an (fnr x fnc) matrix column by column gives
+   fnc rows of fnr bits */
+static int dl_read(const char *path, dm *out)
+{
+  FILE *fp = fopen(path, "rb");
+  if (!fp) { fprintf(stderr, "lko_dcc_bins: cannot open %s\n", path); return -1; }
+  int hdr[3];
+  if (fread(hdr, sizeof(int), 3, fp) != 3) { fclose(fp); return -1; }
+  int bitj = hdr[0] > 0 ? hdr[0] : 16, fnr = hdr[1], fnc = hdr[2];
+  size_t wp = (size_t)(fnr + (bitj - (fnr % bitj)) % bitj) / 8;
+  *out = dm_new(fnc, fnr);
+  for (int r = 0; r < fnc; r++)
+    if (wp && fread(out->b + (size_t)r * out->rb, 1, wp, fp) != wp) { fclose(fp); return -1; }
+  fclose(fp);
+  return 0;
+}
+/* an (n x 1) mask file, or all n set when none is named */
+static int dl_mask(const char *path, int n, uint8_t *m)
+{
+  if (!path) { for (int i = 0; i < n; i++) bs(m, i); return 0; }
+  dm t; if (dl_read(path, &t)) return -1;
+  if (t.nr != 1 || t.nc != n) { fprintf(stderr, "lko_dcc_bins: mask %s has %d entries, expected %d\n", path, t.nc, n); free(t.b); return -1; }
+  for (int i = 0; i < n; i++) if (bg(t.b, i)) bs(m, i);
+  free(t.b);
+  return 0;
+}
+/* X and its transpose from a pair of files, checked against the expected shape */
+static int dl_pair(const char *pt, const char *pn, int nrows, int ncols, dm *Xn, dm *Xt)
+{
+  if (!pt || !pn) return -1;
+  if (dl_read(pt, Xn) || dl_read(pn, Xt)) return -1;
+  if (Xn->nr != nrows || Xn->nc != ncols || Xt->nr != ncols || Xt->nc != nrows) {
+    fprintf(stderr, "lko_dcc_bins: %s / %s are %dx%d / %dx%d, expected %dx%d\n", pt, pn, Xn->nr, Xn->nc, Xt->nr, Xt->nc, nrows, ncols);
+    return -1;
+  }
+  return 0;
+}
+/* the intercept covariate when no T/S file is named (dcc.c:374-384): one column, set in every row */
+static void dl_intercept(int nrows, dm *Xn, dm *Xt)
+{
+  *Xn = dm_new(nrows, 1); *Xt = dm_new(1, nrows);
+  for (int r = 0; r < nrows; r++) { bs(Xn->b + (size_t)r * Xn->rb, 0); bs(Xt->b, r); }
+}
+/* sizes from a .b16 header: the file's rows (fnr) and columns (fnc) */
+static int dl_size(const char *path, int *fnr, int *fnc)
+{
+  FILE *fp = fopen(path, "rb"); if (!fp) return -1;
+  int hdr[3]; int ok = fread(hdr, sizeof(int), 3, fp) == 3; fclose(fp);
+  if (!ok) return -1;
+  *fnr = hdr[1]; *fnc = hdr[2];
+  return 0;
+}
+
+/* dcc_ajdk_load, dcc_single_load_M_An per bin, dcc_load_A_p (dcc.c:54-115, 251-419, 875-991) */
+static int dcc_load(dcc_t *D, const dcfg *c)
+{
+  memset(D, 0, sizeof *D);
+  D->p.nbins = c->nbins; D->p.B_MLT = c->B; D->p.gamma = c->gamma; D->p.gamma_type = c->gamma_type;
+  D->p.kappa2 = c->kappa2; D->p.sparse = c->sparse; D->p.niter = 0;
+  D->nbins = c->nbins; D->B = c->B; D->D_MLT = 1LL << c->B;
+  int fnr = 0, fnc = 0;
+  /* columns from bin 0's files: A (SNPs), Y (controls' SNPs), T (covariates) */
+  if (!c->A_t.v[0] || dl_size(c->A_t.v[0], &fnr, &fnc)) return -1;
+  D->M = fnr;
+  D->NY = 0; if (c->Y_t.v[0]) { if (dl_size(c->Y_t.v[0], &fnr, &fnc)) return -1; D->NY = fnr; }
+  D->T = 1; if (c->T_t.v[0]) { if (dl_size(c->T_t.v[0], &fnr, &fnc)) return -1; D->T = fnr; }
+  /* every bin names the same kinds of file: Y (W) and T (S) in all bins or in none */
+  for (int nb = 0; nb < D->nbins; nb++)
+    if (!c->A_t.v[nb] || !c->A_n.v[nb] || (!c->Y_t.v[nb]) != (!c->Y_t.v[0]) || (!c->T_t.v[nb]) != (!c->T_t.v[0]) ||
+        (!c->Z_t.v[nb]) != (!c->Z_n.v[nb])) {
+      fprintf(stderr, "lko_dcc_bins: bin %d names a different set of files\n", nb); return -1;
+    }
+  if ((c->A_ncols && c->A_ncols != D->M) || (c->Y_ncols && c->Y_ncols != D->NY)) {
+    fprintf(stderr, "lko_dcc_bins: column counts of the .in and the files differ\n"); return -1;
+  }
+  if (D->T > 8) return -1;
+  D->Ac_b = mask_new(D->M); D->Ac_j = mask_new(D->M); D->Ac_rtn = mask_new(D->M); D->Ac_rmv = mask_new(D->M);
+  D->Yc_b = mask_new(D->NY); D->Yc_j = mask_new(D->NY); D->Tc_b = mask_new(D->T); D->Tc_j = mask_new(D->T);
+  if (dl_mask(c->A_cind, D->M, D->Ac_b) || dl_mask(c->Y_cind, D->NY, D->Yc_b)) return -1;
+  /* covariates: the T_n_cind mask, else the first GLOBAL_T_n_cols of them (all when 0), as
+     the one-bin loader takes them */
+  if (c->T_cind) { if (dl_mask(c->T_cind, D->T, D->Tc_b)) return -1; }
+  else { const int nt = c->T_ncols > 0 ? minimum(c->T_ncols, D->T) : D->T; for (int t = 0; t < nt; t++) bs(D->Tc_b, t); }
+  memcpy(D->Ac_j, D->Ac_b, lko_bsize(D->M)); memcpy(D->Yc_j, D->Yc_b, lko_bsize(D->NY)); memcpy(D->Tc_j, D->Tc_b, lko_bsize(D->T));
+  D->tb = (int *)zalloc(D->T + 1, sizeof(int));
+  for (int t = 0, b = 0; t < D->T; t++) { D->tb[t] = b; if (bg(D->Tc_b, t)) b++; }
+  D->E = (dbin *)zalloc(D->nbins, sizeof(dbin));
+  for (int nb = 0; nb < D->nbins; nb++) {
+    dbin *E = &D->E[nb];
+    if (!c->A_t.v[nb] || dl_size(c->A_t.v[nb], &fnr, &fnc)) return -1;
+    E->nA = fnc;
+    if (dl_pair(c->A_t.v[nb], c->A_n.v[nb], E->nA, D->M, &E->An, &E->At)) return -1;
+    E->nZ = 0;
+    if (c->Z_t.v[nb]) { if (dl_size(c->Z_t.v[nb], &fnr, &fnc)) return -1; E->nZ = fnc; }
+    if (E->nZ > 0) { if (dl_pair(c->Z_t.v[nb], c->Z_n.v[nb], E->nZ, D->M, &E->Zn, &E->Zt)) return -1; }
+    else { E->Zn = dm_new(0, D->M); E->Zt = dm_new(D->M, 0); }
+    if (D->NY > 0 && c->Y_t.v[nb]) { if (dl_pair(c->Y_t.v[nb], c->Y_n.v[nb], E->nA, D->NY, &E->Yn, &E->Yt)) return -1; }
+    else { E->Yn = dm_new(E->nA, D->NY); E->Yt = dm_new(D->NY, E->nA); }
+    if (D->NY > 0 && E->nZ > 0) { if (dl_pair(c->W_t.v[nb], c->W_n.v[nb], E->nZ, D->NY, &E->Wn, &E->Wt)) return -1; }
+    else { E->Wn = dm_new(E->nZ, D->NY); E->Wt = dm_new(D->NY, E->nZ); }
+    if (c->T_t.v[nb]) { if (dl_pair(c->T_t.v[nb], c->T_n.v[nb], E->nA, D->T, &E->Tn, &E->Tt)) return -1; }
+    else if (D->T == 1) dl_intercept(E->nA, &E->Tn, &E->Tt);
+    else return -1;
+    if (E->nZ > 0 && c->S_t.v[nb]) { if (dl_pair(c->S_t.v[nb], c->S_n.v[nb], E->nZ, D->T, &E->Sn, &E->St)) return -1; }
+    else if (D->T == 1 || E->nZ == 0) dl_intercept(E->nZ, &E->Sn, &E->St);
+    else return -1;
+    E->Ar_b = mask_new(E->nA); E->Ar_j = mask_new(E->nA); E->Ar_rtn = mask_new(E->nA); E->Ar_rmv = mask_new(E->nA);
+    E->Zr_b = mask_new(E->nZ); E->Zr_j = mask_new(E->nZ);
+    if (dl_mask(c->A_rind.v[nb], E->nA, E->Ar_b) || dl_mask(c->Z_rind.v[nb], E->nZ, E->Zr_b)) return -1;
+    memcpy(E->Ar_j, E->Ar_b, lko_bsize(E->nA)); memcpy(E->Zr_j, E->Zr_b, lko_bsize(E->nZ));
+  }
+  /* dcc_load_A_p: per column the loaded rows' counts of A and Z (Y and W) over the bins,
+     then per 1920-column block the frequency over the block's loaded columns */
+  int arb = 0, zrb = 0;
+  for (int nb = 0; nb < D->nbins; nb++) { arb += popn(D->E[nb].Ar_b, D->E[nb].nA); zrb += popn(D->E[nb].Zr_b, D->E[nb].nZ); }
+  double *AZ = (double *)zalloc(D->M + 1, sizeof(double)), *YW = (double *)zalloc(D->NY + 1, sizeof(double));
+  for (int nb = 0; nb < D->nbins; nb++) {
+    const dbin *E = &D->E[nb];
+    for (int k = 0; k < D->M; k++) {
+      const double a = bg(D->Ac_b, k) ? (double)pop_m(row(&E->At, k), E->Ar_b, lko_bsize(E->nA)) : 0.0;
+      const double z = (bg(D->Ac_b, k) && E->nZ > 0) ? (double)pop_m(row(&E->Zt, k), E->Zr_b, lko_bsize(E->nZ)) : 0.0;
+      AZ[k] += a; AZ[k] += z;
+    }
+    for (int k = 0; k < D->NY; k++) {
+      const double y = bg(D->Yc_b, k) ? (double)pop_m(row(&E->Yt, k), E->Ar_b, lko_bsize(E->nA)) : 0.0;
+      const double w = (bg(D->Yc_b, k) && E->nZ > 0) ? (double)pop_m(row(&E->Wt, k), E->Zr_b, lko_bsize(E->nZ)) : 0.0;
+      YW[k] += y; YW[k] += w;
+    }
+  }
+  D->Apc = lko_psize(D->M); D->A_p = (double *)zalloc(D->Apc, sizeof(double));
+  D->Ypc = lko_psize(D->NY); D->Y_p = (double *)zalloc(D->Ypc + 1, sizeof(double));
+  const double *rs[2] = { AZ, YW }; const uint8_t *cm[2] = { D->Ac_b, D->Yc_b }; double *pp[2] = { D->A_p, D->Y_p };
+  const int nx[2] = { D->M, D->NY }, npc[2] = { D->Apc, D->Ypc };
+  for (int v = 0; v < 2; v++) for (int b = 0; b < npc[v]; b++) {
+    pp[v][b] = 0.5;
+    if (!c->sparse) continue;
+    const int s0 = minimum(nx[v] - 1, maximum(0, b * LKO_POPB * 8)), s1 = minimum(nx[v] - 1, maximum(0, (b + 1) * LKO_POPB * 8 - 1));
+    double tsum = 0, tnum = 0;
+    for (int k = s0; k <= s1; k++) if (bg(cm[v], k)) { tsum += rs[v][k]; tnum += 1; }
+    if (tnum == 0) pp[v][b] = 0.5;
+    else { double q = (tsum) / (tnum * (arb + zrb)); pp[v][b] = q < 0.01 ? 0.01 : (q > 0.99 ? 0.99 : q); }
+  }
+  free(AZ); free(YW);
+  D->A_ajdk = (double *)zalloc((size_t)LKO_AJDK * D->Apc, sizeof(double)); lko_calc_ajdk(D->A_p, D->Apc, D->A_ajdk);
+  D->Y_ajdk = (double *)zalloc((size_t)LKO_AJDK * (D->Ypc + 1), sizeof(double)); lko_calc_ajdk(D->Y_p, D->Ypc, D->Y_ajdk);
+  dcc_alloc(D);
+  return 0;
 }
 
 /* dcc_M_mxset (dcc.c:1092-1113): active counts from the masks */
@@ -680,4 +937,33 @@ int lko_dcc_sumscores_test(const lko_dcc_params *p, const char *out_dir, double 
   if (xdrop_b) memcpy(xdrop_b, D.xdrop_b, sizeof(int) * 2 * D.n_xdrop);
   dcc_free(&D);
   return 0;
+}
+
+/* dexcluster_driver (dexcluster_driver.c:5-57) on the .in's files, with several bins and
+   Y/W genetic controls: the loop to the end, the three output files in out_dir */
+int lko_dcc_driver_bins(const char *in_text, const char *out_dir, int *n_trace, int *n_xdrop)
+{
+  if (!in_text || !out_dir) return -1;
+  dcfg c;
+  if (dcfg_parse(in_text, &c)) { dcfg_free(&c); return -1; }
+  if (c.nbins < 1 || c.nbins > DL_MAXBINS || c.Ireq > 0 || c.scramble_num > 0 || c.A_p || c.Y_p || c.B < 1 || c.B > 32) {
+    dcfg_free(&c); return -1;
+  }
+  dcc_t D;
+  const int rc = dcc_load(&D, &c);
+  dcfg_free(&c);
+  if (rc) return -3;
+  dcc_M_mxset(&D);
+  subscores(&D);
+  while (D.A_rpop_j_total > 0 || D.A_cpop_j > 0) {
+    st_nrm(&D); st_ifT(&D); /* dcc_sumscores_srt: nothing with GLOBAL_Ireq <= 0 */ st_cmb(&D);
+    st_xij(&D); st_mxA(&D);
+    st_mxdup(&D); dcc_M_mxset(&D);
+    subscores(&D);
+  }
+  const int wr = write_outputs(&D, out_dir);
+  if (n_trace) *n_trace = D.n_trace;
+  if (n_xdrop) *n_xdrop = D.n_xdrop;
+  dcc_free(&D);
+  return wr ? -2 : 0;
 }

@@ -107,6 +107,9 @@ typedef struct {
   int nbins, A_n_rows, A_n_cols, Z_n_rows, Y_n_cols, T_n_cols, B_MLT, niter, sparse, gamma_type;
   double mrand, gamma, kappa2;
 } lko_dcc_params;
+/* dexcluster_driver on the .in's per-bin files (several bins, Y/W controls): runs to the
+   end and writes out_trace.txt / out_xdrop_a.txt / out_xdrop_b.txt into out_dir */
+int lko_dcc_driver_bins(const char *in_text, const char *out_dir, int *n_trace, int *n_xdrop);
 int lko_dcc_sumscores_test(const lko_dcc_params *p, const char *out_dir, double *trace, int *n_trace,
                            int *xdrop_a, int *xdrop_b, int *n_xdrop);
 
