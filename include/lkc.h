@@ -71,6 +71,16 @@ int lkc_dcc_driver_bins(const char *in_text, const char *out_dir);
    GLOBAL_DIR_BASE/dir_<GLOBAL_out_name>). */
 int lkc_dcc_sumscores_test(const char *in_text, const char *out_dir);
 
+/* bcc_sumscores_test (bcc_sumscores.c:781-880; GLOBAL_TEST_TYPE= bcc_sumscores at
+   lakcluster_ver18.c:335): the synthetic problem of bcc_init (bcc.c:1542-1600; bcc_ajdk_init,
+   bcc_single_init_M_An, bcc_init_A_p) drawn from the GLOBAL_TEST_* keys with glibc rand()
+   from its initial state, then lakcluster_driver's stages for GLOBAL_TEST_niter passes (all
+   of them when 0); the three output files.  One bin, GLOBAL_TEST_Y_n_cols= 0,
+   GLOBAL_TEST_TYP2= error, the YnWt strategies; every covariate column must be drawn active.
+   lkc_bcc_load on such an .in gives the same problem as a handle.  The reference's _uu
+   error printouts are not produced. */
+int lkc_bcc_sumscores_test(const char *in_text, const char *out_dir);
+
 /* lakcluster_scorebox (lakcluster_scorebox.c:265-271, dispatched on GLOBAL_TEST_TYPE=
    lakcluster_scorebox at lakcluster_ver18.c:338): replays the elimination order of
    GLOBAL_scorebox_out_xdrop over the grid of GLOBAL_scorebox_row_num x _col_num

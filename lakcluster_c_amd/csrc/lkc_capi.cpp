@@ -258,6 +258,22 @@ int lkc_dcc_sumscores_test(const char* in_text, const char* out_dir)
   });
 }
 
+int lkc_bcc_sumscores_test(const char* in_text, const char* out_dir)
+{
+  lkc_bcc* D = nullptr;
+  int rc = lkc_bcc_load(in_text, &D);
+  if (rc) return rc;
+  if (!D->run.synthetic) {
+    g_last_error = "bcc_sumscores_test needs GLOBAL_TEST_TYPE= bcc_sumscores";
+    lkc_bcc_free(D);
+    return LKC_ERR_CONFIG;
+  }
+  rc = lkc_bcc_run(D, D->run.c.test_niter > 0 ? D->run.c.test_niter : -1);
+  if (!rc) rc = lkc_bcc_write_outputs(D, out_dir);
+  lkc_bcc_free(D);
+  return rc;
+}
+
 int lkc_bcc_n_iter(const lkc_bcc* D) { return D ? D->run.n_iter : -1; }
 int lkc_bcc_n_trace(const lkc_bcc* D) { return D ? (int)(D->run.trace.size() / 6) : -1; }
 int lkc_bcc_get_trace(const lkc_bcc* D, double* out)

@@ -3,8 +3,8 @@
 // the elimination on device 0, writing out_trace.txt / out_xdrop_{a,b}.txt, or with
 // GLOBAL_TEST_TYPE= lakcluster_scorebox the scorebox replay and its .mda files, with
 // GLOBAL_TEST_TYPE= dexcluster_driver the dcc elimination (several bins and Y/W controls
-// included), or with GLOBAL_TEST_TYPE= dcc_sumscores the reference's dcc self-test on its
-// synthetic problem.
+// included), or with GLOBAL_TEST_TYPE= bcc_sumscores / dcc_sumscores the reference's bcc / dcc
+// self-tests on their synthetic problems.
 #include "../../include/lkc.h"
 #include <stdio.h>
 #include <string>
@@ -26,6 +26,11 @@ int main(int argc, char** argv)
     if (v == "dexcluster_driver") {
       if (lkc_dexcluster_driver(text.c_str(), out_dir)) { fprintf(stderr, "lkc: %s\n", lkc_last_error()); return 1; }
       printf("dexcluster_driver done\n");
+      return 0;
+    }
+    if (v == "bcc_sumscores") {
+      if (lkc_bcc_sumscores_test(text.c_str(), out_dir)) { fprintf(stderr, "lkc: %s\n", lkc_last_error()); return 1; }
+      printf("bcc_sumscores done\n");
       return 0;
     }
     if (v == "dcc_sumscores") {

@@ -13,6 +13,7 @@
 #include "lkc_dcc_bins.h"
 #include "lkc_engine.h"
 #include "lkc_stages.h"
+#include "lkc_testgen.h"
 
 #include <math.h>
 #include <stdio.h>
@@ -34,33 +35,6 @@ inline int bg(const uint8_t* b, long i) { return (b[i >> 3] >> (7 - (i & 7))) & 
 inline void bs(uint8_t* b, long i) { b[i >> 3] |= (uint8_t)(1u << (7 - (i & 7))); }
 inline void bc(uint8_t* b, long i) { b[i >> 3] &= (uint8_t)~(1u << (7 - (i & 7))); }
 int popn(const std::vector<uint8_t>& m, int n) { int s = 0; for (int i = 0; i < n; i++) s += bg(m.data(), i); return s; }
-
-// glibc's rand(): the additive generator of random() seeded as srand(1), the state a
-// program that never seeds starts from.  r_i = 16807 r_{i-1} mod (2^31 - 1) for i < 31
-// (r_0 = 1), r_i = r_{i-31} for i = 31..33, then r_i = r_{i-31} + r_{i-3} mod 2^32; the
-// k-th output is r_{k+344} >> 1.
-class GlibcRand {
-  uint32_t r[34]; unsigned long long pos = 34;
-  uint32_t step() { uint32_t v = r[(pos - 31) % 34] + r[(pos - 3) % 34]; r[pos % 34] = v; pos++; return v; }
- public:
-  GlibcRand()
-  {
-    r[0] = 1;
-    for (int k = 1; k < 31; k++) { long long v = (16807LL * (int32_t)r[k - 1]) % 2147483647LL; if (v < 0) v += 2147483647LL; r[k] = (uint32_t)v; }
-    for (int k = 31; k < 34; k++) r[k] = r[k - 31];
-    for (int k = 34; k < 344; k++) step();
-  }
-  int next() { return (int)(step() >> 1); }
-  double r01() { return (double)next() / (double)2147483647; }   // rand01 (lakcluster_header.h:261)
-};
-// the reference's maximum / minimum macros (lakcluster_header.h:257-258) evaluate the
-// chosen argument again: maximum(0.1, minimum(0.9, rand01)) draws 2 to 4 times
-double crop_draw(GlibcRand& g)
-{
-  double m1 = 0.9 < g.r01() ? 0.9 : g.r01();
-  if (0.1 > m1) return 0.1;
-  return 0.9 < g.r01() ? 0.9 : g.r01();
-}
 
 struct HBits { std::vector<uint8_t> b; int nr = 0, nc = 0; size_t rb = 0;
   void init(int r, int c) { nr = r; nc = c; rb = bsize(c); b.assign((size_t)std::max(r, 1) * rb, 0); } };

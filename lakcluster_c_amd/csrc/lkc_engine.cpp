@@ -6,6 +6,7 @@
 // on the device; per step only the two score columns that are sorted travel to
 // the host, and the new masks and index lists travel back.
 #include "lkc_engine.h"
+#include "lkc_testgen.h"
 
 #include <algorithm>
 #include <mutex>
@@ -493,6 +494,7 @@ void Run::load(const char* in_text)
 {
   double t0 = now_s();
   parse_cfg(in_text, c);
+  if (c.test_type == "bcc_sumscores") { load_synthetic(t0); return; }
   if (c.nbins != 1) throw Error(LKC_ERR_CONFIG, "only GLOBAL_NBINS= 1 is supported");
   bool y_files = false;
   for (const std::string& f : c.Y_t_) y_files = y_files || !f.empty();
@@ -580,6 +582,73 @@ void Run::load(const char* in_text)
   setup_device(t0);
 }
 
+// bcc_init (bcc.c:1542-1600) for one bin without Y/W controls, GLOBAL_TEST_TYP2= error, as
+// bcc_sumscores_test (bcc_sumscores.c:781-880) calls it: bcc_ajdk_init's column masks
+// (bcc.c:117-174; the covariate mask with its first and last column forced on),
+// bcc_single_init_M_An's row masks and matrices (bcc.c:415-520; the draws of
+// wrap_M_setup_test_excerpt_1 / _0, An_ajdk_v.c:32-98: column by column over every row,
+// bit = draw > 1/2; the covariates' column 0 set in every row), then bcc_init_A_p's block
+// frequencies (bcc.c:1315-1330).  The draws replay glibc rand() from its initial state.  A
+// mask bit is on in _b when its draw exceeds mrand^2 and in _j when it exceeds mrand: rows
+// and columns may start loaded but inactive.  Every covariate column must be drawn active.
+void Run::load_synthetic(double t0)
+{
+  if (c.nbins != 1) throw Error(LKC_ERR_CONFIG, "bcc_sumscores: only GLOBAL_NBINS= 1 is supported");
+  if (c.test_Y_cols != 0) throw Error(LKC_ERR_CONFIG, "bcc_sumscores: Y/W genetic controls are not supported (GLOBAL_TEST_Y_n_cols must be 0)");
+  if (c.test_typ2 != "error") throw Error(LKC_ERR_CONFIG, "bcc_sumscores: GLOBAL_TEST_TYP2 must be error (random data)");
+  if (c.QR.find("YnWt") == std::string::npos || c.QC.find("YnWt") == std::string::npos ||
+      c.QR.find("condense") != std::string::npos || c.QC.find("store one") != std::string::npos ||
+      c.QR.find("ZtSWn") != std::string::npos || c.QC.find("ZtSWn") != std::string::npos)
+    throw Error(LKC_ERR_CONFIG, "only the YnWt strategies are supported");
+  if (c.Ireq > 0) throw Error(LKC_ERR_CONFIG, "GLOBAL_Ireq > 0 (per-bin sorting) is not supported");
+  if (c.scramble_num > 0) throw Error(LKC_ERR_CONFIG, "bcc_scramble is not part of bcc_sumscores_test");
+  if (c.B < 1 || c.B > 32) throw Error(LKC_ERR_CONFIG, "GLOBAL_B_MLT must be in [1,32]");
+  if (c.test_A_cols < 1 || c.test_T_cols < 1 || c.test_T_cols > kMaxT) throw Error(LKC_ERR_CONFIG, "bcc_sumscores: bad problem size");
+  synthetic = true; dcc = 0;
+  GlibcRand g;
+  const double mrnd = c.test_mrand;
+  M = c.test_A_cols; T = c.test_T_cols;
+  NA = std::max(0, c.test_A_rows + (0 % 3) - 1); NZ = std::max(0, c.test_Z_rows + (0 % 5) - 2);
+  has_Z = NZ > 0;
+  auto mk = [](int n) { return std::vector<uint8_t>(bsize(n) + 8, 0); };
+  auto mask_draw = [&](int n, std::vector<uint8_t>& mb, std::vector<uint8_t>& mj) {
+    mb = mk(n); mj = mk(n);
+    for (int i = 0; i < n; i++) { const double d = g.r01(); if (d > mrnd * mrnd) bset(mb.data(), i); if (d > mrnd) bset(mj.data(), i); }
+  };
+  auto bits_draw = [&](int nrows, int ncols, HostBits& Xn, HostBits& Xt) {
+    Xn.nr = nrows; Xn.nc = ncols; Xn.rb = bsize(ncols); Xn.b.assign((size_t)std::max(nrows, 1) * Xn.rb, 0);
+    Xt.nr = ncols; Xt.nc = nrows; Xt.rb = bsize(nrows); Xt.b.assign((size_t)std::max(ncols, 1) * Xt.rb, 0);
+    for (int k = 0; k < ncols; k++) for (int r = 0; r < nrows; r++)
+      if (g.r01() > 0.5) { bset(Xn.b.data() + (size_t)r * Xn.rb, k); bset(Xt.b.data() + (size_t)k * Xt.rb, r); }
+  };
+  std::vector<uint8_t> tcb, tcj;
+  mask_draw(M, hAc_b, hAc_j);
+  mask_draw(T, tcb, tcj);
+  bset(tcb.data(), 0); bset(tcj.data(), 0); bset(tcb.data(), T - 1); bset(tcj.data(), T - 1);
+  HostBits hAn, hAt, hZn, hZt, hTn, hTt, hSn, hSt;
+  mask_draw(NA, hAr_b, hAr_j);
+  bits_draw(NA, M, hAn, hAt);
+  mask_draw(NZ, hZr_b, hZr_j);
+  bits_draw(NZ, M, hZn, hZt);
+  bits_draw(NA, T, hTn, hTt);   // Y and W have no columns: no draws
+  for (int r = 0; r < NA; r++) bset(hTt.b.data(), r);
+  bits_draw(NZ, T, hSn, hSt);
+  for (int r = 0; r < NZ; r++) bset(hSt.b.data(), r);
+  pcols = psize(M);
+  hA_p.assign(pcols, 0.5);
+  for (int b = 0; b < pcols; b++) hA_p[b] = crop_draw(g);
+  if (c.sparse == 0) std::fill(hA_p.begin(), hA_p.end(), 0.5);
+  if (popmask(tcj, T) != T) throw Error(LKC_ERR_CONFIG, "bcc_sumscores: the drawn covariate mask leaves a column out (not supported)");
+  hTc_j = tcj;
+  hAr_rtn = mk(NA); hAr_rmv = mk(NA); hAc_rtn = mk(M); hAc_rmv = mk(M);
+  make_stream(&st, dev);
+  finish_masks();
+  upload_bits(dev, An, hAn); upload_bits(dev, At, hAt);
+  upload_bits(dev, Zn, hZn); upload_bits(dev, Zt, hZt);
+  upload_bits(dev, Tt, hTt); upload_bits(dev, St, hSt);
+  setup_device(t0);
+}
+
 // A job drawn from a device-resident pool (permutation nulls, serving): the listed
 // rows become A and Z, compacted in the order given; all of them are active; colmask
 // (bsize(M) bytes, NULL = all) selects the columns, as the mr_/mc_ masks of a .in do.
@@ -646,7 +715,8 @@ void Run::masks_from_files()
 void Run::finish_masks()
 {
   for (int i = 1; i < T; i++) if (bget(hTc_j.data(), i) && !bget(hTc_j.data(), i - 1)) throw Error(LKC_ERR_CONFIG, "T column mask must be a prefix");
-  hAr_j = hAr_b; hZr_j = hZr_b; hAc_j = hAc_b;
+  if (!synthetic) { hAr_j = hAr_b; hZr_j = hZr_b; hAc_j = hAc_b; }   // a loaded problem starts with all of it active
+  hAr_j0 = hAr_j; hZr_j0 = hZr_j; hAc_j0 = hAc_j;
   Arb = popmask(hAr_b, NA); Zrb = popmask(hZr_b, NZ);
   if (Arb == 0) throw Error(LKC_ERR_SHAPE, "A has no active rows");
   Ar_bidx.assign(NA + 1, 0); Ac_bidx.assign(M + 1, 0);
@@ -707,10 +777,10 @@ void Run::setup_device(double t0)
   An_c = dev.alloc<uint32_t>((size_t)NA * cmp_Wmax);
   Zn_c = dev.alloc<uint32_t>((size_t)std::max(NZ, 1) * cmp_Wmax);
 
-  // A_p per 1920-column block (bcc_load_A_p, bcc.c:1197-1313)
-  hA_p.assign(pcols, 0.5);
+  // A_p per 1920-column block (bcc_load_A_p, bcc.c:1197-1313); bcc_init's problem draws its own
+  if (!synthetic) hA_p.assign(pcols, 0.5);
   push_ctl();
-  {
+  if (!synthetic) {
     int* cnt = dev.alloc<int>((size_t)M * 2);
     popc_rows(At.b(), nullptr, M, Ar_b, (int)(At.rb / 4), cnt, st);
     if (has_Z) popc_rows(Zt.b(), nullptr, M, Zr_b, (int)(Zt.rb / 4), cnt + M, st);
@@ -758,7 +828,9 @@ void Run::setup_device(double t0)
   HIPCHK(hipStreamSynchronize(st));
 
   // xcalc orientation flags (bcc_double_init_lf, bcc.c:1005-1150)
-  int rdrop0 = 0; get_xdrop(Arb, popmask(hAc_b, M), c.gamma, c.gamma_type, &rdrop0, NULL);
+  // bcc_double_init_lf (bcc.c:1005-1040): the first step's row drop from the active counts,
+  // the sides by their loaded counts
+  int rdrop0 = 0; get_xdrop(popmask(hAr_j, NA), popmask(hAc_j, M), c.gamma, c.gamma_type, &rdrop0, NULL);
   int rdrop2 = std::max(2, rdrop0);
   trm_AnZt = (Zrb <= Arb) ? 0 : 1;
   trm_knzt = (Zrb <= std::min(rdrop2, Arb)) ? 0 : 1;
@@ -2172,7 +2244,7 @@ void Run::reset()
 {
   HIPCHK(hipStreamSynchronize(st));
   prof.collect();
-  hAr_j = hAr_b; hZr_j = hZr_b; hAc_j = hAc_b;
+  hAr_j = hAr_j0; hZr_j = hZr_j0; hAc_j = hAc_j0;
   std::fill(hAr_rtn.begin(), hAr_rtn.end(), 0); std::fill(hAr_rmv.begin(), hAr_rmv.end(), 0);
   std::fill(hAc_rtn.begin(), hAc_rtn.end(), 0); std::fill(hAc_rmv.begin(), hAc_rmv.end(), 0);
   push_ctl();

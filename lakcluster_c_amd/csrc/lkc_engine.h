@@ -181,7 +181,10 @@ struct Run {
   std::string err;
 
   ~Run();
+  bool synthetic = false;                   // bcc_init's problem: masks and A_p drawn, not read
+  std::vector<uint8_t> hAr_j0, hZr_j0, hAc_j0;   // the working masks a run starts from
   void load(const char* in_text);
+  void load_synthetic(double t0);
   void load_job(const struct Pool& P, const int* rowsA, int nA, const int* rowsZ, int nZ, const uint8_t* colmask,
                 const Cfg& params);
   void masks_from_files();

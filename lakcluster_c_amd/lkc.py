@@ -120,6 +120,7 @@ def lib():
         "lkc_dexcluster_driver": (C.c_int, [cp, cp]),
         "lkc_dcc_sumscores_test": (C.c_int, [cp, cp]),
         "lkc_dcc_driver_bins": (C.c_int, [cp, cp]),
+        "lkc_bcc_sumscores_test": (C.c_int, [cp, cp]),
         "lkc_bcc_is_dcc": (C.c_int, [vp]),
         "lkc_bcc_active": (C.c_int, [vp]),
         "lkc_bcc_sumscores_dmp": (C.c_int, [vp, cp]),
@@ -360,6 +361,12 @@ def dcc_sumscores_test(in_text: str, out_dir: Optional[str] = None) -> None:
     """The reference's dcc self-test (dcc_sumscores_test) on dcc_init's synthetic problem from
     the GLOBAL_TEST_* keys: several bins, Y/W controls; the three output files written."""
     _check(lib().lkc_dcc_sumscores_test(in_text.encode(), out_dir.encode() if out_dir else None))
+
+
+def bcc_sumscores_test(in_text: str, out_dir: Optional[str] = None) -> None:
+    """The reference's bcc self-test (bcc_sumscores_test) on bcc_init's synthetic problem from
+    the GLOBAL_TEST_* keys (one bin, no Y/W); the three output files written."""
+    _check(lib().lkc_bcc_sumscores_test(in_text.encode(), out_dir.encode() if out_dir else None))
 
 
 def tune_set(mode: int) -> None:

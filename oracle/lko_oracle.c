@@ -504,6 +504,9 @@ typedef struct {
   /* lakcluster_scorebox inputs (updateglobals.c:136-140) */
   char test_type[LKO_PATH], sb_file[LKO_PATH];
   int sb_row_max, sb_row_num, sb_col_max, sb_col_num;
+  /* the self-tests' synthetic problem (GLOBAL_TEST_*, defaults of lakcluster_ver18.c:70-79) */
+  char test_typ2[LKO_PATH]; double test_mrand;
+  int test_A_rows, test_A_cols, test_Z_rows, test_Y_cols, test_T_cols, test_niter;
 } lko_cfg;
 
 static void trim(char *s)
@@ -523,6 +526,7 @@ static int cfg_parse(const char *text, lko_cfg *c)
 {
   memset(c, 0, sizeof(*c));
   c->B = 32; c->D_MLT = 4294967296LL; c->gamma = 0.125; c->gamma_type = 0; c->sparse = 1; c->nbins = 0;
+  c->test_A_rows = 12; c->test_A_cols = 13; c->test_Z_rows = 15; c->test_Y_cols = 11; c->test_T_cols = 3; c->test_niter = 2;
   const char *p = text;
   while (*p) {
     const char *eq = strchr(p, '='); if (!eq) break;
@@ -569,6 +573,14 @@ static int cfg_parse(const char *text, lko_cfg *c)
     else if (!strcmp(name, "GLOBAL_scramble_out_xdrop_")) snprintf(c->scr_files_raw, LKO_PATH, "%s", val);
     else if (!strcmp(name, "GLOBAL_scramble_rseed_")) snprintf(c->scr_seeds_raw, LKO_PATH, "%s", val);
     else if (!strcmp(name, "GLOBAL_TEST_TYPE")) snprintf(c->test_type, LKO_PATH, "%s", val);
+    else if (!strcmp(name, "GLOBAL_TEST_TYP2")) snprintf(c->test_typ2, LKO_PATH, "%s", first);
+    else if (!strcmp(name, "GLOBAL_TEST_mrand")) c->test_mrand = atof(val);
+    else if (!strcmp(name, "GLOBAL_TEST_A_n_rows")) c->test_A_rows = atoi(val);
+    else if (!strcmp(name, "GLOBAL_TEST_A_n_cols")) c->test_A_cols = atoi(val);
+    else if (!strcmp(name, "GLOBAL_TEST_Z_n_rows")) c->test_Z_rows = atoi(val);
+    else if (!strcmp(name, "GLOBAL_TEST_Y_n_cols")) c->test_Y_cols = atoi(val);
+    else if (!strcmp(name, "GLOBAL_TEST_T_n_cols")) c->test_T_cols = atoi(val);
+    else if (!strcmp(name, "GLOBAL_TEST_niter")) c->test_niter = atoi(val);
     else if (!strcmp(name, "GLOBAL_scorebox_out_xdrop")) snprintf(c->sb_file, LKO_PATH, "%s", first);
     else if (!strcmp(name, "GLOBAL_scorebox_row_max")) c->sb_row_max = atoi(val);
     else if (!strcmp(name, "GLOBAL_scorebox_row_num")) c->sb_row_num = atoi(val);
@@ -1377,9 +1389,13 @@ static int st_scramble(lko_state *S, lko_result *R)
   return 0;
 }
 
+static int st_init(lko_state *S, lko_result *R);
+static int st_arrays(lko_state *S, lko_result *R);
+
 static int st_load(lko_state *S, lko_result *R)
 {
   lko_cfg *c = &S->c;
+  if (!strcmp(c->test_type, "bcc_sumscores")) return st_init(S, R);
   if (c->nbins != 1) return fail(R, "only GLOBAL_NBINS= 1 is supported");
   if (c->Y_ncols != 0) return fail(R, "Y/W genetic controls are not supported");
   /* dexcluster_driver has no low-rank update and reads no QR/QC strategy (dcc.c:1167) */
@@ -1454,8 +1470,89 @@ static int st_load(lko_state *S, lko_result *R)
   lko_calc_ajdk(S->A_p, S->pcols, S->ajdk);
   /* bcc_scramble runs after bcc_load (lakcluster_driver.c:56-62): A_p is the loaded data's */
   if (c->scramble_num > 0 && st_scramble(S, R)) return -1;
-  /* xcalc orientation flags (bcc_double_init_lf, bcc.c:1005-1150) */
-  int rdrop0 = 0; lko_get_xdrop(Arb, popmask(S->Ac_b, S->M), c->gamma, c->gamma_type, &rdrop0, NULL);
+  return st_arrays(S, R);
+}
+
+/* bcc_init (bcc.c:1542-1600) for one bin without Y/W controls, GLOBAL_TEST_TYP2= error:
+   bcc_ajdk_init's column masks (bcc.c:117-174; the covariate mask with its first and last
+   column forced on), bcc_single_init_M_An's row masks and matrices (bcc.c:415-520; the
+   draws of wrap_M_setup_test_excerpt_1 / _0, An_ajdk_v.c:32-98: column by column over every
+   row, bit = draw > 1/2; the covariates' column 0 set in every row), then bcc_init_A_p's
+   block frequencies (bcc.c:1315-1330).  Draws are glibc rand() from its initial state.
+   A mask bit is on in _b when its draw exceeds mrand^2 and in _j when it exceeds mrand, so
+   rows and columns may start loaded but inactive. */
+#define LK_RAND01 ((double)rand() / (double)RAND_MAX)
+#define LK_MAXIMUM(A, B) ((A) > (B) ? (A) : (B))
+#define LK_MINIMUM(A, B) ((A) < (B) ? (A) : (B))
+static void init_mask(int n, double mrnd, uint8_t *mb, uint8_t *mj)
+{
+  for (int i = 0; i < n; i++) { double d = LK_RAND01; if (d > mrnd * mrnd) bset(mb, i); if (d > mrnd) bset(mj, i); }
+}
+static void init_bits(int nrows, int ncols, bm_t *Xn, bm_t *Xt)
+{
+  *Xn = bm_alloc(nrows, ncols); *Xt = bm_alloc(ncols, nrows);
+  for (int k = 0; k < ncols; k++) for (int r = 0; r < nrows; r++)
+    if (LK_RAND01 > 0.5) { bset(Xn->b + (size_t)r * Xn->rb, k); bset(Xt->b + (size_t)k * Xt->rb, r); }
+}
+static int st_init(lko_state *S, lko_result *R)
+{
+  lko_cfg *c = &S->c;
+  if (c->nbins != 1) return fail(R, "bcc_sumscores: only GLOBAL_NBINS= 1 is supported");
+  if (c->test_Y_cols != 0) return fail(R, "bcc_sumscores: Y/W genetic controls are not supported (GLOBAL_TEST_Y_n_cols must be 0)");
+  if (strcmp(c->test_typ2, "error")) return fail(R, "bcc_sumscores: GLOBAL_TEST_TYP2 must be error (random data)");
+  if (!strstr(c->QR, "YnWt") || !strstr(c->QC, "YnWt") || strstr(c->QR, "condense") || strstr(c->QC, "store one")
+      || strstr(c->QR, "ZtSWn") || strstr(c->QC, "ZtSWn")) return fail(R, "only the YnWt strategies are supported");
+  if (c->Ireq > 0) return fail(R, "GLOBAL_Ireq > 0 (per-bin sorting) is not supported");
+  if (c->test_A_cols < 1 || c->test_T_cols < 1 || c->test_T_cols > 8) return fail(R, "bcc_sumscores: bad problem size");
+  const double mrnd = c->test_mrand;
+  srand(1);
+  S->M = c->test_A_cols; S->T = c->test_T_cols;
+  S->NA = MAXI(0, c->test_A_rows + (0 % 3) - 1); S->NZ = MAXI(0, c->test_Z_rows + (0 % 5) - 2);
+  S->has_Z = S->NZ > 0;
+  S->pcols = lko_psize(S->M);
+  S->Ac_b = mask_alloc(S->M); S->Ac_j = mask_alloc(S->M); S->Ac_rtn = mask_alloc(S->M); S->Ac_rmv = mask_alloc(S->M);
+  uint8_t *Tc_b = mask_alloc(S->T); S->Tc_j = mask_alloc(S->T);
+  init_mask(S->M, mrnd, S->Ac_b, S->Ac_j);
+  init_mask(S->T, mrnd, Tc_b, S->Tc_j);
+  bset(Tc_b, 0); bset(S->Tc_j, 0); bset(Tc_b, S->T - 1); bset(S->Tc_j, S->T - 1);
+  const int tall = popmask(S->Tc_j, S->T) == S->T;
+  free(Tc_b);
+  S->Ar_b = mask_alloc(S->NA); S->Ar_j = mask_alloc(S->NA); S->Ar_rtn = mask_alloc(S->NA); S->Ar_rmv = mask_alloc(S->NA);
+  init_mask(S->NA, mrnd, S->Ar_b, S->Ar_j);
+  init_bits(S->NA, S->M, &S->An, &S->At);
+  S->Zr_b = mask_alloc(S->NZ); S->Zr_j = mask_alloc(S->NZ);
+  init_mask(S->NZ, mrnd, S->Zr_b, S->Zr_j);
+  init_bits(S->NZ, S->M, &S->Zn, &S->Zt);
+  /* Y and W have no columns: no draws */
+  bm_t Tn, Sn;
+  init_bits(S->NA, S->T, &Tn, &S->Tt);
+  for (int r = 0; r < S->NA; r++) { bset(Tn.b + (size_t)r * Tn.rb, 0); bset(S->Tt.b, r); }
+  init_bits(S->NZ, S->T, &Sn, &S->St);
+  for (int r = 0; r < S->NZ; r++) { bset(Sn.b + (size_t)r * Sn.rb, 0); bset(S->St.b, r); }
+  free(Tn.b); free(Sn.b);
+  /* bcc_init_A_p: per block p = maximum(0.1, minimum(0.9, rand01)), the macros drawing again
+     for the value they return; 0.5 everywhere when GLOBAL_TEST_sparse is 0 (after the draws) */
+  S->A_p = dalloc(S->pcols); S->ajdk = dalloc((size_t)S->pcols * LKO_AJDK);
+  for (int b = 0; b < S->pcols; b++) S->A_p[b] = LK_MAXIMUM(0.1, LK_MINIMUM(0.9, LK_RAND01));
+  if (c->sparse == 0) for (int b = 0; b < S->pcols; b++) S->A_p[b] = 0.5;
+  lko_calc_ajdk(S->A_p, S->pcols, S->ajdk);
+  /* the covariates this path carries: every column in use */
+  if (!tall) return fail(R, "bcc_sumscores: the drawn covariate mask leaves a column out (not supported)");
+  if (popmask(S->Ar_b, S->NA) == 0) return fail(R, "bcc_sumscores: A has no loaded rows");
+  S->Ar_bidx = (int *)xcalloc(S->NA + 1, sizeof(int)); S->Ac_bidx = (int *)xcalloc(S->M + 1, sizeof(int));
+  for (int i = 0, b = 0; i < S->NA; i++) { S->Ar_bidx[i] = b; if (bget(S->Ar_b, i)) b++; }
+  for (int i = 0, b = 0; i < S->M; i++) { S->Ac_bidx[i] = b; if (bget(S->Ac_b, i)) b++; }
+  return st_arrays(S, R);
+}
+
+/* xcalc orientation flags (bcc_double_init_lf, bcc.c:1005-1150: the first step's row drop
+   from the active counts, the sides by their loaded counts) and the score arrays */
+static int st_arrays(lko_state *S, lko_result *R)
+{
+  lko_cfg *c = &S->c;
+  (void)R;
+  int Arb = popmask(S->Ar_b, S->NA), Zrb = popmask(S->Zr_b, S->NZ);
+  int rdrop0 = 0; lko_get_xdrop(popmask(S->Ar_j, S->NA), popmask(S->Ac_j, S->M), c->gamma, c->gamma_type, &rdrop0, NULL);
   int rdrop2 = MAXI(2, rdrop0);
   S->trm_AnZt = (Zrb <= Arb) ? 0 : 1;
   { int n1b = MINI(rdrop2, Arb), n2b = Zrb; S->trm_knzt = (n2b <= n1b) ? 0 : 1; S->trm_hnvt = S->trm_knzt; }
@@ -1528,6 +1625,10 @@ static void *run_impl(const char *in_text, const char *out_dir, int max_iter, in
   if (cfg_parse(in_text, &S->c)) { fail(R, "bad .in"); free(S); return R; }
   double t0 = now_s();
   if (st_load(S, R)) { st_free(S); free(S); return R; }
+  /* bcc_sumscores_test stops after GLOBAL_TEST_niter passes when that is positive
+     (bcc_sumscores.c:815) */
+  if (!strcmp(S->c.test_type, "bcc_sumscores") && S->c.test_niter > 0 && (max_iter < 0 || max_iter > S->c.test_niter))
+    max_iter = S->c.test_niter;
   double t1 = now_s();
   const int dcc = !strcmp(S->c.test_type, "dexcluster_driver");
   /* initial loop-subscores (lakcluster_driver.c:73-106; dexcluster_driver.c:22-25) */
@@ -1805,6 +1906,7 @@ long lko_state_get(void *h, const char *name, double *out, long n)
   };
   struct { const char *nm; const uint8_t *m; int n; } mt[] = {
     { "mask_Ar_j", S->Ar_j, S->NA }, { "mask_Zr_j", S->Zr_j, S->NZ }, { "mask_Ac_j", S->Ac_j, S->M },
+    { "mask_Ar_b", S->Ar_b, S->NA }, { "mask_Zr_b", S->Zr_b, S->NZ }, { "mask_Ac_b", S->Ac_b, S->M },
   };
   /* the bit matrices as 0/1 values, row-major (after any scramble) */
   if (!strcmp(name, "An_bits") || !strcmp(name, "At_bits") || !strcmp(name, "Zn_bits") || !strcmp(name, "Tt_bits") ||

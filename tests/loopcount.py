@@ -156,7 +156,19 @@ def load_problem(d, nZ, T):
     return A, Z, Tm, Sm
 
 
-def check_run(oracle, text, d, nZ, T, kappa2, dcc=False, max_steps=None, rtol=1e-6):
+def problem_of(S, T):
+    """The bits of the problem an oracle State holds (A, Z or None, T, S or None), for
+    problems that come from no files (bcc_init's)."""
+    NA = S.get("mask_Ar_j").size
+    A = S.get("An_bits").reshape(NA, -1).astype(bool)
+    NZ = S.get("mask_Zr_j").size
+    Z = S.get("Zn_bits").reshape(NZ, -1).astype(bool) if NZ else None
+    Tm = S.get("Tt_bits").reshape(T, NA).T.astype(bool)
+    Sm = S.get("St_bits").reshape(T, NZ).T.astype(bool) if NZ else None
+    return A, Z, Tm, Sm
+
+
+def check_run(oracle, text, d, nZ, T, kappa2, dcc=False, max_steps=None, rtol=1e-6, problem=None):
     """Walks a whole oracle elimination: at every step the trace's mean row and column
     scores equal the means of the brute-force final scores, and the rows and columns the step
     drops are the lowest-scoring ones.  Tolerance: rtol of the larger of the step's largest
@@ -167,7 +179,7 @@ def check_run(oracle, text, d, nZ, T, kappa2, dcc=False, max_steps=None, rtol=1e
     Returns (steps checked, drop sets compared)."""
     run = oracle.run_text(text)
     S = oracle.State(text)
-    A, Z, Tm, Sm = load_problem(d, nZ, T)
+    A, Z, Tm, Sm = problem if problem is not None else load_problem(d, nZ, T)
     tab = S.get("ajdk").reshape(15, -1)
     counts = loop_counts if not dcc else halfloop_counts
     tr = run.trace
