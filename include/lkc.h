@@ -88,6 +88,16 @@ int lkc_bcc_sumscores_test(const char *in_text, const char *out_dir);
    (S_init.c:79-89) scorebox_r<row_max>n<row_num>_c<col_max>n<col_num>_{A_rpop_j_total,
    A_cpop_j,Irem,QR_avg,QC_avg}.mda into out_dir (NULL: GLOBAL_DIR_NAME). */
 int lkc_lakcluster_scorebox(const char *in_text, const char *out_dir);
+
+/* dexcluster_scorebox (dexcluster_scorebox.c:221-227, dispatched on GLOBAL_TEST_TYPE=
+   dexcluster_scorebox at lakcluster_ver18.c:346): the same grid replay on the dcc
+   half-loop elimination (dexcluster_scorebox_rc / _cr, :67-165), the subscores recomputed
+   from the masks at every point, and the same five .mda files.  As in the reference, the
+   inner loop's problem (D_sub) is loaded once and never copied from the outer one's: it
+   keeps its own drops across inner loops and never takes the outer loop's, and the first
+   point's drops do not take effect.  lkc_lakcluster_scorebox on a dexcluster_scorebox .in
+   runs the bcc replay, as the reference function would. */
+int lkc_dexcluster_scorebox(const char *in_text, const char *out_dir);
 /* xcc_scorebox_xdrop_array (bcc_scorebox.c:282-308): xval[x_num] index values from
    x_max down to 0, xdrop[x_num] the number of indices removed at each step */
 int lkc_scorebox_xdrop_array(int nxols, int x_max, int x_num, int *xval, int *xdrop);

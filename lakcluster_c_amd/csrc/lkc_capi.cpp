@@ -194,18 +194,25 @@ int lkc_bcc_write_scorebox(const lkc_bcc* D, const char* out_dir)
   });
 }
 
-int lkc_bcc_is_scorebox(const lkc_bcc* D) { return D && D->run.c.test_type == "lakcluster_scorebox" ? 1 : 0; }
+int lkc_bcc_is_scorebox(const lkc_bcc* D)
+{
+  return D && (D->run.c.test_type == "lakcluster_scorebox" || D->run.c.test_type == "dexcluster_scorebox") ? 1 : 0;
+}
 
-int lkc_lakcluster_scorebox(const char* in_text, const char* out_dir)
+static int scorebox_run(const char* in_text, const char* out_dir, int dcc)
 {
   lkc_bcc* D = nullptr;
   int rc = lkc_bcc_load(in_text, &D);
   if (rc) return rc;
+  D->run.dcc = dcc;
   rc = lkc_bcc_scorebox(D);
   if (!rc) rc = lkc_bcc_write_scorebox(D, out_dir);
   lkc_bcc_free(D);
   return rc;
 }
+
+int lkc_lakcluster_scorebox(const char* in_text, const char* out_dir) { return scorebox_run(in_text, out_dir, 0); }
+int lkc_dexcluster_scorebox(const char* in_text, const char* out_dir) { return scorebox_run(in_text, out_dir, 1); }
 
 int lkc_scorebox_xdrop_array(int nxols, int x_max, int x_num, int* xval, int* xdrop)
 {

@@ -501,7 +501,7 @@ void Run::load(const char* in_text)
   if (c.Y_ncols != 0 || y_files) throw Error(LKC_ERR_CONFIG, "Y/W genetic controls are not supported here (the file-driven dexcluster_driver takes them)");
   // GLOBAL_TEST_TYPE selects the elimination (lakcluster_ver18.c:337-345); the dcc one has
   // no low-rank update, so no QR/QC strategy
-  dcc = c.test_type == "dexcluster_driver" ? 1 : 0;
+  dcc = (c.test_type == "dexcluster_driver" || c.test_type == "dexcluster_scorebox") ? 1 : 0;
   if (!dcc && (c.QR.find("YnWt") == std::string::npos || c.QC.find("YnWt") == std::string::npos ||
                c.QR.find("condense") != std::string::npos || c.QC.find("store one") != std::string::npos ||
                c.QR.find("ZtSWn") != std::string::npos || c.QC.find("ZtSWn") != std::string::npos))
@@ -2119,6 +2119,7 @@ void Run::scorebox(Scorebox& S)
   S.rpop.assign(ng, 0); S.cpop.assign(ng, 0); S.irem.assign(ng, 0); S.qr.assign(ng, 0.0); S.qc.assign(ng, 0.0);
   const bool rows_outer = !(S.row_max > S.col_max);     // lakcluster_scorebox_cr, else _rc
   const int n_out = rows_outer ? row_num : col_num, n_in = rows_outer ? col_num : row_num;
+  if (dcc) { scorebox_dcc(S, rord, cord, rdrop, cdrop, rows_outer); return; }
   // everything an update reads or accumulates into
   size_t nA = NA, nZ = std::max(NZ, 1), nM = M, nT = T;
   std::vector<std::pair<double*, size_t>> keep = {
@@ -2171,6 +2172,51 @@ void Run::scorebox(Scorebox& S)
   HIPCHK(hipGetLastError());
   prof.collect();
   for (double* p : snap) dev.release(p);
+  inited = true;
+  loop_s = now_s() - t0 - init_s;
+}
+
+// dexcluster_scorebox_rc / _cr (dexcluster_scorebox.c:67-165).  The dcc subscores are
+// recomputed from the masks at every grid point, so a problem state is its masks: D_ori
+// along the outer loop, D_sub along the inner one.  D_sub starts from the loaded masks and is
+// never copied from D_ori, so it keeps its own drops from one inner loop to the next and never
+// takes the outer loop's.  The first point runs dexcluster_scorebox_excerpt_1, which computes
+// before its drop takes effect (no dcc_lrup_mxdup): that drop is lost.
+void Run::scorebox_dcc(Scorebox& S, const std::vector<int>& rord, const std::vector<int>& cord,
+                       const std::vector<int>& rdrop, const std::vector<int>& cdrop, bool rows_outer)
+{
+  const int row_num = S.row_num, col_num = S.col_num;
+  const int n_out = rows_outer ? row_num : col_num, n_in = rows_outer ? col_num : row_num;
+  struct Masks { std::vector<uint8_t> ar, ac; };
+  auto get = [&]() { return Masks{ hAr_j, hAc_j }; };
+  auto set = [&](const Masks& m) { hAr_j = m.ar; hAc_j = m.ac; push_ctl(); };
+  auto compute = [&]() { st_ajdk(); st_ZtSn(); st_dcc(); };
+  const Masks loaded = get();
+  Masks ori = loaded, sub = loaded;
+  double t0 = now_s();
+  for (int o = 0; o < n_out; o++) {
+    set(ori);
+    if (o == 0) {
+      sb_drop(rord, cord, rdrop[0], cdrop[0]);   // checks the order; excerpt_1 drops nothing
+      set(ori);
+    } else {
+      sb_drop(rord, cord, rows_outer ? rdrop[o] : 0, rows_outer ? 0 : cdrop[o]);
+    }
+    compute();
+    if (o == 0) init_s = now_s() - t0;
+    sb_record(S, rows_outer ? o : o * row_num);
+    ori = get();
+    set(sub);
+    for (int i = 1; i < n_in; i++) {
+      sb_drop(rord, cord, rows_outer ? 0 : rdrop[i], rows_outer ? cdrop[i] : 0);
+      compute();
+      sb_record(S, rows_outer ? o + i * row_num : i + o * row_num);
+    }
+    sub = get();
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipGetLastError());
+  prof.collect();
   inited = true;
   loop_s = now_s() - t0 - init_s;
 }

@@ -218,6 +218,8 @@ struct Run {
   void sb_drop(const std::vector<int>& rord, const std::vector<int>& cord, int rdrop, int cdrop);
   void sb_record(Scorebox& S, int tab);
   void scorebox(Scorebox& S);
+  void scorebox_dcc(Scorebox& S, const std::vector<int>& rord, const std::vector<int>& cord,
+                    const std::vector<int>& rdrop, const std::vector<int>& cdrop, bool rows_outer);
   void gram_any(const GramMArgs& g);
   bool active() const;
   void iterate(int* done);

@@ -112,6 +112,7 @@ def lib():
         "lkc_batch_results_free": (None, [C.POINTER(JobResult), C.c_int]),
         "lkc_job_bytes": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_int]),
         "lkc_lakcluster_scorebox": (C.c_int, [cp, cp]),
+        "lkc_dexcluster_scorebox": (C.c_int, [cp, cp]),
         "lkc_scorebox_xdrop_array": (C.c_int, [C.c_int, C.c_int, C.c_int, ip, ip]),
         "lkc_bcc_scorebox": (C.c_int, [vp]),
         "lkc_bcc_get_scorebox": (C.c_int, [vp, ip, ip, ip, ip, dp, dp]),
@@ -319,6 +320,11 @@ class Scorebox:
 def lakcluster_scorebox(in_text: str, out_dir: Optional[str] = None) -> None:
     """Whole scorebox run with its five .mda files written (lakcluster_scorebox)."""
     _check(lib().lkc_lakcluster_scorebox(in_text.encode(), out_dir.encode() if out_dir else None))
+
+
+def dexcluster_scorebox(in_text: str, out_dir: Optional[str] = None) -> None:
+    """Whole dcc scorebox run with its five .mda files written (dexcluster_scorebox)."""
+    _check(lib().lkc_dexcluster_scorebox(in_text.encode(), out_dir.encode() if out_dir else None))
 
 
 def scorebox_xdrop_array(nxols: int, x_max: int, x_num: int) -> Tuple[np.ndarray, np.ndarray]:

@@ -1399,7 +1399,7 @@ static int st_load(lko_state *S, lko_result *R)
   if (c->nbins != 1) return fail(R, "only GLOBAL_NBINS= 1 is supported");
   if (c->Y_ncols != 0) return fail(R, "Y/W genetic controls are not supported");
   /* dexcluster_driver has no low-rank update and reads no QR/QC strategy (dcc.c:1167) */
-  int dcc = !strcmp(c->test_type, "dexcluster_driver");
+  int dcc = !strcmp(c->test_type, "dexcluster_driver") || !strcmp(c->test_type, "dexcluster_scorebox");
   if (!dcc && (!strstr(c->QR, "YnWt") || !strstr(c->QC, "YnWt") || strstr(c->QR, "condense") || strstr(c->QC, "store one")
                || strstr(c->QR, "ZtSWn") || strstr(c->QC, "ZtSWn"))) return fail(R, "only the YnWt strategies are supported");
   if (c->Ireq > 0) return fail(R, "GLOBAL_Ireq > 0 (per-bin sorting) is not supported");
@@ -1741,9 +1741,9 @@ static int sb_drop(lko_state *S, const int *rord, int nro, const int *cord, int 
 }
 /* lakcluster_scorebox_excerpt_0 with bcc_scorebox_svalue: scores of the active rows and
    columns in index order, first covariate column */
-static void sb_record(lko_state *S, lko_sbout *B, int tab)
+static void sb_record_x(lko_state *S, lko_sbout *B, int tab, int dcc)
 {
-  st_nrm_ifT_cmb(S);
+  st_nrm_ifT_cmb_x(S, dcc);
   ilist ar = il_from_mask(S->Ar_j, S->NA), kl = il_from_mask(S->Ac_j, S->M);
   double *v = dalloc((size_t)MAXI(ar.n, kl.n));
   for (int i = 0; i < ar.n; i++) v[i] = S->QR_TAT_nrm[(size_t)ar.ix[i] * S->T + 0];
@@ -1754,6 +1754,15 @@ static void sb_record(lko_state *S, lko_sbout *B, int tab)
   B->qc[tab] = m;
   B->rpop[tab] = ar.n; B->cpop[tab] = kl.n; B->irem[tab] = ar.n > 0 ? 1 : 0;
   free(v); free(ar.ix); free(kl.ix);
+}
+static void sb_record(lko_state *S, lko_sbout *B, int tab) { sb_record_x(S, B, tab, 0); }
+/* dexcluster_scorebox_excerpt_1 / _2 (dexcluster_scorebox.c:21-65): the dcc subscores from
+   scratch, after dcc_lrup_mxdup + dcc_M_mxset (the drop of the preceding mxA taking effect)
+   only in _2 */
+static void sb_dcc(lko_state *S, int dup)
+{
+  if (dup) { memcpy(S->Ar_j, S->Ar_rtn, lko_bsize(S->NA)); memcpy(S->Ac_j, S->Ac_rtn, lko_bsize(S->M)); }
+  st_ajdk(S); st_ZtSn(S); st_dcc(S);
 }
 static void sb_full(lko_state *S)
 {
@@ -1820,7 +1829,24 @@ static int sb_run(const char *in_text, lko_sbout *B, char *err, int nerr, int *d
     const int rows_outer = !(c->sb_row_max > c->sb_col_max);   /* lakcluster_scorebox_cr, else _rc */
     const int n_out = rows_outer ? row_num : col_num, n_in = rows_outer ? col_num : row_num;
     int bad = 0;
-    for (int o = 0; o < n_out && !bad; o++) {
+    /* dexcluster_scorebox_rc / _cr (dexcluster_scorebox.c:67-165): D_ori along the outer loop,
+       D_sub along the inner one; D_sub is loaded once and never copied from D_ori, so it
+       carries its own drops from one inner loop to the next (and never the outer loop's);
+       the first point's drops (excerpt_1) do not take effect */
+    const int dcc = !strcmp(c->test_type, "dexcluster_scorebox");
+    for (int o = 0; o < n_out && !bad && dcc; o++) {
+      const int r_o = o == 0 ? rdrop[0] : (rows_outer ? rdrop[o] : 0), c_o = o == 0 ? cdrop[0] : (rows_outer ? 0 : cdrop[o]);
+      if (sb_drop(S, rord, nro, cord, nco, r_o, c_o)) { bad = 1; break; }
+      sb_dcc(S, o > 0);
+      sb_record_x(S, B, rows_outer ? o : o * row_num, 1);
+      for (int i = 1; i < n_in; i++) {
+        const int r_i = rows_outer ? 0 : rdrop[i], c_i = rows_outer ? cdrop[i] : 0;
+        if (sb_drop(U, rord, nro, cord, nco, r_i, c_i)) { bad = 1; break; }
+        sb_dcc(U, 1);
+        sb_record_x(U, B, rows_outer ? o + i * row_num : i + o * row_num, 1);
+      }
+    }
+    for (int o = 0; o < n_out && !bad && !dcc; o++) {
       const int r_o = rows_outer ? rdrop[o] : 0, c_o = rows_outer ? 0 : cdrop[o];
       if (o == 0) {
         if (sb_drop(S, rord, nro, cord, nco, rdrop[0], cdrop[0])) { bad = 1; break; }
