@@ -225,6 +225,32 @@ def dcc_leg(text, max_iter=-1):
     return out
 
 
+def dcc_bins_leg(in_path, kw, root, nbins=4, ny=8192):
+    """The dcc elimination with several bins and Y/W genetic controls: the same patients
+    split into nbins bins with ny Y/W columns added, through lkc_dcc_driver_bins (the
+    whole call: .b16 files read and uploaded, every pass, the three files written); the
+    one-bin split through the same entry beside it.  One untimed call each first."""
+    from lakcluster_c_amd import gen, lkc
+    ds = gen.Dataset(in_path, os.path.join(root, "unused_out"), kw["nA"], kw["nZ"], kw["ncols"], kw["T"],
+                     kw["gamma"], kw["B"])
+    out = {"workload": f"{kw['nA']}+{kw['nZ']}x{kw['ncols']} split into {nbins} bins, {ny} Y/W columns",
+           "includes": "file load, upload, every pass, output files"}
+    for tag, nb, y in (("bins", nbins, ny), ("one_bin", 1, 0)):
+        t0 = time.time()
+        bds = gen.split_into_bins(ds, root, f"dccbins_{tag}", nb, ny=y, seed=7)
+        text = open(bds.in_path).read()
+        gen_s = time.time() - t0
+        odir = os.path.join(root, f"dccbins_{tag}_out")
+        lkc.dcc_driver_bins(text, odir)
+        t0 = time.perf_counter()
+        lkc.dcc_driver_bins(text, odir)
+        el = time.perf_counter() - t0
+        passes = sum(1 for _ in open(os.path.join(odir, "out_trace.txt")))
+        out[tag] = {"call_s": el, "passes": passes, "bins": nb, "Y_cols": y}
+        print(f"[bench] dcc {tag}: {el:.3f} s, {passes} passes (split in {gen_s:.1f} s)", file=sys.stderr, flush=True)
+    return out
+
+
 def large_leg(rows=65536, cols=524288, passes=20, gamma=0.01, B=24, seed=1, check_valu=False, repeat=False):
     """The large single run (A and Z rows x cols each, p_j ~ U[0.01, 0.5], biclusters
     of rows**0.5, rows**0.55 and rows**0.6 rows planted in A, T/S 3 columns, gamma 0.01,
@@ -364,6 +390,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-null and serving legs")
     ap.add_argument("--no-dcc", action="store_true", help="skip the dcc half-loop elimination leg")
     ap.add_argument("--dcc-only", action="store_true", help="run only the dcc leg and print its JSON object")
+    ap.add_argument("--dcc-bins-only", action="store_true",
+                    help="run only the several-bin dcc leg (4 bins, 8192 Y/W columns) and print its JSON object")
     ap.add_argument("--large", type=int, nargs=2, metavar=("ROWS", "COLS"),
                     help="run only the large single-run leg (e.g. 65536 524288) for --max-iter passes "
                          "(default: to the end) and print its JSON object")
@@ -419,6 +447,9 @@ def main():
     text = open(in_path).read()
     if args.dcc_only:
         print(json.dumps({"workload": args.config, "dcc": dcc_leg(text, args.max_iter)}), flush=True)
+        return
+    if args.dcc_bins_only:
+        print(json.dumps({"workload": args.config, "dcc_bins": dcc_bins_leg(in_path, kw, args.data_root)}), flush=True)
         return
     print("[bench] loading ...", file=sys.stderr, flush=True)
     run = lkc.Bcc(text)
@@ -499,6 +530,9 @@ def main():
     if not args.no_dcc:
         print("[bench] dcc half-loop elimination ...", file=sys.stderr, flush=True)
         line["dcc"] = dcc_leg(text, args.max_iter)
+        if args.max_iter < 0 and kw.get("nZ", 0) > 0:
+            print("[bench] dcc with several bins and Y/W controls ...", file=sys.stderr, flush=True)
+            line["dcc_bins"] = dcc_bins_leg(in_path, kw, args.data_root)
     if not args.no_batch:
         print("[bench] batched nulls and serving ...", file=sys.stderr, flush=True)
         line.update(batched_legs(n_null=args.null_runs, c2_tbitops=value))

@@ -74,7 +74,9 @@ struct DccDev {
   double* QCn[4] = {};                                       // [nbx*T*X + t*X + k]
   double *L = nullptr, *xwork = nullptr, *dsv = nullptr;
   int* cnt = nullptr;
-  Grow g_ztsn;
+  std::vector<FillJob> pass_fills;                           // zeroed at the start of every pass
+  std::vector<void*> prep;                                   // digit planes per (nb2, term)
+  size_t prep_bytes = 0;
   int A_nrows_total = 0, A_rpop_b_total = 0, Z_rpop_b_total = 0, A_rpop_j_total = 0, Z_rpop_j_total = 0, A_cpop_j = 0, Irem = 0;
   std::vector<double> QC_sv, QR_sv; std::vector<int> QC_perm, QR_perm, QR_bin, QR_row;
   DccTestOut* out = nullptr;
@@ -306,6 +308,29 @@ void DccDev::setup()
     }
   }
   for (int q = 0; q < 4; q++) { QCn[q] = dbl((size_t)nbins * nbins * T * X(q)); fill(QCn[q], (long long)nbins * nbins * T * X(q), 0.0, st); }
+  // what every pass zeroes: the pair terms, the bins' scalar sums, and the normalised block of
+  // each pair whose term is on (dcc_sumscores_nrm zeroes only those, right before it writes
+  // them; a block of a term that is off keeps what the in-place bin sums left there)
+  pass_fills.clear();
+  for (int nb1 = 0; nb1 < nbins; nb1++) for (int nb2 = 0; nb2 < nbins; nb2++) {
+    Pair& f = F[nb1 + nb2 * nbins]; Bin& e1 = E[nb1];
+    for (int q = 0; q < 4; q++) {
+      pass_fills.push_back(FillJob{ f.qr[q], (long long)T * e1.nA });
+      pass_fills.push_back(FillJob{ f.qc[q], (long long)T * X(q) });
+      if (!on(q, nb1, nb2)) continue;
+      pass_fills.push_back(FillJob{ e1.QRn[q] + (size_t)nb2 * T * e1.nA, (long long)T * e1.nA });
+      pass_fills.push_back(FillJob{ QCn[q] + (size_t)(nb1 + nb2 * nbins) * T * X(q), (long long)T * X(q) });
+    }
+  }
+  for (Bin& e : E) pass_fills.push_back(FillJob{ e.sc, 6LL * T });
+  // the digit planes of each bin's quantised products, shared by the first bins' row sets
+  int max_nA = 0; for (const Bin& e : E) max_nA = std::max(max_nA, e.nA);
+  // (room for the row partials of any row count up to the largest bin's: how many parts a
+  // contraction splits into grows as its row count falls)
+  prep_bytes = std::max(ztsn_mfma_work_bytes(0, T, bsize(M), B), ztsn_mfma_work_bytes(0, T, bsize(NY), B)) +
+               sizeof(unsigned long long) * 34 * (size_t)std::max(max_nA, 1) + 512;
+  prep.assign((size_t)nbins * 4, nullptr);
+  for (auto& p : prep) p = dev.alloc<uint8_t>(prep_bytes);
   L = dbl((size_t)T * std::max(M, NY)); xwork = dbl(768);
   dsv = dbl((size_t)M + A_nrows_total);
   cnt = dev.alloc<int>(std::max(std::max(M, NY), T) + 1);
@@ -376,6 +401,7 @@ void DccDev::mxset()
 void DccDev::subscores()
 {
   const int nt = tl.n(), nk = kl.n(), ny = yl.n();
+  fill_jobs(pass_fills, st);
   for (Bin& e : E) {
     // dcc_An_ajdk (dcc_lf.c:7-51)
     if (e.nA > 0 && M > 0) an_ajdk(e.An.b(), e.ar.d, e.ar.n(), Ac_j, Apc, A_ajdk, mlfA, e.lfA, e.nA, st);
@@ -395,7 +421,6 @@ void DccDev::subscores()
     ops += (long long)(nk + ny) * nt * ((long long)bsize(e.nA) + (long long)bsize(e.nZ)) * 8;
     // TAnZtS stage 0 (TAnZtS.c:5-209): et_Tn, et_Sn, the columns' et over the active rows,
     // and the ordered a1d1 sums
-    fill(e.sc, 6LL * T, 0.0, st);
     popc_rows(e.Tt.b(), tl.d, nt, e.Ar_j, (int)(e.Tt.rb / 4), cnt, st);
     et_from_counts(cnt, tl.d, nt, e.ar.n(), e.sc + 4 * T, 1, st);
     popc_rows(e.St.b(), tl.d, nt, e.Zr_j, (int)(e.St.rb / 4), cnt, st);
@@ -427,22 +452,29 @@ void DccDev::subscores()
       ops += (long long)n * nt * (B + 1) * 8;
     }
   }
+  // the digit planes of each second bin's quantised products, once per pass
+  for (int nb2 = 0; nb2 < nbins; nb2++) for (int q = 0; q < 4; q++) {
+    const Bin& e2 = E[nb2];
+    const IList& l = (q == YW || q == YY) ? yl : kl;
+    const bool used = (q == YW || q == AZ) ? e2.nZ > 0 : e2.nA > 0;
+    if (!used || nt <= 0 || l.n() <= 0 || X(q) <= 0) continue;
+    const size_t rb = bsize(X(q));
+    HIPCHK(ztsn_mfma_prep(rb, (q == YW || q == YY) ? Yc_j : Ac_j, e2.cz[q], X(q), X(q), nt, B, prep[(size_t)nb2 * 4 + q],
+                          prep_bytes, st));
+  }
+  std::vector<PairQC> qcj; std::vector<PairQR> qrj;
   for (int nb1 = 0; nb1 < nbins; nb1++) for (int nb2 = 0; nb2 < nbins; nb2++) {
     Bin &e1 = E[nb1], &e2 = E[nb2];
     Pair& f = F[nb1 + nb2 * nbins];
     const int na = e1.ar.n();
     const bool same = nb1 == nb2;
-    for (int q = 0; q < 4; q++) {
-      fill(f.qr[q], (long long)T * e1.nA, 0.0, st);
-      fill(f.qc[q], (long long)T * X(q), 0.0, st);
-    }
     // dcc_lf_D_AtTn_ZtSn_vv (dcc_lf.c:102-163) into the half-loop's QC arrays
     auto qc = [&](int q, const IList& l, long long ld, const double* L1, const double* L2, const double* etS,
                   const double* etA, const double* ajdk, int pc) {
       if (!on(q, nb1, nb2)) return;
       PairQC a{ l.d, l.n(), tl.d, nt, ld, L1, L2, e1.sc + 4 * T, etS, etA, na, ajdk, pc, {}, f.qc[q] };
       std::copy(tb, tb + kMaxT, a.tb);
-      pair_qc(a, st);
+      qcj.push_back(a);
     };
     qc(YW, yl, NY, e1.YtTn, e2.WtSn, e2.sc + 5 * T, nullptr, Y_ajdk, Ypc);
     qc(YY, yl, NY, e1.YtTn, e2.YtTn, e2.sc + 4 * T, same ? e1.etY : nullptr, Y_ajdk, Ypc);
@@ -453,19 +485,21 @@ void DccDev::subscores()
     auto qr = [&](int q, const DMat& Xn, const uint32_t* cmask, const IList& l, int ld, const double* a1,
                   const double* et, const double* x6, const double* mlf, const DMat* Tt) {
       if (!on(q, nb1, nb2) || na <= 0 || nt <= 0 || l.n() <= 0) return;
-      const size_t wb = ztsn_mfma_work_bytes(na, nt, Xn.rb, B);
-      HIPCHK(ztsn_mfma(Xn.b(), e1.ar.d, na, cmask, e2.cz[q], ld, ld, nt, tl.d, B, e2.scal[q], f.x[q], e1.nA,
-                       g_ztsn.need(dev, wb), wb, st));
+      (void)ld;
+      HIPCHK(ztsn_mfma_run(Xn.b(), e1.ar.d, na, cmask, nt, tl.d, B, e2.scal[q], f.x[q], e1.nA,
+                           prep[(size_t)nb2 * 4 + q], prep_bytes, st));
       ops += (long long)na * nt * (B + 1) * (long long)Xn.rb * 8;
       PairQR a{ e1.ar.d, na, tl.d, nt, e1.nA, f.x[q], a1, et, x6, mlf, Tt ? Tt->p : nullptr, Tt ? Tt->rb : 0, {}, f.qr[q] };
       std::copy(tb, tb + kMaxT, a.tb);
-      pair_qr(a, st);
+      qrj.push_back(a);
     };
     qr(YW, e1.Yn, Yc_j, yl, NY, e2.sc + 3 * T, e2.sc + 5 * T, e1.lfY + 6 * (size_t)e1.nA, mlfY, nullptr);
     qr(AZ, e1.An, Ac_j, kl, M, e2.sc + 1 * T, e2.sc + 5 * T, e1.lfA + 6 * (size_t)e1.nA, mlfA, nullptr);
     qr(YY, e1.Yn, Yc_j, yl, NY, e2.sc + 2 * T, e2.sc + 4 * T, e1.lfY + 6 * (size_t)e1.nA, mlfY, same ? &e1.Tt : nullptr);
     qr(AA, e1.An, Ac_j, kl, M, e2.sc + 0 * T, e2.sc + 4 * T, e1.lfA + 6 * (size_t)e1.nA, mlfA, same ? &e1.Tt : nullptr);
   }
+  pair_qc_jobs(qcj, st);
+  pair_qr_jobs(qrj, st);
 }
 
 // dcc_sumscores_nrm with GLOBAL_Ireq <= 0 (dcc_sumscores.c:569-699): each bin pair's terms over
@@ -477,19 +511,20 @@ void DccDev::nrm()
   auto den = [](long long lld) { double d = (double)lld; if (fabs(d) < 0.000000000001) d = 1; return d; };
   const long long lldr[4] = { Zt * Yc, Zt * Ac, (At - 1) * Yc, (At - 1) * Ac };
   const long long lldc[4] = { At * Zt, At * Zt, At * (At - 1), At * (At - 1) };
+  // (the blocks of the terms that are on were zeroed at the start of the pass)
+  std::vector<NrmJob> jobs;
   for (int nb1 = 0; nb1 < nbins; nb1++) for (int nb2 = 0; nb2 < nbins; nb2++) {
     const int nbx = nb1 + nb2 * nbins; Bin& e = E[nb1]; Pair& f = F[nbx];
     for (int q = 0; q < 4; q++) {
       if (!on(q, nb1, nb2)) continue;
       double* dr = e.QRn[q] + (size_t)nb2 * T * e.nA;
-      fill(dr, (long long)T * e.nA, 0.0, st);
-      score_nrm(ScoreStage{ e.ar.d, e.ar.n(), tl.d, tl.n(), e.nA }, f.qr[q], den(lldr[q]), dr, st);
+      jobs.push_back(NrmJob{ ScoreStage{ e.ar.d, e.ar.n(), tl.d, tl.n(), e.nA }, f.qr[q], den(lldr[q]), dr });
       const IList& l = (q == YW || q == YY) ? yl : kl;
       double* dc = QCn[q] + (size_t)nbx * T * X(q);
-      fill(dc, (long long)T * X(q), 0.0, st);
-      score_nrm(ScoreStage{ l.d, l.n(), tl.d, tl.n(), X(q) }, f.qc[q], den(lldc[q]), dc, st);
+      jobs.push_back(NrmJob{ ScoreStage{ l.d, l.n(), tl.d, tl.n(), X(q) }, f.qc[q], den(lldc[q]), dc });
     }
   }
+  nrm_jobs(jobs, st);
   for (int nb1 = 0; nb1 < nbins; nb1++) {
     Bin& e = E[nb1];
     if (e.nA <= 0) continue;

@@ -3,6 +3,7 @@
 // dcc_sumscores_test on its synthetic problem and dexcluster_driver on files.
 #pragma once
 #include "lkc_kernels.h"
+#include "lkc_stages.h"
 #include <string>
 #include <vector>
 
@@ -32,6 +33,19 @@ struct PairQR {
   double* out;
 };
 void pair_qr(const PairQR& a, hipStream_t st);
+
+// the same terms for many bin pairs, a few launches each (kernel-argument sized groups)
+struct PairQCJobs { static constexpr int kMax = 24; int n; PairQC job[kMax]; };
+struct PairQRJobs { static constexpr int kMax = 24; int n; PairQR job[kMax]; };
+void pair_qc_jobs(const std::vector<PairQC>& v, hipStream_t st);
+void pair_qr_jobs(const std::vector<PairQR>& v, hipStream_t st);
+// dcc_sumscores_nrm's divisions, and zero fills, of many arrays per launch
+struct NrmJob { ScoreStage s; const double* src; double den; double* dst; };
+struct NrmJobs { static constexpr int kMax = 48; int n; NrmJob job[kMax]; };
+void nrm_jobs(const std::vector<NrmJob>& v, hipStream_t st);
+struct FillJob { double* p; long long n; };
+struct FillJobs { static constexpr int kMax = 192; int n; FillJob job[kMax]; };
+void fill_jobs(const std::vector<FillJob>& v, hipStream_t st);
 
 void bins_sumx(const int* list, int n, const int* tl, int nt, long long ld, double* x, long long stride, int nsum,
                hipStream_t st);

@@ -114,6 +114,12 @@ size_t ztsn_mfma_work_bytes(int na, int ns, size_t prb, int B);
 hipError_t ztsn_mfma(const Bits& A, const int* alist, int na, const uint32_t* mask, const uint32_t* cz, long long ldc,
                int n_total, int ns, const int* slist, int B, const double* scal, double* out, long long ldo,
                void* work, size_t work_bytes, hipStream_t st);
+// ztsn_mfma in two halves: the digit planes of cz (shared by every row set contracted with
+// the same cz, mask and row width) and the contraction of one row set with them
+hipError_t ztsn_mfma_prep(size_t rb, const uint32_t* mask, const uint32_t* cz, long long ldc, int n_total, int ns, int B,
+                          void* work, size_t work_bytes, hipStream_t st);
+hipError_t ztsn_mfma_run(const Bits& A, const int* alist, int na, const uint32_t* mask, int ns, const int* slist, int B,
+                         const double* scal, double* out, long long ldo, void* work, size_t work_bytes, hipStream_t st);
 
 // AnZt_S_WnYt_vv with X==Y: out[tlist[t]*ldo + a] = Σ_z (X(a,z)*vS(z))*X(a,z) in list
 // order of z, with X(a,z) = X[z*ldx + a] (x_rows 0) or X[a*ldx + z] (x_rows 1).

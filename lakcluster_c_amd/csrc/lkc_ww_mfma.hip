@@ -802,9 +802,28 @@ size_t ztsn_mfma_work_bytes(int na, int ns, size_t prb, int B)
   return b;
 }
 
-hipError_t ztsn_mfma(const Bits& A, const int* alist, int na, const uint32_t* mask, const uint32_t* cz, long long ldc,
-               int n_total, int ns, const int* slist, int B, const double* scal, double* out, long long ldo,
-               void* work, size_t work_bytes, hipStream_t st)
+// The digit planes of cz and their column sums (the first half of ztsn_mfma): they depend on
+// cz, its mask and the row width only, so several row sets may share them.
+hipError_t ztsn_mfma_prep(size_t rb, const uint32_t* mask, const uint32_t* cz, long long ldc, int n_total, int ns, int B,
+                          void* work, size_t work_bytes, hipStream_t st)
+{
+  if (ns <= 0) return hipSuccess;
+  const int ndig = (B + 7) / 8;
+  const long long ldn = (long long)rb * 8;
+  if (ndig * ns > 32 || work_bytes < ztsn_mfma_work_bytes(0, ns, rb, B)) return hipErrorInvalidValue;
+  int8_t* Dg = (int8_t*)work;
+  long long* Gsum = (long long*)((uint8_t*)work + (((size_t)ndig * ns * ldn + 255) & ~(size_t)255));
+  const int nt_ = n_total < ldn ? n_total : (int)ldn;
+  const long long tot = (long long)ns * (ldn / 4);
+  hipLaunchKernelGGL(k_ww_digits, dim3(cdiv(tot, 256) < 65536 ? cdiv(tot, 256) : 65536), dim3(256), 0, st,
+                     cz, ldc, ns, ns, nt_, (const uint8_t*)mask, ndig, ldn, Dg);
+  hipLaunchKernelGGL(k_cz_sums, dim3(ZT_PARTS, ns), dim3(256), 0, st, cz, ldc, nt_, mask, Gsum);
+  return hipGetLastError();
+}
+
+// The contraction of the listed rows of A with digit planes ztsn_mfma_prep left in work.
+hipError_t ztsn_mfma_run(const Bits& A, const int* alist, int na, const uint32_t* mask, int ns, const int* slist, int B,
+                         const double* scal, double* out, long long ldo, void* work, size_t work_bytes, hipStream_t st)
 {
   if (na <= 0 || ns <= 0) return hipSuccess;
   const int ndig = (B + 7) / 8;
@@ -814,11 +833,6 @@ hipError_t ztsn_mfma(const Bits& A, const int* alist, int na, const uint32_t* ma
   long long* Gsum = (long long*)((uint8_t*)work + (((size_t)ndig * ns * ldn + 255) & ~(size_t)255));
   unsigned long long* Pg = (unsigned long long*)((((uintptr_t)(Gsum + (size_t)ns * ZT_PARTS + 8)) + 255) & ~(uintptr_t)255);
   const int parts = ztsn_parts(na, A.rb);
-  const int nt_ = n_total < ldn ? n_total : (int)ldn;
-  const long long tot = (long long)ns * (ldn / 4);
-  hipLaunchKernelGGL(k_ww_digits, dim3(cdiv(tot, 256) < 65536 ? cdiv(tot, 256) : 65536), dim3(256), 0, st,
-                     cz, ldc, ns, ns, nt_, (const uint8_t*)mask, ndig, ldn, Dg);
-  hipLaunchKernelGGL(k_cz_sums, dim3(ZT_PARTS, ns), dim3(256), 0, st, cz, ldc, nt_, mask, Gsum);
   if (parts > 1) {
     const long long np = 34LL * na;
     hipLaunchKernelGGL(k_ztsn_zero, dim3(cdiv(np, 256) < 65536 ? cdiv(np, 256) : 65536), dim3(256), 0, st, Pg, np);
@@ -834,6 +848,17 @@ hipError_t ztsn_mfma(const Bits& A, const int* alist, int na, const uint32_t* ma
   switch (ndig) { case 1: ZT_LAUNCH(1); break; case 2: ZT_LAUNCH(2); break; case 3: ZT_LAUNCH(3); break; default: ZT_LAUNCH(4); break; }
 #undef ZT_LAUNCH
   return hipGetLastError();
+}
+
+hipError_t ztsn_mfma(const Bits& A, const int* alist, int na, const uint32_t* mask, const uint32_t* cz, long long ldc,
+               int n_total, int ns, const int* slist, int B, const double* scal, double* out, long long ldo,
+               void* work, size_t work_bytes, hipStream_t st)
+{
+  if (na <= 0 || ns <= 0) return hipSuccess;
+  if ((B + 7) / 8 * ns > 32 || work_bytes < ztsn_mfma_work_bytes(na, ns, A.rb, B)) return hipErrorInvalidValue;
+  hipError_t e = ztsn_mfma_prep(A.rb, mask, cz, ldc, n_total, ns, B, work, work_bytes, st);
+  if (e != hipSuccess) return e;
+  return ztsn_mfma_run(A, alist, na, mask, ns, slist, B, scal, out, ldo, work, work_bytes, st);
 }
 
 }  // namespace lkc

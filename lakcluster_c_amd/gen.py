@@ -302,6 +302,83 @@ def make_bins_dataset(root: str, name: str, bins: Sequence[Tuple[int, int]], nco
     return Dataset(in_path, d_out, nA, nZ, ncols, T, gamma, B, prow, pcol)
 
 
+def split_into_bins(ds: Dataset, root: str, name: str, nbins: int, ny: int = 0, seed: int = 1,
+                    pmin: float = 0.05, pmax: float = 0.5) -> Dataset:
+    """The patients of a make_dataset problem split into nbins equal bins for
+    dexcluster_driver, with ny Y/W genetic-control columns added (Bernoulli columns with
+    p ~ U[pmin, pmax], drawn for every case and control row).
+
+    Bin b takes the next block of case rows and of control rows.  The .b16 files are cut by
+    byte slicing (a row block of a column-by-column file is a byte range of every column),
+    so the rows per bin must be a multiple of 16 in cases and controls."""
+    d_src = os.path.dirname(ds.in_path)
+    nA, nZ = ds.nA, ds.nZ
+    if nA % (nbins * BITJ) or nZ % (nbins * BITJ):
+        raise ValueError("rows per bin must be a multiple of 16")
+    d_in = os.path.join(root, f"dir_{name}_in")
+    d_out = os.path.join(root, f"dir_{name}_out")
+    os.makedirs(d_in, exist_ok=True)
+    os.makedirs(d_out, exist_ok=True)
+
+    def cut(key, rows, b):
+        nb = rows // nbins
+        r0 = b * nb
+        for suffix, patients_are_rows in (("n", True), ("t", False)):
+            src = os.path.join(d_src, f"{key}_{suffix}.b16")
+            with open(src, "rb") as fp:
+                bitj, nr, nc = (int(v) for v in np.fromfile(fp, dtype=np.int32, count=3))
+            wp = (nr + (bitj - nr % bitj) % bitj) // 8
+            raw = np.memmap(src, dtype=np.uint8, mode="r", offset=12, shape=(nc, wp))
+            dst = os.path.join(d_in, f"{key}_{suffix}_{b}.b16")
+            with open(dst, "wb") as fp:
+                if patients_are_rows:      # the (patients x columns) matrix: a byte range per column
+                    np.array([bitj, nb, nc], dtype=np.int32).tofile(fp)
+                    np.ascontiguousarray(raw[:, r0 // 8:(r0 + nb) // 8]).tofile(fp)
+                else:                      # its transpose: one file column per patient
+                    np.array([bitj, nr, nb], dtype=np.int32).tofile(fp)
+                    np.ascontiguousarray(raw[r0:r0 + nb]).tofile(fp)
+            del raw
+
+    keys = {k: [""] * nbins for k in ("A_n", "A_t", "Z_n", "Z_t", "Y_n", "Y_t", "W_n", "W_t", "T_n", "T_t",
+                                      "S_n", "S_t")}
+    has_T = os.path.exists(os.path.join(d_src, "T_n.b16"))
+    for b in range(nbins):
+        for key, rows, present in (("A", nA, True), ("Z", nZ, nZ > 0), ("T", nA, has_T), ("S", nZ, has_T and nZ > 0)):
+            if present:
+                cut(key, rows, b)
+                keys[f"{key}_n"][b] = f"{key}_n_{b}.b16"
+                keys[f"{key}_t"][b] = f"{key}_t_{b}.b16"
+    if ny > 0:
+        rng = np.random.default_rng(seed)
+        py = rng.uniform(pmin, pmax, size=ny).astype(np.float32)
+        for key, rows in (("Y", nA), ("W", nZ)):
+            if rows == 0:
+                continue
+            X = _bernoulli_columns(rng, rows, py)
+            nb = rows // nbins
+            for b in range(nbins):
+                write_b16(os.path.join(d_in, f"{key}_n_{b}.b16"), X[b * nb:(b + 1) * nb])
+                write_b16(os.path.join(d_in, f"{key}_t_{b}.b16"), X[b * nb:(b + 1) * nb].T)
+                keys[f"{key}_n"][b] = f"{key}_n_{b}.b16"
+                keys[f"{key}_t"][b] = f"{key}_t_{b}.b16"
+            del X
+    lines = ["GLOBAL_verbose= 0;", "GLOBAL_TEST_TYPE= dexcluster_driver;", f"GLOBAL_NBINS= {nbins};",
+             f"GLOBAL_B_MLT= {ds.B};", f"GLOBAL_gamma= {ds.gamma:.4f};", f"GLOBAL_DIR_XPRE= {d_in};",
+             f"GLOBAL_A_n_cols= {ds.ncols};", f"GLOBAL_Y_n_cols= {ny};", f"GLOBAL_T_n_cols= {ds.T};"]
+    names = {"A_n": "GLOBAL_A_n_name_", "A_t": "GLOBAL_A_t_name_", "Z_n": "GLOBAL_Z_n_name_",
+             "Z_t": "GLOBAL_Z_t_name_", "Y_n": "GLOBAL_Y_n_name_", "Y_t": "GLOBAL_Y_t_name_",
+             "W_n": "GLOBAL_W_n_name_", "W_t": "GLOBAL_W_t_name_", "T_n": "GLOBAL_T_n_name_",
+             "T_t": "GLOBAL_T_t_name_", "S_n": "GLOBAL_S_n_name_", "S_t": "GLOBAL_S_t_name_"}
+    for k, key in names.items():
+        if any(keys[k]):
+            lines.append(f"{key}= {', '.join(keys[k])};")
+    lines += [f"GLOBAL_DIR_NAME= {d_out};", "END= 0;"]
+    in_path = os.path.join(d_in, f"{name}.in")
+    with open(in_path, "w") as fp:
+        fp.write("\n".join(lines) + "\n")
+    return Dataset(in_path, d_out, nA, nZ, ds.ncols, ds.T, ds.gamma, ds.B)
+
+
 # the benchmark-sized problems (rows x SNP columns)
 CONFIGS = {
     "case_only_4096x16384": dict(nA=4096, ncols=16384, nZ=0, T=1, pmin=0.05, pmax=0.5,

@@ -93,3 +93,23 @@ def test_device_pool_generator_matches_its_definition():
     sub = bits[r][:, c]
     agree = (sub[:, None, :] == sub[None, :, :]).mean(axis=2)[np.triu_indices(len(r), 1)]
     assert np.abs(agree - 0.5).mean() > 0.25
+
+
+def test_split_into_bins_cuts_the_rows(tmp_path):
+    """A make_dataset problem split into bins by byte slicing: the bins' matrices stack back
+    into the original, patient-major and SNP-major copies agree, Y/W have the asked columns."""
+    import os
+    from lakcluster_c_amd import gen
+    ds = gen.make_dataset(str(tmp_path), "src", nA=64, ncols=300, nZ=32, T=3, gamma=0.1, seed=3)
+    b = gen.split_into_bins(ds, str(tmp_path), "split", 2, ny=20, seed=4)
+    d0, d1 = os.path.dirname(ds.in_path), os.path.dirname(b.in_path)
+    for key, rows in (("A", 64), ("Z", 32), ("T", 64), ("S", 32)):
+        whole = gen.read_b16(f"{d0}/{key}_n.b16")
+        parts = [gen.read_b16(f"{d1}/{key}_n_{i}.b16") for i in range(2)]
+        assert np.array_equal(np.vstack(parts), whole)
+        for i in range(2):
+            assert np.array_equal(gen.read_b16(f"{d1}/{key}_t_{i}.b16"), parts[i].T)
+    assert gen.read_b16(f"{d1}/Y_n_0.b16").shape == (32, 20) and gen.read_b16(f"{d1}/W_n_1.b16").shape == (16, 20)
+    text = open(b.in_path).read()
+    assert "GLOBAL_NBINS= 2;" in text and "GLOBAL_Y_n_cols= 20;" in text
+
