@@ -58,7 +58,8 @@ int lkc_dexcluster_driver(const char *in_text, const char *out_dir);
    shared GLOBAL_A_n_cind / Y_n_cind / T_n_cind masks; A_p and Y_p come from the data
    (dcc_load_A_p, dcc.c:875-991).  The subscores of every bin pair are recomputed after
    each drop (dexcluster_driver.c:22-53); the same three output files, row indices over
-   the bins in bin order.  Every bin names the same kinds of file.  Any number of bins
+   the bins in bin order; GLOBAL_A_p_name / GLOBAL_Y_p_name, when given, replace the data's
+   frequencies.  Every bin names the same kinds of file.  Any number of bins
    from 1 to 64 (1 bin without Y gives what lkc_dexcluster_driver gives). */
 int lkc_dcc_driver_bins(const char *in_text, const char *out_dir);
 

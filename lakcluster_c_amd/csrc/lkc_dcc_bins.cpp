@@ -373,6 +373,17 @@ void DccDev::data_freqs()
     }
   };
   freq(AZ, hAc_b, M, hA_p); freq(YW, hYc_b, NY, hY_p);
+  // GLOBAL_A_p_name / GLOBAL_Y_p_name (mda_read_d2_r8): the frequencies from a file instead
+  const std::string* pf[2] = { &c.A_p, &c.Y_p };
+  std::vector<double>* pv[2] = { &hA_p, &hY_p };
+  for (int v = 0; v < 2; v++) {
+    if (pf[v]->empty()) continue;
+    FILE* fp = fopen(pf[v]->c_str(), "rb"); int hdr[3];
+    const bool ok = fp && fread(hdr, sizeof(int), 3, fp) == 3 && hdr[1] == (int)pv[v]->size() &&
+                    fread(pv[v]->data(), sizeof(double), pv[v]->size(), fp) == pv[v]->size();
+    if (fp) fclose(fp);
+    if (!ok) throw Error(LKC_ERR_IO, "cannot read " + *pf[v]);
+  }
   ajdk_of(hA_p, hA_ajdk); ajdk_of(hY_p, hY_ajdk);
   HIPCHK(hipMemcpyAsync(A_ajdk, hA_ajdk.data(), sizeof(double) * hA_ajdk.size(), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(Y_ajdk, hY_ajdk.data(), sizeof(double) * hY_ajdk.size(), hipMemcpyHostToDevice, st));
@@ -689,7 +700,6 @@ void dcc_driver_bins(const Cfg& c, DccTestOut& out, const std::string& out_dir)
   if (c.nbins < 1 || c.nbins > 64) throw Error(LKC_ERR_CONFIG, "GLOBAL_NBINS must be in [1,64]");
   if (c.Ireq > 0) throw Error(LKC_ERR_CONFIG, "GLOBAL_Ireq > 0 (per-bin sorting) is not supported");
   if (c.scramble_num > 0) throw Error(LKC_ERR_CONFIG, "bcc_scramble is not part of dexcluster_driver");
-  if (!c.A_p.empty() || !c.Y_p.empty()) throw Error(LKC_ERR_CONFIG, "GLOBAL_A_p_name / GLOBAL_Y_p_name are not supported with several bins");
   if (c.B < 1 || c.B > 32) throw Error(LKC_ERR_CONFIG, "GLOBAL_B_MLT must be in [1,32]");
   DccDev D;
   D.c = c; D.out = &out; D.B = c.B;

@@ -453,6 +453,16 @@ static int dcc_load(dcc_t *D, const dcfg *c)
     else { double q = (tsum) / (tnum * (arb + zrb)); pp[v][b] = q < 0.01 ? 0.01 : (q > 0.99 ? 0.99 : q); }
   }
   free(AZ); free(YW);
+  /* GLOBAL_A_p_name / GLOBAL_Y_p_name (mda_read_d2_r8): the frequencies from a file instead */
+  const char *pf[2] = { c->A_p, c->Y_p };
+  for (int v = 0; v < 2; v++) {
+    if (!pf[v]) continue;
+    FILE *fp = fopen(pf[v], "rb"); int hdr[3];
+    const int ok = fp && fread(hdr, sizeof(int), 3, fp) == 3 && hdr[1] == npc[v] &&
+                   fread(pp[v], sizeof(double), (size_t)npc[v], fp) == (size_t)npc[v];
+    if (fp) fclose(fp);
+    if (!ok) { fprintf(stderr, "lko_dcc_bins: cannot read %s\n", pf[v]); return -1; }
+  }
   D->A_ajdk = (double *)zalloc((size_t)LKO_AJDK * D->Apc, sizeof(double)); lko_calc_ajdk(D->A_p, D->Apc, D->A_ajdk);
   D->Y_ajdk = (double *)zalloc((size_t)LKO_AJDK * (D->Ypc + 1), sizeof(double)); lko_calc_ajdk(D->Y_p, D->Ypc, D->Y_ajdk);
   dcc_alloc(D);
@@ -946,7 +956,7 @@ int lko_dcc_driver_bins(const char *in_text, const char *out_dir, int *n_trace, 
   if (!in_text || !out_dir) return -1;
   dcfg c;
   if (dcfg_parse(in_text, &c)) { dcfg_free(&c); return -1; }
-  if (c.nbins < 1 || c.nbins > DL_MAXBINS || c.Ireq > 0 || c.scramble_num > 0 || c.A_p || c.Y_p || c.B < 1 || c.B > 32) {
+  if (c.nbins < 1 || c.nbins > DL_MAXBINS || c.Ireq > 0 || c.scramble_num > 0 || c.B < 1 || c.B > 32) {
     dcfg_free(&c); return -1;
   }
   dcc_t D;

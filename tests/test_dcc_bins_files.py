@@ -133,6 +133,37 @@ def test_oracle_bins_run_invariants(oracle, tmp_path, case):
     assert np.all(np.isfinite(tr[:, 3:5]))
 
 
+def _write_p(path, p):
+    """An A_p / Y_p file as mda_write_d2_r8 writes it: int32 rank 2, the two dimensions, the doubles."""
+    with open(path, "wb") as f:
+        np.array([2, len(p), 1], dtype=np.int32).tofile(f)
+        np.asarray(p, dtype=np.float64).tofile(f)
+
+
+def test_oracle_bins_p_files(oracle, tmp_path):
+    """GLOBAL_A_p_name / GLOBAL_Y_p_name replace the frequencies from the data (dcc_load_A_p,
+    dcc.c:936-939, 962-965): files of 0.5 give what GLOBAL_TEST_sparse= 0 gives, byte for byte;
+    other values change the run."""
+    bins = [(40, 30), (50, 40)]
+    ds = _bins(tmp_path, "pf", bins, 2500, ny=2100, T=3, gamma=0.1, seed=8)
+    d = os.path.dirname(ds.in_path)
+    pa, py = oracle.lib().lko_psize(2500), oracle.lib().lko_psize(2100)
+    _write_p(os.path.join(d, "A_p_half.mda"), [0.5] * pa)
+    _write_p(os.path.join(d, "Y_p_half.mda"), [0.5] * py)
+    _write_p(os.path.join(d, "A_p_var.mda"), np.linspace(0.2, 0.7, pa))
+    text = open(ds.in_path).read()
+    half = text.replace("END= 0;", "GLOBAL_A_p_name= A_p_half.mda;\nGLOBAL_Y_p_name= Y_p_half.mda;\nEND= 0;")
+    dense = text.replace("END= 0;", "GLOBAL_TEST_sparse= 0;\nEND= 0;")
+    var = text.replace("END= 0;", "GLOBAL_A_p_name= A_p_var.mda;\nEND= 0;")
+    a, b, c, e = (str(tmp_path / n) for n in ("half", "dense", "var", "data"))
+    oracle.dcc_driver_bins(half, a)
+    oracle.dcc_driver_bins(dense, b)
+    oracle.dcc_driver_bins(var, c)
+    oracle.dcc_driver_bins(text, e)
+    assert _files(a) == _files(b)
+    assert _files(c)[0] != _files(e)[0] and _files(c)[0] != _files(a)[0]
+
+
 def test_bins_config_errors_before_any_device_work(tmp_path):
     """Configuration errors of the multi-bin driver come back as LKC_ERR_CONFIG from the
     loader, which runs on the host before any device call."""
@@ -141,7 +172,6 @@ def test_bins_config_errors_before_any_device_work(tmp_path):
     text = open(ds.in_path).read()
     bad = {
         "GLOBAL_Ireq": text.replace("END= 0;", "GLOBAL_Ireq= 1;\nEND= 0;"),
-        "A_p": text.replace("END= 0;", "GLOBAL_A_p_name= A_p.mda;\nEND= 0;"),
         "bin 1": text.replace("GLOBAL_A_t_name_= A_t_0.b16, A_t_1.b16;", "GLOBAL_A_t_name_= A_t_0.b16;"),
         "different set": text.replace("GLOBAL_Y_t_name_= Y_t_0.b16, Y_t_1.b16;", "GLOBAL_Y_t_name_= Y_t_0.b16;"),
     }
@@ -163,6 +193,7 @@ GPU_CASES = {
     "b16": dict(bins=[(50, 40), (40, 50)], ncols=600, ny=40, T=2, seed=15, B=16),
     "wide": dict(bins=[(120, 100), (130, 90)], ncols=4100, ny=2000, T=3, seed=16),
     "one_bin_y": dict(bins=[(80, 60)], ncols=500, ny=70, T=3, seed=17),
+    "p_files": dict(bins=[(50, 40), (60, 30)], ncols=2500, ny=2100, T=2, seed=18),
 }
 
 
@@ -173,6 +204,11 @@ def test_engine_dcc_bins_matches_oracle(oracle, tmp_path, name):
     kw = dict(GPU_CASES[name])
     ds = _bins(tmp_path, name, gamma=kw.pop("gamma", 0.1), **kw)
     text = open(ds.in_path).read()
+    if name == "p_files":
+        d = os.path.dirname(ds.in_path)
+        _write_p(os.path.join(d, "A_p.mda"), np.linspace(0.15, 0.8, oracle.lib().lko_psize(2500)))
+        _write_p(os.path.join(d, "Y_p.mda"), np.linspace(0.6, 0.3, oracle.lib().lko_psize(2100)))
+        text = text.replace("END= 0;", "GLOBAL_A_p_name= A_p.mda;\nGLOBAL_Y_p_name= Y_p.mda;\nEND= 0;")
     print(f"dcc bins {name}", flush=True)
     a, b = str(tmp_path / "eng"), str(tmp_path / "ora")
     lkc.dcc_driver_bins(text, a)
