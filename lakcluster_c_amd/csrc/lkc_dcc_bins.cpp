@@ -698,7 +698,8 @@ void run_loop(DccDev& D, int max_iter)
 void dcc_driver_bins(const Cfg& c, DccTestOut& out, const std::string& out_dir)
 {
   if (c.nbins < 1 || c.nbins > 64) throw Error(LKC_ERR_CONFIG, "GLOBAL_NBINS must be in [1,64]");
-  if (c.Ireq > 0) throw Error(LKC_ERR_CONFIG, "GLOBAL_Ireq > 0 (per-bin sorting) is not supported");
+  // (GLOBAL_Ireq > 0 with one bin is the GLOBAL_Ireq <= 0 run: one value sorted per row)
+  if (c.Ireq > 0 && c.nbins > 1) throw Error(LKC_ERR_CONFIG, "GLOBAL_Ireq > 0 (per-bin sorting) is not supported");
   if (c.scramble_num > 0) throw Error(LKC_ERR_CONFIG, "bcc_scramble is not part of dexcluster_driver");
   if (c.B < 1 || c.B > 32) throw Error(LKC_ERR_CONFIG, "GLOBAL_B_MLT must be in [1,32]");
   DccDev D;
@@ -715,7 +716,7 @@ void dcc_sumscores_test(const Cfg& c, DccTestOut& out, const std::string& out_di
 {
   if (c.nbins < 1) throw Error(LKC_ERR_CONFIG, "dcc_sumscores needs GLOBAL_NBINS >= 1");
   if (c.test_typ2 != "error") throw Error(LKC_ERR_CONFIG, "dcc_sumscores: GLOBAL_TEST_TYP2 must be error (random data)");
-  if (c.Ireq > 0) throw Error(LKC_ERR_CONFIG, "GLOBAL_Ireq > 0 is not supported");
+  if (c.Ireq > 0 && c.nbins > 1) throw Error(LKC_ERR_CONFIG, "GLOBAL_Ireq > 0 (per-bin sorting) is not supported");
   if (c.test_T_cols < 1 || c.test_T_cols > kMaxT) throw Error(LKC_ERR_CONFIG, "dcc_sumscores: GLOBAL_TEST_T_n_cols must be 1..8");
   if (c.test_A_cols < 1 || c.test_Y_cols < 0 || c.test_A_rows < 0 || c.test_Z_rows < 0 || c.B < 1 || c.B > 32)
     throw Error(LKC_ERR_CONFIG, "dcc_sumscores: bad problem size");

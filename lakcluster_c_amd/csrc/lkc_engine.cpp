@@ -506,8 +506,11 @@ void Run::load(const char* in_text)
                c.QR.find("condense") != std::string::npos || c.QC.find("store one") != std::string::npos ||
                c.QR.find("ZtSWn") != std::string::npos || c.QC.find("ZtSWn") != std::string::npos))
     throw Error(LKC_ERR_CONFIG, "only the YnWt strategies are supported");
-  // GLOBAL_Ireq > 0 sorts and drops per bin (bcc_sumscores_srt, bcc_sumscores.c:358-481)
-  if (c.Ireq > 0) throw Error(LKC_ERR_CONFIG, "GLOBAL_Ireq > 0 (per-bin sorting) is not supported");
+  // GLOBAL_Ireq > 0 sorts each row's scores over the bins and normalises per bin
+  // (bcc_sumscores_srt / _nrm, bcc_sumscores.c:358-481, 583-628; dcc_sumscores.c:283-407,
+  // 490-541): with one bin that sorts one value and gives the same normalisers, so the run is
+  // the GLOBAL_Ireq <= 0 one bit for bit; with several bins it is not supported
+  if (c.Ireq > 0 && c.nbins > 1) throw Error(LKC_ERR_CONFIG, "GLOBAL_Ireq > 0 (per-bin sorting) is not supported");
   for (int i = 0; i < c.scramble_num; i++)
     if (c.scramble_files[i].empty()) throw Error(LKC_ERR_CONFIG, "GLOBAL_scramble_out_xdrop_ names fewer files than GLOBAL_scramble_num");
   if (c.B < 1 || c.B > 32) throw Error(LKC_ERR_CONFIG, "GLOBAL_B_MLT must be in [1,32]");
@@ -600,7 +603,7 @@ void Run::load_synthetic(double t0)
       c.QR.find("condense") != std::string::npos || c.QC.find("store one") != std::string::npos ||
       c.QR.find("ZtSWn") != std::string::npos || c.QC.find("ZtSWn") != std::string::npos)
     throw Error(LKC_ERR_CONFIG, "only the YnWt strategies are supported");
-  if (c.Ireq > 0) throw Error(LKC_ERR_CONFIG, "GLOBAL_Ireq > 0 (per-bin sorting) is not supported");
+  if (c.Ireq > 0 && c.nbins > 1) throw Error(LKC_ERR_CONFIG, "GLOBAL_Ireq > 0 (per-bin sorting) is not supported");
   if (c.scramble_num > 0) throw Error(LKC_ERR_CONFIG, "bcc_scramble is not part of bcc_sumscores_test");
   if (c.B < 1 || c.B > 32) throw Error(LKC_ERR_CONFIG, "GLOBAL_B_MLT must be in [1,32]");
   if (c.test_A_cols < 1 || c.test_T_cols < 1 || c.test_T_cols > kMaxT) throw Error(LKC_ERR_CONFIG, "bcc_sumscores: bad problem size");

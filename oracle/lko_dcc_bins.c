@@ -956,7 +956,7 @@ int lko_dcc_driver_bins(const char *in_text, const char *out_dir, int *n_trace, 
   if (!in_text || !out_dir) return -1;
   dcfg c;
   if (dcfg_parse(in_text, &c)) { dcfg_free(&c); return -1; }
-  if (c.nbins < 1 || c.nbins > DL_MAXBINS || c.Ireq > 0 || c.scramble_num > 0 || c.B < 1 || c.B > 32) {
+  if (c.nbins < 1 || c.nbins > DL_MAXBINS || (c.Ireq > 0 && c.nbins > 1) || c.scramble_num > 0 || c.B < 1 || c.B > 32) {
     dcfg_free(&c); return -1;
   }
   dcc_t D;

@@ -1402,7 +1402,8 @@ static int st_load(lko_state *S, lko_result *R)
   int dcc = !strcmp(c->test_type, "dexcluster_driver") || !strcmp(c->test_type, "dexcluster_scorebox");
   if (!dcc && (!strstr(c->QR, "YnWt") || !strstr(c->QC, "YnWt") || strstr(c->QR, "condense") || strstr(c->QC, "store one")
                || strstr(c->QR, "ZtSWn") || strstr(c->QC, "ZtSWn"))) return fail(R, "only the YnWt strategies are supported");
-  if (c->Ireq > 0) return fail(R, "GLOBAL_Ireq > 0 (per-bin sorting) is not supported");
+  /* GLOBAL_Ireq > 0 with one bin: one value sorted per row, the same normalisers
+     (bcc_sumscores.c:358-481, 583-628): the GLOBAL_Ireq <= 0 run */
   if (!c->A_t[0] || !c->A_n[0]) return fail(R, "A_n/A_t files required");
   if (b16_read(c->A_t, &S->An) || b16_read(c->A_n, &S->At)) return fail(R, "cannot read A");
   S->NA = S->An.nr; S->M = S->An.nc;
@@ -1502,7 +1503,7 @@ static int st_init(lko_state *S, lko_result *R)
   if (strcmp(c->test_typ2, "error")) return fail(R, "bcc_sumscores: GLOBAL_TEST_TYP2 must be error (random data)");
   if (!strstr(c->QR, "YnWt") || !strstr(c->QC, "YnWt") || strstr(c->QR, "condense") || strstr(c->QC, "store one")
       || strstr(c->QR, "ZtSWn") || strstr(c->QC, "ZtSWn")) return fail(R, "only the YnWt strategies are supported");
-  if (c->Ireq > 0) return fail(R, "GLOBAL_Ireq > 0 (per-bin sorting) is not supported");
+
   if (c->test_A_cols < 1 || c->test_T_cols < 1 || c->test_T_cols > 8) return fail(R, "bcc_sumscores: bad problem size");
   const double mrnd = c->test_mrand;
   srand(1);

@@ -72,7 +72,6 @@ def test_config_errors_are_rejected_before_any_device_work():
     the bcc elimination; the dcc elimination reads no strategy (dcc.c:1167)."""
     base = "GLOBAL_NBINS= 1; GLOBAL_QR_strategy= YnWt; GLOBAL_QC_strategy= YnWt; GLOBAL_Y_n_cols= 0; "
     for text, why in ((base + "GLOBAL_gama= 0.1; END= 0;", "unknown key GLOBAL_gama"),
-                      (base + "GLOBAL_Ireq= 1; END= 0;", "GLOBAL_Ireq"),
                       ("GLOBAL_NBINS= 1; GLOBAL_QR_strategy= YnWt; GLOBAL_QC_strategy= YnWt store one; END= 0;", "YnWt"),
                       ("GLOBAL_NBINS= 1; END= 0;", "YnWt"),
                       (base.replace("GLOBAL_NBINS= 1", "GLOBAL_NBINS= 2") + "END= 0;", "NBINS")):
@@ -199,3 +198,38 @@ def test_dquicksort_xij_matches_oracle(oracle):
         np.testing.assert_array_equal(ix, oix)
         assert depth == oracle.quicksort.depth
     assert lkc.dquicksort_xij(np.arange(40000, dtype=float))[2] == 32768
+
+
+def _with_ireq(text, ireq):
+    return text.replace("END= 0;", f"GLOBAL_Ireq= {ireq};\nEND= 0;")
+
+
+@pytest.mark.parametrize("dcc", [False, True])
+def test_oracle_ireq_with_one_bin_is_the_plain_run(datasets, oracle, tmp_path, dcc):
+    """GLOBAL_Ireq > 0 sorts each row's scores over the bins and normalises per bin
+    (bcc_sumscores.c:358-481, 583-628; dcc_sumscores.c:283-407, 490-541); with one bin that
+    sorts a single value and the normalisers are the same, so the files are those of
+    GLOBAL_Ireq= 0 byte for byte."""
+    text = open(datasets["casectrl"].in_path).read()
+    if dcc:
+        text = text.replace("GLOBAL_TEST_TYPE= lakcluster_driver;", "GLOBAL_TEST_TYPE= dexcluster_driver;")
+    a, b = str(tmp_path / "i0"), str(tmp_path / "i2")
+    oracle.run_text(text, out_dir=a, write_files=True)
+    oracle.run_text(_with_ireq(text, 2), out_dir=b, write_files=True)
+    for f in ("out_trace.txt", "out_xdrop_a.txt", "out_xdrop_b.txt"):
+        assert open(f"{a}/{f}").read() == open(f"{b}/{f}").read(), f
+
+
+@pytest.mark.gpu
+def test_engine_ireq_with_one_bin_is_the_plain_run(datasets, oracle, tmp_path):
+    from lakcluster_c_amd import lkc
+    text = open(datasets["casectrl"].in_path).read()
+    dtext = text.replace("GLOBAL_TEST_TYPE= lakcluster_driver;", "GLOBAL_TEST_TYPE= dexcluster_driver;")
+    for name, t, run in (("bcc", text, lkc.lakcluster_driver), ("dcc", dtext, lkc.dexcluster_driver)):
+        a, b, c = (str(tmp_path / f"{name}_{k}") for k in ("eng1", "ora0", "eng0"))
+        run(_with_ireq(t, 1), a)
+        run(t, c)
+        oracle.run_text(t, out_dir=b, write_files=True)
+        for f in ("out_trace.txt", "out_xdrop_a.txt", "out_xdrop_b.txt"):
+            assert open(f"{a}/{f}").read() == open(f"{b}/{f}").read() == open(f"{c}/{f}").read(), (name, f)
+
