@@ -1,6 +1,7 @@
 /*
  * lko_dcc_bins.c -- CPU oracle for the dcc half-loop elimination with several bins
- * and Y/W genetic controls, on the reference's own synthetic problem.
+ * and Y/W genetic controls, on the reference's own synthetic problem and on the
+ * .in's per-bin files.
  *
  * TEST INFRASTRUCTURE ONLY: the checker for the HIP product path.  Only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
@@ -16,7 +17,11 @@
  *   - dcc_sumscores_nrm / ifT / srt / cmb / xij / mxA / mxB / mxC / dmp
  *     (dcc_sumscores.c:5-699), dcc_lrup_mxdup (dcc_lf.c:531-542), dcc_M_mxset
  *     (dcc.c:1092-1113);
- *   - the loop of dcc_sumscores_test (dcc_sumscores.c:700-747).
+ *   - the loop of dcc_sumscores_test (dcc_sumscores.c:700-747);
+ *   - dcc_load (dcc.c:1157-1215): the per-bin file lists of updateglobals
+ *     (updateglobals.c:76-117), dcc_ajdk_load's column masks (dcc.c:54-115),
+ *     dcc_single_load_M_An (dcc.c:251-419), dcc_load_A_p (dcc.c:875-991), and the
+ *     dexcluster_driver loop to the end (dexcluster_driver.c:5-57).
  * Bit matrices are kept whole ("a" indices); loaded-row and loaded-column masks
  * (*_b) and active masks (*_j, a subset) select what the reference's compacted
  * storage would hold.  Double expressions keep the reference's order (build with

@@ -2,7 +2,10 @@
  * lko_oracle.c -- CPU oracle for the lakcluster `bcc` elimination path.
  *
  * TEST INFRASTRUCTURE ONLY: the checker for the HIP product path (see the
- * header).  Single bin, no Y/W, QR/QC strategy "YnWt" (store all).
+ * header).  Single bin, no Y/W, QR/QC strategy "YnWt" (store all): the problem from
+ * the .in's files (bcc_load) or drawn by bcc_init (bcc.c:1542-1600, st_init); the bcc
+ * elimination, the one-bin dcc elimination, bcc_scramble, and the scorebox replays of
+ * both (lakcluster_scorebox, dexcluster_scorebox).
  *
  * Conventions
  *   - bit matrices are row-major packed bit rows, most significant bit first
