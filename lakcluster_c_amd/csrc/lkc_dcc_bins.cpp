@@ -57,7 +57,7 @@ struct Bin {
   double* QRn[4] = {};                                       // [nb2*T*nA + t*nA + r]
   std::vector<int> g_a, g_b;                                 // global row indices (dcc_init_QX)
 };
-struct Pair { double *x[4] = {}, *qr[4] = {}, *qc[4] = {}; };
+struct Pair { double *x[4] = {}, *qr[4] = {}, *qc[4] = {}; unsigned long long* pg[4] = {}; };
 
 struct DccDev {
   Cfg c;
@@ -305,6 +305,7 @@ void DccDev::setup()
     Pair& f = F[nb1 + nb2 * nbins];
     for (int q = 0; q < 4; q++) {
       f.x[q] = dbl((size_t)T * E[nb1].nA); f.qr[q] = dbl((size_t)T * E[nb1].nA); f.qc[q] = dbl((size_t)T * X(q));
+      f.pg[q] = dev.alloc<unsigned long long>(34 * (size_t)std::max(E[nb1].nA, 1));
     }
   }
   for (int q = 0; q < 4; q++) { QCn[q] = dbl((size_t)nbins * nbins * T * X(q)); fill(QCn[q], (long long)nbins * nbins * T * X(q), 0.0, st); }
@@ -473,7 +474,7 @@ void DccDev::subscores()
     HIPCHK(ztsn_mfma_prep(rb, (q == YW || q == YY) ? Yc_j : Ac_j, e2.cz[q], X(q), X(q), nt, B, prep[(size_t)nb2 * 4 + q],
                           prep_bytes, st));
   }
-  std::vector<PairQC> qcj; std::vector<PairQR> qrj;
+  std::vector<PairQC> qcj; std::vector<PairQR> qrj; std::vector<ZtsnJob> zj;
   for (int nb1 = 0; nb1 < nbins; nb1++) for (int nb2 = 0; nb2 < nbins; nb2++) {
     Bin &e1 = E[nb1], &e2 = E[nb2];
     Pair& f = F[nb1 + nb2 * nbins];
@@ -497,8 +498,10 @@ void DccDev::subscores()
                   const double* et, const double* x6, const double* mlf, const DMat* Tt) {
       if (!on(q, nb1, nb2) || na <= 0 || nt <= 0 || l.n() <= 0) return;
       (void)ld;
-      HIPCHK(ztsn_mfma_run(Xn.b(), e1.ar.d, na, cmask, nt, tl.d, B, e2.scal[q], f.x[q], e1.nA,
-                           prep[(size_t)nb2 * 4 + q], prep_bytes, st));
+      const void* w = prep[(size_t)nb2 * 4 + q];
+      zj.push_back(ZtsnJob{ Xn.p, Xn.rb, e1.ar.d, na, (const uint8_t*)cmask, ztsn_prep_digits(w), (long long)Xn.rb * 8,
+                            nt, ztsn_prep_sums(w, nt, Xn.rb, B), tl.d, e2.scal[q], f.x[q], e1.nA, f.pg[q],
+                            ztsn_job_parts(na, Xn.rb) });
       ops += (long long)na * nt * (B + 1) * (long long)Xn.rb * 8;
       PairQR a{ e1.ar.d, na, tl.d, nt, e1.nA, f.x[q], a1, et, x6, mlf, Tt ? Tt->p : nullptr, Tt ? Tt->rb : 0, {}, f.qr[q] };
       std::copy(tb, tb + kMaxT, a.tb);
@@ -509,6 +512,7 @@ void DccDev::subscores()
     qr(YY, e1.Yn, Yc_j, yl, NY, e2.sc + 2 * T, e2.sc + 4 * T, e1.lfY + 6 * (size_t)e1.nA, mlfY, same ? &e1.Tt : nullptr);
     qr(AA, e1.An, Ac_j, kl, M, e2.sc + 0 * T, e2.sc + 4 * T, e1.lfA + 6 * (size_t)e1.nA, mlfA, same ? &e1.Tt : nullptr);
   }
+  HIPCHK(ztsn_mfma_jobs(zj, B, st));
   pair_qc_jobs(qcj, st);
   pair_qr_jobs(qrj, st);
 }

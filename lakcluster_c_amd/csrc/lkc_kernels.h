@@ -6,6 +6,7 @@
 // (row, covariate) pair are stored covariate-major: X[t*ld + row].
 // All launchers are asynchronous on `st`.
 #pragma once
+#include <vector>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stddef.h>
@@ -120,6 +121,22 @@ hipError_t ztsn_mfma_prep(size_t rb, const uint32_t* mask, const uint32_t* cz, l
                           void* work, size_t work_bytes, hipStream_t st);
 hipError_t ztsn_mfma_run(const Bits& A, const int* alist, int na, const uint32_t* mask, int ns, const int* slist, int B,
                          const double* scal, double* out, long long ldo, void* work, size_t work_bytes, hipStream_t st);
+// one contraction of a group: digit planes and column sums from ztsn_mfma_prep (Dg at the
+// start of its work, Gsum after them), its own Pg of 34 * na words, and its n parts
+struct ZtsnJob {
+  const uint8_t* A; size_t rbA; const int* alist; int na; const uint8_t* mask; const int8_t* Dg; long long ldn;
+  int ns; const long long* Gsum; const int* slist; const double* scal; double* out; long long ldo;
+  unsigned long long* Pg; int parts;
+};
+struct ZtsnJobs { static constexpr int kMax = 24; int n; ZtsnJob job[kMax]; };
+int ztsn_job_parts(int na, size_t rb);
+// where ztsn_mfma_prep leaves the digit planes and column sums in work
+inline const int8_t* ztsn_prep_digits(const void* work) { return (const int8_t*)work; }
+inline const long long* ztsn_prep_sums(const void* work, int ns, size_t rb, int B)
+{
+  return (const long long*)((const uint8_t*)work + (((size_t)((B + 7) / 8) * ns * rb * 8 + 255) & ~(size_t)255));
+}
+hipError_t ztsn_mfma_jobs(const std::vector<ZtsnJob>& v, int B, hipStream_t st);
 
 // AnZt_S_WnYt_vv with X==Y: out[tlist[t]*ldo + a] = Σ_z (X(a,z)*vS(z))*X(a,z) in list
 // order of z, with X(a,z) = X[z*ldx + a] (x_rows 0) or X[a*ldx + z] (x_rows 1).

@@ -11,6 +11,8 @@
 // integer is then turned into the reference's double term and summed over w in
 // the reference's order (ascending w), per (k, t).
 #include "lkc_kernels.h"
+
+#include <vector>
 #include <stdio.h>
 #include <algorithm>
 
@@ -669,15 +671,16 @@ __global__ __launch_bounds__(256) void k_cz_sums(const uint32_t* cz, long long l
 // them to Pg / Cg / Mg with 64-bit integer atomics (exact in any order) and k_ztsn_fin
 // forms the outputs.
 template <int ND, bool PARTS>
-__global__ __launch_bounds__(256) void k_ztsn_mfma(const uint8_t* A, size_t rbA, const int* alist, int na,
-                                                   const uint8_t* mask, const int8_t* Dg, long long ldn, int ns,
-                                                   const long long* Gsum, const int* slist, const double* scal,
-                                                   double* out, long long ldo, unsigned long long* Pg)
+__device__ __forceinline__ void ztsn_block(const uint8_t* A, size_t rbA, const int* alist, int na,
+                                           const uint8_t* mask, const int8_t* Dg, long long ldn, int ns,
+                                           const long long* Gsum, const int* slist, const double* scal,
+                                           double* out, long long ldo, unsigned long long* Pg, int bx, int part,
+                                           int nparts)
 {
   __shared__ int Ps[32][33];
   __shared__ int Cs[32], Ms[32];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, g = lane >> 5, r = lane & 31;
-  const int i0 = blockIdx.x * 32;
+  const int i0 = bx * 32;
   const uint8_t* Arow = A + (size_t)alist[min(i0 + r, na - 1)] * rbA + 8 * g;
   const uint8_t* mrow = mask + 8 * g;
   const int ncol = ND * ns;
@@ -689,8 +692,8 @@ __global__ __launch_bounds__(256) void k_ztsn_mfma(const uint8_t* A, size_t rbA,
   for (int v = 0; v < 16; v++) acc[v] = 0;
   int cnt = 0, mcnt = 0;
   const int nchunk = (int)(ldn / 128);
-  const int c_lo = (int)((long long)blockIdx.y * nchunk / gridDim.y);
-  const int c_hi = (int)((long long)(blockIdx.y + 1) * nchunk / gridDim.y);
+  const int c_lo = (int)((long long)part * nchunk / nparts);
+  const int c_hi = (int)((long long)(part + 1) * nchunk / nparts);
   // chunk c of this wave: its A and mask words and its 4 digit fragments
   uint64_t a_n = 0, m_n = 0;
   v4i b_n[4];
@@ -750,15 +753,33 @@ __global__ __launch_bounds__(256) void k_ztsn_mfma(const uint8_t* A, size_t rbA,
     out[(long long)slist[s] * ldo + alist[i0 + rr]] = (double)(base + tmp) / mlt;
   }
 }
+template <int ND, bool PARTS>
+__global__ __launch_bounds__(256) void k_ztsn_mfma(const uint8_t* A, size_t rbA, const int* alist, int na,
+                                                   const uint8_t* mask, const int8_t* Dg, long long ldn, int ns,
+                                                   const long long* Gsum, const int* slist, const double* scal,
+                                                   double* out, long long ldo, unsigned long long* Pg)
+{
+  ztsn_block<ND, PARTS>(A, rbA, alist, na, mask, Dg, ldn, ns, Gsum, slist, scal, out, ldo, Pg, blockIdx.x, blockIdx.y,
+                        gridDim.y);
+}
+// several contractions in one launch (blockIdx.z picks the job), each split over its own
+// number of n parts into its own Pg; blocks past a job's rows or parts leave at once
+template <int ND>
+__global__ __launch_bounds__(256) void k_ztsn_mfma_jobs(ZtsnJobs J)
+{
+  const ZtsnJob& j = J.job[blockIdx.z];
+  if ((int)blockIdx.x * 32 >= j.na || (int)blockIdx.y >= j.parts) return;
+  ztsn_block<ND, true>(j.A, j.rbA, j.alist, j.na, j.mask, j.Dg, j.ldn, j.ns, j.Gsum, j.slist, j.scal, j.out, j.ldo,
+                       j.Pg, blockIdx.x, blockIdx.y, j.parts);
+}
 
 // the outputs of the split form from the summed integers, one thread per (row, covariate):
 // the same expressions as the one-part kernel's
 template <int ND>
-__global__ void k_ztsn_fin(const unsigned long long* Pg, int na, int ns, const long long* Gsum, const int* alist,
-                           const int* slist, const double* scal, double* out, long long ldo)
+__device__ __forceinline__ void ztsn_fin_at(const unsigned long long* Pg, int na, int ns, const long long* Gsum,
+                                            const int* alist, const int* slist, const double* scal, double* out,
+                                            long long ldo, int e)
 {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= na * ns) return;
   const int rr = e % na, s = e / na;
   const long long* row = (const long long*)Pg + (size_t)rr * 34;
   const double mn = scal[2], mlt = scal[4];
@@ -776,6 +797,26 @@ __global__ void k_ztsn_fin(const unsigned long long* Pg, int na, int ns, const l
   const double base = ll * mn * mlt;
   const double tmp = (double)x;
   out[(long long)slist[s] * ldo + alist[rr]] = (double)(base + tmp) / mlt;
+}
+template <int ND>
+__global__ void k_ztsn_fin(const unsigned long long* Pg, int na, int ns, const long long* Gsum, const int* alist,
+                           const int* slist, const double* scal, double* out, long long ldo)
+{
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < na * ns) ztsn_fin_at<ND>(Pg, na, ns, Gsum, alist, slist, scal, out, ldo, e);
+}
+template <int ND>
+__global__ void k_ztsn_fin_jobs(ZtsnJobs J)
+{
+  const ZtsnJob& j = J.job[blockIdx.y];
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < j.na * j.ns) ztsn_fin_at<ND>(j.Pg, j.na, j.ns, j.Gsum, j.alist, j.slist, j.scal, j.out, j.ldo, e);
+}
+__global__ void k_ztsn_zero_jobs(ZtsnJobs J)
+{
+  const ZtsnJob& j = J.job[blockIdx.y];
+  const long long n = 34LL * j.na;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) j.Pg[i] = 0ull;
 }
 
 __global__ void k_ztsn_zero(unsigned long long* p, long long n)
@@ -847,6 +888,42 @@ hipError_t ztsn_mfma_run(const Bits& A, const int* alist, int na, const uint32_t
                               Dg, ldn, ns, Gsum, slist, scal, out, ldo, Pg); } while (0)
   switch (ndig) { case 1: ZT_LAUNCH(1); break; case 2: ZT_LAUNCH(2); break; case 3: ZT_LAUNCH(3); break; default: ZT_LAUNCH(4); break; }
 #undef ZT_LAUNCH
+  return hipGetLastError();
+}
+
+int ztsn_job_parts(int na, size_t rb) { return ztsn_parts(na, rb); }
+
+// Many contractions with prepared digit planes (ztsn_mfma_prep), each with its own Pg of
+// 34 * na words: zeroed, accumulated and finished in three launches per group of jobs.  The
+// integer sums are exact in any order, so each output is the one ztsn_mfma_run writes.
+hipError_t ztsn_mfma_jobs(const std::vector<ZtsnJob>& v, int B, hipStream_t st)
+{
+  const int ndig = (B + 7) / 8;
+  for (size_t j0 = 0; j0 < v.size(); j0 += ZtsnJobs::kMax) {
+    ZtsnJobs J;
+    J.n = (int)std::min<size_t>(ZtsnJobs::kMax, v.size() - j0);
+    int rows = 0, parts = 0; long long outs = 0, words = 0;
+    for (int j = 0; j < J.n; j++) {
+      J.job[j] = v[j0 + j];
+      if (ndig * J.job[j].ns > 32) return hipErrorInvalidValue;
+      rows = std::max(rows, J.job[j].na); parts = std::max(parts, J.job[j].parts);
+      outs = std::max(outs, (long long)J.job[j].na * J.job[j].ns); words = std::max(words, 34LL * J.job[j].na);
+    }
+    if (rows <= 0) continue;
+    hipLaunchKernelGGL(k_ztsn_zero_jobs, dim3(std::min<unsigned>(cdiv(words, 256), 1024), J.n), dim3(256), 0, st, J);
+    const dim3 grid(cdiv(rows, 32), parts, J.n);
+    const dim3 gfin(cdiv(outs, 256), J.n);
+    switch (ndig) {
+      case 1: hipLaunchKernelGGL((k_ztsn_mfma_jobs<1>), grid, dim3(256), 0, st, J);
+              hipLaunchKernelGGL((k_ztsn_fin_jobs<1>), gfin, dim3(256), 0, st, J); break;
+      case 2: hipLaunchKernelGGL((k_ztsn_mfma_jobs<2>), grid, dim3(256), 0, st, J);
+              hipLaunchKernelGGL((k_ztsn_fin_jobs<2>), gfin, dim3(256), 0, st, J); break;
+      case 3: hipLaunchKernelGGL((k_ztsn_mfma_jobs<3>), grid, dim3(256), 0, st, J);
+              hipLaunchKernelGGL((k_ztsn_fin_jobs<3>), gfin, dim3(256), 0, st, J); break;
+      default: hipLaunchKernelGGL((k_ztsn_mfma_jobs<4>), grid, dim3(256), 0, st, J);
+               hipLaunchKernelGGL((k_ztsn_fin_jobs<4>), gfin, dim3(256), 0, st, J); break;
+    }
+  }
   return hipGetLastError();
 }
 
